@@ -1,0 +1,202 @@
+/*
+ * pyoomph HIP assembly-and-solve library: C ABI.
+ *
+ * Plain pointers and sizes only. "dev" pointers are device (HBM) addresses, everything
+ * else is host memory. Every entry point returns PYHIP_OK (0) or a negative status;
+ * pyhip_last_error() gives the message of the last failure on the calling thread (the
+ * reference throws std::runtime_error via throw_runtime_error, src/exception.cpp, which
+ * pybind turns into RuntimeError; the Python mirror raises RuntimeError on a negative
+ * status).
+ *
+ * Which reference interface each entry point replaces is cited beside it.
+ */
+#ifndef PYOOMPH_HIP_H
+#define PYOOMPH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------- */
+#define PYHIP_OK 0
+#define PYHIP_ERR_INVALID (-1)       /* bad argument / inconsistent sizes              */
+#define PYHIP_ERR_HIP (-2)           /* HIP runtime error                              */
+#define PYHIP_ERR_NOMEM (-3)         /* allocation failed or over the HBM budget        */
+#define PYHIP_ERR_NOT_CONVERGED (-4) /* Krylov solver did not reach its tolerance       */
+#define PYHIP_ERR_BREAKDOWN (-5)     /* zero pivot / non-finite value in a factorization */
+#define PYHIP_ERR_NEWTON (-6)        /* NewtonSolverError (oomph problem.cc:9283-9298)  */
+
+/* ---- element families and weak forms ---------------------------------------------- */
+#define PYHIP_ELEM_TRI_P1 1  /* BulkElementTri2dC1   (src/elements.hpp:899-942)   */
+#define PYHIP_ELEM_TRI_P2 2  /* BulkElementTri2dC2   (src/elements.hpp:990-1044)  */
+#define PYHIP_ELEM_QUAD_Q2 3 /* BulkElementQuad2dC2  (src/elements.hpp:831-897)   */
+#define PYHIP_ELEM_HEX_Q2 4  /* BulkElementBrick3dC2 (src/elements.hpp:1143-1213) */
+
+#define PYHIP_FORM_POISSON 1            /* pyoomph/equations/poisson.py:70-74           */
+#define PYHIP_FORM_BRATU 2              /* -lap u = lambda e^u, global parameter lambda */
+#define PYHIP_FORM_REACTION_DIFFUSION 3 /* -lap u + u^3 = f, f a nodal field            */
+#define PYHIP_FORM_NAVIER_STOKES_TH 4   /* pyoomph/equations/navier_stokes.py:255-347,
+                                           481-503, Taylor-Hood Q2/Q1, steady           */
+
+#define PYHIP_F64 0
+#define PYHIP_F32 1
+
+const char *pyhip_last_error(void);
+int pyhip_version(void);
+int pyhip_device_count(int *count);
+
+/* ---- element table ---------------------------------------------------------------------
+ * Replaces JIT_ELEMENT_init(JITFuncSpec_Table_FiniteElement_t*) (src/jitbridge.h:329-499,
+ * emitted by src/codegen.cpp:6398-7090): fills the description of one weak form on one
+ * element family. Residual 0 is the steady residual ("ResidualAndJacobianSteady",
+ * codegen.cpp:4718-4730); has_param_derivative[p] marks dResidual/dParameter_p
+ * (codegen.cpp:4733-4752). The functions themselves are the batched device entry points
+ * below, not per-element function pointers. */
+typedef struct {
+    int form, elem_kind;
+    int num_res_jacs;                  /* num_res_jacs                                */
+    const char *res_jac_names[2];      /* res_jac_names                               */
+    int nodal_values;                  /* values per node (numfields_C2+numfields_C1)  */
+    int nslot;                         /* element dofs incl. pinned ones              */
+    int nnode;                         /* nodes per element                           */
+    int dim;                           /* nodal dimension                             */
+    int integration_order;             /* integration_order (0 = element default)     */
+    int nqp;                           /* points of the scheme that is used            */
+    int numglobal_params;              /* numglobal_params                            */
+    const char *global_param_names[2];
+    int has_param_derivative[2];       /* ParameterDerivative[0][p] != NULL           */
+    int moving_nodes;                  /* always 0 here                               */
+} pyhip_element_table;
+int pyhip_element_init(int form, int elem_kind, pyhip_element_table *table);
+
+/* ---- problems ----------------------------------------------------------------------------
+ * A problem is one or more independent sub-problems (each: one mesh, one weak form, its
+ * own parameters) held device-resident with a block-diagonal sparsity pattern. One
+ * sub-problem is pyoomph's Problem with one bulk mesh; several are the batched / served
+ * case. Numbering, pinning and boundary values come from the host (the reference numbers
+ * equations in oomph Mesh::assign_global_eqn_numbers, mesh.cc:686-708). */
+typedef struct {
+    int form, elem_kind;
+    int64_t nnode, nelem;
+    const double *coords;        /* [nnode][dim]                                        */
+    const int32_t *conn;         /* [nelem][nnode_per_element], oomph local node order  */
+    int nvalue;                  /* values per node                                     */
+    const int32_t *eqn;          /* [nnode][nvalue] sub-problem equation, -1 pinned     */
+    const double *pinned_values; /* [nnode][nvalue] value used where eqn == -1          */
+    int64_t ndof;
+    const double *params;        /* form parameters                     */
+    int nparams;
+    const double *aux;           /* [nnode] auxiliary nodal field, or NULL              */
+} pyhip_subproblem_desc;
+
+typedef struct pyhip_problem pyhip_problem;
+
+/* Builds the sparsity pattern (host), uploads mesh, maps and pattern (stream: hipStream_t
+ * or NULL for the default stream). */
+int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stream,
+                         pyhip_problem **out);
+int pyhip_problem_destroy(pyhip_problem *p);
+/* ndof, nnz of the structural (column-sorted) pattern, number of element contributions,
+ * bytes of device memory held. */
+int pyhip_problem_info(const pyhip_problem *p, int64_t *ndof, int64_t *nnz,
+                       int64_t *ncontrib, int64_t *device_bytes);
+/* dof offset of each sub-problem, host [nsub+1] */
+int pyhip_problem_dof_offsets(const pyhip_problem *p, int64_t *offsets);
+int pyhip_problem_set_params(pyhip_problem *p, int sub, const double *params, int nparams);
+/* Column-sorted structural pattern to host buffers [ndof+1], [nnz]. */
+int pyhip_pattern_copy(const pyhip_problem *p, int32_t *row_start, int32_t *col_index);
+
+/* ---- assembly ----------------------------------------------------------------------------
+ * Replaces oomph Problem::get_residuals (problem.cc:3810-3900) and Problem::get_jacobian
+ * with the vectors_of_pairs method (problem.cc:4038-4170, 5332-5685) for elements whose
+ * functions come from the JIT table (src/elements.cpp:5054-5127).
+ *   dofs_dev      [ndof] double
+ *   flag          0: residual only, 1: residual and Jacobian (jitbridge.h:285 flag)
+ *   deriv_param   -1, or p >= 0: d/d(global parameter p) of both
+ *                 (get_derivative_wrt_global_parameter, src/problem.cpp:922;
+ *                 fill_in_generic_dresidual_contribution_jit, src/elements.cpp:5005-5052)
+ *   precision     PYHIP_F64 / PYHIP_F32: arithmetic of the element kernels
+ *   residual_dev  [ndof] double (always FP64 accumulation of FP64 or FP32 element values)
+ *   values_dev    [nnz] double (F64) or float (F32), column-sorted structural pattern;
+ *                 may be NULL when flag == 0
+ * Element-order, atomics-free summation: values are bit-for-bit reproducible. */
+int pyhip_assemble(pyhip_problem *p, const double *dofs_dev, int flag, int deriv_param,
+                   int precision, double *residual_dev, void *values_dev);
+
+/* The Jacobian of the last FP64 flag=1 assembly in the reference's CSR form: exact zeros
+ * dropped per element contribution (problem.cc:5524, Numerical_zero_for_sparse_assembly
+ * = 0.0, :110) and columns in first-touch order (problem.cc:5618-5659).
+ * row_start_dev [ndof+1], col_dev / val_dev [nnz] (structural nnz is an upper bound).
+ * *nnz_out receives the number of entries written. */
+int pyhip_export_reference_csr(pyhip_problem *p, const double *values_dev,
+                               int32_t *row_start_dev, int32_t *col_dev, double *val_dev,
+                               int64_t *nnz_out);
+
+/* ---- linear solver ------------------------------------------------------------------------
+ * Preconditioned restarted GMRES on the device, batched over sub-problems (each has its
+ * own Krylov scalars and convergence test). Replaces the SuperLU solve
+ * (linear_solver.cc:846-1105 -> src/pybind/solver.cpp:102-142 ->
+ * pyoomph/solvers/scipy.py:47-104). */
+#define PYHIP_PRECOND_JACOBI 0
+#define PYHIP_PRECOND_ILU0 1
+typedef struct {
+    int precond;        /* PYHIP_PRECOND_*                                            */
+    int precision;      /* Krylov + preconditioner arithmetic: PYHIP_F64 / PYHIP_F32   */
+    int restart;        /* GMRES restart length m                                     */
+    int max_iters;      /* total inner iterations per solve                           */
+    double rtol;        /* relative residual target of the inner solve                */
+    int max_refinements;/* FP64 iterative-refinement sweeps around an FP32 solve       */
+    double refine_rtol; /* relative residual target of the refined solution            */
+} pyhip_linear_opts;
+typedef struct {
+    int iterations;     /* inner iterations summed over sub-problems' maximum          */
+    int refinements;
+    double rel_residual;/* max over sub-problems of |b - A x| / |b| (FP64)             */
+} pyhip_linear_stats;
+void pyhip_linear_opts_default(pyhip_linear_opts *o);
+
+/* ---- Newton ---------------------------------------------------------------------------------
+ * Replaces oomph Problem::newton_solve (problem.cc:8986-9345) / steady_newton_solve
+ * (:9495-9577) with pyoomph's defaults (pyoomph/generic/problem.py:409-411,536). Every
+ * sub-problem runs its own Newton iteration (own convergence test and error state); the
+ * device work of all of them is batched in shared launches. */
+typedef struct {
+    double tol;                      /* Newton_solver_tolerance (1e-8)                */
+    int max_iter;                    /* Max_newton_iterations (10)                     */
+    double max_residuals;            /* Max_residuals (1e10)                           */
+    int always_take_one_newton_step; /* (1)                                            */
+    double relaxation;               /* Relaxation_factor (1.0)                        */
+    int jacobian_precision;          /* element kernels for J: PYHIP_F64 / PYHIP_F32   */
+    pyhip_linear_opts linear;
+} pyhip_newton_opts;
+typedef struct {
+    int newton_iters;       /* Newton steps taken                                     */
+    int linear_iters;       /* inner Krylov iterations, all steps                      */
+    int converged;          /* 1 converged, 0 NewtonSolverError                        */
+    double final_maxres;    /* max |R| after the last step                             */
+    double history[16];     /* max |R| per residual evaluation (Max_res)               */
+    int nhistory;
+} pyhip_newton_stats;
+void pyhip_newton_opts_default(pyhip_newton_opts *o);
+/* dofs_dev [ndof] in/out. stats: host array [nsub]. Returns PYHIP_ERR_NEWTON if any
+ * sub-problem failed (its stats say which). */
+int pyhip_newton_solve(pyhip_problem *p, double *dofs_dev, const pyhip_newton_opts *opts,
+                       pyhip_newton_stats *stats);
+
+/* ---- linear-solver plugin entry --------------------------------------------------------------
+ * Same arguments and meaning as GenericLinearSystemSolver.solve_serial
+ * (pyoomph/solvers/generic.py:77-78) and the extern "C" superlu() wrapper
+ * (src/pybind/solver.cpp:105-108): op_flag 1 = factorise (here: upload + preconditioner
+ * setup), 2 = solve in place in b, 3 = free. The matrix arrives as oomph passes it to
+ * SuperLU: compressed-column arrays (values, rowind, colptr) of J^T, and transpose != 0
+ * asks for J x = b. Host arrays; *handle keeps the factorisation between calls. */
+int pyhip_solve_serial(int op_flag, int n, int nnz, int nrhs, const double *values,
+                       const int32_t *rowind, const int32_t *colptr, double *b, int ldb,
+                       int transpose, void **handle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,169 @@
+"""Oracle element loop, residual and CSR assembly (oomph vectors_of_pairs semantics).
+
+Test infrastructure only (see ``oracle/__init__``).
+
+* ``OracleProblem`` binds a mesh, the values per node, pinning and the weak form, and
+  builds the per-element local equation tables the way oomph's
+  ``FiniteElement::assign_nodal_local_eqn_numbers`` does (nodes in element order, values
+  in value order, pinned values skipped) -- the element's local dof order that indexes the
+  dense element Jacobian.
+* ``get_residuals`` / ``get_jacobian`` restate oomph ``Problem::get_residuals``
+  (problem.cc:3810-3900) and ``Problem::get_jacobian`` with the default
+  ``vectors_of_pairs`` method (problem.cc:4038-4170, 5332-5685), done in ``oracle.c``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import tables as _tables
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+FORM_POISSON, FORM_BRATU, FORM_RD, FORM_NS = 1, 2, 3, 4
+
+# Taylor-Hood slot table (node, value) in oomph local order; must match oracle.c.
+TH_SLOTS = [(0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2), (3, 0), (3, 1),
+            (4, 0), (4, 1), (5, 0), (5, 1), (6, 0), (6, 1), (6, 2), (7, 0), (7, 1), (8, 0),
+            (8, 1), (8, 2)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "_build", "liboracle.so")
+        if not os.path.exists(path):
+            import subprocess
+            subprocess.check_call(["make", "-s", "-C", _HERE])
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        lp = ctypes.POINTER(ctypes.c_long)
+        L.oracle_elements.restype = ctypes.c_int
+        L.oracle_elements.argtypes = [ctypes.c_int] * 6 + [dp] * 4 + [ctypes.c_int, dp, dp, dp,
+                                                                       lp, lp, dp, ctypes.c_int,
+                                                                       ctypes.c_int, ctypes.c_int,
+                                                                       ctypes.c_int, dp, dp]
+        L.oracle_vectors_of_pairs.restype = ctypes.c_long
+        L.oracle_vectors_of_pairs.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_int, lp, lp,
+                                              dp, dp, dp, lp, ctypes.POINTER(lp),
+                                              ctypes.POINTER(dp)]
+        L.oracle_free.argtypes = [ctypes.c_void_p]
+        L.oracle_assemble_residual.restype = None
+        L.oracle_assemble_residual.argtypes = [ctypes.c_int, ctypes.c_int, lp, lp, dp, dp]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+
+
+def _lp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_long))
+
+
+class OracleProblem:
+    """A steady problem on one mesh.
+
+    mesh      -- ``oracle.meshes.OracleMesh``
+    form      -- FORM_* id
+    nvalue    -- values per node (1 scalar, 3 for Taylor-Hood [ux, uy, p])
+    pinned    -- bool [nnode, nvalue]
+    bc        -- float [nnode, nvalue] values of pinned entries
+    params    -- form parameters (Poisson: 4 per Gaussian cx,cy,w,a; Bratu: lambda;
+                 Navier-Stokes: mu, rho)
+    aux       -- nodal auxiliary field (reaction-diffusion source f) or None
+    """
+
+    def __init__(self, mesh, form, nvalue, pinned, bc, params, aux=None):
+        from .meshes import number_equations
+        self.mesh, self.form, self.nvalue = mesh, form, nvalue
+        self.pinned = np.asarray(pinned, dtype=bool)
+        self.bc = np.asarray(bc, dtype=np.float64)
+        self.params = np.ascontiguousarray(params, dtype=np.float64)
+        self.aux = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+        self.eqn, self.ndof = number_equations(mesh.nnode, nvalue, self.pinned)
+        self.tab = _tables.tables(mesh.kind)
+        E = mesh.elements
+        if form == FORM_NS:
+            self.slots = TH_SLOTS
+        else:
+            self.slots = [(l, 0) for l in range(E.shape[1])]
+        ns = len(self.slots)
+        sn = np.array([s[0] for s in self.slots])
+        sv = np.array([s[1] for s in self.slots])
+        self.slot_node = E[:, sn]                     # [nel, nslot] global node
+        self.slot_val = np.broadcast_to(sv, self.slot_node.shape)
+        g = self.eqn[self.slot_node, self.slot_val]   # [nel, nslot] global eqn or -1
+        self.slot_geqn = g
+        self.leqn = np.full(g.shape, -1, dtype=np.int64)
+        free = g >= 0
+        self.leqn[free] = (np.cumsum(free, axis=1) - 1)[free]
+        self.nvar = free.sum(axis=1).astype(np.int64)
+        self.maxdof = ns
+        self.geqn = np.full((E.shape[0], ns), -1, dtype=np.int64)
+        rows = np.nonzero(free)
+        self.geqn[rows[0], self.leqn[rows]] = g[rows]
+
+    def nodal_values(self, x):
+        U = self.bc.copy()
+        m = self.eqn >= 0
+        U[m] = x[self.eqn[m]]
+        return U
+
+    def element_outputs(self, x, flag, deriv=0):
+        """Per-element R [nel, maxdof] and J [nel, maxdof, maxdof] in local numbering."""
+        mesh, tab = self.mesh, self.tab
+        U = self.nodal_values(x)
+        nel, nn = mesh.elements.shape
+        dim = mesh.coords.shape[1]
+        X = np.ascontiguousarray(mesh.coords[mesh.elements])            # [nel, nn, dim]
+        V = np.ascontiguousarray(U[self.slot_node, self.slot_val])       # [nel, nslot]
+        aux = None
+        if self.aux is not None:
+            aux = np.ascontiguousarray(self.aux[mesh.elements])
+        R = np.zeros((nel, self.maxdof))
+        J = np.zeros((nel, self.maxdof, self.maxdof)) if flag else None
+        psi1 = tab.get("psi1")
+        psi1 = None if psi1 is None else np.ascontiguousarray(psi1)
+        rc = lib().oracle_elements(
+            self.form, nel, dim, nn, len(tab["w"]), 4 if psi1 is not None else 0,
+            _dp(np.ascontiguousarray(tab["w"])), _dp(np.ascontiguousarray(tab["psi"])),
+            _dp(np.ascontiguousarray(tab["dpsi"])), _dp(psi1), self.maxdof, _dp(X), _dp(V),
+            _dp(aux), _lp(np.ascontiguousarray(self.leqn)), _lp(self.nvar), _dp(self.params),
+            len(self.params), int(flag), int(deriv), self.maxdof, _dp(R), _dp(J))
+        if rc != 0:
+            raise RuntimeError("oracle_elements failed: %d" % rc)
+        return R, J
+
+    def get_residuals(self, x, deriv=0):
+        """oomph Problem::get_residuals: residuals[eqn] += el_res[i] in element order."""
+        R, _ = self.element_outputs(x, 0, deriv)
+        res = np.zeros(self.ndof)
+        lib().oracle_assemble_residual(R.shape[0], self.maxdof, _lp(self.nvar),
+                                       _lp(np.ascontiguousarray(self.geqn)), _dp(R), _dp(res))
+        return res
+
+    def get_jacobian(self, x, deriv=0):
+        """oomph Problem::get_jacobian (vectors_of_pairs). Returns residual, row_start,
+        column_index (first-touch order), values."""
+        # element Jacobians: block e starts at e*maxdof^2, rows have stride nvar[e]
+        R, J = self.element_outputs(x, 1, deriv)
+        nel = R.shape[0]
+        res = np.zeros(self.ndof)
+        row_start = np.zeros(self.ndof + 1, dtype=np.int64)
+        colp = ctypes.POINTER(ctypes.c_long)()
+        valp = ctypes.POINTER(ctypes.c_double)()
+        L = lib()
+        nnz = L.oracle_vectors_of_pairs(self.ndof, nel, self.maxdof, _lp(self.nvar),
+                                        _lp(np.ascontiguousarray(self.geqn)), _dp(R),
+                                        _dp(J), _dp(res), _lp(row_start),
+                                        ctypes.byref(colp), ctypes.byref(valp))
+        if nnz < 0:
+            raise MemoryError("oracle_vectors_of_pairs")
+        col = np.ctypeslib.as_array(colp, shape=(max(nnz, 1),))[:nnz].copy()
+        val = np.ctypeslib.as_array(valp, shape=(max(nnz, 1),))[:nnz].copy()
+        L.oracle_free(ctypes.cast(colp, ctypes.c_void_p))
+        L.oracle_free(ctypes.cast(valp, ctypes.c_void_p))
+        return res, row_start, col, val

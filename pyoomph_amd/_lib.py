@@ -1,0 +1,131 @@
+"""ctypes binding of libpyoomph_hip.so (the C ABI declared in include/pyoomph_hip.h).
+
+The library is built in-tree (``pyoomph_amd/libpyoomph_hip.so``). Loading fails loudly
+when it is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpyoomph_hip.so")
+
+PYHIP_OK = 0
+PYHIP_ERR_INVALID = -1
+PYHIP_ERR_HIP = -2
+PYHIP_ERR_NOMEM = -3
+PYHIP_ERR_NOT_CONVERGED = -4
+PYHIP_ERR_BREAKDOWN = -5
+PYHIP_ERR_NEWTON = -6
+
+ELEM_TRI_P1, ELEM_TRI_P2, ELEM_QUAD_Q2, ELEM_HEX_Q2 = 1, 2, 3, 4
+ELEM_KIND = {"tri_p1": ELEM_TRI_P1, "tri_p2": ELEM_TRI_P2, "quad_q2": ELEM_QUAD_Q2,
+             "hex_q2": ELEM_HEX_Q2}
+FORM_POISSON, FORM_BRATU, FORM_REACTION_DIFFUSION, FORM_NAVIER_STOKES_TH = 1, 2, 3, 4
+F64, F32 = 0, 1
+PRECOND_JACOBI, PRECOND_ILU0 = 0, 1
+
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class SubproblemDesc(ctypes.Structure):
+    _fields_ = [("form", ctypes.c_int), ("elem_kind", ctypes.c_int),
+                ("nnode", ctypes.c_int64), ("nelem", ctypes.c_int64),
+                ("coords", c_dp), ("conn", c_i32p), ("nvalue", ctypes.c_int),
+                ("eqn", c_i32p), ("pinned_values", c_dp), ("ndof", ctypes.c_int64),
+                ("params", c_dp), ("nparams", ctypes.c_int), ("aux", c_dp)]
+
+
+class ElementTable(ctypes.Structure):
+    _fields_ = [("form", ctypes.c_int), ("elem_kind", ctypes.c_int),
+                ("num_res_jacs", ctypes.c_int), ("res_jac_names", ctypes.c_char_p * 2),
+                ("nodal_values", ctypes.c_int), ("nslot", ctypes.c_int),
+                ("nnode", ctypes.c_int), ("dim", ctypes.c_int),
+                ("integration_order", ctypes.c_int), ("nqp", ctypes.c_int),
+                ("numglobal_params", ctypes.c_int),
+                ("global_param_names", ctypes.c_char_p * 2),
+                ("has_param_derivative", ctypes.c_int * 2), ("moving_nodes", ctypes.c_int)]
+
+
+class LinearOpts(ctypes.Structure):
+    _fields_ = [("precond", ctypes.c_int), ("precision", ctypes.c_int),
+                ("restart", ctypes.c_int), ("max_iters", ctypes.c_int),
+                ("rtol", ctypes.c_double), ("max_refinements", ctypes.c_int),
+                ("refine_rtol", ctypes.c_double)]
+
+
+class NewtonOpts(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int),
+                ("max_residuals", ctypes.c_double),
+                ("always_take_one_newton_step", ctypes.c_int),
+                ("relaxation", ctypes.c_double), ("jacobian_precision", ctypes.c_int),
+                ("linear", LinearOpts)]
+
+
+class NewtonStats(ctypes.Structure):
+    _fields_ = [("newton_iters", ctypes.c_int), ("linear_iters", ctypes.c_int),
+                ("converged", ctypes.c_int), ("final_maxres", ctypes.c_double),
+                ("history", ctypes.c_double * 16), ("nhistory", ctypes.c_int)]
+
+
+# every symbol declared in include/pyoomph_hip.h, with (restype, argtypes)
+SIGNATURES = {
+    "pyhip_last_error": (ctypes.c_char_p, []),
+    "pyhip_version": (ctypes.c_int, []),
+    "pyhip_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "pyhip_element_init": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ElementTable)]),
+    "pyhip_problem_create": (ctypes.c_int, [ctypes.POINTER(SubproblemDesc), ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "pyhip_problem_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "pyhip_problem_info": (ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p, c_i64p, c_i64p]),
+    "pyhip_problem_dof_offsets": (ctypes.c_int, [ctypes.c_void_p, c_i64p]),
+    "pyhip_problem_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dp, ctypes.c_int]),
+    "pyhip_pattern_copy": (ctypes.c_int, [ctypes.c_void_p, c_i32p, c_i32p]),
+    "pyhip_assemble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "pyhip_export_reference_csr": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_void_p, ctypes.c_void_p, c_i64p]),
+    "pyhip_linear_opts_default": (None, [ctypes.POINTER(LinearOpts)]),
+    "pyhip_newton_opts_default": (None, [ctypes.POINTER(NewtonOpts)]),
+    "pyhip_newton_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
+                                          ctypes.POINTER(NewtonStats)]),
+    "pyhip_solve_serial": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          c_dp, c_i32p, c_i32p, c_dp, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libpyoomph_hip.so is not built (run __graft_entry__.build() or "
+                           "make -C pyoomph_amd/csrc); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != PYHIP_OK:
+        msg = load().pyhip_last_error().decode(errors="replace")
+        raise RuntimeError("pyoomph HIP error %d: %s" % (rc, msg))
+    return rc
+
+
+def ptr(t):
+    """Raw pointer of a torch tensor / numpy array."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return ctypes.c_void_p(t.data_ptr())
+    return t.ctypes.data_as(ctypes.c_void_p)

@@ -1,0 +1,422 @@
+// C ABI: element table, problem set-up, assembly and reference-CSR export.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+
+#include "problem.h"
+
+namespace pyhip {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+static int form_nslot(int form, int kind) {
+    if (form == PYHIP_FORM_NAVIER_STOKES_TH) return 22;
+    return elem_nnode(kind);
+}
+static int form_nvalue(int form) { return form == PYHIP_FORM_NAVIER_STOKES_TH ? 3 : 1; }
+
+// (node, value) of each slot (oomph local dof order). Taylor-Hood: see element_kernels.h.
+static void slot_map(int form, int kind, int s, int &node, int &val) {
+    if (form == PYHIP_FORM_NAVIER_STOKES_TH) {
+        static const int n[22] = {0, 0, 0, 1, 1, 2, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6, 6, 7, 7, 8, 8, 8};
+        static const int v[22] = {0, 1, 2, 0, 1, 0, 1, 2, 0, 1, 0, 1, 0, 1, 0, 1, 2, 0, 1, 0, 1, 2};
+        node = n[s];
+        val = v[s];
+    } else {
+        node = s;
+        val = 0;
+    }
+    (void)kind;
+}
+
+template <typename T>
+static int upload_tables(int kind, DevTables<T> **out, int64_t &bytes) {
+    RefTables t;
+    if (fill_ref_tables(kind, &t) != 0) {
+        set_error("no tables for element family");
+        return PYHIP_ERR_INVALID;
+    }
+    DevTables<T> h;
+    memset(&h, 0, sizeof(h));
+    for (int i = 0; i < 27; i++) h.w[i] = T(t.w[i]);
+    for (int i = 0; i < 27 * 27; i++) h.psi[i] = T(t.psi[i]);
+    for (int i = 0; i < 27 * 27 * 3; i++) h.dpsi[i] = T(t.dpsi[i]);
+    for (int i = 0; i < 27 * 4; i++) h.psi1[i] = T(t.psi1[i]);
+    int rc = dev_alloc(out, 1, bytes);
+    if (rc) return rc;
+    PYHIP_CHECK_HIP(hipMemcpy(*out, &h, sizeof(h), hipMemcpyHostToDevice));
+    return PYHIP_OK;
+}
+
+}  // namespace pyhip
+
+using namespace pyhip;
+
+extern "C" {
+
+const char *pyhip_last_error(void) { return g_last_error.c_str(); }
+int pyhip_version(void) { return 1; }
+int pyhip_device_count(int *count) {
+    PYHIP_CHECK_HIP(hipGetDeviceCount(count));
+    return PYHIP_OK;
+}
+
+int pyhip_element_init(int form, int elem_kind, pyhip_element_table *t) {
+    PYHIP_REQUIRE(t, "null table");
+    PYHIP_REQUIRE(elem_nnode(elem_kind) > 0, "unknown element family");
+    PYHIP_REQUIRE(form >= 1 && form <= 4, "unknown weak form");
+    PYHIP_REQUIRE(form != PYHIP_FORM_NAVIER_STOKES_TH || elem_kind == PYHIP_ELEM_QUAD_Q2,
+                  "Taylor-Hood needs Q2 quadrilaterals");
+    memset(t, 0, sizeof(*t));
+    t->form = form;
+    t->elem_kind = elem_kind;
+    t->num_res_jacs = 1;
+    t->res_jac_names[0] = "";
+    t->nodal_values = form_nvalue(form);
+    t->nslot = form_nslot(form, elem_kind);
+    t->nnode = elem_nnode(elem_kind);
+    t->dim = elem_dim(elem_kind);
+    t->integration_order = 0;
+    RefTables rt;
+    fill_ref_tables(elem_kind, &rt);
+    t->nqp = rt.nqp;
+    if (form == PYHIP_FORM_BRATU) {
+        t->numglobal_params = 1;
+        t->global_param_names[0] = "lambda";
+        t->has_param_derivative[0] = 1;
+    }
+    t->moving_nodes = 0;
+    return PYHIP_OK;
+}
+
+int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stream,
+                         pyhip_problem **out) {
+    PYHIP_REQUIRE(subs && nsub > 0 && out, "invalid arguments");
+    *out = nullptr;
+    std::unique_ptr<pyhip_problem> P(new pyhip_problem());
+    P->st = (hipStream_t)stream;
+    P->nsub = nsub;
+    // validate, count, group by (form, kind) in first-appearance order
+    std::vector<std::pair<int, int>> gkeys;
+    for (int s = 0; s < nsub; s++) {
+        const auto &d = subs[s];
+        PYHIP_REQUIRE(elem_nnode(d.elem_kind) > 0, "unknown element family");
+        PYHIP_REQUIRE(d.form >= 1 && d.form <= 4, "unknown weak form");
+        PYHIP_REQUIRE(d.form != PYHIP_FORM_NAVIER_STOKES_TH || d.elem_kind == PYHIP_ELEM_QUAD_Q2,
+                      "Taylor-Hood needs Q2 quadrilaterals");
+        PYHIP_REQUIRE(d.nvalue == form_nvalue(d.form), "values per node do not match the form");
+        PYHIP_REQUIRE(d.nnode > 0 && d.nelem > 0 && d.coords && d.conn && d.eqn && d.pinned_values,
+                      "empty mesh or missing array");
+        PYHIP_REQUIRE(d.nparams >= 0 && d.nparams <= kParamStride, "too many parameters");
+        PYHIP_REQUIRE(d.form != PYHIP_FORM_REACTION_DIFFUSION || d.aux,
+                      "reaction-diffusion needs the nodal source field (aux)");
+        PYHIP_REQUIRE(d.ndof >= 0, "negative ndof");
+        std::pair<int, int> k(d.form, d.elem_kind);
+        if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
+    }
+    P->groups.resize(gkeys.size());
+    for (size_t g = 0; g < gkeys.size(); g++) {
+        Group &G = P->groups[g];
+        G.form = gkeys[g].first;
+        G.kind = gkeys[g].second;
+        G.ns = form_nslot(G.form, G.kind);
+        G.nn = elem_nnode(G.kind);
+        G.dim = elem_dim(G.kind);
+    }
+    P->subs.resize(nsub);
+    P->dof_off_h.assign(nsub + 1, 0);
+    int64_t node_off = 0;
+    for (int s = 0; s < nsub; s++) {
+        const auto &d = subs[s];
+        SubInfo &S = P->subs[s];
+        S.form = d.form;
+        S.kind = d.elem_kind;
+        S.nvalue = d.nvalue;
+        S.dof_off = P->dof_off_h[s];
+        S.ndof = d.ndof;
+        S.node_off = node_off;
+        S.nnode = d.nnode;
+        S.group = (int)(std::find(gkeys.begin(), gkeys.end(), std::make_pair(d.form, d.elem_kind)) -
+                        gkeys.begin());
+        S.el_off = P->groups[S.group].nel;
+        S.nel = d.nelem;
+        P->groups[S.group].nel += d.nelem;
+        P->dof_off_h[s + 1] = S.dof_off + d.ndof;
+        node_off += d.nnode;
+    }
+    P->ndof = P->dof_off_h[nsub];
+    P->nnode = node_off;
+    PYHIP_REQUIRE(P->ndof < INT32_MAX, "too many dofs");
+    int64_t er = 0, ej = 0;
+    for (auto &G : P->groups) {
+        G.er_off = er;
+        G.ej_off = ej;
+        er += G.nel * G.ns;
+        ej += G.nel * G.ns * G.ns;
+    }
+    P->n_er = er;
+    P->n_ej = ej;
+
+    // host staging: element-major global equations per group, slot-major copies
+    std::vector<std::vector<int32_t>> seqn_em(P->groups.size()), conn_sm(P->groups.size()),
+        seqn_sm(P->groups.size()), sub_g(P->groups.size());
+    for (size_t g = 0; g < P->groups.size(); g++) {
+        const Group &G = P->groups[g];
+        seqn_em[g].resize(G.nel * G.ns);
+        conn_sm[g].resize(G.nn * G.nel);
+        seqn_sm[g].resize(G.ns * G.nel);
+        sub_g[g].resize(G.nel);
+    }
+    std::vector<double> coords_h(P->nnode * 3, 0.0), pinned_h(P->nnode * 3, 0.0),
+        aux_h(P->nnode, 0.0);
+    P->params_h.assign((size_t)nsub * kParamStride, 0.0);
+    for (int s = 0; s < nsub; s++) {
+        const auto &d = subs[s];
+        const SubInfo &S = P->subs[s];
+        const Group &G = P->groups[S.group];
+        const int dim = G.dim;
+        for (int64_t n = 0; n < d.nnode; n++) {
+            for (int k = 0; k < dim; k++) coords_h[(S.node_off + n) * 3 + k] = d.coords[n * dim + k];
+            for (int v = 0; v < d.nvalue; v++) {
+                pinned_h[(S.node_off + n) * 3 + v] = d.pinned_values[n * d.nvalue + v];
+                const int32_t q = d.eqn[n * d.nvalue + v];
+                PYHIP_REQUIRE(q >= -1 && q < d.ndof, "equation number out of range");
+            }
+            if (d.aux) aux_h[S.node_off + n] = d.aux[n];
+        }
+        for (int k = 0; k < d.nparams; k++) P->params_h[(size_t)s * kParamStride + k] = d.params[k];
+        std::vector<int32_t> &sem = seqn_em[S.group];
+        for (int64_t e = 0; e < d.nelem; e++) {
+            const int64_t ge = S.el_off + e;
+            sub_g[S.group][ge] = s;
+            for (int l = 0; l < G.nn; l++) {
+                const int32_t n = d.conn[e * G.nn + l];
+                PYHIP_REQUIRE(n >= 0 && n < d.nnode, "connectivity out of range");
+                conn_sm[S.group][(int64_t)l * G.nel + ge] = (int32_t)(S.node_off + n);
+            }
+            for (int sl = 0; sl < G.ns; sl++) {
+                int ln, lv;
+                slot_map(G.form, G.kind, sl, ln, lv);
+                const int32_t n = d.conn[e * G.nn + ln];
+                const int32_t q = d.eqn[(int64_t)n * d.nvalue + lv];
+                const int32_t gq = q >= 0 ? (int32_t)(q + S.dof_off) : -1;
+                sem[ge * G.ns + sl] = gq;
+                seqn_sm[S.group][(int64_t)sl * G.nel + ge] = gq;
+            }
+        }
+    }
+    // pattern: element blocks in element order of each sub-problem
+    std::vector<ElementBlock> blocks;
+    for (int s = 0; s < nsub; s++) {
+        const SubInfo &S = P->subs[s];
+        const Group &G = P->groups[S.group];
+        ElementBlock B;
+        B.ns = G.ns;
+        B.nel = S.nel;
+        B.seqn = seqn_em[S.group].data() + S.el_off * G.ns;
+        B.er_off = G.er_off + S.el_off * G.ns;
+        B.ej_off = G.ej_off + S.el_off * G.ns * G.ns;
+        blocks.push_back(B);
+    }
+    HostPattern hp;
+    std::string err;
+    if (build_pattern(P->ndof, blocks, hp, err) != 0) {
+        set_error("pattern: " + err);
+        return PYHIP_ERR_INVALID;
+    }
+    P->nnz = hp.nnz;
+    P->nocc = hp.nocc;
+    P->ncontrib = hp.ncontrib;
+    std::vector<int32_t> dof_sub_h(P->ndof);
+    for (int s = 0; s < nsub; s++)
+        for (int64_t r = P->dof_off_h[s]; r < P->dof_off_h[s + 1]; r++) dof_sub_h[r] = s;
+
+    // uploads
+    int64_t &B = P->device_bytes;
+    int rc;
+#define UP(dst, src, n)                                                                   \
+    do {                                                                                  \
+        rc = dev_alloc(&(dst), (n), B);                                                   \
+        if (rc) return rc;                                                                \
+        if ((n) > 0)                                                                      \
+            PYHIP_CHECK_HIP(hipMemcpy((dst), (src), (size_t)(n) * sizeof(*(dst)),         \
+                                      hipMemcpyHostToDevice));                            \
+    } while (0)
+    UP(P->coords, coords_h.data(), coords_h.size());
+    UP(P->pinned, pinned_h.data(), pinned_h.size());
+    UP(P->aux, aux_h.data(), aux_h.size());
+    UP(P->params, P->params_h.data(), P->params_h.size());
+    for (size_t g = 0; g < P->groups.size(); g++) {
+        Group &G = P->groups[g];
+        UP(G.conn, conn_sm[g].data(), conn_sm[g].size());
+        UP(G.seqn, seqn_sm[g].data(), seqn_sm[g].size());
+        UP(G.sub, sub_g[g].data(), sub_g[g].size());
+        if (!P->tab64[G.kind]) {
+            rc = upload_tables<double>(G.kind, &P->tab64[G.kind], B);
+            if (rc) return rc;
+            rc = upload_tables<float>(G.kind, &P->tab32[G.kind], B);
+            if (rc) return rc;
+        }
+    }
+    UP(P->row_start, hp.row_start.data(), hp.row_start.size());
+    UP(P->col, hp.col.data(), hp.col.size());
+    UP(P->diag, hp.diag_pos.data(), hp.diag_pos.size());
+    UP(P->occ_ptr, hp.occ_ptr.data(), hp.occ_ptr.size());
+    UP(P->occ_roff, hp.occ_roff.data(), hp.occ_roff.size());
+    UP(P->occ_eoff, hp.occ_eoff.data(), hp.occ_eoff.size());
+    UP(P->occ_pos, hp.occ_pos.data(), hp.occ_pos.size());
+    UP(P->dof_sub, dof_sub_h.data(), dof_sub_h.size());
+    UP(P->dof_off, P->dof_off_h.data(), P->dof_off_h.size());
+#undef UP
+    P->row_start_h = std::move(hp.row_start);
+    P->col_h = std::move(hp.col);
+    *out = P.release();
+    return PYHIP_OK;
+}
+
+void pyhip_krylov_free(pyhip_problem *p);
+
+int pyhip_problem_destroy(pyhip_problem *p) {
+    if (!p) return PYHIP_OK;
+    pyhip_krylov_free(p);
+    for (auto &G : p->groups) {
+        dfree(G.conn);
+        dfree(G.seqn);
+        dfree(G.sub);
+    }
+    void *ptrs[] = {p->coords, p->pinned, p->aux, p->params, p->row_start, p->col, p->diag,
+                    p->occ_ptr, p->occ_roff, p->occ_eoff, p->occ_pos, p->dof_sub, p->dof_off,
+                    p->Er64, p->Ej64, p->Er32, p->Ej32, p->keys, p->kept};
+    for (void *q : ptrs)
+        dfree(q);
+    for (int k = 0; k < 5; k++) {
+        dfree(p->tab64[k]);
+        dfree(p->tab32[k]);
+    }
+    delete p;
+    return PYHIP_OK;
+}
+
+int pyhip_problem_info(const pyhip_problem *p, int64_t *ndof, int64_t *nnz, int64_t *ncontrib,
+                       int64_t *device_bytes) {
+    PYHIP_REQUIRE(p, "null problem");
+    if (ndof) *ndof = p->ndof;
+    if (nnz) *nnz = p->nnz;
+    if (ncontrib) *ncontrib = p->ncontrib;
+    if (device_bytes) *device_bytes = p->device_bytes;
+    return PYHIP_OK;
+}
+
+int pyhip_problem_dof_offsets(const pyhip_problem *p, int64_t *offsets) {
+    PYHIP_REQUIRE(p && offsets, "null argument");
+    for (int s = 0; s <= p->nsub; s++) offsets[s] = p->dof_off_h[s];
+    return PYHIP_OK;
+}
+
+int pyhip_problem_set_params(pyhip_problem *p, int sub, const double *params, int nparams) {
+    PYHIP_REQUIRE(p && sub >= 0 && sub < p->nsub, "bad sub-problem index");
+    PYHIP_REQUIRE(nparams >= 0 && nparams <= kParamStride && (params || nparams == 0),
+                  "bad parameters");
+    for (int k = 0; k < nparams; k++) p->params_h[(size_t)sub * kParamStride + k] = params[k];
+    PYHIP_CHECK_HIP(hipMemcpyAsync(p->params + (size_t)sub * kParamStride,
+                                   p->params_h.data() + (size_t)sub * kParamStride,
+                                   kParamStride * sizeof(double), hipMemcpyHostToDevice, p->st));
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    return PYHIP_OK;
+}
+
+int pyhip_pattern_copy(const pyhip_problem *p, int32_t *row_start, int32_t *col_index) {
+    PYHIP_REQUIRE(p && row_start && col_index, "null argument");
+    memcpy(row_start, p->row_start_h.data(), p->row_start_h.size() * sizeof(int32_t));
+    memcpy(col_index, p->col_h.data(), p->col_h.size() * sizeof(int32_t));
+    return PYHIP_OK;
+}
+
+int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_param,
+                   int precision, double *residual, void *values) {
+    PYHIP_REQUIRE(p && dofs && residual, "null argument");
+    PYHIP_REQUIRE(flag == 0 || flag == 1, "flag must be 0 or 1");
+    PYHIP_REQUIRE(precision == PYHIP_F64 || precision == PYHIP_F32, "bad precision");
+    PYHIP_REQUIRE(flag == 0 || values, "values buffer required for the Jacobian");
+    PYHIP_REQUIRE(deriv_param == -1 || deriv_param == 0, "unknown global parameter");
+    for (auto &G : p->groups)
+        PYHIP_REQUIRE(deriv_param < 0 || G.form == PYHIP_FORM_BRATU,
+                      "parameter derivative only defined for the Bratu form");
+    int rc;
+    if (precision == PYHIP_F64) {
+        if (!p->Er64) {
+            if ((rc = dev_alloc(&p->Er64, p->n_er, p->device_bytes))) return rc;
+        }
+        if (flag && !p->Ej64) {
+            if ((rc = dev_alloc(&p->Ej64, p->n_ej, p->device_bytes))) return rc;
+        }
+    } else {
+        if (!p->Er32) {
+            if ((rc = dev_alloc(&p->Er32, p->n_er, p->device_bytes))) return rc;
+        }
+        if (flag && !p->Ej32) {
+            if ((rc = dev_alloc(&p->Ej32, p->n_ej, p->device_bytes))) return rc;
+        }
+    }
+    for (auto &G : p->groups) {
+        if (precision == PYHIP_F64) {
+            ElemArgs<double> a{};
+            a.nel = (int)G.nel;
+            a.conn = G.conn; a.seqn = G.seqn; a.sub = G.sub;
+            a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
+            a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
+            a.tab = p->tab64[G.kind];
+            a.Er = p->Er64 + G.er_off;
+            a.Ej = flag ? p->Ej64 + G.ej_off : nullptr;
+            a.flag = flag; a.deriv = deriv_param >= 0;
+            if ((rc = launch_elements_f64(G.kind, G.form, a, p->st))) return rc;
+        } else {
+            ElemArgs<float> a{};
+            a.nel = (int)G.nel;
+            a.conn = G.conn; a.seqn = G.seqn; a.sub = G.sub;
+            a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
+            a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
+            a.tab = p->tab32[G.kind];
+            a.Er = p->Er32 + G.er_off;
+            a.Ej = flag ? p->Ej32 + G.ej_off : nullptr;
+            a.flag = flag; a.deriv = deriv_param >= 0;
+            if ((rc = launch_elements_f32(G.kind, G.form, a, p->st))) return rc;
+        }
+    }
+    if (precision == PYHIP_F64) {
+        if ((rc = launch_gather_residual_f64(p->ndof, p->occ_ptr, p->occ_roff, p->Er64, residual, p->st)))
+            return rc;
+        if (flag) {
+            if ((rc = launch_gather_values_f64(p->ndof, p->row_start, p->occ_ptr, p->occ_eoff,
+                                               p->occ_pos, p->Ej64, (double *)values, p->st)))
+                return rc;
+            p->have_ej64 = (deriv_param < 0);
+        }
+    } else {
+        if ((rc = launch_gather_residual_f32(p->ndof, p->occ_ptr, p->occ_roff, p->Er32, residual, p->st)))
+            return rc;
+        if (flag) {
+            if ((rc = launch_gather_values_f32(p->ndof, p->row_start, p->occ_ptr, p->occ_eoff,
+                                               p->occ_pos, p->Ej32, (float *)values, p->st)))
+                return rc;
+        }
+    }
+    return PYHIP_OK;
+}
+
+int pyhip_export_reference_csr(pyhip_problem *p, const double *values, int32_t *rs_out,
+                               int32_t *col_out, double *val_out, int64_t *nnz_out) {
+    PYHIP_REQUIRE(p && values && rs_out && col_out && val_out && nnz_out, "null argument");
+    PYHIP_REQUIRE(p->have_ej64 && p->Ej64, "no FP64 Jacobian assembled yet");
+    int rc;
+    if (!p->keys) {
+        if ((rc = dev_alloc(&p->keys, p->nnz, p->device_bytes))) return rc;
+        if ((rc = dev_alloc(&p->kept, p->ndof + 1, p->device_bytes))) return rc;
+    }
+    return export_reference_csr(p->ndof, p->row_start, p->col, p->occ_ptr, p->occ_eoff,
+                                p->occ_pos, p->Ej64, values, p->keys, p->kept, rs_out, col_out,
+                                val_out, nnz_out, p->st);
+}
+
+}  // extern "C"

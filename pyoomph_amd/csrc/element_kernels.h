@@ -1,0 +1,434 @@
+// Batched element residual / Jacobian / parameter-derivative kernels.
+//
+// Included by elements_f64.hip (built with -ffp-contract=off: the FP64 kernels execute
+// exactly the written operation order, the same order as the CPU restatement of the
+// generated element code) and elements_f32.hip (FP32, contraction allowed).
+//
+// What each thread computes follows the code pyoomph generates per element
+// (src/codegen.cpp:3912-4109, 2015-2257; accumulation macros src/jitbridge_nohang.h:
+// 23-50): integration-point loop, geometry as BulkElementBase::fill_shape_info_at_s
+// (src/elements.cpp:3593-3700, 3800-3840, 3960-3990) with weight*det
+// (src/elements.cpp:4564-4575), field interpolation, then the test-function row and its
+// shape-function columns, skipping pinned equations (local_eqn < 0).
+//
+// Mapping (one workgroup = a block of elements, 64-wide waves):
+//   thread = (element, slot row i); EPB = 256 / NS elements per workgroup.
+//   Per element the nodal coordinates, slot values, equations and auxiliary field are
+//   staged once in LDS (coalesced slot-major gathers), so the NS threads of an element
+//   read them as LDS broadcasts. Per integration point every thread forms the 2x2/3x3
+//   metric (cheap, redundant), the thread of node l writes dx_shape[l] to LDS, and every
+//   thread then runs its Jacobian row over the columns reading dx_shape from LDS.
+//   Reference-element tables (weights, psi, dpsi/ds) are read with uniform indices, so
+//   they come through the scalar cache.
+// Output: element vectors Er[e][i] and matrices Ej[e][i][j] in slot numbering; the
+// deterministic gather (scatter.hip) sums them into the residual and the CSR values.
+#pragma once
+#include "common.h"
+
+namespace pyhip {
+
+template <typename T>
+struct DevTables {
+    T w[27];
+    T psi[27 * 27];
+    T dpsi[27 * 27 * 3];
+    T psi1[27 * 4];
+};
+
+template <typename T>
+struct ElemArgs {
+    int nel;                  // elements in this launch (one element group)
+    const int32_t *conn;      // [nnode_el][nel] global node (slot-major)
+    const int32_t *seqn;      // [nslot][nel] global equation, -1 pinned
+    const int32_t *sub;       // [nel] sub-problem id
+    const double *coords;     // [nnode_total][dim]
+    const double *dofs;       // [ndof_total]
+    const double *pinned;     // [nnode_total][nvalue]
+    const double *aux;        // [nnode_total] or nullptr
+    const double *params;     // [nsub][pstride]; Poisson: n, then (cx, cy, w, a) * n
+    int pstride;
+    int nvalue;
+    const DevTables<T> *tab;
+    T *Er;                    // [nel][nslot]
+    T *Ej;                    // [nel][nslot][nslot] or nullptr
+    int flag;                 // 0 residual, 1 residual + Jacobian
+    int deriv;                // 1: d/dlambda (Bratu)
+};
+
+template <int KIND> struct ElemTraits;
+template <> struct ElemTraits<ELEM_TRI_P1> { static constexpr int NN = 3, DIM = 2, NQ = 3; };
+template <> struct ElemTraits<ELEM_TRI_P2> { static constexpr int NN = 6, DIM = 2, NQ = 7; };
+template <> struct ElemTraits<ELEM_QUAD_Q2> { static constexpr int NN = 9, DIM = 2, NQ = 9; };
+template <> struct ElemTraits<ELEM_HEX_Q2> { static constexpr int NN = 27, DIM = 3, NQ = 27; };
+
+// Taylor-Hood Q2/Q1 slot layout = oomph local dof order (node-major, value-minor):
+// node 0: ux uy p | 1: ux uy | 2: ux uy p | 3,4,5: ux uy | 6: ux uy p | 7: ux uy | 8: ux uy p
+__device__ __constant__ const int8_t kTHSlotNode[22] = {0, 0, 0, 1, 1, 2, 2, 2, 3, 3, 4,
+                                                        4, 5, 5, 6, 6, 6, 7, 7, 8, 8, 8};
+__device__ __constant__ const int8_t kTHSlotKind[22] = {0, 1, 2, 0, 1, 0, 1, 2, 0, 1, 0,
+                                                        1, 0, 1, 0, 1, 2, 0, 1, 0, 1, 2};
+__device__ __constant__ const int8_t kTHSlotP[22] = {-1, -1, 0,  -1, -1, -1, -1, 1,  -1, -1, -1,
+                                                     -1, -1, -1, -1, -1, 2,  -1, -1, -1, -1, 3};
+// velocity-x slot of each node (velocity-y is the next slot)
+__device__ __constant__ const int8_t kTHNodeSlot[9] = {0, 3, 5, 8, 10, 12, 14, 17, 19};
+
+__device__ __forceinline__ double texp(double x) { return ::exp(x); }
+__device__ __forceinline__ float texp(float x) { return ::expf(x); }
+__device__ __forceinline__ double tsqrt(double x) { return ::sqrt(x); }
+__device__ __forceinline__ float tsqrt(float x) { return ::sqrtf(x); }
+
+// Metric inverse and g^{ab} g_a (src/elements.cpp:3655-3700 / 3800-3840), from the
+// tangents t[a][i]. Returns w*sqrt(det).
+template <typename T, int DIM>
+__device__ __forceinline__ T metric(const T (&t)[DIM][DIM], T wq, T (&g)[DIM][DIM]) {
+    T am[DIM][DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; a++)
+#pragma unroll
+        for (int b = 0; b < DIM; b++) {
+            T s = T(0);
+#pragma unroll
+            for (int i = 0; i < DIM; i++) s = s + t[a][i] * t[b][i];
+            am[a][b] = s;
+        }
+    T au[DIM][DIM];
+    T det_a;
+    if constexpr (DIM == 2) {
+        det_a = am[0][0] * am[1][1] - am[0][1] * am[1][0];
+        au[0][0] = am[1][1] / det_a;
+        au[0][1] = -am[0][1] / det_a;
+        au[1][0] = -am[1][0] / det_a;
+        au[1][1] = am[0][0] / det_a;
+    } else {
+        det_a = am[0][0] * am[1][1] * am[2][2] + am[0][1] * am[1][2] * am[2][0] +
+                am[0][2] * am[1][0] * am[2][1] - am[0][0] * am[1][2] * am[2][1] -
+                am[0][1] * am[1][0] * am[2][2] - am[0][2] * am[1][1] * am[2][0];
+        au[0][0] = (am[1][1] * am[2][2] - am[1][2] * am[2][1]) / det_a;
+        au[0][1] = -(am[0][1] * am[2][2] - am[0][2] * am[2][1]) / det_a;
+        au[0][2] = (am[0][1] * am[1][2] - am[0][2] * am[1][1]) / det_a;
+        au[1][0] = -(am[1][0] * am[2][2] - am[1][2] * am[2][0]) / det_a;
+        au[1][1] = (am[0][0] * am[2][2] - am[0][2] * am[2][0]) / det_a;
+        au[1][2] = -(am[0][0] * am[1][2] - am[0][2] * am[1][0]) / det_a;
+        au[2][0] = (am[1][0] * am[2][1] - am[1][1] * am[2][0]) / det_a;
+        au[2][1] = -(am[0][0] * am[2][1] - am[0][1] * am[2][0]) / det_a;
+        au[2][2] = (am[0][0] * am[1][1] - am[0][1] * am[1][0]) / det_a;
+    }
+#pragma unroll
+    for (int b = 0; b < DIM; b++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            T s = au[0][b] * t[0][i] + au[1][b] * t[1][i];
+            if constexpr (DIM == 3) s = s + au[2][b] * t[2][i];
+            g[b][i] = s;
+        }
+    return wq * tsqrt(det_a);
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ T dotv(const T *a, const T *b) {
+    T s = a[0] * b[0] + a[1] * b[1];
+    if constexpr (DIM == 3) s = s + a[2] * b[2];
+    return s;
+}
+
+// Scalar weak forms (Poisson / Bratu / reaction-diffusion): slot = node.
+template <typename T, int KIND, int FORM>
+__global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
+    using Tr = ElemTraits<KIND>;
+    constexpr int NN = Tr::NN, DIM = Tr::DIM, NQ = Tr::NQ, NS = NN;
+    constexpr int EPB = 256 / NS;
+    __shared__ T sX[EPB][NN][DIM];
+    __shared__ T sU[EPB][NN];
+    __shared__ T sF[EPB][NN];
+    __shared__ T sDx[EPB][NN][DIM];
+    __shared__ int sEq[EPB][NS];
+
+    const int le = threadIdx.x / NS;        // element within block
+    const int row = threadIdx.x % NS;       // slot row of this thread
+    const int e = blockIdx.x * EPB + le;
+    const bool active = (le < EPB) && (e < a.nel);
+
+    if (active) {
+        const int node = a.conn[(int64_t)row * a.nel + e];
+        const int eq = a.seqn[(int64_t)row * a.nel + e];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) sX[le][row][d] = T(a.coords[(int64_t)node * 3 + d]);
+        sU[le][row] = T(eq >= 0 ? a.dofs[eq] : a.pinned[(int64_t)node * a.nvalue]);
+        if constexpr (FORM == PYHIP_FORM_REACTION_DIFFUSION) sF[le][row] = T(a.aux[node]);
+        sEq[le][row] = eq;
+    }
+    __syncthreads();
+
+    T par0 = T(0);
+    const double *par = nullptr;
+    if (active) {
+        par = a.params + (int64_t)a.sub[e] * a.pstride;
+        par0 = T(par[0]);
+    }
+    const bool myrow = active && sEq[le][row] >= 0;
+    T Racc = T(0);
+    T Jacc[NS];
+#pragma unroll
+    for (int j = 0; j < NS; j++) Jacc[j] = T(0);
+
+    for (int q = 0; q < NQ; q++) {
+        const T *D = a.tab->dpsi + q * NN * DIM;
+        const T *P = a.tab->psi + q * NN;
+        T W = T(0);
+        T g[DIM][DIM];
+        if (active) {
+            T t[DIM][DIM];
+#pragma unroll
+            for (int i = 0; i < DIM; i++)
+#pragma unroll
+                for (int j = 0; j < DIM; j++) t[i][j] = T(0);
+#pragma unroll
+            for (int l = 0; l < NN; l++)
+#pragma unroll
+                for (int i = 0; i < DIM; i++)
+#pragma unroll
+                    for (int j = 0; j < DIM; j++)
+                        t[j][i] = t[j][i] + sX[le][l][i] * D[l * DIM + j];
+            W = metric<T, DIM>(t, a.tab->w[q], g);
+#pragma unroll
+            for (int i = 0; i < DIM; i++) {
+                T s = T(0);
+#pragma unroll
+                for (int b = 0; b < DIM; b++) s = s + g[b][i] * D[row * DIM + b];
+                sDx[le][row][i] = s;
+            }
+        }
+        __syncthreads();
+        if (active) {
+            T u = T(0), gu[DIM];
+#pragma unroll
+            for (int l = 0; l < NN; l++) u = u + sU[le][l] * P[l];
+#pragma unroll
+            for (int i = 0; i < DIM; i++) {
+                T s = T(0);
+#pragma unroll
+                for (int l = 0; l < NN; l++) s = s + sU[le][l] * sDx[le][l][i];
+                gu[i] = s;
+            }
+            T src = T(0), dsrc = T(0);
+            if constexpr (FORM == PYHIP_FORM_POISSON) {
+                T x = T(0), y = T(0);
+#pragma unroll
+                for (int l = 0; l < NN; l++) x = x + sX[le][l][0] * P[l];
+#pragma unroll
+                for (int l = 0; l < NN; l++) y = y + sX[le][l][1] * P[l];
+                T f = T(0);
+                const int ng = (int)par[0];  // Gaussians follow as (cx, cy, w, a)
+                for (int k = 1; k < 1 + 4 * ng; k += 4) {
+                    T ddx = x - T(par[k + 0]), ddy = y - T(par[k + 1]);
+                    T r2 = ddx * ddx + ddy * ddy;
+                    T wk = T(par[k + 2]);
+                    f = f + T(par[k + 3]) * texp(-r2 / (wk * wk));
+                }
+                src = -f;
+            } else if constexpr (FORM == PYHIP_FORM_BRATU) {
+                T ev = texp(u);
+                if (a.deriv) {
+                    src = -ev;
+                    dsrc = -ev;
+                } else {
+                    src = -par0 * ev;
+                    dsrc = -par0 * ev;
+                }
+            } else {
+                T f = T(0);
+#pragma unroll
+                for (int l = 0; l < NN; l++) f = f + sF[le][l] * P[l];
+                src = u * u * u - f;
+                dsrc = T(3) * (u * u);
+            }
+            if (myrow) {
+                const T *dxi = sDx[le][row];
+                if (a.deriv)
+                    Racc = Racc + W * (src * P[row]);
+                else
+                    Racc = Racc + W * (dotv<T, DIM>(gu, dxi) + src * P[row]);
+                if (a.flag) {
+#pragma unroll
+                    for (int j = 0; j < NS; j++) {
+                        if (sEq[le][j] < 0) continue;
+                        const T *dxj = sDx[le][j];
+                        T v;
+                        if (a.deriv)
+                            v = W * ((dsrc * P[j]) * P[row]);
+                        else if constexpr (FORM == PYHIP_FORM_POISSON)
+                            v = W * dotv<T, DIM>(dxj, dxi);
+                        else
+                            v = W * (dotv<T, DIM>(dxj, dxi) + (dsrc * P[j]) * P[row]);
+                        Jacc[j] = Jacc[j] + v;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (myrow) {
+        a.Er[(int64_t)e * NS + row] = Racc;
+        if (a.flag) {
+            T *dst = a.Ej + ((int64_t)e * NS + row) * NS;
+#pragma unroll
+            for (int j = 0; j < NS; j++) dst[j] = Jacc[j];
+        }
+    }
+}
+
+// Taylor-Hood Q2/Q1 steady Navier-Stokes (22 slots, 9 geometric nodes).
+template <typename T>
+__global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
+    constexpr int NN = 9, DIM = 2, NQ = 9, NS = 22;
+    constexpr int EPB = 256 / NS;  // 11 elements, 242 threads
+    __shared__ T sX[EPB][NN][DIM];
+    __shared__ T sV[EPB][NS];
+    __shared__ T sDx[EPB][NN][DIM];
+    __shared__ int sEq[EPB][NS];
+
+    const int le = threadIdx.x / NS;
+    const int slot = threadIdx.x % NS;
+    const int e = blockIdx.x * EPB + le;
+    const bool active = (le < EPB) && (e < a.nel);
+    if (active) {
+        const int ln = kTHSlotNode[slot];
+        const int kind = kTHSlotKind[slot];
+        const int node = a.conn[(int64_t)ln * a.nel + e];
+        const int eq = a.seqn[(int64_t)slot * a.nel + e];
+        if (slot < NN) {
+            const int gn = a.conn[(int64_t)slot * a.nel + e];
+            sX[le][slot][0] = T(a.coords[(int64_t)gn * 3 + 0]);
+            sX[le][slot][1] = T(a.coords[(int64_t)gn * 3 + 1]);
+        }
+        sV[le][slot] = T(eq >= 0 ? a.dofs[eq] : a.pinned[(int64_t)node * a.nvalue + kind]);
+        sEq[le][slot] = eq;
+    }
+    __syncthreads();
+    T mu = T(0), rho = T(0);
+    if (active) {
+        const double *par = a.params + (int64_t)a.sub[e] * a.pstride;
+        mu = T(par[0]);
+        rho = T(par[1]);
+    }
+    const bool myrow = active && sEq[le][slot] >= 0;
+    const int kt = kTHSlotKind[slot], lt = kTHSlotNode[slot];
+    T Racc = T(0);
+    T Jacc[NS];
+#pragma unroll
+    for (int j = 0; j < NS; j++) Jacc[j] = T(0);
+
+    for (int q = 0; q < NQ; q++) {
+        const T *D = a.tab->dpsi + q * NN * DIM;
+        const T *P = a.tab->psi + q * NN;
+        const T *C = a.tab->psi1 + q * 4;
+        T W = T(0);
+        if (active) {
+            T t[DIM][DIM] = {{T(0), T(0)}, {T(0), T(0)}};
+#pragma unroll
+            for (int l = 0; l < NN; l++)
+#pragma unroll
+                for (int i = 0; i < DIM; i++)
+#pragma unroll
+                    for (int j = 0; j < DIM; j++)
+                        t[j][i] = t[j][i] + sX[le][l][i] * D[l * DIM + j];
+            T g[DIM][DIM];
+            W = metric<T, DIM>(t, a.tab->w[q], g);
+            if (slot < NN) {
+#pragma unroll
+                for (int i = 0; i < DIM; i++) {
+                    T s = T(0);
+#pragma unroll
+                    for (int b = 0; b < DIM; b++) s = s + g[b][i] * D[slot * DIM + b];
+                    sDx[le][slot][i] = s;
+                }
+            }
+        }
+        __syncthreads();
+        if (myrow) {
+            // nodal values by (node, component) from the slot table
+            T u0 = T(0), u1 = T(0), p = T(0);
+            T du[2][2];
+#pragma unroll
+            for (int l = 0; l < NN; l++) u0 = u0 + sV[le][kTHNodeSlot[l]] * P[l];
+#pragma unroll
+            for (int l = 0; l < NN; l++) u1 = u1 + sV[le][1 + kTHNodeSlot[l]] * P[l];
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                T s0 = T(0), s1 = T(0);
+#pragma unroll
+                for (int l = 0; l < NN; l++) {
+                    const int sx = kTHNodeSlot[l];
+                    s0 = s0 + sV[le][sx] * sDx[le][l][b];
+                }
+#pragma unroll
+                for (int l = 0; l < NN; l++) {
+                    const int sx = kTHNodeSlot[l];
+                    s1 = s1 + sV[le][sx + 1] * sDx[le][l][b];
+                }
+                du[0][b] = s0;
+                du[1][b] = s1;
+            }
+            // pressure slots 2, 7, 16, 21 hold corner nodes 0, 2, 6, 8
+            p = p + sV[le][2] * C[0];
+            p = p + sV[le][7] * C[1];
+            p = p + sV[le][16] * C[2];
+            p = p + sV[le][21] * C[3];
+            T sig[2][2];
+#pragma unroll
+            for (int aa = 0; aa < 2; aa++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    T s = mu * (du[aa][b] + du[b][aa]);
+                    if (aa == b) s = s - p;
+                    sig[aa][b] = s;
+                }
+            const T *dxl = sDx[le][lt];
+            if (kt < 2) {
+                const T conv = rho * (u0 * du[kt][0] + u1 * du[kt][1]);
+                Racc = Racc + W * ((sig[kt][0] * dxl[0] + sig[kt][1] * dxl[1]) + conv * P[lt]);
+            } else {
+                const int pidx = kTHSlotP[slot];
+                Racc = Racc + W * ((du[0][0] + du[1][1]) * C[pidx]);
+            }
+            if (a.flag) {
+#pragma unroll
+                for (int ss = 0; ss < NS; ss++) {
+                    if (sEq[le][ss] < 0) continue;
+                    const int ks = kTHSlotKind[ss], ls = kTHSlotNode[ss];
+                    T v;
+                    if (kt < 2 && ks < 2) {
+                        const T *dxs = sDx[le][ls];
+                        T visc = dxs[kt] * dxl[ks];
+                        T cv = P[ls] * du[kt][ks];
+                        if (kt == ks) {
+                            visc = (dxs[0] * dxl[0] + dxs[1] * dxl[1]) + visc;
+                            cv = cv + (u0 * dxs[0] + u1 * dxs[1]);
+                        }
+                        v = W * (mu * visc + (rho * cv) * P[lt]);
+                    } else if (kt < 2 && ks == 2) {
+                        const int pidx = kTHSlotP[ss];
+                        v = W * (-(C[pidx] * dxl[kt]));
+                    } else if (kt == 2 && ks < 2) {
+                        const int pidx = kTHSlotP[slot];
+                        v = W * (sDx[le][ls][ks] * C[pidx]);
+                    } else {
+                        continue;
+                    }
+                    Jacc[ss] = Jacc[ss] + v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (myrow) {
+        a.Er[(int64_t)e * NS + slot] = Racc;
+        if (a.flag) {
+            T *dst = a.Ej + ((int64_t)e * NS + slot) * NS;
+#pragma unroll
+            for (int j = 0; j < NS; j++) dst[j] = Jacc[j];
+        }
+    }
+}
+
+}  // namespace pyhip

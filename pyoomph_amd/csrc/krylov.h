@@ -1,0 +1,71 @@
+// Batched Krylov solver workspace (GMRES + Jacobi / ILU(0)), see krylov.hip.
+#pragma once
+#include <vector>
+
+#include "problem.h"
+
+namespace pyhip {
+
+constexpr int kChunk = 1024;  // rows per reduction chunk (chunks never straddle sub-problems)
+
+struct IluPlan {
+    bool built = false;
+    int64_t n = 0, nnz = 0;
+    int32_t *perm = nullptr;      // [n] new -> old
+    int32_t *rs = nullptr;        // [n+1] permuted, column-sorted
+    int32_t *col = nullptr;       // [nnz]
+    int32_t *src = nullptr;       // [nnz] index of the entry in the problem's CSR values
+    int32_t *diag = nullptr;      // [n]
+    int32_t *lo_rows = nullptr;   // rows grouped by lower-solve / factorisation level
+    int32_t *up_rows = nullptr;   // rows grouped by upper-solve level
+    std::vector<int64_t> lo_ptr, up_ptr;   // level boundaries (host)
+    void *lu = nullptr;           // [nnz] factor values (T)
+    int lu_prec = -1;
+    double *tmp64 = nullptr;      // [n] permuted work vectors
+    double *tmp64b = nullptr;
+    float *tmp32 = nullptr;
+    float *tmp32b = nullptr;
+};
+
+struct KrylovWork {
+    int64_t n = 0;
+    int nsub = 0;
+    int m = 0;                    // restart length the buffers are sized for
+    int64_t nchunk = 0;
+    int32_t *chunk_sub = nullptr;     // [nchunk]
+    int64_t *chunk_begin = nullptr;   // [nchunk+1] (row ranges; chunk c = [begin[c], end[c]))
+    int64_t *chunk_end = nullptr;
+    int32_t *sub_chunk_ptr = nullptr; // [nsub+1]
+    double *partial = nullptr;        // [nchunk][m+2]
+    double *dots = nullptr;           // [m+2][nsub]
+    // GMRES state per sub-problem
+    double *H = nullptr;              // [nsub][m+1][m]
+    double *cs = nullptr, *sn = nullptr;  // [nsub][m]
+    double *g = nullptr;              // [nsub][m+1]
+    double *beta0 = nullptr;          // [nsub]
+    double *bnorm = nullptr;          // [nsub]
+    int32_t *kcols = nullptr;         // [nsub] columns in use
+    int32_t *active = nullptr;        // [nsub] 1 while iterating
+    int32_t *nactive_h = nullptr;     // pinned host counter
+    int32_t *nactive_d = nullptr;
+    double *y = nullptr;              // [nsub][m]
+    // vectors
+    void *V = nullptr;                // [(m+1)][n] basis (T)
+    int vprec = -1;
+    double *r64 = nullptr, *x64 = nullptr, *d64 = nullptr, *b64 = nullptr;  // [n]
+    float *r32 = nullptr, *d32 = nullptr, *w32 = nullptr, *t32 = nullptr;
+    double *w64 = nullptr, *t64 = nullptr;
+    float *vals32 = nullptr;          // FP32 copy of the Jacobian values (when needed)
+    double *vals64 = nullptr;         // FP64 Jacobian values (Newton)
+    double *res64 = nullptr;          // Newton residual
+    double *maxres = nullptr;         // [nsub]
+    double *maxres_h = nullptr;       // pinned [nsub]
+    double *sub_scalar = nullptr;     // [nsub] scratch
+    double *jdiag64 = nullptr;        // [n] inverse diagonal (Jacobi)
+    float *jdiag32 = nullptr;
+    int32_t *dof_mask = nullptr;      // [nsub] Newton: 1 if the sub-problem takes a step
+    IluPlan ilu;
+    int64_t bytes = 0;
+};
+
+}  // namespace pyhip

@@ -1,0 +1,992 @@
+// Batched preconditioned GMRES on the device.
+//
+// Every vector spans all sub-problems of a problem (block-diagonal system). Every scalar
+// (dot products, norms, Hessenberg entries, Givens rotations, convergence state) is per
+// sub-problem. Reductions are two-level and deterministic: fixed 1024-row chunks that
+// never straddle a sub-problem, a fixed-tree block reduction per chunk, then a
+// sequential sum of a sub-problem's chunk partials in chunk order. A sub-problem's
+// arithmetic therefore depends only on its own data: it gives the same bits alone and in
+// any batch.
+//
+// GMRES(m) with right preconditioning and classical Gram-Schmidt with one
+// re-orthogonalisation (CGS2); Hessenberg least squares by Givens rotations on the
+// device (one thread per sub-problem). The host reads one counter per iteration (the
+// number of sub-problems still iterating).
+//
+// Preconditioners: Jacobi, and ILU(0) on the structural pattern, level-scheduled
+// (rows of one level are independent; a level is one launch; levels of all
+// sub-problems are merged by level index). Taylor-Hood sub-problems are factorised with
+// the velocity unknowns ordered before the pressure unknowns (natural order within
+// each), which keeps ILU(0) pivots non-zero for the zero pressure-pressure block.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "krylov.h"
+
+namespace pyhip {
+
+// ------------------------------------------------------------------------------------
+// kernels
+
+template <typename TV, typename TX, typename TY, int LPR>
+__global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__restrict__ rs,
+                                                   const int32_t *__restrict__ col,
+                                                   const TV *__restrict__ val,
+                                                   const TX *__restrict__ x, TY *__restrict__ y,
+                                                   const TY *__restrict__ b) {
+    const int64_t row = (int64_t)blockIdx.x * (256 / LPR) + threadIdx.x / LPR;
+    const int lane = threadIdx.x % LPR;
+    if (row >= n) return;
+    TY acc = TY(0);
+    for (int32_t k = rs[row] + lane; k < rs[row + 1]; k += LPR) acc += TY(val[k]) * TY(x[col[k]]);
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+    if (lane == 0) y[row] = b ? b[row] - acc : acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void multidot_kernel(int nvec, const T *__restrict__ V,
+                                                      int64_t ldv, const T *__restrict__ w,
+                                                      const int64_t *__restrict__ cb,
+                                                      const int64_t *__restrict__ ce,
+                                                      double *__restrict__ partial, int pstride) {
+    __shared__ double red[256];
+    const int64_t c = blockIdx.x;
+    const int64_t b0 = cb[c], b1 = ce[c];
+    for (int j = 0; j < nvec; j++) {
+        const T *v = V + (int64_t)j * ldv;
+        double acc = 0.0;
+        for (int64_t i = b0 + threadIdx.x; i < b1; i += 256) acc += double(v[i]) * double(w[i]);
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[c * pstride + j] = red[0];
+        __syncthreads();
+    }
+}
+
+// max |v| per chunk (non-finite -> +inf), DoubleVector::max (double_vector.cc:611-630)
+__global__ __launch_bounds__(256) void maxabs_kernel(const double *__restrict__ v,
+                                                    const int64_t *__restrict__ cb,
+                                                    const int64_t *__restrict__ ce,
+                                                    double *__restrict__ partial, int pstride) {
+    __shared__ double red[256];
+    const int64_t c = blockIdx.x;
+    double m = 0.0;
+    for (int64_t i = cb[c] + threadIdx.x; i < ce[c]; i += 256) {
+        const double a = v[i];
+        if (!isfinite(a)) m = INFINITY;
+        else m = fmax(m, fabs(a));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[c * pstride] = red[0];
+}
+
+__global__ void reduce_partials_kernel(int nvec, int nsub, const int32_t *__restrict__ scp,
+                                       const double *__restrict__ partial, int pstride,
+                                       double *__restrict__ out, int is_max) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nvec * nsub) return;
+    const int j = t / nsub, s = t % nsub;
+    double acc = 0.0;
+    for (int c = scp[s]; c < scp[s + 1]; c++) {
+        const double v = partial[(int64_t)c * pstride + j];
+        acc = is_max ? fmax(acc, v) : acc + v;
+    }
+    out[j * nsub + s] = acc;
+}
+
+// w -= sum_j h[j][sub] V_j  (h = dots of this Gram-Schmidt pass)
+template <typename T>
+__global__ __launch_bounds__(256) void cgs_update_kernel(int64_t n, int nvec, int nsub,
+                                                         const T *__restrict__ V, int64_t ldv,
+                                                         const double *__restrict__ h,
+                                                         const int32_t *__restrict__ dsub,
+                                                         const int32_t *__restrict__ active,
+                                                         T *__restrict__ w) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int s = dsub[i];
+    if (!active[s]) return;
+    double acc = double(w[i]);
+    for (int j = 0; j < nvec; j++) acc -= h[j * nsub + s] * double(V[(int64_t)j * ldv + i]);
+    w[i] = T(acc);
+}
+
+// out = in * (1 / nrm[sub]) (0 where nrm == 0); only active sub-problems
+template <typename T>
+__global__ __launch_bounds__(256) void scale_kernel(int64_t n, const T *in,
+                                                    const double *__restrict__ nrm2,
+                                                    const int32_t *__restrict__ dsub,
+                                                    const int32_t *__restrict__ active,
+                                                    T *out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int s = dsub[i];
+    if (!active[s]) return;
+    const double nr = sqrt(nrm2[s]);
+    out[i] = nr > 0.0 ? T(double(in[i]) / nr) : T(0);
+}
+
+__global__ void sqrt_copy_kernel(int n, const double *__restrict__ a, double *__restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = sqrt(a[i]);
+}
+
+// Start of a cycle: beta = |r|, g = (beta, 0, ...), active if the sub-problem still has
+// work (need[s]) and beta > tol * |b|.
+__global__ void gmres_start_kernel(int nsub, int m, const double *__restrict__ nrm2,
+                                   const double *__restrict__ bnorm, double tol,
+                                   const int32_t *__restrict__ need, double *__restrict__ g,
+                                   int32_t *__restrict__ active, int32_t *__restrict__ kcols,
+                                   double *__restrict__ beta0, int32_t *__restrict__ nactive) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsub) return;
+    const double beta = sqrt(nrm2[s]);
+    kcols[s] = 0;
+    beta0[s] = beta;
+    for (int i = 0; i <= m; i++) g[s * (m + 1) + i] = 0.0;
+    g[s * (m + 1)] = beta;
+    const int a = need[s] && (beta > tol * bnorm[s]) && isfinite(beta);
+    active[s] = a;
+    if (a) atomicAdd(nactive, 1);
+}
+
+// Column k of the Hessenberg: h[0..k] = dots (CGS pass 1 + pass 2), h[k+1] = |w|.
+__global__ void gmres_givens_kernel(int nsub, int m, int k, const double *__restrict__ d1,
+                                    const double *__restrict__ d2, const double *__restrict__ wn2,
+                                    double *__restrict__ H, double *__restrict__ cs,
+                                    double *__restrict__ sn, double *__restrict__ g,
+                                    const double *__restrict__ bnorm, double tol,
+                                    int32_t *__restrict__ active, int32_t *__restrict__ kcols,
+                                    int32_t *__restrict__ nactive) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsub || !active[s]) return;
+    double *Hs = H + (int64_t)s * (m + 1) * m;
+    double *cs_s = cs + (int64_t)s * m, *sn_s = sn + (int64_t)s * m, *gs = g + (int64_t)s * (m + 1);
+    double h[65];
+    for (int i = 0; i <= k; i++) h[i] = d1[i * nsub + s] + d2[i * nsub + s];
+    const double hk1 = sqrt(wn2[s]);
+    h[k + 1] = hk1;
+    for (int i = 0; i < k; i++) {
+        const double t = cs_s[i] * h[i] + sn_s[i] * h[i + 1];
+        h[i + 1] = -sn_s[i] * h[i] + cs_s[i] * h[i + 1];
+        h[i] = t;
+    }
+    const double den = hypot(h[k], h[k + 1]);
+    double c = 1.0, sv = 0.0;
+    if (den > 0.0) {
+        c = h[k] / den;
+        sv = h[k + 1] / den;
+    }
+    cs_s[k] = c;
+    sn_s[k] = sv;
+    h[k] = den;
+    h[k + 1] = 0.0;
+    gs[k + 1] = -sv * gs[k];
+    gs[k] = c * gs[k];
+    for (int i = 0; i <= k; i++) Hs[i * m + k] = h[i];
+    kcols[s] = k + 1;
+    const bool done = !(fabs(gs[k + 1]) > tol * bnorm[s]) || hk1 == 0.0 || den == 0.0 ||
+                      !isfinite(gs[k + 1]) || (k + 1 == m);
+    if (done) active[s] = 0;
+    else atomicAdd(nactive, 1);
+}
+
+// y = R^{-1} g (upper triangular back substitution of the rotated Hessenberg)
+__global__ void gmres_backsolve_kernel(int nsub, int m, const double *__restrict__ H,
+                                       const double *__restrict__ g,
+                                       const int32_t *__restrict__ kcols,
+                                       double *__restrict__ y) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsub) return;
+    const double *Hs = H + (int64_t)s * (m + 1) * m;
+    const double *gs = g + (int64_t)s * (m + 1);
+    double *ys = y + (int64_t)s * m;
+    const int k = kcols[s];
+    for (int i = k - 1; i >= 0; i--) {
+        double acc = gs[i];
+        for (int j = i + 1; j < k; j++) acc -= Hs[i * m + j] * ys[j];
+        const double d = Hs[i * m + i];
+        ys[i] = d != 0.0 ? acc / d : 0.0;
+    }
+    for (int i = k; i < m; i++) ys[i] = 0.0;
+}
+
+// t = sum_{j < kcols[sub]} y_j V_j
+template <typename T>
+__global__ __launch_bounds__(256) void combine_kernel(int64_t n, int m, const T *__restrict__ V,
+                                                     int64_t ldv, const double *__restrict__ y,
+                                                     const int32_t *__restrict__ kcols,
+                                                     const int32_t *__restrict__ dsub,
+                                                     T *__restrict__ t) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int s = dsub[i];
+    const int k = kcols[s];
+    double acc = 0.0;
+    for (int j = 0; j < k; j++) acc += y[(int64_t)s * m + j] * double(V[(int64_t)j * ldv + i]);
+    t[i] = T(acc);
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void axpy_kernel(int64_t n, const TA *__restrict__ a,
+                                                   TB *__restrict__ b, double alpha,
+                                                   const int32_t *__restrict__ dsub,
+                                                   const int32_t *__restrict__ mask) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (mask && !mask[dsub[i]]) return;
+    b[i] = TB(double(b[i]) + alpha * double(a[i]));
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void convert_kernel(int64_t n, const TA *__restrict__ a,
+                                                      TB *__restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = TB(a[i]);
+}
+
+// v[i] = mask[sub] ? v : 0
+template <typename T>
+__global__ __launch_bounds__(256) void mask_kernel(int64_t n, T *__restrict__ v,
+                                                   const int32_t *__restrict__ dsub,
+                                                   const int32_t *__restrict__ mask) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n && !mask[dsub[i]]) v[i] = T(0);
+}
+
+template <typename TV, typename T>
+__global__ __launch_bounds__(256) void jacobi_setup_kernel(int64_t n, const TV *__restrict__ vals,
+                                                           const int32_t *__restrict__ diag,
+                                                           T *__restrict__ jd) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double d = diag[i] >= 0 ? double(vals[diag[i]]) : 0.0;
+    jd[i] = T(d != 0.0 ? 1.0 / d : 1.0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void jacobi_apply_kernel(int64_t n, const T *__restrict__ jd,
+                                                           const T *__restrict__ v,
+                                                           T *__restrict__ z) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) z[i] = jd[i] * v[i];
+}
+
+// ---- ILU(0) -------------------------------------------------------------------------
+
+template <typename TV, typename T>
+__global__ __launch_bounds__(256) void ilu_copy_kernel(int64_t nnz, const int32_t *__restrict__ src,
+                                                       const TV *__restrict__ vals,
+                                                       T *__restrict__ lu) {
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k < nnz) lu[k] = T(vals[src[k]]);
+}
+
+// One level of the row-wise (IKJ) ILU(0) factorisation; one wave per row.
+template <typename T>
+__global__ __launch_bounds__(256) void ilu_factor_level_kernel(int64_t nrows,
+                                                               const int32_t *__restrict__ rows,
+                                                               const int32_t *__restrict__ rs,
+                                                               const int32_t *__restrict__ col,
+                                                               const int32_t *__restrict__ diag,
+                                                               T *__restrict__ lu,
+                                                               int32_t *__restrict__ err) {
+    const int64_t w = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (w >= nrows) return;
+    const int32_t i = rows[w];
+    const int32_t r0 = rs[i], r1 = rs[i + 1], di = diag[i];
+    for (int32_t kk = r0; kk < di; kk++) {
+        const int32_t k = col[kk];
+        const T piv = lu[diag[k]];
+        const T l = lu[kk] / piv;  // every lane computes the same value
+        if (!isfinite(double(l))) {
+            if (lane == 0) err[0] = 1;
+        }
+        if (lane == 0) lu[kk] = l;
+        const int32_t k0 = diag[k] + 1, k1 = rs[k + 1];
+        for (int32_t jj = kk + 1 + lane; jj < r1; jj += 64) {
+            const int32_t c = col[jj];
+            int32_t lo = k0, hi = k1;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (col[mid] < c) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < k1 && col[lo] == c) lu[jj] = lu[jj] - l * lu[lo];
+        }
+        // the next pivot column was updated by other lanes of this wave
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    if (lane == 0 && !(double(lu[di]) != 0.0 && isfinite(double(lu[di])))) err[0] = 1;
+}
+
+// One level of L y = b (unit lower) or U z = y; 16 lanes per row.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void trsv_level_kernel(int64_t nrows, const int32_t *__restrict__ rows,
+                                                         const int32_t *__restrict__ rs,
+                                                         const int32_t *__restrict__ col,
+                                                         const int32_t *__restrict__ diag,
+                                                         const T *__restrict__ lu,
+                                                         const T *__restrict__ b,
+                                                         T *__restrict__ x) {
+    const int64_t w = (int64_t)blockIdx.x * 16 + threadIdx.x / 16;
+    const int lane = threadIdx.x % 16;
+    if (w >= nrows) return;
+    const int32_t i = rows[w];
+    const int32_t k0 = LOWER ? rs[i] : diag[i] + 1;
+    const int32_t k1 = LOWER ? diag[i] : rs[i + 1];
+    T acc = T(0);
+    for (int32_t k = k0 + lane; k < k1; k += 16) acc += lu[k] * x[col[k]];
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+    if (lane == 0) x[i] = LOWER ? b[i] - acc : (b[i] - acc) / lu[diag[i]];
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void permute_kernel(int64_t n, const int32_t *__restrict__ perm,
+                                                      const TA *__restrict__ a, TB *__restrict__ b,
+                                                      int forward) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (forward) b[i] = TB(a[perm[i]]);   // permuted[i] = original[perm[i]]
+    else b[perm[i]] = TB(a[i]);
+}
+
+// ------------------------------------------------------------------------------------
+// host side
+
+static inline unsigned nblk(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+
+#define KCHECK() PYHIP_CHECK_HIP(hipGetLastError())
+
+template <typename T>
+static int kalloc(T **p, size_t n, KrylovWork *W) {
+    return dev_alloc(p, n, W->bytes);
+}
+
+static int krylov_setup(pyhip_problem *p, int m) {
+    if (p->kw && p->kw->m >= m) return PYHIP_OK;
+    KrylovWork *W = p->kw;
+    if (!W) {
+        W = new KrylovWork();
+        p->kw = W;
+        W->n = p->ndof;
+        W->nsub = p->nsub;
+        // chunks
+        std::vector<int32_t> csub, scp(p->nsub + 1, 0);
+        std::vector<int64_t> cb, ce;
+        for (int s = 0; s < p->nsub; s++) {
+            scp[s] = (int32_t)cb.size();
+            for (int64_t b = p->dof_off_h[s]; b < p->dof_off_h[s + 1]; b += kChunk) {
+                cb.push_back(b);
+                ce.push_back(std::min<int64_t>(b + kChunk, p->dof_off_h[s + 1]));
+                csub.push_back(s);
+            }
+        }
+        scp[p->nsub] = (int32_t)cb.size();
+        W->nchunk = (int64_t)cb.size();
+        int rc;
+#define KUP(dst, v)                                                                        \
+    if ((rc = kalloc(&(dst), (v).size(), W))) return rc;                                   \
+    if (!(v).empty())                                                                      \
+        PYHIP_CHECK_HIP(hipMemcpy(dst, (v).data(), (v).size() * sizeof((v)[0]), hipMemcpyHostToDevice));
+        KUP(W->chunk_sub, csub);
+        KUP(W->chunk_begin, cb);
+        KUP(W->chunk_end, ce);
+        KUP(W->sub_chunk_ptr, scp);
+#undef KUP
+        const int64_t n = W->n, ns = W->nsub;
+        if ((rc = kalloc(&W->beta0, ns, W)) || (rc = kalloc(&W->bnorm, ns, W)) ||
+            (rc = kalloc(&W->kcols, ns, W)) || (rc = kalloc(&W->active, ns, W)) ||
+            (rc = kalloc(&W->nactive_d, 1, W)) || (rc = kalloc(&W->r64, n, W)) ||
+            (rc = kalloc(&W->x64, n, W)) || (rc = kalloc(&W->d64, n, W)) ||
+            (rc = kalloc(&W->b64, n, W)) || (rc = kalloc(&W->w64, n, W)) ||
+            (rc = kalloc(&W->t64, n, W)) || (rc = kalloc(&W->r32, n, W)) ||
+            (rc = kalloc(&W->d32, n, W)) || (rc = kalloc(&W->w32, n, W)) ||
+            (rc = kalloc(&W->t32, n, W)) || (rc = kalloc(&W->res64, n, W)) ||
+            (rc = kalloc(&W->maxres, ns, W)) || (rc = kalloc(&W->sub_scalar, ns, W)) ||
+            (rc = kalloc(&W->jdiag64, n, W)) || (rc = kalloc(&W->jdiag32, n, W)) ||
+            (rc = kalloc(&W->dof_mask, ns, W)))
+            return rc;
+        PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->nactive_h, sizeof(int32_t), hipHostMallocDefault));
+        PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->maxres_h, ns * sizeof(double), hipHostMallocDefault));
+    }
+    // restart-length dependent buffers
+    int rc;
+    dfree(W->partial);
+    dfree(W->dots);
+    dfree(W->H);
+    dfree(W->cs);
+    dfree(W->sn);
+    dfree(W->g);
+    dfree(W->y);
+    dfree(W->V);
+    W->V = nullptr;
+    W->vprec = -1;
+    const int64_t ns = W->nsub;
+    if ((rc = kalloc(&W->partial, (size_t)W->nchunk * (m + 2) + 1, W)) ||
+        (rc = kalloc(&W->dots, (size_t)3 * (m + 2) * ns, W)) ||
+        (rc = kalloc(&W->H, (size_t)ns * (m + 1) * m, W)) || (rc = kalloc(&W->cs, (size_t)ns * m, W)) ||
+        (rc = kalloc(&W->sn, (size_t)ns * m, W)) || (rc = kalloc(&W->g, (size_t)ns * (m + 1), W)) ||
+        (rc = kalloc(&W->y, (size_t)ns * m, W)))
+        return rc;
+    W->m = m;
+    return PYHIP_OK;
+}
+
+// Per-sub-problem dot products of nvec vectors with w: out[j][sub].
+template <typename T>
+static int batched_dots(pyhip_problem *p, KrylovWork *W, int nvec, const T *V, int64_t ldv,
+                        const T *w, double *out) {
+    const int pstride = W->m + 2;
+    if (W->nchunk == 0) return PYHIP_OK;
+    hipLaunchKernelGGL((multidot_kernel<T>), dim3((unsigned)W->nchunk), dim3(256), 0, p->st, nvec,
+                       V, ldv, w, W->chunk_begin, W->chunk_end, W->partial, pstride);
+    KCHECK();
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblk((int64_t)nvec * W->nsub, 64)), dim3(64), 0,
+                       p->st, nvec, W->nsub, W->sub_chunk_ptr, W->partial, pstride, out, 0);
+    KCHECK();
+    return PYHIP_OK;
+}
+
+int batched_maxabs(pyhip_problem *p, const double *v, double *out) {
+    KrylovWork *W = p->kw;
+    const int pstride = W->m + 2;
+    if (W->nchunk == 0) return PYHIP_OK;
+    hipLaunchKernelGGL(maxabs_kernel, dim3((unsigned)W->nchunk), dim3(256), 0, p->st, v,
+                       W->chunk_begin, W->chunk_end, W->partial, pstride);
+    KCHECK();
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblk(W->nsub, 64)), dim3(64), 0, p->st, 1,
+                       W->nsub, W->sub_chunk_ptr, W->partial, pstride, out, 1);
+    KCHECK();
+    return PYHIP_OK;
+}
+
+template <typename TV, typename TX, typename TY>
+static int spmv(pyhip_problem *p, const TV *vals, const TX *x, TY *y, const TY *b) {
+    const int64_t n = p->ndof;
+    if (n == 0) return PYHIP_OK;
+    const double avg = p->nnz / (double)n;
+    if (avg > 48) {
+        hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 32>), dim3(nblk(n, 256 / 32)), dim3(256), 0, p->st,
+                           n, p->row_start, p->col, vals, x, y, b);
+    } else if (avg > 12) {
+        hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 16>), dim3(nblk(n, 256 / 16)), dim3(256), 0, p->st,
+                           n, p->row_start, p->col, vals, x, y, b);
+    } else {
+        hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 8>), dim3(nblk(n, 256 / 8)), dim3(256), 0, p->st,
+                           n, p->row_start, p->col, vals, x, y, b);
+    }
+    KCHECK();
+    return PYHIP_OK;
+}
+
+// ---- ILU(0) plan: permutation, permuted pattern, level sets (host, once per pattern)
+static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
+    IluPlan &L = W->ilu;
+    if (L.built) return PYHIP_OK;
+    const int64_t n = p->ndof;
+    // permutation: Taylor-Hood sub-problems put pressure unknowns last
+    std::vector<int32_t> perm(n), iperm(n);
+    {
+        std::vector<char> is_p(n, 0);
+        if ((int64_t)p->ilu_last_h.size() == n) is_p = p->ilu_last_h;
+        // pressure rows are rows whose diagonal structural entry receives no element
+        // contribution from a velocity slot; identified from the slot layout: eqn of
+        // pressure slots (2, 7, 16, 21) of Taylor-Hood elements.
+        for (auto &G : p->groups) {
+            if (G.form != PYHIP_FORM_NAVIER_STOKES_TH) continue;
+            std::vector<int32_t> seqn((size_t)G.ns * G.nel);
+            PYHIP_CHECK_HIP(hipMemcpy(seqn.data(), G.seqn, seqn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+            const int ps[4] = {2, 7, 16, 21};
+            for (int k = 0; k < 4; k++)
+                for (int64_t e = 0; e < G.nel; e++) {
+                    const int32_t q = seqn[(size_t)ps[k] * G.nel + e];
+                    if (q >= 0) is_p[q] = 1;
+                }
+        }
+        int64_t w = 0;
+        for (int s = 0; s < p->nsub; s++) {
+            for (int64_t i = p->dof_off_h[s]; i < p->dof_off_h[s + 1]; i++)
+                if (!is_p[i]) perm[w++] = (int32_t)i;
+            for (int64_t i = p->dof_off_h[s]; i < p->dof_off_h[s + 1]; i++)
+                if (is_p[i]) perm[w++] = (int32_t)i;
+        }
+        for (int64_t i = 0; i < n; i++) iperm[perm[i]] = (int32_t)i;
+    }
+    const auto &rs = p->row_start_h;
+    const auto &col = p->col_h;
+    std::vector<int32_t> prs(n + 1, 0), pcol(p->nnz), psrc(p->nnz), pdiag(n, -1);
+    {
+        std::vector<std::pair<int32_t, int32_t>> tmp;
+        for (int64_t i = 0; i < n; i++) {
+            const int32_t o = perm[i];
+            tmp.clear();
+            for (int32_t k = rs[o]; k < rs[o + 1]; k++) tmp.push_back({iperm[col[k]], k});
+            std::sort(tmp.begin(), tmp.end());
+            prs[i + 1] = prs[i] + (int32_t)tmp.size();
+            for (size_t t = 0; t < tmp.size(); t++) {
+                pcol[prs[i] + t] = tmp[t].first;
+                psrc[prs[i] + t] = tmp[t].second;
+                if (tmp[t].first == i) pdiag[i] = prs[i] + (int32_t)t;
+            }
+            if (pdiag[i] < 0) {
+                set_error("ILU(0): a row has no structural diagonal entry");
+                return PYHIP_ERR_BREAKDOWN;
+            }
+        }
+    }
+    // level sets (lower: depends on smaller columns; upper: on larger columns)
+    std::vector<int32_t> lev(n, 0);
+    int32_t nlo = 0;
+    for (int64_t i = 0; i < n; i++) {
+        int32_t l = 0;
+        for (int32_t k = prs[i]; k < pdiag[i]; k++) l = std::max(l, lev[pcol[k]] + 1);
+        lev[i] = l;
+        nlo = std::max(nlo, l + 1);
+    }
+    auto bucket = [&](int32_t nl, std::vector<int32_t> &rows, std::vector<int64_t> &ptr) {
+        ptr.assign(nl + 1, 0);
+        for (int64_t i = 0; i < n; i++) ptr[lev[i] + 1]++;
+        for (int32_t l = 0; l < nl; l++) ptr[l + 1] += ptr[l];
+        rows.resize(n);
+        std::vector<int64_t> f(ptr.begin(), ptr.end() - 1);
+        for (int64_t i = 0; i < n; i++) rows[f[lev[i]]++] = (int32_t)i;
+    };
+    std::vector<int32_t> lo_rows, up_rows;
+    bucket(nlo, lo_rows, L.lo_ptr);
+    int32_t nup = 0;
+    for (int64_t i = n - 1; i >= 0; i--) {
+        int32_t l = 0;
+        for (int32_t k = pdiag[i] + 1; k < prs[i + 1]; k++) l = std::max(l, lev[pcol[k]] + 1);
+        lev[i] = l;
+        nup = std::max(nup, l + 1);
+    }
+    // upper levels were computed in decreasing row order; lev[] now holds them
+    bucket(nup, up_rows, L.up_ptr);
+    int rc;
+#define LUP(dst, v)                                                                        \
+    if ((rc = kalloc(&(dst), (v).size(), W))) return rc;                                   \
+    PYHIP_CHECK_HIP(hipMemcpy(dst, (v).data(), (v).size() * sizeof((v)[0]), hipMemcpyHostToDevice));
+    LUP(L.perm, perm);
+    LUP(L.rs, prs);
+    LUP(L.col, pcol);
+    LUP(L.src, psrc);
+    LUP(L.diag, pdiag);
+    LUP(L.lo_rows, lo_rows);
+    LUP(L.up_rows, up_rows);
+#undef LUP
+    if ((rc = kalloc(&L.tmp64, n, W)) || (rc = kalloc(&L.tmp64b, n, W)) ||
+        (rc = kalloc(&L.tmp32, n, W)) || (rc = kalloc(&L.tmp32b, n, W)))
+        return rc;
+    L.n = n;
+    L.nnz = p->nnz;
+    L.built = true;
+    return PYHIP_OK;
+}
+
+template <typename TV, typename T>
+static int ilu_factor(pyhip_problem *p, KrylovWork *W, const TV *vals) {
+    IluPlan &L = W->ilu;
+    int rc;
+    const int prec = std::is_same<T, float>::value ? PYHIP_F32 : PYHIP_F64;
+    if (L.lu_prec != prec) {
+        dfree(L.lu);
+        L.lu = nullptr;
+        if ((rc = dev_alloc((T **)&L.lu, L.nnz, W->bytes))) return rc;
+        L.lu_prec = prec;
+    }
+    T *lu = (T *)L.lu;
+    hipLaunchKernelGGL((ilu_copy_kernel<TV, T>), dim3(nblk(L.nnz)), dim3(256), 0, p->st, L.nnz, L.src,
+                       vals, lu);
+    KCHECK();
+    PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
+    for (size_t l = 0; l + 1 < L.lo_ptr.size(); l++) {
+        const int64_t nr = L.lo_ptr[l + 1] - L.lo_ptr[l];
+        hipLaunchKernelGGL((ilu_factor_level_kernel<T>), dim3(nblk(nr, 4)), dim3(256), 0, p->st, nr,
+                           L.lo_rows + L.lo_ptr[l], L.rs, L.col, L.diag, lu, W->nactive_d);
+    }
+    KCHECK();
+    PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    if (*W->nactive_h) {
+        set_error("ILU(0) breakdown: zero or non-finite pivot");
+        return PYHIP_ERR_BREAKDOWN;
+    }
+    return PYHIP_OK;
+}
+
+template <typename T>
+static int ilu_apply(pyhip_problem *p, KrylovWork *W, const T *v, T *z) {
+    IluPlan &L = W->ilu;
+    const T *lu = (const T *)L.lu;
+    T *a = std::is_same<T, float>::value ? (T *)L.tmp32 : (T *)L.tmp64;
+    T *b = std::is_same<T, float>::value ? (T *)L.tmp32b : (T *)L.tmp64b;
+    const int64_t n = L.n;
+    hipLaunchKernelGGL((permute_kernel<T, T>), dim3(nblk(n)), dim3(256), 0, p->st, n, L.perm, v, a, 1);
+    for (size_t l = 0; l + 1 < L.lo_ptr.size(); l++) {
+        const int64_t nr = L.lo_ptr[l + 1] - L.lo_ptr[l];
+        hipLaunchKernelGGL((trsv_level_kernel<T, true>), dim3(nblk(nr, 16)), dim3(256), 0, p->st, nr,
+                           L.lo_rows + L.lo_ptr[l], L.rs, L.col, L.diag, lu, a, b);
+    }
+    for (size_t l = 0; l + 1 < L.up_ptr.size(); l++) {
+        const int64_t nr = L.up_ptr[l + 1] - L.up_ptr[l];
+        hipLaunchKernelGGL((trsv_level_kernel<T, false>), dim3(nblk(nr, 16)), dim3(256), 0, p->st, nr,
+                           L.up_rows + L.up_ptr[l], L.rs, L.col, L.diag, lu, b, a);
+    }
+    hipLaunchKernelGGL((permute_kernel<T, T>), dim3(nblk(n)), dim3(256), 0, p->st, n, L.perm, a, z, 0);
+    KCHECK();
+    return PYHIP_OK;
+}
+
+template <typename T>
+static int precond_apply(pyhip_problem *p, KrylovWork *W, int pc, const T *v, T *z) {
+    if (pc == PYHIP_PRECOND_ILU0) return ilu_apply<T>(p, W, v, z);
+    const T *jd = std::is_same<T, float>::value ? (const T *)W->jdiag32 : (const T *)W->jdiag64;
+    hipLaunchKernelGGL((jacobi_apply_kernel<T>), dim3(nblk(p->ndof)), dim3(256), 0, p->st, p->ndof, jd, v, z);
+    KCHECK();
+    return PYHIP_OK;
+}
+
+template <typename TV, typename T>
+static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals) {
+    if (pc == PYHIP_PRECOND_ILU0) {
+        int rc = ilu_plan_build(p, W);
+        if (rc) return rc;
+        return ilu_factor<TV, T>(p, W, vals);
+    }
+    T *jd = std::is_same<T, float>::value ? (T *)W->jdiag32 : (T *)W->jdiag64;
+    hipLaunchKernelGGL((jacobi_setup_kernel<TV, T>), dim3(nblk(p->ndof)), dim3(256), 0, p->st, p->ndof,
+                       vals, p->diag, jd);
+    KCHECK();
+    return PYHIP_OK;
+}
+
+// GMRES in precision T on matrix values TV: x = A^{-1} b per sub-problem with need[s].
+// b, x: device vectors of type T. Returns the iterations of the longest-running one.
+template <typename T, typename TV>
+static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T *x,
+                 const pyhip_linear_opts &o, const int32_t *need, int *iters) {
+    const int64_t n = p->ndof;
+    const int m = W->m, ns = W->nsub;
+    const int prec = std::is_same<T, float>::value ? PYHIP_F32 : PYHIP_F64;
+    int rc;
+    if (W->vprec != prec) {
+        dfree(W->V);
+        W->V = nullptr;
+        if ((rc = dev_alloc((T **)&W->V, (size_t)(m + 1) * n, W->bytes))) return rc;
+        W->vprec = prec;
+    }
+    T *V = (T *)W->V;
+    T *w = std::is_same<T, float>::value ? (T *)W->w32 : (T *)W->w64;
+    T *t = std::is_same<T, float>::value ? (T *)W->t32 : (T *)W->t64;
+    double *d1 = W->dots, *d2 = W->dots + (m + 2) * ns, *d3 = W->dots + 2 * (m + 2) * ns;
+    // |b| per sub-problem
+    if ((rc = batched_dots<T>(p, W, 1, b, n, b, d3))) return rc;
+    hipLaunchKernelGGL(sqrt_copy_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d3, W->bnorm);
+    KCHECK();
+    // x0 = 0
+    PYHIP_CHECK_HIP(hipMemsetAsync(x, 0, n * sizeof(T), p->st));
+    int total = 0;
+    const int max_cycles = std::max(1, (o.max_iters + m - 1) / m);
+    for (int cyc = 0; cyc < max_cycles; cyc++) {
+        // r = b - A x  -> V_0 (unnormalised)
+        if ((rc = spmv<TV, T, T>(p, vals, x, V, b))) return rc;
+        if ((rc = batched_dots<T>(p, W, 1, V, n, V, d3))) return rc;
+        PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
+        hipLaunchKernelGGL(gmres_start_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, m, d3,
+                           W->bnorm, o.rtol, need, W->g, W->active, W->kcols, W->beta0, W->nactive_d);
+        KCHECK();
+        PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+        PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+        if (*W->nactive_h == 0) break;
+        hipLaunchKernelGGL((scale_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, V, d3, p->dof_sub,
+                           W->active, V);
+        KCHECK();
+        int k = 0;
+        for (; k < m; k++) {
+            T *vk = V + (int64_t)k * n;
+            if ((rc = precond_apply<T>(p, W, o.precond, vk, t))) return rc;
+            if ((rc = spmv<TV, T, T>(p, vals, t, w, (const T *)nullptr))) return rc;
+            // CGS2
+            if ((rc = batched_dots<T>(p, W, k + 1, V, n, w, d1))) return rc;
+            hipLaunchKernelGGL((cgs_update_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, k + 1, ns, V, n,
+                               d1, p->dof_sub, W->active, w);
+            if ((rc = batched_dots<T>(p, W, k + 1, V, n, w, d2))) return rc;
+            hipLaunchKernelGGL((cgs_update_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, k + 1, ns, V, n,
+                               d2, p->dof_sub, W->active, w);
+            if ((rc = batched_dots<T>(p, W, 1, w, n, w, d3))) return rc;
+            hipLaunchKernelGGL((scale_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, w, d3, p->dof_sub,
+                               W->active, V + (int64_t)(k + 1) * n);
+            PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
+            hipLaunchKernelGGL(gmres_givens_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, m, k, d1, d2,
+                               d3, W->H, W->cs, W->sn, W->g, W->bnorm, o.rtol, W->active, W->kcols,
+                               W->nactive_d);
+            KCHECK();
+            PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+            PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+            total++;
+            if (*W->nactive_h == 0) break;
+        }
+        // x += M^{-1} V y
+        hipLaunchKernelGGL(gmres_backsolve_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, m, W->H, W->g,
+                           W->kcols, W->y);
+        hipLaunchKernelGGL((combine_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, m, V, n, W->y,
+                           W->kcols, p->dof_sub, w);
+        KCHECK();
+        if ((rc = precond_apply<T>(p, W, o.precond, w, t))) return rc;
+        hipLaunchKernelGGL((axpy_kernel<T, T>), dim3(nblk(n)), dim3(256), 0, p->st, n, t, x, 1.0,
+                           p->dof_sub, need);
+        KCHECK();
+        if (total >= o.max_iters) break;
+    }
+    *iters = total;
+    return PYHIP_OK;
+}
+
+}  // namespace pyhip
+
+// ------------------------------------------------------------------------------------
+// Newton
+
+namespace pyhip {
+
+__global__ void newton_state_kernel(int nsub, const int32_t *__restrict__ running,
+                                    int32_t *__restrict__ mask) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsub) mask[s] = running[s];
+}
+
+// Solve J d = r for the masked sub-problems. r: FP64 right-hand side; d: FP64 result.
+static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts &o, int jac_prec,
+                        const double *r, double *d, const int32_t *mask, int *iters,
+                        int *refinements) {
+    int rc;
+    const int64_t n = p->ndof;
+    *iters = 0;
+    *refinements = 0;
+    if (o.precision == PYHIP_F64) {
+        PYHIP_REQUIRE(jac_prec == PYHIP_F64, "an FP64 Krylov solve needs the FP64 Jacobian");
+        if ((rc = precond_setup<double, double>(p, W, o.precond, W->vals64))) return rc;
+        return gmres<double, double>(p, W, W->vals64, r, d, o, mask, iters);
+    }
+    // FP32 inner solves inside FP64 iterative refinement
+    if (jac_prec == PYHIP_F64) {
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(nblk(p->nnz)), dim3(256), 0, p->st,
+                           p->nnz, W->vals64, W->vals32);
+        KCHECK();
+    }
+    if ((rc = precond_setup<float, float>(p, W, o.precond, W->vals32))) return rc;
+    PYHIP_CHECK_HIP(hipMemsetAsync(d, 0, n * sizeof(double), p->st));
+    hipLaunchKernelGGL((convert_kernel<double, double>), dim3(nblk(n)), dim3(256), 0, p->st, n, r, W->r64);
+    KCHECK();
+    const int ns = W->nsub;
+    double *d3 = W->dots + 2 * (W->m + 2) * ns;
+    // |r| of the original right-hand side per sub-problem
+    if ((rc = batched_dots<double>(p, W, 1, r, n, r, W->dots))) return rc;
+    std::vector<double> rn0(ns), rn(ns);
+    PYHIP_CHECK_HIP(hipMemcpyAsync(rn0.data(), W->dots, ns * sizeof(double), hipMemcpyDeviceToHost, p->st));
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    std::vector<int32_t> need_h(ns);
+    PYHIP_CHECK_HIP(hipMemcpy(need_h.data(), mask, ns * sizeof(int32_t), hipMemcpyDeviceToHost));
+    int32_t *need_d = nullptr;
+    if ((rc = dev_alloc(&need_d, ns, W->bytes))) return rc;
+    PYHIP_CHECK_HIP(hipMemcpy(need_d, need_h.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice));
+    for (int it = 0; it < std::max(1, o.max_refinements); it++) {
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(nblk(n)), dim3(256), 0, p->st, n, W->r64, W->r32);
+        KCHECK();
+        int its = 0;
+        if ((rc = gmres<float, float>(p, W, W->vals32, W->r32, W->d32, o, need_d, &its))) {
+            dfree(need_d);
+            return rc;
+        }
+        *iters += its;
+        (*refinements)++;
+        hipLaunchKernelGGL((axpy_kernel<float, double>), dim3(nblk(n)), dim3(256), 0, p->st, n, W->d32, d, 1.0,
+                           p->dof_sub, need_d);
+        // r64 = r - J d  (FP64 arithmetic on the stored Jacobian)
+        if (jac_prec == PYHIP_F64) rc = spmv<double, double, double>(p, W->vals64, d, W->r64, r);
+        else rc = spmv<float, double, double>(p, W->vals32, d, W->r64, r);
+        if (rc) {
+            dfree(need_d);
+            return rc;
+        }
+        if ((rc = batched_dots<double>(p, W, 1, W->r64, n, W->r64, d3))) {
+            dfree(need_d);
+            return rc;
+        }
+        PYHIP_CHECK_HIP(hipMemcpyAsync(rn.data(), d3, ns * sizeof(double), hipMemcpyDeviceToHost, p->st));
+        PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+        int left = 0;
+        for (int s = 0; s < ns; s++) {
+            if (!need_h[s]) continue;
+            if (!(std::sqrt(rn[s]) > o.refine_rtol * std::sqrt(rn0[s]))) need_h[s] = 0;
+            left += need_h[s];
+        }
+        if (!left) break;
+        PYHIP_CHECK_HIP(hipMemcpy(need_d, need_h.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    dfree(need_d);
+    return PYHIP_OK;
+}
+
+int plugin_setup(pyhip_problem *p, const pyhip_linear_opts &o) {
+    int rc = krylov_setup(p, o.restart);
+    if (rc) return rc;
+    KrylovWork *W = p->kw;
+    if (!W->vals64 && (rc = dev_alloc(&W->vals64, p->nnz, W->bytes))) return rc;
+    return PYHIP_OK;
+}
+double *plugin_values(pyhip_problem *p) { return p->kw->vals64; }
+int plugin_factor(pyhip_problem *p, const pyhip_linear_opts &o) {
+    return precond_setup<double, double>(p, p->kw, o.precond, p->kw->vals64);
+}
+int plugin_solve(pyhip_problem *p, const pyhip_linear_opts &o, const double *b, double *x, int *iters) {
+    KrylovWork *W = p->kw;
+    int32_t one = 1;
+    PYHIP_CHECK_HIP(hipMemcpy(W->dof_mask, &one, sizeof(int32_t), hipMemcpyHostToDevice));
+    return gmres<double, double>(p, W, W->vals64, b, x, o, W->dof_mask, iters);
+}
+
+}  // namespace pyhip
+
+using namespace pyhip;
+
+extern "C" {
+
+void pyhip_linear_opts_default(pyhip_linear_opts *o) {
+    o->precond = PYHIP_PRECOND_ILU0;
+    o->precision = PYHIP_F64;
+    o->restart = 50;
+    o->max_iters = 2000;
+    o->rtol = 1e-10;
+    o->max_refinements = 8;
+    o->refine_rtol = 1e-10;
+}
+
+void pyhip_newton_opts_default(pyhip_newton_opts *o) {
+    o->tol = 1e-8;
+    o->max_iter = 10;
+    o->max_residuals = 1e10;
+    o->always_take_one_newton_step = 1;
+    o->relaxation = 1.0;
+    o->jacobian_precision = PYHIP_F64;
+    pyhip_linear_opts_default(&o->linear);
+}
+
+void pyhip_krylov_free(pyhip_problem *p) {
+    KrylovWork *W = p->kw;
+    if (!W) return;
+    void *ptrs[] = {W->chunk_sub, W->chunk_begin, W->chunk_end, W->sub_chunk_ptr, W->partial, W->dots,
+                    W->H, W->cs, W->sn, W->g, W->beta0, W->bnorm, W->kcols, W->active, W->nactive_d,
+                    W->y, W->V, W->r64, W->x64, W->d64, W->b64, W->r32, W->d32, W->w32, W->t32,
+                    W->w64, W->t64, W->vals32, W->vals64, W->res64, W->maxres, W->sub_scalar,
+                    W->jdiag64, W->jdiag32, W->dof_mask, W->ilu.perm, W->ilu.rs, W->ilu.col,
+                    W->ilu.src, W->ilu.diag, W->ilu.lo_rows, W->ilu.up_rows, W->ilu.lu,
+                    W->ilu.tmp64, W->ilu.tmp64b, W->ilu.tmp32, W->ilu.tmp32b};
+    for (void *q : ptrs)
+        dfree(q);
+    if (W->nactive_h) (void)hipHostFree(W->nactive_h);
+    if (W->maxres_h) (void)hipHostFree(W->maxres_h);
+    delete W;
+    p->kw = nullptr;
+}
+
+int pyhip_newton_solve(pyhip_problem *p, double *x, const pyhip_newton_opts *o,
+                       pyhip_newton_stats *stats) {
+    PYHIP_REQUIRE(p && x && o && stats, "null argument");
+    PYHIP_REQUIRE(o->linear.restart >= 1 && o->linear.restart <= 64, "restart must be in 1..64");
+    PYHIP_REQUIRE(o->jacobian_precision == PYHIP_F64 || o->jacobian_precision == PYHIP_F32,
+                  "bad Jacobian precision");
+    int rc;
+    if ((rc = krylov_setup(p, o->linear.restart))) return rc;
+    KrylovWork *W = p->kw;
+    const int ns = p->nsub;
+    const int64_t n = p->ndof;
+    if (o->jacobian_precision == PYHIP_F64 && !W->vals64)
+        if ((rc = dev_alloc(&W->vals64, p->nnz, W->bytes))) return rc;
+    if (o->linear.precision == PYHIP_F32 && !W->vals32)
+        if ((rc = dev_alloc(&W->vals32, p->nnz, W->bytes))) return rc;
+    memset(stats, 0, sizeof(pyhip_newton_stats) * ns);
+    std::vector<int32_t> running(ns, 1);
+    std::vector<int> count(ns, 0);
+    auto record = [&](int s, double v) {
+        auto &S = stats[s];
+        if (S.nhistory < 16) S.history[S.nhistory++] = v;
+        S.final_maxres = v;
+    };
+    auto residual_norms = [&]() -> int {
+        int r2;
+        if ((r2 = pyhip_assemble(p, x, 0, -1, PYHIP_F64, W->res64, nullptr))) return r2;
+        if ((r2 = batched_maxabs(p, W->res64, W->maxres))) return r2;
+        PYHIP_CHECK_HIP(hipMemcpyAsync(W->maxres_h, W->maxres, ns * sizeof(double), hipMemcpyDeviceToHost, p->st));
+        PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+        return PYHIP_OK;
+    };
+    // first residual (count == 1)
+    if ((rc = residual_norms())) return rc;
+    for (int s = 0; s < ns; s++) {
+        record(s, W->maxres_h[s]);
+        if (p->dof_off_h[s + 1] == p->dof_off_h[s]) running[s] = 0, stats[s].converged = 1;
+        else if (W->maxres_h[s] < o->tol && !o->always_take_one_newton_step)
+            running[s] = 0, stats[s].converged = 1;
+    }
+    int any_failed = 0;
+    while (true) {
+        int nrun = 0;
+        for (int s = 0; s < ns; s++) nrun += running[s];
+        if (!nrun) break;
+        for (int s = 0; s < ns; s++) count[s] += running[s];
+        PYHIP_CHECK_HIP(hipMemcpyAsync(W->dof_mask, running.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, p->st));
+        // Jacobian at x (its residual is the one computed last, bit-identical)
+        if (o->jacobian_precision == PYHIP_F64) {
+            if ((rc = pyhip_assemble(p, x, 1, -1, PYHIP_F64, W->w64, W->vals64))) return rc;
+        } else {
+            if ((rc = pyhip_assemble(p, x, 1, -1, PYHIP_F32, W->w64, W->vals32))) return rc;
+        }
+        int its = 0, refs = 0;
+        if ((rc = linear_solve(p, W, o->linear, o->jacobian_precision, W->res64, W->d64, W->dof_mask, &its, &refs)))
+            return rc;
+        for (int s = 0; s < ns; s++)
+            if (running[s]) stats[s].linear_iters += its;
+        hipLaunchKernelGGL((axpy_kernel<double, double>), dim3(nblk(n)), dim3(256), 0, p->st, n, W->d64, x,
+                           -o->relaxation, p->dof_sub, W->dof_mask);
+        KCHECK();
+        if ((rc = residual_norms())) return rc;
+        for (int s = 0; s < ns; s++) {
+            if (!running[s]) continue;
+            const double mr = W->maxres_h[s];
+            record(s, mr);
+            stats[s].newton_iters = count[s];
+            if (mr < o->tol) {
+                running[s] = 0;
+                stats[s].converged = 1;
+            } else if (mr > o->max_residuals || count[s] >= o->max_iter) {
+                running[s] = 0;
+                stats[s].converged = 0;
+                any_failed = 1;
+            }
+        }
+    }
+    if (any_failed) {
+        set_error("NewtonSolverError: a sub-problem did not converge (see stats)");
+        return PYHIP_ERR_NEWTON;
+    }
+    return PYHIP_OK;
+}
+
+}  // extern "C"

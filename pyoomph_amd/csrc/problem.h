@@ -1,0 +1,102 @@
+// Internal definition of pyhip_problem (device-resident batched problem).
+#pragma once
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "element_kernels.h"
+#include "pattern.h"
+
+namespace pyhip {
+
+constexpr int kParamStride = 64;  // doubles of parameters per sub-problem
+
+struct SubInfo {
+    int form, kind, nvalue;
+    int64_t dof_off, ndof, node_off, nnode;
+    int group;               // element group holding its elements
+    int64_t el_off;          // first element inside that group
+    int64_t nel;
+};
+
+struct Group {
+    int kind, form, ns, nn, dim;
+    int64_t nel = 0;
+    int64_t er_off = 0, ej_off = 0;   // offsets into the element vector / matrix buffers
+    int32_t *conn = nullptr;          // [nn][nel]
+    int32_t *seqn = nullptr;          // [ns][nel]
+    int32_t *sub = nullptr;           // [nel]
+};
+
+struct KrylovWork;   // krylov.hip
+
+int launch_elements_f64(int kind, int form, const ElemArgs<double> &a, hipStream_t s);
+int launch_elements_f32(int kind, int form, const ElemArgs<float> &a, hipStream_t s);
+int launch_gather_values_f64(int64_t nrow, const int32_t *rs, const int64_t *occ_ptr,
+                             const int64_t *occ_eoff, const uint8_t *occ_pos, const double *Ej,
+                             double *vals, hipStream_t st);
+int launch_gather_values_f32(int64_t nrow, const int32_t *rs, const int64_t *occ_ptr,
+                             const int64_t *occ_eoff, const uint8_t *occ_pos, const float *Ej,
+                             float *vals, hipStream_t st);
+int launch_gather_residual_f64(int64_t nrow, const int64_t *occ_ptr, const int64_t *occ_roff,
+                               const double *Er, double *res, hipStream_t st);
+int launch_gather_residual_f32(int64_t nrow, const int64_t *occ_ptr, const int64_t *occ_roff,
+                               const float *Er, double *res, hipStream_t st);
+int export_reference_csr(int64_t nrow, const int32_t *rs, const int32_t *col,
+                         const int64_t *occ_ptr, const int64_t *occ_eoff,
+                         const uint8_t *occ_pos, const double *Ej, const double *vals,
+                         int32_t *keys, int32_t *kept, int32_t *out_rs, int32_t *out_col,
+                         double *out_val, int64_t *nnz_out, hipStream_t st);
+
+// Device allocation tracked against the problem.
+template <typename T>
+int dev_alloc(T **p, size_t n, int64_t &counter) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        *p = nullptr;
+        return PYHIP_ERR_NOMEM;
+    }
+    counter += (int64_t)(n * sizeof(T));
+    return PYHIP_OK;
+}
+
+}  // namespace pyhip
+
+struct pyhip_problem {
+    hipStream_t st = nullptr;
+    int nsub = 0;
+    std::vector<pyhip::SubInfo> subs;
+    std::vector<pyhip::Group> groups;
+    std::vector<int64_t> dof_off_h;          // [nsub+1]
+    int64_t ndof = 0, nnode = 0, nnz = 0, nocc = 0, ncontrib = 0, n_er = 0, n_ej = 0;
+    int64_t device_bytes = 0;
+    double *coords = nullptr;   // [nnode][3]
+    double *pinned = nullptr;   // [nnode][3]
+    double *aux = nullptr;      // [nnode]
+    double *params = nullptr;   // [nsub][kParamStride]
+    std::vector<double> params_h;
+    pyhip::DevTables<double> *tab64[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    pyhip::DevTables<float> *tab32[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // pattern / map
+    int32_t *row_start = nullptr, *col = nullptr, *diag = nullptr;
+    int64_t *occ_ptr = nullptr, *occ_roff = nullptr, *occ_eoff = nullptr;
+    uint8_t *occ_pos = nullptr;
+    int32_t *dof_sub = nullptr;             // [ndof] sub-problem of each row
+    int64_t *dof_off = nullptr;             // [nsub+1]
+    std::vector<int32_t> row_start_h, col_h;
+    // element buffers (lazy)
+    double *Er64 = nullptr, *Ej64 = nullptr;
+    float *Er32 = nullptr, *Ej32 = nullptr;
+    bool have_ej64 = false;                  // Ej64 holds the last FP64 Jacobian
+    // export scratch (lazy)
+    int32_t *keys = nullptr, *kept = nullptr;
+    // Newton / Krylov workspace (lazy)
+    pyhip::KrylovWork *kw = nullptr;
+    // ILU(0) ordering: rows listed here go after all other rows of their sub-problem
+    // (set for matrices that arrive without element information)
+    std::vector<char> ilu_last_h;
+};

@@ -1,0 +1,275 @@
+"""Host-side mirror of pyoomph's ``Problem`` for the assembly-and-solve path.
+
+A ``Problem`` holds one or more sub-problems (``MeshProblem``: one mesh, one set of
+equations, Dirichlet conditions). pyoomph's own ``Problem`` with one bulk mesh is a
+single sub-problem; several sub-problems are solved as one batch with independent
+Newton iterations (the served / swept case).
+
+Reference interfaces mirrored here:
+* Newton knobs and defaults: ``newton_solver_tolerance`` 1e-8,
+  ``max_newton_iterations`` 10, ``max_residuals`` 1e10,
+  ``always_take_one_newton_step`` True (pyoomph/generic/problem.py:409-411, 536).
+* ``Problem.solve()`` (problem.py:4721) -> steady Newton; raises ``NewtonSolverError``
+  (a ``RuntimeError``) like oomph's ``NewtonSolverError`` (problem.cc:9283-9298).
+* ``Problem.assemble_jacobian()`` (problem.py:711-717): residual vector and the CSR
+  Jacobian in oomph's form (first-touch column order, exact zeros dropped, int32 /
+  float64), as ``_assemble_residual_jacobian`` returns it (src/pybind/problem.cpp:572-602).
+* ``get_current_dofs`` / ``set_current_dofs``.
+* Equation numbering: nodes in mesh order, values in value order, pinned values skipped
+  (oomph mesh.cc:686-708, nodes.cc:896-927).
+All numerical work runs in libpyoomph_hip.so on the GPU; this module only builds the
+numbering and moves arrays.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .equations import Equations, NavierStokesEquations
+
+__all__ = ["MeshProblem", "Problem", "NewtonSolverError"]
+
+
+class NewtonSolverError(RuntimeError):
+    """Raised when a Newton solve exceeds max_residuals or max_newton_iterations."""
+
+
+class MeshProblem:
+    """One mesh with its equations and Dirichlet conditions.
+
+    dirichlet: list of (boundary names, {value_index: value}) applied in order (a node on
+    several boundaries keeps the value of the last condition that names it), as
+    successive ``DirichletBC(...) @ boundary`` do in pyoomph.
+    """
+
+    def __init__(self, mesh, equations, dirichlet=(), pin_extra=()):
+        if not isinstance(equations, Equations):
+            raise TypeError("equations must be an Equations instance")
+        self.mesh = mesh
+        self.equations = equations
+        equations.check_mesh(mesh)
+        nv = equations.nvalue
+        self.pinned = np.zeros((mesh.nnode, nv), dtype=bool)
+        self.values = np.zeros((mesh.nnode, nv), dtype=np.float64)
+        for names, vals in dirichlet:
+            m = mesh.on_boundary(list(names))
+            for vi, v in vals.items():
+                self.pinned[m, vi] = True
+                self.values[m, vi] = v
+        # values a node does not carry (e.g. pressure on Q2-only nodes) are pinned dummies
+        # (BulkElementBase::pin_dummy_values, src/elements.cpp:2271-2330)
+        dummy = equations.dummy_values(mesh)
+        if dummy is not None:
+            self.pinned |= dummy
+        for node, vi, v in pin_extra:
+            self.pinned[node, vi] = True
+            self.values[node, vi] = v
+        free = ~self.pinned.reshape(-1)
+        eqn = np.full(free.shape, -1, dtype=np.int64)
+        eqn[free] = np.arange(int(free.sum()))
+        self.eqn = eqn.reshape(mesh.nnode, nv).astype(np.int32)
+        self.ndof = int(free.sum())
+
+    def nodal_values(self, dofs):
+        U = self.values.copy()
+        m = self.eqn >= 0
+        U[m] = dofs[self.eqn[m]]
+        return U
+
+
+class Problem:
+    """Device-resident steady problem (one or more independent sub-problems)."""
+
+    def __init__(self, parts, device="cuda:0"):
+        import torch
+        if isinstance(parts, MeshProblem):
+            parts = [parts]
+        self.parts = list(parts)
+        if not self.parts:
+            raise ValueError("no sub-problems")
+        self.newton_solver_tolerance = 1e-8
+        self.max_newton_iterations = 10
+        self.max_residuals = 1e10
+        self.always_take_one_newton_step = True
+        self.newton_relaxation_factor = 1.0
+        self.jacobian_precision = "f64"
+        self.linear_precision = "f64"
+        self.preconditioner = "ilu0"
+        self.gmres_restart = 50
+        self.gmres_max_iters = 3000
+        self.gmres_rtol = 1e-10
+        self.max_refinements = 10
+        self.refine_rtol = 1e-10
+        self.torch = torch
+        self.device = torch.device(device)
+        self._handle = None
+        self._keep = []
+        self._build()
+
+    # -- set-up ---------------------------------------------------------------------
+    def _build(self):
+        L = _lib.load()
+        torch = self.torch
+        descs = (_lib.SubproblemDesc * len(self.parts))()
+        keep = []
+        for i, mp in enumerate(self.parts):
+            eq = mp.equations
+            coords = np.ascontiguousarray(mp.mesh.coords, dtype=np.float64)
+            conn = np.ascontiguousarray(mp.mesh.elements, dtype=np.int32)
+            eqn = np.ascontiguousarray(mp.eqn, dtype=np.int32)
+            pv = np.ascontiguousarray(mp.values, dtype=np.float64)
+            par = np.ascontiguousarray(eq.params(), dtype=np.float64)
+            aux = eq.aux(mp.mesh)
+            aux = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+            keep += [coords, conn, eqn, pv, par, aux]
+            d = descs[i]
+            d.form = eq.form
+            d.elem_kind = _lib.ELEM_KIND[mp.mesh.kind]
+            d.nnode = mp.mesh.nnode
+            d.nelem = mp.mesh.nelement
+            d.coords = coords.ctypes.data_as(_lib.c_dp)
+            d.conn = conn.ctypes.data_as(_lib.c_i32p)
+            d.nvalue = eq.nvalue
+            d.eqn = eqn.ctypes.data_as(_lib.c_i32p)
+            d.pinned_values = pv.ctypes.data_as(_lib.c_dp)
+            d.ndof = mp.ndof
+            d.params = par.ctypes.data_as(_lib.c_dp) if par.size else None
+            d.nparams = int(par.size)
+            d.aux = aux.ctypes.data_as(_lib.c_dp) if aux is not None else None
+        torch.cuda.set_device(self.device)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        h = ctypes.c_void_p()
+        _lib.check(L.pyhip_problem_create(descs, len(self.parts), ctypes.c_void_p(stream),
+                                          ctypes.byref(h)))
+        self._handle = h
+        ndof, nnz, nc, nb = (ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64())
+        _lib.check(L.pyhip_problem_info(h, ctypes.byref(ndof), ctypes.byref(nnz), ctypes.byref(nc),
+                                        ctypes.byref(nb)))
+        self.ndof, self.nnz, self.ncontrib = ndof.value, nnz.value, nc.value
+        off = np.zeros(len(self.parts) + 1, dtype=np.int64)
+        _lib.check(L.pyhip_problem_dof_offsets(h, off.ctypes.data_as(_lib.c_i64p)))
+        self.dof_offsets = off
+        self.dofs = torch.zeros(max(self.ndof, 1), dtype=torch.float64, device=self.device)
+
+    def __del__(self):
+        try:
+            if self._handle is not None:
+                _lib.load().pyhip_problem_destroy(self._handle)
+                self._handle = None
+        except Exception:
+            pass
+
+    # -- dofs -------------------------------------------------------------------------
+    def get_current_dofs(self):
+        return self.dofs[:self.ndof].cpu().numpy().copy()
+
+    def set_current_dofs(self, x):
+        x = np.asarray(x, dtype=np.float64)
+        if x.shape != (self.ndof,):
+            raise ValueError("expected %d dofs" % self.ndof)
+        self.dofs[:self.ndof].copy_(self.torch.from_numpy(x))
+
+    def sub_dofs(self, i, x=None):
+        x = self.get_current_dofs() if x is None else x
+        return x[self.dof_offsets[i]:self.dof_offsets[i + 1]]
+
+    # -- assembly ---------------------------------------------------------------------
+    def get_residuals(self, dparameter=None):
+        """Residual vector (oomph Problem::get_residuals); ``dparameter="lambda"`` gives
+        dR/dlambda (get_derivative_wrt_global_parameter, src/problem.cpp:922)."""
+        torch = self.torch
+        res = torch.empty(max(self.ndof, 1), dtype=torch.float64, device=self.device)
+        deriv = -1 if dparameter is None else self._param_index(dparameter)
+        _lib.check(_lib.load().pyhip_assemble(self._handle, _lib.ptr(self.dofs), 0, deriv, _lib.F64,
+                                               _lib.ptr(res), None))
+        torch.cuda.synchronize(self.device)
+        return res[:self.ndof].cpu().numpy()
+
+    def _param_index(self, name):
+        if name != "lambda":
+            raise ValueError("unknown global parameter %r" % name)
+        return 0
+
+    def assemble_device(self, precision="f64", dparameter=None):
+        """Residual and column-sorted CSR values on the device (torch tensors)."""
+        torch = self.torch
+        prec = _lib.F64 if precision == "f64" else _lib.F32
+        res = torch.empty(max(self.ndof, 1), dtype=torch.float64, device=self.device)
+        vals = torch.empty(max(self.nnz, 1), dtype=torch.float64 if prec == _lib.F64 else torch.float32,
+                           device=self.device)
+        deriv = -1 if dparameter is None else self._param_index(dparameter)
+        _lib.check(_lib.load().pyhip_assemble(self._handle, _lib.ptr(self.dofs), 1, deriv, prec,
+                                               _lib.ptr(res), _lib.ptr(vals)))
+        return res, vals
+
+    def structural_pattern(self):
+        rs = np.zeros(self.ndof + 1, dtype=np.int32)
+        col = np.zeros(max(self.nnz, 1), dtype=np.int32)
+        _lib.check(_lib.load().pyhip_pattern_copy(self._handle, rs.ctypes.data_as(_lib.c_i32p),
+                                                   col.ctypes.data_as(_lib.c_i32p)))
+        return rs, col[:self.nnz]
+
+    def assemble_jacobian(self, dparameter=None):
+        """(residual, scipy.sparse.csr_matrix) in the reference's CSR form."""
+        import scipy.sparse
+        torch = self.torch
+        res, vals = self.assemble_device("f64", dparameter)
+        if dparameter is not None:
+            # the export needs the Jacobian itself: assemble it again (dR/dp kept above)
+            res_d = res
+            res, vals_j = self.assemble_device("f64")
+            rs, col, val = self._export(vals)
+            res = res_d
+        else:
+            rs, col, val = self._export(vals)
+        R = res[:self.ndof].cpu().numpy()
+        J = scipy.sparse.csr_matrix((val, col, rs), shape=(self.ndof, self.ndof))
+        return R, J
+
+    def _export(self, vals):
+        torch = self.torch
+        rs = torch.empty(self.ndof + 1, dtype=torch.int32, device=self.device)
+        col = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=self.device)
+        val = torch.empty(max(self.nnz, 1), dtype=torch.float64, device=self.device)
+        nnz = ctypes.c_int64()
+        _lib.check(_lib.load().pyhip_export_reference_csr(self._handle, _lib.ptr(vals), _lib.ptr(rs),
+                                                          _lib.ptr(col), _lib.ptr(val),
+                                                          ctypes.byref(nnz)))
+        n = nnz.value
+        return rs.cpu().numpy(), col[:n].cpu().numpy(), val[:n].cpu().numpy()
+
+    # -- Newton -----------------------------------------------------------------------
+    def newton_options(self):
+        o = _lib.NewtonOpts()
+        _lib.load().pyhip_newton_opts_default(ctypes.byref(o))
+        o.tol = self.newton_solver_tolerance
+        o.max_iter = self.max_newton_iterations
+        o.max_residuals = self.max_residuals
+        o.always_take_one_newton_step = int(self.always_take_one_newton_step)
+        o.relaxation = self.newton_relaxation_factor
+        o.jacobian_precision = _lib.F64 if self.jacobian_precision == "f64" else _lib.F32
+        o.linear.precision = _lib.F64 if self.linear_precision == "f64" else _lib.F32
+        o.linear.precond = _lib.PRECOND_ILU0 if self.preconditioner == "ilu0" else _lib.PRECOND_JACOBI
+        o.linear.restart = self.gmres_restart
+        o.linear.max_iters = self.gmres_max_iters
+        o.linear.rtol = self.gmres_rtol
+        o.linear.max_refinements = self.max_refinements
+        o.linear.refine_rtol = self.refine_rtol
+        return o
+
+    def solve(self, raise_on_failure=True):
+        """Steady Newton solve of every sub-problem; returns per-sub-problem stats."""
+        o = self.newton_options()
+        stats = (_lib.NewtonStats * len(self.parts))()
+        rc = _lib.load().pyhip_newton_solve(self._handle, _lib.ptr(self.dofs), ctypes.byref(o), stats)
+        self.torch.cuda.synchronize(self.device)
+        out = [{"newton_iters": s.newton_iters, "linear_iters": s.linear_iters,
+                "converged": bool(s.converged), "final_maxres": s.final_maxres,
+                "history": list(s.history[:s.nhistory])} for s in stats]
+        self.last_stats = out
+        if rc == _lib.PYHIP_ERR_NEWTON:
+            if raise_on_failure:
+                raise NewtonSolverError(_lib.load().pyhip_last_error().decode())
+            return out
+        _lib.check(rc)
+        return out
