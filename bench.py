@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Benchmark of the assembly-and-solve path on one MI355X.
+
+Workload (headline): the batched sweep of independent 2D Bratu problems
+-lap u = lambda e^u on the unit square, 512 problems, N ~ U{16..128} per side, P1 or P2
+50/50, lambda ~ U(0.5, 3.0), u = 0 on the boundary, all device-resident in one batch.
+A step is one full steady Newton solve of every problem from u = 0 (pyoomph's Newton
+semantics: tolerance 1e-8 on max |R|, always one step); the Jacobians are assembled by
+the FP64 element kernels and the linear systems solved by batched preconditioned CG,
+each problem with its own scalars and convergence test.
+
+Reported: solves/s over the timed steps (value), p50/p99 of the per-problem completion
+time inside a step, the max relative error against the FP64 CPU oracle on a sample,
+FP64 assembly GB/s against the 8 TB/s HBM peak, the roofline of the dominant kernel
+(HIP events on the problem's stream around every launch), the plain FP64
+single-problem GPU path on a sample, and the oracle timed on the host (cpu_baseline).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Newton solves/s & p99 latency at 1e-8 rel err; assembly HBM GB/s vs peak"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nprob", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="bound on the CPU-baseline sample (seconds of oracle work)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-single", action="store_true")
+    return ap.parse_args()
+
+
+def spmv_bytes(P):
+    """Algorithmic bytes of one FP64 CSR SpMV: values + column indices, row starts,
+    x (once per dof) and y."""
+    return 12 * P.nnz + 4 * (P.ndof + 1) + 8 * P.ndof + 8 * P.ndof
+
+
+def assembly_bytes(P):
+    """Minimal bytes of one FP64 residual + Jacobian assembly: nodal values, coordinates,
+    connectivity read once; CSR values and residual written once."""
+    b = 0
+    for mp in P.parts:
+        m = mp.mesh
+        b += 8 * m.nnode * mp.equations.nvalue + 8 * m.nnode * m.dim
+        b += 4 * m.nelement * m.elements.shape[1] + 8 * mp.ndof
+    return b + 8 * P.nnz
+
+
+def timing(P, fam):
+    import ctypes
+    from pyoomph_amd import _lib
+    n = ctypes.c_int64()
+    ms = ctypes.c_double()
+    _lib.check(_lib.load().pyhip_timing_read(P._handle, fam.encode(), ctypes.byref(n), ctypes.byref(ms)))
+    return n.value, ms.value
+
+
+def configure(P):
+    P.krylov = "cg"
+    P.preconditioner = "jacobi"
+    P.jacobian_precision = "f64"
+    P.linear_precision = "f64"
+    P.gmres_max_iters = 20000
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X; "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from pyoomph_amd import _lib, workloads as wl
+    from pyoomph_amd.problem import Problem
+
+    torch.cuda.set_device(0)
+    specs = wl.bratu_sweep_specs(args.nprob, seed=args.seed)
+    parts = [wl.bratu(N, sp, lam) for N, sp, lam in specs]
+    P = Problem(parts)
+    configure(P)
+    print("# batch: %d problems, %d dofs, %d nnz" % (len(parts), P.ndof, P.nnz), file=sys.stderr, flush=True)
+
+    def step():
+        P.dofs.zero_()
+        return P.solve()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.check(_lib.load().pyhip_timing_enable(P._handle, 1))
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = step()
+        lat += [s["done_ms"] for s in st]
+        assert all(s["converged"] for s in st)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    fams = {f: timing(P, f) for f in ("element", "gather_values", "gather_residual", "spmv",
+                                       "precond", "vector")}
+    _lib.check(_lib.load().pyhip_timing_enable(P._handle, 0))
+    x_gpu = P.get_current_dofs()
+    solves_per_s = args.nprob * args.steps / elapsed
+    ms_per_step = elapsed / args.steps * 1e3
+
+    # FP64 assembly (element kernels + gathers), timed with the same events
+    _lib.check(_lib.load().pyhip_timing_enable(P._handle, 1))
+    for _ in range(5):
+        P.assemble_device("f64")
+    torch.cuda.synchronize()
+    asm_ms = sum(timing(P, f)[1] for f in ("element", "gather_values", "gather_residual")) / 5
+    asm_el = timing(P, "element")[1] / 5
+    _lib.check(_lib.load().pyhip_timing_enable(P._handle, 0))
+    asm_gbs = assembly_bytes(P) / (asm_ms * 1e-3) / 1e9
+
+    # dominant kernel family of the timed region and its roofline
+    dom = max(fams, key=lambda f: fams[f][1])
+    n_l, ms_l = fams[dom]
+    avg_ms = ms_l / max(n_l, 1)
+    if dom == "spmv":
+        alg = spmv_bytes(P)
+    elif dom in ("element", "gather_values", "gather_residual"):
+        alg = assembly_bytes(P)
+    else:
+        alg = 16 * P.ndof   # two vectors read per launch (dot products / updates)
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s_bratu_sweep.txt" % dom)
+    if os.path.exists(tfile):
+        for line in open(tfile):
+            if line.startswith("traffic_bytes_per_launch="):
+                traffic = float(line.split("=", 1)[1])
+
+    # plain FP64 single-problem GPU path on a sample (one problem at a time)
+    single = None
+    if not args.no_single:
+        t_single, n_single = 0.0, 0
+        for N, sp, lam in specs[:8]:
+            Q = Problem(wl.bratu(N, sp, lam))
+            configure(Q)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            Q.solve()
+            torch.cuda.synchronize()
+            t_single += time.perf_counter() - t
+            n_single += 1
+            del Q
+        single = n_single / t_single
+
+    # oracle: accuracy sample and CPU baseline (single thread)
+    cpu = None
+    max_err = None
+    if not args.no_cpu_baseline:
+        from oracle import newton as on
+        from tests.oracle_bridge import oracle_problem
+        errs, n_cpu, t_cpu = [], 0, 0.0
+        order = sorted(range(len(specs)), key=lambda i: specs[i][0])
+        for i in order:
+            N, sp, lam = specs[i]
+            op = oracle_problem("bratu", N=N, space=sp, lam=lam)
+            t = time.perf_counter()
+            xo = on.newton_solve(op, np.zeros(op.ndof))
+            t_cpu += time.perf_counter() - t
+            n_cpu += 1
+            xg = P.sub_dofs(i, x_gpu)
+            errs.append(float(np.max(np.abs(xg - xo)) / max(np.max(np.abs(xo)), 1e-300)))
+            if t_cpu > args.cpu_seconds:
+                break
+        max_err = max(errs)
+        sizes = [specs[i][0] for i in order[:n_cpu]]
+        cpu = {"value": n_cpu / t_cpu, "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": "%d smallest problems of the sweep (N=%d..%d), oracle C element code + "
+                         "vectors_of_pairs CSR + scipy SuperLU, one thread" % (n_cpu, min(sizes), max(sizes))}
+
+    lat = np.array(lat)
+    out = {
+        "metric": METRIC,
+        "value": solves_per_s,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "bratu_sweep: %d independent -lap u = lambda e^u problems, "
+                               "N~U{16..128}, P1/P2 50/50, lambda~U(0.5,3)" % args.nprob,
+                   "problems": args.nprob, "dofs": P.ndof, "nnz": P.nnz,
+                   "solver": "Newton + batched CG/Jacobi, FP64", "seed": args.seed},
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "max_rel_err_vs_oracle": max_err,
+        "assembly_fp64_ms": asm_ms,
+        "assembly_element_kernel_ms": asm_el,
+        "assembly_GBps": asm_gbs,
+        "assembly_frac_of_hbm_peak": asm_gbs / HBM_PEAK_GBS,
+        "single_problem_fp64_solves_per_s": single,
+        "batched_speedup_vs_single": (solves_per_s / single) if single else None,
+        "kernel_ms": {f: {"launches": v[0], "ms": v[1]} for f, v in fams.items()},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -86,7 +86,8 @@ typedef struct {
     const int32_t *eqn;          /* [nnode][nvalue] sub-problem equation, -1 pinned     */
     const double *pinned_values; /* [nnode][nvalue] value used where eqn == -1          */
     int64_t ndof;
-    const double *params;        /* form parameters                     */
+    const double *params;        /* Poisson: n, then (cx, cy, w, a) per Gaussian source;
+                                    Bratu: lambda; Navier-Stokes: mu, rho               */
     int nparams;
     const double *aux;           /* [nnode] auxiliary nodal field, or NULL              */
 } pyhip_subproblem_desc;
@@ -141,20 +142,20 @@ int pyhip_export_reference_csr(pyhip_problem *p, const double *values_dev,
  * pyoomph/solvers/scipy.py:47-104). */
 #define PYHIP_PRECOND_JACOBI 0
 #define PYHIP_PRECOND_ILU0 1
+#define PYHIP_KRYLOV_GMRES 0
+#define PYHIP_KRYLOV_CG 1    /* symmetric positive definite Jacobians (Poisson, Bratu
+                                below its fold, reaction-diffusion with u^3)           */
 typedef struct {
+    int krylov;         /* PYHIP_KRYLOV_*                                             */
     int precond;        /* PYHIP_PRECOND_*                                            */
     int precision;      /* Krylov + preconditioner arithmetic: PYHIP_F64 / PYHIP_F32   */
     int restart;        /* GMRES restart length m                                     */
     int max_iters;      /* total inner iterations per solve                           */
-    double rtol;        /* relative residual target of the inner solve                */
+    double rtol;        /* relative residual target of an FP64 inner solve            */
+    double rtol_f32;    /* relative residual target of an FP32 inner solve            */
     int max_refinements;/* FP64 iterative-refinement sweeps around an FP32 solve       */
     double refine_rtol; /* relative residual target of the refined solution            */
 } pyhip_linear_opts;
-typedef struct {
-    int iterations;     /* inner iterations summed over sub-problems' maximum          */
-    int refinements;
-    double rel_residual;/* max over sub-problems of |b - A x| / |b| (FP64)             */
-} pyhip_linear_stats;
 void pyhip_linear_opts_default(pyhip_linear_opts *o);
 
 /* ---- Newton ---------------------------------------------------------------------------------
@@ -174,16 +175,29 @@ typedef struct {
 typedef struct {
     int newton_iters;       /* Newton steps taken                                     */
     int linear_iters;       /* inner Krylov iterations, all steps                      */
+    int linear_failures;    /* linear solves that stopped above their tolerance        */
     int converged;          /* 1 converged, 0 NewtonSolverError                        */
     double final_maxres;    /* max |R| after the last step                             */
     double history[16];     /* max |R| per residual evaluation (Max_res)               */
     int nhistory;
+    double done_ms;         /* wall time from the start of the solve to the Newton step
+                               that ended this sub-problem's iteration                  */
 } pyhip_newton_stats;
 void pyhip_newton_opts_default(pyhip_newton_opts *o);
 /* dofs_dev [ndof] in/out. stats: host array [nsub]. Returns PYHIP_ERR_NEWTON if any
  * sub-problem failed (its stats say which). */
 int pyhip_newton_solve(pyhip_problem *p, double *dofs_dev, const pyhip_newton_opts *opts,
                        pyhip_newton_stats *stats);
+
+/* ---- kernel timing -----------------------------------------------------------------------
+ * Measurement aid (no counterpart in the reference, whose only timers are wall clocks,
+ * problem.cc:8990,9149): while enabled, the problem records a pair of HIP events on its
+ * stream around every launch of the named kernel families, "element" (element
+ * residual / Jacobian kernels), "gather_values", "gather_residual", "spmv",
+ * "precond" (Jacobi / ILU(0) apply and factorisation), "vector" (dots, updates).
+ * pyhip_timing_read returns the launches and summed device milliseconds so far. */
+int pyhip_timing_enable(pyhip_problem *p, int enable);
+int pyhip_timing_read(pyhip_problem *p, const char *family, int64_t *launches, double *total_ms);
 
 /* ---- linear-solver plugin entry --------------------------------------------------------------
  * Same arguments and meaning as GenericLinearSystemSolver.solve_serial

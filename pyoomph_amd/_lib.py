@@ -23,6 +23,7 @@ ELEM_KIND = {"tri_p1": ELEM_TRI_P1, "tri_p2": ELEM_TRI_P2, "quad_q2": ELEM_QUAD_
 FORM_POISSON, FORM_BRATU, FORM_REACTION_DIFFUSION, FORM_NAVIER_STOKES_TH = 1, 2, 3, 4
 F64, F32 = 0, 1
 PRECOND_JACOBI, PRECOND_ILU0 = 0, 1
+KRYLOV_GMRES, KRYLOV_CG = 0, 1
 
 c_i32p = ctypes.POINTER(ctypes.c_int32)
 c_i64p = ctypes.POINTER(ctypes.c_int64)
@@ -49,9 +50,10 @@ class ElementTable(ctypes.Structure):
 
 
 class LinearOpts(ctypes.Structure):
-    _fields_ = [("precond", ctypes.c_int), ("precision", ctypes.c_int),
+    _fields_ = [("krylov", ctypes.c_int), ("precond", ctypes.c_int), ("precision", ctypes.c_int),
                 ("restart", ctypes.c_int), ("max_iters", ctypes.c_int),
-                ("rtol", ctypes.c_double), ("max_refinements", ctypes.c_int),
+                ("rtol", ctypes.c_double), ("rtol_f32", ctypes.c_double),
+                ("max_refinements", ctypes.c_int),
                 ("refine_rtol", ctypes.c_double)]
 
 
@@ -65,8 +67,9 @@ class NewtonOpts(ctypes.Structure):
 
 class NewtonStats(ctypes.Structure):
     _fields_ = [("newton_iters", ctypes.c_int), ("linear_iters", ctypes.c_int),
-                ("converged", ctypes.c_int), ("final_maxres", ctypes.c_double),
-                ("history", ctypes.c_double * 16), ("nhistory", ctypes.c_int)]
+                ("linear_failures", ctypes.c_int), ("converged", ctypes.c_int), ("final_maxres", ctypes.c_double),
+                ("history", ctypes.c_double * 16), ("nhistory", ctypes.c_int),
+                ("done_ms", ctypes.c_double)]
 
 
 # every symbol declared in include/pyoomph_hip.h, with (restype, argtypes)
@@ -90,6 +93,9 @@ SIGNATURES = {
     "pyhip_newton_opts_default": (None, [ctypes.POINTER(NewtonOpts)]),
     "pyhip_newton_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
                                           ctypes.POINTER(NewtonStats)]),
+    "pyhip_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "pyhip_timing_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, c_i64p,
+                                         ctypes.POINTER(ctypes.c_double)]),
     "pyhip_solve_serial": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           c_dp, c_i32p, c_i32p, c_dp, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_void_p)]),

@@ -370,6 +370,7 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.Er = p->Er64 + G.er_off;
             a.Ej = flag ? p->Ej64 + G.ej_off : nullptr;
             a.flag = flag; a.deriv = deriv_param >= 0;
+            TimeScope ts(&p->timer, "element", p->st);
             if ((rc = launch_elements_f64(G.kind, G.form, a, p->st))) return rc;
         } else {
             ElemArgs<float> a{};
@@ -381,22 +382,29 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.Er = p->Er32 + G.er_off;
             a.Ej = flag ? p->Ej32 + G.ej_off : nullptr;
             a.flag = flag; a.deriv = deriv_param >= 0;
+            TimeScope ts(&p->timer, "element", p->st);
             if ((rc = launch_elements_f32(G.kind, G.form, a, p->st))) return rc;
         }
     }
     if (precision == PYHIP_F64) {
-        if ((rc = launch_gather_residual_f64(p->ndof, p->occ_ptr, p->occ_roff, p->Er64, residual, p->st)))
-            return rc;
+        {
+            TimeScope ts(&p->timer, "gather_residual", p->st);
+            if ((rc = launch_gather_residual_f64(p->ndof, p->occ_ptr, p->occ_roff, p->Er64, residual, p->st)))
+                return rc;
+        }
         if (flag) {
+            TimeScope ts(&p->timer, "gather_values", p->st);
             if ((rc = launch_gather_values_f64(p->ndof, p->row_start, p->occ_ptr, p->occ_eoff,
                                                p->occ_pos, p->Ej64, (double *)values, p->st)))
                 return rc;
             p->have_ej64 = (deriv_param < 0);
         }
     } else {
+        TimeScope ts(&p->timer, "gather_residual", p->st);
         if ((rc = launch_gather_residual_f32(p->ndof, p->occ_ptr, p->occ_roff, p->Er32, residual, p->st)))
             return rc;
         if (flag) {
+            TimeScope ts2(&p->timer, "gather_values", p->st);
             if ((rc = launch_gather_values_f32(p->ndof, p->row_start, p->occ_ptr, p->occ_eoff,
                                                p->occ_pos, p->Ej32, (float *)values, p->st)))
                 return rc;
@@ -417,6 +425,25 @@ int pyhip_export_reference_csr(pyhip_problem *p, const double *values, int32_t *
     return export_reference_csr(p->ndof, p->row_start, p->col, p->occ_ptr, p->occ_eoff,
                                 p->occ_pos, p->Ej64, values, p->keys, p->kept, rs_out, col_out,
                                 val_out, nnz_out, p->st);
+}
+
+int pyhip_timing_enable(pyhip_problem *p, int enable) {
+    PYHIP_REQUIRE(p, "null problem");
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    p->timer.flush();
+    p->timer.total.clear();
+    p->timer.on = enable != 0;
+    return PYHIP_OK;
+}
+
+int pyhip_timing_read(pyhip_problem *p, const char *family, int64_t *launches, double *total_ms) {
+    PYHIP_REQUIRE(p && family && launches && total_ms, "null argument");
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    p->timer.flush();
+    auto it = p->timer.total.find(family);
+    *launches = it == p->timer.total.end() ? 0 : it->second.first;
+    *total_ms = it == p->timer.total.end() ? 0.0 : it->second.second;
+    return PYHIP_OK;
 }
 
 }  // extern "C"

@@ -64,6 +64,9 @@ struct KrylovWork {
     double *jdiag64 = nullptr;        // [n] inverse diagonal (Jacobi)
     float *jdiag32 = nullptr;
     int32_t *dof_mask = nullptr;      // [nsub] Newton: 1 if the sub-problem takes a step
+    int32_t *need = nullptr;          // [nsub] refinement: sub-problems still refining
+    double *cg_rz = nullptr, *cg_pq = nullptr, *cg_beta = nullptr;   // [nsub]
+    int32_t *nfail_d = nullptr;       // breakdown / non-convergence counter
     IluPlan ilu;
     int64_t bytes = 0;
 };

@@ -19,6 +19,7 @@
 // the velocity unknowns ordered before the pressure unknowns (natural order within
 // each), which keeps ILU(0) pivots non-zero for the zero pressure-pressure block.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -420,7 +421,9 @@ static int krylov_setup(pyhip_problem *p, int m) {
             (rc = kalloc(&W->t32, n, W)) || (rc = kalloc(&W->res64, n, W)) ||
             (rc = kalloc(&W->maxres, ns, W)) || (rc = kalloc(&W->sub_scalar, ns, W)) ||
             (rc = kalloc(&W->jdiag64, n, W)) || (rc = kalloc(&W->jdiag32, n, W)) ||
-            (rc = kalloc(&W->dof_mask, ns, W)))
+            (rc = kalloc(&W->dof_mask, ns, W)) || (rc = kalloc(&W->need, ns, W)) ||
+            (rc = kalloc(&W->cg_rz, ns, W)) || (rc = kalloc(&W->cg_pq, ns, W)) ||
+            (rc = kalloc(&W->cg_beta, ns, W)) || (rc = kalloc(&W->nfail_d, 1, W)))
             return rc;
         PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->nactive_h, sizeof(int32_t), hipHostMallocDefault));
         PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->maxres_h, ns * sizeof(double), hipHostMallocDefault));
@@ -452,6 +455,7 @@ static int krylov_setup(pyhip_problem *p, int m) {
 template <typename T>
 static int batched_dots(pyhip_problem *p, KrylovWork *W, int nvec, const T *V, int64_t ldv,
                         const T *w, double *out) {
+    TimeScope ts(&p->timer, "vector", p->st);
     const int pstride = W->m + 2;
     if (W->nchunk == 0) return PYHIP_OK;
     hipLaunchKernelGGL((multidot_kernel<T>), dim3((unsigned)W->nchunk), dim3(256), 0, p->st, nvec,
@@ -478,6 +482,7 @@ int batched_maxabs(pyhip_problem *p, const double *v, double *out) {
 
 template <typename TV, typename TX, typename TY>
 static int spmv(pyhip_problem *p, const TV *vals, const TX *x, TY *y, const TY *b) {
+    TimeScope ts(&p->timer, "spmv", p->st);
     const int64_t n = p->ndof;
     if (n == 0) return PYHIP_OK;
     const double avg = p->nnz / (double)n;
@@ -655,6 +660,7 @@ static int ilu_apply(pyhip_problem *p, KrylovWork *W, const T *v, T *z) {
 
 template <typename T>
 static int precond_apply(pyhip_problem *p, KrylovWork *W, int pc, const T *v, T *z) {
+    TimeScope ts(&p->timer, "precond", p->st);
     if (pc == PYHIP_PRECOND_ILU0) return ilu_apply<T>(p, W, v, z);
     const T *jd = std::is_same<T, float>::value ? (const T *)W->jdiag32 : (const T *)W->jdiag64;
     hipLaunchKernelGGL((jacobi_apply_kernel<T>), dim3(nblk(p->ndof)), dim3(256), 0, p->st, p->ndof, jd, v, z);
@@ -664,6 +670,7 @@ static int precond_apply(pyhip_problem *p, KrylovWork *W, int pc, const T *v, T 
 
 template <typename TV, typename T>
 static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals) {
+    TimeScope ts(&p->timer, "precond", p->st);
     if (pc == PYHIP_PRECOND_ILU0) {
         int rc = ilu_plan_build(p, W);
         if (rc) return rc;
@@ -680,7 +687,8 @@ static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals
 // b, x: device vectors of type T. Returns the iterations of the longest-running one.
 template <typename T, typename TV>
 static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T *x,
-                 const pyhip_linear_opts &o, const int32_t *need, int *iters) {
+                 const pyhip_linear_opts &o, double rtol, const int32_t *need, int *iters,
+                 int *failed) {
     const int64_t n = p->ndof;
     const int m = W->m, ns = W->nsub;
     const int prec = std::is_same<T, float>::value ? PYHIP_F32 : PYHIP_F64;
@@ -709,7 +717,7 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
         if ((rc = batched_dots<T>(p, W, 1, V, n, V, d3))) return rc;
         PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
         hipLaunchKernelGGL(gmres_start_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, m, d3,
-                           W->bnorm, o.rtol, need, W->g, W->active, W->kcols, W->beta0, W->nactive_d);
+                           W->bnorm, rtol, need, W->g, W->active, W->kcols, W->beta0, W->nactive_d);
         KCHECK();
         PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
         PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
@@ -734,7 +742,7 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
                                W->active, V + (int64_t)(k + 1) * n);
             PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
             hipLaunchKernelGGL(gmres_givens_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, m, k, d1, d2,
-                               d3, W->H, W->cs, W->sn, W->g, W->bnorm, o.rtol, W->active, W->kcols,
+                               d3, W->H, W->cs, W->sn, W->g, W->bnorm, rtol, W->active, W->kcols,
                                W->nactive_d);
             KCHECK();
             PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
@@ -754,7 +762,208 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
         KCHECK();
         if (total >= o.max_iters) break;
     }
+    // sub-problems still above tolerance (true residual of the returned x)
+    if ((rc = spmv<TV, T, T>(p, vals, x, V, b))) return rc;
+    if ((rc = batched_dots<T>(p, W, 1, V, n, V, d3))) return rc;
+    PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
+    hipLaunchKernelGGL(gmres_start_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, m, d3, W->bnorm,
+                       rtol * 1.01, need, W->g, W->active, W->kcols, W->beta0, W->nactive_d);
+    KCHECK();
+    PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    *failed = *W->nactive_h;
     *iters = total;
+    return PYHIP_OK;
+}
+
+
+
+// ---- preconditioned conjugate gradients (symmetric positive definite systems) ----------
+
+template <typename T>
+__global__ __launch_bounds__(256) void dot2_kernel(const T *__restrict__ a, const T *__restrict__ b,
+                                                   const T *__restrict__ c, const T *__restrict__ d,
+                                                   const int64_t *__restrict__ cb,
+                                                   const int64_t *__restrict__ ce,
+                                                   double *__restrict__ partial, int pstride) {
+    __shared__ double r0[256], r1[256];
+    const int64_t ch = blockIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t i = cb[ch] + threadIdx.x; i < ce[ch]; i += 256) {
+        s0 += double(a[i]) * double(b[i]);
+        s1 += double(c[i]) * double(d[i]);
+    }
+    r0[threadIdx.x] = s0;
+    r1[threadIdx.x] = s1;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            r0[threadIdx.x] += r0[threadIdx.x + s];
+            r1[threadIdx.x] += r1[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[ch * pstride] = r0[0];
+        partial[ch * pstride + 1] = r1[0];
+    }
+}
+
+// start: bnorm = |r0|, rz = (r0, z0); active where needed and |b| > 0
+__global__ void cg_start_kernel(int nsub, const double *__restrict__ d2,
+                                const int32_t *__restrict__ need, double *__restrict__ bnorm,
+                                double *__restrict__ rz, int32_t *__restrict__ active,
+                                int32_t *__restrict__ nactive) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsub) return;
+    bnorm[s] = sqrt(d2[s]);
+    rz[s] = d2[nsub + s];
+    const int a = need[s] && d2[s] > 0.0 && isfinite(d2[s]);
+    active[s] = a;
+    if (a) atomicAdd(nactive, 1);
+}
+
+// alpha = rz / (p, Ap); a non-positive curvature stops the sub-problem (breakdown)
+__global__ void cg_alpha_kernel(int nsub, const double *__restrict__ pq_in, double *__restrict__ pq,
+                                int32_t *__restrict__ active, int32_t *__restrict__ nfail) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsub || !active[s]) return;
+    const double v = pq_in[s];
+    pq[s] = v;
+    if (!(v > 0.0) || !isfinite(v)) {
+        active[s] = 0;
+        atomicAdd(nfail, 1);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void cg_xr_kernel(int64_t n, const T *__restrict__ p,
+                                                    const T *__restrict__ q, T *__restrict__ x,
+                                                    T *__restrict__ r, const double *__restrict__ rz,
+                                                    const double *__restrict__ pq,
+                                                    const int32_t *__restrict__ dsub,
+                                                    const int32_t *__restrict__ active) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int s = dsub[i];
+    if (!active[s]) return;
+    const double a = rz[s] / pq[s];
+    x[i] = T(double(x[i]) + a * double(p[i]));
+    r[i] = T(double(r[i]) - a * double(q[i]));
+}
+
+// convergence test on |r|, then beta = (r,z)_new / (r,z)_old
+__global__ void cg_check_kernel(int nsub, const double *__restrict__ d2,
+                                const double *__restrict__ bnorm, double rtol,
+                                double *__restrict__ rz, double *__restrict__ beta,
+                                int32_t *__restrict__ active, int32_t *__restrict__ nactive) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsub || !active[s]) return;
+    const double rr = d2[s], rzn = d2[nsub + s];
+    if (!(sqrt(rr) > rtol * bnorm[s]) || !isfinite(rr)) {
+        active[s] = 0;
+        return;
+    }
+    beta[s] = rzn / rz[s];
+    rz[s] = rzn;
+    atomicAdd(nactive, 1);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void cg_p_kernel(int64_t n, const T *__restrict__ z,
+                                                   T *__restrict__ p, const double *__restrict__ beta,
+                                                   const int32_t *__restrict__ dsub,
+                                                   const int32_t *__restrict__ active) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int s = dsub[i];
+    if (!active[s]) return;
+    p[i] = T(double(z[i]) + beta[s] * double(p[i]));
+}
+
+template <typename T>
+static int batched_dot2(pyhip_problem *p, KrylovWork *W, const T *a, const T *b, const T *c,
+                        const T *d, double *out) {
+    TimeScope ts(&p->timer, "vector", p->st);
+    const int pstride = W->m + 2;
+    if (W->nchunk == 0) return PYHIP_OK;
+    hipLaunchKernelGGL((dot2_kernel<T>), dim3((unsigned)W->nchunk), dim3(256), 0, p->st, a, b, c, d,
+                       W->chunk_begin, W->chunk_end, W->partial, pstride);
+    KCHECK();
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblk((int64_t)2 * W->nsub, 64)), dim3(64), 0, p->st,
+                       2, W->nsub, W->sub_chunk_ptr, W->partial, pstride, out, 0);
+    KCHECK();
+    return PYHIP_OK;
+}
+
+static int read_counter(pyhip_problem *p, KrylovWork *W, int32_t *dev, int32_t *out) {
+    PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, dev, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    *out = *W->nactive_h;
+    return PYHIP_OK;
+}
+
+// x = A^{-1} b per sub-problem with need[s]; CG in precision T on values TV.
+template <typename T, typename TV>
+static int cg(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T *x,
+              const pyhip_linear_opts &o, double rtol, const int32_t *need, int *iters, int *failed) {
+    const int64_t n = p->ndof;
+    const int ns = W->nsub;
+    const int prec = std::is_same<T, float>::value ? PYHIP_F32 : PYHIP_F64;
+    int rc;
+    if (W->vprec != prec) {
+        dfree(W->V);
+        W->V = nullptr;
+        if ((rc = dev_alloc((T **)&W->V, (size_t)(W->m + 1) * n, W->bytes))) return rc;
+        W->vprec = prec;
+    }
+    T *pv = (T *)W->V;
+    T *qv = pv + n;
+    T *r = std::is_same<T, float>::value ? (T *)W->w32 : (T *)W->w64;
+    T *z = std::is_same<T, float>::value ? (T *)W->t32 : (T *)W->t64;
+    double *d2 = W->dots;
+    PYHIP_CHECK_HIP(hipMemsetAsync(x, 0, n * sizeof(T), p->st));
+    hipLaunchKernelGGL((convert_kernel<T, T>), dim3(nblk(n)), dim3(256), 0, p->st, n, b, r);
+    if ((rc = precond_apply<T>(p, W, o.precond, r, z))) return rc;
+    if ((rc = batched_dot2<T>(p, W, r, r, r, z, d2))) return rc;
+    PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
+    PYHIP_CHECK_HIP(hipMemsetAsync(W->nfail_d, 0, sizeof(int32_t), p->st));
+    hipLaunchKernelGGL(cg_start_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2, need, W->bnorm,
+                       W->cg_rz, W->active, W->nactive_d);
+    hipLaunchKernelGGL((convert_kernel<T, T>), dim3(nblk(n)), dim3(256), 0, p->st, n, z, pv);
+    KCHECK();
+    int32_t nact = 0;
+    if ((rc = read_counter(p, W, W->nactive_d, &nact))) return rc;
+    int it = 0;
+    while (nact > 0 && it < o.max_iters) {
+        if ((rc = spmv<TV, T, T>(p, vals, pv, qv, (const T *)nullptr))) return rc;
+        if ((rc = batched_dots<T>(p, W, 1, pv, n, qv, d2 + 2 * ns))) return rc;
+        hipLaunchKernelGGL(cg_alpha_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2 + 2 * ns,
+                           W->cg_pq, W->active, W->nfail_d);
+        {
+            TimeScope ts(&p->timer, "vector", p->st);
+            hipLaunchKernelGGL((cg_xr_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, pv, qv, x, r,
+                               W->cg_rz, W->cg_pq, p->dof_sub, W->active);
+        }
+        KCHECK();
+        if ((rc = precond_apply<T>(p, W, o.precond, r, z))) return rc;
+        if ((rc = batched_dot2<T>(p, W, r, r, r, z, d2))) return rc;
+        PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
+        hipLaunchKernelGGL(cg_check_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2, W->bnorm, rtol,
+                           W->cg_rz, W->cg_beta, W->active, W->nactive_d);
+        {
+            TimeScope ts(&p->timer, "vector", p->st);
+            hipLaunchKernelGGL((cg_p_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, z, pv, W->cg_beta,
+                               p->dof_sub, W->active);
+        }
+        KCHECK();
+        it++;
+        if ((rc = read_counter(p, W, W->nactive_d, &nact))) return rc;
+    }
+    int32_t nf = 0;
+    if ((rc = read_counter(p, W, W->nfail_d, &nf))) return rc;
+    *iters = it;
+    *failed = nf + nact;   // breakdowns + sub-problems still above tolerance
     return PYHIP_OK;
 }
 
@@ -772,19 +981,29 @@ __global__ void newton_state_kernel(int nsub, const int32_t *__restrict__ runnin
 }
 
 // Solve J d = r for the masked sub-problems. r: FP64 right-hand side; d: FP64 result.
+template <typename T, typename TV>
+static int krylov(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T *x,
+                  const pyhip_linear_opts &o, double rtol, const int32_t *need, int *iters,
+                  int *failed) {
+    if (o.krylov == PYHIP_KRYLOV_CG) return cg<T, TV>(p, W, vals, b, x, o, rtol, need, iters, failed);
+    return gmres<T, TV>(p, W, vals, b, x, o, rtol, need, iters, failed);
+}
+
 static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts &o, int jac_prec,
                         const double *r, double *d, const int32_t *mask, int *iters,
-                        int *refinements) {
+                        int *refinements, int *failed) {
     int rc;
     const int64_t n = p->ndof;
     *iters = 0;
     *refinements = 0;
+    *failed = 0;
     if (o.precision == PYHIP_F64) {
         PYHIP_REQUIRE(jac_prec == PYHIP_F64, "an FP64 Krylov solve needs the FP64 Jacobian");
         if ((rc = precond_setup<double, double>(p, W, o.precond, W->vals64))) return rc;
-        return gmres<double, double>(p, W, W->vals64, r, d, o, mask, iters);
+        return krylov<double, double>(p, W, W->vals64, r, d, o, o.rtol, mask, iters, failed);
     }
-    // FP32 inner solves inside FP64 iterative refinement
+    // FP32 inner solves inside FP64 iterative refinement:
+    //   d = 0; repeat: s = A32^{-1} rr (FP32, to rtol_f32); d += s; rr = r - J d (FP64)
     if (jac_prec == PYHIP_F64) {
         hipLaunchKernelGGL((convert_kernel<double, float>), dim3(nblk(p->nnz)), dim3(256), 0, p->st,
                            p->nnz, W->vals64, W->vals32);
@@ -796,51 +1015,41 @@ static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts
     KCHECK();
     const int ns = W->nsub;
     double *d3 = W->dots + 2 * (W->m + 2) * ns;
-    // |r| of the original right-hand side per sub-problem
     if ((rc = batched_dots<double>(p, W, 1, r, n, r, W->dots))) return rc;
     std::vector<double> rn0(ns), rn(ns);
     PYHIP_CHECK_HIP(hipMemcpyAsync(rn0.data(), W->dots, ns * sizeof(double), hipMemcpyDeviceToHost, p->st));
-    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
     std::vector<int32_t> need_h(ns);
-    PYHIP_CHECK_HIP(hipMemcpy(need_h.data(), mask, ns * sizeof(int32_t), hipMemcpyDeviceToHost));
-    int32_t *need_d = nullptr;
-    if ((rc = dev_alloc(&need_d, ns, W->bytes))) return rc;
-    PYHIP_CHECK_HIP(hipMemcpy(need_d, need_h.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice));
+    PYHIP_CHECK_HIP(hipMemcpyAsync(need_h.data(), mask, ns * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    PYHIP_CHECK_HIP(hipMemcpyAsync(W->need, need_h.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, p->st));
+    int left = 0;
     for (int it = 0; it < std::max(1, o.max_refinements); it++) {
         hipLaunchKernelGGL((convert_kernel<double, float>), dim3(nblk(n)), dim3(256), 0, p->st, n, W->r64, W->r32);
         KCHECK();
-        int its = 0;
-        if ((rc = gmres<float, float>(p, W, W->vals32, W->r32, W->d32, o, need_d, &its))) {
-            dfree(need_d);
+        int its = 0, f = 0;
+        if ((rc = krylov<float, float>(p, W, W->vals32, W->r32, W->d32, o, o.rtol_f32, W->need, &its, &f)))
             return rc;
-        }
         *iters += its;
         (*refinements)++;
         hipLaunchKernelGGL((axpy_kernel<float, double>), dim3(nblk(n)), dim3(256), 0, p->st, n, W->d32, d, 1.0,
-                           p->dof_sub, need_d);
-        // r64 = r - J d  (FP64 arithmetic on the stored Jacobian)
+                           p->dof_sub, W->need);
+        // rr = r - J d  (FP64 arithmetic on the stored Jacobian)
         if (jac_prec == PYHIP_F64) rc = spmv<double, double, double>(p, W->vals64, d, W->r64, r);
         else rc = spmv<float, double, double>(p, W->vals32, d, W->r64, r);
-        if (rc) {
-            dfree(need_d);
-            return rc;
-        }
-        if ((rc = batched_dots<double>(p, W, 1, W->r64, n, W->r64, d3))) {
-            dfree(need_d);
-            return rc;
-        }
+        if (rc) return rc;
+        if ((rc = batched_dots<double>(p, W, 1, W->r64, n, W->r64, d3))) return rc;
         PYHIP_CHECK_HIP(hipMemcpyAsync(rn.data(), d3, ns * sizeof(double), hipMemcpyDeviceToHost, p->st));
         PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
-        int left = 0;
+        left = 0;
         for (int s = 0; s < ns; s++) {
             if (!need_h[s]) continue;
             if (!(std::sqrt(rn[s]) > o.refine_rtol * std::sqrt(rn0[s]))) need_h[s] = 0;
             left += need_h[s];
         }
         if (!left) break;
-        PYHIP_CHECK_HIP(hipMemcpy(need_d, need_h.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice));
+        PYHIP_CHECK_HIP(hipMemcpyAsync(W->need, need_h.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, p->st));
     }
-    dfree(need_d);
+    *failed = left;
     return PYHIP_OK;
 }
 
@@ -858,8 +1067,15 @@ int plugin_factor(pyhip_problem *p, const pyhip_linear_opts &o) {
 int plugin_solve(pyhip_problem *p, const pyhip_linear_opts &o, const double *b, double *x, int *iters) {
     KrylovWork *W = p->kw;
     int32_t one = 1;
+    int failed = 0;
     PYHIP_CHECK_HIP(hipMemcpy(W->dof_mask, &one, sizeof(int32_t), hipMemcpyHostToDevice));
-    return gmres<double, double>(p, W, W->vals64, b, x, o, W->dof_mask, iters);
+    int rc = krylov<double, double>(p, W, W->vals64, b, x, o, o.rtol, W->dof_mask, iters, &failed);
+    if (rc) return rc;
+    if (failed) {
+        set_error("Krylov solver did not reach its tolerance");
+        return PYHIP_ERR_NOT_CONVERGED;
+    }
+    return PYHIP_OK;
 }
 
 }  // namespace pyhip
@@ -869,11 +1085,13 @@ using namespace pyhip;
 extern "C" {
 
 void pyhip_linear_opts_default(pyhip_linear_opts *o) {
+    o->krylov = PYHIP_KRYLOV_GMRES;
     o->precond = PYHIP_PRECOND_ILU0;
     o->precision = PYHIP_F64;
     o->restart = 50;
-    o->max_iters = 2000;
+    o->max_iters = 5000;
     o->rtol = 1e-10;
+    o->rtol_f32 = 1e-5;
     o->max_refinements = 8;
     o->refine_rtol = 1e-10;
 }
@@ -895,7 +1113,8 @@ void pyhip_krylov_free(pyhip_problem *p) {
                     W->H, W->cs, W->sn, W->g, W->beta0, W->bnorm, W->kcols, W->active, W->nactive_d,
                     W->y, W->V, W->r64, W->x64, W->d64, W->b64, W->r32, W->d32, W->w32, W->t32,
                     W->w64, W->t64, W->vals32, W->vals64, W->res64, W->maxres, W->sub_scalar,
-                    W->jdiag64, W->jdiag32, W->dof_mask, W->ilu.perm, W->ilu.rs, W->ilu.col,
+                    W->jdiag64, W->jdiag32, W->dof_mask, W->need, W->cg_rz, W->cg_pq,
+                    W->cg_beta, W->nfail_d, W->ilu.perm, W->ilu.rs, W->ilu.col,
                     W->ilu.src, W->ilu.diag, W->ilu.lo_rows, W->ilu.up_rows, W->ilu.lu,
                     W->ilu.tmp64, W->ilu.tmp64b, W->ilu.tmp32, W->ilu.tmp32b};
     for (void *q : ptrs)
@@ -922,6 +1141,10 @@ int pyhip_newton_solve(pyhip_problem *p, double *x, const pyhip_newton_opts *o,
     if (o->linear.precision == PYHIP_F32 && !W->vals32)
         if ((rc = dev_alloc(&W->vals32, p->nnz, W->bytes))) return rc;
     memset(stats, 0, sizeof(pyhip_newton_stats) * ns);
+    const auto t_start = std::chrono::steady_clock::now();
+    auto elapsed_ms = [&]() {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    };
     std::vector<int32_t> running(ns, 1);
     std::vector<int> count(ns, 0);
     auto record = [&](int s, double v) {
@@ -944,6 +1167,7 @@ int pyhip_newton_solve(pyhip_problem *p, double *x, const pyhip_newton_opts *o,
         if (p->dof_off_h[s + 1] == p->dof_off_h[s]) running[s] = 0, stats[s].converged = 1;
         else if (W->maxres_h[s] < o->tol && !o->always_take_one_newton_step)
             running[s] = 0, stats[s].converged = 1;
+        if (!running[s]) stats[s].done_ms = elapsed_ms();
     }
     int any_failed = 0;
     while (true) {
@@ -958,11 +1182,15 @@ int pyhip_newton_solve(pyhip_problem *p, double *x, const pyhip_newton_opts *o,
         } else {
             if ((rc = pyhip_assemble(p, x, 1, -1, PYHIP_F32, W->w64, W->vals32))) return rc;
         }
-        int its = 0, refs = 0;
-        if ((rc = linear_solve(p, W, o->linear, o->jacobian_precision, W->res64, W->d64, W->dof_mask, &its, &refs)))
+        int its = 0, refs = 0, lfail = 0;
+        if ((rc = linear_solve(p, W, o->linear, o->jacobian_precision, W->res64, W->d64, W->dof_mask,
+                               &its, &refs, &lfail)))
             return rc;
         for (int s = 0; s < ns; s++)
-            if (running[s]) stats[s].linear_iters += its;
+            if (running[s]) {
+                stats[s].linear_iters += its;
+                stats[s].linear_failures += lfail > 0;
+            }
         hipLaunchKernelGGL((axpy_kernel<double, double>), dim3(nblk(n)), dim3(256), 0, p->st, n, W->d64, x,
                            -o->relaxation, p->dof_sub, W->dof_mask);
         KCHECK();
@@ -980,6 +1208,7 @@ int pyhip_newton_solve(pyhip_problem *p, double *x, const pyhip_newton_opts *o,
                 stats[s].converged = 0;
                 any_failed = 1;
             }
+            if (!running[s]) stats[s].done_ms = elapsed_ms();
         }
     }
     if (any_failed) {

@@ -1,5 +1,6 @@
 // Internal definition of pyhip_problem (device-resident batched problem).
 #pragma once
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -30,6 +31,65 @@ struct Group {
 };
 
 struct KrylovWork;   // krylov.hip
+
+// HIP-event timing of kernel families on the problem's stream (measurement aid).
+struct KernelTimer {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::map<std::string, std::pair<int64_t, double>> total;
+    hipEvent_t take() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void flush() {
+        for (auto &kv : pending) {
+            auto &t = total[kv.first];
+            for (auto &pr : kv.second) {
+                float ms = 0.f;
+                if (hipEventSynchronize(pr.second) == hipSuccess &&
+                    hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+                    t.first++;
+                    t.second += ms;
+                }
+            }
+        }
+        pending.clear();
+        used = 0;
+    }
+    ~KernelTimer() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// Records start/end events around the launches made while it is alive.
+struct TimeScope {
+    KernelTimer *t;
+    const char *fam;
+    hipStream_t st;
+    hipEvent_t a = nullptr;
+    TimeScope(KernelTimer *timer, const char *family, hipStream_t s) : t(timer), fam(family), st(s) {
+        if (t && t->on) {
+            if (t->used + 2 > 4096) t->flush();
+            a = t->take();
+            if (a) (void)hipEventRecord(a, st);
+        }
+    }
+    ~TimeScope() {
+        if (a) {
+            hipEvent_t b = t->take();
+            if (b) {
+                (void)hipEventRecord(b, st);
+                t->pending[fam].push_back({a, b});
+            }
+        }
+    }
+};
 
 int launch_elements_f64(int kind, int form, const ElemArgs<double> &a, hipStream_t s);
 int launch_elements_f32(int kind, int form, const ElemArgs<float> &a, hipStream_t s);
@@ -99,4 +159,5 @@ struct pyhip_problem {
     // ILU(0) ordering: rows listed here go after all other rows of their sub-problem
     // (set for matrices that arrive without element information)
     std::vector<char> ilu_last_h;
+    pyhip::KernelTimer timer;
 };

@@ -28,6 +28,7 @@ class Equations:
     form = None
     nvalue = 1
     space = "C2"
+    symmetric = False   # Jacobian symmetric positive definite (conjugate gradients apply)
 
     def params(self):
         return np.zeros(0)
@@ -69,6 +70,7 @@ class GaussianSource:
 
 class PoissonEquation(Equations):
     form = _lib.FORM_POISSON
+    symmetric = True
 
     def __init__(self, name="u", *, space="C2", source=None, coefficient=1):
         if coefficient != 1:
@@ -85,6 +87,7 @@ class PoissonEquation(Equations):
 
 class BratuEquation(Equations):
     form = _lib.FORM_BRATU
+    symmetric = True   # lambda below the fold of the solution branch
     global_parameters = ("lambda",)
 
     def __init__(self, name="u", *, space="C2", lambda_=1.0):
@@ -96,6 +99,7 @@ class BratuEquation(Equations):
 
 class ReactionDiffusionEquation(Equations):
     form = _lib.FORM_REACTION_DIFFUSION
+    symmetric = True
 
     def __init__(self, name="u", *, space="C2", source_field=None):
         if source_field is None:

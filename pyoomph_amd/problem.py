@@ -95,9 +95,13 @@ class Problem:
         self.jacobian_precision = "f64"
         self.linear_precision = "f64"
         self.preconditioner = "ilu0"
+        # "auto": conjugate gradients where every sub-problem's Jacobian is symmetric
+        # positive definite (Poisson, Bratu, reaction-diffusion), GMRES otherwise
+        self.krylov = "auto"
         self.gmres_restart = 50
-        self.gmres_max_iters = 3000
+        self.gmres_max_iters = 5000
         self.gmres_rtol = 1e-10
+        self.rtol_f32 = 1e-5
         self.max_refinements = 10
         self.refine_rtol = 1e-10
         self.torch = torch
@@ -250,6 +254,11 @@ class Problem:
         o.jacobian_precision = _lib.F64 if self.jacobian_precision == "f64" else _lib.F32
         o.linear.precision = _lib.F64 if self.linear_precision == "f64" else _lib.F32
         o.linear.precond = _lib.PRECOND_ILU0 if self.preconditioner == "ilu0" else _lib.PRECOND_JACOBI
+        kry = self.krylov
+        if kry == "auto":
+            kry = "cg" if all(getattr(mp.equations, "symmetric", False) for mp in self.parts) else "gmres"
+        o.linear.krylov = _lib.KRYLOV_CG if kry == "cg" else _lib.KRYLOV_GMRES
+        o.linear.rtol_f32 = self.rtol_f32
         o.linear.restart = self.gmres_restart
         o.linear.max_iters = self.gmres_max_iters
         o.linear.rtol = self.gmres_rtol
@@ -264,6 +273,7 @@ class Problem:
         rc = _lib.load().pyhip_newton_solve(self._handle, _lib.ptr(self.dofs), ctypes.byref(o), stats)
         self.torch.cuda.synchronize(self.device)
         out = [{"newton_iters": s.newton_iters, "linear_iters": s.linear_iters,
+                "linear_failures": s.linear_failures,
                 "converged": bool(s.converged), "final_maxres": s.final_maxres,
                 "history": list(s.history[:s.nhistory])} for s in stats]
         self.last_stats = out
