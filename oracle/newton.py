@@ -44,8 +44,13 @@ def max_abs(v):
 
 
 def superlu_solve(n, row_start, col, val, rhs):
-    """scipy.py:67 + :96 with transpose=1: splu of the CSR read as CSC, solve with "T"."""
-    A = scipy.sparse.csc_matrix((val, col, row_start), shape=(n, n))
+    """scipy.py:67 + :96 with transpose=1: splu of the CSR read as CSC, solve with "T".
+
+    The arrays are copied: scipy sorts a CSC matrix's indices in place, which would
+    reorder the caller's values relative to its column indices."""
+    A = scipy.sparse.csc_matrix((np.array(val, dtype=np.float64, copy=True),
+                                 np.array(col, dtype=np.int32, copy=True),
+                                 np.array(row_start, dtype=np.int32, copy=True)), shape=(n, n))
     lu = scipy.sparse.linalg.splu(A)
     return lu.solve(np.asarray(rhs, dtype=np.float64), "T")
 

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "krylov.h"
@@ -372,6 +373,12 @@ __global__ __launch_bounds__(256) void permute_kernel(int64_t n, const int32_t *
 
 static inline unsigned nblk(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
+static bool debug_enabled() {
+    static int v = -1;
+    if (v < 0) v = getenv("PYHIP_DEBUG") ? 1 : 0;
+    return v == 1;
+}
+
 #define KCHECK() PYHIP_CHECK_HIP(hipGetLastError())
 
 template <typename T>
@@ -721,6 +728,13 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
         KCHECK();
         PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
         PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+        if (debug_enabled()) {
+            double b0 = 0, bn = 0;
+            (void)hipMemcpy(&b0, W->beta0, sizeof(double), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(&bn, W->bnorm, sizeof(double), hipMemcpyDeviceToHost);
+            fprintf(stderr, "[pyhip] gmres cycle %d: active %d |r| %.3e |b| %.3e iters %d\n", cyc,
+                    *W->nactive_h, b0, bn, total);
+        }
         if (*W->nactive_h == 0) break;
         hipLaunchKernelGGL((scale_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, V, d3, p->dof_sub,
                            W->active, V);

@@ -5,9 +5,11 @@
 // arrays of J^T with transpose = 1 (linear_solver.cc:2253-2311): colptr is J's row
 // start, rowind J's column index.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
+#include "plugin_csr.h"
 #include "problem.h"
 
 namespace pyhip {
@@ -54,38 +56,24 @@ extern "C" int pyhip_solve_serial(int op_flag, int n, int nnz, int nrhs, const d
             delete H;
             *handle = nullptr;
         }
-        // CSR of the matrix to solve with
-        std::vector<int32_t> rs(n + 1, 0), cl(nnz);
-        std::vector<int64_t> src(nnz);
-        if (transpose) {
-            // CSC of J^T == CSR of J
-            for (int i = 0; i <= n; i++) rs[i] = colptr[i];
-            for (int i = 0; i < n; i++) {
-                std::vector<std::pair<int32_t, int64_t>> row;
-                for (int32_t k = colptr[i]; k < colptr[i + 1]; k++) row.push_back({rowind[k], k});
-                std::sort(row.begin(), row.end());
-                for (size_t t = 0; t < row.size(); t++) {
-                    cl[rs[i] + t] = row[t].first;
-                    src[rs[i] + t] = row[t].second;
-                }
-            }
-        } else {
-            // CSC of A: transpose the structure (row r of A = column entries with rowind r)
-            for (int k = 0; k < nnz; k++) rs[rowind[k] + 1]++;
-            for (int i = 0; i < n; i++) rs[i + 1] += rs[i];
-            std::vector<int32_t> fill(rs.begin(), rs.end() - 1);
-            for (int c = 0; c < n; c++)
-                for (int32_t k = colptr[c]; k < colptr[c + 1]; k++) {
-                    const int32_t r = rowind[k];
-                    cl[fill[r]] = c;
-                    src[fill[r]++] = k;
-                }
+        // CSR of the matrix to solve with; rows are column-sorted and a structural zero
+        // diagonal is added where the caller's CSR has none (the reference drops exact
+        // zeros, e.g. the pressure-pressure block of Taylor-Hood), so ILU(0) can fill it
+        std::vector<int32_t> rs, cl;
+        std::vector<int64_t> src;
+        std::vector<char> last;
+        std::string err;
+        if (plugin_build_csr(n, nnz, values, rowind, colptr, transpose, rs, cl, src, last, err)) {
+            set_error(err);
+            return PYHIP_ERR_INVALID;
         }
-        for (int k = 0; k < nnz; k++) PYHIP_REQUIRE(cl[k] >= 0 && cl[k] < n, "index out of range");
+        nnz = rs[n];
         H = new SolveHandle();
         H->n = n;
         H->src = src;
         pyhip_linear_opts_default(&H->o);
+        if (const char *pc = getenv("PYHIP_SOLVE_SERIAL_PRECOND"))
+            H->o.precond = (std::string(pc) == "jacobi") ? PYHIP_PRECOND_JACOBI : PYHIP_PRECOND_ILU0;
         pyhip_problem *P = new pyhip_problem();
         H->p = P;
         P->nsub = 1;
@@ -95,18 +83,9 @@ extern "C" int pyhip_solve_serial(int op_flag, int n, int nnz, int nrhs, const d
         P->row_start_h = rs;
         P->col_h = cl;
         std::vector<int32_t> diag(n, -1), dsub(n, 0);
-        std::vector<char> last(n, 0);
-        for (int i = 0; i < n; i++) {
+        for (int i = 0; i < n; i++)
             for (int32_t k = rs[i]; k < rs[i + 1]; k++)
                 if (cl[k] == i) diag[i] = k;
-            if (diag[i] < 0) {
-                delete H;
-                pyhip_problem_destroy(P);
-                set_error("matrix has a row without a diagonal entry");
-                return PYHIP_ERR_BREAKDOWN;
-            }
-            last[i] = values[src[diag[i]]] == 0.0;
-        }
         P->ilu_last_h = last;
         int64_t &B = P->device_bytes;
         int rc;
@@ -126,7 +105,7 @@ extern "C" int pyhip_solve_serial(int op_flag, int n, int nnz, int nrhs, const d
         PYHIP_CHECK_HIP(hipMemcpy(P->dof_off, off.data(), 2 * 8, hipMemcpyHostToDevice));
         if ((rc = plugin_setup(P, H->o))) return rc;
         std::vector<double> v(nnz);
-        for (int k = 0; k < nnz; k++) v[k] = values[src[k]];
+        for (int k = 0; k < nnz; k++) v[k] = src[k] >= 0 ? values[src[k]] : 0.0;
         PYHIP_CHECK_HIP(hipMemcpy(plugin_values(P), v.data(), (size_t)nnz * 8, hipMemcpyHostToDevice));
         *handle = H;
         return plugin_factor(P, H->o);

@@ -84,3 +84,11 @@ def test_superlu_transpose_convention():
     b = rng.normal(size=5)
     x = on.superlu_solve(5, S.indptr, S.indices, S.data, b)
     assert np.allclose(A @ x, b)
+
+
+def test_superlu_solve_leaves_inputs_untouched():
+    op = oracle_problem("poisson", N=3)
+    r, rs, col, val = op.get_jacobian(np.zeros(op.ndof))
+    c0, v0 = col.copy(), val.copy()
+    on.superlu_solve(op.ndof, rs, col, val, r)
+    assert np.array_equal(col, c0) and np.array_equal(val, v0)
