@@ -14,10 +14,10 @@
 // Mapping (one workgroup = a block of elements, 64-wide waves):
 //   thread = (element, slot row i); EPB = 256 / NS elements per workgroup.
 //   Per element the nodal coordinates, slot values, equations and auxiliary field are
-//   staged once in LDS (coalesced slot-major gathers), so the NS threads of an element
-//   read them as LDS broadcasts. Per integration point every thread forms the 2x2/3x3
-//   metric (cheap, redundant), the thread of node l writes dx_shape[l] to LDS, and every
-//   thread then runs its Jacobian row over the columns reading dx_shape from LDS.
+//   staged once in LDS (coalesced slot-major gathers). Per integration point, one thread
+//   computes the geometry (tangents, metric inverse, w*det, dx_shape of every node) and
+//   the interpolated fields once into LDS; then every thread runs its residual entry and
+//   Jacobian row over the points, reading them as LDS broadcasts.
 //   Reference-element tables (weights, psi, dpsi/ds) are read with uniform indices, so
 //   they come through the scalar cache.
 // Output: element vectors Er[e][i] and matrices Ej[e][i][j] in slot numbering; the
@@ -131,17 +131,60 @@ __device__ __forceinline__ T dotv(const T *a, const T *b) {
     return s;
 }
 
+// Per-integration-point geometry and field values, computed once per (element, point)
+// by one thread and shared through LDS: tangents, metric inverse, w*det, dx_shape of
+// every node, and the interpolated fields. Same operation order as the CPU restatement
+// of the generated code (fill_shape_info_at_s, then the interpolation loops).
+template <typename T, int NN, int DIM>
+__device__ __forceinline__ T point_geometry(const T (*X)[DIM], const T *__restrict__ D, T wq,
+                                            T (*dx)[DIM]) {
+    T t[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+        for (int j = 0; j < DIM; j++) t[i][j] = T(0);
+#pragma unroll
+    for (int l = 0; l < NN; l++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++)
+#pragma unroll
+            for (int j = 0; j < DIM; j++) t[j][i] = t[j][i] + X[l][i] * D[l * DIM + j];
+    T g[DIM][DIM];
+    const T W = metric<T, DIM>(t, wq, g);
+#pragma unroll
+    for (int l = 0; l < NN; l++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            T s = T(0);
+#pragma unroll
+            for (int b = 0; b < DIM; b++) s = s + g[b][i] * D[l * DIM + b];
+            dx[l][i] = s;
+        }
+    return W;
+}
+
 // Scalar weak forms (Poisson / Bratu / reaction-diffusion): slot = node.
+// Phase A (per chunk of integration points): thread t of an element computes points
+// q = t, t + NS, ... of the chunk: geometry, dx_shape of all nodes, u, grad u and the
+// source term and its derivative. Phase B: thread = test row i, accumulates its residual
+// and Jacobian row over the chunk's points in increasing point order.
+template <int KIND> struct ScalarChunk { static constexpr int QC = ElemTraits<KIND>::NQ; };
+template <> struct ScalarChunk<ELEM_HEX_Q2> { static constexpr int QC = 9; };
+
 template <typename T, int KIND, int FORM>
 __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
     using Tr = ElemTraits<KIND>;
     constexpr int NN = Tr::NN, DIM = Tr::DIM, NQ = Tr::NQ, NS = NN;
     constexpr int EPB = 256 / NS;
+    constexpr int QC = ScalarChunk<KIND>::QC;
+    constexpr int NQV = 2 + DIM;   // per point: src, dsrc, grad u
     __shared__ T sX[EPB][NN][DIM];
     __shared__ T sU[EPB][NN];
-    __shared__ T sF[EPB][NN];
-    __shared__ T sDx[EPB][NN][DIM];
+    __shared__ T sF[FORM == PYHIP_FORM_REACTION_DIFFUSION ? EPB : 1][NN];
     __shared__ int sEq[EPB][NS];
+    __shared__ T sW[EPB][QC];
+    __shared__ T sDx[EPB][QC][NN][DIM];
+    __shared__ T sQ[EPB][QC][NQV];
 
     const int le = threadIdx.x / NS;        // element within block
     const int row = threadIdx.x % NS;       // slot row of this thread
@@ -171,88 +214,79 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
 #pragma unroll
     for (int j = 0; j < NS; j++) Jacc[j] = T(0);
 
-    for (int q = 0; q < NQ; q++) {
-        const T *D = a.tab->dpsi + q * NN * DIM;
-        const T *P = a.tab->psi + q * NN;
-        T W = T(0);
-        T g[DIM][DIM];
+    for (int q0 = 0; q0 < NQ; q0 += QC) {
+        // ---- phase A: per-point quantities
         if (active) {
-            T t[DIM][DIM];
+            for (int qq = row; qq < QC && q0 + qq < NQ; qq += NS) {
+                const int q = q0 + qq;
+                const T *D = a.tab->dpsi + q * NN * DIM;
+                const T *P = a.tab->psi + q * NN;
+                const T W = point_geometry<T, NN, DIM>(sX[le], D, a.tab->w[q], sDx[le][qq]);
+                sW[le][qq] = W;
+                T u = T(0);
 #pragma unroll
-            for (int i = 0; i < DIM; i++)
+                for (int l = 0; l < NN; l++) u = u + sU[le][l] * P[l];
 #pragma unroll
-                for (int j = 0; j < DIM; j++) t[i][j] = T(0);
+                for (int i = 0; i < DIM; i++) {
+                    T sacc = T(0);
 #pragma unroll
-            for (int l = 0; l < NN; l++)
+                    for (int l = 0; l < NN; l++) sacc = sacc + sU[le][l] * sDx[le][qq][l][i];
+                    sQ[le][qq][2 + i] = sacc;
+                }
+                T src = T(0), dsrc = T(0);
+                if constexpr (FORM == PYHIP_FORM_POISSON) {
+                    T x = T(0), y = T(0);
 #pragma unroll
-                for (int i = 0; i < DIM; i++)
+                    for (int l = 0; l < NN; l++) x = x + sX[le][l][0] * P[l];
 #pragma unroll
-                    for (int j = 0; j < DIM; j++)
-                        t[j][i] = t[j][i] + sX[le][l][i] * D[l * DIM + j];
-            W = metric<T, DIM>(t, a.tab->w[q], g);
+                    for (int l = 0; l < NN; l++) y = y + sX[le][l][1] * P[l];
+                    T f = T(0);
+                    const int ng = (int)par[0];  // Gaussians follow as (cx, cy, w, a)
+                    for (int k = 1; k < 1 + 4 * ng; k += 4) {
+                        T ddx = x - T(par[k + 0]), ddy = y - T(par[k + 1]);
+                        T r2 = ddx * ddx + ddy * ddy;
+                        T wk = T(par[k + 2]);
+                        f = f + T(par[k + 3]) * texp(-r2 / (wk * wk));
+                    }
+                    src = -f;
+                } else if constexpr (FORM == PYHIP_FORM_BRATU) {
+                    T ev = texp(u);
+                    if (a.deriv) {
+                        src = -ev;
+                        dsrc = -ev;
+                    } else {
+                        src = -par0 * ev;
+                        dsrc = -par0 * ev;
+                    }
+                } else {
+                    T f = T(0);
 #pragma unroll
-            for (int i = 0; i < DIM; i++) {
-                T s = T(0);
-#pragma unroll
-                for (int b = 0; b < DIM; b++) s = s + g[b][i] * D[row * DIM + b];
-                sDx[le][row][i] = s;
+                    for (int l = 0; l < NN; l++) f = f + sF[le][l] * P[l];
+                    src = u * u * u - f;
+                    dsrc = T(3) * (u * u);
+                }
+                sQ[le][qq][0] = src;
+                sQ[le][qq][1] = dsrc;
             }
         }
         __syncthreads();
-        if (active) {
-            T u = T(0), gu[DIM];
-#pragma unroll
-            for (int l = 0; l < NN; l++) u = u + sU[le][l] * P[l];
-#pragma unroll
-            for (int i = 0; i < DIM; i++) {
-                T s = T(0);
-#pragma unroll
-                for (int l = 0; l < NN; l++) s = s + sU[le][l] * sDx[le][l][i];
-                gu[i] = s;
-            }
-            T src = T(0), dsrc = T(0);
-            if constexpr (FORM == PYHIP_FORM_POISSON) {
-                T x = T(0), y = T(0);
-#pragma unroll
-                for (int l = 0; l < NN; l++) x = x + sX[le][l][0] * P[l];
-#pragma unroll
-                for (int l = 0; l < NN; l++) y = y + sX[le][l][1] * P[l];
-                T f = T(0);
-                const int ng = (int)par[0];  // Gaussians follow as (cx, cy, w, a)
-                for (int k = 1; k < 1 + 4 * ng; k += 4) {
-                    T ddx = x - T(par[k + 0]), ddy = y - T(par[k + 1]);
-                    T r2 = ddx * ddx + ddy * ddy;
-                    T wk = T(par[k + 2]);
-                    f = f + T(par[k + 3]) * texp(-r2 / (wk * wk));
-                }
-                src = -f;
-            } else if constexpr (FORM == PYHIP_FORM_BRATU) {
-                T ev = texp(u);
-                if (a.deriv) {
-                    src = -ev;
-                    dsrc = -ev;
-                } else {
-                    src = -par0 * ev;
-                    dsrc = -par0 * ev;
-                }
-            } else {
-                T f = T(0);
-#pragma unroll
-                for (int l = 0; l < NN; l++) f = f + sF[le][l] * P[l];
-                src = u * u * u - f;
-                dsrc = T(3) * (u * u);
-            }
-            if (myrow) {
-                const T *dxi = sDx[le][row];
+        // ---- phase B: this thread's residual entry and Jacobian row
+        if (myrow) {
+            for (int qq = 0; qq < QC && q0 + qq < NQ; qq++) {
+                const int q = q0 + qq;
+                const T *P = a.tab->psi + q * NN;
+                const T W = sW[le][qq];
+                const T *dxi = sDx[le][qq][row];
+                const T src = sQ[le][qq][0], dsrc = sQ[le][qq][1];
                 if (a.deriv)
                     Racc = Racc + W * (src * P[row]);
                 else
-                    Racc = Racc + W * (dotv<T, DIM>(gu, dxi) + src * P[row]);
+                    Racc = Racc + W * (dotv<T, DIM>(&sQ[le][qq][2], dxi) + src * P[row]);
                 if (a.flag) {
 #pragma unroll
                     for (int j = 0; j < NS; j++) {
                         if (sEq[le][j] < 0) continue;
-                        const T *dxj = sDx[le][j];
+                        const T *dxj = sDx[le][qq][j];
                         T v;
                         if (a.deriv)
                             v = W * ((dsrc * P[j]) * P[row]);
@@ -265,7 +299,7 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
                 }
             }
         }
-        __syncthreads();
+        if (q0 + QC < NQ) __syncthreads();
     }
     if (myrow) {
         a.Er[(int64_t)e * NS + row] = Racc;
@@ -277,15 +311,19 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
     }
 }
 
-// Taylor-Hood Q2/Q1 steady Navier-Stokes (22 slots, 9 geometric nodes).
+// Taylor-Hood Q2/Q1 steady Navier-Stokes (22 slots, 9 geometric nodes). Phase A: thread
+// t < 9 of an element computes integration point t (geometry, dx_shape, u, grad u, p);
+// phase B: thread = slot row.
 template <typename T>
 __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
     constexpr int NN = 9, DIM = 2, NQ = 9, NS = 22;
     constexpr int EPB = 256 / NS;  // 11 elements, 242 threads
     __shared__ T sX[EPB][NN][DIM];
     __shared__ T sV[EPB][NS];
-    __shared__ T sDx[EPB][NN][DIM];
     __shared__ int sEq[EPB][NS];
+    __shared__ T sW[EPB][NQ];
+    __shared__ T sDx[EPB][NQ][NN][DIM];
+    __shared__ T sQ[EPB][NQ][7];   // u0, u1, du00, du01, du10, du11, p
 
     const int le = threadIdx.x / NS;
     const int slot = threadIdx.x % NS;
@@ -311,123 +349,104 @@ __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
         mu = T(par[0]);
         rho = T(par[1]);
     }
+    if (active && slot < NQ) {
+        const int q = slot;
+        const T *D = a.tab->dpsi + q * NN * DIM;
+        const T *P = a.tab->psi + q * NN;
+        const T *C = a.tab->psi1 + q * 4;
+        sW[le][q] = point_geometry<T, NN, DIM>(sX[le], D, a.tab->w[q], sDx[le][q]);
+        T u0 = T(0), u1 = T(0), p = T(0);
+#pragma unroll
+        for (int l = 0; l < NN; l++) u0 = u0 + sV[le][kTHNodeSlot[l]] * P[l];
+#pragma unroll
+        for (int l = 0; l < NN; l++) u1 = u1 + sV[le][1 + kTHNodeSlot[l]] * P[l];
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            T s0 = T(0), s1 = T(0);
+#pragma unroll
+            for (int l = 0; l < NN; l++) s0 = s0 + sV[le][kTHNodeSlot[l]] * sDx[le][q][l][b];
+#pragma unroll
+            for (int l = 0; l < NN; l++) s1 = s1 + sV[le][kTHNodeSlot[l] + 1] * sDx[le][q][l][b];
+            sQ[le][q][2 + b] = s0;       // du[0][b]
+            sQ[le][q][4 + b] = s1;       // du[1][b]
+        }
+        // pressure slots 2, 7, 16, 21 hold corner nodes 0, 2, 6, 8
+        p = p + sV[le][2] * C[0];
+        p = p + sV[le][7] * C[1];
+        p = p + sV[le][16] * C[2];
+        p = p + sV[le][21] * C[3];
+        sQ[le][q][0] = u0;
+        sQ[le][q][1] = u1;
+        sQ[le][q][6] = p;
+    }
+    __syncthreads();
     const bool myrow = active && sEq[le][slot] >= 0;
+    if (!myrow) return;
     const int kt = kTHSlotKind[slot], lt = kTHSlotNode[slot];
     T Racc = T(0);
     T Jacc[NS];
 #pragma unroll
     for (int j = 0; j < NS; j++) Jacc[j] = T(0);
-
     for (int q = 0; q < NQ; q++) {
-        const T *D = a.tab->dpsi + q * NN * DIM;
         const T *P = a.tab->psi + q * NN;
         const T *C = a.tab->psi1 + q * 4;
-        T W = T(0);
-        if (active) {
-            T t[DIM][DIM] = {{T(0), T(0)}, {T(0), T(0)}};
+        const T W = sW[le][q];
+        const T u0 = sQ[le][q][0], u1 = sQ[le][q][1], p = sQ[le][q][6];
+        T du[2][2];
+        du[0][0] = sQ[le][q][2];
+        du[0][1] = sQ[le][q][3];
+        du[1][0] = sQ[le][q][4];
+        du[1][1] = sQ[le][q][5];
+        T sig[2][2];
 #pragma unroll
-            for (int l = 0; l < NN; l++)
-#pragma unroll
-                for (int i = 0; i < DIM; i++)
-#pragma unroll
-                    for (int j = 0; j < DIM; j++)
-                        t[j][i] = t[j][i] + sX[le][l][i] * D[l * DIM + j];
-            T g[DIM][DIM];
-            W = metric<T, DIM>(t, a.tab->w[q], g);
-            if (slot < NN) {
-#pragma unroll
-                for (int i = 0; i < DIM; i++) {
-                    T s = T(0);
-#pragma unroll
-                    for (int b = 0; b < DIM; b++) s = s + g[b][i] * D[slot * DIM + b];
-                    sDx[le][slot][i] = s;
-                }
-            }
-        }
-        __syncthreads();
-        if (myrow) {
-            // nodal values by (node, component) from the slot table
-            T u0 = T(0), u1 = T(0), p = T(0);
-            T du[2][2];
-#pragma unroll
-            for (int l = 0; l < NN; l++) u0 = u0 + sV[le][kTHNodeSlot[l]] * P[l];
-#pragma unroll
-            for (int l = 0; l < NN; l++) u1 = u1 + sV[le][1 + kTHNodeSlot[l]] * P[l];
+        for (int aa = 0; aa < 2; aa++)
 #pragma unroll
             for (int b = 0; b < 2; b++) {
-                T s0 = T(0), s1 = T(0);
-#pragma unroll
-                for (int l = 0; l < NN; l++) {
-                    const int sx = kTHNodeSlot[l];
-                    s0 = s0 + sV[le][sx] * sDx[le][l][b];
-                }
-#pragma unroll
-                for (int l = 0; l < NN; l++) {
-                    const int sx = kTHNodeSlot[l];
-                    s1 = s1 + sV[le][sx + 1] * sDx[le][l][b];
-                }
-                du[0][b] = s0;
-                du[1][b] = s1;
+                T s = mu * (du[aa][b] + du[b][aa]);
+                if (aa == b) s = s - p;
+                sig[aa][b] = s;
             }
-            // pressure slots 2, 7, 16, 21 hold corner nodes 0, 2, 6, 8
-            p = p + sV[le][2] * C[0];
-            p = p + sV[le][7] * C[1];
-            p = p + sV[le][16] * C[2];
-            p = p + sV[le][21] * C[3];
-            T sig[2][2];
-#pragma unroll
-            for (int aa = 0; aa < 2; aa++)
-#pragma unroll
-                for (int b = 0; b < 2; b++) {
-                    T s = mu * (du[aa][b] + du[b][aa]);
-                    if (aa == b) s = s - p;
-                    sig[aa][b] = s;
-                }
-            const T *dxl = sDx[le][lt];
-            if (kt < 2) {
-                const T conv = rho * (u0 * du[kt][0] + u1 * du[kt][1]);
-                Racc = Racc + W * ((sig[kt][0] * dxl[0] + sig[kt][1] * dxl[1]) + conv * P[lt]);
-            } else {
-                const int pidx = kTHSlotP[slot];
-                Racc = Racc + W * ((du[0][0] + du[1][1]) * C[pidx]);
-            }
-            if (a.flag) {
-#pragma unroll
-                for (int ss = 0; ss < NS; ss++) {
-                    if (sEq[le][ss] < 0) continue;
-                    const int ks = kTHSlotKind[ss], ls = kTHSlotNode[ss];
-                    T v;
-                    if (kt < 2 && ks < 2) {
-                        const T *dxs = sDx[le][ls];
-                        T visc = dxs[kt] * dxl[ks];
-                        T cv = P[ls] * du[kt][ks];
-                        if (kt == ks) {
-                            visc = (dxs[0] * dxl[0] + dxs[1] * dxl[1]) + visc;
-                            cv = cv + (u0 * dxs[0] + u1 * dxs[1]);
-                        }
-                        v = W * (mu * visc + (rho * cv) * P[lt]);
-                    } else if (kt < 2 && ks == 2) {
-                        const int pidx = kTHSlotP[ss];
-                        v = W * (-(C[pidx] * dxl[kt]));
-                    } else if (kt == 2 && ks < 2) {
-                        const int pidx = kTHSlotP[slot];
-                        v = W * (sDx[le][ls][ks] * C[pidx]);
-                    } else {
-                        continue;
-                    }
-                    Jacc[ss] = Jacc[ss] + v;
-                }
-            }
+        const T *dxl = sDx[le][q][lt];
+        if (kt < 2) {
+            const T conv = rho * (u0 * du[kt][0] + u1 * du[kt][1]);
+            Racc = Racc + W * ((sig[kt][0] * dxl[0] + sig[kt][1] * dxl[1]) + conv * P[lt]);
+        } else {
+            const int pidx = kTHSlotP[slot];
+            Racc = Racc + W * ((du[0][0] + du[1][1]) * C[pidx]);
         }
-        __syncthreads();
-    }
-    if (myrow) {
-        a.Er[(int64_t)e * NS + slot] = Racc;
         if (a.flag) {
-            T *dst = a.Ej + ((int64_t)e * NS + slot) * NS;
 #pragma unroll
-            for (int j = 0; j < NS; j++) dst[j] = Jacc[j];
+            for (int ss = 0; ss < NS; ss++) {
+                if (sEq[le][ss] < 0) continue;
+                const int ks = kTHSlotKind[ss], ls = kTHSlotNode[ss];
+                T v;
+                if (kt < 2 && ks < 2) {
+                    const T *dxs = sDx[le][q][ls];
+                    T visc = dxs[kt] * dxl[ks];
+                    T cv = P[ls] * du[kt][ks];
+                    if (kt == ks) {
+                        visc = (dxs[0] * dxl[0] + dxs[1] * dxl[1]) + visc;
+                        cv = cv + (u0 * dxs[0] + u1 * dxs[1]);
+                    }
+                    v = W * (mu * visc + (rho * cv) * P[lt]);
+                } else if (kt < 2 && ks == 2) {
+                    const int pidx = kTHSlotP[ss];
+                    v = W * (-(C[pidx] * dxl[kt]));
+                } else if (kt == 2 && ks < 2) {
+                    const int pidx = kTHSlotP[slot];
+                    v = W * (sDx[le][q][ls][ks] * C[pidx]);
+                } else {
+                    continue;
+                }
+                Jacc[ss] = Jacc[ss] + v;
+            }
         }
+    }
+    a.Er[(int64_t)e * NS + slot] = Racc;
+    if (a.flag) {
+        T *dst = a.Ej + ((int64_t)e * NS + slot) * NS;
+#pragma unroll
+        for (int j = 0; j < NS; j++) dst[j] = Jacc[j];
     }
 }
 

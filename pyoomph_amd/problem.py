@@ -275,7 +275,7 @@ class Problem:
         out = [{"newton_iters": s.newton_iters, "linear_iters": s.linear_iters,
                 "linear_failures": s.linear_failures,
                 "converged": bool(s.converged), "final_maxres": s.final_maxres,
-                "history": list(s.history[:s.nhistory])} for s in stats]
+                "history": list(s.history[:s.nhistory]), "done_ms": s.done_ms} for s in stats]
         self.last_stats = out
         if rc == _lib.PYHIP_ERR_NEWTON:
             if raise_on_failure:
