@@ -195,9 +195,13 @@ int pyhip_newton_solve(pyhip_problem *p, double *dofs_dev, const pyhip_newton_op
  * stream around every launch of the named kernel families, "element" (element
  * residual / Jacobian kernels), "gather_values", "gather_residual", "spmv",
  * "precond" (Jacobi / ILU(0) apply and factorisation), "vector" (dots, updates).
- * pyhip_timing_read returns the launches and summed device milliseconds so far. */
+ * pyhip_timing_read returns the launches and summed device milliseconds so far;
+ * pyhip_timing_bytes the algorithmic bytes those launches moved, where the family counts
+ * them ("spmv": matrix values and column indices once, row starts, the input vector once
+ * per row, the output once per row, over the rows each launch actually processed). */
 int pyhip_timing_enable(pyhip_problem *p, int enable);
 int pyhip_timing_read(pyhip_problem *p, const char *family, int64_t *launches, double *total_ms);
+int pyhip_timing_bytes(pyhip_problem *p, const char *family, int64_t *bytes);
 
 /* ---- linear-solver plugin entry --------------------------------------------------------------
  * Same arguments and meaning as GenericLinearSystemSolver.solve_serial

@@ -96,6 +96,7 @@ SIGNATURES = {
     "pyhip_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pyhip_timing_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, c_i64p,
                                          ctypes.POINTER(ctypes.c_double)]),
+    "pyhip_timing_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, c_i64p]),
     "pyhip_solve_serial": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           c_dp, c_i32p, c_i32p, c_dp, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_void_p)]),

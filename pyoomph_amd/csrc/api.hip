@@ -264,8 +264,8 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
     UP(P->diag, hp.diag_pos.data(), hp.diag_pos.size());
     UP(P->occ_ptr, hp.occ_ptr.data(), hp.occ_ptr.size());
     UP(P->occ_roff, hp.occ_roff.data(), hp.occ_roff.size());
-    UP(P->occ_eoff, hp.occ_eoff.data(), hp.occ_eoff.size());
-    UP(P->occ_pos, hp.occ_pos.data(), hp.occ_pos.size());
+    UP(P->cptr, hp.cptr.data(), hp.cptr.size());
+    UP(P->cidx, hp.cidx.data(), hp.cidx.size());
     UP(P->dof_sub, dof_sub_h.data(), dof_sub_h.size());
     UP(P->dof_off, P->dof_off_h.data(), P->dof_off_h.size());
 #undef UP
@@ -286,7 +286,8 @@ int pyhip_problem_destroy(pyhip_problem *p) {
         dfree(G.sub);
     }
     void *ptrs[] = {p->coords, p->pinned, p->aux, p->params, p->row_start, p->col, p->diag,
-                    p->occ_ptr, p->occ_roff, p->occ_eoff, p->occ_pos, p->dof_sub, p->dof_off,
+                    p->occ_ptr, p->occ_roff, p->cptr, p->cidx,
+                    p->dof_sub, p->dof_off,
                     p->Er64, p->Ej64, p->Er32, p->Ej32, p->keys, p->kept};
     for (void *q : ptrs)
         dfree(q);
@@ -386,27 +387,26 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             if ((rc = launch_elements_f32(G.kind, G.form, a, p->st))) return rc;
         }
     }
+    const GatherMap m = p->gmap();
     if (precision == PYHIP_F64) {
-        {
+        if (flag) {
+            TimeScope ts(&p->timer, "gather_values", p->st);
+            if ((rc = launch_gather_values_f64(m, p->Ej64, p->Er64, (double *)values, residual, p->st)))
+                return rc;
+            p->have_ej64 = (deriv_param < 0);
+        } else {
             TimeScope ts(&p->timer, "gather_residual", p->st);
             if ((rc = launch_gather_residual_f64(p->ndof, p->occ_ptr, p->occ_roff, p->Er64, residual, p->st)))
                 return rc;
         }
+    } else {
         if (flag) {
             TimeScope ts(&p->timer, "gather_values", p->st);
-            if ((rc = launch_gather_values_f64(p->ndof, p->row_start, p->occ_ptr, p->occ_eoff,
-                                               p->occ_pos, p->Ej64, (double *)values, p->st)))
+            if ((rc = launch_gather_values_f32(m, p->Ej32, p->Er32, (float *)values, residual, p->st)))
                 return rc;
-            p->have_ej64 = (deriv_param < 0);
-        }
-    } else {
-        TimeScope ts(&p->timer, "gather_residual", p->st);
-        if ((rc = launch_gather_residual_f32(p->ndof, p->occ_ptr, p->occ_roff, p->Er32, residual, p->st)))
-            return rc;
-        if (flag) {
-            TimeScope ts2(&p->timer, "gather_values", p->st);
-            if ((rc = launch_gather_values_f32(p->ndof, p->row_start, p->occ_ptr, p->occ_eoff,
-                                               p->occ_pos, p->Ej32, (float *)values, p->st)))
+        } else {
+            TimeScope ts(&p->timer, "gather_residual", p->st);
+            if ((rc = launch_gather_residual_f32(p->ndof, p->occ_ptr, p->occ_roff, p->Er32, residual, p->st)))
                 return rc;
         }
     }
@@ -422,9 +422,8 @@ int pyhip_export_reference_csr(pyhip_problem *p, const double *values, int32_t *
         if ((rc = dev_alloc(&p->keys, p->nnz, p->device_bytes))) return rc;
         if ((rc = dev_alloc(&p->kept, p->ndof + 1, p->device_bytes))) return rc;
     }
-    return export_reference_csr(p->ndof, p->row_start, p->col, p->occ_ptr, p->occ_eoff,
-                                p->occ_pos, p->Ej64, values, p->keys, p->kept, rs_out, col_out,
-                                val_out, nnz_out, p->st);
+    return export_reference_csr(p->gmap(), p->col, p->Ej64, values, p->keys, p->kept, rs_out,
+                                col_out, val_out, nnz_out, p->st);
 }
 
 int pyhip_timing_enable(pyhip_problem *p, int enable) {
@@ -432,6 +431,7 @@ int pyhip_timing_enable(pyhip_problem *p, int enable) {
     PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
     p->timer.flush();
     p->timer.total.clear();
+    p->timer.work.clear();
     p->timer.on = enable != 0;
     return PYHIP_OK;
 }
@@ -443,6 +443,13 @@ int pyhip_timing_read(pyhip_problem *p, const char *family, int64_t *launches, d
     auto it = p->timer.total.find(family);
     *launches = it == p->timer.total.end() ? 0 : it->second.first;
     *total_ms = it == p->timer.total.end() ? 0.0 : it->second.second;
+    return PYHIP_OK;
+}
+
+int pyhip_timing_bytes(pyhip_problem *p, const char *family, int64_t *bytes) {
+    PYHIP_REQUIRE(p && family && bytes, "null argument");
+    auto it = p->timer.work.find(family);
+    *bytes = it == p->timer.work.end() ? 0 : it->second;
     return PYHIP_OK;
 }
 

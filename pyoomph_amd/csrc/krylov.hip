@@ -31,20 +31,56 @@ namespace pyhip {
 // ------------------------------------------------------------------------------------
 // kernels
 
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Row sum of val[k] * x[col[k]] over k in [k0, k1) in one canonical order: stripe t
+// (t = 0..7) accumulates the entries k0+t, k0+t+8, ... with fused multiply-adds, and the
+// stripes combine as ((S0+S4)+(S2+S6)) + ((S1+S5)+(S3+S7)). LPR lanes per row (1, 2, 4 or
+// 8; lane l owns stripes l, l+LPR, ...) all give the same bits, so the lane count can be
+// tuned per problem without changing any result. All LPR lanes of a row call this.
+template <typename TV, typename TX, typename TY, int LPR>
+__device__ __forceinline__ TY row_dot(int32_t k0, int32_t k1, int lane, const int32_t *__restrict__ col,
+                                      const TV *__restrict__ val, const TX *__restrict__ x) {
+    constexpr int NS = 8 / LPR;
+    TY acc[NS];
+#pragma unroll
+    for (int j = 0; j < NS; j++) acc[j] = TY(0);
+    for (int32_t b = k0 + lane; b < k1; b += 8) {
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+            const int32_t k = b + j * LPR;
+            if (k < k1) acc[j] = fmadd(TY(val[k]), TY(x[col[k]]), acc[j]);
+        }
+    }
+    TY v;
+    if constexpr (LPR == 8) v = acc[0];
+    else if constexpr (LPR == 4) v = acc[0] + acc[1];
+    else if constexpr (LPR == 2) v = (acc[0] + acc[2]) + (acc[1] + acc[3]);
+    else v = ((acc[0] + acc[4]) + (acc[2] + acc[6])) + ((acc[1] + acc[5]) + (acc[3] + acc[7]));
+    if constexpr (LPR >= 8) v = v + __shfl_xor(v, 4, LPR);
+    if constexpr (LPR >= 4) v = v + __shfl_xor(v, 2, LPR);
+    if constexpr (LPR >= 2) v = v + __shfl_xor(v, 1, LPR);
+    return v;
+}
+
+// y = A x (or b - A x); 256 rows per block in LPR passes of 256/LPR rows.
 template <typename TV, typename TX, typename TY, int LPR>
 __global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__restrict__ rs,
                                                    const int32_t *__restrict__ col,
                                                    const TV *__restrict__ val,
                                                    const TX *__restrict__ x, TY *__restrict__ y,
                                                    const TY *__restrict__ b) {
-    const int64_t row = (int64_t)blockIdx.x * (256 / LPR) + threadIdx.x / LPR;
+    constexpr int RPP = 256 / LPR;
     const int lane = threadIdx.x % LPR;
-    if (row >= n) return;
-    TY acc = TY(0);
-    for (int32_t k = rs[row] + lane; k < rs[row + 1]; k += LPR) acc += TY(val[k]) * TY(x[col[k]]);
-#pragma unroll
-    for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
-    if (lane == 0) y[row] = b ? b[row] - acc : acc;
+    const int64_t r0 = (int64_t)blockIdx.x * 256;
+    for (int pass = 0; pass < LPR; pass++) {
+        const int64_t row = r0 + pass * RPP + threadIdx.x / LPR;
+        const bool ok = row < n;
+        const int32_t k0 = ok ? rs[row] : 0, k1 = ok ? rs[row + 1] : 0;
+        const TY v = row_dot<TV, TX, TY, LPR>(k0, k1, lane, col, val, x);
+        if (ok && lane == 0) y[row] = b ? b[row] - v : v;
+    }
 }
 
 template <typename T>
@@ -93,18 +129,24 @@ __global__ __launch_bounds__(256) void maxabs_kernel(const double *__restrict__ 
     if (threadIdx.x == 0) partial[c * pstride] = red[0];
 }
 
-__global__ void reduce_partials_kernel(int nvec, int nsub, const int32_t *__restrict__ scp,
-                                       const double *__restrict__ partial, int pstride,
-                                       double *__restrict__ out, int is_max) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nvec * nsub) return;
-    const int j = t / nsub, s = t % nsub;
+// out[j][s] = sum (or max) of the chunk partials of sub-problem s, vector j: one 64-lane
+// block per (j, s); lane t takes chunks t, t+64, ... in order, then a fixed xor tree.
+__global__ __launch_bounds__(64) void reduce_partials_kernel(int nvec, int nsub,
+                                                            const int32_t *__restrict__ scp,
+                                                            const double *__restrict__ partial,
+                                                            int pstride, double *__restrict__ out,
+                                                            int is_max) {
+    const int j = blockIdx.x / nsub, s = blockIdx.x % nsub;
     double acc = 0.0;
-    for (int c = scp[s]; c < scp[s + 1]; c++) {
+    for (int c = scp[s] + threadIdx.x; c < scp[s + 1]; c += 64) {
         const double v = partial[(int64_t)c * pstride + j];
         acc = is_max ? fmax(acc, v) : acc + v;
     }
-    out[j * nsub + s] = acc;
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(acc, off, 64);
+        acc = is_max ? fmax(acc, o) : acc + o;
+    }
+    if (threadIdx.x == 0) out[j * nsub + s] = acc;
 }
 
 // w -= sum_j h[j][sub] V_j  (h = dots of this Gram-Schmidt pass)
@@ -432,6 +474,20 @@ static int krylov_setup(pyhip_problem *p, int m) {
             (rc = kalloc(&W->cg_rz, ns, W)) || (rc = kalloc(&W->cg_pq, ns, W)) ||
             (rc = kalloc(&W->cg_beta, ns, W)) || (rc = kalloc(&W->nfail_d, 1, W)))
             return rc;
+        // single-reduction CG work list
+        std::vector<int64_t> snnz(ns);
+        for (int s = 0; s < ns; s++)
+            snnz[s] = (int64_t)p->row_start_h[p->dof_off_h[s + 1]] - p->row_start_h[p->dof_off_h[s]];
+        const size_t ngp = (size_t)3 * W->nchunk * kGroupsPerChunk;
+        if ((rc = kalloc(&W->achunk, std::max<int64_t>(1, W->nchunk), W)) ||
+            (rc = kalloc(&W->sub_nnz, ns, W)) || (rc = kalloc(&W->gpart, ngp, W)) ||
+            (rc = kalloc(&W->cgst, 1, W)))
+            return rc;
+        if (ns) PYHIP_CHECK_HIP(hipMemcpy(W->sub_nnz, snnz.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice));
+        if (ngp) PYHIP_CHECK_HIP(hipMemset(W->gpart, 0, ngp * sizeof(double)));
+        PYHIP_CHECK_HIP(hipMemset(W->cgst, 0, sizeof(CgState)));
+        PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->cgst_h, sizeof(CgState), hipHostMallocDefault));
+        memset(W->cgst_h, 0, sizeof(CgState));
         PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->nactive_h, sizeof(int32_t), hipHostMallocDefault));
         PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->maxres_h, ns * sizeof(double), hipHostMallocDefault));
     }
@@ -447,6 +503,7 @@ static int krylov_setup(pyhip_problem *p, int m) {
     dfree(W->V);
     W->V = nullptr;
     W->vprec = -1;
+    W->vcount = 0;
     const int64_t ns = W->nsub;
     if ((rc = kalloc(&W->partial, (size_t)W->nchunk * (m + 2) + 1, W)) ||
         (rc = kalloc(&W->dots, (size_t)3 * (m + 2) * ns, W)) ||
@@ -455,6 +512,22 @@ static int krylov_setup(pyhip_problem *p, int m) {
         (rc = kalloc(&W->y, (size_t)ns * m, W)))
         return rc;
     W->m = m;
+    return PYHIP_OK;
+}
+
+// Krylov basis / work vectors V: at least nvec vectors of n entries in precision T.
+template <typename T>
+static int ensure_basis(KrylovWork *W, int64_t n, int nvec) {
+    const int prec = std::is_same<T, float>::value ? PYHIP_F32 : PYHIP_F64;
+    if (W->V && W->vprec == prec && W->vcount >= nvec) return PYHIP_OK;
+    dfree(W->V);
+    W->V = nullptr;
+    W->vprec = -1;
+    W->vcount = 0;
+    int rc;
+    if ((rc = dev_alloc((T **)&W->V, (size_t)nvec * n, W->bytes))) return rc;
+    W->vprec = prec;
+    W->vcount = nvec;
     return PYHIP_OK;
 }
 
@@ -468,8 +541,8 @@ static int batched_dots(pyhip_problem *p, KrylovWork *W, int nvec, const T *V, i
     hipLaunchKernelGGL((multidot_kernel<T>), dim3((unsigned)W->nchunk), dim3(256), 0, p->st, nvec,
                        V, ldv, w, W->chunk_begin, W->chunk_end, W->partial, pstride);
     KCHECK();
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblk((int64_t)nvec * W->nsub, 64)), dim3(64), 0,
-                       p->st, nvec, W->nsub, W->sub_chunk_ptr, W->partial, pstride, out, 0);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)(nvec * W->nsub)), dim3(64), 0, p->st, nvec,
+                       W->nsub, W->sub_chunk_ptr, W->partial, pstride, out, 0);
     KCHECK();
     return PYHIP_OK;
 }
@@ -481,29 +554,75 @@ int batched_maxabs(pyhip_problem *p, const double *v, double *out) {
     hipLaunchKernelGGL(maxabs_kernel, dim3((unsigned)W->nchunk), dim3(256), 0, p->st, v,
                        W->chunk_begin, W->chunk_end, W->partial, pstride);
     KCHECK();
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblk(W->nsub, 64)), dim3(64), 0, p->st, 1,
-                       W->nsub, W->sub_chunk_ptr, W->partial, pstride, out, 1);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)W->nsub), dim3(64), 0, p->st, 1, W->nsub,
+                       W->sub_chunk_ptr, W->partial, pstride, out, 1);
     KCHECK();
     return PYHIP_OK;
 }
 
 template <typename TV, typename TX, typename TY>
+static void launch_spmv(int lpr, hipStream_t st, int64_t n, const int32_t *rs, const int32_t *col,
+                        const TV *vals, const TX *x, TY *y, const TY *b) {
+    const dim3 g(nblk(n, 256)), t(256);
+    switch (lpr) {
+    case 1: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 1>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    case 2: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 2>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    case 4: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 4>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    default: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 8>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    }
+}
+
+// Lanes per row of this problem's spmv: the fastest of 1, 2, 4 and 8 over two timed
+// launches each (all give the same bits), or PYHIP_SPMV_LPR. y is scratch here.
+template <typename TV, typename TX, typename TY>
+static int spmv_tune(pyhip_problem *p, KrylovWork *W, const TV *vals, const TX *x, TY *y) {
+    if (W->lpr) return PYHIP_OK;
+    if (const char *e = getenv("PYHIP_SPMV_LPR")) {
+        const int v = atoi(e);
+        if (v == 1 || v == 2 || v == 4 || v == 8) {
+            W->lpr = v;
+            return PYHIP_OK;
+        }
+    }
+    const int64_t n = p->ndof;
+    W->lpr = 8;
+    if (n == 0) return PYHIP_OK;
+    hipEvent_t e0, e1;
+    PYHIP_CHECK_HIP(hipEventCreate(&e0));
+    PYHIP_CHECK_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    for (int lpr : {1, 2, 4, 8}) {
+        launch_spmv<TV, TX, TY>(lpr, p->st, n, p->row_start, p->col, vals, x, y, nullptr);
+        (void)hipEventRecord(e0, p->st);
+        for (int k = 0; k < 2; k++)
+            launch_spmv<TV, TX, TY>(lpr, p->st, n, p->row_start, p->col, vals, x, y, nullptr);
+        (void)hipEventRecord(e1, p->st);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+            ms < best) {
+            best = ms;
+            W->lpr = lpr;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    KCHECK();
+    if (debug_enabled()) fprintf(stderr, "[pyhip] spmv lanes per row: %d\n", W->lpr);
+    return PYHIP_OK;
+}
+
+// y = A x, or y = b - A x when b is given.
+template <typename TV, typename TX, typename TY>
 static int spmv(pyhip_problem *p, const TV *vals, const TX *x, TY *y, const TY *b) {
-    TimeScope ts(&p->timer, "spmv", p->st);
     const int64_t n = p->ndof;
     if (n == 0) return PYHIP_OK;
-    const double avg = p->nnz / (double)n;
-    if (avg > 48) {
-        hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 32>), dim3(nblk(n, 256 / 32)), dim3(256), 0, p->st,
-                           n, p->row_start, p->col, vals, x, y, b);
-    } else if (avg > 12) {
-        hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 16>), dim3(nblk(n, 256 / 16)), dim3(256), 0, p->st,
-                           n, p->row_start, p->col, vals, x, y, b);
-    } else {
-        hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 8>), dim3(nblk(n, 256 / 8)), dim3(256), 0, p->st,
-                           n, p->row_start, p->col, vals, x, y, b);
-    }
+    int rc;
+    if ((rc = spmv_tune<TV, TX, TY>(p, p->kw, vals, x, y))) return rc;
+    TimeScope ts(&p->timer, "spmv", p->st);
+    launch_spmv<TV, TX, TY>(p->kw->lpr, p->st, n, p->row_start, p->col, vals, x, y, b);
     KCHECK();
+    p->timer.add_work("spmv", (int64_t)(sizeof(TV) + 4) * p->nnz +
+                                  (int64_t)(4 + sizeof(TX) + sizeof(TY) + (b ? sizeof(TY) : 0)) * n + 4);
     return PYHIP_OK;
 }
 
@@ -700,12 +819,7 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
     const int m = W->m, ns = W->nsub;
     const int prec = std::is_same<T, float>::value ? PYHIP_F32 : PYHIP_F64;
     int rc;
-    if (W->vprec != prec) {
-        dfree(W->V);
-        W->V = nullptr;
-        if ((rc = dev_alloc((T **)&W->V, (size_t)(m + 1) * n, W->bytes))) return rc;
-        W->vprec = prec;
-    }
+    if ((rc = ensure_basis<T>(W, n, m + 1))) return rc;
     T *V = (T *)W->V;
     T *w = std::is_same<T, float>::value ? (T *)W->w32 : (T *)W->w64;
     T *t = std::is_same<T, float>::value ? (T *)W->t32 : (T *)W->t64;
@@ -904,8 +1018,8 @@ static int batched_dot2(pyhip_problem *p, KrylovWork *W, const T *a, const T *b,
     hipLaunchKernelGGL((dot2_kernel<T>), dim3((unsigned)W->nchunk), dim3(256), 0, p->st, a, b, c, d,
                        W->chunk_begin, W->chunk_end, W->partial, pstride);
     KCHECK();
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblk((int64_t)2 * W->nsub, 64)), dim3(64), 0, p->st,
-                       2, W->nsub, W->sub_chunk_ptr, W->partial, pstride, out, 0);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)(2 * W->nsub)), dim3(64), 0, p->st, 2,
+                       W->nsub, W->sub_chunk_ptr, W->partial, pstride, out, 0);
     KCHECK();
     return PYHIP_OK;
 }
@@ -917,20 +1031,16 @@ static int read_counter(pyhip_problem *p, KrylovWork *W, int32_t *dev, int32_t *
     return PYHIP_OK;
 }
 
-// x = A^{-1} b per sub-problem with need[s]; CG in precision T on values TV.
+// x = A^{-1} b per sub-problem with need[s]; CG in precision T on values TV, with any
+// preconditioner (used for ILU(0); Jacobi goes through cg_worklist below).
 template <typename T, typename TV>
-static int cg(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T *x,
+static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T *x,
               const pyhip_linear_opts &o, double rtol, const int32_t *need, int *iters, int *failed) {
     const int64_t n = p->ndof;
     const int ns = W->nsub;
     const int prec = std::is_same<T, float>::value ? PYHIP_F32 : PYHIP_F64;
     int rc;
-    if (W->vprec != prec) {
-        dfree(W->V);
-        W->V = nullptr;
-        if ((rc = dev_alloc((T **)&W->V, (size_t)(W->m + 1) * n, W->bytes))) return rc;
-        W->vprec = prec;
-    }
+    if ((rc = ensure_basis<T>(W, n, W->m + 1))) return rc;
     T *pv = (T *)W->V;
     T *qv = pv + n;
     T *r = std::is_same<T, float>::value ? (T *)W->w32 : (T *)W->w64;
@@ -981,6 +1091,407 @@ static int cg(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T *x,
     return PYHIP_OK;
 }
 
+// ---- single-reduction CG on a work list of active chunks (Jacobi preconditioner) -------
+//
+// Chronopoulos-Gear CG: one fused reduction per iteration instead of two. With u = M r
+// and w = A u (M = diag(A)^-1):
+//   gamma = (r, u), delta = (w, u), rho = (r, r)
+//   beta = gamma / gamma_old (0 first), alpha = gamma / (delta - beta gamma / alpha_old)
+//   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = M r
+// which equals preconditioned CG in exact arithmetic. Per iteration: work-list spmv
+// (w, delta partials), one reduction per sub-problem (convergence, alpha, beta, work
+// list rebuild by the last block), and one fused vector update (gamma, rho partials).
+// Only chunks of sub-problems still iterating are visited. Dot products are summed in
+// fixed 32-row groups (xor tree), then per sub-problem over its groups (strided lanes,
+// fixed tree): a sub-problem gets the same bits alone and in any batch.
+
+// sum over the 32 lanes of a half-wave, fixed xor tree
+__device__ __forceinline__ double group_sum(double v) {
+    v += __shfl_xor(v, 16, 32);
+    v += __shfl_xor(v, 8, 32);
+    v += __shfl_xor(v, 4, 32);
+    v += __shfl_xor(v, 2, 32);
+    v += __shfl_xor(v, 1, 32);
+    return v;
+}
+
+// rows [r0, r1) of this block's chunk; false when the block has no rows
+__device__ __forceinline__ bool wl_block(const int32_t *__restrict__ achunk, const CgState *st,
+                                         const int64_t *__restrict__ cb, const int64_t *__restrict__ ce,
+                                         int32_t &c, int &part, int64_t &r0, int64_t &r1) {
+    const int ci = blockIdx.x / kWlBlocksPerChunk;
+    if (ci >= st->nachunk) return false;
+    c = achunk[ci];
+    part = blockIdx.x % kWlBlocksPerChunk;
+    r0 = cb[c] + (int64_t)part * kWlRows;
+    r1 = ce[c];
+    return r0 < r1;
+}
+
+// Work list of the sub-problems with flag[s] != 0 (one block of 256 threads), and the spmv
+// work accounting: with `accumulate`, the previous list is counted as one spmv launch.
+__device__ void build_worklist(int nsub, const int32_t *flag, const int32_t *__restrict__ scp,
+                               const int64_t *__restrict__ dof_off, const int64_t *__restrict__ sub_nnz,
+                               int32_t *__restrict__ achunk, CgState *st, bool accumulate) {
+    __shared__ int32_t scan[256];
+    __shared__ int64_t red_rows[256], red_nnz[256];
+    __shared__ int32_t red_act[256];
+    const int t = threadIdx.x;
+    int32_t base = 0, nact = 0;
+    int64_t rows = 0, nnz = 0;
+    for (int s0 = 0; s0 < nsub; s0 += 256) {
+        const int s = s0 + t;
+        const int a = s < nsub ? __hip_atomic_load(flag + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        const int32_t cnt = a ? scp[s + 1] - scp[s] : 0;
+        if (a) {
+            rows += dof_off[s + 1] - dof_off[s];
+            nnz += sub_nnz[s];
+            nact++;
+        }
+        scan[t] = cnt;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int32_t v = t >= off ? scan[t - off] : 0;
+            __syncthreads();
+            scan[t] += v;
+            __syncthreads();
+        }
+        const int32_t excl = base + scan[t] - cnt;
+        for (int32_t k = 0; k < cnt; k++) achunk[excl + k] = scp[s] + k;
+        base += scan[255];
+        __syncthreads();
+    }
+    red_rows[t] = rows;
+    red_nnz[t] = nnz;
+    red_act[t] = nact;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) {
+            red_rows[t] += red_rows[t + off];
+            red_nnz[t] += red_nnz[t + off];
+            red_act[t] += red_act[t + off];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (accumulate) {
+            st->tot_rows += st->cur_rows;
+            st->tot_nnz += st->cur_nnz;
+            st->spmv_launches++;
+        }
+        st->cur_rows = red_rows[0];
+        st->cur_nnz = red_nnz[0];
+        st->nachunk = base;
+        st->nactive = red_act[0];
+        st->done = 0;
+    }
+}
+
+// start of a solve: active = need, counters cleared, work list of the needed sub-problems
+__global__ __launch_bounds__(256) void cg_begin_kernel(int nsub, const int32_t *__restrict__ need,
+                                                       int32_t *__restrict__ active,
+                                                       const int32_t *__restrict__ scp,
+                                                       const int64_t *__restrict__ dof_off,
+                                                       const int64_t *__restrict__ sub_nnz,
+                                                       int32_t *__restrict__ achunk, CgState *st) {
+    for (int s = threadIdx.x; s < nsub; s += 256) active[s] = need[s] ? 1 : 0;
+    if (threadIdx.x == 0) {
+        st->nfail = 0;
+        st->cur_rows = st->cur_nnz = 0;
+        st->tot_rows = st->tot_nnz = 0;
+        st->spmv_launches = 0;
+    }
+    __threadfence_block();
+    __syncthreads();
+    build_worklist(nsub, active, scp, dof_off, sub_nnz, achunk, st, false);
+}
+
+// r = b, u = M b, x = p = s = 0; gamma and rho partials
+template <typename T>
+__global__ __launch_bounds__(256) void cg_init_kernel(const int32_t *__restrict__ achunk, const CgState *st,
+                                                      const int64_t *__restrict__ cb,
+                                                      const int64_t *__restrict__ ce,
+                                                      const T *__restrict__ b, const T *__restrict__ jd,
+                                                      T *__restrict__ x, T *__restrict__ r,
+                                                      T *__restrict__ u, T *__restrict__ pv,
+                                                      T *__restrict__ sv, double *__restrict__ part,
+                                                      int64_t pstride) {
+    int32_t c;
+    int part_i;
+    int64_t r0, r1;
+    if (!wl_block(achunk, st, cb, ce, c, part_i, r0, r1)) return;
+    const int64_t row = r0 + threadIdx.x;
+    double g = 0.0, rho = 0.0;
+    if (row < r1) {
+        const T ri = b[row];
+        const T ui = T(double(jd[row]) * double(ri));
+        r[row] = ri;
+        u[row] = ui;
+        x[row] = T(0);
+        pv[row] = T(0);
+        sv[row] = T(0);
+        g = double(ri) * double(ui);
+        rho = double(ri) * double(ri);
+    }
+    g = group_sum(g);
+    rho = group_sum(rho);
+    if ((threadIdx.x & 31) == 0) {
+        const int64_t slot = (int64_t)c * kGroupsPerChunk + part_i * (kWlRows / kGroup) + threadIdx.x / kGroup;
+        part[slot] = g;
+        part[pstride + slot] = rho;
+    }
+}
+
+// w = A u on the work list; delta = (w, u) partials
+template <typename TV, typename T, int LPR>
+__global__ __launch_bounds__(256) void cg_spmv_kernel(const int32_t *__restrict__ achunk, const CgState *st,
+                                                      const int64_t *__restrict__ cb,
+                                                      const int64_t *__restrict__ ce,
+                                                      const int32_t *__restrict__ rs,
+                                                      const int32_t *__restrict__ col,
+                                                      const TV *__restrict__ val, const T *__restrict__ u,
+                                                      T *__restrict__ w, double *__restrict__ part_delta) {
+    __shared__ double sp[kWlRows];
+    int32_t c;
+    int part_i;
+    int64_t r0, r1;
+    if (!wl_block(achunk, st, cb, ce, c, part_i, r0, r1)) return;
+    constexpr int RPP = kWlRows / LPR;
+    const int lane = threadIdx.x % LPR;
+    for (int pass = 0; pass < LPR; pass++) {
+        const int loc = pass * RPP + threadIdx.x / LPR;
+        const int64_t row = r0 + loc;
+        const bool ok = row < r1;
+        const int32_t k0 = ok ? rs[row] : 0, k1 = ok ? rs[row + 1] : 0;
+        const T v = row_dot<TV, T, T, LPR>(k0, k1, lane, col, val, u);
+        if (lane == 0) {
+            double d = 0.0;
+            if (ok) {
+                w[row] = v;
+                d = double(u[row]) * double(v);
+            }
+            sp[loc] = d;
+        }
+    }
+    __syncthreads();
+    const double d = group_sum(sp[threadIdx.x]);
+    if ((threadIdx.x & 31) == 0)
+        part_delta[(int64_t)c * kGroupsPerChunk + part_i * (kWlRows / kGroup) + threadIdx.x / kGroup] = d;
+}
+
+// One block per sub-problem: sums its gamma / rho / delta group partials and updates its
+// scalars. mode 0: start (bnorm, gamma; active = need and r != 0); mode 1: first
+// iteration (beta = 0); mode 2: later iterations. The last block to finish rebuilds the
+// work list from the new flags.
+__global__ __launch_bounds__(256) void cg_reduce_kernel(int nsub, int mode, double rtol,
+                                                        const int32_t *__restrict__ scp,
+                                                        const double *__restrict__ part, int64_t pstride,
+                                                        const int32_t *__restrict__ need,
+                                                        double *__restrict__ bnorm, double *__restrict__ gam,
+                                                        double *__restrict__ alpha, double *__restrict__ beta,
+                                                        int32_t *active, const int64_t *__restrict__ dof_off,
+                                                        const int64_t *__restrict__ sub_nnz,
+                                                        int32_t *__restrict__ achunk, CgState *st) {
+    __shared__ double red[3][256];
+    __shared__ int last;
+    const int sb = blockIdx.x, t = threadIdx.x;
+    const bool run = mode == 0 ? need[sb] != 0 : active[sb] != 0;
+    if (run) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        const int64_t g1 = (int64_t)scp[sb + 1] * kGroupsPerChunk;
+        for (int64_t g = (int64_t)scp[sb] * kGroupsPerChunk + t; g < g1; g += 256) {
+            a0 += part[g];
+            a1 += part[pstride + g];
+            a2 += part[2 * pstride + g];
+        }
+        red[0][t] = a0;
+        red[1][t] = a1;
+        red[2][t] = a2;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if (t < off) {
+                red[0][t] += red[0][t + off];
+                red[1][t] += red[1][t + off];
+                red[2][t] += red[2][t + off];
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            const double g = red[0][0], rho = red[1][0], dl = red[2][0];
+            int a = 1;
+            if (mode == 0) {
+                bnorm[sb] = sqrt(rho);
+                gam[sb] = g;
+                a = rho > 0.0 && isfinite(rho);
+            } else if (!isfinite(rho) || !isfinite(g)) {
+                a = 0;
+                atomicAdd(&st->nfail, 1);
+            } else if (!(sqrt(rho) > rtol * bnorm[sb])) {
+                a = 0;   // converged
+            } else {
+                const double bt = mode == 1 ? 0.0 : g / gam[sb];
+                const double den = mode == 1 ? dl : dl - bt * g / alpha[sb];
+                if (!(g > 0.0) || !(den > 0.0) || !isfinite(den)) {
+                    a = 0;   // indefinite system or preconditioner
+                    atomicAdd(&st->nfail, 1);
+                } else {
+                    alpha[sb] = g / den;
+                    beta[sb] = bt;
+                    gam[sb] = g;
+                }
+            }
+            active[sb] = a;
+        }
+    } else if (mode == 0 && t == 0) {
+        active[sb] = 0;
+    }
+    __syncthreads();
+    if (t == 0) {
+        __threadfence();
+        last = atomicAdd(&st->done, 1) == nsub - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    build_worklist(nsub, active, scp, dof_off, sub_nnz, achunk, st, mode != 0);
+}
+
+// p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = M r; gamma, rho partials
+template <typename T>
+__global__ __launch_bounds__(256) void cg_update_kernel(const int32_t *__restrict__ achunk, const CgState *st,
+                                                        const int64_t *__restrict__ cb,
+                                                        const int64_t *__restrict__ ce,
+                                                        const int32_t *__restrict__ chunk_sub,
+                                                        const double *__restrict__ alpha,
+                                                        const double *__restrict__ beta,
+                                                        const T *__restrict__ jd, const T *__restrict__ w,
+                                                        T *__restrict__ u, T *__restrict__ pv,
+                                                        T *__restrict__ sv, T *__restrict__ x,
+                                                        T *__restrict__ r, double *__restrict__ part,
+                                                        int64_t pstride) {
+    int32_t c;
+    int part_i;
+    int64_t r0, r1;
+    if (!wl_block(achunk, st, cb, ce, c, part_i, r0, r1)) return;
+    const int sb = chunk_sub[c];
+    const double a = alpha[sb], bt = beta[sb];
+    const int64_t row = r0 + threadIdx.x;
+    double g = 0.0, rho = 0.0;
+    if (row < r1) {
+        const T pi = T(double(u[row]) + bt * double(pv[row]));
+        const T si = T(double(w[row]) + bt * double(sv[row]));
+        x[row] = T(double(x[row]) + a * double(pi));
+        const T ri = T(double(r[row]) - a * double(si));
+        const T ui = T(double(jd[row]) * double(ri));
+        pv[row] = pi;
+        sv[row] = si;
+        r[row] = ri;
+        u[row] = ui;
+        g = double(ri) * double(ui);
+        rho = double(ri) * double(ri);
+    }
+    g = group_sum(g);
+    rho = group_sum(rho);
+    if ((threadIdx.x & 31) == 0) {
+        const int64_t slot = (int64_t)c * kGroupsPerChunk + part_i * (kWlRows / kGroup) + threadIdx.x / kGroup;
+        part[slot] = g;
+        part[pstride + slot] = rho;
+    }
+}
+
+template <typename TV, typename T>
+static void launch_cg_spmv(int lpr, hipStream_t st, unsigned grid, KrylovWork *W, pyhip_problem *p,
+                           const TV *vals, const T *u, T *w, double *pd) {
+    const dim3 g(grid), t(256);
+#define CG_SPMV(L)                                                                                 \
+    hipLaunchKernelGGL((cg_spmv_kernel<TV, T, L>), g, t, 0, st, W->achunk, W->cgst, W->chunk_begin, \
+                       W->chunk_end, p->row_start, p->col, vals, u, w, pd)
+    switch (lpr) {
+    case 1: CG_SPMV(1); break;
+    case 2: CG_SPMV(2); break;
+    case 4: CG_SPMV(4); break;
+    default: CG_SPMV(8); break;
+    }
+#undef CG_SPMV
+}
+
+static int read_cg_state(pyhip_problem *p, KrylovWork *W) {
+    PYHIP_CHECK_HIP(hipMemcpyAsync(W->cgst_h, W->cgst, sizeof(CgState), hipMemcpyDeviceToHost, p->st));
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    return PYHIP_OK;
+}
+
+// x = A^{-1} b for the sub-problems with need[s], Jacobi-preconditioned single-reduction
+// CG in precision T on values TV. The host reads the state every kCgCheck iterations;
+// launches in between size their grids from the last read (an upper bound, since the
+// work list only shrinks) and blocks past the current list exit at once.
+constexpr int kCgCheck = 8;
+
+template <typename T, typename TV>
+static int cg_worklist(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T *x,
+                       const pyhip_linear_opts &o, double rtol, const int32_t *need, int *iters,
+                       int *failed) {
+    const int64_t n = p->ndof;
+    const int ns = W->nsub;
+    const int prec = std::is_same<T, float>::value ? PYHIP_F32 : PYHIP_F64;
+    int rc;
+    *iters = 0;
+    *failed = 0;
+    if (n == 0 || W->nchunk == 0) return PYHIP_OK;
+    PYHIP_REQUIRE(W->achunk && W->sub_nnz && W->gpart && W->cgst && W->cgst_h && W->active && need &&
+                      W->sub_chunk_ptr && p->dof_off && W->chunk_begin && W->chunk_end && W->chunk_sub,
+                  "CG work-list buffers missing");
+    if ((rc = ensure_basis<T>(W, n, std::max(W->m + 1, 3)))) return rc;
+    T *pv = (T *)W->V, *sv = pv + n, *wv = pv + 2 * n;
+    T *r = std::is_same<T, float>::value ? (T *)W->w32 : (T *)W->w64;
+    T *u = std::is_same<T, float>::value ? (T *)W->t32 : (T *)W->t64;
+    const T *jd = std::is_same<T, float>::value ? (const T *)W->jdiag32 : (const T *)W->jdiag64;
+    if ((rc = spmv_tune<TV, T, T>(p, W, vals, b, wv))) return rc;
+    const int64_t pstride = W->nchunk * kGroupsPerChunk;
+    double *pg = W->gpart, *pd = W->gpart + 2 * pstride;
+    const unsigned full = (unsigned)(W->nchunk * kWlBlocksPerChunk);
+    {
+        TimeScope ts(&p->timer, "vector", p->st);
+        hipLaunchKernelGGL(cg_begin_kernel, dim3(1), dim3(256), 0, p->st, ns, need, W->active,
+                           W->sub_chunk_ptr, p->dof_off, W->sub_nnz, W->achunk, W->cgst);
+        hipLaunchKernelGGL((cg_init_kernel<T>), dim3(full), dim3(256), 0, p->st, W->achunk, W->cgst,
+                           W->chunk_begin, W->chunk_end, b, jd, x, r, u, pv, sv, pg, pstride);
+        hipLaunchKernelGGL(cg_reduce_kernel, dim3(ns), dim3(256), 0, p->st, ns, 0, rtol, W->sub_chunk_ptr,
+                           W->gpart, pstride, need, W->bnorm, W->cg_rz, W->cg_pq, W->cg_beta, W->active,
+                           p->dof_off, W->sub_nnz, W->achunk, W->cgst);
+        KCHECK();
+    }
+    if ((rc = read_cg_state(p, W))) return rc;
+    int it = 0;
+    while (W->cgst_h->nactive > 0 && it < o.max_iters) {
+        const unsigned grid = (unsigned)std::max<int64_t>(1, (int64_t)W->cgst_h->nachunk * kWlBlocksPerChunk);
+        const int batch = std::min(kCgCheck, o.max_iters - it);
+        for (int k = 0; k < batch; k++, it++) {
+            {
+                TimeScope ts(&p->timer, "spmv", p->st);
+                launch_cg_spmv<TV, T>(W->lpr, p->st, grid, W, p, vals, u, wv, pd);
+            }
+            TimeScope ts(&p->timer, "vector", p->st);
+            hipLaunchKernelGGL(cg_reduce_kernel, dim3(ns), dim3(256), 0, p->st, ns, it == 0 ? 1 : 2, rtol,
+                               W->sub_chunk_ptr, W->gpart, pstride, need, W->bnorm, W->cg_rz, W->cg_pq,
+                               W->cg_beta, W->active, p->dof_off, W->sub_nnz, W->achunk, W->cgst);
+            hipLaunchKernelGGL((cg_update_kernel<T>), dim3(grid), dim3(256), 0, p->st, W->achunk, W->cgst,
+                               W->chunk_begin, W->chunk_end, W->chunk_sub, W->cg_pq, W->cg_beta, jd, wv, u,
+                               pv, sv, x, r, pg, pstride);
+        }
+        KCHECK();
+        if ((rc = read_cg_state(p, W))) return rc;
+    }
+    const CgState &S = *W->cgst_h;
+    p->timer.add_work("spmv", (int64_t)(sizeof(TV) + 4) * S.tot_nnz + (int64_t)(4 + 2 * sizeof(T)) * S.tot_rows);
+    if (debug_enabled())
+        fprintf(stderr, "[pyhip] cg: %d iterations, %d still active, %d failed, %lld spmv launches with work\n",
+                it, S.nactive, S.nfail, (long long)S.spmv_launches);
+    *iters = it;
+    *failed = S.nfail + S.nactive;
+    return PYHIP_OK;
+}
+
 }  // namespace pyhip
 
 // ------------------------------------------------------------------------------------
@@ -999,7 +1510,9 @@ template <typename T, typename TV>
 static int krylov(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T *x,
                   const pyhip_linear_opts &o, double rtol, const int32_t *need, int *iters,
                   int *failed) {
-    if (o.krylov == PYHIP_KRYLOV_CG) return cg<T, TV>(p, W, vals, b, x, o, rtol, need, iters, failed);
+    if (o.krylov == PYHIP_KRYLOV_CG && o.precond == PYHIP_PRECOND_JACOBI)
+        return cg_worklist<T, TV>(p, W, vals, b, x, o, rtol, need, iters, failed);
+    if (o.krylov == PYHIP_KRYLOV_CG) return cg_generic<T, TV>(p, W, vals, b, x, o, rtol, need, iters, failed);
     return gmres<T, TV>(p, W, vals, b, x, o, rtol, need, iters, failed);
 }
 
@@ -1130,11 +1643,13 @@ void pyhip_krylov_free(pyhip_problem *p) {
                     W->jdiag64, W->jdiag32, W->dof_mask, W->need, W->cg_rz, W->cg_pq,
                     W->cg_beta, W->nfail_d, W->ilu.perm, W->ilu.rs, W->ilu.col,
                     W->ilu.src, W->ilu.diag, W->ilu.lo_rows, W->ilu.up_rows, W->ilu.lu,
-                    W->ilu.tmp64, W->ilu.tmp64b, W->ilu.tmp32, W->ilu.tmp32b};
+                    W->ilu.tmp64, W->ilu.tmp64b, W->ilu.tmp32, W->ilu.tmp32b, W->achunk,
+                    W->sub_nnz, W->gpart, W->cgst};
     for (void *q : ptrs)
         dfree(q);
     if (W->nactive_h) (void)hipHostFree(W->nactive_h);
     if (W->maxres_h) (void)hipHostFree(W->maxres_h);
+    if (W->cgst_h) (void)hipHostFree(W->cgst_h);
     delete W;
     p->kw = nullptr;
 }

@@ -6,11 +6,12 @@
 // mesh and numbering, and the per-assembly work becomes a gather:
 //   * for every global row r, its "occurrences" (element e, local row i) in element
 //     order, i.e. the element rows that add into r (problem.cc:5519, residuals[eqn] +=),
-//   * for every occurrence and element column j, the position of column eqn(e, j) inside
-//     row r of the column-sorted pattern (255 for pinned columns).
-// Summing the element rows of r in occurrence order reproduces oomph's element-order
-// summation of every entry; the first-touch column order and the exact-zero drop of the
-// reference CSR are recovered from the same map on export.
+//   * for every entry of the column-sorted pattern, the list of element-matrix entries
+//     that add into it, in element order ("contributions", CSR over the nnz entries).
+// Summing an entry's contributions in list order reproduces oomph's element-order
+// summation. Within a row the element-matrix index of a contribution grows with
+// (occurrence, element column), the reference's visiting order, so an entry's first
+// non-zero contribution index is its first-touch key for the reference-order export.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -18,8 +19,8 @@
 
 namespace pyhip {
 
-constexpr int kMaxSlot = 27;     // element dofs per element, padded
-constexpr uint8_t kNoPos = 255;  // pinned column / unused slot
+constexpr int kMaxSlot = 27;     // element dofs per element (Q2 hex)
+constexpr int kMaxRowLen = 255;  // entries per row
 
 struct ElementBlock {
     int ns;                      // slots per element
@@ -35,9 +36,18 @@ struct HostPattern {
     std::vector<int32_t> col;         // [nnz]
     std::vector<int64_t> occ_ptr;     // [ndof+1]
     std::vector<int64_t> occ_roff;    // [nocc] element-vector entry of the occurrence
-    std::vector<int64_t> occ_eoff;    // [nocc] first entry of the element-matrix row
-    std::vector<uint8_t> occ_pos;     // [nocc][kMaxSlot]
+    std::vector<int32_t> cptr;        // [nnz+1] contributions of each entry
+    std::vector<uint32_t> cidx;       // [ncontrib] element-matrix index, element order
     std::vector<int32_t> diag_pos;    // [ndof] index of the diagonal entry (or -1)
+};
+
+// Device copies of the map above, as the gather kernels take them.
+struct GatherMap {
+    int64_t nrow, nnz;
+    const int32_t *rs;
+    const int64_t *occ_ptr, *occ_roff;
+    const int32_t *cptr;
+    const uint32_t *cidx;
 };
 
 // Blocks must be given in element order (the reference's mesh element order; several

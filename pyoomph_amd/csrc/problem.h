@@ -32,6 +32,7 @@ struct Group {
 
 struct KrylovWork;   // krylov.hip
 
+
 // HIP-event timing of kernel families on the problem's stream (measurement aid).
 struct KernelTimer {
     bool on = false;
@@ -39,6 +40,10 @@ struct KernelTimer {
     size_t used = 0;
     std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::map<std::string, std::pair<int64_t, double>> total;
+    std::map<std::string, int64_t> work;   // algorithmic bytes moved by the family's kernels
+    void add_work(const char *fam, int64_t bytes) {
+        if (on) work[fam] += bytes;
+    }
     hipEvent_t take() {
         if (used == pool.size()) {
             hipEvent_t e;
@@ -93,21 +98,17 @@ struct TimeScope {
 
 int launch_elements_f64(int kind, int form, const ElemArgs<double> &a, hipStream_t s);
 int launch_elements_f32(int kind, int form, const ElemArgs<float> &a, hipStream_t s);
-int launch_gather_values_f64(int64_t nrow, const int32_t *rs, const int64_t *occ_ptr,
-                             const int64_t *occ_eoff, const uint8_t *occ_pos, const double *Ej,
-                             double *vals, hipStream_t st);
-int launch_gather_values_f32(int64_t nrow, const int32_t *rs, const int64_t *occ_ptr,
-                             const int64_t *occ_eoff, const uint8_t *occ_pos, const float *Ej,
-                             float *vals, hipStream_t st);
+int launch_gather_values_f64(const GatherMap &m, const double *Ej, const double *Er, double *vals,
+                             double *res, hipStream_t st);
+int launch_gather_values_f32(const GatherMap &m, const float *Ej, const float *Er, float *vals,
+                             double *res, hipStream_t st);
 int launch_gather_residual_f64(int64_t nrow, const int64_t *occ_ptr, const int64_t *occ_roff,
                                const double *Er, double *res, hipStream_t st);
 int launch_gather_residual_f32(int64_t nrow, const int64_t *occ_ptr, const int64_t *occ_roff,
                                const float *Er, double *res, hipStream_t st);
-int export_reference_csr(int64_t nrow, const int32_t *rs, const int32_t *col,
-                         const int64_t *occ_ptr, const int64_t *occ_eoff,
-                         const uint8_t *occ_pos, const double *Ej, const double *vals,
-                         int32_t *keys, int32_t *kept, int32_t *out_rs, int32_t *out_col,
-                         double *out_val, int64_t *nnz_out, hipStream_t st);
+int export_reference_csr(const GatherMap &m, const int32_t *col, const double *Ej,
+                         const double *vals, uint32_t *keys, int32_t *kept, int32_t *out_rs,
+                         int32_t *out_col, double *out_val, int64_t *nnz_out, hipStream_t st);
 
 // Device allocation tracked against the problem.
 template <typename T>
@@ -143,8 +144,9 @@ struct pyhip_problem {
     pyhip::DevTables<float> *tab32[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // pattern / map
     int32_t *row_start = nullptr, *col = nullptr, *diag = nullptr;
-    int64_t *occ_ptr = nullptr, *occ_roff = nullptr, *occ_eoff = nullptr;
-    uint8_t *occ_pos = nullptr;
+    int64_t *occ_ptr = nullptr, *occ_roff = nullptr;
+    int32_t *cptr = nullptr;
+    uint32_t *cidx = nullptr;
     int32_t *dof_sub = nullptr;             // [ndof] sub-problem of each row
     int64_t *dof_off = nullptr;             // [nsub+1]
     std::vector<int32_t> row_start_h, col_h;
@@ -153,11 +155,15 @@ struct pyhip_problem {
     float *Er32 = nullptr, *Ej32 = nullptr;
     bool have_ej64 = false;                  // Ej64 holds the last FP64 Jacobian
     // export scratch (lazy)
-    int32_t *keys = nullptr, *kept = nullptr;
+    uint32_t *keys = nullptr;
+    int32_t *kept = nullptr;
     // Newton / Krylov workspace (lazy)
     pyhip::KrylovWork *kw = nullptr;
     // ILU(0) ordering: rows listed here go after all other rows of their sub-problem
     // (set for matrices that arrive without element information)
     std::vector<char> ilu_last_h;
     pyhip::KernelTimer timer;
+    pyhip::GatherMap gmap() const {
+        return pyhip::GatherMap{ndof, nnz, row_start, occ_ptr, occ_roff, cptr, cidx};
+    }
 };

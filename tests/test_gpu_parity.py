@@ -130,6 +130,40 @@ def test_batch_equals_alone_bitwise(gpu):
         assert np.array_equal(alone.get_current_dofs(), batch.sub_dofs(i, xb))
 
 
+@pytest.mark.parametrize("kind,kw,opts", [
+    ("bratu", dict(N=12, space="C2", lam=2.0), dict(krylov="cg", preconditioner="jacobi")),
+    ("rd", dict(N=3), dict(krylov="cg", preconditioner="jacobi")),
+    ("ns", dict(N=5), dict(krylov="gmres", preconditioner="ilu0")),
+])
+def test_spmv_lane_count_keeps_bits(gpu, monkeypatch, kind, kw, opts):
+    """Every spmv lane count (1, 2, 4, 8 lanes per row) sums a row in the same order."""
+    xs = []
+    for lpr in ("1", "2", "4", "8"):
+        monkeypatch.setenv("PYHIP_SPMV_LPR", lpr)
+        P, op = build(kind, kw)
+        for k, v in opts.items():
+            setattr(P, k, v)
+        if kind == "ns":
+            P.newton_solver_tolerance = 1e-10
+        st = P.solve()
+        assert st[0]["converged"]
+        xs.append(P.get_current_dofs())
+        del P
+    for x in xs[1:]:
+        assert np.array_equal(x, xs[0])
+    xo = on.newton_solve(op, np.zeros(op.ndof), tol=1e-10 if kind == "ns" else 1e-8)
+    assert rel(xs[0], xo) <= 1e-8
+
+
+def test_cg_iteration_cap_reports_failure(gpu):
+    """A CG capped below what a problem needs reports the failure instead of a silent step."""
+    P, _ = build("bratu", dict(N=24, space="C2", lam=1.0))
+    P.krylov, P.preconditioner = "cg", "jacobi"
+    P.gmres_max_iters = 3
+    st = P.solve(raise_on_failure=False)
+    assert st[0]["linear_failures"] > 0
+
+
 def test_newton_error_path(gpu):
     from pyoomph_amd.problem import NewtonSolverError
     P, _ = build("bratu", dict(N=8, space="C2", lam=2.5))
