@@ -41,6 +41,16 @@ def parse():
                     help="bound on the CPU-baseline sample (seconds of oracle work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--precision", choices=("f64", "mixed"), default="f64",
+                    help="f64: FP64 Jacobian and Krylov; mixed: FP32 element kernels for the "
+                         "Jacobian and FP32 inner CG inside FP64 iterative refinement (FP64 "
+                         "residuals, updates and norms)")
+    ap.add_argument("--err-sample", type=int, default=6,
+                    help="problems (spread over the size range, largest included) checked against "
+                         "the FP64 oracle")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a 1 GiB read + 1 GiB write streaming kernel (three launches), the "
+                         "reference for scaling HBM byte counters")
     return ap.parse_args()
 
 
@@ -80,11 +90,11 @@ def timing_bytes(P, fam):
     return b.value
 
 
-def configure(P):
+def configure(P, precision="f64"):
     P.krylov = "cg"
     P.preconditioner = "jacobi"
-    P.jacobian_precision = "f64"
-    P.linear_precision = "f64"
+    P.jacobian_precision = "f32" if precision == "mixed" else "f64"
+    P.linear_precision = "f32" if precision == "mixed" else "f64"
     P.gmres_max_iters = 20000
 
 
@@ -99,10 +109,17 @@ def main():
     from pyoomph_amd.problem import Problem
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
     specs = wl.bratu_sweep_specs(args.nprob, seed=args.seed)
     parts = [wl.bratu(N, sp, lam) for N, sp, lam in specs]
     P = Problem(parts)
-    configure(P)
+    configure(P, args.precision)
     print("# batch: %d problems, %d dofs, %d nnz" % (len(parts), P.ndof, P.nnz), file=sys.stderr, flush=True)
 
     def step():
@@ -166,7 +183,7 @@ def main():
         t_single, n_single = 0.0, 0
         for N, sp, lam in specs[:8]:
             Q = Problem(wl.bratu(N, sp, lam))
-            configure(Q)
+            configure(Q, "f64")
             torch.cuda.synchronize()
             t = time.perf_counter()
             Q.solve()
@@ -176,13 +193,29 @@ def main():
             del Q
         single = n_single / t_single
 
-    # oracle: accuracy sample and CPU baseline (single thread)
+    # CPU-baseline leg: the oracle checks accuracy (problems spread over the size range,
+    # largest included, plus the timed sample) and is timed on the smallest problems
     cpu = None
-    max_err = None
+    errs = []
     if not args.no_cpu_baseline:
         from oracle import newton as on
         from tests.oracle_bridge import oracle_problem
-        errs, n_cpu, t_cpu = [], 0, 0.0
+        order = sorted(range(len(specs)), key=lambda i: (specs[i][0] * (2 if specs[i][1] == "C2" else 1)))
+
+        def err_of(i, xo):
+            xg = P.sub_dofs(i, x_gpu)
+            return float(np.max(np.abs(xg - xo)) / max(np.max(np.abs(xo)), 1e-300))
+
+        if args.err_sample > 0:
+            picks = sorted(set(order[int(round(k * (len(order) - 1) / max(args.err_sample - 1, 1)))]
+                               for k in range(args.err_sample)))
+            for i in picks:
+                N, sp, lam = specs[i]
+                op = oracle_problem("bratu", N=N, space=sp, lam=lam)
+                errs.append(err_of(i, on.newton_solve(op, np.zeros(op.ndof))))
+            print("# accuracy sample: %s" % [(specs[i][0], specs[i][1]) for i in picks], file=sys.stderr,
+                  flush=True)
+        n_cpu, t_cpu = 0, 0.0
         order = sorted(range(len(specs)), key=lambda i: specs[i][0])
         for i in order:
             N, sp, lam = specs[i]
@@ -191,16 +224,16 @@ def main():
             xo = on.newton_solve(op, np.zeros(op.ndof))
             t_cpu += time.perf_counter() - t
             n_cpu += 1
-            xg = P.sub_dofs(i, x_gpu)
-            errs.append(float(np.max(np.abs(xg - xo)) / max(np.max(np.abs(xo)), 1e-300)))
+            errs.append(err_of(i, xo))
             if t_cpu > args.cpu_seconds:
                 break
-        max_err = max(errs)
         sizes = [specs[i][0] for i in order[:n_cpu]]
         cpu = {"value": n_cpu / t_cpu, "unit": "solves/s", "cores": 1, "kind": "port",
+               "host_cores": os.cpu_count(),
                "sample": "%d smallest problems of the sweep (N=%d..%d), oracle C element code + "
                          "vectors_of_pairs CSR + scipy SuperLU, one thread" % (n_cpu, min(sizes), max(sizes))}
 
+    max_err = max(errs) if errs else None
     lat = np.array(lat)
     out = {
         "metric": METRIC,
@@ -213,12 +246,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "f64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": "bratu_sweep: %d independent -lap u = lambda e^u problems, "
                                "N~U{16..128}, P1/P2 50/50, lambda~U(0.5,3)" % args.nprob,
                    "problems": args.nprob, "dofs": P.ndof, "nnz": P.nnz,
-                   "solver": "Newton + batched CG/Jacobi, FP64", "seed": args.seed},
+                   "solver": ("Newton + batched CG/Jacobi, FP64" if args.precision == "f64" else
+                              "Newton (FP64 residual) + FP32 Jacobian, FP32 batched CG/Jacobi inside FP64 "
+                              "iterative refinement"),
+                   "precision": args.precision, "seed": args.seed},
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
         "max_rel_err_vs_oracle": max_err,

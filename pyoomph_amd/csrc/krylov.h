@@ -7,22 +7,8 @@
 namespace pyhip {
 
 constexpr int kChunk = 1024;  // rows per reduction chunk (chunks never straddle sub-problems)
-constexpr int kGroup = 32;    // rows per dot-product group of the CG kernels
-constexpr int kGroupsPerChunk = kChunk / kGroup;
 constexpr int kWlRows = 256;         // rows per block of the work-list kernels
 constexpr int kWlBlocksPerChunk = kChunk / kWlRows;
-
-// Device state of the single-reduction CG: work list size, counters and the spmv work
-// done (algorithmic accounting for measurements). Read by the host through pinned memory.
-struct CgState {
-    int32_t nachunk;          // chunks in the work list
-    int32_t nactive;          // sub-problems still iterating
-    int32_t nfail;            // breakdowns / non-finite residuals
-    int32_t done;             // blocks finished in the current reduction (self-resetting)
-    int64_t cur_rows, cur_nnz;    // rows / entries in the current work list
-    int64_t tot_rows, tot_nnz;    // summed over the spmv launches so far
-    int64_t spmv_launches;
-};
 
 struct IluPlan {
     bool built = false;
@@ -85,11 +71,14 @@ struct KrylovWork {
     double *cg_rz = nullptr, *cg_pq = nullptr, *cg_beta = nullptr;   // [nsub]
     int32_t *nfail_d = nullptr;       // breakdown / non-convergence counter
     // single-reduction CG (work list over active chunks)
-    int32_t *achunk = nullptr;        // [nchunk] active chunks
-    int64_t *sub_nnz = nullptr;       // [nsub] pattern entries per sub-problem
-    double *gpart = nullptr;          // [3][nchunk*kGroupsPerChunk] group partials
-    CgState *cgst = nullptr;          // device
-    CgState *cgst_h = nullptr;        // pinned host copy
+    int32_t *achunk = nullptr;        // [nchunk] chunks of the sub-problems iterating
+    int32_t *achunk_h = nullptr;      // pinned [nchunk]
+    double *gpart = nullptr;          // [3][nchunk*kWlBlocksPerChunk] block partials
+    int32_t *cg_iters = nullptr;      // [nsub] iterations done in the current solve
+    int32_t *cg_fail = nullptr;       // [nsub] breakdown / non-finite residual
+    int32_t *flags_h = nullptr;       // pinned [3][nsub]: active, fail, iterations
+    std::vector<int64_t> sub_nnz_h;   // [nsub] pattern entries per sub-problem
+    std::vector<int32_t> chunk_ptr_h; // [nsub+1] host copy of sub_chunk_ptr
     int lpr = 0;                      // spmv lanes per row (1, 2, 4, 8), 0 = not tuned
     IluPlan ilu;
     int64_t bytes = 0;

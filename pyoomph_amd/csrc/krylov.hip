@@ -64,7 +64,8 @@ __device__ __forceinline__ TY row_dot(int32_t k0, int32_t k1, int lane, const in
     return v;
 }
 
-// y = A x (or b - A x); 256 rows per block in LPR passes of 256/LPR rows.
+// y = A x (or b - A x); 256 rows per block in LPR passes of 256/LPR rows; the grid is
+// a multiple of 8 (spmv_grid).
 template <typename TV, typename TX, typename TY, int LPR>
 __global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__restrict__ rs,
                                                    const int32_t *__restrict__ col,
@@ -73,7 +74,11 @@ __global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__r
                                                    const TY *__restrict__ b) {
     constexpr int RPP = 256 / LPR;
     const int lane = threadIdx.x % LPR;
-    const int64_t r0 = (int64_t)blockIdx.x * 256;
+    // blocks round-robin over the 8 XCDs: XCD x gets the contiguous block range x*per..
+    const int64_t nb = (n + 255) / 256, per = (nb + 7) / 8, k = blockIdx.x / 8;
+    const int64_t lb = (int64_t)(blockIdx.x % 8) * per + k;
+    if (k >= per || lb >= nb) return;
+    const int64_t r0 = lb * 256;
     for (int pass = 0; pass < LPR; pass++) {
         const int64_t row = r0 + pass * RPP + threadIdx.x / LPR;
         const bool ok = row < n;
@@ -449,6 +454,7 @@ static int krylov_setup(pyhip_problem *p, int m) {
         }
         scp[p->nsub] = (int32_t)cb.size();
         W->nchunk = (int64_t)cb.size();
+        W->chunk_ptr_h = scp;
         int rc;
 #define KUP(dst, v)                                                                        \
     if ((rc = kalloc(&(dst), (v).size(), W))) return rc;                                   \
@@ -475,19 +481,19 @@ static int krylov_setup(pyhip_problem *p, int m) {
             (rc = kalloc(&W->cg_beta, ns, W)) || (rc = kalloc(&W->nfail_d, 1, W)))
             return rc;
         // single-reduction CG work list
-        std::vector<int64_t> snnz(ns);
+        W->sub_nnz_h.resize(ns);
         for (int s = 0; s < ns; s++)
-            snnz[s] = (int64_t)p->row_start_h[p->dof_off_h[s + 1]] - p->row_start_h[p->dof_off_h[s]];
-        const size_t ngp = (size_t)3 * W->nchunk * kGroupsPerChunk;
+            W->sub_nnz_h[s] = (int64_t)p->row_start_h[p->dof_off_h[s + 1]] - p->row_start_h[p->dof_off_h[s]];
+        const size_t ngp = (size_t)3 * W->nchunk * kWlBlocksPerChunk;
         if ((rc = kalloc(&W->achunk, std::max<int64_t>(1, W->nchunk), W)) ||
-            (rc = kalloc(&W->sub_nnz, ns, W)) || (rc = kalloc(&W->gpart, ngp, W)) ||
-            (rc = kalloc(&W->cgst, 1, W)))
+            (rc = kalloc(&W->gpart, ngp, W)) || (rc = kalloc(&W->cg_iters, ns, W)) ||
+            (rc = kalloc(&W->cg_fail, ns, W)))
             return rc;
-        if (ns) PYHIP_CHECK_HIP(hipMemcpy(W->sub_nnz, snnz.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice));
         if (ngp) PYHIP_CHECK_HIP(hipMemset(W->gpart, 0, ngp * sizeof(double)));
-        PYHIP_CHECK_HIP(hipMemset(W->cgst, 0, sizeof(CgState)));
-        PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->cgst_h, sizeof(CgState), hipHostMallocDefault));
-        memset(W->cgst_h, 0, sizeof(CgState));
+        PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->achunk_h, std::max<int64_t>(1, W->nchunk) * sizeof(int32_t),
+                                      hipHostMallocDefault));
+        PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->flags_h, std::max<int64_t>(1, 3 * ns) * sizeof(int32_t),
+                                      hipHostMallocDefault));
         PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->nactive_h, sizeof(int32_t), hipHostMallocDefault));
         PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->maxres_h, ns * sizeof(double), hipHostMallocDefault));
     }
@@ -560,10 +566,15 @@ int batched_maxabs(pyhip_problem *p, const double *v, double *out) {
     return PYHIP_OK;
 }
 
+static inline unsigned spmv_grid(int64_t n) {
+    const int64_t nb = std::max<int64_t>(1, (n + 255) / 256);
+    return (unsigned)(((nb + 7) / 8) * 8);
+}
+
 template <typename TV, typename TX, typename TY>
 static void launch_spmv(int lpr, hipStream_t st, int64_t n, const int32_t *rs, const int32_t *col,
                         const TV *vals, const TX *x, TY *y, const TY *b) {
-    const dim3 g(nblk(n, 256)), t(256);
+    const dim3 g(spmv_grid(n)), t(256);
     switch (lpr) {
     case 1: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 1>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
     case 2: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 2>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
@@ -1099,116 +1110,69 @@ static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *
 //   beta = gamma / gamma_old (0 first), alpha = gamma / (delta - beta gamma / alpha_old)
 //   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = M r
 // which equals preconditioned CG in exact arithmetic. Per iteration: work-list spmv
-// (w, delta partials), one reduction per sub-problem (convergence, alpha, beta, work
-// list rebuild by the last block), and one fused vector update (gamma, rho partials).
-// Only chunks of sub-problems still iterating are visited. Dot products are summed in
-// fixed 32-row groups (xor tree), then per sub-problem over its groups (strided lanes,
-// fixed tree): a sub-problem gets the same bits alone and in any batch.
+// (w, delta partials), one reduction block per sub-problem (convergence, alpha, beta),
+// and one fused vector update (gamma, rho partials).
+//
+// Work list: the chunks of the sub-problems still iterating, built by the host each time
+// it reads the flags (every kCgCheck iterations). Between reads the list can only hold
+// extra chunks, whose blocks see their sub-problem's flag cleared and return. Dot
+// products are summed per 256-row block of a chunk (fixed tree), then per sub-problem
+// over its blocks (strided lanes, fixed tree): a sub-problem gets the same bits alone and
+// in any batch.
 
-// sum over the 32 lanes of a half-wave, fixed xor tree
-__device__ __forceinline__ double group_sum(double v) {
+// Sum of one value per thread over a 256-thread block in a fixed order: xor tree within
+// each 32-lane group, then the 8 group sums in increasing order. Result in thread 0.
+__device__ __forceinline__ double block_sum(double v, double *sg) {
     v += __shfl_xor(v, 16, 32);
     v += __shfl_xor(v, 8, 32);
     v += __shfl_xor(v, 4, 32);
     v += __shfl_xor(v, 2, 32);
     v += __shfl_xor(v, 1, 32);
-    return v;
+    if ((threadIdx.x & 31) == 0) sg[threadIdx.x >> 5] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int g = 0; g < kWlRows / 32; g++) s += sg[g];
+    return s;
 }
 
-// rows [r0, r1) of this block's chunk; false when the block has no rows
-__device__ __forceinline__ bool wl_block(const int32_t *__restrict__ achunk, const CgState *st,
+// Work-list block -> (chunk, part). Blocks are dispatched round-robin over the XCDs (block
+// b runs on XCD b % kXcd), so each XCD is given a contiguous slice of the work list and
+// the entries of the gathered vector its rows touch stay in that XCD's L2. The grid is
+// wl_grid(nachunk). Rows [r0, r1); false when the block has none or its sub-problem has
+// stopped iterating.
+constexpr int kXcd = 8;
+
+__device__ __forceinline__ bool wl_block(const int32_t *__restrict__ achunk, int nachunk,
+                                         const int32_t *__restrict__ chunk_sub,
+                                         const int32_t *__restrict__ active,
                                          const int64_t *__restrict__ cb, const int64_t *__restrict__ ce,
-                                         int32_t &c, int &part, int64_t &r0, int64_t &r1) {
-    const int ci = blockIdx.x / kWlBlocksPerChunk;
-    if (ci >= st->nachunk) return false;
-    c = achunk[ci];
-    part = blockIdx.x % kWlBlocksPerChunk;
+                                         int32_t &c, int &sb, int &part, int64_t &r0, int64_t &r1) {
+    const int64_t A = (int64_t)nachunk * kWlBlocksPerChunk;
+    const int64_t per = (A + kXcd - 1) / kXcd;
+    const int64_t k = blockIdx.x / kXcd;
+    if (k >= per) return false;
+    const int64_t lb = (int64_t)(blockIdx.x % kXcd) * per + k;
+    if (lb >= A) return false;
+    c = achunk[lb / kWlBlocksPerChunk];
+    sb = chunk_sub[c];
+    if (!active[sb]) return false;
+    part = (int)(lb % kWlBlocksPerChunk);
     r0 = cb[c] + (int64_t)part * kWlRows;
     r1 = ce[c];
     return r0 < r1;
 }
 
-// Work list of the sub-problems with flag[s] != 0 (one block of 256 threads), and the spmv
-// work accounting: with `accumulate`, the previous list is counted as one spmv launch.
-__device__ void build_worklist(int nsub, const int32_t *flag, const int32_t *__restrict__ scp,
-                               const int64_t *__restrict__ dof_off, const int64_t *__restrict__ sub_nnz,
-                               int32_t *__restrict__ achunk, CgState *st, bool accumulate) {
-    __shared__ int32_t scan[256];
-    __shared__ int64_t red_rows[256], red_nnz[256];
-    __shared__ int32_t red_act[256];
-    const int t = threadIdx.x;
-    int32_t base = 0, nact = 0;
-    int64_t rows = 0, nnz = 0;
-    for (int s0 = 0; s0 < nsub; s0 += 256) {
-        const int s = s0 + t;
-        const int a = s < nsub ? __hip_atomic_load(flag + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-        const int32_t cnt = a ? scp[s + 1] - scp[s] : 0;
-        if (a) {
-            rows += dof_off[s + 1] - dof_off[s];
-            nnz += sub_nnz[s];
-            nact++;
-        }
-        scan[t] = cnt;
-        __syncthreads();
-        for (int off = 1; off < 256; off <<= 1) {
-            const int32_t v = t >= off ? scan[t - off] : 0;
-            __syncthreads();
-            scan[t] += v;
-            __syncthreads();
-        }
-        const int32_t excl = base + scan[t] - cnt;
-        for (int32_t k = 0; k < cnt; k++) achunk[excl + k] = scp[s] + k;
-        base += scan[255];
-        __syncthreads();
-    }
-    red_rows[t] = rows;
-    red_nnz[t] = nnz;
-    red_act[t] = nact;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (t < off) {
-            red_rows[t] += red_rows[t + off];
-            red_nnz[t] += red_nnz[t + off];
-            red_act[t] += red_act[t + off];
-        }
-        __syncthreads();
-    }
-    if (t == 0) {
-        if (accumulate) {
-            st->tot_rows += st->cur_rows;
-            st->tot_nnz += st->cur_nnz;
-            st->spmv_launches++;
-        }
-        st->cur_rows = red_rows[0];
-        st->cur_nnz = red_nnz[0];
-        st->nachunk = base;
-        st->nactive = red_act[0];
-        st->done = 0;
-    }
-}
-
-// start of a solve: active = need, counters cleared, work list of the needed sub-problems
-__global__ __launch_bounds__(256) void cg_begin_kernel(int nsub, const int32_t *__restrict__ need,
-                                                       int32_t *__restrict__ active,
-                                                       const int32_t *__restrict__ scp,
-                                                       const int64_t *__restrict__ dof_off,
-                                                       const int64_t *__restrict__ sub_nnz,
-                                                       int32_t *__restrict__ achunk, CgState *st) {
-    for (int s = threadIdx.x; s < nsub; s += 256) active[s] = need[s] ? 1 : 0;
-    if (threadIdx.x == 0) {
-        st->nfail = 0;
-        st->cur_rows = st->cur_nnz = 0;
-        st->tot_rows = st->tot_nnz = 0;
-        st->spmv_launches = 0;
-    }
-    __threadfence_block();
-    __syncthreads();
-    build_worklist(nsub, active, scp, dof_off, sub_nnz, achunk, st, false);
+static inline unsigned wl_grid(int64_t chunks) {
+    const int64_t A = std::max<int64_t>(1, chunks * kWlBlocksPerChunk);
+    return (unsigned)(((A + kXcd - 1) / kXcd) * kXcd);
 }
 
 // r = b, u = M b, x = p = s = 0; gamma and rho partials
 template <typename T>
-__global__ __launch_bounds__(256) void cg_init_kernel(const int32_t *__restrict__ achunk, const CgState *st,
+__global__ __launch_bounds__(256) void cg_init_kernel(const int32_t *__restrict__ achunk, int nachunk,
+                                                      const int32_t *__restrict__ chunk_sub,
+                                                      const int32_t *__restrict__ need,
                                                       const int64_t *__restrict__ cb,
                                                       const int64_t *__restrict__ ce,
                                                       const T *__restrict__ b, const T *__restrict__ jd,
@@ -1217,9 +1181,9 @@ __global__ __launch_bounds__(256) void cg_init_kernel(const int32_t *__restrict_
                                                       T *__restrict__ sv, double *__restrict__ part,
                                                       int64_t pstride) {
     int32_t c;
-    int part_i;
+    int sb, part_i;
     int64_t r0, r1;
-    if (!wl_block(achunk, st, cb, ce, c, part_i, r0, r1)) return;
+    if (!wl_block(achunk, nachunk, chunk_sub, need, cb, ce, c, sb, part_i, r0, r1)) return;
     const int64_t row = r0 + threadIdx.x;
     double g = 0.0, rho = 0.0;
     if (row < r1) {
@@ -1233,10 +1197,11 @@ __global__ __launch_bounds__(256) void cg_init_kernel(const int32_t *__restrict_
         g = double(ri) * double(ui);
         rho = double(ri) * double(ri);
     }
-    g = group_sum(g);
-    rho = group_sum(rho);
-    if ((threadIdx.x & 31) == 0) {
-        const int64_t slot = (int64_t)c * kGroupsPerChunk + part_i * (kWlRows / kGroup) + threadIdx.x / kGroup;
+    __shared__ double sg[2][kWlRows / 32];
+    g = block_sum(g, sg[0]);
+    rho = block_sum(rho, sg[1]);
+    if (threadIdx.x == 0) {
+        const int64_t slot = (int64_t)c * kWlBlocksPerChunk + part_i;
         part[slot] = g;
         part[pstride + slot] = rho;
     }
@@ -1244,7 +1209,9 @@ __global__ __launch_bounds__(256) void cg_init_kernel(const int32_t *__restrict_
 
 // w = A u on the work list; delta = (w, u) partials
 template <typename TV, typename T, int LPR>
-__global__ __launch_bounds__(256) void cg_spmv_kernel(const int32_t *__restrict__ achunk, const CgState *st,
+__global__ __launch_bounds__(256) void cg_spmv_kernel(const int32_t *__restrict__ achunk, int nachunk,
+                                                      const int32_t *__restrict__ chunk_sub,
+                                                      const int32_t *__restrict__ active,
                                                       const int64_t *__restrict__ cb,
                                                       const int64_t *__restrict__ ce,
                                                       const int32_t *__restrict__ rs,
@@ -1252,10 +1219,11 @@ __global__ __launch_bounds__(256) void cg_spmv_kernel(const int32_t *__restrict_
                                                       const TV *__restrict__ val, const T *__restrict__ u,
                                                       T *__restrict__ w, double *__restrict__ part_delta) {
     __shared__ double sp[kWlRows];
+    __shared__ double sg[kWlRows / 32];
     int32_t c;
-    int part_i;
+    int sb, part_i;
     int64_t r0, r1;
-    if (!wl_block(achunk, st, cb, ce, c, part_i, r0, r1)) return;
+    if (!wl_block(achunk, nachunk, chunk_sub, active, cb, ce, c, sb, part_i, r0, r1)) return;
     constexpr int RPP = kWlRows / LPR;
     const int lane = threadIdx.x % LPR;
     for (int pass = 0; pass < LPR; pass++) {
@@ -1274,94 +1242,80 @@ __global__ __launch_bounds__(256) void cg_spmv_kernel(const int32_t *__restrict_
         }
     }
     __syncthreads();
-    const double d = group_sum(sp[threadIdx.x]);
-    if ((threadIdx.x & 31) == 0)
-        part_delta[(int64_t)c * kGroupsPerChunk + part_i * (kWlRows / kGroup) + threadIdx.x / kGroup] = d;
+    const double d = block_sum(sp[threadIdx.x], sg);
+    if (threadIdx.x == 0) part_delta[(int64_t)c * kWlBlocksPerChunk + part_i] = d;
 }
 
-// One block per sub-problem: sums its gamma / rho / delta group partials and updates its
-// scalars. mode 0: start (bnorm, gamma; active = need and r != 0); mode 1: first
-// iteration (beta = 0); mode 2: later iterations. The last block to finish rebuilds the
-// work list from the new flags.
-__global__ __launch_bounds__(256) void cg_reduce_kernel(int nsub, int mode, double rtol,
-                                                        const int32_t *__restrict__ scp,
-                                                        const double *__restrict__ part, int64_t pstride,
-                                                        const int32_t *__restrict__ need,
-                                                        double *__restrict__ bnorm, double *__restrict__ gam,
-                                                        double *__restrict__ alpha, double *__restrict__ beta,
-                                                        int32_t *active, const int64_t *__restrict__ dof_off,
-                                                        const int64_t *__restrict__ sub_nnz,
-                                                        int32_t *__restrict__ achunk, CgState *st) {
-    __shared__ double red[3][256];
-    __shared__ int last;
+// One 64-lane block per sub-problem: sums its gamma / rho / delta block partials and
+// updates its scalars. mode 0: start (bnorm, gamma; active = need and r != 0); mode 1:
+// first iteration (beta = 0); mode 2: later iterations. Counts the iterations taken.
+__global__ __launch_bounds__(64) void cg_reduce_kernel(int mode, double rtol, const int32_t *__restrict__ scp,
+                                                       const double *__restrict__ part, int64_t pstride,
+                                                       const int32_t *__restrict__ need,
+                                                       double *__restrict__ bnorm, double *__restrict__ gam,
+                                                       double *__restrict__ alpha, double *__restrict__ beta,
+                                                       int32_t *__restrict__ active, int32_t *__restrict__ iters,
+                                                       int32_t *__restrict__ fail) {
     const int sb = blockIdx.x, t = threadIdx.x;
     const bool run = mode == 0 ? need[sb] != 0 : active[sb] != 0;
-    if (run) {
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        const int64_t g1 = (int64_t)scp[sb + 1] * kGroupsPerChunk;
-        for (int64_t g = (int64_t)scp[sb] * kGroupsPerChunk + t; g < g1; g += 256) {
-            a0 += part[g];
-            a1 += part[pstride + g];
-            a2 += part[2 * pstride + g];
+    if (!run) {
+        if (mode == 0 && t == 0) {
+            active[sb] = 0;
+            iters[sb] = 0;
+            fail[sb] = 0;
         }
-        red[0][t] = a0;
-        red[1][t] = a1;
-        red[2][t] = a2;
-        __syncthreads();
-        for (int off = 128; off > 0; off >>= 1) {
-            if (t < off) {
-                red[0][t] += red[0][t + off];
-                red[1][t] += red[1][t + off];
-                red[2][t] += red[2][t + off];
-            }
-            __syncthreads();
-        }
-        if (t == 0) {
-            const double g = red[0][0], rho = red[1][0], dl = red[2][0];
-            int a = 1;
-            if (mode == 0) {
-                bnorm[sb] = sqrt(rho);
-                gam[sb] = g;
-                a = rho > 0.0 && isfinite(rho);
-            } else if (!isfinite(rho) || !isfinite(g)) {
-                a = 0;
-                atomicAdd(&st->nfail, 1);
-            } else if (!(sqrt(rho) > rtol * bnorm[sb])) {
-                a = 0;   // converged
-            } else {
-                const double bt = mode == 1 ? 0.0 : g / gam[sb];
-                const double den = mode == 1 ? dl : dl - bt * g / alpha[sb];
-                if (!(g > 0.0) || !(den > 0.0) || !isfinite(den)) {
-                    a = 0;   // indefinite system or preconditioner
-                    atomicAdd(&st->nfail, 1);
-                } else {
-                    alpha[sb] = g / den;
-                    beta[sb] = bt;
-                    gam[sb] = g;
-                }
-            }
-            active[sb] = a;
-        }
-    } else if (mode == 0 && t == 0) {
-        active[sb] = 0;
+        return;
     }
-    __syncthreads();
-    if (t == 0) {
-        __threadfence();
-        last = atomicAdd(&st->done, 1) == nsub - 1;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const int64_t g1 = (int64_t)scp[sb + 1] * kWlBlocksPerChunk;
+    for (int64_t g = (int64_t)scp[sb] * kWlBlocksPerChunk + t; g < g1; g += 64) {
+        a0 += part[g];
+        a1 += part[pstride + g];
+        a2 += part[2 * pstride + g];
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    build_worklist(nsub, active, scp, dof_off, sub_nnz, achunk, st, mode != 0);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, 64);
+        a1 += __shfl_xor(a1, off, 64);
+        a2 += __shfl_xor(a2, off, 64);
+    }
+    if (t != 0) return;
+    const double g = a0, rho = a1, dl = a2;
+    int a = 1;
+    if (mode == 0) {
+        bnorm[sb] = sqrt(rho);
+        gam[sb] = g;
+        a = rho > 0.0 && isfinite(rho);
+        iters[sb] = 0;
+        fail[sb] = 0;
+    } else if (!isfinite(rho) || !isfinite(g)) {
+        a = 0;
+        fail[sb] = 1;
+    } else if (!(sqrt(rho) > rtol * bnorm[sb])) {
+        a = 0;   // converged
+    } else {
+        const double bt = mode == 1 ? 0.0 : g / gam[sb];
+        const double den = mode == 1 ? dl : dl - bt * g / alpha[sb];
+        if (!(g > 0.0) || !(den > 0.0) || !isfinite(den)) {
+            a = 0;   // indefinite system or preconditioner
+            fail[sb] = 1;
+        } else {
+            alpha[sb] = g / den;
+            beta[sb] = bt;
+            gam[sb] = g;
+            iters[sb] = iters[sb] + 1;
+        }
+    }
+    active[sb] = a;
 }
 
 // p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = M r; gamma, rho partials
 template <typename T>
-__global__ __launch_bounds__(256) void cg_update_kernel(const int32_t *__restrict__ achunk, const CgState *st,
+__global__ __launch_bounds__(256) void cg_update_kernel(const int32_t *__restrict__ achunk, int nachunk,
+                                                        const int32_t *__restrict__ chunk_sub,
+                                                        const int32_t *__restrict__ active,
                                                         const int64_t *__restrict__ cb,
                                                         const int64_t *__restrict__ ce,
-                                                        const int32_t *__restrict__ chunk_sub,
                                                         const double *__restrict__ alpha,
                                                         const double *__restrict__ beta,
                                                         const T *__restrict__ jd, const T *__restrict__ w,
@@ -1370,10 +1324,9 @@ __global__ __launch_bounds__(256) void cg_update_kernel(const int32_t *__restric
                                                         T *__restrict__ r, double *__restrict__ part,
                                                         int64_t pstride) {
     int32_t c;
-    int part_i;
+    int sb, part_i;
     int64_t r0, r1;
-    if (!wl_block(achunk, st, cb, ce, c, part_i, r0, r1)) return;
-    const int sb = chunk_sub[c];
+    if (!wl_block(achunk, nachunk, chunk_sub, active, cb, ce, c, sb, part_i, r0, r1)) return;
     const double a = alpha[sb], bt = beta[sb];
     const int64_t row = r0 + threadIdx.x;
     double g = 0.0, rho = 0.0;
@@ -1390,22 +1343,23 @@ __global__ __launch_bounds__(256) void cg_update_kernel(const int32_t *__restric
         g = double(ri) * double(ui);
         rho = double(ri) * double(ri);
     }
-    g = group_sum(g);
-    rho = group_sum(rho);
-    if ((threadIdx.x & 31) == 0) {
-        const int64_t slot = (int64_t)c * kGroupsPerChunk + part_i * (kWlRows / kGroup) + threadIdx.x / kGroup;
+    __shared__ double sg[2][kWlRows / 32];
+    g = block_sum(g, sg[0]);
+    rho = block_sum(rho, sg[1]);
+    if (threadIdx.x == 0) {
+        const int64_t slot = (int64_t)c * kWlBlocksPerChunk + part_i;
         part[slot] = g;
         part[pstride + slot] = rho;
     }
 }
 
 template <typename TV, typename T>
-static void launch_cg_spmv(int lpr, hipStream_t st, unsigned grid, KrylovWork *W, pyhip_problem *p,
-                           const TV *vals, const T *u, T *w, double *pd) {
+static void launch_cg_spmv(int lpr, hipStream_t st, unsigned grid, int nachunk, KrylovWork *W,
+                           pyhip_problem *p, const TV *vals, const T *u, T *w, double *pd) {
     const dim3 g(grid), t(256);
-#define CG_SPMV(L)                                                                                 \
-    hipLaunchKernelGGL((cg_spmv_kernel<TV, T, L>), g, t, 0, st, W->achunk, W->cgst, W->chunk_begin, \
-                       W->chunk_end, p->row_start, p->col, vals, u, w, pd)
+#define CG_SPMV(L)                                                                                  \
+    hipLaunchKernelGGL((cg_spmv_kernel<TV, T, L>), g, t, 0, st, W->achunk, nachunk, W->chunk_sub,    \
+                       W->active, W->chunk_begin, W->chunk_end, p->row_start, p->col, vals, u, w, pd)
     switch (lpr) {
     case 1: CG_SPMV(1); break;
     case 2: CG_SPMV(2); break;
@@ -1415,16 +1369,35 @@ static void launch_cg_spmv(int lpr, hipStream_t st, unsigned grid, KrylovWork *W
 #undef CG_SPMV
 }
 
-static int read_cg_state(pyhip_problem *p, KrylovWork *W) {
-    PYHIP_CHECK_HIP(hipMemcpyAsync(W->cgst_h, W->cgst, sizeof(CgState), hipMemcpyDeviceToHost, p->st));
+// Host side of the work list: reads flags[] (ns int32 from dev) and uploads the chunks of
+// the sub-problems with a non-zero flag. Returns the number of chunks.
+static int cg_refresh(pyhip_problem *p, KrylovWork *W, const int32_t *flags_dev, bool with_stats,
+                      int *nachunk, int *nactive) {
+    const int ns = W->nsub;
+    int32_t *f = W->flags_h;
+    PYHIP_CHECK_HIP(hipMemcpyAsync(f, flags_dev, ns * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+    if (with_stats) {
+        PYHIP_CHECK_HIP(hipMemcpyAsync(f + ns, W->cg_fail, ns * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+        PYHIP_CHECK_HIP(hipMemcpyAsync(f + 2 * ns, W->cg_iters, ns * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+    }
     PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    // the stream is idle here, so the pinned list can be rewritten
+    int n = 0, na = 0;
+    for (int s = 0; s < ns; s++) {
+        if (!f[s]) continue;
+        na++;
+        const int64_t c0 = W->chunk_ptr_h[s], c1 = W->chunk_ptr_h[s + 1];
+        for (int64_t c = c0; c < c1; c++) W->achunk_h[n++] = (int32_t)c;
+    }
+    if (n) PYHIP_CHECK_HIP(hipMemcpyAsync(W->achunk, W->achunk_h, n * sizeof(int32_t), hipMemcpyHostToDevice, p->st));
+    *nachunk = n;
+    *nactive = na;
     return PYHIP_OK;
 }
 
 // x = A^{-1} b for the sub-problems with need[s], Jacobi-preconditioned single-reduction
-// CG in precision T on values TV. The host reads the state every kCgCheck iterations;
-// launches in between size their grids from the last read (an upper bound, since the
-// work list only shrinks) and blocks past the current list exit at once.
+// CG in precision T on values TV. The host refreshes the work list every kCgCheck
+// iterations.
 constexpr int kCgCheck = 8;
 
 template <typename T, typename TV>
@@ -1433,13 +1406,13 @@ static int cg_worklist(pyhip_problem *p, KrylovWork *W, const TV *vals, const T 
                        int *failed) {
     const int64_t n = p->ndof;
     const int ns = W->nsub;
-    const int prec = std::is_same<T, float>::value ? PYHIP_F32 : PYHIP_F64;
     int rc;
     *iters = 0;
     *failed = 0;
     if (n == 0 || W->nchunk == 0) return PYHIP_OK;
-    PYHIP_REQUIRE(W->achunk && W->sub_nnz && W->gpart && W->cgst && W->cgst_h && W->active && need &&
-                      W->sub_chunk_ptr && p->dof_off && W->chunk_begin && W->chunk_end && W->chunk_sub,
+    PYHIP_REQUIRE(W->achunk && W->achunk_h && W->gpart && W->cg_iters && W->cg_fail && W->flags_h &&
+                      W->active && need && W->sub_chunk_ptr && W->chunk_begin && W->chunk_end &&
+                      W->chunk_sub && (int)W->chunk_ptr_h.size() == ns + 1,
                   "CG work-list buffers missing");
     if ((rc = ensure_basis<T>(W, n, std::max(W->m + 1, 3)))) return rc;
     T *pv = (T *)W->V, *sv = pv + n, *wv = pv + 2 * n;
@@ -1447,48 +1420,61 @@ static int cg_worklist(pyhip_problem *p, KrylovWork *W, const TV *vals, const T 
     T *u = std::is_same<T, float>::value ? (T *)W->t32 : (T *)W->t64;
     const T *jd = std::is_same<T, float>::value ? (const T *)W->jdiag32 : (const T *)W->jdiag64;
     if ((rc = spmv_tune<TV, T, T>(p, W, vals, b, wv))) return rc;
-    const int64_t pstride = W->nchunk * kGroupsPerChunk;
+    const int64_t pstride = W->nchunk * kWlBlocksPerChunk;
     double *pg = W->gpart, *pd = W->gpart + 2 * pstride;
-    const unsigned full = (unsigned)(W->nchunk * kWlBlocksPerChunk);
-    {
+    int nachunk = 0, nactive = 0;
+    if ((rc = cg_refresh(p, W, need, false, &nachunk, &nactive))) return rc;
+    if (nachunk) {
         TimeScope ts(&p->timer, "vector", p->st);
-        hipLaunchKernelGGL(cg_begin_kernel, dim3(1), dim3(256), 0, p->st, ns, need, W->active,
-                           W->sub_chunk_ptr, p->dof_off, W->sub_nnz, W->achunk, W->cgst);
-        hipLaunchKernelGGL((cg_init_kernel<T>), dim3(full), dim3(256), 0, p->st, W->achunk, W->cgst,
-                           W->chunk_begin, W->chunk_end, b, jd, x, r, u, pv, sv, pg, pstride);
-        hipLaunchKernelGGL(cg_reduce_kernel, dim3(ns), dim3(256), 0, p->st, ns, 0, rtol, W->sub_chunk_ptr,
-                           W->gpart, pstride, need, W->bnorm, W->cg_rz, W->cg_pq, W->cg_beta, W->active,
-                           p->dof_off, W->sub_nnz, W->achunk, W->cgst);
-        KCHECK();
+        hipLaunchKernelGGL((cg_init_kernel<T>), dim3(wl_grid(nachunk)), dim3(256), 0, p->st, W->achunk, nachunk,
+                           W->chunk_sub, need, W->chunk_begin, W->chunk_end, b, jd, x, r, u, pv, sv, pg, pstride);
     }
-    if ((rc = read_cg_state(p, W))) return rc;
+    hipLaunchKernelGGL(cg_reduce_kernel, dim3(ns), dim3(64), 0, p->st, 0, rtol, W->sub_chunk_ptr, W->gpart,
+                       pstride, need, W->bnorm, W->cg_rz, W->cg_pq, W->cg_beta, W->active, W->cg_iters,
+                       W->cg_fail);
+    KCHECK();
+    if ((rc = cg_refresh(p, W, W->active, false, &nachunk, &nactive))) return rc;
+    const std::vector<int32_t> started(W->flags_h, W->flags_h + ns);
     int it = 0;
-    while (W->cgst_h->nactive > 0 && it < o.max_iters) {
-        const unsigned grid = (unsigned)std::max<int64_t>(1, (int64_t)W->cgst_h->nachunk * kWlBlocksPerChunk);
+    int64_t rows_done = 0, nnz_done = 0;
+    while (nactive > 0 && it < o.max_iters) {
+        const unsigned grid = wl_grid(nachunk);
         const int batch = std::min(kCgCheck, o.max_iters - it);
         for (int k = 0; k < batch; k++, it++) {
             {
                 TimeScope ts(&p->timer, "spmv", p->st);
-                launch_cg_spmv<TV, T>(W->lpr, p->st, grid, W, p, vals, u, wv, pd);
+                launch_cg_spmv<TV, T>(W->lpr, p->st, grid, nachunk, W, p, vals, u, wv, pd);
             }
             TimeScope ts(&p->timer, "vector", p->st);
-            hipLaunchKernelGGL(cg_reduce_kernel, dim3(ns), dim3(256), 0, p->st, ns, it == 0 ? 1 : 2, rtol,
+            hipLaunchKernelGGL(cg_reduce_kernel, dim3(ns), dim3(64), 0, p->st, it == 0 ? 1 : 2, rtol,
                                W->sub_chunk_ptr, W->gpart, pstride, need, W->bnorm, W->cg_rz, W->cg_pq,
-                               W->cg_beta, W->active, p->dof_off, W->sub_nnz, W->achunk, W->cgst);
-            hipLaunchKernelGGL((cg_update_kernel<T>), dim3(grid), dim3(256), 0, p->st, W->achunk, W->cgst,
-                               W->chunk_begin, W->chunk_end, W->chunk_sub, W->cg_pq, W->cg_beta, jd, wv, u,
-                               pv, sv, x, r, pg, pstride);
+                               W->cg_beta, W->active, W->cg_iters, W->cg_fail);
+            hipLaunchKernelGGL((cg_update_kernel<T>), dim3(grid), dim3(256), 0, p->st, W->achunk, nachunk,
+                               W->chunk_sub, W->active, W->chunk_begin, W->chunk_end, W->cg_pq, W->cg_beta,
+                               jd, wv, u, pv, sv, x, r, pg, pstride);
         }
         KCHECK();
-        if ((rc = read_cg_state(p, W))) return rc;
+        if ((rc = cg_refresh(p, W, W->active, false, &nachunk, &nactive))) return rc;
     }
-    const CgState &S = *W->cgst_h;
-    p->timer.add_work("spmv", (int64_t)(sizeof(TV) + 4) * S.tot_nnz + (int64_t)(4 + 2 * sizeof(T)) * S.tot_rows);
+    // statistics: per-sub iterations (each one spmv over the sub-problem's rows) and failures
+    int nfail = 0, maxit = 0;
+    if ((rc = cg_refresh(p, W, W->active, true, &nachunk, &nactive))) return rc;
+    for (int s = 0; s < ns; s++) {
+        const int its = W->flags_h[2 * ns + s];
+        nfail += W->flags_h[ns + s] != 0;
+        maxit = std::max(maxit, its);
+        // every iteration is an spmv then a reduction; a sub-problem stopped by its
+        // reduction (converged or failed) ran one spmv more than its iteration count
+        const int64_t sp = its + (started[s] && !W->flags_h[s] ? 1 : 0);
+        rows_done += sp * (p->dof_off_h[s + 1] - p->dof_off_h[s]);
+        nnz_done += sp * W->sub_nnz_h[s];
+    }
+    p->timer.add_work("spmv", (int64_t)(sizeof(TV) + 4) * nnz_done + (int64_t)(4 + 2 * sizeof(T)) * rows_done);
     if (debug_enabled())
-        fprintf(stderr, "[pyhip] cg: %d iterations, %d still active, %d failed, %lld spmv launches with work\n",
-                it, S.nactive, S.nfail, (long long)S.spmv_launches);
+        fprintf(stderr, "[pyhip] cg: %d iterations (longest sub-problem %d), %d still active, %d failed\n", it,
+                maxit, nactive, nfail);
     *iters = it;
-    *failed = S.nfail + S.nactive;
+    *failed = nfail + nactive;
     return PYHIP_OK;
 }
 
@@ -1644,12 +1630,13 @@ void pyhip_krylov_free(pyhip_problem *p) {
                     W->cg_beta, W->nfail_d, W->ilu.perm, W->ilu.rs, W->ilu.col,
                     W->ilu.src, W->ilu.diag, W->ilu.lo_rows, W->ilu.up_rows, W->ilu.lu,
                     W->ilu.tmp64, W->ilu.tmp64b, W->ilu.tmp32, W->ilu.tmp32b, W->achunk,
-                    W->sub_nnz, W->gpart, W->cgst};
+                    W->gpart, W->cg_iters, W->cg_fail};
     for (void *q : ptrs)
         dfree(q);
     if (W->nactive_h) (void)hipHostFree(W->nactive_h);
     if (W->maxres_h) (void)hipHostFree(W->maxres_h);
-    if (W->cgst_h) (void)hipHostFree(W->cgst_h);
+    if (W->achunk_h) (void)hipHostFree(W->achunk_h);
+    if (W->flags_h) (void)hipHostFree(W->flags_h);
     delete W;
     p->kw = nullptr;
 }

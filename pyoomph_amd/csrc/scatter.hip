@@ -24,7 +24,7 @@ constexpr int kRowLanes = 32;
 constexpr int kRowsPerBlock = 256 / kRowLanes;
 constexpr int kGatherBlock = 256;
 
-// CSR values from element matrices: blocks [0, nb_vals) own one entry per thread; the
+// CSR values from element matrices: the value blocks own one entry per thread; the
 // blocks after them (when res is given) own one residual row per thread.
 template <typename TE, typename TV>
 __global__ __launch_bounds__(kGatherBlock) void gather_values_kernel(
@@ -32,9 +32,14 @@ __global__ __launch_bounds__(kGatherBlock) void gather_values_kernel(
     const uint32_t *__restrict__ cidx, const int64_t *__restrict__ occ_ptr,
     const int64_t *__restrict__ occ_roff, const TE *__restrict__ Ej, const TE *__restrict__ Er,
     TV *__restrict__ vals, double *__restrict__ res) {
-    if ((int64_t)blockIdx.x < nb_vals) {
-        const int64_t i = (int64_t)blockIdx.x * kGatherBlock + threadIdx.x;
-        if (i >= nnz) return;
+    // value blocks: the first nb_pad = 8 * per blocks, dealt round-robin over the 8 XCDs;
+    // XCD x takes the contiguous range x*per.. so neighbouring entries (which share
+    // elements) read the element matrices through the same L2
+    const int64_t per = (nb_vals + 7) / 8, nb_pad = 8 * per;
+    if ((int64_t)blockIdx.x < nb_pad) {
+        const int64_t lb = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+        const int64_t i = lb * kGatherBlock + threadIdx.x;
+        if (lb >= nb_vals || i >= nnz) return;
         TV acc = TV(0);
         for (int32_t c = cptr[i]; c < cptr[i + 1]; c++) {
             const TV v = TV(Ej[cidx[c]]);
@@ -43,7 +48,7 @@ __global__ __launch_bounds__(kGatherBlock) void gather_values_kernel(
         }
         vals[i] = acc;
     } else {
-        const int64_t r = ((int64_t)blockIdx.x - nb_vals) * kGatherBlock + threadIdx.x;
+        const int64_t r = ((int64_t)blockIdx.x - nb_pad) * kGatherBlock + threadIdx.x;
         if (r >= nrow) return;
         double acc = 0.0;
         for (int64_t o = occ_ptr[r]; o < occ_ptr[r + 1]; o++) acc = acc + double(Er[occ_roff[o]]);
@@ -116,9 +121,10 @@ template <typename T>
 static int launch_gather_values(const GatherMap &m, const T *Ej, const T *Er, T *vals, double *res,
                                 hipStream_t st) {
     const int64_t nb_vals = (m.nnz + kGatherBlock - 1) / kGatherBlock;
+    const int64_t nb_pad = 8 * ((nb_vals + 7) / 8);
     const int64_t nb_res = res ? (m.nrow + kGatherBlock - 1) / kGatherBlock : 0;
-    if (nb_vals + nb_res == 0) return PYHIP_OK;
-    hipLaunchKernelGGL((gather_values_kernel<T, T>), dim3((unsigned)(nb_vals + nb_res)),
+    if (nb_pad + nb_res == 0) return PYHIP_OK;
+    hipLaunchKernelGGL((gather_values_kernel<T, T>), dim3((unsigned)(nb_pad + nb_res)),
                        dim3(kGatherBlock), 0, st, m.nnz, m.nrow, nb_vals, m.cptr, m.cidx,
                        m.occ_ptr, m.occ_roff, Ej, Er, vals, res);
     PYHIP_CHECK_HIP(hipGetLastError());
