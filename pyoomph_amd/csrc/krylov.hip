@@ -34,38 +34,74 @@ namespace pyhip {
 __device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-// Row sum of val[k] * x[col[k]] over k in [k0, k1) in one canonical order: stripe t
-// (t = 0..7) accumulates the entries k0+t, k0+t+8, ... with fused multiply-adds, and the
-// stripes combine as ((S0+S4)+(S2+S6)) + ((S1+S5)+(S3+S7)). LPR lanes per row (1, 2, 4 or
-// 8; lane l owns stripes l, l+LPR, ...) all give the same bits, so the lane count can be
-// tuned per problem without changing any result. All LPR lanes of a row call this.
+// Row sums of val[k] * x[col[k]] in one canonical order: stripe t (t = 0..7) of a row
+// accumulates its entries k0+t, k0+t+8, ... with fused multiply-adds, and the stripes
+// combine as ((S0+S4)+(S2+S6)) + ((S1+S5)+(S3+S7)). LPR lanes per row (1, 2, 4 or 8;
+// lane l owns stripes l, l+LPR, ...) all give the same bits, so the lane count can be
+// tuned per problem without changing any result.
+// A thread serves LPR rows of a 256-row block (rows r0 + p*(256/LPR) + threadIdx.x/LPR,
+// p = 0..LPR-1) and issues the loads of all of them together: 8 (row, stripe) streams in
+// flight per thread, whatever LPR. All lanes of a row call this; out[p] is the sum of
+// row p (rows at or past r1: 0).
 template <typename TV, typename TX, typename TY, int LPR>
-__device__ __forceinline__ TY row_dot(int32_t k0, int32_t k1, int lane, const int32_t *__restrict__ col,
-                                      const TV *__restrict__ val, const TX *__restrict__ x) {
-    constexpr int NS = 8 / LPR;
-    TY acc[NS];
+__device__ __forceinline__ void rows_dot(int64_t r0, int64_t r1, const int32_t *__restrict__ rs,
+                                         const int32_t *__restrict__ col, const TV *__restrict__ val,
+                                         const TX *__restrict__ x, TY (&out)[LPR]) {
+    constexpr int NS = 8 / LPR, RPP = 256 / LPR;
+    const int lane = threadIdx.x % LPR;
+    int32_t k0[LPR], k1[LPR];
+    int32_t len = 0;
 #pragma unroll
-    for (int j = 0; j < NS; j++) acc[j] = TY(0);
-    for (int32_t b = k0 + lane; b < k1; b += 8) {
-#pragma unroll
-        for (int j = 0; j < NS; j++) {
-            const int32_t k = b + j * LPR;
-            if (k < k1) acc[j] = fmadd(TY(val[k]), TY(x[col[k]]), acc[j]);
-        }
+    for (int p = 0; p < LPR; p++) {
+        const int64_t row = r0 + p * RPP + threadIdx.x / LPR;
+        const bool ok = row < r1;
+        k0[p] = ok ? rs[row] : 0;
+        k1[p] = ok ? rs[row + 1] : 0;
+        len = max(len, k1[p] - k0[p]);
     }
-    TY v;
-    if constexpr (LPR == 8) v = acc[0];
-    else if constexpr (LPR == 4) v = acc[0] + acc[1];
-    else if constexpr (LPR == 2) v = (acc[0] + acc[2]) + (acc[1] + acc[3]);
-    else v = ((acc[0] + acc[4]) + (acc[2] + acc[6])) + ((acc[1] + acc[5]) + (acc[3] + acc[7]));
-    if constexpr (LPR >= 8) v = v + __shfl_xor(v, 4, LPR);
-    if constexpr (LPR >= 4) v = v + __shfl_xor(v, 2, LPR);
-    if constexpr (LPR >= 2) v = v + __shfl_xor(v, 1, LPR);
-    return v;
+    TY acc[LPR][NS];
+#pragma unroll
+    for (int p = 0; p < LPR; p++)
+#pragma unroll
+        for (int j = 0; j < NS; j++) acc[p][j] = TY(0);
+    for (int32_t base = lane; base < len; base += 8) {
+        int32_t cc[LPR][NS];
+        TY vv[LPR][NS], xx[LPR][NS];
+#pragma unroll
+        for (int p = 0; p < LPR; p++)
+#pragma unroll
+            for (int j = 0; j < NS; j++) {
+                const int32_t k = k0[p] + base + j * LPR;
+                const bool in = k < k1[p];
+                cc[p][j] = in ? col[k] : -1;
+                vv[p][j] = in ? TY(val[k]) : TY(0);
+            }
+#pragma unroll
+        for (int p = 0; p < LPR; p++)
+#pragma unroll
+            for (int j = 0; j < NS; j++) xx[p][j] = cc[p][j] >= 0 ? TY(x[cc[p][j]]) : TY(0);
+#pragma unroll
+        for (int p = 0; p < LPR; p++)
+#pragma unroll
+            for (int j = 0; j < NS; j++)
+                if (cc[p][j] >= 0) acc[p][j] = fmadd(vv[p][j], xx[p][j], acc[p][j]);
+    }
+#pragma unroll
+    for (int p = 0; p < LPR; p++) {
+        TY v;
+        if constexpr (LPR == 8) v = acc[p][0];
+        else if constexpr (LPR == 4) v = acc[p][0] + acc[p][1];
+        else if constexpr (LPR == 2) v = (acc[p][0] + acc[p][2]) + (acc[p][1] + acc[p][3]);
+        else v = ((acc[p][0] + acc[p][4]) + (acc[p][2] + acc[p][6])) + ((acc[p][1] + acc[p][5]) + (acc[p][3] + acc[p][7]));
+        if constexpr (LPR >= 8) v = v + __shfl_xor(v, 4, LPR);
+        if constexpr (LPR >= 4) v = v + __shfl_xor(v, 2, LPR);
+        if constexpr (LPR >= 2) v = v + __shfl_xor(v, 1, LPR);
+        out[p] = v;
+    }
 }
 
-// y = A x (or b - A x); 256 rows per block in LPR passes of 256/LPR rows; the grid is
-// a multiple of 8 (spmv_grid).
+// y = A x (or b - A x); 256 rows per block (rows_dot); the grid is a multiple of 8
+// (spmv_grid).
 template <typename TV, typename TX, typename TY, int LPR>
 __global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__restrict__ rs,
                                                    const int32_t *__restrict__ col,
@@ -73,18 +109,19 @@ __global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__r
                                                    const TX *__restrict__ x, TY *__restrict__ y,
                                                    const TY *__restrict__ b) {
     constexpr int RPP = 256 / LPR;
-    const int lane = threadIdx.x % LPR;
     // blocks round-robin over the 8 XCDs: XCD x gets the contiguous block range x*per..
     const int64_t nb = (n + 255) / 256, per = (nb + 7) / 8, k = blockIdx.x / 8;
     const int64_t lb = (int64_t)(blockIdx.x % 8) * per + k;
     if (k >= per || lb >= nb) return;
     const int64_t r0 = lb * 256;
-    for (int pass = 0; pass < LPR; pass++) {
-        const int64_t row = r0 + pass * RPP + threadIdx.x / LPR;
-        const bool ok = row < n;
-        const int32_t k0 = ok ? rs[row] : 0, k1 = ok ? rs[row + 1] : 0;
-        const TY v = row_dot<TV, TX, TY, LPR>(k0, k1, lane, col, val, x);
-        if (ok && lane == 0) y[row] = b ? b[row] - v : v;
+    TY v[LPR];
+    rows_dot<TV, TX, TY, LPR>(r0, n, rs, col, val, x, v);
+    if (threadIdx.x % LPR == 0) {
+#pragma unroll
+        for (int p = 0; p < LPR; p++) {
+            const int64_t row = r0 + p * RPP + threadIdx.x / LPR;
+            if (row < n) y[row] = b ? b[row] - v[p] : v[p];
+        }
     }
 }
 
@@ -1225,18 +1262,17 @@ __global__ __launch_bounds__(256) void cg_spmv_kernel(const int32_t *__restrict_
     int64_t r0, r1;
     if (!wl_block(achunk, nachunk, chunk_sub, active, cb, ce, c, sb, part_i, r0, r1)) return;
     constexpr int RPP = kWlRows / LPR;
-    const int lane = threadIdx.x % LPR;
-    for (int pass = 0; pass < LPR; pass++) {
-        const int loc = pass * RPP + threadIdx.x / LPR;
-        const int64_t row = r0 + loc;
-        const bool ok = row < r1;
-        const int32_t k0 = ok ? rs[row] : 0, k1 = ok ? rs[row + 1] : 0;
-        const T v = row_dot<TV, T, T, LPR>(k0, k1, lane, col, val, u);
-        if (lane == 0) {
+    T v[LPR];
+    rows_dot<TV, T, T, LPR>(r0, r1, rs, col, val, u, v);
+    if (threadIdx.x % LPR == 0) {
+#pragma unroll
+        for (int p = 0; p < LPR; p++) {
+            const int loc = p * RPP + threadIdx.x / LPR;
+            const int64_t row = r0 + loc;
             double d = 0.0;
-            if (ok) {
-                w[row] = v;
-                d = double(u[row]) * double(v);
+            if (row < r1) {
+                w[row] = v[p];
+                d = double(u[row]) * double(v[p]);
             }
             sp[loc] = d;
         }
