@@ -230,19 +230,23 @@ __global__ void sqrt_copy_kernel(int n, const double *__restrict__ a, double *__
 
 // Start of a cycle: beta = |r|, g = (beta, 0, ...), active if the sub-problem still has
 // work (need[s]) and beta > tol * |b|.
+// Start of a restart cycle. With stag > 0 (inner solves of iterative refinement), a
+// sub-problem whose last cycle reduced |r| by less than the factor stag stops: its
+// precision floor is reached and the outer refinement takes over.
 __global__ void gmres_start_kernel(int nsub, int m, const double *__restrict__ nrm2,
-                                   const double *__restrict__ bnorm, double tol,
+                                   const double *__restrict__ bnorm, double tol, double stag, int first,
                                    const int32_t *__restrict__ need, double *__restrict__ g,
                                    int32_t *__restrict__ active, int32_t *__restrict__ kcols,
                                    double *__restrict__ beta0, int32_t *__restrict__ nactive) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsub) return;
     const double beta = sqrt(nrm2[s]);
+    const bool stalled = !first && stag > 0.0 && !(beta < stag * beta0[s]);
     kcols[s] = 0;
     beta0[s] = beta;
     for (int i = 0; i <= m; i++) g[s * (m + 1) + i] = 0.0;
     g[s * (m + 1)] = beta;
-    const int a = need[s] && (beta > tol * bnorm[s]) && isfinite(beta);
+    const int a = need[s] && (beta > tol * bnorm[s]) && isfinite(beta) && !stalled;
     active[s] = a;
     if (a) atomicAdd(nactive, 1);
 }
@@ -861,7 +865,7 @@ static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals
 // b, x: device vectors of type T. Returns the iterations of the longest-running one.
 template <typename T, typename TV>
 static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T *x,
-                 const pyhip_linear_opts &o, double rtol, const int32_t *need, int *iters,
+                 const pyhip_linear_opts &o, double rtol, double stag, const int32_t *need, int *iters,
                  int *failed) {
     const int64_t n = p->ndof;
     const int m = W->m, ns = W->nsub;
@@ -886,7 +890,8 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
         if ((rc = batched_dots<T>(p, W, 1, V, n, V, d3))) return rc;
         PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
         hipLaunchKernelGGL(gmres_start_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, m, d3,
-                           W->bnorm, rtol, need, W->g, W->active, W->kcols, W->beta0, W->nactive_d);
+                           W->bnorm, rtol, stag, cyc == 0 ? 1 : 0, need, W->g, W->active, W->kcols, W->beta0,
+                           W->nactive_d);
         KCHECK();
         PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
         PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
@@ -943,7 +948,7 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
     if ((rc = batched_dots<T>(p, W, 1, V, n, V, d3))) return rc;
     PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
     hipLaunchKernelGGL(gmres_start_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, m, d3, W->bnorm,
-                       rtol * 1.01, need, W->g, W->active, W->kcols, W->beta0, W->nactive_d);
+                       rtol * 1.01, 0.0, 1, need, W->g, W->active, W->kcols, W->beta0, W->nactive_d);
     KCHECK();
     PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
     PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
@@ -1530,13 +1535,17 @@ __global__ void newton_state_kernel(int nsub, const int32_t *__restrict__ runnin
 // Solve J d = r for the masked sub-problems. r: FP64 right-hand side; d: FP64 result.
 template <typename T, typename TV>
 static int krylov(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T *x,
-                  const pyhip_linear_opts &o, double rtol, const int32_t *need, int *iters,
+                  const pyhip_linear_opts &o, double rtol, double stag, const int32_t *need, int *iters,
                   int *failed) {
     if (o.krylov == PYHIP_KRYLOV_CG && o.precond == PYHIP_PRECOND_JACOBI)
         return cg_worklist<T, TV>(p, W, vals, b, x, o, rtol, need, iters, failed);
     if (o.krylov == PYHIP_KRYLOV_CG) return cg_generic<T, TV>(p, W, vals, b, x, o, rtol, need, iters, failed);
-    return gmres<T, TV>(p, W, vals, b, x, o, rtol, need, iters, failed);
+    return gmres<T, TV>(p, W, vals, b, x, o, rtol, stag, need, iters, failed);
 }
+
+// An FP32 inner GMRES of the refinement stops a sub-problem whose restart cycle reduced the
+// residual by less than this factor (FP32 floor reached; the FP64 refinement continues).
+constexpr double kInnerStagnation = 0.9;
 
 static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts &o, int jac_prec,
                         const double *r, double *d, const int32_t *mask, int *iters,
@@ -1549,7 +1558,7 @@ static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts
     if (o.precision == PYHIP_F64) {
         PYHIP_REQUIRE(jac_prec == PYHIP_F64, "an FP64 Krylov solve needs the FP64 Jacobian");
         if ((rc = precond_setup<double, double>(p, W, o.precond, W->vals64))) return rc;
-        return krylov<double, double>(p, W, W->vals64, r, d, o, o.rtol, mask, iters, failed);
+        return krylov<double, double>(p, W, W->vals64, r, d, o, o.rtol, 0.0, mask, iters, failed);
     }
     // FP32 inner solves inside FP64 iterative refinement:
     //   d = 0; repeat: s = A32^{-1} rr (FP32, to rtol_f32); d += s; rr = r - J d (FP64)
@@ -1576,7 +1585,8 @@ static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts
         hipLaunchKernelGGL((convert_kernel<double, float>), dim3(nblk(n)), dim3(256), 0, p->st, n, W->r64, W->r32);
         KCHECK();
         int its = 0, f = 0;
-        if ((rc = krylov<float, float>(p, W, W->vals32, W->r32, W->d32, o, o.rtol_f32, W->need, &its, &f)))
+        if ((rc = krylov<float, float>(p, W, W->vals32, W->r32, W->d32, o, o.rtol_f32, kInnerStagnation,
+                                       W->need, &its, &f)))
             return rc;
         *iters += its;
         (*refinements)++;
@@ -1618,7 +1628,7 @@ int plugin_solve(pyhip_problem *p, const pyhip_linear_opts &o, const double *b, 
     int32_t one = 1;
     int failed = 0;
     PYHIP_CHECK_HIP(hipMemcpy(W->dof_mask, &one, sizeof(int32_t), hipMemcpyHostToDevice));
-    int rc = krylov<double, double>(p, W, W->vals64, b, x, o, o.rtol, W->dof_mask, iters, &failed);
+    int rc = krylov<double, double>(p, W, W->vals64, b, x, o, o.rtol, 0.0, W->dof_mask, iters, &failed);
     if (rc) return rc;
     if (failed) {
         set_error("Krylov solver did not reach its tolerance");
