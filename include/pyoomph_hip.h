@@ -100,7 +100,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
                          pyhip_problem **out);
 int pyhip_problem_destroy(pyhip_problem *p);
 /* ndof, nnz of the structural (column-sorted) pattern, number of element contributions,
- * bytes of device memory held. */
+ * bytes of device memory held (problem data plus the solver workspace allocated so far). */
 int pyhip_problem_info(const pyhip_problem *p, int64_t *ndof, int64_t *nnz,
                        int64_t *ncontrib, int64_t *device_bytes);
 /* dof offset of each sub-problem, host [nsub+1] */

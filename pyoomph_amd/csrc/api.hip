@@ -276,6 +276,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
 }
 
 void pyhip_krylov_free(pyhip_problem *p);
+int64_t pyhip_krylov_bytes(const pyhip_problem *p);
 
 int pyhip_problem_destroy(pyhip_problem *p) {
     if (!p) return PYHIP_OK;
@@ -305,7 +306,7 @@ int pyhip_problem_info(const pyhip_problem *p, int64_t *ndof, int64_t *nnz, int6
     if (ndof) *ndof = p->ndof;
     if (nnz) *nnz = p->nnz;
     if (ncontrib) *ncontrib = p->ncontrib;
-    if (device_bytes) *device_bytes = p->device_bytes;
+    if (device_bytes) *device_bytes = p->device_bytes + pyhip_krylov_bytes(p);
     return PYHIP_OK;
 }
 

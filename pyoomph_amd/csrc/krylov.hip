@@ -462,9 +462,8 @@ __global__ __launch_bounds__(256) void permute_kernel(int64_t n, const int32_t *
 static inline unsigned nblk(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
 static bool debug_enabled() {
-    static int v = -1;
-    if (v < 0) v = getenv("PYHIP_DEBUG") ? 1 : 0;
-    return v == 1;
+    static const bool v = getenv("PYHIP_DEBUG") != nullptr;
+    return v;
 }
 
 #define KCHECK() PYHIP_CHECK_HIP(hipGetLastError())
@@ -1664,6 +1663,8 @@ void pyhip_newton_opts_default(pyhip_newton_opts *o) {
     o->jacobian_precision = PYHIP_F64;
     pyhip_linear_opts_default(&o->linear);
 }
+
+int64_t pyhip_krylov_bytes(const pyhip_problem *p) { return p->kw ? p->kw->bytes : 0; }
 
 void pyhip_krylov_free(pyhip_problem *p) {
     KrylovWork *W = p->kw;

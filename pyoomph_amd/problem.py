@@ -141,7 +141,10 @@ class Problem:
             d.nparams = int(par.size)
             d.aux = aux.ctypes.data_as(_lib.c_dp) if aux is not None else None
         torch.cuda.set_device(self.device)
-        stream = torch.cuda.current_stream(self.device).cuda_stream
+        # the problem runs on the stream current at creation; every host wait below is on
+        # this stream only, so problems on other streams keep running
+        self._stream = torch.cuda.current_stream(self.device)
+        stream = self._stream.cuda_stream
         h = ctypes.c_void_p()
         _lib.check(L.pyhip_problem_create(descs, len(self.parts), ctypes.c_void_p(stream),
                                           ctypes.byref(h)))
@@ -150,10 +153,18 @@ class Problem:
         _lib.check(L.pyhip_problem_info(h, ctypes.byref(ndof), ctypes.byref(nnz), ctypes.byref(nc),
                                         ctypes.byref(nb)))
         self.ndof, self.nnz, self.ncontrib = ndof.value, nnz.value, nc.value
+        self.device_bytes = nb.value
         off = np.zeros(len(self.parts) + 1, dtype=np.int64)
         _lib.check(L.pyhip_problem_dof_offsets(h, off.ctypes.data_as(_lib.c_i64p)))
         self.dof_offsets = off
-        self.dofs = torch.zeros(max(self.ndof, 1), dtype=torch.float64, device=self.device)
+        with torch.cuda.stream(self._stream):
+            self.dofs = torch.zeros(max(self.ndof, 1), dtype=torch.float64, device=self.device)
+        self._stream.synchronize()
+
+    def _host(self, t):
+        """Copy a device tensor of this problem to a numpy array (waits on its stream)."""
+        with self.torch.cuda.stream(self._stream):
+            return t.cpu().numpy()
 
     def __del__(self):
         try:
@@ -165,13 +176,15 @@ class Problem:
 
     # -- dofs -------------------------------------------------------------------------
     def get_current_dofs(self):
-        return self.dofs[:self.ndof].cpu().numpy().copy()
+        return self._host(self.dofs[:self.ndof]).copy()
 
     def set_current_dofs(self, x):
         x = np.asarray(x, dtype=np.float64)
         if x.shape != (self.ndof,):
             raise ValueError("expected %d dofs" % self.ndof)
-        self.dofs[:self.ndof].copy_(self.torch.from_numpy(x))
+        with self.torch.cuda.stream(self._stream):
+            self.dofs[:self.ndof].copy_(self.torch.from_numpy(x))
+        self._stream.synchronize()
 
     def sub_dofs(self, i, x=None):
         x = self.get_current_dofs() if x is None else x
@@ -182,12 +195,12 @@ class Problem:
         """Residual vector (oomph Problem::get_residuals); ``dparameter="lambda"`` gives
         dR/dlambda (get_derivative_wrt_global_parameter, src/problem.cpp:922)."""
         torch = self.torch
-        res = torch.empty(max(self.ndof, 1), dtype=torch.float64, device=self.device)
+        with torch.cuda.stream(self._stream):
+            res = torch.empty(max(self.ndof, 1), dtype=torch.float64, device=self.device)
         deriv = -1 if dparameter is None else self._param_index(dparameter)
         _lib.check(_lib.load().pyhip_assemble(self._handle, _lib.ptr(self.dofs), 0, deriv, _lib.F64,
                                                _lib.ptr(res), None))
-        torch.cuda.synchronize(self.device)
-        return res[:self.ndof].cpu().numpy()
+        return self._host(res[:self.ndof])
 
     def _param_index(self, name):
         if name != "lambda":
@@ -198,9 +211,10 @@ class Problem:
         """Residual and column-sorted CSR values on the device (torch tensors)."""
         torch = self.torch
         prec = _lib.F64 if precision == "f64" else _lib.F32
-        res = torch.empty(max(self.ndof, 1), dtype=torch.float64, device=self.device)
-        vals = torch.empty(max(self.nnz, 1), dtype=torch.float64 if prec == _lib.F64 else torch.float32,
-                           device=self.device)
+        with torch.cuda.stream(self._stream):
+            res = torch.empty(max(self.ndof, 1), dtype=torch.float64, device=self.device)
+            vals = torch.empty(max(self.nnz, 1), dtype=torch.float64 if prec == _lib.F64 else torch.float32,
+                               device=self.device)
         deriv = -1 if dparameter is None else self._param_index(dparameter)
         _lib.check(_lib.load().pyhip_assemble(self._handle, _lib.ptr(self.dofs), 1, deriv, prec,
                                                _lib.ptr(res), _lib.ptr(vals)))
@@ -226,21 +240,22 @@ class Problem:
             res = res_d
         else:
             rs, col, val = self._export(vals)
-        R = res[:self.ndof].cpu().numpy()
+        R = self._host(res[:self.ndof])
         J = scipy.sparse.csr_matrix((val, col, rs), shape=(self.ndof, self.ndof))
         return R, J
 
     def _export(self, vals):
         torch = self.torch
-        rs = torch.empty(self.ndof + 1, dtype=torch.int32, device=self.device)
-        col = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=self.device)
-        val = torch.empty(max(self.nnz, 1), dtype=torch.float64, device=self.device)
+        with torch.cuda.stream(self._stream):
+            rs = torch.empty(self.ndof + 1, dtype=torch.int32, device=self.device)
+            col = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=self.device)
+            val = torch.empty(max(self.nnz, 1), dtype=torch.float64, device=self.device)
         nnz = ctypes.c_int64()
         _lib.check(_lib.load().pyhip_export_reference_csr(self._handle, _lib.ptr(vals), _lib.ptr(rs),
                                                           _lib.ptr(col), _lib.ptr(val),
                                                           ctypes.byref(nnz)))
         n = nnz.value
-        return rs.cpu().numpy(), col[:n].cpu().numpy(), val[:n].cpu().numpy()
+        return self._host(rs), self._host(col[:n]), self._host(val[:n])
 
     # -- Newton -----------------------------------------------------------------------
     def newton_options(self):
@@ -271,7 +286,7 @@ class Problem:
         o = self.newton_options()
         stats = (_lib.NewtonStats * len(self.parts))()
         rc = _lib.load().pyhip_newton_solve(self._handle, _lib.ptr(self.dofs), ctypes.byref(o), stats)
-        self.torch.cuda.synchronize(self.device)
+        self._stream.synchronize()
         out = [{"newton_iters": s.newton_iters, "linear_iters": s.linear_iters,
                 "linear_failures": s.linear_failures,
                 "converged": bool(s.converged), "final_maxres": s.final_maxres,
