@@ -45,6 +45,8 @@ def parse():
                     help="f64: FP64 Jacobian and Krylov; mixed: FP32 element kernels for the "
                          "Jacobian and FP32 inner CG inside FP64 iterative refinement (FP64 "
                          "residuals, updates and norms)")
+    ap.add_argument("--precond", choices=("jacobi", "amg"), default="jacobi",
+                    help="CG preconditioner: Jacobi, or the smoothed-aggregation multigrid V-cycle")
     ap.add_argument("--err-sample", type=int, default=6,
                     help="problems (spread over the size range, largest included) checked against "
                          "the FP64 oracle")
@@ -90,9 +92,9 @@ def timing_bytes(P, fam):
     return b.value
 
 
-def configure(P, precision="f64"):
+def configure(P, precision="f64", precond="jacobi"):
     P.krylov = "cg"
-    P.preconditioner = "jacobi"
+    P.preconditioner = precond
     P.jacobian_precision = "f32" if precision == "mixed" else "f64"
     P.linear_precision = "f32" if precision == "mixed" else "f64"
     P.gmres_max_iters = 20000
@@ -119,7 +121,7 @@ def main():
     specs = wl.bratu_sweep_specs(args.nprob, seed=args.seed)
     parts = [wl.bratu(N, sp, lam) for N, sp, lam in specs]
     P = Problem(parts)
-    configure(P, args.precision)
+    configure(P, args.precision, args.precond)
     print("# batch: %d problems, %d dofs, %d nnz" % (len(parts), P.ndof, P.nnz), file=sys.stderr, flush=True)
 
     def step():
@@ -183,7 +185,7 @@ def main():
         t_single, n_single = 0.0, 0
         for N, sp, lam in specs[:8]:
             Q = Problem(wl.bratu(N, sp, lam))
-            configure(Q, "f64")
+            configure(Q, "f64", "jacobi")
             torch.cuda.synchronize()
             t = time.perf_counter()
             Q.solve()
@@ -251,10 +253,10 @@ def main():
         "config": {"workload": "bratu_sweep: %d independent -lap u = lambda e^u problems, "
                                "N~U{16..128}, P1/P2 50/50, lambda~U(0.5,3)" % args.nprob,
                    "problems": args.nprob, "dofs": P.ndof, "nnz": P.nnz,
-                   "solver": ("Newton + batched CG/Jacobi, FP64" if args.precision == "f64" else
+                   "solver": ("Newton + batched CG/%s, FP64" % args.precond if args.precision == "f64" else
                               "Newton (FP64 residual) + FP32 Jacobian, FP32 batched CG/Jacobi inside FP64 "
                               "iterative refinement"),
-                   "precision": args.precision, "seed": args.seed},
+                   "precision": args.precision, "preconditioner": args.precond, "seed": args.seed},
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
         "max_rel_err_vs_oracle": max_err,

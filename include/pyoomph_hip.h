@@ -142,6 +142,7 @@ int pyhip_export_reference_csr(pyhip_problem *p, const double *values_dev,
  * pyoomph/solvers/scipy.py:47-104). */
 #define PYHIP_PRECOND_JACOBI 0
 #define PYHIP_PRECOND_ILU0 1
+#define PYHIP_PRECOND_AMG 2    /* smoothed-aggregation multigrid V-cycle (FP64 solves) */
 #define PYHIP_KRYLOV_GMRES 0
 #define PYHIP_KRYLOV_CG 1    /* symmetric positive definite Jacobians (Poisson, Bratu
                                 below its fold, reaction-diffusion with u^3)           */

@@ -1,0 +1,53 @@
+// Smoothed-aggregation multigrid preconditioner: device data and entry points (amg.hip).
+#pragma once
+#include <vector>
+
+#include "amg_plan.h"
+#include "common.h"
+
+namespace pyhip {
+
+constexpr int kAggregateCap = 12;      // neighbours an aggregate seed takes at most
+constexpr int kBottomRows = 48;        // a sub-problem stops coarsening at this many rows
+constexpr int kBottomMaxRows = 64;     // dense Cholesky size of the coarsest solve (LDS)
+constexpr int kMaxLevels = 16;
+
+struct AmgDevLevel {
+    int64_t n = 0, nc = 0, nnz = 0, nnzp = 0, nnzq = 0;
+    int32_t *rs = nullptr, *col = nullptr, *diag = nullptr, *agg = nullptr;
+    int32_t *prs = nullptr, *pcol = nullptr, *pcptr = nullptr, *pcidx = nullptr;
+    int32_t *trs = nullptr, *trow = nullptr, *tsrc = nullptr, *qrs = nullptr, *qcol = nullptr;
+    int32_t *row_sub = nullptr;
+    int64_t *sub_off = nullptr;
+    int32_t *cont = nullptr;           // [nsub] 1: the sub-problem continues below this level
+    int32_t *ends = nullptr;           // sub-problems solved on this level (coarsest solve)
+    int nends = 0;
+    const double *vals = nullptr;      // A_l values (level 0: the problem's)
+    double *own_vals = nullptr;        // A_l values computed here (levels > 0)
+    double *d = nullptr, *w = nullptr, *pv = nullptr, *qv = nullptr, *ptv = nullptr;
+    double *b = nullptr, *x = nullptr, *x2 = nullptr, *r = nullptr, *pw = nullptr, *py = nullptr;
+};
+
+struct AmgDev {
+    bool built = false;
+    int nsub = 0;
+    int max_bot_rows = 0;
+    std::vector<AmgDevLevel> lv;       // levels with a coarser level
+    AmgDevLevel bot;                   // coarsest level
+    double *nrm = nullptr;             // [2*nsub] power-iteration norms
+    double *chol = nullptr;            // [nsub][kBottomMaxRows^2] coarsest-level Cholesky factors
+    int32_t *chol_fail = nullptr;      // [nsub] 1 if a pivot was not positive
+};
+
+// plan (host) and device buffers, once per pattern; sizes counted in `bytes`
+int amg_build(AmgDev &A, int64_t n, const int32_t *rs_h, const int32_t *col_h,
+              const std::vector<int64_t> &sub_off, int64_t &bytes);
+// numeric set-up for the matrix values `vals` (level-0 pattern)
+int amg_setup(AmgDev &A, hipStream_t st, const double *vals);
+// the finest level's values (the current matrix; the rest of the hierarchy is kept)
+int amg_set_fine(AmgDev &A, const double *vals);
+// z = M^-1 b
+int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z);
+void amg_free(AmgDev &A);
+
+}  // namespace pyhip

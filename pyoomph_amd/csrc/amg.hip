@@ -1,0 +1,487 @@
+// Smoothed-aggregation multigrid preconditioner on the device (FP64).
+//
+// Symbolic plan: amg_plan.h (host, once per pattern). For every new matrix (each Newton
+// step) the device computes, level by level:
+//   d = diag(A_l); lambda_s ~ lambda_max(D^-1 A_l) of every sub-problem s by ten power
+//   iterations from a fixed oscillating start; w_i = 4 / (3 * 1.1 * lambda_s) / d_i;
+//   P = (I - W A_l) P_t; Q = A_l P; A_{l+1} = P^T Q   (fixed summation orders).
+// One application z = M^-1 b is a V-cycle: on each level one weighted-Jacobi sweep from
+// zero (x = W b), the residual restricted by P^T, the coarse correction prolonged by P,
+// one weighted-Jacobi sweep; a sub-problem's coarsest level (the level where it stops
+// coarsening, at most kBottomRows rows) is solved exactly with its dense Cholesky factor,
+// computed in the set-up, in one workgroup.
+// Every operation is row-local or per sub-problem, and every sub-problem has its own
+// hierarchy, so its preconditioner is the same alone and in a batch; the V-cycle is a
+// symmetric operator (usable inside conjugate gradients).
+#include <algorithm>
+#include <cstring>
+
+#include "amg.h"
+#include "spmv.h"
+
+namespace pyhip {
+
+constexpr int kPowerIters = 10;
+constexpr double kLambdaSafety = 1.1;
+constexpr int kCholThreads = 64;
+
+static inline unsigned ablk(int64_t n, int per = 256) { return (unsigned)std::max<int64_t>(1, (n + per - 1) / per); }
+
+// ---- numeric set-up kernels --------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void amg_diag_kernel(int64_t n, const int32_t *__restrict__ diag,
+                                                       const double *__restrict__ vals, double *__restrict__ d) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) d[i] = vals[diag[i]];
+}
+
+// fixed oscillating start vector of the power iteration (depends on the row's index
+// within its sub-problem only)
+__global__ __launch_bounds__(256) void amg_power_start_kernel(int64_t n, const int32_t *__restrict__ row_sub,
+                                                              const int64_t *__restrict__ sub_off,
+                                                              double *__restrict__ v) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = (uint32_t)(i - sub_off[row_sub[i]]);
+    uint32_t h = k * 2654435761u;
+    h ^= h >> 15;
+    v[i] = ((h & 1u) ? 1.0 : -1.0) * (1.0 + (double)((h >> 8) & 7u) * 0.125);
+}
+
+// per sub-problem, with y = A v: out[2s] = (D^-1 y, D^-1 y), out[2s+1] = (v, v); one
+// block per sub-problem, strided rows then a fixed tree
+__global__ __launch_bounds__(256) void amg_sub_norms_kernel(const int64_t *__restrict__ sub_off,
+                                                            const double *__restrict__ d,
+                                                            const double *__restrict__ y,
+                                                            const double *__restrict__ v,
+                                                            double *__restrict__ out) {
+    __shared__ double sy[256], sv[256];
+    const int s = blockIdx.x, t = threadIdx.x;
+    double ay = 0.0, av = 0.0;
+    for (int64_t i = sub_off[s] + t; i < sub_off[s + 1]; i += 256) {
+        const double yi = y[i] / d[i];
+        ay += yi * yi;
+        av += v[i] * v[i];
+    }
+    sy[t] = ay;
+    sv[t] = av;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) {
+            sy[t] += sy[t + off];
+            sv[t] += sv[t + off];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[2 * s] = sy[0];
+        out[2 * s + 1] = sv[0];
+    }
+}
+
+// v = D^-1 y / |D^-1 y|_s
+__global__ __launch_bounds__(256) void amg_power_scale_kernel(int64_t n, const int32_t *__restrict__ row_sub,
+                                                              const double *__restrict__ nrm,
+                                                              const double *__restrict__ d,
+                                                              const double *__restrict__ y, double *__restrict__ v) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double yy = nrm[2 * row_sub[i]];
+    v[i] = yy > 0.0 ? (y[i] / d[i]) / sqrt(yy) : 0.0;
+}
+
+// w_i = 4 / (3 * safety * lambda_s) / d_i, lambda_s = sqrt((y,y) / (v,v)) of the last step
+__global__ __launch_bounds__(256) void amg_weights_kernel(int64_t n, const int32_t *__restrict__ row_sub,
+                                                          const double *__restrict__ nrm,
+                                                          const double *__restrict__ d, double *__restrict__ w) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int s = row_sub[i];
+    const double vv = nrm[2 * s + 1];
+    const double lam = vv > 0.0 ? sqrt(nrm[2 * s] / vv) : 1.0;
+    const double om = (lam > 0.0 && isfinite(lam)) ? 4.0 / (3.0 * kLambdaSafety * lam) : 0.0;
+    w[i] = (d[i] != 0.0 && isfinite(d[i])) ? om / d[i] : 0.0;
+}
+
+// P[q] = [pcol(q) == agg(i)] - w_i * (sum of the A entries listed for q)
+__global__ __launch_bounds__(256) void amg_pvals_kernel(int64_t n, const int32_t *__restrict__ prs,
+                                                        const int32_t *__restrict__ pcol,
+                                                        const int32_t *__restrict__ pcptr,
+                                                        const int32_t *__restrict__ pcidx,
+                                                        const int32_t *__restrict__ agg,
+                                                        const double *__restrict__ vals,
+                                                        const double *__restrict__ w, double *__restrict__ pv) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double wi = w[i];
+    const int32_t ai = agg[i];
+    for (int32_t q = prs[i]; q < prs[i + 1]; q++) {
+        double s = 0.0;
+        for (int32_t c = pcptr[q]; c < pcptr[q + 1]; c++) s += vals[pcidx[c]];
+        pv[q] = (pcol[q] == ai ? 1.0 : 0.0) - wi * s;
+    }
+}
+
+// position of column j in the sorted row [b, e) of cols, or -1
+__device__ __forceinline__ int32_t find_col(const int32_t *__restrict__ cols, int32_t b, int32_t e, int32_t j) {
+    while (b < e) {
+        const int32_t m = (b + e) >> 1;
+        const int32_t c = cols[m];
+        if (c == j) return m;
+        if (c < j) b = m + 1;
+        else e = m;
+    }
+    return -1;
+}
+
+// Q[i][J] = sum over k in row i (increasing) of A[i][k] P[k][J]; kQLanes lanes per row,
+// one entry per lane at a time
+constexpr int kQLanes = 16, kCLanes = 32;
+
+__global__ __launch_bounds__(256) void amg_qvals_kernel(int64_t n, const int32_t *__restrict__ rs,
+                                                        const int32_t *__restrict__ col,
+                                                        const double *__restrict__ vals,
+                                                        const int32_t *__restrict__ prs,
+                                                        const int32_t *__restrict__ pcol,
+                                                        const double *__restrict__ pv,
+                                                        const int32_t *__restrict__ qrs,
+                                                        const int32_t *__restrict__ qcol, double *__restrict__ qv) {
+    const int64_t i = (int64_t)blockIdx.x * (256 / kQLanes) + threadIdx.x / kQLanes;
+    if (i >= n) return;
+    for (int32_t q = qrs[i] + threadIdx.x % kQLanes; q < qrs[i + 1]; q += kQLanes) {
+        const int32_t J = qcol[q];
+        double s = 0.0;
+        for (int32_t k = rs[i]; k < rs[i + 1]; k++) {
+            const int32_t c = col[k];
+            const int32_t e = find_col(pcol, prs[c], prs[c + 1], J);
+            if (e >= 0) s += vals[k] * pv[e];
+        }
+        qv[q] = s;
+    }
+}
+
+// A_{l+1}[I][J] = sum over the fine rows i of P^T row I (increasing) of P[i][I] Q[i][J];
+// kCLanes lanes per coarse row, one entry per lane at a time
+__global__ __launch_bounds__(256) void amg_galerkin_kernel(int64_t nc, const int32_t *__restrict__ trs,
+                                                           const int32_t *__restrict__ trow,
+                                                           const double *__restrict__ ptv,
+                                                           const int32_t *__restrict__ qrs,
+                                                           const int32_t *__restrict__ qcol,
+                                                           const double *__restrict__ qv,
+                                                           const int32_t *__restrict__ crs,
+                                                           const int32_t *__restrict__ ccol, double *__restrict__ cv) {
+    const int64_t I = (int64_t)blockIdx.x * (256 / kCLanes) + threadIdx.x / kCLanes;
+    if (I >= nc) return;
+    for (int32_t q = crs[I] + threadIdx.x % kCLanes; q < crs[I + 1]; q += kCLanes) {
+        const int32_t J = ccol[q];
+        double s = 0.0;
+        for (int32_t t = trs[I]; t < trs[I + 1]; t++) {
+            const int32_t i = trow[t];
+            const int32_t e = find_col(qcol, qrs[i], qrs[i + 1], J);
+            if (e >= 0) s += ptv[t] * qv[e];
+        }
+        cv[q] = s;
+    }
+}
+
+// P^T values in P^T order
+__global__ __launch_bounds__(256) void amg_ptvals_kernel(int64_t nnzp, const int32_t *__restrict__ tsrc,
+                                                         const double *__restrict__ pv, double *__restrict__ ptv) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < nnzp) ptv[t] = pv[tsrc[t]];
+}
+
+// ---- V-cycle kernels -----------------------------------------------------------------------
+
+// x = W b
+__global__ __launch_bounds__(256) void amg_wb_kernel(int64_t n, const double *__restrict__ w,
+                                                     const double *__restrict__ b, double *__restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) x[i] = w[i] * b[i];
+}
+
+// out = x + W r
+__global__ __launch_bounds__(256) void amg_update_kernel(int64_t n, const double *__restrict__ w,
+                                                         const double *__restrict__ r, const double *x,
+                                                         double *out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = x[i] + w[i] * r[i];
+}
+
+// Dense Cholesky factor of the listed sub-problems' matrices (<= kBottomMaxRows rows),
+// one workgroup per sub-problem, right-looking, fixed order. A non-positive pivot is
+// replaced by 1 and flagged.
+__global__ __launch_bounds__(kCholThreads) void amg_chol_factor_kernel(const int32_t *__restrict__ subs,
+                                                                       const int64_t *__restrict__ sub_off,
+                                                                       const int32_t *__restrict__ rs,
+                                                                       const int32_t *__restrict__ col,
+                                                                       const double *__restrict__ vals,
+                                                                       double *__restrict__ chol,
+                                                                       int32_t *__restrict__ fail) {
+    __shared__ double a[kBottomMaxRows][kBottomMaxRows + 1];
+    const int s = subs[blockIdx.x], t = threadIdx.x;
+    const int64_t r0 = sub_off[s];
+    const int m = (int)(sub_off[s + 1] - r0);
+    for (int i = t; i < kBottomMaxRows; i += kCholThreads)
+        for (int j = 0; j < kBottomMaxRows; j++) a[i][j] = 0.0;
+    __syncthreads();
+    for (int i = t; i < m; i += kCholThreads)
+        for (int32_t e = rs[r0 + i]; e < rs[r0 + i + 1]; e++) a[i][col[e] - r0] = vals[e];
+    __syncthreads();
+    for (int k = 0; k < m; k++) {
+        if (t == 0) {
+            double pv = a[k][k];
+            if (!(pv > 0.0) || !isfinite(pv)) {
+                fail[s] = 1;
+                pv = 1.0;
+            }
+            a[k][k] = sqrt(pv);
+        }
+        __syncthreads();
+        for (int i = k + 1 + t; i < m; i += kCholThreads) a[i][k] = a[i][k] / a[k][k];
+        __syncthreads();
+        for (int i = k + 1 + t; i < m; i += kCholThreads)
+            for (int j = k + 1; j <= i; j++) a[i][j] = a[i][j] - a[i][k] * a[j][k];
+        __syncthreads();
+    }
+    double *L = chol + (int64_t)s * kBottomMaxRows * kBottomMaxRows;
+    for (int i = t; i < m; i += kCholThreads)
+        for (int j = 0; j <= i; j++) L[i * kBottomMaxRows + j] = a[i][j];
+}
+
+// x = (L L^T)^-1 b for the listed sub-problems (column-oriented triangular solves)
+__global__ __launch_bounds__(kCholThreads) void amg_chol_solve_kernel(const int32_t *__restrict__ subs,
+                                                                      const int64_t *__restrict__ sub_off,
+                                                                      const double *__restrict__ chol,
+                                                                      const double *__restrict__ b,
+                                                                      double *__restrict__ x) {
+    __shared__ double y[kBottomMaxRows];
+    const int s = subs[blockIdx.x], t = threadIdx.x;
+    const int64_t r0 = sub_off[s];
+    const int m = (int)(sub_off[s + 1] - r0);
+    const double *L = chol + (int64_t)s * kBottomMaxRows * kBottomMaxRows;
+    for (int i = t; i < m; i += kCholThreads) y[i] = b[r0 + i];
+    __syncthreads();
+    for (int k = 0; k < m; k++) {           // L y = b
+        if (t == 0) y[k] = y[k] / L[k * kBottomMaxRows + k];
+        __syncthreads();
+        for (int i = k + 1 + t; i < m; i += kCholThreads) y[i] = y[i] - L[i * kBottomMaxRows + k] * y[k];
+        __syncthreads();
+    }
+    for (int k = m - 1; k >= 0; k--) {      // L^T x = y
+        if (t == 0) y[k] = y[k] / L[k * kBottomMaxRows + k];
+        __syncthreads();
+        for (int i = t; i < k; i += kCholThreads) y[i] = y[i] - L[k * kBottomMaxRows + i] * y[k];
+        __syncthreads();
+    }
+    for (int i = t; i < m; i += kCholThreads) x[r0 + i] = y[i];
+}
+
+// ---- host ------------------------------------------------------------------------------------
+
+#define ACHECK() PYHIP_CHECK_HIP(hipGetLastError())
+
+template <typename T>
+static int aup(T **dst, const std::vector<T> &v, int64_t &bytes) {
+    int rc = dev_alloc(dst, v.size(), bytes);
+    if (rc) return rc;
+    if (!v.empty()) PYHIP_CHECK_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return PYHIP_OK;
+}
+
+static std::vector<int32_t> row_sub_of(const std::vector<int64_t> &off) {
+    std::vector<int32_t> rsub(off.back());
+    for (size_t s = 0; s + 1 < off.size(); s++)
+        for (int64_t i = off[s]; i < off[s + 1]; i++) rsub[i] = (int32_t)s;
+    return rsub;
+}
+
+void amg_free(AmgDev &A) {
+    auto fl = [](AmgDevLevel &L) {
+        void *ptrs[] = {L.rs, L.col, L.diag, L.agg, L.prs, L.pcol, L.pcptr, L.pcidx, L.trs, L.trow, L.tsrc,
+                        L.qrs, L.qcol, L.row_sub, L.sub_off, L.own_vals, L.d, L.w, L.pv, L.qv, L.b, L.x,
+                        L.x2, L.r, L.pw, L.py, L.cont, L.ends, L.ptv};
+        for (void *q : ptrs) dfree(q);
+        L = AmgDevLevel();
+    };
+    for (auto &L : A.lv) fl(L);
+    fl(A.bot);
+    dfree(A.nrm);
+    dfree(A.chol);
+    dfree(A.chol_fail);
+    A = AmgDev();
+}
+
+static int upload_level(AmgDevLevel &D, const std::vector<int32_t> &rs, const std::vector<int32_t> &col,
+                        const std::vector<int32_t> &diag, const std::vector<int64_t> &sub_off, bool own_vals,
+                        int64_t &B) {
+    int rc;
+    D.n = (int64_t)rs.size() - 1;
+    D.nnz = rs.back();
+    if ((rc = aup(&D.rs, rs, B)) || (rc = aup(&D.col, col, B)) || (rc = aup(&D.diag, diag, B)) ||
+        (rc = aup(&D.sub_off, sub_off, B)) || (rc = aup(&D.row_sub, row_sub_of(sub_off), B)))
+        return rc;
+    if (own_vals && (rc = dev_alloc(&D.own_vals, D.nnz, B))) return rc;
+    if ((rc = dev_alloc(&D.d, D.n, B)) || (rc = dev_alloc(&D.w, D.n, B)) || (rc = dev_alloc(&D.b, D.n, B)) ||
+        (rc = dev_alloc(&D.x, D.n, B)) || (rc = dev_alloc(&D.x2, D.n, B)) || (rc = dev_alloc(&D.r, D.n, B)) ||
+        (rc = dev_alloc(&D.pw, D.n, B)) || (rc = dev_alloc(&D.py, D.n, B)))
+        return rc;
+    return PYHIP_OK;
+}
+
+int amg_build(AmgDev &A, int64_t n, const int32_t *rs_h, const int32_t *col_h, const std::vector<int64_t> &sub_off,
+              int64_t &B) {
+    if (A.built) return PYHIP_OK;
+    AmgPlan plan;
+    std::string err;
+    if (amg_plan_build(n, rs_h, col_h, sub_off, kAggregateCap, kBottomRows, kMaxLevels, plan, err)) {
+        set_error(err);
+        return PYHIP_ERR_INVALID;
+    }
+    if (plan.max_bot_rows > kBottomMaxRows) {
+        set_error("multigrid: coarsest sub-problem larger than the dense coarsest solve");
+        return PYHIP_ERR_INVALID;
+    }
+    A.nsub = (int)sub_off.size() - 1;
+    int rc;
+    A.lv.resize(plan.lv.size());
+    for (size_t l = 0; l < plan.lv.size(); l++) {
+        const AmgLevelPlan &P = plan.lv[l];
+        AmgDevLevel &D = A.lv[l];
+        if ((rc = upload_level(D, P.rs, P.col, P.diag, P.sub_off, l > 0, B))) return rc;
+        D.nc = P.nc;
+        D.nends = (int)P.ends.size();
+        if ((rc = aup(&D.cont, P.cont, B)) || (rc = aup(&D.ends, P.ends, B))) return rc;
+        D.nnzp = P.prs.back();
+        D.nnzq = P.qrs.back();
+        if ((rc = aup(&D.agg, P.agg, B)) || (rc = aup(&D.prs, P.prs, B)) || (rc = aup(&D.pcol, P.pcol, B)) ||
+            (rc = aup(&D.pcptr, P.pcptr, B)) || (rc = aup(&D.pcidx, P.pcidx, B)) || (rc = aup(&D.trs, P.trs, B)) ||
+            (rc = aup(&D.trow, P.trow, B)) || (rc = aup(&D.tsrc, P.tsrc, B)) || (rc = aup(&D.qrs, P.qrs, B)) ||
+            (rc = aup(&D.qcol, P.qcol, B)) || (rc = dev_alloc(&D.pv, D.nnzp, B)) ||
+            (rc = dev_alloc(&D.ptv, D.nnzp, B)) ||
+            (rc = dev_alloc(&D.qv, D.nnzq, B)))
+            return rc;
+    }
+    if ((rc = upload_level(A.bot, plan.bot_rs, plan.bot_col, plan.bot_diag, plan.bot_off, !plan.lv.empty(), B)))
+        return rc;
+    A.bot.nends = (int)plan.bot_subs.size();
+    if ((rc = aup(&A.bot.ends, plan.bot_subs, B))) return rc;
+    if ((rc = dev_alloc(&A.nrm, 2 * std::max(1, A.nsub), B)) ||
+        (rc = dev_alloc(&A.chol, (size_t)std::max(1, A.nsub) * kBottomMaxRows * kBottomMaxRows, B)) ||
+        (rc = dev_alloc(&A.chol_fail, std::max(1, A.nsub), B)))
+        return rc;
+    A.max_bot_rows = plan.max_bot_rows;
+    A.built = true;
+    return PYHIP_OK;
+}
+
+// lanes per row of the canonical spmv for a matrix with this mean row length
+static inline int lanes_for(int64_t nnz, int64_t n) { return (n > 0 && nnz <= 12 * n) ? 4 : 8; }
+
+static int level_weights(hipStream_t st, AmgDevLevel &L, const double *vals, double *nrm, int nsub) {
+    const int64_t n = L.n;
+    const int lpr = lanes_for(L.nnz, n);
+    hipLaunchKernelGGL(amg_diag_kernel, dim3(ablk(n)), dim3(256), 0, st, n, L.diag, vals, L.d);
+    hipLaunchKernelGGL(amg_power_start_kernel, dim3(ablk(n)), dim3(256), 0, st, n, L.row_sub, L.sub_off, L.pw);
+    for (int it = 0; it < kPowerIters; it++) {
+        launch_spmv<double, double, double>(lpr, st, n, L.rs, L.col, vals, L.pw, L.py, nullptr);
+        hipLaunchKernelGGL(amg_sub_norms_kernel, dim3(nsub), dim3(256), 0, st, L.sub_off, L.d, L.py, L.pw, nrm);
+        if (it + 1 < kPowerIters)
+            hipLaunchKernelGGL(amg_power_scale_kernel, dim3(ablk(n)), dim3(256), 0, st, n, L.row_sub, nrm, L.d, L.py,
+                               L.pw);
+    }
+    hipLaunchKernelGGL(amg_weights_kernel, dim3(ablk(n)), dim3(256), 0, st, n, L.row_sub, nrm, L.d, L.w);
+    ACHECK();
+    return PYHIP_OK;
+}
+
+int amg_setup(AmgDev &A, hipStream_t st, const double *vals0) {
+    if (!A.built) {
+        set_error("multigrid plan not built");
+        return PYHIP_ERR_INVALID;
+    }
+    PYHIP_CHECK_HIP(hipMemsetAsync(A.chol_fail, 0, std::max(1, A.nsub) * sizeof(int32_t), st));
+    const double *vals = vals0;
+    int rc;
+    for (size_t l = 0; l < A.lv.size(); l++) {
+        AmgDevLevel &L = A.lv[l];
+        if (l > 0) vals = L.own_vals;
+        L.vals = vals;
+        if ((rc = level_weights(st, L, vals, A.nrm, A.nsub))) return rc;
+        if (L.nends)
+            hipLaunchKernelGGL(amg_chol_factor_kernel, dim3(L.nends), dim3(kCholThreads), 0, st, L.ends, L.sub_off,
+                               L.rs, L.col, vals, A.chol, A.chol_fail);
+        hipLaunchKernelGGL(amg_pvals_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.prs, L.pcol, L.pcptr,
+                           L.pcidx, L.agg, vals, L.w, L.pv);
+        hipLaunchKernelGGL(amg_ptvals_kernel, dim3(ablk(L.nnzp)), dim3(256), 0, st, L.nnzp, L.tsrc, L.pv, L.ptv);
+        hipLaunchKernelGGL(amg_qvals_kernel, dim3(ablk(L.n, 256 / kQLanes)), dim3(256), 0, st, L.n, L.rs, L.col,
+                           vals, L.prs, L.pcol, L.pv, L.qrs, L.qcol, L.qv);
+        AmgDevLevel &C = (l + 1 < A.lv.size()) ? A.lv[l + 1] : A.bot;
+        if (L.nc)
+            hipLaunchKernelGGL(amg_galerkin_kernel, dim3(ablk(L.nc, 256 / kCLanes)), dim3(256), 0, st, L.nc, L.trs,
+                               L.trow, L.ptv, L.qrs, L.qcol, L.qv, C.rs, C.col, C.own_vals);
+        ACHECK();
+    }
+    A.bot.vals = A.lv.empty() ? vals0 : A.bot.own_vals;
+    if (A.bot.nends)
+        hipLaunchKernelGGL(amg_chol_factor_kernel, dim3(A.bot.nends), dim3(kCholThreads), 0, st, A.bot.ends,
+                           A.bot.sub_off, A.bot.rs, A.bot.col, A.bot.vals, A.chol, A.chol_fail);
+    ACHECK();
+    return PYHIP_OK;
+}
+
+int amg_set_fine(AmgDev &A, const double *vals) {
+    if (!A.built) {
+        set_error("multigrid plan not built");
+        return PYHIP_ERR_INVALID;
+    }
+    if (A.lv.empty()) A.bot.vals = vals;
+    else A.lv[0].vals = vals;
+    return PYHIP_OK;
+}
+
+// z = M^-1 b (both [n0])
+int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z) {
+    const int nl = (int)A.lv.size();
+    AmgDevLevel &Bt = A.bot;
+    if (nl == 0) {
+        if (Bt.nends)
+            hipLaunchKernelGGL(amg_chol_solve_kernel, dim3(Bt.nends), dim3(kCholThreads), 0, st, Bt.ends, Bt.sub_off,
+                               A.chol, b, z);
+        ACHECK();
+        return PYHIP_OK;
+    }
+    // down: level l works on right-hand side bl (level 0: b). The rows of sub-problems that
+    // end on level l are computed too (their couplings stay in their sub-problem) and are
+    // then overwritten by their coarsest solve.
+    for (int l = 0; l < nl; l++) {
+        AmgDevLevel &L = A.lv[l];
+        const double *bl = l == 0 ? b : L.b;
+        AmgDevLevel &C = (l + 1 < nl) ? A.lv[l + 1] : A.bot;
+        hipLaunchKernelGGL(amg_wb_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.w, bl, L.x);
+        launch_spmv<double, double, double>(lanes_for(L.nnz, L.n), st, L.n, L.rs, L.col, L.vals, L.x, L.r, bl);
+        if (L.nc) launch_spmv<double, double, double>(8, st, L.nc, L.trs, L.trow, L.ptv, L.r, C.b, nullptr);
+    }
+    if (Bt.nends)
+        hipLaunchKernelGGL(amg_chol_solve_kernel, dim3(Bt.nends), dim3(kCholThreads), 0, st, Bt.ends, Bt.sub_off,
+                           A.chol, Bt.b, Bt.x);
+    // up: coarse correction and post-sweep; sub-problems ending on a level get their
+    // coarsest solve there
+    for (int l = nl - 1; l >= 0; l--) {
+        AmgDevLevel &L = A.lv[l];
+        const double *bl = l == 0 ? b : L.b;
+        AmgDevLevel &C = (l + 1 < nl) ? A.lv[l + 1] : A.bot;
+        if (L.nc) launch_spmv_add<double, double, double>(4, st, L.n, L.prs, L.pcol, L.pv, C.x, L.x, L.x);
+        launch_spmv<double, double, double>(lanes_for(L.nnz, L.n), st, L.n, L.rs, L.col, L.vals, L.x, L.r, bl);
+        double *out = l == 0 ? z : L.x2;
+        hipLaunchKernelGGL(amg_update_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.w, L.r, L.x, out);
+        if (l > 0) std::swap(L.x, L.x2);   // the level's result is in L.x for the prolongation above
+        if (L.nends)
+            hipLaunchKernelGGL(amg_chol_solve_kernel, dim3(L.nends), dim3(kCholThreads), 0, st, L.ends, L.sub_off,
+                               A.chol, bl, l == 0 ? z : L.x);
+    }
+    ACHECK();
+    return PYHIP_OK;
+}
+
+}  // namespace pyhip

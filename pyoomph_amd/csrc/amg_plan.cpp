@@ -1,0 +1,231 @@
+// Symbolic set-up of the smoothed-aggregation multigrid preconditioner (see amg_plan.h).
+#include "amg_plan.h"
+
+#include <algorithm>
+#include <thread>
+
+namespace pyhip {
+
+namespace {
+
+template <typename F>
+void parallel_for(int64_t n, F &&fn) {
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 50000) nt = 1;
+    const int64_t chunk = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++) {
+        const int64_t b = (int64_t)t * chunk, e = std::min<int64_t>(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([&fn, b, e]() { fn(b, e); });
+    }
+    for (auto &x : th) x.join();
+}
+
+// CSR whose row r holds the sorted distinct values produced by gen(r, push)
+template <typename G>
+bool build_rows(int64_t n, G &&gen, std::vector<int32_t> &rs, std::vector<int32_t> &col, std::string &err) {
+    std::vector<int64_t> len(n + 1, 0);
+    parallel_for(n, [&](int64_t b, int64_t e) {
+        std::vector<int32_t> buf;
+        for (int64_t r = b; r < e; r++) {
+            buf.clear();
+            gen(r, buf);
+            std::sort(buf.begin(), buf.end());
+            len[r + 1] = std::unique(buf.begin(), buf.end()) - buf.begin();
+        }
+    });
+    rs.assign(n + 1, 0);
+    int64_t acc = 0;
+    for (int64_t r = 0; r < n; r++) {
+        acc += len[r + 1];
+        if (acc > INT32_MAX) {
+            err = "multigrid pattern exceeds int32 range";
+            return false;
+        }
+        rs[r + 1] = (int32_t)acc;
+    }
+    col.resize(acc);
+    parallel_for(n, [&](int64_t b, int64_t e) {
+        std::vector<int32_t> buf;
+        for (int64_t r = b; r < e; r++) {
+            buf.clear();
+            gen(r, buf);
+            std::sort(buf.begin(), buf.end());
+            buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+            std::copy(buf.begin(), buf.end(), col.begin() + rs[r]);
+        }
+    });
+    return true;
+}
+
+bool diag_positions(int64_t n, const std::vector<int32_t> &rs, const std::vector<int32_t> &col,
+                    std::vector<int32_t> &diag, std::string &err) {
+    diag.assign(n, -1);
+    bool ok = true;
+    parallel_for(n, [&](int64_t b, int64_t e) {
+        for (int64_t r = b; r < e; r++) {
+            const int32_t *c0 = col.data() + rs[r], *c1 = col.data() + rs[r + 1];
+            const int32_t *d = std::lower_bound(c0, c1, (int32_t)r);
+            if (d == c1 || *d != r) ok = false;
+            else diag[r] = (int32_t)(d - col.data());
+        }
+    });
+    if (!ok) err = "multigrid: a row has no diagonal entry";
+    return ok;
+}
+
+}  // namespace
+
+int amg_plan_build(int64_t n0, const int32_t *rs0, const int32_t *col0, const std::vector<int64_t> &sub_off0,
+                   int cap, int bottom_rows, int max_levels, AmgPlan &plan, std::string &err) {
+    plan = AmgPlan();
+    const int nsub = (int)sub_off0.size() - 1;
+    std::vector<int32_t> rs(rs0, rs0 + n0 + 1), col(col0, col0 + rs0[n0]);
+    std::vector<int64_t> sub_off = sub_off0;
+    int64_t n = n0;
+    auto max_sub = [&](const std::vector<int64_t> &off) {
+        int64_t m = 0;
+        for (int s = 0; s < nsub; s++) m = std::max(m, off[s + 1] - off[s]);
+        return m;
+    };
+    while ((int)plan.lv.size() < max_levels) {
+        std::vector<int32_t> cont(nsub, 0);
+        bool any = false;
+        for (int s = 0; s < nsub; s++) {
+            cont[s] = (sub_off[s + 1] - sub_off[s]) > bottom_rows;
+            any = any || cont[s];
+        }
+        if (!any) break;
+        AmgLevelPlan L;
+        L.n = n;
+        L.sub_off = sub_off;
+        L.cont = cont;
+        for (int s = 0; s < nsub; s++)
+            if (!cont[s] && sub_off[s + 1] > sub_off[s]) L.ends.push_back(s);
+        if (!diag_positions(n, rs, col, L.diag, err)) return -1;
+        // aggregation of the continuing sub-problems (independent), numbered in row order
+        L.agg.assign(n, -1);
+        std::vector<int64_t> nagg(nsub + 1, 0);
+        bool stalled = false;
+        parallel_for(nsub, [&](int64_t b, int64_t e) {
+            for (int64_t s = b; s < e; s++) {
+                if (!cont[s]) continue;
+                int32_t na = 0;
+                for (int64_t r = sub_off[s]; r < sub_off[s + 1]; r++) {
+                    if (L.agg[r] >= 0) continue;
+                    L.agg[r] = na;
+                    int taken = 0;
+                    for (int32_t k = rs[r]; k < rs[r + 1] && taken < cap; k++) {
+                        const int32_t c = col[k];
+                        // the pattern is block-diagonal over sub-problems; a column of another
+                        // sub-problem is never taken, so aggregates stay inside their own
+                        if (c != r && c >= sub_off[s] && c < sub_off[s + 1] && L.agg[c] < 0) {
+                            L.agg[c] = na;
+                            taken++;
+                        }
+                    }
+                    na++;
+                }
+                nagg[s + 1] = na;
+                if (na >= sub_off[s + 1] - sub_off[s]) stalled = true;
+            }
+        });
+        if (stalled) {
+            err = "multigrid: a sub-problem's rows do not coarsen (no couplings)";
+            return -1;
+        }
+        std::vector<int64_t> coff(nsub + 1, 0);
+        for (int s = 0; s < nsub; s++) coff[s + 1] = coff[s] + nagg[s + 1];
+        parallel_for(nsub, [&](int64_t b, int64_t e) {
+            for (int64_t s = b; s < e; s++)
+                if (cont[s])
+                    for (int64_t r = sub_off[s]; r < sub_off[s + 1]; r++) L.agg[r] += (int32_t)coff[s];
+        });
+        const int64_t nc = coff[nsub];
+        L.nc = nc;
+        // P pattern: aggregates of the row's neighbours
+        if (!build_rows(n, [&](int64_t r, std::vector<int32_t> &out) {
+                if (L.agg[r] < 0) return;   // the sub-problem ends on this level
+                for (int32_t k = rs[r]; k < rs[r + 1]; k++) out.push_back(L.agg[col[k]]);
+            }, L.prs, L.pcol, err))
+            return -1;
+        // P contributions: for entry (r, J) the A entries k of row r with agg(col k) == J
+        const int64_t nnzp = L.prs[n];
+        L.pcptr.assign(nnzp + 1, 0);
+        {
+            std::vector<int32_t> cnt(nnzp, 0);
+            parallel_for(n, [&](int64_t b, int64_t e) {
+                for (int64_t r = b; r < e; r++)
+                    for (int32_t q = L.prs[r]; q < L.prs[r + 1]; q++) {
+                        int32_t c = 0;
+                        for (int32_t k = rs[r]; k < rs[r + 1]; k++) c += L.agg[col[k]] == L.pcol[q];
+                        cnt[q] = c;
+                    }
+            });
+            for (int64_t q = 0; q < nnzp; q++) L.pcptr[q + 1] = L.pcptr[q] + cnt[q];
+            L.pcidx.resize(L.pcptr[nnzp]);
+            parallel_for(n, [&](int64_t b, int64_t e) {
+                for (int64_t r = b; r < e; r++)
+                    for (int32_t q = L.prs[r]; q < L.prs[r + 1]; q++) {
+                        int32_t o = L.pcptr[q];
+                        for (int32_t k = rs[r]; k < rs[r + 1]; k++)
+                            if (L.agg[col[k]] == L.pcol[q]) L.pcidx[o++] = k;
+                    }
+            });
+        }
+        // P^T (coarse row J: fine rows in increasing order)
+        L.trs.assign(nc + 1, 0);
+        for (int64_t q = 0; q < nnzp; q++) L.trs[L.pcol[q] + 1]++;
+        for (int64_t J = 0; J < nc; J++) L.trs[J + 1] += L.trs[J];
+        L.trow.resize(nnzp);
+        L.tsrc.resize(nnzp);
+        {
+            std::vector<int32_t> fill(L.trs.begin(), L.trs.end() - 1);
+            for (int64_t r = 0; r < n; r++)
+                for (int32_t q = L.prs[r]; q < L.prs[r + 1]; q++) {
+                    const int32_t o = fill[L.pcol[q]]++;
+                    L.trow[o] = (int32_t)r;
+                    L.tsrc[o] = q;
+                }
+        }
+        // Q = A P pattern
+        if (!build_rows(n, [&](int64_t r, std::vector<int32_t> &out) {
+                for (int32_t k = rs[r]; k < rs[r + 1]; k++) {
+                    const int32_t c = col[k];
+                    for (int32_t q = L.prs[c]; q < L.prs[c + 1]; q++) out.push_back(L.pcol[q]);
+                }
+            }, L.qrs, L.qcol, err))
+            return -1;
+        // A_{l+1} = P^T Q pattern
+        std::vector<int32_t> crs, ccol;
+        if (!build_rows(nc, [&](int64_t J, std::vector<int32_t> &out) {
+                for (int32_t t = L.trs[J]; t < L.trs[J + 1]; t++) {
+                    const int32_t r = L.trow[t];
+                    for (int32_t q = L.qrs[r]; q < L.qrs[r + 1]; q++) out.push_back(L.qcol[q]);
+                }
+            }, crs, ccol, err))
+            return -1;
+        L.rs = std::move(rs);
+        L.col = std::move(col);
+        plan.lv.push_back(std::move(L));
+        rs = std::move(crs);
+        col = std::move(ccol);
+        sub_off = coff;
+        n = nc;
+    }
+    plan.nbot = n;
+    plan.bot_off = sub_off;
+    for (int s = 0; s < nsub; s++)
+        if (sub_off[s + 1] > sub_off[s]) plan.bot_subs.push_back(s);
+    if (!diag_positions(n, rs, col, plan.bot_diag, err)) return -1;
+    plan.bot_rs = std::move(rs);
+    plan.bot_col = std::move(col);
+    int64_t mb = max_sub(plan.bot_off);
+    for (const auto &L : plan.lv)
+        for (int s : L.ends) mb = std::max(mb, L.sub_off[s + 1] - L.sub_off[s]);
+    plan.max_bot_rows = (int)mb;
+    return 0;
+}
+
+}  // namespace pyhip

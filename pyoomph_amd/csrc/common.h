@@ -65,4 +65,19 @@ inline int elem_nnode(int kind) {
 }
 inline int elem_dim(int kind) { return kind == ELEM_HEX_Q2 ? 3 : 2; }
 
+// Device allocation counted in `counter` (bytes); n == 0 allocates one element.
+template <typename T>
+int dev_alloc(T **p, size_t n, int64_t &counter) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        *p = nullptr;
+        return PYHIP_ERR_NOMEM;
+    }
+    counter += (int64_t)(n * sizeof(T));
+    return PYHIP_OK;
+}
+
 }  // namespace pyhip

@@ -2,6 +2,7 @@
 #pragma once
 #include <vector>
 
+#include "amg.h"
 #include "problem.h"
 
 namespace pyhip {
@@ -81,6 +82,8 @@ struct KrylovWork {
     std::vector<int32_t> chunk_ptr_h; // [nsub+1] host copy of sub_chunk_ptr
     int lpr = 0;                      // spmv lanes per row (1, 2, 4, 8), 0 = not tuned
     IluPlan ilu;
+    AmgDev amg;
+    bool amg_stale = true;            // multigrid values to be rebuilt at the next set-up
     int64_t bytes = 0;
 };
 

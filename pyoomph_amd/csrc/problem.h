@@ -110,20 +110,6 @@ int export_reference_csr(const GatherMap &m, const int32_t *col, const double *E
                          const double *vals, uint32_t *keys, int32_t *kept, int32_t *out_rs,
                          int32_t *out_col, double *out_val, int64_t *nnz_out, hipStream_t st);
 
-// Device allocation tracked against the problem.
-template <typename T>
-int dev_alloc(T **p, size_t n, int64_t &counter) {
-    *p = nullptr;
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
-    if (e != hipSuccess) {
-        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
-        *p = nullptr;
-        return PYHIP_ERR_NOMEM;
-    }
-    counter += (int64_t)(n * sizeof(T));
-    return PYHIP_OK;
-}
 
 }  // namespace pyhip
 

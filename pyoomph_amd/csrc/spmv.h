@@ -1,0 +1,156 @@
+// CSR sparse matrix-vector products in one canonical summation order (shared by the
+// Krylov solvers and the multigrid preconditioner).
+#pragma once
+#include <algorithm>
+
+#include "common.h"
+
+namespace pyhip {
+
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Row sums of val[k] * x[col[k]] in one canonical order: stripe t (t = 0..7) of a row
+// accumulates its entries k0+t, k0+t+8, ... with fused multiply-adds, and the stripes
+// combine as ((S0+S4)+(S2+S6)) + ((S1+S5)+(S3+S7)). LPR lanes per row (1, 2, 4 or 8;
+// lane l owns stripes l, l+LPR, ...) all give the same bits, so the lane count can be
+// tuned per problem without changing any result.
+// A thread serves LPR rows of a 256-row block (rows r0 + p*(256/LPR) + threadIdx.x/LPR,
+// p = 0..LPR-1) and issues the loads of all of them together: 8 (row, stripe) streams in
+// flight per thread, whatever LPR. All lanes of a row call this; out[p] is the sum of
+// row p (rows at or past r1: 0).
+template <typename TV, typename TX, typename TY, int LPR>
+__device__ __forceinline__ void rows_dot(int64_t r0, int64_t r1, const int32_t *__restrict__ rs,
+                                         const int32_t *__restrict__ col, const TV *__restrict__ val,
+                                         const TX *__restrict__ x, TY (&out)[LPR]) {
+    constexpr int NS = 8 / LPR, RPP = 256 / LPR;
+    const int lane = threadIdx.x % LPR;
+    int32_t k0[LPR], k1[LPR];
+    int32_t len = 0;
+#pragma unroll
+    for (int p = 0; p < LPR; p++) {
+        const int64_t row = r0 + p * RPP + threadIdx.x / LPR;
+        const bool ok = row < r1;
+        k0[p] = ok ? rs[row] : 0;
+        k1[p] = ok ? rs[row + 1] : 0;
+        len = max(len, k1[p] - k0[p]);
+    }
+    TY acc[LPR][NS];
+#pragma unroll
+    for (int p = 0; p < LPR; p++)
+#pragma unroll
+        for (int j = 0; j < NS; j++) acc[p][j] = TY(0);
+    for (int32_t base = lane; base < len; base += 8) {
+        int32_t cc[LPR][NS];
+        TY vv[LPR][NS], xx[LPR][NS];
+#pragma unroll
+        for (int p = 0; p < LPR; p++)
+#pragma unroll
+            for (int j = 0; j < NS; j++) {
+                const int32_t k = k0[p] + base + j * LPR;
+                const bool in = k < k1[p];
+                cc[p][j] = in ? col[k] : -1;
+                vv[p][j] = in ? TY(val[k]) : TY(0);
+            }
+#pragma unroll
+        for (int p = 0; p < LPR; p++)
+#pragma unroll
+            for (int j = 0; j < NS; j++) xx[p][j] = cc[p][j] >= 0 ? TY(x[cc[p][j]]) : TY(0);
+#pragma unroll
+        for (int p = 0; p < LPR; p++)
+#pragma unroll
+            for (int j = 0; j < NS; j++)
+                if (cc[p][j] >= 0) acc[p][j] = fmadd(vv[p][j], xx[p][j], acc[p][j]);
+    }
+#pragma unroll
+    for (int p = 0; p < LPR; p++) {
+        TY v;
+        if constexpr (LPR == 8) v = acc[p][0];
+        else if constexpr (LPR == 4) v = acc[p][0] + acc[p][1];
+        else if constexpr (LPR == 2) v = (acc[p][0] + acc[p][2]) + (acc[p][1] + acc[p][3]);
+        else v = ((acc[p][0] + acc[p][4]) + (acc[p][2] + acc[p][6])) + ((acc[p][1] + acc[p][5]) + (acc[p][3] + acc[p][7]));
+        if constexpr (LPR >= 8) v = v + __shfl_xor(v, 4, LPR);
+        if constexpr (LPR >= 4) v = v + __shfl_xor(v, 2, LPR);
+        if constexpr (LPR >= 2) v = v + __shfl_xor(v, 1, LPR);
+        out[p] = v;
+    }
+}
+
+// y = A x (or b - A x); 256 rows per block (rows_dot); the grid is a multiple of 8
+// (spmv_grid).
+template <typename TV, typename TX, typename TY, int LPR>
+__global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__restrict__ rs,
+                                                   const int32_t *__restrict__ col,
+                                                   const TV *__restrict__ val,
+                                                   const TX *__restrict__ x, TY *__restrict__ y,
+                                                   const TY *__restrict__ b) {
+    constexpr int RPP = 256 / LPR;
+    // blocks round-robin over the 8 XCDs: XCD x gets the contiguous block range x*per..
+    const int64_t nb = (n + 255) / 256, per = (nb + 7) / 8, k = blockIdx.x / 8;
+    const int64_t lb = (int64_t)(blockIdx.x % 8) * per + k;
+    if (k >= per || lb >= nb) return;
+    const int64_t r0 = lb * 256;
+    TY v[LPR];
+    rows_dot<TV, TX, TY, LPR>(r0, n, rs, col, val, x, v);
+    if (threadIdx.x % LPR == 0) {
+#pragma unroll
+        for (int p = 0; p < LPR; p++) {
+            const int64_t row = r0 + p * RPP + threadIdx.x / LPR;
+            if (row < n) y[row] = b ? b[row] - v[p] : v[p];
+        }
+    }
+}
+
+// y = b + A x (rows_dot), same grid as spmv_kernel; y may alias b
+template <typename TV, typename TX, typename TY, int LPR>
+__global__ __launch_bounds__(256) void spmv_add_kernel(int64_t n, const int32_t *__restrict__ rs,
+                                                       const int32_t *__restrict__ col,
+                                                       const TV *__restrict__ val,
+                                                       const TX *__restrict__ x, TY *y,
+                                                       const TY *b) {
+    constexpr int RPP = 256 / LPR;
+    const int64_t nb = (n + 255) / 256, per = (nb + 7) / 8, k = blockIdx.x / 8;
+    const int64_t lb = (int64_t)(blockIdx.x % 8) * per + k;
+    if (k >= per || lb >= nb) return;
+    const int64_t r0 = lb * 256;
+    TY v[LPR];
+    rows_dot<TV, TX, TY, LPR>(r0, n, rs, col, val, x, v);
+    if (threadIdx.x % LPR == 0) {
+#pragma unroll
+        for (int p = 0; p < LPR; p++) {
+            const int64_t row = r0 + p * RPP + threadIdx.x / LPR;
+            if (row < n) y[row] = b[row] + v[p];
+        }
+    }
+}
+
+static inline unsigned spmv_grid(int64_t n) {
+    const int64_t nb = std::max<int64_t>(1, (n + 255) / 256);
+    return (unsigned)(((nb + 7) / 8) * 8);
+}
+
+template <typename TV, typename TX, typename TY>
+static void launch_spmv(int lpr, hipStream_t st, int64_t n, const int32_t *rs, const int32_t *col,
+                        const TV *vals, const TX *x, TY *y, const TY *b) {
+    const dim3 g(spmv_grid(n)), t(256);
+    switch (lpr) {
+    case 1: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 1>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    case 2: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 2>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    case 4: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 4>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    default: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 8>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    }
+}
+
+template <typename TV, typename TX, typename TY>
+static void launch_spmv_add(int lpr, hipStream_t st, int64_t n, const int32_t *rs, const int32_t *col,
+                            const TV *vals, const TX *x, TY *y, const TY *b) {
+    const dim3 g(spmv_grid(n)), t(256);
+    switch (lpr) {
+    case 1: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 1>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    case 2: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 2>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    case 4: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 4>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    default: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 8>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    }
+}
+
+}  // namespace pyhip

@@ -101,6 +101,11 @@ def test_fp32_kernels_close_to_fp64(gpu, kind, kw):
                                                    linear_precision="f32", jacobian_precision="f32")),
     ("rd", dict(N=3), dict(krylov="cg", preconditioner="ilu0")),
     ("ns", dict(N=8), dict(krylov="gmres", preconditioner="ilu0")),
+    ("poisson", dict(N=24, seed=2), dict(krylov="cg", preconditioner="amg")),
+    ("bratu", dict(N=40, space="C2", lam=2.5), dict(krylov="cg", preconditioner="amg")),
+    ("bratu", dict(N=33, space="C1", lam=0.7), dict(krylov="cg", preconditioner="amg")),
+    ("rd", dict(N=6), dict(krylov="cg", preconditioner="amg")),
+    ("bratu", dict(N=12, space="C2", lam=1.5), dict(krylov="gmres", preconditioner="amg")),
 ])
 def test_newton_matches_superlu_oracle(gpu, kind, kw, opts):
     P, op = build(kind, kw)
@@ -162,6 +167,23 @@ def test_cg_iteration_cap_reports_failure(gpu):
     P.gmres_max_iters = 3
     st = P.solve(raise_on_failure=False)
     assert st[0]["linear_failures"] > 0
+
+
+def test_multigrid_batch_equals_alone_bitwise(gpu):
+    """The multigrid hierarchy is built per sub-problem: the same bits alone and in a batch."""
+    from pyoomph_amd import workloads as wl
+    from pyoomph_amd.problem import Problem
+    specs = [(45, "C2", 1.2), (70, "C1", 2.9), (9, "C2", 0.7), (3, "C1", 1.0), (26, "C1", 1.9)]
+    batch = Problem([wl.bratu(*s) for s in specs])
+    batch.krylov, batch.preconditioner = "cg", "amg"
+    st = batch.solve()
+    xb = batch.get_current_dofs()
+    for i, s in enumerate(specs):
+        alone = Problem(wl.bratu(*s))
+        alone.krylov, alone.preconditioner = "cg", "amg"
+        sa = alone.solve()
+        assert np.array_equal(alone.get_current_dofs(), batch.sub_dofs(i, xb)), s
+        assert sa[0]["newton_iters"] == st[i]["newton_iters"]
 
 
 def test_newton_error_path(gpu):
