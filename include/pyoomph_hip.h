@@ -143,6 +143,14 @@ int pyhip_export_reference_csr(pyhip_problem *p, const double *values_dev,
 #define PYHIP_PRECOND_JACOBI 0
 #define PYHIP_PRECOND_ILU0 1
 #define PYHIP_PRECOND_AMG 2    /* smoothed-aggregation multigrid V-cycle (FP64 solves) */
+/* ILU(0) row order within a sub-problem (Taylor-Hood: velocity unknowns, then pressure).
+ * NATURAL keeps the equation order; MULTICOLOR sorts each group by a greedy colouring,
+ * so the level-scheduled triangular solves need one launch per colour instead of one per
+ * level of the natural order (O(rows per side)), for a weaker factor. AUTO takes
+ * MULTICOLOR when the natural order has more than 64 lower-solve levels. */
+#define PYHIP_ILU_ORDER_AUTO 0
+#define PYHIP_ILU_ORDER_NATURAL 1
+#define PYHIP_ILU_ORDER_MULTICOLOR 2
 #define PYHIP_KRYLOV_GMRES 0
 #define PYHIP_KRYLOV_CG 1    /* symmetric positive definite Jacobians (Poisson, Bratu
                                 below its fold, reaction-diffusion with u^3)           */
@@ -150,12 +158,13 @@ typedef struct {
     int krylov;         /* PYHIP_KRYLOV_*                                             */
     int precond;        /* PYHIP_PRECOND_*                                            */
     int precision;      /* Krylov + preconditioner arithmetic: PYHIP_F64 / PYHIP_F32   */
-    int restart;        /* GMRES restart length m                                     */
+    int restart;        /* GMRES restart length m (1..1000)                           */
     int max_iters;      /* total inner iterations per solve                           */
     double rtol;        /* relative residual target of an FP64 inner solve            */
     double rtol_f32;    /* relative residual target of an FP32 inner solve            */
     int max_refinements;/* FP64 iterative-refinement sweeps around an FP32 solve       */
     double refine_rtol; /* relative residual target of the refined solution            */
+    int ilu_order;      /* PYHIP_ILU_ORDER_*: row order of the ILU(0) factor           */
 } pyhip_linear_opts;
 void pyhip_linear_opts_default(pyhip_linear_opts *o);
 

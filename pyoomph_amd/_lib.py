@@ -23,6 +23,7 @@ ELEM_KIND = {"tri_p1": ELEM_TRI_P1, "tri_p2": ELEM_TRI_P2, "quad_q2": ELEM_QUAD_
 FORM_POISSON, FORM_BRATU, FORM_REACTION_DIFFUSION, FORM_NAVIER_STOKES_TH = 1, 2, 3, 4
 F64, F32 = 0, 1
 PRECOND_JACOBI, PRECOND_ILU0, PRECOND_AMG = 0, 1, 2
+ILU_ORDER_AUTO, ILU_ORDER_NATURAL, ILU_ORDER_MULTICOLOR = 0, 1, 2
 KRYLOV_GMRES, KRYLOV_CG = 0, 1
 
 c_i32p = ctypes.POINTER(ctypes.c_int32)
@@ -54,7 +55,7 @@ class LinearOpts(ctypes.Structure):
                 ("restart", ctypes.c_int), ("max_iters", ctypes.c_int),
                 ("rtol", ctypes.c_double), ("rtol_f32", ctypes.c_double),
                 ("max_refinements", ctypes.c_int),
-                ("refine_rtol", ctypes.c_double)]
+                ("refine_rtol", ctypes.c_double), ("ilu_order", ctypes.c_int)]
 
 
 class NewtonOpts(ctypes.Structure):

@@ -8,11 +8,14 @@
 namespace pyhip {
 
 constexpr int kChunk = 1024;  // rows per reduction chunk (chunks never straddle sub-problems)
+constexpr int kMaxRestart = 1000;   // longest GMRES restart cycle
 constexpr int kWlRows = 256;         // rows per block of the work-list kernels
 constexpr int kWlBlocksPerChunk = kChunk / kWlRows;
 
 struct IluPlan {
     bool built = false;
+    int requested = -1;           // PYHIP_ILU_ORDER_* the plan was built for
+    bool multicolor = false;      // order taken (AUTO resolves to one of the two)
     int64_t n = 0, nnz = 0;
     int32_t *perm = nullptr;      // [n] new -> old
     int32_t *rs = nullptr;        // [n+1] permuted, column-sorted
@@ -82,6 +85,7 @@ struct KrylovWork {
     std::vector<int32_t> chunk_ptr_h; // [nsub+1] host copy of sub_chunk_ptr
     int lpr = 0;                      // spmv lanes per row (1, 2, 4, 8), 0 = not tuned
     IluPlan ilu;
+    int ilu_order = 0;                // PYHIP_ILU_ORDER_* of the current solve
     AmgDev amg;
     bool amg_stale = true;            // multigrid values to be rebuilt at the next set-up
     int64_t bytes = 0;

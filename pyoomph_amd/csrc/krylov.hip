@@ -32,27 +32,28 @@ namespace pyhip {
 // ------------------------------------------------------------------------------------
 // kernels
 
+// partial[c][j] = sum over chunk c of V_j . w, for the vectors j of block column
+// blockIdx.y (kDotVecs of them): wave q of the block takes j = q, q + 4, ...; its lanes
+// stride the chunk's rows, and the wave sums them with a fixed xor-shuffle tree.
+constexpr int kDotVecs = 16;
 template <typename T>
 __global__ __launch_bounds__(256) void multidot_kernel(int nvec, const T *__restrict__ V,
                                                       int64_t ldv, const T *__restrict__ w,
                                                       const int64_t *__restrict__ cb,
                                                       const int64_t *__restrict__ ce,
                                                       double *__restrict__ partial, int pstride) {
-    __shared__ double red[256];
     const int64_t c = blockIdx.x;
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int64_t b0 = cb[c], b1 = ce[c];
-    for (int j = 0; j < nvec; j++) {
+    for (int jj = q; jj < kDotVecs; jj += 4) {
+        const int j = blockIdx.y * kDotVecs + jj;
+        if (j >= nvec) break;
         const T *v = V + (int64_t)j * ldv;
         double acc = 0.0;
-        for (int64_t i = b0 + threadIdx.x; i < b1; i += 256) acc += double(v[i]) * double(w[i]);
-        red[threadIdx.x] = acc;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) partial[c * pstride + j] = red[0];
-        __syncthreads();
+#pragma unroll 4
+        for (int64_t i = b0 + lane; i < b1; i += 64) acc += double(v[i]) * double(w[i]);
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) partial[c * pstride + j] = acc;
     }
 }
 
@@ -170,28 +171,26 @@ __global__ void gmres_givens_kernel(int nsub, int m, int k, const double *__rest
     if (s >= nsub || !active[s]) return;
     double *Hs = H + (int64_t)s * (m + 1) * m;
     double *cs_s = cs + (int64_t)s * m, *sn_s = sn + (int64_t)s * m, *gs = g + (int64_t)s * (m + 1);
-    double h[65];
-    for (int i = 0; i <= k; i++) h[i] = d1[i * nsub + s] + d2[i * nsub + s];
-    const double hk1 = sqrt(wn2[s]);
-    h[k + 1] = hk1;
+    // column k is rotated in place: Hs[i * m + k], i = 0..k (row k + 1 is only carried in
+    // a register, since the rotation zeroes it)
+    double hi = d1[s] + d2[s];
     for (int i = 0; i < k; i++) {
-        const double t = cs_s[i] * h[i] + sn_s[i] * h[i + 1];
-        h[i + 1] = -sn_s[i] * h[i] + cs_s[i] * h[i + 1];
-        h[i] = t;
+        const double hn = d1[(i + 1) * nsub + s] + d2[(i + 1) * nsub + s];
+        Hs[i * m + k] = cs_s[i] * hi + sn_s[i] * hn;
+        hi = -sn_s[i] * hi + cs_s[i] * hn;
     }
-    const double den = hypot(h[k], h[k + 1]);
+    const double hk1 = sqrt(wn2[s]);
+    const double den = hypot(hi, hk1);
     double c = 1.0, sv = 0.0;
     if (den > 0.0) {
-        c = h[k] / den;
-        sv = h[k + 1] / den;
+        c = hi / den;
+        sv = hk1 / den;
     }
     cs_s[k] = c;
     sn_s[k] = sv;
-    h[k] = den;
-    h[k + 1] = 0.0;
+    Hs[k * m + k] = den;
     gs[k + 1] = -sv * gs[k];
     gs[k] = c * gs[k];
-    for (int i = 0; i <= k; i++) Hs[i * m + k] = h[i];
     kcols[s] = k + 1;
     const bool done = !(fabs(gs[k + 1]) > tol * bnorm[s]) || hk1 == 0.0 || den == 0.0 ||
                       !isfinite(gs[k + 1]) || (k + 1 == m);
@@ -381,6 +380,7 @@ static int kalloc(T **p, size_t n, KrylovWork *W) {
 }
 
 static int krylov_setup(pyhip_problem *p, int m) {
+    PYHIP_REQUIRE(m >= 1 && m <= kMaxRestart, "restart must be in 1..1000");
     if (p->kw && p->kw->m >= m) return PYHIP_OK;
     KrylovWork *W = p->kw;
     if (!W) {
@@ -491,7 +491,8 @@ static int batched_dots(pyhip_problem *p, KrylovWork *W, int nvec, const T *V, i
     TimeScope ts(&p->timer, "vector", p->st);
     const int pstride = W->m + 2;
     if (W->nchunk == 0) return PYHIP_OK;
-    hipLaunchKernelGGL((multidot_kernel<T>), dim3((unsigned)W->nchunk), dim3(256), 0, p->st, nvec,
+    hipLaunchKernelGGL((multidot_kernel<T>), dim3((unsigned)W->nchunk, (unsigned)((nvec + kDotVecs - 1) / kDotVecs)),
+                       dim3(256), 0, p->st, nvec,
                        V, ldv, w, W->chunk_begin, W->chunk_end, W->partial, pstride);
     KCHECK();
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)(nvec * W->nsub)), dim3(64), 0, p->st, nvec,
@@ -568,18 +569,42 @@ static int spmv(pyhip_problem *p, const TV *vals, const TX *x, TY *y, const TY *
 }
 
 // ---- ILU(0) plan: permutation, permuted pattern, level sets (host, once per pattern)
+//
+// Order within a sub-problem: Taylor-Hood velocity unknowns first, then its pressure
+// unknowns (other forms: one group). Within a group either the natural order, or a
+// multicolour order: a greedy colouring of the group's rows (in natural order, each row
+// takes the smallest colour none of its already coloured neighbours in the group has),
+// rows sorted by (colour, index). Rows of one colour of one group are independent, so
+// the triangular solves have at most as many levels as there are colours, against
+// O(rows per side) levels in the natural order, at the price of a weaker factor.
+// PYHIP_ILU_ORDER_AUTO takes the multicolour order when the natural one has more than
+// kIluAutoLevels lower-solve levels.
+constexpr int kIluAutoLevels = 64;
+
+static void ilu_free(KrylovWork *W) {
+    IluPlan &L = W->ilu;
+    void *ptrs[] = {L.perm, L.rs, L.col, L.src, L.diag, L.lo_rows, L.up_rows, L.lu,
+                    L.tmp64, L.tmp64b, L.tmp32, L.tmp32b};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    L = IluPlan();
+}
+
 static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
     IluPlan &L = W->ilu;
-    if (L.built) return PYHIP_OK;
+    if (L.built && L.requested == W->ilu_order) return PYHIP_OK;
+    if (L.built) {
+        PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+        ilu_free(W);
+    }
     const int64_t n = p->ndof;
-    // permutation: Taylor-Hood sub-problems put pressure unknowns last
-    std::vector<int32_t> perm(n), iperm(n);
+    const auto &rs = p->row_start_h;
+    const auto &col = p->col_h;
+    // group of each row: 0 velocity / other, 1 Taylor-Hood pressure
+    std::vector<char> is_p(n, 0);
     {
-        std::vector<char> is_p(n, 0);
         if ((int64_t)p->ilu_last_h.size() == n) is_p = p->ilu_last_h;
-        // pressure rows are rows whose diagonal structural entry receives no element
-        // contribution from a velocity slot; identified from the slot layout: eqn of
-        // pressure slots (2, 7, 16, 21) of Taylor-Hood elements.
+        // pressure unknowns: eqn of the pressure slots (2, 7, 16, 21) of Taylor-Hood elements
         for (auto &G : p->groups) {
             if (G.form != PYHIP_FORM_NAVIER_STOKES_TH) continue;
             std::vector<int32_t> seqn((size_t)G.ns * G.nel);
@@ -591,19 +616,40 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
                     if (q >= 0) is_p[q] = 1;
                 }
         }
-        int64_t w = 0;
-        for (int s = 0; s < p->nsub; s++) {
-            for (int64_t i = p->dof_off_h[s]; i < p->dof_off_h[s + 1]; i++)
-                if (!is_p[i]) perm[w++] = (int32_t)i;
-            for (int64_t i = p->dof_off_h[s]; i < p->dof_off_h[s + 1]; i++)
-                if (is_p[i]) perm[w++] = (int32_t)i;
-        }
-        for (int64_t i = 0; i < n; i++) iperm[perm[i]] = (int32_t)i;
     }
-    const auto &rs = p->row_start_h;
-    const auto &col = p->col_h;
-    std::vector<int32_t> prs(n + 1, 0), pcol(p->nnz), psrc(p->nnz), pdiag(n, -1);
-    {
+    std::vector<int32_t> perm(n), iperm(n);
+    auto make_perm = [&](bool multicolor) {
+        std::vector<int32_t> color(n, 0);
+        if (multicolor) {
+            std::vector<int32_t> mark;   // mark[c] == row: colour c taken by a neighbour
+            for (int64_t i = 0; i < n; i++) {
+                for (int32_t k = rs[i]; k < rs[i + 1]; k++) {
+                    const int32_t c = col[k];
+                    if (c < i && is_p[c] == is_p[i]) {
+                        if ((size_t)color[c] >= mark.size()) mark.resize(color[c] + 1, -1);
+                        mark[color[c]] = (int32_t)i;
+                    }
+                }
+                int32_t c = 0;
+                while ((size_t)c < mark.size() && mark[c] == i) c++;
+                color[i] = c;
+            }
+        }
+        int64_t w = 0;
+        std::vector<int32_t> rows;
+        for (int s = 0; s < p->nsub; s++)
+            for (char g = 0; g < 2; g++) {
+                rows.clear();
+                for (int64_t i = p->dof_off_h[s]; i < p->dof_off_h[s + 1]; i++)
+                    if (is_p[i] == g) rows.push_back((int32_t)i);
+                std::stable_sort(rows.begin(), rows.end(),
+                                 [&](int32_t x, int32_t y) { return color[x] < color[y]; });
+                for (int32_t r : rows) perm[w++] = r;
+            }
+        for (int64_t i = 0; i < n; i++) iperm[perm[i]] = (int32_t)i;
+    };
+    std::vector<int32_t> prs(n + 1, 0), pcol(p->nnz), psrc(p->nnz), pdiag(n, -1), lev(n, 0);
+    auto make_pattern = [&]() -> int {
         std::vector<std::pair<int32_t, int32_t>> tmp;
         for (int64_t i = 0; i < n; i++) {
             const int32_t o = perm[i];
@@ -611,6 +657,7 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
             for (int32_t k = rs[o]; k < rs[o + 1]; k++) tmp.push_back({iperm[col[k]], k});
             std::sort(tmp.begin(), tmp.end());
             prs[i + 1] = prs[i] + (int32_t)tmp.size();
+            pdiag[i] = -1;
             for (size_t t = 0; t < tmp.size(); t++) {
                 pcol[prs[i] + t] = tmp[t].first;
                 psrc[prs[i] + t] = tmp[t].second;
@@ -621,15 +668,29 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
                 return PYHIP_ERR_BREAKDOWN;
             }
         }
-    }
-    // level sets (lower: depends on smaller columns; upper: on larger columns)
-    std::vector<int32_t> lev(n, 0);
-    int32_t nlo = 0;
-    for (int64_t i = 0; i < n; i++) {
-        int32_t l = 0;
-        for (int32_t k = prs[i]; k < pdiag[i]; k++) l = std::max(l, lev[pcol[k]] + 1);
-        lev[i] = l;
-        nlo = std::max(nlo, l + 1);
+        return PYHIP_OK;
+    };
+    // lower levels (depends on smaller columns)
+    auto lower_levels = [&]() {
+        int32_t nl = 0;
+        for (int64_t i = 0; i < n; i++) {
+            int32_t l = 0;
+            for (int32_t k = prs[i]; k < pdiag[i]; k++) l = std::max(l, lev[pcol[k]] + 1);
+            lev[i] = l;
+            nl = std::max(nl, l + 1);
+        }
+        return nl;
+    };
+    int rc0;
+    bool mc = W->ilu_order == PYHIP_ILU_ORDER_MULTICOLOR;
+    make_perm(mc);
+    if ((rc0 = make_pattern())) return rc0;
+    int32_t nlo = lower_levels();
+    if (W->ilu_order == PYHIP_ILU_ORDER_AUTO && nlo > kIluAutoLevels) {
+        mc = true;
+        make_perm(true);
+        if ((rc0 = make_pattern())) return rc0;
+        nlo = lower_levels();
     }
     auto bucket = [&](int32_t nl, std::vector<int32_t> &rows, std::vector<int64_t> &ptr) {
         ptr.assign(nl + 1, 0);
@@ -641,6 +702,7 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
     };
     std::vector<int32_t> lo_rows, up_rows;
     bucket(nlo, lo_rows, L.lo_ptr);
+    // upper levels (depends on larger columns), computed in decreasing row order
     int32_t nup = 0;
     for (int64_t i = n - 1; i >= 0; i--) {
         int32_t l = 0;
@@ -648,8 +710,9 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
         lev[i] = l;
         nup = std::max(nup, l + 1);
     }
-    // upper levels were computed in decreasing row order; lev[] now holds them
     bucket(nup, up_rows, L.up_ptr);
+    L.requested = W->ilu_order;
+    L.multicolor = mc;
     int rc;
 #define LUP(dst, v)                                                                        \
     if ((rc = kalloc(&(dst), (v).size(), W))) return rc;                                   \
@@ -821,14 +884,23 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
             if ((rc = spmv<TV, T, T>(p, vals, t, w, (const T *)nullptr))) return rc;
             // CGS2
             if ((rc = batched_dots<T>(p, W, k + 1, V, n, w, d1))) return rc;
-            hipLaunchKernelGGL((cgs_update_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, k + 1, ns, V, n,
-                               d1, p->dof_sub, W->active, w);
+            {
+                TimeScope tsu(&p->timer, "vector", p->st);
+                hipLaunchKernelGGL((cgs_update_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, k + 1, ns, V,
+                                   n, d1, p->dof_sub, W->active, w);
+            }
             if ((rc = batched_dots<T>(p, W, k + 1, V, n, w, d2))) return rc;
-            hipLaunchKernelGGL((cgs_update_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, k + 1, ns, V, n,
-                               d2, p->dof_sub, W->active, w);
+            {
+                TimeScope tsu(&p->timer, "vector", p->st);
+                hipLaunchKernelGGL((cgs_update_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, k + 1, ns, V,
+                                   n, d2, p->dof_sub, W->active, w);
+            }
             if ((rc = batched_dots<T>(p, W, 1, w, n, w, d3))) return rc;
-            hipLaunchKernelGGL((scale_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, w, d3, p->dof_sub,
-                               W->active, V + (int64_t)(k + 1) * n);
+            {
+                TimeScope tsu(&p->timer, "vector", p->st);
+                hipLaunchKernelGGL((scale_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, w, d3, p->dof_sub,
+                                   W->active, V + (int64_t)(k + 1) * n);
+            }
             PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
             hipLaunchKernelGGL(gmres_givens_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, m, k, d1, d2,
                                d3, W->H, W->cs, W->sn, W->g, W->bnorm, rtol, W->active, W->kcols,
@@ -1465,6 +1537,7 @@ static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts
     *failed = 0;
     if (o.precision == PYHIP_F64) {
         PYHIP_REQUIRE(jac_prec == PYHIP_F64, "an FP64 Krylov solve needs the FP64 Jacobian");
+        W->ilu_order = o.ilu_order;
         if ((rc = precond_setup<double, double>(p, W, o.precond, W->vals64))) return rc;
         return krylov<double, double>(p, W, W->vals64, r, d, o, o.rtol, 0.0, mask, iters, failed);
     }
@@ -1475,6 +1548,7 @@ static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts
                            p->nnz, W->vals64, W->vals32);
         KCHECK();
     }
+    W->ilu_order = o.ilu_order;
     if ((rc = precond_setup<float, float>(p, W, o.precond, W->vals32))) return rc;
     PYHIP_CHECK_HIP(hipMemsetAsync(d, 0, n * sizeof(double), p->st));
     hipLaunchKernelGGL((convert_kernel<double, double>), dim3(nblk(n)), dim3(256), 0, p->st, n, r, W->r64);
@@ -1529,6 +1603,7 @@ int plugin_setup(pyhip_problem *p, const pyhip_linear_opts &o) {
 }
 double *plugin_values(pyhip_problem *p) { return p->kw->vals64; }
 int plugin_factor(pyhip_problem *p, const pyhip_linear_opts &o) {
+    p->kw->ilu_order = o.ilu_order;
     return precond_setup<double, double>(p, p->kw, o.precond, p->kw->vals64);
 }
 int plugin_solve(pyhip_problem *p, const pyhip_linear_opts &o, const double *b, double *x, int *iters) {
@@ -1561,6 +1636,7 @@ void pyhip_linear_opts_default(pyhip_linear_opts *o) {
     o->rtol_f32 = 1e-5;
     o->max_refinements = 8;
     o->refine_rtol = 1e-10;
+    o->ilu_order = PYHIP_ILU_ORDER_AUTO;
 }
 
 void pyhip_newton_opts_default(pyhip_newton_opts *o) {
@@ -1601,7 +1677,8 @@ void pyhip_krylov_free(pyhip_problem *p) {
 int pyhip_newton_solve(pyhip_problem *p, double *x, const pyhip_newton_opts *o,
                        pyhip_newton_stats *stats) {
     PYHIP_REQUIRE(p && x && o && stats, "null argument");
-    PYHIP_REQUIRE(o->linear.restart >= 1 && o->linear.restart <= 64, "restart must be in 1..64");
+    PYHIP_REQUIRE(o->linear.restart >= 1 && o->linear.restart <= kMaxRestart,
+                  "restart must be in 1..1000");
     PYHIP_REQUIRE(o->jacobian_precision == PYHIP_F64 || o->jacobian_precision == PYHIP_F32,
                   "bad Jacobian precision");
     int rc;

@@ -99,6 +99,9 @@ class Problem:
         # positive definite (Poisson, Bratu, reaction-diffusion), GMRES otherwise
         self.krylov = "auto"
         self.gmres_restart = 50
+        # ILU(0) row order: "auto" (multicolour when the natural order has more than 64
+        # triangular-solve levels), "natural" or "multicolor"
+        self.ilu_order = "auto"
         self.gmres_max_iters = 5000
         self.gmres_rtol = 1e-10
         self.rtol_f32 = 1e-5
@@ -280,6 +283,8 @@ class Problem:
         o.linear.rtol = self.gmres_rtol
         o.linear.max_refinements = self.max_refinements
         o.linear.refine_rtol = self.refine_rtol
+        o.linear.ilu_order = {"auto": _lib.ILU_ORDER_AUTO, "natural": _lib.ILU_ORDER_NATURAL,
+                              "multicolor": _lib.ILU_ORDER_MULTICOLOR}[self.ilu_order]
         return o
 
     def solve(self, raise_on_failure=True):

@@ -101,6 +101,11 @@ def test_fp32_kernels_close_to_fp64(gpu, kind, kw):
                                                    linear_precision="f32", jacobian_precision="f32")),
     ("rd", dict(N=3), dict(krylov="cg", preconditioner="ilu0")),
     ("ns", dict(N=8), dict(krylov="gmres", preconditioner="ilu0")),
+    ("ns", dict(N=12), dict(krylov="gmres", preconditioner="ilu0", ilu_order="natural",
+                            gmres_restart=100)),
+    ("ns", dict(N=16), dict(krylov="gmres", preconditioner="ilu0", ilu_order="multicolor",
+                            gmres_restart=150)),
+    ("poisson", dict(N=20), dict(krylov="gmres", preconditioner="ilu0", ilu_order="multicolor")),
     ("poisson", dict(N=24, seed=2), dict(krylov="cg", preconditioner="amg")),
     ("bratu", dict(N=40, space="C2", lam=2.5), dict(krylov="cg", preconditioner="amg")),
     ("bratu", dict(N=33, space="C1", lam=0.7), dict(krylov="cg", preconditioner="amg")),
@@ -210,6 +215,19 @@ def test_full_size_poisson_against_oracle(gpu):
     P.krylov, P.preconditioner = "cg", "ilu0"
     P.solve()
     xo = on.newton_solve(op, np.zeros(op.ndof))
+    assert rel(P.get_current_dofs(), xo) <= 1e-8
+
+
+def test_cavity_64_multicolor_ilu_against_oracle(gpu):
+    """Lid-driven cavity at 64 x 64 Taylor-Hood elements (36 k unknowns) from u = 0 to
+    max|R| < 1e-10: GMRES(300) with multicolour ILU(0) against the SuperLU Newton."""
+    P, op = build("ns", dict(N=64))
+    P.krylov, P.preconditioner, P.ilu_order, P.gmres_restart = "gmres", "ilu0", "auto", 300
+    P.gmres_max_iters = 8000
+    P.newton_solver_tolerance = 1e-10
+    st = P.solve()
+    assert st[0]["converged"] and st[0]["final_maxres"] < 1e-10
+    xo = on.newton_solve(op, np.zeros(op.ndof), tol=1e-10)
     assert rel(P.get_current_dofs(), xo) <= 1e-8
 
 
