@@ -203,12 +203,16 @@ int pyhip_newton_solve(pyhip_problem *p, double *dofs_dev, const pyhip_newton_op
  * Measurement aid (no counterpart in the reference, whose only timers are wall clocks,
  * problem.cc:8990,9149): while enabled, the problem records a pair of HIP events on its
  * stream around every launch of the named kernel families, "element" (element
- * residual / Jacobian kernels), "gather_values", "gather_residual", "spmv",
- * "precond" (Jacobi / ILU(0) apply and factorisation), "vector" (dots, updates).
+ * residual / Jacobian kernels), "gather_values", "gather_residual", "spmv" (every sparse
+ * matrix-vector launch: the Krylov solver's and, inside the multigrid V-cycle, those of
+ * each level's matrix, restriction and prolongation), "precond" (whole preconditioner
+ * applies and factorisations, including the spmv launches they make), "vector" (dots,
+ * updates).
  * pyhip_timing_read returns the launches and summed device milliseconds so far;
  * pyhip_timing_bytes the algorithmic bytes those launches moved, where the family counts
- * them ("spmv": matrix values and column indices once, row starts, the input vector once
- * per row, the output once per row, over the rows each launch actually processed). */
+ * them ("spmv": matrix values and column indices once, row starts, each entry of the input
+ * vector once, the output and the added vector once per row, over the rows each launch
+ * actually processed). */
 int pyhip_timing_enable(pyhip_problem *p, int enable);
 int pyhip_timing_read(pyhip_problem *p, const char *family, int64_t *launches, double *total_ms);
 int pyhip_timing_bytes(pyhip_problem *p, const char *family, int64_t *bytes);

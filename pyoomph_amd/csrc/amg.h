@@ -26,6 +26,9 @@ struct AmgDevLevel {
     double *own_vals = nullptr;        // A_l values computed here (levels > 0)
     double *d = nullptr, *w = nullptr, *pv = nullptr, *qv = nullptr, *ptv = nullptr;
     double *b = nullptr, *x = nullptr, *x2 = nullptr, *r = nullptr, *pw = nullptr, *py = nullptr;
+    // host: per sub-problem rows, A entries and P entries (for counting the bytes of the
+    // rows a filtered V-cycle visits)
+    std::vector<int64_t> h_rows, h_nnz, h_nnzp;
 };
 
 struct AmgDev {
@@ -42,12 +45,17 @@ struct AmgDev {
 // plan (host) and device buffers, once per pattern; sizes counted in `bytes`
 int amg_build(AmgDev &A, int64_t n, const int32_t *rs_h, const int32_t *col_h,
               const std::vector<int64_t> &sub_off, int64_t &bytes);
-// numeric set-up for the matrix values `vals` (level-0 pattern)
-int amg_setup(AmgDev &A, hipStream_t st, const double *vals);
+struct KernelTimer;
+// numeric set-up for the matrix values `vals` (level-0 pattern); with a timer, the
+// power-iteration spmv launches are timed and counted as "spmv" launches
+int amg_setup(AmgDev &A, hipStream_t st, const double *vals, KernelTimer *tm = nullptr);
 // the finest level's values (the current matrix; the rest of the hierarchy is kept)
 int amg_set_fine(AmgDev &A, const double *vals);
-// z = M^-1 b
-int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z);
+// z = M^-1 b. With a timer, every level spmv is timed and counted as an "spmv" launch.
+// With act (device [nsub], host copy act_h) only the rows of sub-problems with act[s] != 0
+// are computed; the other rows of z keep their values.
+int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer *tm = nullptr,
+              const int32_t *act = nullptr, const int32_t *act_h = nullptr);
 void amg_free(AmgDev &A);
 
 }  // namespace pyhip

@@ -17,6 +17,7 @@
 #include <cstring>
 
 #include "amg.h"
+#include "problem.h"
 #include "spmv.h"
 
 namespace pyhip {
@@ -194,18 +195,24 @@ __global__ __launch_bounds__(256) void amg_ptvals_kernel(int64_t nnzp, const int
 // ---- V-cycle kernels -----------------------------------------------------------------------
 
 // x = W b
+// (rows of sub-problems with act[row_sub] == 0 are skipped when act is given)
 __global__ __launch_bounds__(256) void amg_wb_kernel(int64_t n, const double *__restrict__ w,
-                                                     const double *__restrict__ b, double *__restrict__ x) {
+                                                     const double *__restrict__ b, double *__restrict__ x,
+                                                     const int32_t *__restrict__ rsub,
+                                                     const int32_t *__restrict__ act) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) x[i] = w[i] * b[i];
+    if (i >= n || (act && !act[rsub[i]])) return;
+    x[i] = w[i] * b[i];
 }
 
 // out = x + W r
 __global__ __launch_bounds__(256) void amg_update_kernel(int64_t n, const double *__restrict__ w,
                                                          const double *__restrict__ r, const double *x,
-                                                         double *out) {
+                                                         double *out, const int32_t *__restrict__ rsub,
+                                                         const int32_t *__restrict__ act) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) out[i] = x[i] + w[i] * r[i];
+    if (i >= n || (act && !act[rsub[i]])) return;
+    out[i] = x[i] + w[i] * r[i];
 }
 
 // Dense Cholesky factor of the listed sub-problems' matrices (<= kBottomMaxRows rows),
@@ -254,9 +261,11 @@ __global__ __launch_bounds__(kCholThreads) void amg_chol_solve_kernel(const int3
                                                                       const int64_t *__restrict__ sub_off,
                                                                       const double *__restrict__ chol,
                                                                       const double *__restrict__ b,
-                                                                      double *__restrict__ x) {
+                                                                      double *__restrict__ x,
+                                                                      const int32_t *__restrict__ act) {
     __shared__ double y[kBottomMaxRows];
     const int s = subs[blockIdx.x], t = threadIdx.x;
+    if (act && !act[s]) return;   // the whole workgroup leaves together
     const int64_t r0 = sub_off[s];
     const int m = (int)(sub_off[s + 1] - r0);
     const double *L = chol + (int64_t)s * kBottomMaxRows * kBottomMaxRows;
@@ -321,6 +330,14 @@ static int upload_level(AmgDevLevel &D, const std::vector<int32_t> &rs, const st
     if ((rc = aup(&D.rs, rs, B)) || (rc = aup(&D.col, col, B)) || (rc = aup(&D.diag, diag, B)) ||
         (rc = aup(&D.sub_off, sub_off, B)) || (rc = aup(&D.row_sub, row_sub_of(sub_off), B)))
         return rc;
+    const size_t ns = sub_off.size() - 1;
+    D.h_rows.assign(ns, 0);
+    D.h_nnz.assign(ns, 0);
+    D.h_nnzp.assign(ns, 0);
+    for (size_t s = 0; s < ns; s++) {
+        D.h_rows[s] = sub_off[s + 1] - sub_off[s];
+        D.h_nnz[s] = (int64_t)rs[sub_off[s + 1]] - rs[sub_off[s]];
+    }
     if (own_vals && (rc = dev_alloc(&D.own_vals, D.nnz, B))) return rc;
     if ((rc = dev_alloc(&D.d, D.n, B)) || (rc = dev_alloc(&D.w, D.n, B)) || (rc = dev_alloc(&D.b, D.n, B)) ||
         (rc = dev_alloc(&D.x, D.n, B)) || (rc = dev_alloc(&D.x2, D.n, B)) || (rc = dev_alloc(&D.r, D.n, B)) ||
@@ -353,6 +370,8 @@ int amg_build(AmgDev &A, int64_t n, const int32_t *rs_h, const int32_t *col_h, c
         D.nends = (int)P.ends.size();
         if ((rc = aup(&D.cont, P.cont, B)) || (rc = aup(&D.ends, P.ends, B))) return rc;
         D.nnzp = P.prs.back();
+        for (size_t q = 0; q + 1 < P.sub_off.size(); q++)
+            D.h_nnzp[q] = (int64_t)P.prs[P.sub_off[q + 1]] - P.prs[P.sub_off[q]];
         D.nnzq = P.qrs.back();
         if ((rc = aup(&D.agg, P.agg, B)) || (rc = aup(&D.prs, P.prs, B)) || (rc = aup(&D.pcol, P.pcol, B)) ||
             (rc = aup(&D.pcptr, P.pcptr, B)) || (rc = aup(&D.pcidx, P.pcidx, B)) || (rc = aup(&D.trs, P.trs, B)) ||
@@ -378,13 +397,39 @@ int amg_build(AmgDev &A, int64_t n, const int32_t *rs_h, const int32_t *col_h, c
 // lanes per row of the canonical spmv for a matrix with this mean row length
 static inline int lanes_for(int64_t nnz, int64_t n) { return (n > 0 && nnz <= 12 * n) ? 4 : 8; }
 
-static int level_weights(hipStream_t st, AmgDevLevel &L, const double *vals, double *nrm, int nsub) {
+// One level spmv, y = A x, b - A x (SUB) or b + A x (ADD), timed as an "spmv" launch
+// with its algorithmic bytes (values and column indices, row starts, x, y, b once each).
+enum { kSpmvPlain, kSpmvSub, kSpmvAdd };
+// Rows filtered by (rsub, act): the bytes count the rows, entries and input entries of the
+// visited sub-problems (nnz, rows, ncols are those counts).
+static void level_spmv(KernelTimer *tm, int mode, int lpr, hipStream_t st, int64_t n, int64_t nnz, int64_t rows,
+                       int64_t ncols, const int32_t *rs, const int32_t *col, const double *vals, const double *x,
+                       double *y, const double *b, const int32_t *rsub = nullptr, const int32_t *act = nullptr) {
+    TimeScope ts(tm, "spmv", st);
+    if (mode == kSpmvAdd) launch_spmv_add<double, double, double>(lpr, st, n, rs, col, vals, x, y, b, rsub, act);
+    else
+        launch_spmv<double, double, double>(lpr, st, n, rs, col, vals, x, y, mode == kSpmvSub ? b : nullptr, rsub,
+                                            act);
+    if (tm) tm->add_work("spmv", 12 * nnz + 4 * (rows + 1) + 8 * ncols + 8 * rows + (mode != kSpmvPlain ? 8 * rows : 0));
+}
+
+// sum of v[s] over the sub-problems with act_h[s] != 0 (all without act_h)
+static int64_t active_sum(const std::vector<int64_t> &v, const int32_t *act_h) {
+    int64_t t = 0;
+    for (size_t s = 0; s < v.size(); s++)
+        if (!act_h || act_h[s]) t += v[s];
+    return t;
+}
+
+
+static int level_weights(hipStream_t st, AmgDevLevel &L, const double *vals, double *nrm, int nsub,
+                         KernelTimer *tm) {
     const int64_t n = L.n;
     const int lpr = lanes_for(L.nnz, n);
     hipLaunchKernelGGL(amg_diag_kernel, dim3(ablk(n)), dim3(256), 0, st, n, L.diag, vals, L.d);
     hipLaunchKernelGGL(amg_power_start_kernel, dim3(ablk(n)), dim3(256), 0, st, n, L.row_sub, L.sub_off, L.pw);
     for (int it = 0; it < kPowerIters; it++) {
-        launch_spmv<double, double, double>(lpr, st, n, L.rs, L.col, vals, L.pw, L.py, nullptr);
+        level_spmv(tm, kSpmvPlain, lpr, st, n, L.nnz, n, n, L.rs, L.col, vals, L.pw, L.py, nullptr);
         hipLaunchKernelGGL(amg_sub_norms_kernel, dim3(nsub), dim3(256), 0, st, L.sub_off, L.d, L.py, L.pw, nrm);
         if (it + 1 < kPowerIters)
             hipLaunchKernelGGL(amg_power_scale_kernel, dim3(ablk(n)), dim3(256), 0, st, n, L.row_sub, nrm, L.d, L.py,
@@ -395,7 +440,7 @@ static int level_weights(hipStream_t st, AmgDevLevel &L, const double *vals, dou
     return PYHIP_OK;
 }
 
-int amg_setup(AmgDev &A, hipStream_t st, const double *vals0) {
+int amg_setup(AmgDev &A, hipStream_t st, const double *vals0, KernelTimer *tm) {
     if (!A.built) {
         set_error("multigrid plan not built");
         return PYHIP_ERR_INVALID;
@@ -407,7 +452,7 @@ int amg_setup(AmgDev &A, hipStream_t st, const double *vals0) {
         AmgDevLevel &L = A.lv[l];
         if (l > 0) vals = L.own_vals;
         L.vals = vals;
-        if ((rc = level_weights(st, L, vals, A.nrm, A.nsub))) return rc;
+        if ((rc = level_weights(st, L, vals, A.nrm, A.nsub, tm))) return rc;
         if (L.nends)
             hipLaunchKernelGGL(amg_chol_factor_kernel, dim3(L.nends), dim3(kCholThreads), 0, st, L.ends, L.sub_off,
                                L.rs, L.col, vals, A.chol, A.chol_fail);
@@ -441,16 +486,25 @@ int amg_set_fine(AmgDev &A, const double *vals) {
 }
 
 // z = M^-1 b (both [n0])
-int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z) {
+int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer *tm, const int32_t *act,
+              const int32_t *act_h) {
     const int nl = (int)A.lv.size();
     AmgDevLevel &Bt = A.bot;
     if (nl == 0) {
         if (Bt.nends)
             hipLaunchKernelGGL(amg_chol_solve_kernel, dim3(Bt.nends), dim3(kCholThreads), 0, st, Bt.ends, Bt.sub_off,
-                               A.chol, b, z);
+                               A.chol, b, z, act);
         ACHECK();
         return PYHIP_OK;
     }
+    // visited rows, entries and P entries per level (all sub-problems without a filter)
+    std::vector<int64_t> rows(nl + 1), nnz(nl), nnzp(nl);
+    for (int l = 0; l < nl; l++) {
+        rows[l] = act ? active_sum(A.lv[l].h_rows, act_h) : A.lv[l].n;
+        nnz[l] = act ? active_sum(A.lv[l].h_nnz, act_h) : A.lv[l].nnz;
+        nnzp[l] = act ? active_sum(A.lv[l].h_nnzp, act_h) : A.lv[l].nnzp;
+    }
+    rows[nl] = act ? active_sum(Bt.h_rows, act_h) : Bt.n;
     // down: level l works on right-hand side bl (level 0: b). The rows of sub-problems that
     // end on level l are computed too (their couplings stay in their sub-problem) and are
     // then overwritten by their coarsest solve.
@@ -458,27 +512,34 @@ int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z) {
         AmgDevLevel &L = A.lv[l];
         const double *bl = l == 0 ? b : L.b;
         AmgDevLevel &C = (l + 1 < nl) ? A.lv[l + 1] : A.bot;
-        hipLaunchKernelGGL(amg_wb_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.w, bl, L.x);
-        launch_spmv<double, double, double>(lanes_for(L.nnz, L.n), st, L.n, L.rs, L.col, L.vals, L.x, L.r, bl);
-        if (L.nc) launch_spmv<double, double, double>(8, st, L.nc, L.trs, L.trow, L.ptv, L.r, C.b, nullptr);
+        hipLaunchKernelGGL(amg_wb_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.w, bl, L.x, L.row_sub, act);
+        level_spmv(tm, kSpmvSub, lanes_for(L.nnz, L.n), st, L.n, nnz[l], rows[l], rows[l], L.rs, L.col, L.vals, L.x,
+                   L.r, bl, L.row_sub, act);
+        if (L.nc)
+            level_spmv(tm, kSpmvPlain, 8, st, L.nc, nnzp[l], rows[l + 1], rows[l], L.trs, L.trow, L.ptv, L.r, C.b,
+                       nullptr, C.row_sub, act);
     }
     if (Bt.nends)
         hipLaunchKernelGGL(amg_chol_solve_kernel, dim3(Bt.nends), dim3(kCholThreads), 0, st, Bt.ends, Bt.sub_off,
-                           A.chol, Bt.b, Bt.x);
+                           A.chol, Bt.b, Bt.x, act);
     // up: coarse correction and post-sweep; sub-problems ending on a level get their
     // coarsest solve there
     for (int l = nl - 1; l >= 0; l--) {
         AmgDevLevel &L = A.lv[l];
         const double *bl = l == 0 ? b : L.b;
         AmgDevLevel &C = (l + 1 < nl) ? A.lv[l + 1] : A.bot;
-        if (L.nc) launch_spmv_add<double, double, double>(4, st, L.n, L.prs, L.pcol, L.pv, C.x, L.x, L.x);
-        launch_spmv<double, double, double>(lanes_for(L.nnz, L.n), st, L.n, L.rs, L.col, L.vals, L.x, L.r, bl);
+        if (L.nc)
+            level_spmv(tm, kSpmvAdd, 4, st, L.n, nnzp[l], rows[l], rows[l + 1], L.prs, L.pcol, L.pv, C.x, L.x, L.x,
+                       L.row_sub, act);
+        level_spmv(tm, kSpmvSub, lanes_for(L.nnz, L.n), st, L.n, nnz[l], rows[l], rows[l], L.rs, L.col, L.vals, L.x,
+                   L.r, bl, L.row_sub, act);
         double *out = l == 0 ? z : L.x2;
-        hipLaunchKernelGGL(amg_update_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.w, L.r, L.x, out);
+        hipLaunchKernelGGL(amg_update_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.w, L.r, L.x, out, L.row_sub,
+                           act);
         if (l > 0) std::swap(L.x, L.x2);   // the level's result is in L.x for the prolongation above
         if (L.nends)
             hipLaunchKernelGGL(amg_chol_solve_kernel, dim3(L.nends), dim3(kCholThreads), 0, st, L.ends, L.sub_off,
-                               A.chol, bl, l == 0 ? z : L.x);
+                               A.chol, bl, l == 0 ? z : L.x, act);
     }
     ACHECK();
     return PYHIP_OK;

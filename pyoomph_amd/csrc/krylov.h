@@ -84,6 +84,7 @@ struct KrylovWork {
     std::vector<int64_t> sub_nnz_h;   // [nsub] pattern entries per sub-problem
     std::vector<int32_t> chunk_ptr_h; // [nsub+1] host copy of sub_chunk_ptr
     int lpr = 0;                      // spmv lanes per row (1, 2, 4, 8), 0 = not tuned
+    int32_t *act_h = nullptr;         // [nsub] host copy of the active flags (pinned)
     IluPlan ilu;
     int ilu_order = 0;                // PYHIP_ILU_ORDER_* of the current solve
     AmgDev amg;

@@ -442,6 +442,8 @@ static int krylov_setup(pyhip_problem *p, int m) {
         PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->flags_h, std::max<int64_t>(1, 3 * ns) * sizeof(int32_t),
                                       hipHostMallocDefault));
         PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->nactive_h, sizeof(int32_t), hipHostMallocDefault));
+        PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->act_h, std::max<int64_t>(1, ns) * sizeof(int32_t),
+                                      hipHostMallocDefault));
         PYHIP_CHECK_HIP(hipHostMalloc((void **)&W->maxres_h, ns * sizeof(double), hipHostMallocDefault));
     }
     // restart-length dependent buffers
@@ -553,18 +555,32 @@ static int spmv_tune(pyhip_problem *p, KrylovWork *W, const TV *vals, const TX *
     return PYHIP_OK;
 }
 
-// y = A x, or y = b - A x when b is given.
+// y = A x, or y = b - A x when b is given. With act (device [nsub], host copy act_h)
+// only the rows of sub-problems with act[s] != 0 are computed.
 template <typename TV, typename TX, typename TY>
-static int spmv(pyhip_problem *p, const TV *vals, const TX *x, TY *y, const TY *b) {
+static int spmv(pyhip_problem *p, const TV *vals, const TX *x, TY *y, const TY *b, const int32_t *act = nullptr,
+                const int32_t *act_h = nullptr) {
     const int64_t n = p->ndof;
     if (n == 0) return PYHIP_OK;
     int rc;
     if ((rc = spmv_tune<TV, TX, TY>(p, p->kw, vals, x, y))) return rc;
     TimeScope ts(&p->timer, "spmv", p->st);
-    launch_spmv<TV, TX, TY>(p->kw->lpr, p->st, n, p->row_start, p->col, vals, x, y, b);
+    launch_spmv<TV, TX, TY>(p->kw->lpr, p->st, n, p->row_start, p->col, vals, x, y, b, act ? p->dof_sub : nullptr,
+                            act);
     KCHECK();
-    p->timer.add_work("spmv", (int64_t)(sizeof(TV) + 4) * p->nnz +
-                                  (int64_t)(4 + sizeof(TX) + sizeof(TY) + (b ? sizeof(TY) : 0)) * n + 4);
+    if (p->timer.on) {
+        int64_t rows = n, nnz = p->nnz;
+        if (act) {
+            rows = nnz = 0;
+            for (int s = 0; s < p->nsub; s++)
+                if (act_h[s]) {
+                    rows += p->dof_off_h[s + 1] - p->dof_off_h[s];
+                    nnz += p->kw->sub_nnz_h[s];
+                }
+        }
+        p->timer.add_work("spmv", (int64_t)(sizeof(TV) + 4) * nnz +
+                                      (int64_t)(4 + sizeof(TX) + sizeof(TY) + (b ? sizeof(TY) : 0)) * rows + 4);
+    }
     return PYHIP_OK;
 }
 
@@ -789,11 +805,16 @@ static int ilu_apply(pyhip_problem *p, KrylovWork *W, const T *v, T *z) {
 }
 
 template <typename T>
-static int precond_apply(pyhip_problem *p, KrylovWork *W, int pc, const T *v, T *z) {
+// z = M^-1 v. The multigrid V-cycle computes only the rows of sub-problems with
+// act[s] != 0 when act (device, host copy act_h) is given; the other preconditioners
+// compute every row.
+static int precond_apply(pyhip_problem *p, KrylovWork *W, int pc, const T *v, T *z, const int32_t *act = nullptr,
+                         const int32_t *act_h = nullptr) {
     TimeScope ts(&p->timer, "precond", p->st);
     if (pc == PYHIP_PRECOND_ILU0) return ilu_apply<T>(p, W, v, z);
     if (pc == PYHIP_PRECOND_AMG) {
-        if constexpr (std::is_same<T, double>::value) return amg_apply(W->amg, p->st, v, z);
+        if constexpr (std::is_same<T, double>::value)
+            return amg_apply(W->amg, p->st, v, z, &p->timer, act, act_h);
         set_error("the multigrid preconditioner runs FP64 solves only");
         return PYHIP_ERR_INVALID;
     }
@@ -815,7 +836,7 @@ static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals
             // always uses the current matrix
             if (!W->amg_stale) return amg_set_fine(W->amg, vals);
             W->amg_stale = false;
-            return amg_setup(W->amg, p->st, vals);
+            return amg_setup(W->amg, p->st, vals, &p->timer);
         }
         set_error("the multigrid preconditioner runs FP64 solves only");
         return PYHIP_ERR_INVALID;
@@ -1057,8 +1078,11 @@ static int batched_dot2(pyhip_problem *p, KrylovWork *W, const T *a, const T *b,
     return PYHIP_OK;
 }
 
-static int read_counter(pyhip_problem *p, KrylovWork *W, int32_t *dev, int32_t *out) {
+static int read_counter(pyhip_problem *p, KrylovWork *W, int32_t *dev, int32_t *out,
+                        const int32_t *flags = nullptr, int32_t *flags_h = nullptr) {
     PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, dev, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+    if (flags)
+        PYHIP_CHECK_HIP(hipMemcpyAsync(flags_h, flags, W->nsub * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
     PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
     *out = *W->nactive_h;
     return PYHIP_OK;
@@ -1081,7 +1105,13 @@ static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *
     double *d2 = W->dots;
     PYHIP_CHECK_HIP(hipMemsetAsync(x, 0, n * sizeof(T), p->st));
     hipLaunchKernelGGL((convert_kernel<T, T>), dim3(nblk(n)), dim3(256), 0, p->st, n, b, r);
-    if ((rc = precond_apply<T>(p, W, o.precond, r, z))) return rc;
+    // Matrix and preconditioner work visits only the sub-problems being solved (need, then
+    // those still iterating): the spmv and the multigrid V-cycle skip the rows of the
+    // others, whose scalars the per-sub-problem kernels leave alone.
+    int32_t *act_h = W->act_h;
+    PYHIP_CHECK_HIP(hipMemcpyAsync(act_h, need, ns * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    if ((rc = precond_apply<T>(p, W, o.precond, r, z, need, act_h))) return rc;
     if ((rc = batched_dot2<T>(p, W, r, r, r, z, d2))) return rc;
     PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
     PYHIP_CHECK_HIP(hipMemsetAsync(W->nfail_d, 0, sizeof(int32_t), p->st));
@@ -1090,10 +1120,10 @@ static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *
     hipLaunchKernelGGL((convert_kernel<T, T>), dim3(nblk(n)), dim3(256), 0, p->st, n, z, pv);
     KCHECK();
     int32_t nact = 0;
-    if ((rc = read_counter(p, W, W->nactive_d, &nact))) return rc;
+    if ((rc = read_counter(p, W, W->nactive_d, &nact, W->active, act_h))) return rc;
     int it = 0;
     while (nact > 0 && it < o.max_iters) {
-        if ((rc = spmv<TV, T, T>(p, vals, pv, qv, (const T *)nullptr))) return rc;
+        if ((rc = spmv<TV, T, T>(p, vals, pv, qv, (const T *)nullptr, W->active, act_h))) return rc;
         if ((rc = batched_dots<T>(p, W, 1, pv, n, qv, d2 + 2 * ns))) return rc;
         hipLaunchKernelGGL(cg_alpha_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2 + 2 * ns,
                            W->cg_pq, W->active, W->nfail_d);
@@ -1103,7 +1133,7 @@ static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *
                                W->cg_rz, W->cg_pq, p->dof_sub, W->active);
         }
         KCHECK();
-        if ((rc = precond_apply<T>(p, W, o.precond, r, z))) return rc;
+        if ((rc = precond_apply<T>(p, W, o.precond, r, z, W->active, act_h))) return rc;
         if ((rc = batched_dot2<T>(p, W, r, r, r, z, d2))) return rc;
         PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
         hipLaunchKernelGGL(cg_check_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2, W->bnorm, rtol,
@@ -1115,7 +1145,7 @@ static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *
         }
         KCHECK();
         it++;
-        if ((rc = read_counter(p, W, W->nactive_d, &nact))) return rc;
+        if ((rc = read_counter(p, W, W->nactive_d, &nact, W->active, act_h))) return rc;
     }
     int32_t nf = 0;
     if ((rc = read_counter(p, W, W->nfail_d, &nf))) return rc;
@@ -1248,7 +1278,8 @@ __global__ __launch_bounds__(256) void cg_spmv_kernel(const int32_t *__restrict_
     if (!wl_block(achunk, nachunk, chunk_sub, active, cb, ce, c, sb, part_i, r0, r1)) return;
     constexpr int RPP = kWlRows / LPR;
     T v[LPR];
-    rows_dot<TV, T, T, LPR>(r0, r1, rs, col, val, u, v);
+    bool live[LPR];
+    rows_dot<TV, T, T, LPR>(r0, r1, rs, col, val, u, nullptr, nullptr, v, live);
     if (threadIdx.x % LPR == 0) {
 #pragma unroll
         for (int p = 0; p < LPR; p++) {
@@ -1666,6 +1697,7 @@ void pyhip_krylov_free(pyhip_problem *p) {
     for (void *q : ptrs)
         dfree(q);
     if (W->nactive_h) (void)hipHostFree(W->nactive_h);
+    if (W->act_h) (void)hipHostFree(W->act_h);
     if (W->maxres_h) (void)hipHostFree(W->maxres_h);
     amg_free(W->amg);
     if (W->achunk_h) (void)hipHostFree(W->achunk_h);

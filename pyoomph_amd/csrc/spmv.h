@@ -18,11 +18,14 @@ __device__ __forceinline__ float fmadd(float a, float b, float c) { return __bui
 // A thread serves LPR rows of a 256-row block (rows r0 + p*(256/LPR) + threadIdx.x/LPR,
 // p = 0..LPR-1) and issues the loads of all of them together: 8 (row, stripe) streams in
 // flight per thread, whatever LPR. All lanes of a row call this; out[p] is the sum of
-// row p (rows at or past r1: 0).
+// row p (rows at or past r1: 0). With a row filter (rsub, act non-null) a row whose
+// sub-problem rsub[row] has act[...] == 0 is skipped: live[p] tells which rows to store.
 template <typename TV, typename TX, typename TY, int LPR>
 __device__ __forceinline__ void rows_dot(int64_t r0, int64_t r1, const int32_t *__restrict__ rs,
                                          const int32_t *__restrict__ col, const TV *__restrict__ val,
-                                         const TX *__restrict__ x, TY (&out)[LPR]) {
+                                         const TX *__restrict__ x, const int32_t *__restrict__ rsub,
+                                         const int32_t *__restrict__ act, TY (&out)[LPR],
+                                         bool (&live)[LPR]) {
     constexpr int NS = 8 / LPR, RPP = 256 / LPR;
     const int lane = threadIdx.x % LPR;
     int32_t k0[LPR], k1[LPR];
@@ -30,7 +33,8 @@ __device__ __forceinline__ void rows_dot(int64_t r0, int64_t r1, const int32_t *
 #pragma unroll
     for (int p = 0; p < LPR; p++) {
         const int64_t row = r0 + p * RPP + threadIdx.x / LPR;
-        const bool ok = row < r1;
+        const bool ok = row < r1 && (act == nullptr || act[rsub[row]] != 0);
+        live[p] = ok;
         k0[p] = ok ? rs[row] : 0;
         k1[p] = ok ? rs[row + 1] : 0;
         len = max(len, k1[p] - k0[p]);
@@ -77,13 +81,15 @@ __device__ __forceinline__ void rows_dot(int64_t r0, int64_t r1, const int32_t *
 }
 
 // y = A x (or b - A x); 256 rows per block (rows_dot); the grid is a multiple of 8
-// (spmv_grid).
+// (spmv_grid). Rows the filter skips keep their y.
 template <typename TV, typename TX, typename TY, int LPR>
 __global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__restrict__ rs,
                                                    const int32_t *__restrict__ col,
                                                    const TV *__restrict__ val,
                                                    const TX *__restrict__ x, TY *__restrict__ y,
-                                                   const TY *__restrict__ b) {
+                                                   const TY *__restrict__ b,
+                                                   const int32_t *__restrict__ rsub,
+                                                   const int32_t *__restrict__ act) {
     constexpr int RPP = 256 / LPR;
     // blocks round-robin over the 8 XCDs: XCD x gets the contiguous block range x*per..
     const int64_t nb = (n + 255) / 256, per = (nb + 7) / 8, k = blockIdx.x / 8;
@@ -91,12 +97,13 @@ __global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__r
     if (k >= per || lb >= nb) return;
     const int64_t r0 = lb * 256;
     TY v[LPR];
-    rows_dot<TV, TX, TY, LPR>(r0, n, rs, col, val, x, v);
+    bool live[LPR];
+    rows_dot<TV, TX, TY, LPR>(r0, n, rs, col, val, x, rsub, act, v, live);
     if (threadIdx.x % LPR == 0) {
 #pragma unroll
         for (int p = 0; p < LPR; p++) {
             const int64_t row = r0 + p * RPP + threadIdx.x / LPR;
-            if (row < n) y[row] = b ? b[row] - v[p] : v[p];
+            if (live[p]) y[row] = b ? b[row] - v[p] : v[p];
         }
     }
 }
@@ -107,19 +114,21 @@ __global__ __launch_bounds__(256) void spmv_add_kernel(int64_t n, const int32_t 
                                                        const int32_t *__restrict__ col,
                                                        const TV *__restrict__ val,
                                                        const TX *__restrict__ x, TY *y,
-                                                       const TY *b) {
+                                                       const TY *b, const int32_t *__restrict__ rsub,
+                                                       const int32_t *__restrict__ act) {
     constexpr int RPP = 256 / LPR;
     const int64_t nb = (n + 255) / 256, per = (nb + 7) / 8, k = blockIdx.x / 8;
     const int64_t lb = (int64_t)(blockIdx.x % 8) * per + k;
     if (k >= per || lb >= nb) return;
     const int64_t r0 = lb * 256;
     TY v[LPR];
-    rows_dot<TV, TX, TY, LPR>(r0, n, rs, col, val, x, v);
+    bool live[LPR];
+    rows_dot<TV, TX, TY, LPR>(r0, n, rs, col, val, x, rsub, act, v, live);
     if (threadIdx.x % LPR == 0) {
 #pragma unroll
         for (int p = 0; p < LPR; p++) {
             const int64_t row = r0 + p * RPP + threadIdx.x / LPR;
-            if (row < n) y[row] = b[row] + v[p];
+            if (live[p]) y[row] = b[row] + v[p];
         }
     }
 }
@@ -131,25 +140,27 @@ static inline unsigned spmv_grid(int64_t n) {
 
 template <typename TV, typename TX, typename TY>
 static void launch_spmv(int lpr, hipStream_t st, int64_t n, const int32_t *rs, const int32_t *col,
-                        const TV *vals, const TX *x, TY *y, const TY *b) {
+                        const TV *vals, const TX *x, TY *y, const TY *b, const int32_t *rsub = nullptr,
+                        const int32_t *act = nullptr) {
     const dim3 g(spmv_grid(n)), t(256);
     switch (lpr) {
-    case 1: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 1>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
-    case 2: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 2>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
-    case 4: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 4>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
-    default: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 8>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    case 1: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 1>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
+    case 2: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 2>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
+    case 4: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 4>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
+    default: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 8>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
     }
 }
 
 template <typename TV, typename TX, typename TY>
 static void launch_spmv_add(int lpr, hipStream_t st, int64_t n, const int32_t *rs, const int32_t *col,
-                            const TV *vals, const TX *x, TY *y, const TY *b) {
+                            const TV *vals, const TX *x, TY *y, const TY *b, const int32_t *rsub = nullptr,
+                            const int32_t *act = nullptr) {
     const dim3 g(spmv_grid(n)), t(256);
     switch (lpr) {
-    case 1: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 1>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
-    case 2: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 2>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
-    case 4: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 4>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
-    default: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 8>), g, t, 0, st, n, rs, col, vals, x, y, b); break;
+    case 1: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 1>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
+    case 2: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 2>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
+    case 4: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 4>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
+    default: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 8>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
     }
 }
 
