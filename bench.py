@@ -38,7 +38,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nprob", type=int, default=512)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+    ap.add_argument("--cpu-sample", type=int, default=16,
+                    help="problems in the stratified CPU-baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=40.0,
                     help="bound on the CPU-baseline sample (seconds of oracle work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-single", action="store_true")
@@ -91,6 +93,13 @@ def timing_bytes(P, fam):
     b = ctypes.c_int64()
     _lib.check(_lib.load().pyhip_timing_bytes(P._handle, fam.encode(), ctypes.byref(b)))
     return b.value
+
+
+def wl_ndof(spec):
+    """Unknowns of a sweep problem: interior nodes of N x N P1 or P2 triangles."""
+    N, sp, _ = spec
+    m = 2 * N - 1 if sp == "C2" else N - 1
+    return m * m
 
 
 def configure(P, precision="f64", precond="jacobi"):
@@ -179,8 +188,10 @@ def main():
         alg = 16 * P.ndof   # two vectors read per launch (dot products / updates)
     achieved = alg / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_%s_bratu_sweep.txt" % dom)
-    if os.path.exists(tfile):
+    # measured HBM traffic of this kernel family on this configuration (two rocprofv3 PMC passes)
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s_bratu_sweep%s.txt"
+                         % (dom, "" if args.precond == "amg" else "_" + args.precond))
+    if os.path.exists(tfile) and args.precision == "f64":
         for line in open(tfile):
             if line.startswith("traffic_bytes_per_launch="):
                 traffic = float(line.split("=", 1)[1])
@@ -223,9 +234,12 @@ def main():
                 errs.append(err_of(i, on.newton_solve(op, np.zeros(op.ndof))))
             print("# accuracy sample: %s" % [(specs[i][0], specs[i][1]) for i in picks], file=sys.stderr,
                   flush=True)
+        # timed sample: a stratified slice of the sweep (every len/16-th problem in order of
+        # unknowns), so its solves/s is that of the sweep's size mix
+        k = min(args.cpu_sample, len(order))
+        strat = [order[int((j + 0.5) * len(order) / k)] for j in range(k)]
         n_cpu, t_cpu = 0, 0.0
-        order = sorted(range(len(specs)), key=lambda i: specs[i][0])
-        for i in order:
+        for i in strat:
             N, sp, lam = specs[i]
             op = oracle_problem("bratu", N=N, space=sp, lam=lam)
             t = time.perf_counter()
@@ -235,11 +249,14 @@ def main():
             errs.append(err_of(i, xo))
             if t_cpu > args.cpu_seconds:
                 break
-        sizes = [specs[i][0] for i in order[:n_cpu]]
+        done = strat[:n_cpu]
         cpu = {"value": n_cpu / t_cpu, "unit": "solves/s", "cores": 1, "kind": "port",
                "host_cores": os.cpu_count(),
-               "sample": "%d smallest problems of the sweep (N=%d..%d), oracle C element code + "
-                         "vectors_of_pairs CSR + scipy SuperLU, one thread" % (n_cpu, min(sizes), max(sizes))}
+               "sample": "%d of %d problems of a stratified slice of the sweep (every %d-th by unknowns, "
+                         "%d..%d unknowns), oracle C element code + vectors_of_pairs CSR + scipy SuperLU, "
+                         "one thread, %.1f s" % (n_cpu, k, len(order) // k,
+                                                 min(wl_ndof(specs[i]) for i in done),
+                                                 max(wl_ndof(specs[i]) for i in done), t_cpu)}
 
     max_err = max(errs) if errs else None
     lat = np.array(lat)
