@@ -53,6 +53,12 @@ def parse():
     ap.add_argument("--err-sample", type=int, default=6,
                     help="problems (spread over the size range, largest included) checked against "
                          "the FP64 oracle")
+    ap.add_argument("--serving", default=None, metavar="RATES",
+                    help="instead of the sweep: serve a seeded Poisson-arrival stream of mixed requests "
+                         "at each of these comma-separated rates (req/s) and print one JSON line per rate")
+    ap.add_argument("--serving-requests", type=int, default=48)
+    ap.add_argument("--serving-precond", choices=("jacobi", "amg"), default="jacobi",
+                    help="preconditioner of the CG class of served requests")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB read + 1 GiB write streaming kernel (three launches), the "
                          "reference for scaling HBM byte counters")
@@ -110,8 +116,72 @@ def configure(P, precision="f64", precond="jacobi"):
     P.gmres_max_iters = 20000
 
 
+def serving_main(args):
+    """Served Newton solves: requests (2D P2 Poisson, 3D Q2 reaction-diffusion, 2D
+    Taylor-Hood cavity) of 5 k - 600 k unknowns (cavity up to 12 k) arrive as a seeded
+    Poisson process, each with its mesh; admission under a 200 GB HBM budget, continuous
+    batching per solver class on 4 worker streams (pattern set-up, upload, Newton solve
+    and download inside the latency). Per rate: solves/s over the stream (first arrival
+    to last completion), p50/p99 latency (arrival to result), and the oracle's error on
+    the smallest request of each kind."""
+    import torch
+    from pyoomph_amd.serving import GpuExecutor, ServingLoop, poisson_stream, summarize
+    from oracle import newton as on
+    from tests.oracle_bridge import oracle_problem
+    torch.cuda.set_device(0)
+    ex = GpuExecutor(cg_precond=args.serving_precond)
+    # warm the library and the element kernels once (not timed)
+    ServingLoop(ex, workers=1).run(poisson_stream(1000.0, 3, seed=99, dof_range=(2000, 4000),
+                                                  ns_max_dofs=3000), realtime=False)
+    for rate in [float(r) for r in args.serving.split(",")]:
+        reqs = poisson_stream(rate, args.serving_requests, seed=args.seed,
+                              dof_range=(5_000, 600_000), ns_max_dofs=12_000)
+        for q in reqs:      # requests arrive with their meshes (built before the clock starts)
+            q.attach_payload()
+        loop = ServingLoop(ex, hbm_budget=200e9, workers=4, max_batch_dofs=4_000_000)
+        torch.cuda.synchronize()
+        res = loop.run(reqs, realtime=True)
+        s = summarize(res)
+        errs = {}
+        for kind in ("poisson", "rd", "ns"):
+            cand = [(q, r) for q, r in zip(sorted(reqs, key=lambda q: q.rid), res) if q.kind == kind]
+            if not cand:
+                continue
+            q, r = min(cand, key=lambda qr: qr[0].ndof)
+            mp = q.mesh_problem()
+            if kind == "poisson":
+                op, tol = oracle_problem("poisson", N=q.N, seed=q.seed), 1e-8
+            elif kind == "rd":
+                op, tol = oracle_problem("rd", N=q.N, f=mp.equations.source_field), 1e-8
+            else:
+                op, tol = oracle_problem("ns", N=q.N, Re=100.0), 1e-10
+            xo = on.newton_solve(op, np.zeros(op.ndof), tol=tol)
+            errs[kind] = float(np.max(np.abs(r.dofs - xo)) / max(np.max(np.abs(xo)), 1e-300))
+        kinds = {k: sum(q.kind == k for q in reqs) for k in ("poisson", "rd", "ns")}
+        out = {"metric": "served Newton solves/s & p99 latency at 1e-8 rel err", "value": s["solves_per_s"],
+               "unit": "solves/s", "n_gpus": 1, "higher_is_better": True, "rate_req_per_s": rate,
+               "requests": s["requests"], "p50_ms": s["p50_s"] * 1e3, "p99_ms": s["p99_s"] * 1e3,
+               "mean_batch": s["mean_batch"], "batches": len(loop.batches),
+               "peak_reserved_GB": loop.peak_reserved / 1e9,
+               "converged": int(sum(bool(r.stats.get("converged")) for r in res)),
+               "max_rel_err_vs_oracle": max(errs.values()) if errs else None, "err_by_kind": errs,
+               "dtype": "f64", "data": "synthetic",
+               "config": {"workload": "serving: Poisson arrivals, P2 Poisson / Q2 reaction-diffusion / "
+                                      "Taylor-Hood cavity requests, 5k-600k unknowns (cavity <= 12k)",
+                          "mix": kinds, "total_dofs": int(sum(q.ndof for q in reqs)), "seed": args.seed,
+                          "hbm_budget_GB": 200, "workers": 4, "max_batch_dofs": 4_000_000,
+                          "cg_precond": args.serving_precond}}
+        print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
+    if args.serving:
+        if args.gpus != 1:
+            print("bench.py measures one GPU: --gpus %d is not supported" % args.gpus, file=sys.stderr)
+            sys.exit(2)
+        serving_main(args)
+        return
     if args.precision == "mixed" and args.precond == "amg":
         print("# the multigrid preconditioner runs FP64 solves only: mixed precision uses Jacobi",
               file=sys.stderr)

@@ -78,6 +78,7 @@ class Request:
     N: int             # mesh size (elements per side)
     seed: int
     arrival: float     # seconds after the start of the stream
+    payload: object = field(default=None, repr=False, compare=False)   # the request's mesh problem
 
     @property
     def ndof(self):
@@ -87,7 +88,15 @@ class Request:
     def solver(self):
         return SOLVER_CLASS[self.kind]
 
+    def attach_payload(self):
+        """Builds the request's mesh problem now (the client side of a request: it
+        arrives with its mesh, fields and parameters)."""
+        self.payload = self.mesh_problem()
+        return self
+
     def mesh_problem(self):
+        if self.payload is not None:
+            return self.payload
         from . import workloads as wl
         if self.kind == "poisson":
             return wl.poisson_gaussians(self.N, seed=self.seed)
@@ -257,21 +266,25 @@ def summarize(results):
 
 class GpuExecutor:
     """Runs a batch with the HIP library: one batched ``Problem`` on the worker's stream,
-    FP64 Newton; CG + Jacobi for the symmetric class, GMRES + ILU(0) otherwise."""
+    FP64 Newton; CG for the symmetric class (Jacobi or the multigrid V-cycle,
+    ``cg_precond``), GMRES(``gmres_restart``) + ILU(0) otherwise."""
 
-    def __init__(self, device="cuda:0"):
+    def __init__(self, device="cuda:0", cg_precond="jacobi", gmres_restart=200):
         import torch
         self.torch = torch
         self.device = torch.device(device)
+        self.cg_precond = cg_precond
+        self.gmres_restart = gmres_restart
 
     def make_stream(self):
         return self.torch.cuda.Stream(device=self.device)
 
     def configure(self, P, cls):
         if cls == "cg":
-            P.krylov, P.preconditioner = "cg", "jacobi"
+            P.krylov, P.preconditioner = "cg", self.cg_precond
         else:
             P.krylov, P.preconditioner = "gmres", "ilu0"
+            P.gmres_restart = self.gmres_restart
             P.newton_solver_tolerance = 1e-10
         P.gmres_max_iters = 20000
 
