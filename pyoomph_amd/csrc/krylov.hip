@@ -596,6 +596,7 @@ static int spmv(pyhip_problem *p, const TV *vals, const TX *x, TY *y, const TY *
 // PYHIP_ILU_ORDER_AUTO takes the multicolour order when the natural one has more than
 // kIluAutoLevels lower-solve levels.
 constexpr int kIluAutoLevels = 64;
+constexpr int kGmresCheck = 8;   // GMRES iterations between host reads of the active count
 
 static void ilu_free(KrylovWork *W) {
     IluPlan &L = W->ilu;
@@ -927,10 +928,16 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
                                d3, W->H, W->cs, W->sn, W->g, W->bnorm, rtol, W->active, W->kcols,
                                W->nactive_d);
             KCHECK();
-            PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
-            PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
             total++;
-            if (*W->nactive_h == 0) break;
+            // the host reads the number of sub-problems still iterating every kGmresCheck
+            // iterations (and at the end of the cycle); in between, a sub-problem that
+            // has converged keeps its state (every kernel skips it) while the others go on
+            if ((k + 1) % kGmresCheck == 0 || k + 1 == m) {
+                PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                               p->st));
+                PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+                if (*W->nactive_h == 0) break;
+            }
         }
         // x += M^{-1} V y
         hipLaunchKernelGGL(gmres_backsolve_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, m, W->H, W->g,
