@@ -59,6 +59,11 @@ def parse():
     ap.add_argument("--serving-requests", type=int, default=48)
     ap.add_argument("--serving-precond", choices=("jacobi", "amg"), default="jacobi",
                     help="preconditioner of the CG class of served requests")
+    ap.add_argument("--full", choices=("cavity256", "rd64"), default=None,
+                    help="instead of the sweep: one full-size single-problem solve, the 256x256 "
+                         "Taylor-Hood cavity (GMRES(300) + multicolour ILU(0), FP64) or the 64^3 Q2 "
+                         "reaction-diffusion cube (FP32 ILU(0)-GMRES inside FP64 refinement, compared "
+                         "with the FP64 CG solve of the same problem)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB read + 1 GiB write streaming kernel (three launches), the "
                          "reference for scaling HBM byte counters")
@@ -174,8 +179,69 @@ def serving_main(args):
         print(json.dumps(out), flush=True)
 
 
+def full_main(args):
+    """One full-size problem, solved from zero on the device; prints one JSON line."""
+    import torch
+    from pyoomph_amd import _lib, workloads as wl
+    from pyoomph_amd.problem import Problem
+    torch.cuda.set_device(0)
+    fams = ("element", "gather_values", "spmv", "precond", "vector")
+
+    def run(P):
+        P.dofs.zero_()
+        _lib.check(_lib.load().pyhip_timing_enable(P._handle, 1))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = P.solve(raise_on_failure=False)[0]
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        k = {f: dict(zip(("launches", "ms"), timing(P, f))) for f in fams}
+        _lib.check(_lib.load().pyhip_timing_enable(P._handle, 0))
+        st = {k2: st[k2] for k2 in ("newton_iters", "linear_iters", "linear_failures", "converged",
+                                    "final_maxres")}
+        return el, st, k
+
+    if args.full == "cavity256":
+        t = time.perf_counter()
+        P = Problem(wl.lid_driven_cavity(256, Re=100.0))
+        setup = time.perf_counter() - t
+        P.krylov, P.preconditioner, P.ilu_order, P.gmres_restart = "gmres", "ilu0", "auto", 300
+        P.gmres_max_iters, P.newton_solver_tolerance = 8000, 1e-10
+        el, st, k = run(P)
+        out = {"workload": "2D Taylor-Hood lid-driven cavity, 256x256 Q2/Q1, Re=100, Newton from 0 to "
+                           "max|R| < 1e-10", "ndof": P.ndof, "nnz": P.nnz, "setup_s": setup,
+               "solve_s": el, "solver": "GMRES(300) + multicolour ILU(0), FP64", "stats": st, "kernel_ms": k}
+    else:
+        t = time.perf_counter()
+        P = Problem(wl.reaction_diffusion_cube(64))
+        setup = time.perf_counter() - t
+        # max|R| starts at 4e-7 on this mesh, so pyoomph's absolute 1e-8 stops after one
+        # FP32-Jacobian step (8.5e-6 from the FP64 solution); the run goes on to 1e-15
+        P.krylov, P.preconditioner = "gmres", "ilu0"
+        P.jacobian_precision, P.linear_precision = "f32", "f32"
+        P.gmres_max_iters, P.newton_solver_tolerance = 8000, 1e-15
+        el, st, k = run(P)
+        x_mixed = P.get_current_dofs()
+        P.krylov, P.preconditioner = "cg", "jacobi"
+        P.jacobian_precision, P.linear_precision = "f64", "f64"
+        P.newton_solver_tolerance = 1e-15
+        el64, st64, k64 = run(P)
+        x64 = P.get_current_dofs()
+        diff = float(np.max(np.abs(x_mixed - x64)) / max(np.max(np.abs(x64)), 1e-300))
+        out = {"workload": "3D reaction-diffusion -lap u + u^3 = f, 64^3 Q2 bricks, f = smoothed U(-1,1) "
+                           "noise, Newton from 0 to max|R| < 1e-15", "ndof": P.ndof, "nnz": P.nnz, "setup_s": setup,
+               "solve_s": el, "solver": "FP32 element kernels + FP32 ILU(0)-GMRES(50) inside FP64 iterative "
+                                        "refinement, FP64 residual", "stats": st, "kernel_ms": k,
+               "fp64_cg_jacobi_solve_s": el64, "fp64_stats": st64,
+               "max_rel_diff_vs_fp64_solve": diff}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
+    if args.full:
+        full_main(args)
+        return
     if args.serving:
         if args.gpus != 1:
             print("bench.py measures one GPU: --gpus %d is not supported" % args.gpus, file=sys.stderr)
