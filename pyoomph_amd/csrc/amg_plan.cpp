@@ -11,7 +11,7 @@ namespace {
 template <typename F>
 void parallel_for(int64_t n, F &&fn) {
     unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (n < 50000) nt = 1;
+    if (n < 256) nt = 1;   // coarse levels have few rows but long ones: thread from 256 rows
     const int64_t chunk = (n + nt - 1) / nt;
     std::vector<std::thread> th;
     for (unsigned t = 0; t < nt; t++) {
@@ -22,17 +22,34 @@ void parallel_for(int64_t n, F &&fn) {
     for (auto &x : th) x.join();
 }
 
-// CSR whose row r holds the sorted distinct values produced by gen(r, push)
+// Sorted distinct values of buf (all in [0, ncols)), using a per-thread marker array
+// (mark[x] == stamp: x already taken), so long product lists are not sorted whole.
+static void distinct_sorted(std::vector<int32_t> &buf, std::vector<int64_t> &mark, int64_t stamp) {
+    size_t w = 0;
+    for (size_t t = 0; t < buf.size(); t++) {
+        const int32_t x = buf[t];
+        if (mark[x] != stamp) {
+            mark[x] = stamp;
+            buf[w++] = x;
+        }
+    }
+    buf.resize(w);
+    std::sort(buf.begin(), buf.end());
+}
+
+// CSR whose row r holds the sorted distinct values in [0, ncols) produced by gen(r, push)
 template <typename G>
-bool build_rows(int64_t n, G &&gen, std::vector<int32_t> &rs, std::vector<int32_t> &col, std::string &err) {
+bool build_rows(int64_t n, int64_t ncols, G &&gen, std::vector<int32_t> &rs, std::vector<int32_t> &col,
+                std::string &err) {
     std::vector<int64_t> len(n + 1, 0);
     parallel_for(n, [&](int64_t b, int64_t e) {
         std::vector<int32_t> buf;
+        std::vector<int64_t> mark(std::max<int64_t>(1, ncols), -1);
         for (int64_t r = b; r < e; r++) {
             buf.clear();
             gen(r, buf);
-            std::sort(buf.begin(), buf.end());
-            len[r + 1] = std::unique(buf.begin(), buf.end()) - buf.begin();
+            distinct_sorted(buf, mark, r);
+            len[r + 1] = (int64_t)buf.size();
         }
     });
     rs.assign(n + 1, 0);
@@ -48,11 +65,11 @@ bool build_rows(int64_t n, G &&gen, std::vector<int32_t> &rs, std::vector<int32_
     col.resize(acc);
     parallel_for(n, [&](int64_t b, int64_t e) {
         std::vector<int32_t> buf;
+        std::vector<int64_t> mark(std::max<int64_t>(1, ncols), -1);
         for (int64_t r = b; r < e; r++) {
             buf.clear();
             gen(r, buf);
-            std::sort(buf.begin(), buf.end());
-            buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+            distinct_sorted(buf, mark, r);
             std::copy(buf.begin(), buf.end(), col.begin() + rs[r]);
         }
     });
@@ -145,7 +162,7 @@ int amg_plan_build(int64_t n0, const int32_t *rs0, const int32_t *col0, const st
         const int64_t nc = coff[nsub];
         L.nc = nc;
         // P pattern: aggregates of the row's neighbours
-        if (!build_rows(n, [&](int64_t r, std::vector<int32_t> &out) {
+        if (!build_rows(n, nc, [&](int64_t r, std::vector<int32_t> &out) {
                 if (L.agg[r] < 0) return;   // the sub-problem ends on this level
                 for (int32_t k = rs[r]; k < rs[r + 1]; k++) out.push_back(L.agg[col[k]]);
             }, L.prs, L.pcol, err))
@@ -154,24 +171,26 @@ int amg_plan_build(int64_t n0, const int32_t *rs0, const int32_t *col0, const st
         const int64_t nnzp = L.prs[n];
         L.pcptr.assign(nnzp + 1, 0);
         {
+            // entry of P row r holding aggregate agg(col k) (P rows are sorted and distinct)
+            auto pentry = [&](int64_t r, int32_t k) {
+                const int32_t *b0 = L.pcol.data() + L.prs[r], *b1 = L.pcol.data() + L.prs[r + 1];
+                return (int32_t)(std::lower_bound(b0, b1, L.agg[col[k]]) - L.pcol.data());
+            };
             std::vector<int32_t> cnt(nnzp, 0);
             parallel_for(n, [&](int64_t b, int64_t e) {
                 for (int64_t r = b; r < e; r++)
-                    for (int32_t q = L.prs[r]; q < L.prs[r + 1]; q++) {
-                        int32_t c = 0;
-                        for (int32_t k = rs[r]; k < rs[r + 1]; k++) c += L.agg[col[k]] == L.pcol[q];
-                        cnt[q] = c;
-                    }
+                    if (L.agg[r] >= 0)
+                        for (int32_t k = rs[r]; k < rs[r + 1]; k++) cnt[pentry(r, k)]++;
             });
             for (int64_t q = 0; q < nnzp; q++) L.pcptr[q + 1] = L.pcptr[q] + cnt[q];
             L.pcidx.resize(L.pcptr[nnzp]);
+            // each entry's list in increasing k: a row's entries are walked in k order, and
+            // the entries of P row r belong to row r only
+            std::vector<int32_t> fill(L.pcptr.begin(), L.pcptr.end() - 1);
             parallel_for(n, [&](int64_t b, int64_t e) {
                 for (int64_t r = b; r < e; r++)
-                    for (int32_t q = L.prs[r]; q < L.prs[r + 1]; q++) {
-                        int32_t o = L.pcptr[q];
-                        for (int32_t k = rs[r]; k < rs[r + 1]; k++)
-                            if (L.agg[col[k]] == L.pcol[q]) L.pcidx[o++] = k;
-                    }
+                    if (L.agg[r] >= 0)
+                        for (int32_t k = rs[r]; k < rs[r + 1]; k++) L.pcidx[fill[pentry(r, k)]++] = k;
             });
         }
         // P^T (coarse row J: fine rows in increasing order)
@@ -190,7 +209,7 @@ int amg_plan_build(int64_t n0, const int32_t *rs0, const int32_t *col0, const st
                 }
         }
         // Q = A P pattern
-        if (!build_rows(n, [&](int64_t r, std::vector<int32_t> &out) {
+        if (!build_rows(n, nc, [&](int64_t r, std::vector<int32_t> &out) {
                 for (int32_t k = rs[r]; k < rs[r + 1]; k++) {
                     const int32_t c = col[k];
                     for (int32_t q = L.prs[c]; q < L.prs[c + 1]; q++) out.push_back(L.pcol[q]);
@@ -199,7 +218,7 @@ int amg_plan_build(int64_t n0, const int32_t *rs0, const int32_t *col0, const st
             return -1;
         // A_{l+1} = P^T Q pattern
         std::vector<int32_t> crs, ccol;
-        if (!build_rows(nc, [&](int64_t J, std::vector<int32_t> &out) {
+        if (!build_rows(nc, nc, [&](int64_t J, std::vector<int32_t> &out) {
                 for (int32_t t = L.trs[J]; t < L.trs[J + 1]; t++) {
                     const int32_t r = L.trow[t];
                     for (int32_t q = L.qrs[r]; q < L.qrs[r + 1]; q++) out.push_back(L.qcol[q]);
