@@ -147,7 +147,8 @@ int pyhip_export_reference_csr(pyhip_problem *p, const double *values_dev,
  * NATURAL keeps the equation order; MULTICOLOR sorts each group by a greedy colouring,
  * so the level-scheduled triangular solves need one launch per colour instead of one per
  * level of the natural order (O(rows per side)), for a weaker factor. AUTO takes
- * MULTICOLOR when the natural order has more than 64 lower-solve levels. */
+ * MULTICOLOR for each sub-problem whose natural order has more than 64 lower-solve
+ * levels. */
 #define PYHIP_ILU_ORDER_AUTO 0
 #define PYHIP_ILU_ORDER_NATURAL 1
 #define PYHIP_ILU_ORDER_MULTICOLOR 2

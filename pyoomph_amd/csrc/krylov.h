@@ -15,7 +15,7 @@ constexpr int kWlBlocksPerChunk = kChunk / kWlRows;
 struct IluPlan {
     bool built = false;
     int requested = -1;           // PYHIP_ILU_ORDER_* the plan was built for
-    bool multicolor = false;      // order taken (AUTO resolves to one of the two)
+    bool multicolor = false;      // some sub-problem takes the multicolour order
     int64_t n = 0, nnz = 0;
     int32_t *perm = nullptr;      // [n] new -> old
     int32_t *rs = nullptr;        // [n+1] permuted, column-sorted

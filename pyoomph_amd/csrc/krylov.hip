@@ -593,8 +593,9 @@ static int spmv(pyhip_problem *p, const TV *vals, const TX *x, TY *y, const TY *
 // rows sorted by (colour, index). Rows of one colour of one group are independent, so
 // the triangular solves have at most as many levels as there are colours, against
 // O(rows per side) levels in the natural order, at the price of a weaker factor.
-// PYHIP_ILU_ORDER_AUTO takes the multicolour order when the natural one has more than
-// kIluAutoLevels lower-solve levels.
+// PYHIP_ILU_ORDER_AUTO takes the multicolour order for each sub-problem whose natural
+// order has more than kIluAutoLevels lower-solve levels (decided per sub-problem, so a
+// sub-problem gets the same factor alone and in any batch).
 constexpr int kIluAutoLevels = 64;
 constexpr int kGmresCheck = 8;   // GMRES iterations between host reads of the active count
 
@@ -635,9 +636,13 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
         }
     }
     std::vector<int32_t> perm(n), iperm(n);
-    auto make_perm = [&](bool multicolor) {
+    // multicolour order for the sub-problems with mc[s] (colours are computed for all
+    // rows; the neighbours of a row are in its own sub-problem)
+    auto make_perm = [&](const std::vector<char> &mc) {
         std::vector<int32_t> color(n, 0);
-        if (multicolor) {
+        bool any = false;
+        for (char c : mc) any = any || c;
+        if (any) {
             std::vector<int32_t> mark;   // mark[c] == row: colour c taken by a neighbour
             for (int64_t i = 0; i < n; i++) {
                 for (int32_t k = rs[i]; k < rs[i + 1]; k++) {
@@ -659,8 +664,9 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
                 rows.clear();
                 for (int64_t i = p->dof_off_h[s]; i < p->dof_off_h[s + 1]; i++)
                     if (is_p[i] == g) rows.push_back((int32_t)i);
-                std::stable_sort(rows.begin(), rows.end(),
-                                 [&](int32_t x, int32_t y) { return color[x] < color[y]; });
+                if (mc[s])
+                    std::stable_sort(rows.begin(), rows.end(),
+                                     [&](int32_t x, int32_t y) { return color[x] < color[y]; });
                 for (int32_t r : rows) perm[w++] = r;
             }
         for (int64_t i = 0; i < n; i++) iperm[perm[i]] = (int32_t)i;
@@ -699,16 +705,27 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
         return nl;
     };
     int rc0;
-    bool mc = W->ilu_order == PYHIP_ILU_ORDER_MULTICOLOR;
+    std::vector<char> mc(p->nsub, W->ilu_order == PYHIP_ILU_ORDER_MULTICOLOR ? 1 : 0);
     make_perm(mc);
     if ((rc0 = make_pattern())) return rc0;
     int32_t nlo = lower_levels();
     if (W->ilu_order == PYHIP_ILU_ORDER_AUTO && nlo > kIluAutoLevels) {
-        mc = true;
-        make_perm(true);
-        if ((rc0 = make_pattern())) return rc0;
-        nlo = lower_levels();
+        // per sub-problem, so that its order does not depend on the batch
+        bool any = false;
+        for (int s = 0; s < p->nsub; s++) {
+            int32_t nl = 0;
+            for (int64_t i = p->dof_off_h[s]; i < p->dof_off_h[s + 1]; i++) nl = std::max(nl, lev[i] + 1);
+            mc[s] = nl > kIluAutoLevels;
+            any = any || mc[s];
+        }
+        if (any) {
+            make_perm(mc);
+            if ((rc0 = make_pattern())) return rc0;
+            nlo = lower_levels();
+        }
     }
+    bool mc_any = false;
+    for (char c : mc) mc_any = mc_any || c;
     auto bucket = [&](int32_t nl, std::vector<int32_t> &rows, std::vector<int64_t> &ptr) {
         ptr.assign(nl + 1, 0);
         for (int64_t i = 0; i < n; i++) ptr[lev[i] + 1]++;
@@ -729,7 +746,7 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
     }
     bucket(nup, up_rows, L.up_ptr);
     L.requested = W->ilu_order;
-    L.multicolor = mc;
+    L.multicolor = mc_any;
     int rc;
 #define LUP(dst, v)                                                                        \
     if ((rc = kalloc(&(dst), (v).size(), W))) return rc;                                   \

@@ -218,6 +218,26 @@ def test_full_size_poisson_against_oracle(gpu):
     assert rel(P.get_current_dofs(), xo) <= 1e-8
 
 
+def test_ilu_order_is_decided_per_sub_problem(gpu):
+    """With the automatic ILU(0) order, a small cavity (natural order, fewer than 64
+    levels) keeps its order next to a large one (multicolour): same bits alone and in the
+    batch."""
+    from pyoomph_amd import workloads as wl
+    from pyoomph_amd.problem import Problem
+
+    def solve(parts):
+        P = Problem(parts)
+        P.krylov, P.preconditioner, P.gmres_restart = "gmres", "ilu0", 100
+        P.newton_solver_tolerance = 1e-10
+        st = P.solve()
+        assert all(s["converged"] for s in st)
+        return P, P.get_current_dofs()
+
+    Pb, xb = solve([wl.lid_driven_cavity(24), wl.lid_driven_cavity(4)])
+    Ps, xs = solve([wl.lid_driven_cavity(4)])
+    assert np.array_equal(Pb.sub_dofs(1, xb), Ps.sub_dofs(0, xs))
+
+
 def test_cavity_64_multicolor_ilu_against_oracle(gpu):
     """Lid-driven cavity at 64 x 64 Taylor-Hood elements (36 k unknowns) from u = 0 to
     max|R| < 1e-10: GMRES(300) with multicolour ILU(0) against the SuperLU Newton."""
