@@ -106,6 +106,26 @@ def timing_bytes(P, fam):
     return b.value
 
 
+def copy_bandwidth():
+    """Measured streaming bandwidth: a 1 GiB read + 1 GiB write elementwise kernel
+    (torch.mul into a second FP64 tensor), best of 5, timed with events."""
+    import torch
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.mul(a, 1.0, out=b)
+    best = None
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        torch.mul(a, 1.0, out=b)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del a, b
+    return 2 * (1 << 30) / (best * 1e-3) / 1e9
+
+
 def wl_ndof(spec):
     """Unknowns of a sweep problem: interior nodes of N x N P1 or P2 triangles."""
     N, sp, _ = spec
@@ -308,6 +328,7 @@ def main():
     asm_el = timing(P, "element")[1] / 5
     _lib.check(_lib.load().pyhip_timing_enable(P._handle, 0))
     asm_gbs = assembly_bytes(P) / (asm_ms * 1e-3) / 1e9
+    copy_gbs = copy_bandwidth()
 
     # dominant kernel family of the timed region and its roofline ("precond" times whole
     # preconditioner applies, a composite whose spmv launches are in the "spmv" family)
@@ -423,13 +444,15 @@ def main():
         "assembly_element_kernel_ms": asm_el,
         "assembly_GBps": asm_gbs,
         "assembly_frac_of_hbm_peak": asm_gbs / HBM_PEAK_GBS,
+        "copy_kernel_GBps": copy_gbs,
+        "assembly_frac_of_copy": asm_gbs / copy_gbs,
         "single_problem_fp64_solves_per_s": single,
         "batched_speedup_vs_single": (solves_per_s / single) if single else None,
         "kernel_ms": {f: {"launches": v[0], "ms": v[1]} for f, v in fams.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg_ms,
-                     "share_of_step": ms_l / (elapsed * 1e3),
+                     "share_of_step": ms_l / (elapsed * 1e3), "frac_of_copy": achieved / copy_gbs,
                      "full_matrix_bytes": spmv_bytes(P)},
         "cpu_baseline": cpu,
     }

@@ -12,7 +12,8 @@
 // shape-function columns, skipping pinned equations (local_eqn < 0).
 //
 // Mapping (one workgroup = a block of elements, 64-wide waves):
-//   thread = (element, slot row i); EPB = 256 / NS elements per workgroup.
+//   thread = (element, slot row i) in phase B and (element, integration point) in
+//   phase A; EPB = 256 / max(NS, points per chunk) elements per workgroup.
 //   Per element the nodal coordinates, slot values, equations and auxiliary field are
 //   staged once in LDS (coalesced slot-major gathers). Per integration point, one thread
 //   computes the geometry (tangents, metric inverse, w*det, dx_shape of every node) and
@@ -170,12 +171,18 @@ __device__ __forceinline__ T point_geometry(const T (*X)[DIM], const T *__restri
 // and Jacobian row over the chunk's points in increasing point order.
 template <int KIND> struct ScalarChunk { static constexpr int QC = ElemTraits<KIND>::NQ; };
 template <> struct ScalarChunk<ELEM_HEX_Q2> { static constexpr int QC = 9; };
+// Elements per 256-thread workgroup: every (element, slot row) pair of phase B and every
+// (element, point) pair of a phase-A chunk gets a thread, so each phase is one round.
+template <int KIND>
+constexpr int scalar_epb() {
+    return 256 / (ElemTraits<KIND>::NN > ScalarChunk<KIND>::QC ? ElemTraits<KIND>::NN : ScalarChunk<KIND>::QC);
+}
 
 template <typename T, int KIND, int FORM>
 __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
     using Tr = ElemTraits<KIND>;
     constexpr int NN = Tr::NN, DIM = Tr::DIM, NQ = Tr::NQ, NS = NN;
-    constexpr int EPB = 256 / NS;
+    constexpr int EPB = scalar_epb<KIND>();
     constexpr int QC = ScalarChunk<KIND>::QC;
     constexpr int NQV = 2 + DIM;   // per point: src, dsrc, grad u
     __shared__ T sX[EPB][NN][DIM];
@@ -202,10 +209,14 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
     }
     __syncthreads();
 
+    // phase-A thread: point qa of the current chunk of element la
+    const int la = threadIdx.x / QC, qa = threadIdx.x % QC;
+    const int ea = blockIdx.x * EPB + la;
+    const bool active_a = (la < EPB) && (ea < a.nel);
     T par0 = T(0);
     const double *par = nullptr;
-    if (active) {
-        par = a.params + (int64_t)a.sub[e] * a.pstride;
+    if (active_a) {
+        par = a.params + (int64_t)a.sub[ea] * a.pstride;
         par0 = T(par[0]);
     }
     const bool myrow = active && sEq[le][row] >= 0;
@@ -215,9 +226,10 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
     for (int j = 0; j < NS; j++) Jacc[j] = T(0);
 
     for (int q0 = 0; q0 < NQ; q0 += QC) {
-        // ---- phase A: per-point quantities
-        if (active) {
-            for (int qq = row; qq < QC && q0 + qq < NQ; qq += NS) {
+        // ---- phase A: per-point quantities (element la, point q0 + qa)
+        if (active_a && q0 + qa < NQ) {
+            {
+                const int le = la, qq = qa;
                 const int q = q0 + qq;
                 const T *D = a.tab->dpsi + q * NN * DIM;
                 const T *P = a.tab->psi + q * NN;

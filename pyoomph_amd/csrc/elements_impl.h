@@ -6,8 +6,7 @@ namespace pyhip {
 
 template <typename T, int KIND, int FORM>
 static int launch_scalar(const ElemArgs<T> &a, hipStream_t s) {
-    constexpr int NS = ElemTraits<KIND>::NN;
-    constexpr int EPB = 256 / NS;
+    constexpr int EPB = scalar_epb<KIND>();
     const int blocks = (a.nel + EPB - 1) / EPB;
     if (blocks == 0) return PYHIP_OK;
     hipLaunchKernelGGL((scalar_element_kernel<T, KIND, FORM>), dim3(blocks), dim3(256), 0, s, a);
