@@ -114,7 +114,9 @@ int pyhip_pattern_copy(const pyhip_problem *p, int32_t *row_start, int32_t *col_
  * with the vectors_of_pairs method (problem.cc:4038-4170, 5332-5685) for elements whose
  * functions come from the JIT table (src/elements.cpp:5054-5127).
  *   dofs_dev      [ndof] double
- *   flag          0: residual only, 1: residual and Jacobian (jitbridge.h:285 flag)
+ *   flag          0: residual only, 1: residual and Jacobian, 2: residual and mass
+ *                 matrix dR/d(du/dt) (jitbridge.h:285 flag; 2 needs time stepping on,
+ *                 else the mass matrix is 0)
  *   deriv_param   -1, or p >= 0: d/d(global parameter p) of both
  *                 (get_derivative_wrt_global_parameter, src/problem.cpp:922;
  *                 fill_in_generic_dresidual_contribution_jit, src/elements.cpp:5005-5052)
@@ -131,6 +133,19 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs_dev, int flag, int deriv
  * = 0.0, :110) and columns in first-touch order (problem.cc:5618-5659).
  * row_start_dev [ndof+1], col_dev / val_dev [nnz] (structural nnz is an upper bound).
  * *nnz_out receives the number of entries written. */
+/* ---- time stepping -------------------------------------------------------------------------
+ * Time derivative of the scalar forms for the assemblies and Newton solves that follow:
+ * du/dt = weights[0] u + weights[1] u(t-1) + weights[2] u(t-2), the weak form gaining
+ * (du/dt) psi_i. The weights are the BDF weights of the reference's MultiTimeStepper
+ * (src/timestepper.cpp:32-59; BDF2, degraded to BDF1 before the first completed step,
+ * src/elements.cpp:4583-4636), 3 host values; hist1_dev / hist2_dev [ndof] hold the previous and
+ * second-previous dofs (device, owned by the caller, read at every assembly). enabled = 0
+ * returns to the steady forms. Taylor-Hood sub-problems are rejected. With time stepping
+ * on, pyhip_assemble with flag 2 returns the residual and the mass matrix dR/d(du/dt)
+ * (the reference's flag-2 output, src/jitbridge.h:285) in values. */
+int pyhip_set_time_stepping(pyhip_problem *p, int enabled, const double *weights,
+                            const double *hist1_dev, const double *hist2_dev);
+
 int pyhip_export_reference_csr(pyhip_problem *p, const double *values_dev,
                                int32_t *row_start_dev, int32_t *col_dev, double *val_dev,
                                int64_t *nnz_out);

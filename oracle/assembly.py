@@ -43,7 +43,8 @@ def lib():
         L.oracle_elements.argtypes = [ctypes.c_int] * 6 + [dp] * 4 + [ctypes.c_int, dp, dp, dp,
                                                                        lp, lp, dp, ctypes.c_int,
                                                                        ctypes.c_int, ctypes.c_int,
-                                                                       ctypes.c_int, dp, dp]
+                                                                       ctypes.c_int, dp,
+                                                                       ctypes.c_double, dp, dp]
         L.oracle_vectors_of_pairs.restype = ctypes.c_long
         L.oracle_vectors_of_pairs.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_int, lp, lp,
                                               dp, dp, dp, lp, ctypes.POINTER(lp),
@@ -106,6 +107,30 @@ class OracleProblem:
         rows = np.nonzero(free)
         self.geqn[rows[0], self.leqn[rows]] = g[rows]
 
+    # -- time stepping (scalar forms) --------------------------------------------------
+    # du/dt = tw[0] u + (tw[1] u(t-1) + tw[2] u(t-2)), nodal history combined per node;
+    # pinned nodes keep their boundary value at every time level.
+    time_weights = None
+    hist = None
+
+    def set_time(self, tw, h1, h2):
+        """BDF weights (current, previous, second previous) and the previous and
+        second-previous dof vectors; tw None returns to the steady problem."""
+        if tw is None:
+            self.time_weights, self.hist = None, None
+        else:
+            self.time_weights = np.asarray(tw, dtype=np.float64)
+            self.hist = (np.asarray(h1, dtype=np.float64), np.asarray(h2, dtype=np.float64))
+
+    def _history_slots(self):
+        if self.time_weights is None:
+            return None, 0.0
+        tw = self.time_weights
+        U1 = self.nodal_values(self.hist[0])
+        U2 = self.nodal_values(self.hist[1])
+        H = tw[1] * U1[self.slot_node, self.slot_val] + tw[2] * U2[self.slot_node, self.slot_val]
+        return np.ascontiguousarray(H), float(tw[0])
+
     def nodal_values(self, x):
         U = self.bc.copy()
         m = self.eqn >= 0
@@ -125,6 +150,7 @@ class OracleProblem:
             aux = np.ascontiguousarray(self.aux[mesh.elements])
         R = np.zeros((nel, self.maxdof))
         J = np.zeros((nel, self.maxdof, self.maxdof)) if flag else None
+        H, tw0 = self._history_slots()
         psi1 = tab.get("psi1")
         psi1 = None if psi1 is None else np.ascontiguousarray(psi1)
         rc = lib().oracle_elements(
@@ -132,7 +158,7 @@ class OracleProblem:
             _dp(np.ascontiguousarray(tab["w"])), _dp(np.ascontiguousarray(tab["psi"])),
             _dp(np.ascontiguousarray(tab["dpsi"])), _dp(psi1), self.maxdof, _dp(X), _dp(V),
             _dp(aux), _lp(np.ascontiguousarray(self.leqn)), _lp(self.nvar), _dp(self.params),
-            len(self.params), int(flag), int(deriv), self.maxdof, _dp(R), _dp(J))
+            len(self.params), int(flag), int(deriv), self.maxdof, _dp(H), tw0, _dp(R), _dp(J))
         if rc != 0:
             raise RuntimeError("oracle_elements failed: %d" % rc)
         return R, J
@@ -145,11 +171,16 @@ class OracleProblem:
                                        _lp(np.ascontiguousarray(self.geqn)), _dp(R), _dp(res))
         return res
 
-    def get_jacobian(self, x, deriv=0):
+    def get_mass_matrix(self, x):
+        """The mass matrix dR/d(du/dt) (flag 2 of the element functions) assembled like the
+        Jacobian: residual, row_start, column_index (first-touch order), values."""
+        return self.get_jacobian(x, flag=2)
+
+    def get_jacobian(self, x, deriv=0, flag=1):
         """oomph Problem::get_jacobian (vectors_of_pairs). Returns residual, row_start,
         column_index (first-touch order), values."""
         # element Jacobians: block e starts at e*maxdof^2, rows have stride nvar[e]
-        R, J = self.element_outputs(x, 1, deriv)
+        R, J = self.element_outputs(x, flag, deriv)
         nel = R.shape[0]
         res = np.zeros(self.ndof)
         row_start = np.zeros(self.ndof + 1, dtype=np.int64)

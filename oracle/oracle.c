@@ -107,10 +107,16 @@ static double dot(int dim, const double *a, const double *b)
     return s;
 }
 
-/* Scalar forms 1-3. Slots are the nodes. */
+/* Scalar forms 1-3. Slots are the nodes.
+ * Time derivative (H non-NULL): d/dt u = tw0 * u + H, H the history combination of each
+ * node (tw1 * u(t-1) + tw2 * u(t-2), BDF weights of MultiTimeStepper::set_weights,
+ * src/timestepper.cpp:32-59, selected as in prepare_shape_buffer_for_integration,
+ * src/elements.cpp:4583-4636); the weak form gains (du/dt) * psi_i. flag 2 writes the mass
+ * matrix dR_i/d(du_j/dt) = int psi_j psi_i into J instead of the Jacobian (zero without a
+ * time derivative). */
 static void scalar_element(int form, const ref_t *r, const double *X, const double *U,
                            const double *aux, const long *leqn, const double *par, int npar,
-                           int flag, int deriv, int nvar, double *R, double *J)
+                           int flag, int deriv, int nvar, const double *H, double tw0, double *R, double *J)
 {
     const int dim = r->dim, nn = r->nnode;
     double dx[MAXN][MAXD];
@@ -148,24 +154,42 @@ static void scalar_element(int form, const ref_t *r, const double *X, const doub
             src = u * u * u - f;
             dsrc = 3.0 * (u * u);
         }
+        double ud = 0.0; /* du/dt at the point */
+        if (H) {
+            double hq = 0.0;
+            for (int l = 0; l < nn; l++) hq = hq + H[l] * P[l];
+            ud = tw0 * u + hq;
+        }
         for (int lt = 0; lt < nn; lt++) {
             long ie = leqn[lt];
             if (ie < 0) continue;
             if (deriv)
                 R[ie] = R[ie] + W * (src * P[lt]);
+            else if (H)
+                R[ie] = R[ie] + W * ((dot(dim, gu, dx[lt]) + src * P[lt]) + ud * P[lt]);
             else
                 R[ie] = R[ie] + W * (dot(dim, gu, dx[lt]) + src * P[lt]);
             if (!flag) continue;
+            if (flag == 2 && !H) continue;
             for (int ls = 0; ls < nn; ls++) {
                 long iu = leqn[ls];
                 if (iu < 0) continue;
                 double v;
-                if (deriv)
+                if (flag == 2)
+                    v = W * (P[ls] * P[lt]);
+                else if (deriv)
                     v = W * ((dsrc * P[ls]) * P[lt]);
-                else if (form == 1)
-                    v = W * dot(dim, dx[ls], dx[lt]);
-                else
-                    v = W * (dot(dim, dx[ls], dx[lt]) + (dsrc * P[ls]) * P[lt]);
+                else {
+                    double inner;
+                    if (form == 1)
+                        inner = dot(dim, dx[ls], dx[lt]);
+                    else
+                        inner = dot(dim, dx[ls], dx[lt]) + (dsrc * P[ls]) * P[lt];
+                    if (H)
+                        v = W * (inner + (tw0 * P[ls]) * P[lt]);
+                    else
+                        v = W * inner;
+                }
                 J[ie * nvar + iu] = J[ie * nvar + iu] + v;
             }
         }
@@ -268,7 +292,8 @@ int oracle_elements(int form, int nel, int dim, int nnode, int nqp, int nnode1,
                     const double *w, const double *psi, const double *dpsi, const double *psi1,
                     int nslot, const double *X, const double *V, const double *aux,
                     const long *leqn, const long *nvar, const double *par, int npar,
-                    int flag, int deriv, int maxdof, double *R, double *J)
+                    int flag, int deriv, int maxdof, const double *Hist, double tw0,
+                    double *R, double *J)
 {
     ref_t r = {dim, nnode, nqp, nnode1, w, psi, dpsi, psi1};
     if (nnode > MAXN || dim > MAXD || nslot > 22 + 5) return -1;
@@ -279,11 +304,12 @@ int oracle_elements(int form, int nel, int dim, int nnode, int nqp, int nnode1,
         double *Re = R + (size_t)e * maxdof;
         double *Je = J ? J + (size_t)e * maxdof * maxdof : NULL;
         if (form == 4) {
-            if (deriv) return -2;
+            if (deriv || Hist || flag == 2) return -2;
             th_element(&r, Xe, Ve, Le, par, flag, (int)nvar[e], Re, Je);
         } else {
             const double *Ae = aux ? aux + (size_t)e * nnode : NULL;
-            scalar_element(form, &r, Xe, Ve, Ae, Le, par, npar, flag, deriv, (int)nvar[e], Re, Je);
+            const double *He = Hist ? Hist + (size_t)e * nnode : NULL;
+            scalar_element(form, &r, Xe, Ve, Ae, Le, par, npar, flag, deriv, (int)nvar[e], He, tw0, Re, Je);
         }
     }
     return 0;

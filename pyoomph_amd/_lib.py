@@ -94,6 +94,8 @@ SIGNATURES = {
     "pyhip_newton_opts_default": (None, [ctypes.POINTER(NewtonOpts)]),
     "pyhip_newton_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
                                           ctypes.POINTER(NewtonStats)]),
+    "pyhip_set_time_stepping": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dp, ctypes.c_void_p,
+                                               ctypes.c_void_p]),
     "pyhip_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pyhip_timing_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, c_i64p,
                                          ctypes.POINTER(ctypes.c_double)]),

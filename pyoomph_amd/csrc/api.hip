@@ -338,9 +338,10 @@ int pyhip_pattern_copy(const pyhip_problem *p, int32_t *row_start, int32_t *col_
 int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_param,
                    int precision, double *residual, void *values) {
     PYHIP_REQUIRE(p && dofs && residual, "null argument");
-    PYHIP_REQUIRE(flag == 0 || flag == 1, "flag must be 0 or 1");
+    PYHIP_REQUIRE(flag == 0 || flag == 1 || flag == 2, "flag must be 0, 1 or 2");
     PYHIP_REQUIRE(precision == PYHIP_F64 || precision == PYHIP_F32, "bad precision");
-    PYHIP_REQUIRE(flag == 0 || values, "values buffer required for the Jacobian");
+    PYHIP_REQUIRE(flag == 0 || values, "values buffer required for the Jacobian / mass matrix");
+    PYHIP_REQUIRE(flag != 2 || deriv_param < 0, "no mass matrix of a parameter derivative");
     PYHIP_REQUIRE(deriv_param == -1 || deriv_param == 0, "unknown global parameter");
     for (auto &G : p->groups)
         PYHIP_REQUIRE(deriv_param < 0 || G.form == PYHIP_FORM_BRATU,
@@ -372,6 +373,9 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.Er = p->Er64 + G.er_off;
             a.Ej = flag ? p->Ej64 + G.ej_off : nullptr;
             a.flag = flag; a.deriv = deriv_param >= 0;
+            a.transient = p->transient ? 1 : 0;
+            a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
+            a.hist1 = p->hist1; a.hist2 = p->hist2;
             TimeScope ts(&p->timer, "element", p->st);
             if ((rc = launch_elements_f64(G.kind, G.form, a, p->st))) return rc;
         } else {
@@ -384,6 +388,9 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.Er = p->Er32 + G.er_off;
             a.Ej = flag ? p->Ej32 + G.ej_off : nullptr;
             a.flag = flag; a.deriv = deriv_param >= 0;
+            a.transient = p->transient ? 1 : 0;
+            a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
+            a.hist1 = p->hist1; a.hist2 = p->hist2;
             TimeScope ts(&p->timer, "element", p->st);
             if ((rc = launch_elements_f32(G.kind, G.form, a, p->st))) return rc;
         }
@@ -394,7 +401,7 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             TimeScope ts(&p->timer, "gather_values", p->st);
             if ((rc = launch_gather_values_f64(m, p->Ej64, p->Er64, (double *)values, residual, p->st)))
                 return rc;
-            p->have_ej64 = (deriv_param < 0);
+            p->have_ej64 = (deriv_param < 0);   // the last FP64 matrix (Jacobian or mass)
         } else {
             TimeScope ts(&p->timer, "gather_residual", p->st);
             if ((rc = launch_gather_residual_f64(p->ndof, p->occ_ptr, p->occ_roff, p->Er64, residual, p->st)))
@@ -411,6 +418,26 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
                 return rc;
         }
     }
+    return PYHIP_OK;
+}
+
+int pyhip_set_time_stepping(pyhip_problem *p, int enabled, const double *weights, const double *hist1,
+                            const double *hist2) {
+    PYHIP_REQUIRE(p, "null argument");
+    if (!enabled) {
+        p->transient = false;
+        p->tw[0] = p->tw[1] = p->tw[2] = 0.0;
+        p->hist1 = p->hist2 = nullptr;
+        return PYHIP_OK;
+    }
+    PYHIP_REQUIRE(weights && hist1 && hist2, "time stepping needs weights and two history vectors");
+    for (auto &G : p->groups)
+        PYHIP_REQUIRE(G.form != PYHIP_FORM_NAVIER_STOKES_TH,
+                      "time stepping covers the scalar forms (Poisson, Bratu, reaction-diffusion)");
+    p->transient = true;
+    for (int t = 0; t < 3; t++) p->tw[t] = weights[t];
+    p->hist1 = hist1;
+    p->hist2 = hist2;
     return PYHIP_OK;
 }
 

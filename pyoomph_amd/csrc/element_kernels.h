@@ -52,8 +52,13 @@ struct ElemArgs {
     const DevTables<T> *tab;
     T *Er;                    // [nel][nslot]
     T *Ej;                    // [nel][nslot][nslot] or nullptr
-    int flag;                 // 0 residual, 1 residual + Jacobian
+    int flag;                 // 0 residual, 1 residual + Jacobian, 2 residual + mass matrix
     int deriv;                // 1: d/dlambda (Bratu)
+    // time derivative of the scalar forms: du/dt = tw0 u + (tw1 u(t-1) + tw2 u(t-2)),
+    // history dofs [ndof_total] (pinned nodes keep their boundary value)
+    int transient;
+    double tw0, tw1, tw2;
+    const double *hist1, *hist2;
 };
 
 template <int KIND> struct ElemTraits;
@@ -178,15 +183,18 @@ constexpr int scalar_epb() {
     return 256 / (ElemTraits<KIND>::NN > ScalarChunk<KIND>::QC ? ElemTraits<KIND>::NN : ScalarChunk<KIND>::QC);
 }
 
-template <typename T, int KIND, int FORM>
+// TR: the form has a time derivative (a.transient); a separate instantiation keeps the
+// steady kernels' LDS and registers.
+template <typename T, int KIND, int FORM, bool TR>
 __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
     using Tr = ElemTraits<KIND>;
     constexpr int NN = Tr::NN, DIM = Tr::DIM, NQ = Tr::NQ, NS = NN;
     constexpr int EPB = scalar_epb<KIND>();
     constexpr int QC = ScalarChunk<KIND>::QC;
-    constexpr int NQV = 2 + DIM;   // per point: src, dsrc, grad u
+    constexpr int NQV = 2 + DIM + (TR ? 1 : 0);   // per point: src, dsrc, grad u[, du/dt]
     __shared__ T sX[EPB][NN][DIM];
     __shared__ T sU[EPB][NN];
+    __shared__ T sH[TR ? EPB : 1][NN];   // history combination tw1 u(t-1) + tw2 u(t-2) per node
     __shared__ T sF[FORM == PYHIP_FORM_REACTION_DIFFUSION ? EPB : 1][NN];
     __shared__ int sEq[EPB][NS];
     __shared__ T sW[EPB][QC];
@@ -205,6 +213,11 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
         for (int d = 0; d < DIM; d++) sX[le][row][d] = T(a.coords[(int64_t)node * 3 + d]);
         sU[le][row] = T(eq >= 0 ? a.dofs[eq] : a.pinned[(int64_t)node * a.nvalue]);
         if constexpr (FORM == PYHIP_FORM_REACTION_DIFFUSION) sF[le][row] = T(a.aux[node]);
+        if constexpr (TR) {
+            const double pin = a.pinned[(int64_t)node * a.nvalue];
+            const double u1 = eq >= 0 ? a.hist1[eq] : pin, u2 = eq >= 0 ? a.hist2[eq] : pin;
+            sH[le][row] = T(a.tw1 * u1 + a.tw2 * u2);
+        }
         sEq[le][row] = eq;
     }
     __syncthreads();
@@ -279,6 +292,12 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
                 }
                 sQ[le][qq][0] = src;
                 sQ[le][qq][1] = dsrc;
+                if constexpr (TR) {
+                    T hq = T(0);
+#pragma unroll
+                    for (int l = 0; l < NN; l++) hq = hq + sH[le][l] * P[l];
+                    sQ[le][qq][NQV - 1] = T(a.tw0) * u + hq;
+                }
             }
         }
         __syncthreads();
@@ -290,22 +309,34 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
                 const T W = sW[le][qq];
                 const T *dxi = sDx[le][qq][row];
                 const T src = sQ[le][qq][0], dsrc = sQ[le][qq][1];
+                const bool tr = TR && !a.deriv;
                 if (a.deriv)
                     Racc = Racc + W * (src * P[row]);
+                else if (tr)
+                    Racc = Racc + W * ((dotv<T, DIM>(&sQ[le][qq][2], dxi) + src * P[row]) +
+                                       sQ[le][qq][NQV - 1] * P[row]);
                 else
                     Racc = Racc + W * (dotv<T, DIM>(&sQ[le][qq][2], dxi) + src * P[row]);
+                if (a.flag == 2 && !TR) continue;   // no time derivative: mass matrix 0
                 if (a.flag) {
+                    const T tw0 = T(a.tw0);
 #pragma unroll
                     for (int j = 0; j < NS; j++) {
                         if (sEq[le][j] < 0) continue;
                         const T *dxj = sDx[le][qq][j];
                         T v;
-                        if (a.deriv)
+                        if (a.flag == 2) {
+                            v = W * (P[j] * P[row]);
+                        } else if (a.deriv) {
                             v = W * ((dsrc * P[j]) * P[row]);
-                        else if constexpr (FORM == PYHIP_FORM_POISSON)
-                            v = W * dotv<T, DIM>(dxj, dxi);
-                        else
-                            v = W * (dotv<T, DIM>(dxj, dxi) + (dsrc * P[j]) * P[row]);
+                        } else {
+                            T inner;
+                            if constexpr (FORM == PYHIP_FORM_POISSON)
+                                inner = dotv<T, DIM>(dxj, dxi);
+                            else
+                                inner = dotv<T, DIM>(dxj, dxi) + (dsrc * P[j]) * P[row];
+                            v = tr ? W * (inner + (tw0 * P[j]) * P[row]) : W * inner;
+                        }
                         Jacc[j] = Jacc[j] + v;
                     }
                 }

@@ -149,6 +149,10 @@ struct pyhip_problem {
     // (set for matrices that arrive without element information)
     std::vector<char> ilu_last_h;
     pyhip::KernelTimer timer;
+    // time stepping of the scalar forms (pyhip_set_time_stepping)
+    bool transient = false;
+    double tw[3] = {0.0, 0.0, 0.0};
+    const double *hist1 = nullptr, *hist2 = nullptr;
     pyhip::GatherMap gmap() const {
         return pyhip::GatherMap{ndof, nnz, row_start, occ_ptr, occ_roff, cptr, cidx};
     }

@@ -287,6 +287,91 @@ class Problem:
                               "multicolor": _lib.ILU_ORDER_MULTICOLOR}[self.ilu_order]
         return o
 
+    # -- time stepping ------------------------------------------------------------------
+    # du/dt of the scalar forms with the reference's BDF weights; the history vectors live
+    # on the device next to the dofs.
+    time = 0.0
+    time_steps_done = 0
+    _hist = None
+    _dt_prev = None
+
+    @staticmethod
+    def bdf_weights(dt, dt_prev=None):
+        """Weights of (u, u(t-1), u(t-2)) in du/dt: BDF2 (MultiTimeStepper::set_weights,
+        src/timestepper.cpp:32-59), or BDF1 when no step has been completed (the degraded
+        BDF2 of prepare_shape_buffer_for_integration, src/elements.cpp:4609-4612)."""
+        if dt_prev is None:
+            return np.array([1.0 / dt, -1.0 / dt, 0.0])
+        return np.array([1.0 / dt + 1.0 / (dt + dt_prev), -(dt + dt_prev) / (dt * dt_prev),
+                         dt / ((dt + dt_prev) * dt_prev)])
+
+    def set_initial_condition(self, x=None, t=0.0):
+        """Current dofs (x, host array, or the current ones) become the initial condition:
+        every history level holds them (impulsive start), time = t, no step done."""
+        if x is not None:
+            self.set_current_dofs(x)
+        torch = self.torch
+        with torch.cuda.stream(self._stream):
+            self._hist = [self.dofs.clone(), self.dofs.clone()]
+        self._stream.synchronize()
+        self.time, self.time_steps_done, self._dt_prev = float(t), 0, None
+
+    def set_time_state(self, weights, hist1, hist2):
+        """Time derivative for the following assemblies (weights of u, u(t-1), u(t-2);
+        hist1 / hist2 device tensors [ndof]); weights None returns to the steady forms."""
+        L = _lib.load()
+        if weights is None:
+            _lib.check(L.pyhip_set_time_stepping(self._handle, 0, None, None, None))
+            self._time_keep = None
+            return
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        self._time_keep = (w, hist1, hist2)
+        _lib.check(L.pyhip_set_time_stepping(self._handle, 1, w.ctypes.data_as(_lib.c_dp), _lib.ptr(hist1),
+                                              _lib.ptr(hist2)))
+
+    def unsteady_newton_solve(self, dt, raise_on_failure=True):
+        """One time step of length dt for every sub-problem (oomph's
+        unsteady_newton_solve): the history shifts by one level, time advances, and
+        Newton starts from the current values with the BDF weights of this step."""
+        if self._hist is None:
+            self.set_initial_condition()
+        torch = self.torch
+        with torch.cuda.stream(self._stream):
+            self._hist[1].copy_(self._hist[0])
+            self._hist[0].copy_(self.dofs)
+        w = self.bdf_weights(dt, None if self.time_steps_done == 0 else self._dt_prev)
+        self.set_time_state(w, self._hist[0], self._hist[1])
+        try:
+            st = self.solve(raise_on_failure=raise_on_failure)
+        finally:
+            self.set_time_state(None, None, None)
+        self.time_steps_done += 1
+        self._dt_prev = dt
+        self.time += dt
+        return st
+
+    def assemble_mass_matrix(self):
+        """The mass matrix dR/d(du/dt) (the reference's flag-2 element output) in the
+        reference's CSR form (first-touch columns, exact zeros dropped)."""
+        import scipy.sparse
+        torch = self.torch
+        with torch.cuda.stream(self._stream):
+            res = torch.empty(max(self.ndof, 1), dtype=torch.float64, device=self.device)
+            vals = torch.empty(max(self.nnz, 1), dtype=torch.float64, device=self.device)
+            h = self.dofs.clone()
+        keep = getattr(self, "_time_keep", None)
+        self.set_time_state(self.bdf_weights(1.0), h, h)
+        try:
+            _lib.check(_lib.load().pyhip_assemble(self._handle, _lib.ptr(self.dofs), 2, -1, _lib.F64,
+                                                   _lib.ptr(res), _lib.ptr(vals)))
+            rs, col, val = self._export(vals)
+        finally:
+            if keep is None:
+                self.set_time_state(None, None, None)
+            else:
+                self.set_time_state(*keep)
+        return scipy.sparse.csr_matrix((val, col, rs), shape=(self.ndof, self.ndof))
+
     def solve(self, raise_on_failure=True):
         """Steady Newton solve of every sub-problem; returns per-sub-problem stats."""
         o = self.newton_options()
