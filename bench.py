@@ -64,6 +64,9 @@ def parse():
                          "Taylor-Hood cavity (GMRES(300) + multicolour ILU(0), FP64) or the 64^3 Q2 "
                          "reaction-diffusion cube (FP32 ILU(0)-GMRES inside FP64 refinement, compared "
                          "with the FP64 CG solve of the same problem)")
+    ap.add_argument("--transient", type=int, default=0, metavar="STEPS",
+                    help="instead of the steady sweep: STEPS BDF time steps (dt 0.02, BDF1 first) of "
+                         "du/dt = lap u + lambda e^u for every problem of the sweep from u = 0")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB read + 1 GiB write streaming kernel (three launches), the "
                          "reference for scaling HBM byte counters")
@@ -257,8 +260,58 @@ def full_main(args):
     print(json.dumps(out), flush=True)
 
 
+def transient_main(args):
+    """Time stepping of the whole sweep: every step is one unsteady Newton solve of all
+    512 problems (BDF2 weights, BDF1 on the first step); problem-steps/s over the timed
+    steps, and the oracle's BDF Newton on three problems of the sweep over the same steps."""
+    import torch
+    from pyoomph_amd import workloads as wl
+    from pyoomph_amd.problem import Problem
+    torch.cuda.set_device(0)
+    specs = wl.bratu_sweep_specs(args.nprob, seed=args.seed)
+    P = Problem([wl.bratu(N, sp, lam) for N, sp, lam in specs])
+    configure(P, "f64", args.precond)
+    dts = [0.02] * args.transient
+    P.set_initial_condition(np.zeros(P.ndof))     # warmup: plans, tuning
+    P.unsteady_newton_solve(dts[0])
+    P.set_initial_condition(np.zeros(P.ndof))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    newton = []
+    for dt in dts:
+        st = P.unsteady_newton_solve(dt)
+        assert all(s["converged"] for s in st)
+        newton.append(max(s["newton_iters"] for s in st))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    x = P.get_current_dofs()
+    errs = []
+    from oracle import timestepping as ots
+    from tests.oracle_bridge import oracle_problem
+    order = sorted(range(len(specs)), key=lambda i: wl_ndof(specs[i]))
+    for i in (order[0], order[len(order) // 4], order[len(order) // 2]):
+        N, sp, lam = specs[i]
+        op = oracle_problem("bratu", N=N, space=sp, lam=lam)
+        ts, xo = ots.TimeState(np.zeros(op.ndof)), np.zeros(op.ndof)
+        for dt in dts:
+            xo = ots.unsteady_newton_step(op, xo, ts, dt)
+        xg = P.sub_dofs(i, x)
+        errs.append(float(np.max(np.abs(xg - xo)) / max(np.max(np.abs(xo)), 1e-300)))
+    out = {"metric": "transient Newton problem-steps/s at 1e-8 rel err", "value": args.nprob * len(dts) / elapsed,
+           "unit": "problem-steps/s", "n_gpus": 1, "higher_is_better": True, "steps": len(dts),
+           "ms_per_step": elapsed / len(dts) * 1e3, "newton_iters_per_step": newton,
+           "max_rel_err_vs_oracle": max(errs), "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "bratu_sweep transient: du/dt = lap u + lambda e^u, %d problems, N~U{16..128}, "
+                                  "P1/P2, dt 0.02, BDF2 (BDF1 first step), from u = 0" % args.nprob,
+                      "dofs": P.ndof, "preconditioner": args.precond, "oracle_sample": "3 problems, same steps"}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
+    if args.transient:
+        transient_main(args)
+        return
     if args.full:
         full_main(args)
         return
