@@ -215,6 +215,35 @@ void pyhip_newton_opts_default(pyhip_newton_opts *o);
 int pyhip_newton_solve(pyhip_problem *p, double *dofs_dev, const pyhip_newton_opts *opts,
                        pyhip_newton_stats *stats);
 
+/* ---- arclength continuation ------------------------------------------------------------------
+ * One pseudo-arclength step in global parameter `param` (0: the Bratu lambda) for every
+ * sub-problem with active[s] != 0, each with its own step length ds[s] and state:
+ * oomph's Problem::arc_length_step_solve (problem.cc:10718-11080, no spatial adaptation,
+ * no bifurcation detection) with newton_solve_continuation (problem.cc:9599-9917), the
+ * block elimination of the bordered system (two solves with J per Newton step,
+ * J y = R and J z = dR/dparam), calculate_continuation_derivatives (:9918-10020,
+ * Scale_arc_length with Desired_proportion_of_arc_length 0.5) and the step-length rule
+ * (Desired_newton_iterations_ds 5: x1.5 below, x2/3 above; a failed Newton solve retries
+ * with ds x2/3 down to Minimum_ds 1e-10). Entered by pyoomph's
+ * Problem.arclength_continuation (pyoomph/generic/problem.py:3585, src/problem.cpp:1010).
+ * The library keeps each sub-problem's Dof_current and Dof_derivative; a state with
+ * step_taken = 0 starts from a zero Dof_derivative. The sub-problems' parameters are
+ * updated in place (pyhip_problem_set_params values). ds_next[s] is the suggested next
+ * step. Linear solves use opts->linear (GMRES: J is indefinite past a fold). */
+typedef struct {
+    double theta_squared;        /* Theta_squared (1.0 initially)                       */
+    double parameter_derivative; /* Parameter_derivative (1.0 initially)                */
+    double parameter_current;    /* Parameter_current (set by the step)                 */
+    double ds_current;           /* Ds_current (set by the step)                        */
+    int step_taken;              /* Arc_length_step_taken (0 initially)                 */
+    double parameter;            /* out: the parameter after the step                   */
+    int newton_iters;            /* out: Newton iterations of the accepted solve        */
+    int rejected;                /* out: retries with a shorter step in this call      */
+} pyhip_arclength_state;
+int pyhip_arclength_step(pyhip_problem *p, double *dofs_dev, int param, const pyhip_newton_opts *opts,
+                         const int32_t *active, const double *ds, pyhip_arclength_state *state,
+                         double *ds_next);
+
 /* ---- kernel timing -----------------------------------------------------------------------
  * Measurement aid (no counterpart in the reference, whose only timers are wall clocks,
  * problem.cc:8990,9149): while enabled, the problem records a pair of HIP events on its

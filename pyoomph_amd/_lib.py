@@ -73,6 +73,13 @@ class NewtonStats(ctypes.Structure):
                 ("done_ms", ctypes.c_double)]
 
 
+class ArclengthState(ctypes.Structure):
+    _fields_ = [("theta_squared", ctypes.c_double), ("parameter_derivative", ctypes.c_double),
+                ("parameter_current", ctypes.c_double), ("ds_current", ctypes.c_double),
+                ("step_taken", ctypes.c_int), ("parameter", ctypes.c_double),
+                ("newton_iters", ctypes.c_int), ("rejected", ctypes.c_int)]
+
+
 # every symbol declared in include/pyoomph_hip.h, with (restype, argtypes)
 SIGNATURES = {
     "pyhip_last_error": (ctypes.c_char_p, []),
@@ -94,6 +101,9 @@ SIGNATURES = {
     "pyhip_newton_opts_default": (None, [ctypes.POINTER(NewtonOpts)]),
     "pyhip_newton_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
                                           ctypes.POINTER(NewtonStats)]),
+    "pyhip_arclength_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.POINTER(NewtonOpts), ctypes.POINTER(ctypes.c_int32),
+                                            c_dp, ctypes.POINTER(ArclengthState), c_dp]),
     "pyhip_set_time_stepping": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dp, ctypes.c_void_p,
                                                ctypes.c_void_p]),
     "pyhip_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),

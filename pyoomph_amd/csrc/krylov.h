@@ -85,6 +85,11 @@ struct KrylovWork {
     std::vector<int32_t> chunk_ptr_h; // [nsub+1] host copy of sub_chunk_ptr
     int lpr = 0;                      // spmv lanes per row (1, 2, 4, 8), 0 = not tuned
     int32_t *act_h = nullptr;         // [nsub] host copy of the active flags (pinned)
+    // arclength continuation: Dof_current, Dof_derivative, y, z, scratch [n]; scalars [nsub]
+    double *ac_cur = nullptr, *ac_der = nullptr, *ac_y = nullptr, *ac_z = nullptr, *ac_t = nullptr;
+    double *ac_zs = nullptr;          // z of each sub-problem's last continuation Newton step
+    double *ac_alpha = nullptr;
+    int32_t *ac_mask = nullptr;
     IluPlan ilu;
     int ilu_order = 0;                // PYHIP_ILU_ORDER_* of the current solve
     AmgDev amg;

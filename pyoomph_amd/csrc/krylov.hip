@@ -362,6 +362,30 @@ __global__ __launch_bounds__(256) void permute_kernel(int64_t n, const int32_t *
     else b[perm[i]] = TB(a[i]);
 }
 
+// ---- arclength continuation vector operations (per sub-problem scalars, masked) ------
+// op 0: out = a + alpha[s] b        (predictor: Dof_current + Ds Dof_derivative)
+// op 1: out = out - (a - alpha[s] b) (Newton update: x -= y - dparam z)
+// op 2: out = -alpha[s] a            (Dof_derivative = -Parameter_derivative z)
+// op 3: out = a - b                  (x - Dof_current)
+// op 4: out = a                      (masked copy)
+// op 5: out = 0
+__global__ __launch_bounds__(256) void cont_vec_kernel(int op, int64_t n, const int32_t *__restrict__ dsub,
+                                                       const int32_t *__restrict__ mask,
+                                                       const double *__restrict__ alpha,
+                                                       const double *__restrict__ a,
+                                                       const double *__restrict__ b, double *out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int s = dsub[i];
+    if (!mask[s]) return;
+    if (op == 0) out[i] = a[i] + alpha[s] * b[i];
+    else if (op == 1) out[i] = out[i] - (a[i] - alpha[s] * b[i]);
+    else if (op == 2) out[i] = -alpha[s] * a[i];
+    else if (op == 3) out[i] = a[i] - b[i];
+    else if (op == 4) out[i] = a[i];
+    else out[i] = 0.0;
+}
+
 // ------------------------------------------------------------------------------------
 // host side
 
@@ -1717,7 +1741,8 @@ void pyhip_krylov_free(pyhip_problem *p) {
                     W->cg_beta, W->nfail_d, W->ilu.perm, W->ilu.rs, W->ilu.col,
                     W->ilu.src, W->ilu.diag, W->ilu.lo_rows, W->ilu.up_rows, W->ilu.lu,
                     W->ilu.tmp64, W->ilu.tmp64b, W->ilu.tmp32, W->ilu.tmp32b, W->achunk,
-                    W->gpart, W->cg_iters, W->cg_fail};
+                    W->gpart, W->cg_iters, W->cg_fail, W->ac_cur, W->ac_der, W->ac_y, W->ac_z,
+                    W->ac_t, W->ac_zs, W->ac_alpha, W->ac_mask};
     for (void *q : ptrs)
         dfree(q);
     if (W->nactive_h) (void)hipHostFree(W->nactive_h);
@@ -1822,6 +1847,202 @@ int pyhip_newton_solve(pyhip_problem *p, double *x, const pyhip_newton_opts *o,
         set_error("NewtonSolverError: a sub-problem did not converge (see stats)");
         return PYHIP_ERR_NEWTON;
     }
+    return PYHIP_OK;
+}
+
+// Arclength continuation (declared in pyoomph_hip.h): the reference's scalar recursion per
+// sub-problem on the host, the vectors and solves batched on the device.
+int pyhip_arclength_step(pyhip_problem *p, double *x, int param, const pyhip_newton_opts *o,
+                         const int32_t *active, const double *ds, pyhip_arclength_state *st,
+                         double *ds_next) {
+    PYHIP_REQUIRE(p && x && o && active && ds && st && ds_next, "null argument");
+    PYHIP_REQUIRE(param == 0, "unknown global parameter");
+    PYHIP_REQUIRE(o->jacobian_precision == PYHIP_F64 && o->linear.precision == PYHIP_F64,
+                  "arclength continuation runs FP64 solves");
+    for (auto &G : p->groups)
+        PYHIP_REQUIRE(G.form == PYHIP_FORM_BRATU, "arclength continuation in lambda needs Bratu sub-problems");
+    int rc;
+    if ((rc = krylov_setup(p, o->linear.restart))) return rc;
+    KrylovWork *W = p->kw;
+    W->amg_stale = true;
+    const int ns = p->nsub;
+    const int64_t n = p->ndof;
+    if (!W->vals64 && (rc = dev_alloc(&W->vals64, p->nnz, W->bytes))) return rc;
+    if (!W->ac_cur) {
+        if ((rc = kalloc(&W->ac_cur, std::max<int64_t>(1, n), W)) || (rc = kalloc(&W->ac_der, std::max<int64_t>(1, n), W)) ||
+            (rc = kalloc(&W->ac_y, std::max<int64_t>(1, n), W)) || (rc = kalloc(&W->ac_z, std::max<int64_t>(1, n), W)) ||
+            (rc = kalloc(&W->ac_t, std::max<int64_t>(1, n), W)) || (rc = kalloc(&W->ac_zs, std::max<int64_t>(1, n), W)) ||
+            (rc = kalloc(&W->ac_alpha, std::max(1, ns), W)) ||
+            (rc = kalloc(&W->ac_mask, std::max(1, ns), W)))
+            return rc;
+        PYHIP_CHECK_HIP(hipMemsetAsync(W->ac_der, 0, std::max<int64_t>(1, n) * sizeof(double), p->st));
+    }
+    const double kMinDs = 1.0e-10, kDesiredProp = 0.5;
+    const int kDesiredIters = 5;
+    std::vector<int32_t> mask(ns, 0);
+    std::vector<double> alpha(ns, 0.0), lam(ns), acr(ns), maxres(ns), dots(2 * ns);
+    auto upload = [&](const std::vector<int32_t> &m, const std::vector<double> &a) -> int {
+        PYHIP_CHECK_HIP(hipMemcpyAsync(W->ac_mask, m.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, p->st));
+        PYHIP_CHECK_HIP(hipMemcpyAsync(W->ac_alpha, a.data(), ns * sizeof(double), hipMemcpyHostToDevice, p->st));
+        return PYHIP_OK;
+    };
+    auto vec = [&](int op, const double *a, const double *b, double *out) -> int {
+        hipLaunchKernelGGL(cont_vec_kernel, dim3(nblk(n)), dim3(256), 0, p->st, op, n, p->dof_sub, W->ac_mask,
+                           W->ac_alpha, a, b, out);
+        KCHECK();
+        return PYHIP_OK;
+    };
+    auto set_lambda = [&]() -> int {
+        for (int s = 0; s < ns; s++) p->params_h[(size_t)s * kParamStride + param] = lam[s];
+        PYHIP_CHECK_HIP(hipMemcpyAsync(p->params, p->params_h.data(), p->params_h.size() * sizeof(double),
+                                       hipMemcpyHostToDevice, p->st));
+        return PYHIP_OK;
+    };
+    // per-sub-problem dot of two vectors, on the host
+    auto sub_dot = [&](const double *a, const double *b, double *out) -> int {
+        int r2;
+        if ((r2 = batched_dots<double>(p, W, 1, a, n, b, W->dots))) return r2;
+        PYHIP_CHECK_HIP(hipMemcpyAsync(out, W->dots, ns * sizeof(double), hipMemcpyDeviceToHost, p->st));
+        PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+        return PYHIP_OK;
+    };
+    // max |R| per sub-problem and the arclength constraint residual (newton_solve_continuation)
+    auto residuals = [&](const std::vector<int32_t> &m) -> int {
+        int r2;
+        if ((r2 = pyhip_assemble(p, x, 0, -1, PYHIP_F64, W->res64, nullptr))) return r2;
+        if ((r2 = batched_maxabs(p, W->res64, W->maxres))) return r2;
+        PYHIP_CHECK_HIP(hipMemcpyAsync(maxres.data(), W->maxres, ns * sizeof(double), hipMemcpyDeviceToHost, p->st));
+        if ((r2 = upload(m, alpha)) || (r2 = vec(3, x, W->ac_cur, W->ac_t))) return r2;
+        if ((r2 = sub_dot(W->ac_der, W->ac_t, dots.data()))) return r2;
+        for (int s = 0; s < ns; s++) {
+            if (!m[s]) continue;
+            double c = dots[s] * st[s].theta_squared;
+            c += st[s].parameter_derivative * (lam[s] - st[s].parameter_current) - st[s].ds_current;
+            acr[s] = c;
+            if (std::fabs(c) > maxres[s]) maxres[s] = std::fabs(c);
+        }
+        return PYHIP_OK;
+    };
+
+    // entry: Parameter_current, Dof_current, Ds_current (problem.cc:10835-10853)
+    std::vector<int32_t> act(ns, 0);
+    for (int s = 0; s < ns; s++) {
+        act[s] = active[s] && p->dof_off_h[s + 1] > p->dof_off_h[s];
+        lam[s] = p->params_h[(size_t)s * kParamStride + param];
+        st[s].newton_iters = 0;
+        st[s].rejected = 0;
+        ds_next[s] = ds[s];
+        if (!act[s]) continue;
+        st[s].parameter_current = lam[s];
+        st[s].ds_current = ds[s];
+        alpha[s] = 0.0;
+    }
+    // Dof_current = x; a sub-problem without a step taken starts from Dof_derivative = 0
+    {
+        std::vector<int32_t> fresh(ns, 0);
+        for (int s = 0; s < ns; s++) fresh[s] = act[s] && !st[s].step_taken;
+        if ((rc = upload(act, alpha)) || (rc = vec(4, x, nullptr, W->ac_cur))) return rc;
+        if ((rc = upload(fresh, alpha)) || (rc = vec(5, nullptr, nullptr, W->ac_der))) return rc;
+    }
+    std::vector<int32_t> pending = act, failed_sub(ns, 0);
+    std::vector<int> count(ns, 0);
+    while (true) {
+        int npend = 0;
+        for (int s = 0; s < ns; s++) npend += pending[s];
+        if (!npend) break;
+        // predictor (problem.cc:10870-10899)
+        for (int s = 0; s < ns; s++) {
+            if (!pending[s]) continue;
+            if (std::fabs(st[s].ds_current) < kMinDs) {
+                set_error("DESIRED ARC-LENGTH STEP HAS FALLEN BELOW MINIMUM TOLERANCE");
+                return PYHIP_ERR_NEWTON;
+            }
+            lam[s] = st[s].parameter_current + st[s].parameter_derivative * st[s].ds_current;
+            st[s].ds_current = (lam[s] - st[s].parameter_current) / st[s].parameter_derivative;
+            alpha[s] = st[s].ds_current;
+        }
+        if ((rc = set_lambda()) || (rc = upload(pending, alpha)) || (rc = vec(0, W->ac_cur, W->ac_der, x))) return rc;
+        // newton_solve_continuation, batched over the pending sub-problems
+        std::vector<int32_t> run = pending;
+        for (int s = 0; s < ns; s++) count[s] = 0, failed_sub[s] = 0;
+        if ((rc = residuals(run))) return rc;   // count == 1 check (always one step)
+        while (true) {
+            int nrun = 0;
+            for (int s = 0; s < ns; s++) nrun += run[s];
+            if (!nrun) break;
+            for (int s = 0; s < ns; s++) count[s] += run[s];
+            PYHIP_CHECK_HIP(hipMemcpyAsync(W->dof_mask, run.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, p->st));
+            // J y = R and J z = dR/dparam with the same Jacobian
+            if ((rc = pyhip_assemble(p, x, 1, -1, PYHIP_F64, W->w64, W->vals64))) return rc;
+            int its = 0, refs = 0, lfail = 0;
+            if ((rc = linear_solve(p, W, o->linear, PYHIP_F64, W->res64, W->ac_y, W->dof_mask, &its, &refs, &lfail)))
+                return rc;
+            if ((rc = pyhip_assemble(p, x, 0, param, PYHIP_F64, W->ac_t, nullptr))) return rc;
+            if ((rc = linear_solve(p, W, o->linear, PYHIP_F64, W->ac_t, W->ac_z, W->dof_mask, &its, &refs, &lfail)))
+                return rc;
+            if ((rc = sub_dot(W->ac_der, W->ac_y, dots.data())) || (rc = sub_dot(W->ac_der, W->ac_z, dots.data() + ns)))
+                return rc;
+            for (int s = 0; s < ns; s++) {
+                if (!run[s]) continue;
+                const double uy = dots[s] * st[s].theta_squared, uz = dots[ns + s] * st[s].theta_squared;
+                const double dparam = (acr[s] - uy) / (st[s].parameter_derivative - uz);
+                lam[s] -= dparam;
+                alpha[s] = dparam;
+            }
+            // x -= y - dparam z; z of this iteration kept per sub-problem (each solve
+            // restarts every row of its output from 0)
+            if ((rc = set_lambda()) || (rc = upload(run, alpha)) || (rc = vec(1, W->ac_y, W->ac_z, x)) ||
+                (rc = vec(4, W->ac_z, nullptr, W->ac_zs)))
+                return rc;
+            if ((rc = residuals(run))) return rc;
+            for (int s = 0; s < ns; s++) {
+                if (!run[s]) continue;
+                if (maxres[s] < o->tol) run[s] = 0;
+                else if (maxres[s] > o->max_residuals || count[s] >= o->max_iter) run[s] = 0, failed_sub[s] = 1;
+            }
+        }
+        // failed Newton solves retry with a shorter step (problem.cc:10918-10928)
+        for (int s = 0; s < ns; s++) {
+            if (!pending[s]) continue;
+            if (failed_sub[s]) {
+                st[s].ds_current *= (2.0 / 3.0);
+                st[s].rejected++;
+            } else {
+                pending[s] = 0;
+                st[s].newton_iters = count[s];
+            }
+        }
+    }
+    // calculate_continuation_derivatives(z) with the last z (problem.cc:9995-10020, :10115-10200)
+    std::vector<double> zz(ns), dz(ns);
+    if ((rc = sub_dot(W->ac_zs, W->ac_zs, zz.data())) || (rc = sub_dot(W->ac_der, W->ac_zs, dz.data()))) return rc;
+    // Continuation_direction = Parameter_derivative - Dof_derivative . z, then
+    // Parameter_derivative = +-1/sqrt(1 + Theta^2 z.z) with its sign, Dof_derivative = -Pd z
+    auto helper = [&](int s, double der_dot_z) {
+        const double cdir = st[s].parameter_derivative - der_dot_z;
+        double pd = 1.0 / std::sqrt(1.0 + st[s].theta_squared * zz[s]);
+        if (pd * cdir < 0.0) pd = -pd;
+        st[s].parameter_derivative = pd;
+    };
+    for (int s = 0; s < ns; s++) {
+        if (!act[s]) continue;
+        helper(s, dz[s]);
+        const double pd = st[s].parameter_derivative;
+        if (pd != 1.0) {
+            st[s].theta_squared *= (pd * pd / kDesiredProp) * ((1.0 - kDesiredProp) / (1.0 - pd * pd));
+            helper(s, -pd * zz[s]);   // Dof_derivative is -pd z after the first call
+        }
+        alpha[s] = st[s].parameter_derivative;
+        if (!st[s].step_taken)
+            st[s].ds_current = (lam[s] - st[s].parameter_current) / st[s].parameter_derivative;
+        st[s].step_taken = 1;
+        const int c = st[s].newton_iters;
+        ds_next[s] = c < kDesiredIters ? st[s].ds_current * 1.5
+                                       : (c > kDesiredIters ? st[s].ds_current * (2.0 / 3.0) : st[s].ds_current);
+    }
+    if ((rc = upload(act, alpha)) || (rc = vec(2, W->ac_zs, nullptr, W->ac_der))) return rc;
+    for (int s = 0; s < ns; s++) st[s].parameter = lam[s];
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
     return PYHIP_OK;
 }
 

@@ -372,6 +372,52 @@ class Problem:
                 self.set_time_state(*keep)
         return scipy.sparse.csr_matrix((val, col, rs), shape=(self.ndof, self.ndof))
 
+    # -- arclength continuation -----------------------------------------------------------
+    _arc = None
+
+    def reset_arc_length_parameters(self):
+        """Theta_squared = Parameter_derivative = 1, no step taken, zero dof derivative."""
+        self._arc = (_lib.ArclengthState * len(self.parts))()
+        for a in self._arc:
+            a.theta_squared, a.parameter_derivative, a.step_taken = 1.0, 1.0, 0
+
+    def arclength_continuation(self, parameter, ds, max_ds=None, active=None):
+        """One pseudo-arclength step in the global parameter (the Bratu "lambda") for
+        every sub-problem (or those with active[i]), each with its own step ds (scalar or
+        per sub-problem) and continuation state: pyoomph's Problem.arclength_continuation
+        (pyoomph/generic/problem.py:3530-3600; max_ds clamps the step as there) over
+        oomph's arc_length_step_solve. Returns the suggested next steps (array). Use
+        GMRES: the Jacobian is indefinite past a fold."""
+        self._param_index(parameter)
+        ns = len(self.parts)
+        if self._arc is None:
+            self.reset_arc_length_parameters()
+        ds = np.broadcast_to(np.asarray(ds, dtype=np.float64), (ns,)).copy()
+        if max_ds is not None:
+            ds = np.minimum(np.abs(ds), abs(float(max_ds))) * np.where(ds > 0, 1.0, -1.0)
+        act = np.ones(ns, dtype=np.int32) if active is None else np.asarray(active, dtype=np.int32).copy()
+        nxt = np.zeros(ns)
+        o = self.newton_options()
+        rc = _lib.load().pyhip_arclength_step(self._handle, _lib.ptr(self.dofs), 0, ctypes.byref(o),
+                                               act.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                               ds.ctypes.data_as(_lib.c_dp), self._arc,
+                                               nxt.ctypes.data_as(_lib.c_dp))
+        self._stream.synchronize()
+        if rc == _lib.PYHIP_ERR_NEWTON:
+            raise NewtonSolverError(_lib.load().pyhip_last_error().decode())
+        _lib.check(rc)
+        return nxt
+
+    def global_parameters(self):
+        """Current value of the continued parameter per sub-problem (after a step)."""
+        if self._arc is None:
+            return None
+        return np.array([a.parameter for a in self._arc])
+
+    def arclength_stats(self):
+        return [{"newton_iters": a.newton_iters, "rejected": a.rejected, "theta_squared": a.theta_squared,
+                 "parameter_derivative": a.parameter_derivative} for a in self._arc]
+
     def solve(self, raise_on_failure=True):
         """Steady Newton solve of every sub-problem; returns per-sub-problem stats."""
         o = self.newton_options()

@@ -1,0 +1,77 @@
+"""Pseudo-arclength continuation on the GPU (batched: each Bratu problem with its own
+lambda, step length and continuation state) against the oracle's restatement of oomph's
+arc_length_step_solve. Bar: every continuation point is a solution for the oracle
+(max|R(x, lambda)| < 1e-8 with the oracle's element code); while the Newton iteration
+counts agree (they set the next step length), lambda and the dofs of every step agree with
+the oracle's to 1e-8; a sub-problem gives the same bits alone and in a batch; the branch
+turns at the fold."""
+import numpy as np
+import pytest
+
+from oracle import continuation as oc
+from oracle.newton import max_abs
+from tests.oracle_bridge import oracle_problem, product_problem
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [dict(N=8, space="C2"), dict(N=10, space="C1"), dict(N=6, space="C2")]
+
+
+def _gpu_run(specs, steps, ds=0.5, max_ds=0.3):
+    from pyoomph_amd.problem import Problem
+    P = Problem([product_problem("bratu", lam=0.0, **sp) for sp in specs])
+    P.krylov, P.preconditioner, P.gmres_restart = "gmres", "ilu0", 100
+    P.gmres_rtol = 1e-12
+    dsv = np.full(len(specs), ds)
+    out = []
+    for _ in range(steps):
+        dsv = P.arclength_continuation("lambda", dsv, max_ds=max_ds)
+        x = P.get_current_dofs()
+        out.append((P.global_parameters().copy(), [P.sub_dofs(i, x).copy() for i in range(len(specs))],
+                    [a["newton_iters"] for a in P.arclength_stats()]))
+    return out
+
+
+def test_continuation_matches_oracle(gpu):
+    steps = 14
+    run = _gpu_run(SPECS, steps)
+    for i, sp in enumerate(SPECS):
+        op = oracle_problem("bratu", lam=0.0, **sp)
+        st = oc.ArclengthState(op.ndof)
+        x = np.zeros(op.ndof)
+        ds = 0.5
+        agree = True
+        for k in range(steps):
+            lam_g, xs, its = run[k][0][i], run[k][1][i], run[k][2][i]
+            chk = oracle_problem("bratu", lam=lam_g, **sp)
+            assert max_abs(chk.get_residuals(xs)) < 1e-8, (i, k)
+            if not agree:
+                continue
+            ds = min(abs(ds), 0.3) * (1 if ds > 0 else -1)
+            x, ds = oc.arc_length_step(op, x, st, ds)
+            if st.newton_iters != its:
+                agree = False          # different step lengths from here on
+                continue
+            assert abs(lam_g - op.params[0]) <= 1e-8 * max(1.0, abs(op.params[0])), (i, k)
+            assert np.max(np.abs(xs - x)) <= 1e-8 * max(1.0, np.max(np.abs(x))), (i, k)
+        assert agree or k > 5
+
+
+def test_continuation_batch_equals_alone_bitwise(gpu):
+    both = _gpu_run(SPECS, 6)
+    for i, sp in enumerate(SPECS):
+        alone = _gpu_run([sp], 6)
+        for k in range(6):
+            assert both[k][0][i] == alone[k][0][0]
+            assert np.array_equal(both[k][1][i], alone[k][1][0])
+
+
+def test_continuation_passes_the_fold(gpu):
+    run = _gpu_run([dict(N=8, space="C2")], 45)
+    lams = np.array([r[0][0] for r in run])
+    norms = np.array([np.max(np.abs(r[1][0])) for r in run])
+    k = int(np.argmax(lams))
+    assert 0 < k < len(lams) - 1
+    c = np.polyfit(norms[k - 1:k + 2], lams[k - 1:k + 2], 2)
+    assert abs(c[2] - c[1] ** 2 / (4 * c[0]) - 6.808) < 0.02
+    assert lams[-1] < lams[k] - 0.5
