@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--transient", type=int, default=0, metavar="STEPS",
                     help="instead of the steady sweep: STEPS BDF time steps (dt 0.02, BDF1 first) of "
                          "du/dt = lap u + lambda e^u for every problem of the sweep from u = 0")
+    ap.add_argument("--continuation", type=int, default=0, metavar="STEPS",
+                    help="instead of the steady sweep: STEPS pseudo-arclength steps in lambda (ds 0.5, "
+                         "max 0.3) of every problem of the sweep from lambda = 0, u = 0 (GMRES + --precond)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB read + 1 GiB write streaming kernel (three launches), the "
                          "reference for scaling HBM byte counters")
@@ -307,8 +310,56 @@ def transient_main(args):
     print(json.dumps(out), flush=True)
 
 
+def continuation_main(args):
+    """Pseudo-arclength continuation of the whole sweep: each step is one batched
+    arclength step of all problems (each with its own lambda, step and state);
+    problem-steps/s, and the oracle's residual at the final point of three problems."""
+    import torch
+    from pyoomph_amd import workloads as wl
+    from pyoomph_amd.problem import Problem
+    from oracle.newton import max_abs
+    from tests.oracle_bridge import oracle_problem
+    torch.cuda.set_device(0)
+    specs = wl.bratu_sweep_specs(args.nprob, seed=args.seed)
+    P = Problem([wl.bratu(N, sp, 0.0) for N, sp, _ in specs])
+    P.krylov, P.preconditioner, P.gmres_restart = "gmres", args.precond, 100
+    ds = np.full(len(specs), 0.5)
+    P.arclength_continuation("lambda", ds, max_ds=0.3)     # warmup: plans, tuning
+    P.set_current_dofs(np.zeros(P.ndof))
+    P.reset_arc_length_parameters()
+    for i in range(len(specs)):
+        P.set_global_parameter(i, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = []
+    for _ in range(args.continuation):
+        ds = P.arclength_continuation("lambda", ds, max_ds=0.3)
+        its.append(max(a["newton_iters"] for a in P.arclength_stats()))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lam = P.global_parameters()
+    x = P.get_current_dofs()
+    order = sorted(range(len(specs)), key=lambda i: wl_ndof(specs[i]))
+    res = []
+    for i in (order[0], order[len(order) // 4], order[len(order) // 2]):
+        N, sp, _ = specs[i]
+        op = oracle_problem("bratu", N=N, space=sp, lam=float(lam[i]))
+        res.append(max_abs(op.get_residuals(P.sub_dofs(i, x))))
+    out = {"metric": "arclength continuation problem-steps/s", "value": args.nprob * args.continuation / elapsed,
+           "unit": "problem-steps/s", "n_gpus": 1, "higher_is_better": True, "steps": args.continuation,
+           "ms_per_step": elapsed / args.continuation * 1e3, "max_newton_iters_per_step": its,
+           "lambda_range_after": [float(lam.min()), float(lam.max())],
+           "oracle_max_residual_at_final_points": max(res), "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "bratu_sweep continuation: %d problems, N~U{16..128}, P1/P2, from lambda = 0, "
+                                  "ds 0.5 (max 0.3), GMRES(100) + %s" % (args.nprob, args.precond), "dofs": P.ndof}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
+    if args.continuation:
+        continuation_main(args)
+        return
     if args.transient:
         transient_main(args)
         return

@@ -408,6 +408,11 @@ class Problem:
         _lib.check(rc)
         return nxt
 
+    def set_global_parameter(self, sub, value):
+        """Sets the continued parameter (the Bratu lambda) of one sub-problem."""
+        par = np.array([value], dtype=np.float64)
+        _lib.check(_lib.load().pyhip_problem_set_params(self._handle, int(sub), par.ctypes.data_as(_lib.c_dp), 1))
+
     def global_parameters(self):
         """Current value of the continued parameter per sub-problem (after a step)."""
         if self._arc is None:

@@ -17,10 +17,10 @@ pytestmark = pytest.mark.gpu
 SPECS = [dict(N=8, space="C2"), dict(N=10, space="C1"), dict(N=6, space="C2")]
 
 
-def _gpu_run(specs, steps, ds=0.5, max_ds=0.3):
+def _gpu_run(specs, steps, ds=0.5, max_ds=0.3, precond="ilu0"):
     from pyoomph_amd.problem import Problem
     P = Problem([product_problem("bratu", lam=0.0, **sp) for sp in specs])
-    P.krylov, P.preconditioner, P.gmres_restart = "gmres", "ilu0", 100
+    P.krylov, P.preconditioner, P.gmres_restart = "gmres", precond, 100
     P.gmres_rtol = 1e-12
     dsv = np.full(len(specs), ds)
     out = []
