@@ -48,7 +48,7 @@ def parse():
                     help="f64: FP64 Jacobian and Krylov; mixed: FP32 element kernels for the "
                          "Jacobian and FP32 inner CG inside FP64 iterative refinement (FP64 "
                          "residuals, updates and norms)")
-    ap.add_argument("--precond", choices=("jacobi", "amg"), default="amg",
+    ap.add_argument("--precond", choices=("jacobi", "amg", "ilu0"), default="amg",
                     help="CG preconditioner: Jacobi, or the smoothed-aggregation multigrid V-cycle")
     ap.add_argument("--err-sample", type=int, default=6,
                     help="problems (spread over the size range, largest included) checked against "
