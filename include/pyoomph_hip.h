@@ -146,6 +146,14 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs_dev, int flag, int deriv
 int pyhip_set_time_stepping(pyhip_problem *p, int enabled, const double *weights,
                             const double *hist1_dev, const double *hist2_dev);
 
+/* Hessian-vector product of the scalar forms: values_dev [nnz] (structural pattern)
+ * receives sum_k dJ_ij/du_k y_k at dofs_dev, residual_dev the residual; the reference's
+ * HessianVectorProduct with one vector (src/codegen.cpp:3646-3910, assembled by
+ * src/elements.cpp:4964-4995, flag 1; for these forms the transposed product, flag 4, is
+ * the same matrix). pyhip_export_reference_csr then gives its reference CSR. */
+int pyhip_assemble_hessian_vector(pyhip_problem *p, const double *dofs_dev, const double *y_dev,
+                                  double *residual_dev, double *values_dev);
+
 int pyhip_export_reference_csr(pyhip_problem *p, const double *values_dev,
                                int32_t *row_start_dev, int32_t *col_dev, double *val_dev,
                                int64_t *nnz_out);

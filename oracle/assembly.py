@@ -44,7 +44,7 @@ def lib():
                                                                        lp, lp, dp, ctypes.c_int,
                                                                        ctypes.c_int, ctypes.c_int,
                                                                        ctypes.c_int, dp,
-                                                                       ctypes.c_double, dp, dp]
+                                                                       ctypes.c_double, dp, dp, dp]
         L.oracle_vectors_of_pairs.restype = ctypes.c_long
         L.oracle_vectors_of_pairs.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_int, lp, lp,
                                               dp, dp, dp, lp, ctypes.POINTER(lp),
@@ -137,8 +137,9 @@ class OracleProblem:
         U[m] = x[self.eqn[m]]
         return U
 
-    def element_outputs(self, x, flag, deriv=0):
-        """Per-element R [nel, maxdof] and J [nel, maxdof, maxdof] in local numbering."""
+    def element_outputs(self, x, flag, deriv=0, y=None):
+        """Per-element R [nel, maxdof] and J [nel, maxdof, maxdof] in local numbering;
+        with y (a dof vector) J holds the Hessian-vector product instead."""
         mesh, tab = self.mesh, self.tab
         U = self.nodal_values(x)
         nel, nn = mesh.elements.shape
@@ -151,6 +152,12 @@ class OracleProblem:
         R = np.zeros((nel, self.maxdof))
         J = np.zeros((nel, self.maxdof, self.maxdof)) if flag else None
         H, tw0 = self._history_slots()
+        Y = None
+        if y is not None:
+            Yn = np.zeros_like(self.bc)            # pinned values do not vary: Y = 0 there
+            m = self.eqn >= 0
+            Yn[m] = np.asarray(y, dtype=np.float64)[self.eqn[m]]
+            Y = np.ascontiguousarray(Yn[self.slot_node, self.slot_val])
         psi1 = tab.get("psi1")
         psi1 = None if psi1 is None else np.ascontiguousarray(psi1)
         rc = lib().oracle_elements(
@@ -158,7 +165,7 @@ class OracleProblem:
             _dp(np.ascontiguousarray(tab["w"])), _dp(np.ascontiguousarray(tab["psi"])),
             _dp(np.ascontiguousarray(tab["dpsi"])), _dp(psi1), self.maxdof, _dp(X), _dp(V),
             _dp(aux), _lp(np.ascontiguousarray(self.leqn)), _lp(self.nvar), _dp(self.params),
-            len(self.params), int(flag), int(deriv), self.maxdof, _dp(H), tw0, _dp(R), _dp(J))
+            len(self.params), int(flag), int(deriv), self.maxdof, _dp(H), tw0, _dp(Y), _dp(R), _dp(J))
         if rc != 0:
             raise RuntimeError("oracle_elements failed: %d" % rc)
         return R, J
@@ -176,11 +183,16 @@ class OracleProblem:
         Jacobian: residual, row_start, column_index (first-touch order), values."""
         return self.get_jacobian(x, flag=2)
 
-    def get_jacobian(self, x, deriv=0, flag=1):
+    def get_hessian_vector_product(self, x, y):
+        """sum_k d J / d u_k y_k assembled like the Jacobian (the reference's
+        HessianVectorProduct with one vector): residual, row_start, column_index, values."""
+        return self.get_jacobian(x, y=y)
+
+    def get_jacobian(self, x, deriv=0, flag=1, y=None):
         """oomph Problem::get_jacobian (vectors_of_pairs). Returns residual, row_start,
         column_index (first-touch order), values."""
         # element Jacobians: block e starts at e*maxdof^2, rows have stride nvar[e]
-        R, J = self.element_outputs(x, flag, deriv)
+        R, J = self.element_outputs(x, flag, deriv, y)
         nel = R.shape[0]
         res = np.zeros(self.ndof)
         row_start = np.zeros(self.ndof + 1, dtype=np.int64)

@@ -114,9 +114,15 @@ static double dot(int dim, const double *a, const double *b)
  * src/elements.cpp:4583-4636); the weak form gains (du/dt) * psi_i. flag 2 writes the mass
  * matrix dR_i/d(du_j/dt) = int psi_j psi_i into J instead of the Jacobian (zero without a
  * time derivative). */
+/* Hessian-vector product (Y non-NULL, the element's slot values of the vector):
+ * J receives sum_k d^2 R_i/(du_j du_k) Y_k = int d2src (sum_k Y_k psi_k) psi_j psi_i, the
+ * product of the reference's HessianVectorProduct (src/codegen.cpp:3646-3910, called from
+ * src/elements.cpp:4964-4995 with flag 1); d2src is the second u-derivative of the
+ * source term (Bratu -lambda e^u, reaction-diffusion 6u, Poisson 0). */
 static void scalar_element(int form, const ref_t *r, const double *X, const double *U,
                            const double *aux, const long *leqn, const double *par, int npar,
-                           int flag, int deriv, int nvar, const double *H, double tw0, double *R, double *J)
+                           int flag, int deriv, int nvar, const double *H, double tw0, const double *Y,
+                           double *R, double *J)
 {
     const int dim = r->dim, nn = r->nnode;
     double dx[MAXN][MAXD];
@@ -154,6 +160,12 @@ static void scalar_element(int form, const ref_t *r, const double *X, const doub
             src = u * u * u - f;
             dsrc = 3.0 * (u * u);
         }
+        double d2src = 0.0, yq = 0.0; /* Hessian-vector product */
+        if (Y) {
+            for (int l = 0; l < nn; l++) yq = yq + Y[l] * P[l];
+            if (form == 2) d2src = -par[0] * exp(u);
+            else if (form == 3) d2src = 6.0 * u;
+        }
         double ud = 0.0; /* du/dt at the point */
         if (H) {
             double hq = 0.0;
@@ -175,7 +187,9 @@ static void scalar_element(int form, const ref_t *r, const double *X, const doub
                 long iu = leqn[ls];
                 if (iu < 0) continue;
                 double v;
-                if (flag == 2)
+                if (Y)
+                    v = W * (((d2src * yq) * P[ls]) * P[lt]);
+                else if (flag == 2)
                     v = W * (P[ls] * P[lt]);
                 else if (deriv)
                     v = W * ((dsrc * P[ls]) * P[lt]);
@@ -293,7 +307,7 @@ int oracle_elements(int form, int nel, int dim, int nnode, int nqp, int nnode1,
                     int nslot, const double *X, const double *V, const double *aux,
                     const long *leqn, const long *nvar, const double *par, int npar,
                     int flag, int deriv, int maxdof, const double *Hist, double tw0,
-                    double *R, double *J)
+                    const double *Yv, double *R, double *J)
 {
     ref_t r = {dim, nnode, nqp, nnode1, w, psi, dpsi, psi1};
     if (nnode > MAXN || dim > MAXD || nslot > 22 + 5) return -1;
@@ -304,12 +318,13 @@ int oracle_elements(int form, int nel, int dim, int nnode, int nqp, int nnode1,
         double *Re = R + (size_t)e * maxdof;
         double *Je = J ? J + (size_t)e * maxdof * maxdof : NULL;
         if (form == 4) {
-            if (deriv || Hist || flag == 2) return -2;
+            if (deriv || Hist || flag == 2 || Yv) return -2;
             th_element(&r, Xe, Ve, Le, par, flag, (int)nvar[e], Re, Je);
         } else {
             const double *Ae = aux ? aux + (size_t)e * nnode : NULL;
             const double *He = Hist ? Hist + (size_t)e * nnode : NULL;
-            scalar_element(form, &r, Xe, Ve, Ae, Le, par, npar, flag, deriv, (int)nvar[e], He, tw0, Re, Je);
+            const double *Ye = Yv ? Yv + (size_t)e * nnode : NULL;
+            scalar_element(form, &r, Xe, Ve, Ae, Le, par, npar, flag, deriv, (int)nvar[e], He, tw0, Ye, Re, Je);
         }
     }
     return 0;

@@ -104,6 +104,8 @@ SIGNATURES = {
     "pyhip_arclength_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                             ctypes.POINTER(NewtonOpts), ctypes.POINTER(ctypes.c_int32),
                                             c_dp, ctypes.POINTER(ArclengthState), c_dp]),
+    "pyhip_assemble_hessian_vector": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                     ctypes.c_void_p, ctypes.c_void_p]),
     "pyhip_set_time_stepping": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dp, ctypes.c_void_p,
                                                ctypes.c_void_p]),
     "pyhip_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),

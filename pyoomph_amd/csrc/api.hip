@@ -376,6 +376,7 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.transient = p->transient ? 1 : 0;
             a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
             a.hist1 = p->hist1; a.hist2 = p->hist2;
+            a.hvec = p->hvec;
             TimeScope ts(&p->timer, "element", p->st);
             if ((rc = launch_elements_f64(G.kind, G.form, a, p->st))) return rc;
         } else {
@@ -391,6 +392,7 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.transient = p->transient ? 1 : 0;
             a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
             a.hist1 = p->hist1; a.hist2 = p->hist2;
+            a.hvec = p->hvec;
             TimeScope ts(&p->timer, "element", p->st);
             if ((rc = launch_elements_f32(G.kind, G.form, a, p->st))) return rc;
         }
@@ -419,6 +421,18 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
         }
     }
     return PYHIP_OK;
+}
+
+int pyhip_assemble_hessian_vector(pyhip_problem *p, const double *dofs, const double *y, double *residual,
+                                  double *values) {
+    PYHIP_REQUIRE(p && dofs && y && residual && values, "null argument");
+    for (auto &G : p->groups)
+        PYHIP_REQUIRE(G.form != PYHIP_FORM_NAVIER_STOKES_TH,
+                      "Hessian-vector products cover the scalar forms (Poisson, Bratu, reaction-diffusion)");
+    p->hvec = y;
+    const int rc = pyhip_assemble(p, dofs, 1, -1, PYHIP_F64, residual, values);
+    p->hvec = nullptr;
+    return rc;
 }
 
 int pyhip_set_time_stepping(pyhip_problem *p, int enabled, const double *weights, const double *hist1,

@@ -59,6 +59,8 @@ struct ElemArgs {
     int transient;
     double tw0, tw1, tw2;
     const double *hist1, *hist2;
+    // Hessian-vector product: Ej receives sum_k dJ/du_k y_k (y dof-indexed, 0 at pinned)
+    const double *hvec;
 };
 
 template <int KIND> struct ElemTraits;
@@ -183,18 +185,23 @@ constexpr int scalar_epb() {
     return 256 / (ElemTraits<KIND>::NN > ScalarChunk<KIND>::QC ? ElemTraits<KIND>::NN : ScalarChunk<KIND>::QC);
 }
 
-// TR: the form has a time derivative (a.transient); a separate instantiation keeps the
-// steady kernels' LDS and registers.
-template <typename T, int KIND, int FORM, bool TR>
+// TR: the form has a time derivative (a.transient); HV: the matrix output is the
+// Hessian-vector product sum_k dJ_ij/du_k y_k = int d2src (sum_k y_k psi_k) psi_j psi_i
+// (d2src: Bratu -lambda e^u, reaction-diffusion 6u, Poisson 0). Separate instantiations
+// keep the steady kernels' LDS and registers.
+template <typename T, int KIND, int FORM, bool TR, bool HV>
 __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
     using Tr = ElemTraits<KIND>;
     constexpr int NN = Tr::NN, DIM = Tr::DIM, NQ = Tr::NQ, NS = NN;
     constexpr int EPB = scalar_epb<KIND>();
     constexpr int QC = ScalarChunk<KIND>::QC;
-    constexpr int NQV = 2 + DIM + (TR ? 1 : 0);   // per point: src, dsrc, grad u[, du/dt]
+    // per point: src, dsrc, grad u[, du/dt][, d2src, y]
+    constexpr int NQV = 2 + DIM + (TR ? 1 : 0) + (HV ? 2 : 0);
+    constexpr int QT = 2 + DIM, QH = 2 + DIM + (TR ? 1 : 0);
     __shared__ T sX[EPB][NN][DIM];
     __shared__ T sU[EPB][NN];
     __shared__ T sH[TR ? EPB : 1][NN];   // history combination tw1 u(t-1) + tw2 u(t-2) per node
+    __shared__ T sY[HV ? EPB : 1][NN];   // Hessian-vector product direction per node
     __shared__ T sF[FORM == PYHIP_FORM_REACTION_DIFFUSION ? EPB : 1][NN];
     __shared__ int sEq[EPB][NS];
     __shared__ T sW[EPB][QC];
@@ -218,6 +225,7 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
             const double u1 = eq >= 0 ? a.hist1[eq] : pin, u2 = eq >= 0 ? a.hist2[eq] : pin;
             sH[le][row] = T(a.tw1 * u1 + a.tw2 * u2);
         }
+        if constexpr (HV) sY[le][row] = T(eq >= 0 ? a.hvec[eq] : 0.0);
         sEq[le][row] = eq;
     }
     __syncthreads();
@@ -296,7 +304,17 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
                     T hq = T(0);
 #pragma unroll
                     for (int l = 0; l < NN; l++) hq = hq + sH[le][l] * P[l];
-                    sQ[le][qq][NQV - 1] = T(a.tw0) * u + hq;
+                    sQ[le][qq][QT] = T(a.tw0) * u + hq;
+                }
+                if constexpr (HV) {
+                    T yq = T(0);
+#pragma unroll
+                    for (int l = 0; l < NN; l++) yq = yq + sY[le][l] * P[l];
+                    T d2 = T(0);
+                    if constexpr (FORM == PYHIP_FORM_BRATU) d2 = -par0 * texp(u);
+                    else if constexpr (FORM == PYHIP_FORM_REACTION_DIFFUSION) d2 = T(6) * u;
+                    sQ[le][qq][QH] = d2;
+                    sQ[le][qq][QH + 1] = yq;
                 }
             }
         }
@@ -314,7 +332,7 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
                     Racc = Racc + W * (src * P[row]);
                 else if (tr)
                     Racc = Racc + W * ((dotv<T, DIM>(&sQ[le][qq][2], dxi) + src * P[row]) +
-                                       sQ[le][qq][NQV - 1] * P[row]);
+                                       sQ[le][qq][QT] * P[row]);
                 else
                     Racc = Racc + W * (dotv<T, DIM>(&sQ[le][qq][2], dxi) + src * P[row]);
                 if (a.flag == 2 && !TR) continue;   // no time derivative: mass matrix 0
@@ -325,7 +343,9 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
                         if (sEq[le][j] < 0) continue;
                         const T *dxj = sDx[le][qq][j];
                         T v;
-                        if (a.flag == 2) {
+                        if constexpr (HV) {
+                            v = W * (((sQ[le][qq][QH] * sQ[le][qq][QH + 1]) * P[j]) * P[row]);
+                        } else if (a.flag == 2) {
                             v = W * (P[j] * P[row]);
                         } else if (a.deriv) {
                             v = W * ((dsrc * P[j]) * P[row]);

@@ -9,10 +9,12 @@ static int launch_scalar(const ElemArgs<T> &a, hipStream_t s) {
     constexpr int EPB = scalar_epb<KIND>();
     const int blocks = (a.nel + EPB - 1) / EPB;
     if (blocks == 0) return PYHIP_OK;
-    if (a.transient)
-        hipLaunchKernelGGL((scalar_element_kernel<T, KIND, FORM, true>), dim3(blocks), dim3(256), 0, s, a);
+    if (a.hvec)
+        hipLaunchKernelGGL((scalar_element_kernel<T, KIND, FORM, false, true>), dim3(blocks), dim3(256), 0, s, a);
+    else if (a.transient)
+        hipLaunchKernelGGL((scalar_element_kernel<T, KIND, FORM, true, false>), dim3(blocks), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL((scalar_element_kernel<T, KIND, FORM, false>), dim3(blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((scalar_element_kernel<T, KIND, FORM, false, false>), dim3(blocks), dim3(256), 0, s, a);
     PYHIP_CHECK_HIP(hipGetLastError());
     return PYHIP_OK;
 }

@@ -153,6 +153,7 @@ struct pyhip_problem {
     bool transient = false;
     double tw[3] = {0.0, 0.0, 0.0};
     const double *hist1 = nullptr, *hist2 = nullptr;
+    const double *hvec = nullptr;            // set while a Hessian-vector product assembles
     pyhip::GatherMap gmap() const {
         return pyhip::GatherMap{ndof, nnz, row_start, occ_ptr, occ_roff, cptr, cidx};
     }

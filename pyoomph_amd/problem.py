@@ -372,6 +372,20 @@ class Problem:
                 self.set_time_state(*keep)
         return scipy.sparse.csr_matrix((val, col, rs), shape=(self.ndof, self.ndof))
 
+    def assemble_hessian_vector_product(self, y):
+        """sum_k dJ/du_k y_k at the current dofs (the reference's HessianVectorProduct
+        with one vector) in the reference's CSR form."""
+        import scipy.sparse
+        torch = self.torch
+        with torch.cuda.stream(self._stream):
+            res = torch.empty(max(self.ndof, 1), dtype=torch.float64, device=self.device)
+            vals = torch.empty(max(self.nnz, 1), dtype=torch.float64, device=self.device)
+            yd = torch.tensor(np.asarray(y, dtype=np.float64), device=self.device)
+        _lib.check(_lib.load().pyhip_assemble_hessian_vector(self._handle, _lib.ptr(self.dofs), _lib.ptr(yd),
+                                                              _lib.ptr(res), _lib.ptr(vals)))
+        rs, col, val = self._export(vals)
+        return scipy.sparse.csr_matrix((val, col, rs), shape=(self.ndof, self.ndof))
+
     # -- arclength continuation -----------------------------------------------------------
     _arc = None
 

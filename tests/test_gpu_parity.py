@@ -272,3 +272,21 @@ def test_solver_plugin_matches_superlu(gpu):
                                     ctypes.byref(h)))
     _lib.check(L.pyhip_solve_serial(3, n, nnz, 1, None, None, None, None, n, 1, ctypes.byref(h)))
     assert rel(b, ref) <= 1e-8
+
+
+@pytest.mark.parametrize("kind,kw", [("bratu", dict(N=7, space="C1", lam=1.5)),
+                                     ("bratu", dict(N=5, space="C2", lam=2.5)),
+                                     ("rd", dict(N=3)), ("poisson", dict(N=4))])
+def test_hessian_vector_product_parity(gpu, kind, kw):
+    """sum_k dJ/du_k y_k: reference CSR (first-touch, exact zeros dropped) and values
+    against the oracle; Poisson's product is identically zero, so no entry survives."""
+    P, op = build(kind, kw)
+    rng = np.random.default_rng(11)
+    x, y = rng.uniform(-0.4, 0.4, P.ndof), rng.uniform(-1, 1, P.ndof)
+    P.set_current_dofs(x)
+    H = P.assemble_hessian_vector_product(y)
+    r, rs, col, val = op.get_hessian_vector_product(x, y)
+    assert np.array_equal(H.indptr, rs) and np.array_equal(H.indices, col)
+    assert rel(H.data, val) <= 1e-12 if val.size else H.nnz == 0
+    if kind != "bratu":
+        assert np.array_equal(H.data, val)

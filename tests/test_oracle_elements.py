@@ -222,3 +222,38 @@ def test_vectors_of_pairs_rules():
     assert rows[2] == [(2, 1.0 - 1.0), (1, 2.0 + 11.0)]
     assert rows[3] == [(3, 9.0), (2, 10.0)]
     assert res.tolist() == [2.0, 3.0 + 4.0, 1.0 + 6.0, 5.0]
+
+
+@pytest.mark.parametrize("kind,kw", [("bratu", dict(N=3, space="C1", lam=2.0)),
+                                     ("bratu", dict(N=2, space="C2", lam=1.3)),
+                                     ("poisson", dict(N=2))])
+def test_hessian_vector_product_matches_finite_differences(kind, kw):
+    """sum_k dJ/du_k y_k against (J(x + h y) - J(x - h y)) / 2h."""
+    op = oracle_problem(kind, **kw)
+    rng = np.random.default_rng(8)
+    x, y = rng.uniform(-0.5, 0.5, op.ndof), rng.uniform(-1, 1, op.ndof)
+
+    def dense_j(v, **kw2):
+        _, rs, col, val = op.get_jacobian(v, **kw2)
+        M = np.zeros((op.ndof, op.ndof))
+        for i in range(op.ndof):
+            M[i, col[rs[i]:rs[i + 1]]] = val[rs[i]:rs[i + 1]]
+        return M
+
+    H = dense_j(x, y=y)
+    h = 1e-5
+    Hfd = (dense_j(x + h * y) - dense_j(x - h * y)) / (2 * h)
+    assert np.max(np.abs(H - Hfd)) < 1e-7 * max(1.0, np.max(np.abs(H)))
+    if kind == "poisson":
+        assert not np.any(H)
+
+
+def test_rd_hessian_vector_product_matches_finite_differences():
+    m = om.cuboid_brick_mesh(1)
+    op = free_problem(m, oa.FORM_RD, [], aux=np.linspace(-1, 1, m.nnode))
+    rng = np.random.default_rng(9)
+    x, y = rng.uniform(-1, 1, op.ndof), rng.uniform(-1, 1, op.ndof)
+    Jp = op.element_outputs(x + 1e-5 * y, 1)[1][0]
+    Jm = op.element_outputs(x - 1e-5 * y, 1)[1][0]
+    H = op.element_outputs(x, 1, y=y)[1][0]
+    assert np.max(np.abs(H - (Jp - Jm) / 2e-5)) < 1e-7 * max(1.0, np.max(np.abs(H)))
