@@ -61,7 +61,7 @@ def parse():
                     help="preconditioner of the CG class of served requests")
     ap.add_argument("--full", choices=("cavity256", "rd64"), default=None,
                     help="instead of the sweep: one full-size single-problem solve, the 256x256 "
-                         "Taylor-Hood cavity (GMRES(300) + multicolour ILU(0), FP64) or the 64^3 Q2 "
+                         "Taylor-Hood cavity (GMRES(150) + multicolour ILU(0), FP64) or the 64^3 Q2 "
                          "reaction-diffusion cube (FP32 ILU(0)-GMRES inside FP64 refinement, compared "
                          "with the FP64 CG solve of the same problem)")
     ap.add_argument("--transient", type=int, default=0, metavar="STEPS",
@@ -231,12 +231,12 @@ def full_main(args):
         t = time.perf_counter()
         P = Problem(wl.lid_driven_cavity(256, Re=100.0))
         setup = time.perf_counter() - t
-        P.krylov, P.preconditioner, P.ilu_order, P.gmres_restart = "gmres", "ilu0", "auto", 300
+        P.krylov, P.preconditioner, P.ilu_order, P.gmres_restart = "gmres", "ilu0", "auto", 150
         P.gmres_max_iters, P.newton_solver_tolerance = 8000, 1e-10
         el, st, k = run(P)
         out = {"workload": "2D Taylor-Hood lid-driven cavity, 256x256 Q2/Q1, Re=100, Newton from 0 to "
                            "max|R| < 1e-10", "ndof": P.ndof, "nnz": P.nnz, "setup_s": setup,
-               "solve_s": el, "solver": "GMRES(300) + multicolour ILU(0), FP64", "stats": st, "kernel_ms": k}
+               "solve_s": el, "solver": "GMRES(150) + multicolour ILU(0), FP64", "stats": st, "kernel_ms": k}
     else:
         t = time.perf_counter()
         P = Problem(wl.reaction_diffusion_cube(64))
