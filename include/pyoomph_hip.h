@@ -134,13 +134,13 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs_dev, int flag, int deriv
  * row_start_dev [ndof+1], col_dev / val_dev [nnz] (structural nnz is an upper bound).
  * *nnz_out receives the number of entries written. */
 /* ---- time stepping -------------------------------------------------------------------------
- * Time derivative of the scalar forms for the assemblies and Newton solves that follow:
+ * Time derivative for the assemblies and Newton solves that follow:
  * du/dt = weights[0] u + weights[1] u(t-1) + weights[2] u(t-2), the weak form gaining
- * (du/dt) psi_i. The weights are the BDF weights of the reference's MultiTimeStepper
+ * (du/dt) psi_i (scalar forms) or rho (du/dt) . psi_i (Taylor-Hood momentum). The weights are the BDF weights of the reference's MultiTimeStepper
  * (src/timestepper.cpp:32-59; BDF2, degraded to BDF1 before the first completed step,
  * src/elements.cpp:4583-4636), 3 host values; hist1_dev / hist2_dev [ndof] hold the previous and
  * second-previous dofs (device, owned by the caller, read at every assembly). enabled = 0
- * returns to the steady forms. Taylor-Hood sub-problems are rejected. With time stepping
+ * returns to the steady forms. With time stepping
  * on, pyhip_assemble with flag 2 returns the residual and the mass matrix dR/d(du/dt)
  * (the reference's flag-2 output, src/jitbridge.h:285) in values. */
 int pyhip_set_time_stepping(pyhip_problem *p, int enabled, const double *weights,

@@ -216,18 +216,23 @@ static const int TH_slot_node[22] = {0, 0, 0, 1, 1, 2, 2, 2, 3, 3, 4, 4, 5, 5, 6
 static const int TH_slot_kind[22] = {0, 1, 2, 0, 1, 0, 1, 2, 0, 1, 0, 1, 0, 1, 0, 1, 2, 0, 1, 0, 1, 2};
 static const int TH_slot_pidx[22] = {-1, -1, 0, -1, -1, -1, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, 2, -1, -1, -1, -1, 3};
 
+/* Time derivative (H non-NULL, H[slot] = w1 v(t-1) + w2 v(t-2)): du_a/dt = w0 u_a + sum_l
+ * H_a[l] psi_l; the momentum rows gain (rho du_a/dt) psi_i and the Jacobian w0 rho psi_j psi_i
+ * on the diagonal velocity blocks; flag 2 writes the mass matrix rho psi_j psi_i there. */
 static void th_element(const ref_t *r, const double *X, const double *V, const long *leqn,
-                       const double *par, int flag, int nvar, double *R, double *J)
+                       const double *par, int flag, int nvar, const double *H, double tw0, double *R, double *J)
 {
     const double mu = par[0], rho = par[1];
     const int nn = 9;
     double dx[MAXN][MAXD];
-    double Ux[9], Uy[9], Pp[4];
+    double Ux[9], Uy[9], Pp[4], Hx[9] = {0}, Hy[9] = {0};
     for (int s = 0; s < 22; s++) {
         int k = TH_slot_kind[s], n = TH_slot_node[s];
         if (k == 0) Ux[n] = V[s];
         else if (k == 1) Uy[n] = V[s];
         else Pp[TH_slot_pidx[s]] = V[s];
+        if (H && k == 0) Hx[n] = H[s];
+        if (H && k == 1) Hy[n] = H[s];
     }
     for (int q = 0; q < r->nqp; q++) {
         const double W = geometry(r, q, X, dx);
@@ -254,23 +259,39 @@ static void th_element(const ref_t *r, const double *X, const double *V, const l
         double conv[2];
         for (int a = 0; a < 2; a++) conv[a] = rho * (u[0] * du[a][0] + u[1] * du[a][1]);
         const double divu = du[0][0] + du[1][1];
+        double ud[2] = {0.0, 0.0};
+        if (H) {
+            double h0 = 0.0, h1 = 0.0;
+            for (int l = 0; l < nn; l++) h0 = h0 + Hx[l] * P[l];
+            for (int l = 0; l < nn; l++) h1 = h1 + Hy[l] * P[l];
+            ud[0] = tw0 * u[0] + h0;
+            ud[1] = tw0 * u[1] + h1;
+        }
         for (int st = 0; st < 22; st++) {
             long ie = leqn[st];
             if (ie < 0) continue;
             int kt = TH_slot_kind[st], lt = TH_slot_node[st];
             if (kt < 2) {
                 int a = kt;
-                R[ie] = R[ie] + W * ((sig[a][0] * dx[lt][0] + sig[a][1] * dx[lt][1]) + conv[a] * P[lt]);
+                if (H)
+                    R[ie] = R[ie] + W * (((sig[a][0] * dx[lt][0] + sig[a][1] * dx[lt][1]) + conv[a] * P[lt]) +
+                                         (rho * ud[a]) * P[lt]);
+                else
+                    R[ie] = R[ie] + W * ((sig[a][0] * dx[lt][0] + sig[a][1] * dx[lt][1]) + conv[a] * P[lt]);
             } else {
                 R[ie] = R[ie] + W * (divu * C[TH_slot_pidx[st]]);
             }
             if (!flag) continue;
+            if (flag == 2 && !H) continue;
             for (int ss = 0; ss < 22; ss++) {
                 long iu = leqn[ss];
                 if (iu < 0) continue;
                 int ks = TH_slot_kind[ss], ls = TH_slot_node[ss];
                 double v;
-                if (kt < 2 && ks < 2) {
+                if (flag == 2) {
+                    if (kt < 2 && kt == ks) v = W * ((rho * P[ls]) * P[lt]);
+                    else continue;
+                } else if (kt < 2 && ks < 2) {
                     int a = kt, c = ks;
                     double visc = dx[ls][a] * dx[lt][c];
                     double cv = P[ls] * du[a][c];
@@ -278,7 +299,10 @@ static void th_element(const ref_t *r, const double *X, const double *V, const l
                         visc = (dx[ls][0] * dx[lt][0] + dx[ls][1] * dx[lt][1]) + visc;
                         cv = cv + (u[0] * dx[ls][0] + u[1] * dx[ls][1]);
                     }
-                    v = W * (mu * visc + (rho * cv) * P[lt]);
+                    if (H && a == c)
+                        v = W * ((mu * visc + (rho * cv) * P[lt]) + ((rho * tw0) * P[ls]) * P[lt]);
+                    else
+                        v = W * (mu * visc + (rho * cv) * P[lt]);
                 } else if (kt < 2 && ks == 2) {
                     v = W * (-(C[TH_slot_pidx[ss]] * dx[lt][kt]));
                 } else if (kt == 2 && ks < 2) {
@@ -318,12 +342,13 @@ int oracle_elements(int form, int nel, int dim, int nnode, int nqp, int nnode1,
         double *Re = R + (size_t)e * maxdof;
         double *Je = J ? J + (size_t)e * maxdof * maxdof : NULL;
         if (form == 4) {
-            if (deriv || Hist || flag == 2 || Yv) return -2;
-            th_element(&r, Xe, Ve, Le, par, flag, (int)nvar[e], Re, Je);
+            if (deriv || Yv) return -2;
+            const double *He = Hist ? Hist + (size_t)e * nslot : NULL;
+            th_element(&r, Xe, Ve, Le, par, flag, (int)nvar[e], He, tw0, Re, Je);
         } else {
             const double *Ae = aux ? aux + (size_t)e * nnode : NULL;
-            const double *He = Hist ? Hist + (size_t)e * nnode : NULL;
-            const double *Ye = Yv ? Yv + (size_t)e * nnode : NULL;
+            const double *He = Hist ? Hist + (size_t)e * nslot : NULL;
+            const double *Ye = Yv ? Yv + (size_t)e * nslot : NULL;
             scalar_element(form, &r, Xe, Ve, Ae, Le, par, npar, flag, deriv, (int)nvar[e], He, tw0, Ye, Re, Je);
         }
     }

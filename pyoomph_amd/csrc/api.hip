@@ -445,9 +445,6 @@ int pyhip_set_time_stepping(pyhip_problem *p, int enabled, const double *weights
         return PYHIP_OK;
     }
     PYHIP_REQUIRE(weights && hist1 && hist2, "time stepping needs weights and two history vectors");
-    for (auto &G : p->groups)
-        PYHIP_REQUIRE(G.form != PYHIP_FORM_NAVIER_STOKES_TH,
-                      "time stepping covers the scalar forms (Poisson, Bratu, reaction-diffusion)");
     p->transient = true;
     for (int t = 0; t < 3; t++) p->tw[t] = weights[t];
     p->hist1 = hist1;

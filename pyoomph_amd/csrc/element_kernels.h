@@ -377,7 +377,10 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
 // Taylor-Hood Q2/Q1 steady Navier-Stokes (22 slots, 9 geometric nodes). Phase A: thread
 // t < 9 of an element computes integration point t (geometry, dx_shape, u, grad u, p);
 // phase B: thread = slot row.
-template <typename T>
+// TR: time derivative of the velocities (a.transient): du/dt = tw0 u + sum_l H psi_l with
+// H = tw1 u(t-1) + tw2 u(t-2) per slot; momentum rows gain (rho du/dt) psi_i, the diagonal
+// velocity blocks tw0 rho psi_j psi_i; flag 2 writes the mass matrix rho psi_j psi_i there.
+template <typename T, bool TR>
 __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
     constexpr int NN = 9, DIM = 2, NQ = 9, NS = 22;
     constexpr int EPB = 256 / NS;  // 11 elements, 242 threads
@@ -386,7 +389,8 @@ __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
     __shared__ int sEq[EPB][NS];
     __shared__ T sW[EPB][NQ];
     __shared__ T sDx[EPB][NQ][NN][DIM];
-    __shared__ T sQ[EPB][NQ][7];   // u0, u1, du00, du01, du10, du11, p
+    __shared__ T sQ[EPB][NQ][TR ? 9 : 7];   // u0, u1, du00, du01, du10, du11, p[, du0/dt, du1/dt]
+    __shared__ T sH[TR ? EPB : 1][NS];
 
     const int le = threadIdx.x / NS;
     const int slot = threadIdx.x % NS;
@@ -403,6 +407,11 @@ __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
             sX[le][slot][1] = T(a.coords[(int64_t)gn * 3 + 1]);
         }
         sV[le][slot] = T(eq >= 0 ? a.dofs[eq] : a.pinned[(int64_t)node * a.nvalue + kind]);
+        if constexpr (TR) {
+            const double pin = a.pinned[(int64_t)node * a.nvalue + kind];
+            const double u1 = eq >= 0 ? a.hist1[eq] : pin, u2 = eq >= 0 ? a.hist2[eq] : pin;
+            sH[le][slot] = T(a.tw1 * u1 + a.tw2 * u2);
+        }
         sEq[le][slot] = eq;
     }
     __syncthreads();
@@ -441,6 +450,15 @@ __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
         sQ[le][q][0] = u0;
         sQ[le][q][1] = u1;
         sQ[le][q][6] = p;
+        if constexpr (TR) {
+            T h0 = T(0), h1 = T(0);
+#pragma unroll
+            for (int l = 0; l < NN; l++) h0 = h0 + sH[le][kTHNodeSlot[l]] * P[l];
+#pragma unroll
+            for (int l = 0; l < NN; l++) h1 = h1 + sH[le][kTHNodeSlot[l] + 1] * P[l];
+            sQ[le][q][7] = T(a.tw0) * u0 + h0;
+            sQ[le][q][8] = T(a.tw0) * u1 + h1;
+        }
     }
     __syncthreads();
     const bool myrow = active && sEq[le][slot] >= 0;
@@ -472,18 +490,26 @@ __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
         const T *dxl = sDx[le][q][lt];
         if (kt < 2) {
             const T conv = rho * (u0 * du[kt][0] + u1 * du[kt][1]);
-            Racc = Racc + W * ((sig[kt][0] * dxl[0] + sig[kt][1] * dxl[1]) + conv * P[lt]);
+            if constexpr (TR)
+                Racc = Racc + W * (((sig[kt][0] * dxl[0] + sig[kt][1] * dxl[1]) + conv * P[lt]) +
+                                   (rho * sQ[le][q][7 + kt]) * P[lt]);
+            else
+                Racc = Racc + W * ((sig[kt][0] * dxl[0] + sig[kt][1] * dxl[1]) + conv * P[lt]);
         } else {
             const int pidx = kTHSlotP[slot];
             Racc = Racc + W * ((du[0][0] + du[1][1]) * C[pidx]);
         }
+        if (a.flag == 2 && !TR) continue;   // no time derivative: mass matrix 0
         if (a.flag) {
 #pragma unroll
             for (int ss = 0; ss < NS; ss++) {
                 if (sEq[le][ss] < 0) continue;
                 const int ks = kTHSlotKind[ss], ls = kTHSlotNode[ss];
                 T v;
-                if (kt < 2 && ks < 2) {
+                if (a.flag == 2) {
+                    if (kt < 2 && kt == ks) v = W * ((rho * P[ls]) * P[lt]);
+                    else continue;
+                } else if (kt < 2 && ks < 2) {
                     const T *dxs = sDx[le][q][ls];
                     T visc = dxs[kt] * dxl[ks];
                     T cv = P[ls] * du[kt][ks];
@@ -491,7 +517,10 @@ __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
                         visc = (dxs[0] * dxl[0] + dxs[1] * dxl[1]) + visc;
                         cv = cv + (u0 * dxs[0] + u1 * dxs[1]);
                     }
-                    v = W * (mu * visc + (rho * cv) * P[lt]);
+                    if (TR && kt == ks)
+                        v = W * ((mu * visc + (rho * cv) * P[lt]) + ((rho * T(a.tw0)) * P[ls]) * P[lt]);
+                    else
+                        v = W * (mu * visc + (rho * cv) * P[lt]);
                 } else if (kt < 2 && ks == 2) {
                     const int pidx = kTHSlotP[ss];
                     v = W * (-(C[pidx] * dxl[kt]));

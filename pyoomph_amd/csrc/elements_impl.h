@@ -41,7 +41,14 @@ static int launch_elements_impl(int kind, int form, const ElemArgs<T> &a, hipStr
         constexpr int EPB = 256 / 22;
         const int blocks = (a.nel + EPB - 1) / EPB;
         if (blocks == 0) return PYHIP_OK;
-        hipLaunchKernelGGL((th_element_kernel<T>), dim3(blocks), dim3(256), 0, s, a);
+        if (a.hvec) {
+            set_error("Hessian-vector products cover the scalar forms");
+            return PYHIP_ERR_INVALID;
+        }
+        if (a.transient)
+            hipLaunchKernelGGL((th_element_kernel<T, true>), dim3(blocks), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((th_element_kernel<T, false>), dim3(blocks), dim3(256), 0, s, a);
         PYHIP_CHECK_HIP(hipGetLastError());
         return PYHIP_OK;
     }

@@ -288,8 +288,8 @@ class Problem:
         return o
 
     # -- time stepping ------------------------------------------------------------------
-    # du/dt of the scalar forms with the reference's BDF weights; the history vectors live
-    # on the device next to the dofs.
+    # du/dt (scalar forms; Taylor-Hood velocities) with the reference's BDF weights; the
+    # history vectors live on the device next to the dofs.
     time = 0.0
     time_steps_done = 0
     _hist = None

@@ -1,5 +1,6 @@
-"""Time stepping on the GPU against the oracle: the time derivative of the scalar forms
-with the reference's BDF weights, the mass matrix (flag 2), unsteady Newton steps.
+"""Time stepping on the GPU against the oracle: the time derivative of the scalar forms and
+of the Taylor-Hood momentum with the reference's BDF weights, the mass matrix (flag 2),
+unsteady Newton steps.
 
 Bar: as the steady parity (CSR pattern bit-exact, FP64 values bitwise except the Bratu
 exponential, ≤1e-12), solutions of every time step within 1e-8 of the oracle's SuperLU
@@ -14,7 +15,7 @@ from tests.test_gpu_parity import build, rel
 pytestmark = pytest.mark.gpu
 
 CASES = [("poisson", dict(N=5)), ("bratu", dict(N=7, space="C1", lam=1.5)),
-         ("bratu", dict(N=5, space="C2", lam=2.5)), ("rd", dict(N=3))]
+         ("bratu", dict(N=5, space="C2", lam=2.5)), ("rd", dict(N=3)), ("ns", dict(N=4, Re=50.0))]
 
 
 @pytest.mark.parametrize("kind,kw", CASES)
@@ -58,18 +59,21 @@ def test_mass_matrix_parity(gpu, kind, kw):
     ("bratu", dict(N=10, space="C2", lam=2.0), dict(krylov="cg", preconditioner="jacobi")),
     ("bratu", dict(N=12, space="C1", lam=3.0), dict(krylov="cg", preconditioner="amg")),
     ("poisson", dict(N=8), dict(krylov="gmres", preconditioner="ilu0")),
+    ("ns", dict(N=6, Re=100.0), dict(krylov="gmres", preconditioner="ilu0")),
 ])
 def test_unsteady_steps_match_oracle(gpu, kind, kw, opts):
     P, op = build(kind, kw)
     for k, v in opts.items():
         setattr(P, k, v)
+    if kind == "ns":
+        P.newton_solver_tolerance = 1e-10
     P.set_initial_condition(np.zeros(P.ndof))
     st = ots.TimeState(np.zeros(op.ndof))
     xo = np.zeros(op.ndof)
     for dt in (0.05, 0.05, 0.08, 0.03):
         s = P.unsteady_newton_solve(dt)
         assert s[0]["converged"]
-        xo = ots.unsteady_newton_step(op, xo, st, dt)
+        xo = ots.unsteady_newton_step(op, xo, st, dt, tol=P.newton_solver_tolerance)
         assert rel(P.get_current_dofs(), xo) <= 1e-8
     assert P.time_steps_done == 4 and abs(P.time - st.time) < 1e-15
 

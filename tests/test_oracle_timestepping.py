@@ -6,7 +6,8 @@
   Q2 quads and Q2 bricks (1D Lagrange mass tensor products), the P2 mass matrix via
   u^T M u = int u^2 for linear u;
 * the time term of the residual equals M (du/dt at the nodes), and the transient
-  Jacobian matches finite differences of the residual with the history held;
+  Jacobian matches finite differences of the residual with the history held (scalar
+  forms and Taylor-Hood, whose mass matrix is rho psi_j psi_i on the velocity blocks);
 * time convergence of the whole stepping loop on the heat equation against the exact
   semi-discrete solution expm(-t M^-1 K) u0: first order for BDF1, second order for BDF2
   with its BDF1 first step.
@@ -95,7 +96,7 @@ def test_q2_and_brick_mass_tensor_products():
 
 
 @pytest.mark.parametrize("kind,kw", [("bratu", dict(N=3, space="C2", lam=2.0)),
-                                     ("poisson", dict(N=3))])
+                                     ("poisson", dict(N=3)), ("ns", dict(N=3, Re=50.0))])
 def test_time_term_is_mass_times_nodal_rate(kind, kw):
     op = oracle_problem(kind, **kw)
     rng = np.random.default_rng(4)
@@ -111,8 +112,9 @@ def test_time_term_is_mass_times_nodal_rate(kind, kw):
     assert np.allclose(r_tr - r_steady, M @ rate, rtol=0, atol=1e-13)
 
 
-def test_transient_jacobian_matches_finite_differences():
-    op = oracle_problem("bratu", N=2, space="C2", lam=1.5)
+@pytest.mark.parametrize("kind,kw", [("bratu", dict(N=2, space="C2", lam=1.5)), ("ns", dict(N=2))])
+def test_transient_jacobian_matches_finite_differences(kind, kw):
+    op = oracle_problem(kind, **kw)
     rng = np.random.default_rng(6)
     x, h1, h2 = (rng.uniform(-0.5, 0.5, op.ndof) for _ in range(3))
     op.set_time(ots.bdf2_weights(0.05, 0.05), h1, h2)
