@@ -59,11 +59,12 @@ def parse():
     ap.add_argument("--serving-requests", type=int, default=48)
     ap.add_argument("--serving-precond", choices=("jacobi", "amg"), default="jacobi",
                     help="preconditioner of the CG class of served requests")
-    ap.add_argument("--full", choices=("cavity256", "rd64"), default=None,
+    ap.add_argument("--full", choices=("cavity256", "rd64", "cavity_startup"), default=None,
                     help="instead of the sweep: one full-size single-problem solve, the 256x256 "
                          "Taylor-Hood cavity (GMRES(150) + multicolour ILU(0), FP64) or the 64^3 Q2 "
                          "reaction-diffusion cube (FP32 ILU(0)-GMRES inside FP64 refinement, compared "
-                         "with the FP64 CG solve of the same problem)")
+                         "with the FP64 CG solve of the same problem), or the start-up of the 128x128 "
+                         "cavity from rest (10 BDF steps of dt 0.01)")
     ap.add_argument("--transient", type=int, default=0, metavar="STEPS",
                     help="instead of the steady sweep: STEPS BDF time steps (dt 0.02, BDF1 first) of "
                          "du/dt = lap u + lambda e^u for every problem of the sweep from u = 0")
@@ -227,6 +228,29 @@ def full_main(args):
                                     "final_maxres")}
         return el, st, k
 
+    if args.full == "cavity_startup":
+        t = time.perf_counter()
+        P = Problem(wl.lid_driven_cavity(128, Re=100.0))
+        setup = time.perf_counter() - t
+        P.krylov, P.preconditioner, P.ilu_order, P.gmres_restart = "gmres", "ilu0", "auto", 150
+        P.gmres_max_iters, P.newton_solver_tolerance = 8000, 1e-10
+        P.set_initial_condition(np.zeros(P.ndof))
+        _lib.check(_lib.load().pyhip_timing_enable(P._handle, 1))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        steps = []
+        for _ in range(10):
+            st = P.unsteady_newton_solve(0.01, raise_on_failure=False)[0]
+            steps.append({k2: st[k2] for k2 in ("newton_iters", "linear_iters", "converged", "final_maxres")})
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        k = {f: dict(zip(("launches", "ms"), timing(P, f))) for f in fams}
+        out = {"workload": "2D Taylor-Hood lid-driven cavity start-up from rest, 128x128 Q2/Q1, Re=100, "
+                           "10 steps dt 0.01 (BDF1 first, then BDF2), Newton to max|R| < 1e-10 each",
+               "ndof": P.ndof, "nnz": P.nnz, "setup_s": setup, "solve_s": el, "steps_per_s": 10 / el,
+               "solver": "GMRES(150) + multicolour ILU(0), FP64", "steps": steps, "kernel_ms": k}
+        print(json.dumps(out), flush=True)
+        return
     if args.full == "cavity256":
         t = time.perf_counter()
         P = Problem(wl.lid_driven_cavity(256, Re=100.0))
