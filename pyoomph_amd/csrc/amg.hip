@@ -422,10 +422,56 @@ static int64_t active_sum(const std::vector<int64_t> &v, const int32_t *act_h) {
 }
 
 
+// Lane counts of each level's three spmv launches: the fastest of 1, 2, 4 and 8 over two
+// timed launches each, once per hierarchy (the canonical row order gives the same bits
+// for every lane count, so the choice changes no result). The outputs are scratch
+// vectors of the level.
+static int tune_levels(AmgDev &A, hipStream_t st) {
+    hipEvent_t e0, e1;
+    PYHIP_CHECK_HIP(hipEventCreate(&e0));
+    PYHIP_CHECK_HIP(hipEventCreate(&e1));
+    auto pick = [&](auto &&launch) {
+        float best = 1e30f;
+        int arg = 8;
+        for (int lpr : {1, 2, 4, 8}) {
+            launch(lpr);
+            (void)hipEventRecord(e0, st);
+            for (int k = 0; k < 2; k++) launch(lpr);
+            (void)hipEventRecord(e1, st);
+            float ms = 0.f;
+            if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+                ms < best) {
+                best = ms;
+                arg = lpr;
+            }
+        }
+        return arg;
+    };
+    for (size_t l = 0; l < A.lv.size(); l++) {
+        AmgDevLevel &L = A.lv[l];
+        AmgDevLevel &C = (l + 1 < A.lv.size()) ? A.lv[l + 1] : A.bot;
+        L.lpr_a = pick([&](int lpr) {
+            launch_spmv<double, double, double>(lpr, st, L.n, L.rs, L.col, L.vals, L.b, L.r, nullptr);
+        });
+        if (L.nc) {
+            L.lpr_t = pick([&](int lpr) {
+                launch_spmv<double, double, double>(lpr, st, L.nc, L.trs, L.trow, L.ptv, L.r, C.b, nullptr);
+            });
+            L.lpr_p = pick([&](int lpr) {
+                launch_spmv_add<double, double, double>(lpr, st, L.n, L.prs, L.pcol, L.pv, C.x, L.x2, L.x2);
+            });
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ACHECK();
+    return PYHIP_OK;
+}
+
 static int level_weights(hipStream_t st, AmgDevLevel &L, const double *vals, double *nrm, int nsub,
                          KernelTimer *tm) {
     const int64_t n = L.n;
-    const int lpr = lanes_for(L.nnz, n);
+    const int lpr = L.lpr_a ? L.lpr_a : lanes_for(L.nnz, n);
     hipLaunchKernelGGL(amg_diag_kernel, dim3(ablk(n)), dim3(256), 0, st, n, L.diag, vals, L.d);
     hipLaunchKernelGGL(amg_power_start_kernel, dim3(ablk(n)), dim3(256), 0, st, n, L.row_sub, L.sub_off, L.pw);
     for (int it = 0; it < kPowerIters; it++) {
@@ -472,6 +518,7 @@ int amg_setup(AmgDev &A, hipStream_t st, const double *vals0, KernelTimer *tm) {
         hipLaunchKernelGGL(amg_chol_factor_kernel, dim3(A.bot.nends), dim3(kCholThreads), 0, st, A.bot.ends,
                            A.bot.sub_off, A.bot.rs, A.bot.col, A.bot.vals, A.chol, A.chol_fail);
     ACHECK();
+    if (!A.lv.empty() && !A.lv[0].lpr_a && (rc = tune_levels(A, st))) return rc;
     return PYHIP_OK;
 }
 
@@ -513,11 +560,12 @@ int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer
         const double *bl = l == 0 ? b : L.b;
         AmgDevLevel &C = (l + 1 < nl) ? A.lv[l + 1] : A.bot;
         hipLaunchKernelGGL(amg_wb_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.w, bl, L.x, L.row_sub, act);
-        level_spmv(tm, kSpmvSub, lanes_for(L.nnz, L.n), st, L.n, nnz[l], rows[l], rows[l], L.rs, L.col, L.vals, L.x,
-                   L.r, bl, L.row_sub, act);
+        const int la = L.lpr_a ? L.lpr_a : lanes_for(L.nnz, L.n);
+        level_spmv(tm, kSpmvSub, la, st, L.n, nnz[l], rows[l], rows[l], L.rs, L.col, L.vals, L.x, L.r, bl,
+                   L.row_sub, act);
         if (L.nc)
-            level_spmv(tm, kSpmvPlain, 8, st, L.nc, nnzp[l], rows[l + 1], rows[l], L.trs, L.trow, L.ptv, L.r, C.b,
-                       nullptr, C.row_sub, act);
+            level_spmv(tm, kSpmvPlain, L.lpr_t ? L.lpr_t : 8, st, L.nc, nnzp[l], rows[l + 1], rows[l], L.trs, L.trow,
+                       L.ptv, L.r, C.b, nullptr, C.row_sub, act);
     }
     if (Bt.nends)
         hipLaunchKernelGGL(amg_chol_solve_kernel, dim3(Bt.nends), dim3(kCholThreads), 0, st, Bt.ends, Bt.sub_off,
@@ -529,10 +577,10 @@ int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer
         const double *bl = l == 0 ? b : L.b;
         AmgDevLevel &C = (l + 1 < nl) ? A.lv[l + 1] : A.bot;
         if (L.nc)
-            level_spmv(tm, kSpmvAdd, 4, st, L.n, nnzp[l], rows[l], rows[l + 1], L.prs, L.pcol, L.pv, C.x, L.x, L.x,
-                       L.row_sub, act);
-        level_spmv(tm, kSpmvSub, lanes_for(L.nnz, L.n), st, L.n, nnz[l], rows[l], rows[l], L.rs, L.col, L.vals, L.x,
-                   L.r, bl, L.row_sub, act);
+            level_spmv(tm, kSpmvAdd, L.lpr_p ? L.lpr_p : 4, st, L.n, nnzp[l], rows[l], rows[l + 1], L.prs, L.pcol,
+                       L.pv, C.x, L.x, L.x, L.row_sub, act);
+        level_spmv(tm, kSpmvSub, L.lpr_a ? L.lpr_a : lanes_for(L.nnz, L.n), st, L.n, nnz[l], rows[l], rows[l], L.rs,
+                   L.col, L.vals, L.x, L.r, bl, L.row_sub, act);
         double *out = l == 0 ? z : L.x2;
         hipLaunchKernelGGL(amg_update_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.w, L.r, L.x, out, L.row_sub,
                            act);
