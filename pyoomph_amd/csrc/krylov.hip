@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "ilu_plan.h"
 #include "krylov.h"
 #include "spmv.h"
 
@@ -659,118 +660,18 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
                 }
         }
     }
-    std::vector<int32_t> perm(n), iperm(n);
-    // multicolour order for the sub-problems with mc[s] (colours are computed for all
-    // rows; the neighbours of a row are in its own sub-problem)
-    auto make_perm = [&](const std::vector<char> &mc) {
-        std::vector<int32_t> color(n, 0);
-        bool any = false;
-        for (char c : mc) any = any || c;
-        if (any) {
-            std::vector<int32_t> mark;   // mark[c] == row: colour c taken by a neighbour
-            for (int64_t i = 0; i < n; i++) {
-                for (int32_t k = rs[i]; k < rs[i + 1]; k++) {
-                    const int32_t c = col[k];
-                    if (c < i && is_p[c] == is_p[i]) {
-                        if ((size_t)color[c] >= mark.size()) mark.resize(color[c] + 1, -1);
-                        mark[color[c]] = (int32_t)i;
-                    }
-                }
-                int32_t c = 0;
-                while ((size_t)c < mark.size() && mark[c] == i) c++;
-                color[i] = c;
-            }
-        }
-        int64_t w = 0;
-        std::vector<int32_t> rows;
-        for (int s = 0; s < p->nsub; s++)
-            for (char g = 0; g < 2; g++) {
-                rows.clear();
-                for (int64_t i = p->dof_off_h[s]; i < p->dof_off_h[s + 1]; i++)
-                    if (is_p[i] == g) rows.push_back((int32_t)i);
-                if (mc[s])
-                    std::stable_sort(rows.begin(), rows.end(),
-                                     [&](int32_t x, int32_t y) { return color[x] < color[y]; });
-                for (int32_t r : rows) perm[w++] = r;
-            }
-        for (int64_t i = 0; i < n; i++) iperm[perm[i]] = (int32_t)i;
-    };
-    std::vector<int32_t> prs(n + 1, 0), pcol(p->nnz), psrc(p->nnz), pdiag(n, -1), lev(n, 0);
-    auto make_pattern = [&]() -> int {
-        std::vector<std::pair<int32_t, int32_t>> tmp;
-        for (int64_t i = 0; i < n; i++) {
-            const int32_t o = perm[i];
-            tmp.clear();
-            for (int32_t k = rs[o]; k < rs[o + 1]; k++) tmp.push_back({iperm[col[k]], k});
-            std::sort(tmp.begin(), tmp.end());
-            prs[i + 1] = prs[i] + (int32_t)tmp.size();
-            pdiag[i] = -1;
-            for (size_t t = 0; t < tmp.size(); t++) {
-                pcol[prs[i] + t] = tmp[t].first;
-                psrc[prs[i] + t] = tmp[t].second;
-                if (tmp[t].first == i) pdiag[i] = prs[i] + (int32_t)t;
-            }
-            if (pdiag[i] < 0) {
-                set_error("ILU(0): a row has no structural diagonal entry");
-                return PYHIP_ERR_BREAKDOWN;
-            }
-        }
-        return PYHIP_OK;
-    };
-    // lower levels (depends on smaller columns)
-    auto lower_levels = [&]() {
-        int32_t nl = 0;
-        for (int64_t i = 0; i < n; i++) {
-            int32_t l = 0;
-            for (int32_t k = prs[i]; k < pdiag[i]; k++) l = std::max(l, lev[pcol[k]] + 1);
-            lev[i] = l;
-            nl = std::max(nl, l + 1);
-        }
-        return nl;
-    };
-    int rc0;
-    std::vector<char> mc(p->nsub, W->ilu_order == PYHIP_ILU_ORDER_MULTICOLOR ? 1 : 0);
-    make_perm(mc);
-    if ((rc0 = make_pattern())) return rc0;
-    int32_t nlo = lower_levels();
-    if (W->ilu_order == PYHIP_ILU_ORDER_AUTO && nlo > kIluAutoLevels) {
-        // per sub-problem, so that its order does not depend on the batch
-        bool any = false;
-        for (int s = 0; s < p->nsub; s++) {
-            int32_t nl = 0;
-            for (int64_t i = p->dof_off_h[s]; i < p->dof_off_h[s + 1]; i++) nl = std::max(nl, lev[i] + 1);
-            mc[s] = nl > kIluAutoLevels;
-            any = any || mc[s];
-        }
-        if (any) {
-            make_perm(mc);
-            if ((rc0 = make_pattern())) return rc0;
-            nlo = lower_levels();
-        }
+    IluHostPlan H;
+    std::string err;
+    if (ilu_plan_host(n, rs.data(), col.data(), p->dof_off_h, is_p, W->ilu_order, kIluAutoLevels, H, err)) {
+        set_error(err);
+        return PYHIP_ERR_BREAKDOWN;
     }
-    bool mc_any = false;
-    for (char c : mc) mc_any = mc_any || c;
-    auto bucket = [&](int32_t nl, std::vector<int32_t> &rows, std::vector<int64_t> &ptr) {
-        ptr.assign(nl + 1, 0);
-        for (int64_t i = 0; i < n; i++) ptr[lev[i] + 1]++;
-        for (int32_t l = 0; l < nl; l++) ptr[l + 1] += ptr[l];
-        rows.resize(n);
-        std::vector<int64_t> f(ptr.begin(), ptr.end() - 1);
-        for (int64_t i = 0; i < n; i++) rows[f[lev[i]]++] = (int32_t)i;
-    };
-    std::vector<int32_t> lo_rows, up_rows;
-    bucket(nlo, lo_rows, L.lo_ptr);
-    // upper levels (depends on larger columns), computed in decreasing row order
-    int32_t nup = 0;
-    for (int64_t i = n - 1; i >= 0; i--) {
-        int32_t l = 0;
-        for (int32_t k = pdiag[i] + 1; k < prs[i + 1]; k++) l = std::max(l, lev[pcol[k]] + 1);
-        lev[i] = l;
-        nup = std::max(nup, l + 1);
-    }
-    bucket(nup, up_rows, L.up_ptr);
+    const auto &perm = H.perm, &prs = H.prs, &pcol = H.pcol, &psrc = H.psrc, &pdiag = H.pdiag;
+    const auto &lo_rows = H.lo_rows, &up_rows = H.up_rows;
+    L.lo_ptr = H.lo_ptr;
+    L.up_ptr = H.up_ptr;
     L.requested = W->ilu_order;
-    L.multicolor = mc_any;
+    L.multicolor = H.multicolor;
     int rc;
 #define LUP(dst, v)                                                                        \
     if ((rc = kalloc(&(dst), (v).size(), W))) return rc;                                   \
