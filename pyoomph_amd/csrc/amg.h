@@ -29,8 +29,8 @@ struct AmgDevLevel {
     // host: per sub-problem rows, A entries and P entries (for counting the bytes of the
     // rows a filtered V-cycle visits)
     std::vector<int64_t> h_rows, h_nnz, h_nnzp;
-    // lanes per row of the level's spmv launches (A_l, restriction P^T, prolongation P):
-    // 0 until tuned (all lane counts give the same bits)
+    // spmv variant (spmv.h) of the level's launches (A_l, restriction P^T, prolongation P):
+    // 0 until tuned (all variants give the same bits)
     int lpr_a = 0, lpr_t = 0, lpr_p = 0;
 };
 

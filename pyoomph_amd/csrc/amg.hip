@@ -14,6 +14,7 @@
 // hierarchy, so its preconditioner is the same alone and in a batch; the V-cycle is a
 // symmetric operator (usable inside conjugate gradients).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "amg.h"
@@ -422,18 +423,25 @@ static int64_t active_sum(const std::vector<int64_t> &v, const int32_t *act_h) {
 }
 
 
-// Lane counts of each level's three spmv launches: the fastest of 1, 2, 4 and 8 over two
-// timed launches each, once per hierarchy (the canonical row order gives the same bits
-// for every lane count, so the choice changes no result). The outputs are scratch
-// vectors of the level.
+// Variants of each level's three spmv launches: the fastest of the canonical spmv's
+// variants (1, 2, 4, 8 lanes per row, LDS staging) over two timed launches each, once per
+// hierarchy (every variant gives the same bits, so the choice changes no result), or
+// PYHIP_SPMV_LPR for all of them. The outputs are scratch vectors of the level.
 static int tune_levels(AmgDev &A, hipStream_t st) {
+    if (const char *e = getenv("PYHIP_SPMV_LPR")) {
+        const int v = atoi(e);
+        if (spmv_variant_ok(v)) {
+            for (auto &L : A.lv) L.lpr_a = L.lpr_t = L.lpr_p = v;
+            return PYHIP_OK;
+        }
+    }
     hipEvent_t e0, e1;
     PYHIP_CHECK_HIP(hipEventCreate(&e0));
     PYHIP_CHECK_HIP(hipEventCreate(&e1));
     auto pick = [&](auto &&launch) {
         float best = 1e30f;
         int arg = 8;
-        for (int lpr : {1, 2, 4, 8}) {
+        for (int lpr : kSpmvVariants) {
             launch(lpr);
             (void)hipEventRecord(e0, st);
             for (int k = 0; k < 2; k++) launch(lpr);

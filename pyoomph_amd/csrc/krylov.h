@@ -83,7 +83,8 @@ struct KrylovWork {
     int32_t *flags_h = nullptr;       // pinned [3][nsub]: active, fail, iterations
     std::vector<int64_t> sub_nnz_h;   // [nsub] pattern entries per sub-problem
     std::vector<int32_t> chunk_ptr_h; // [nsub+1] host copy of sub_chunk_ptr
-    int lpr = 0;                      // spmv lanes per row (1, 2, 4, 8), 0 = not tuned
+    int lpr = 0;                      // work-list spmv lanes per row (1, 2, 4, 8), 0 = not tuned
+    int smode = 0;                    // spmv variant (spmv.h: lanes per row or LDS staging)
     int32_t *act_h = nullptr;         // [nsub] host copy of the active flags (pinned)
     // arclength continuation: Dof_current, Dof_derivative, y, z, scratch [n]; scalars [nsub]
     double *ac_cur = nullptr, *ac_der = nullptr, *ac_y = nullptr, *ac_z = nullptr, *ac_t = nullptr;

@@ -541,42 +541,48 @@ int batched_maxabs(pyhip_problem *p, const double *v, double *out) {
     return PYHIP_OK;
 }
 
-// Lanes per row of this problem's spmv: the fastest of 1, 2, 4 and 8 over two timed
-// launches each (all give the same bits), or PYHIP_SPMV_LPR. y is scratch here.
+// Variant of this problem's spmv: the fastest of the canonical spmv's variants (1, 2, 4 or
+// 8 lanes per row, LDS staging) over two timed launches each, all giving the same bits, or
+// PYHIP_SPMV_LPR. The work-list CG kernel takes the fastest lane count. y is scratch here.
 template <typename TV, typename TX, typename TY>
 static int spmv_tune(pyhip_problem *p, KrylovWork *W, const TV *vals, const TX *x, TY *y) {
-    if (W->lpr) return PYHIP_OK;
+    if (W->smode) return PYHIP_OK;
     if (const char *e = getenv("PYHIP_SPMV_LPR")) {
         const int v = atoi(e);
-        if (v == 1 || v == 2 || v == 4 || v == 8) {
-            W->lpr = v;
+        if (spmv_variant_ok(v)) {
+            W->smode = v;
+            W->lpr = v <= 8 ? v : 8;
             return PYHIP_OK;
         }
     }
     const int64_t n = p->ndof;
-    W->lpr = 8;
+    W->smode = W->lpr = 8;
     if (n == 0) return PYHIP_OK;
     hipEvent_t e0, e1;
     PYHIP_CHECK_HIP(hipEventCreate(&e0));
     PYHIP_CHECK_HIP(hipEventCreate(&e1));
-    float best = 1e30f;
-    for (int lpr : {1, 2, 4, 8}) {
-        launch_spmv<TV, TX, TY>(lpr, p->st, n, p->row_start, p->col, vals, x, y, nullptr);
+    float best = 1e30f, best_lanes = 1e30f;
+    for (int v : kSpmvVariants) {
+        launch_spmv<TV, TX, TY>(v, p->st, n, p->row_start, p->col, vals, x, y, nullptr);
         (void)hipEventRecord(e0, p->st);
         for (int k = 0; k < 2; k++)
-            launch_spmv<TV, TX, TY>(lpr, p->st, n, p->row_start, p->col, vals, x, y, nullptr);
+            launch_spmv<TV, TX, TY>(v, p->st, n, p->row_start, p->col, vals, x, y, nullptr);
         (void)hipEventRecord(e1, p->st);
         float ms = 0.f;
-        if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
-            ms < best) {
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
+        if (ms < best) {
             best = ms;
-            W->lpr = lpr;
+            W->smode = v;
+        }
+        if (v <= 8 && ms < best_lanes) {
+            best_lanes = ms;
+            W->lpr = v;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     KCHECK();
-    if (debug_enabled()) fprintf(stderr, "[pyhip] spmv lanes per row: %d\n", W->lpr);
+    if (debug_enabled()) fprintf(stderr, "[pyhip] spmv variant: %d, work-list lanes per row: %d\n", W->smode, W->lpr);
     return PYHIP_OK;
 }
 
@@ -590,7 +596,7 @@ static int spmv(pyhip_problem *p, const TV *vals, const TX *x, TY *y, const TY *
     int rc;
     if ((rc = spmv_tune<TV, TX, TY>(p, p->kw, vals, x, y))) return rc;
     TimeScope ts(&p->timer, "spmv", p->st);
-    launch_spmv<TV, TX, TY>(p->kw->lpr, p->st, n, p->row_start, p->col, vals, x, y, b, act ? p->dof_sub : nullptr,
+    launch_spmv<TV, TX, TY>(p->kw->smode, p->st, n, p->row_start, p->col, vals, x, y, b, act ? p->dof_sub : nullptr,
                             act);
     KCHECK();
     if (p->timer.on) {

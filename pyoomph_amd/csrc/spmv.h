@@ -133,33 +133,140 @@ __global__ __launch_bounds__(256) void spmv_add_kernel(int64_t n, const int32_t 
     }
 }
 
+// The same row sums through LDS, for short rows: a block of 256 rows streams its entries
+// in chunks of CAP with coalesced loads (each lane loads entries t, t+256, ... of the chunk:
+// the value and x[col], both kept in LDS), then one thread per row adds the chunk's entries
+// of its row to its eight stripe sums (entry k0+8g+j to stripe j, in increasing k), so the
+// result has rows_dot's bits. Rows longer than a chunk span several chunks; their stripe
+// sums stay in registers. ADD: y = b + A x; otherwise y = A x, or b - A x when b is given.
+// Rows the filter skips keep their y; a block none of whose rows is live returns at once.
+template <typename TV, typename TX, typename TY, int CAP, bool ADD>
+__global__ __launch_bounds__(256) void spmv_lds_kernel(int64_t n, const int32_t *__restrict__ rs,
+                                                       const int32_t *__restrict__ col,
+                                                       const TV *__restrict__ val, const TX *__restrict__ x,
+                                                       TY *y, const TY *b, const int32_t *__restrict__ rsub,
+                                                       const int32_t *__restrict__ act) {
+    constexpr int U = (CAP + 255) / 256;
+    __shared__ TY sv[CAP], sx[CAP];
+    const int64_t nb = (n + 255) / 256, per = (nb + 7) / 8, kb = blockIdx.x / 8;
+    const int64_t lb = (int64_t)(blockIdx.x % 8) * per + kb;
+    if (kb >= per || lb >= nb) return;
+    const int64_t r0 = lb * 256, row = r0 + threadIdx.x, rend = r0 + 256 < n ? r0 + 256 : n;
+    const bool live = row < n && (act == nullptr || act[rsub[row]] != 0);
+    if (!__syncthreads_or(live)) return;
+    const int32_t k0 = live ? rs[row] : 0, k1 = live ? rs[row + 1] : 0;
+    const int32_t kb0 = rs[r0], kb1 = rs[rend];
+    TY acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc[j] = TY(0);
+    for (int32_t cs = kb0; cs < kb1; cs += CAP) {
+        const int32_t ce = min(cs + CAP, kb1);
+        int32_t cc[U];
+        TY vv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int32_t k = cs + u * 256 + (int32_t)threadIdx.x;
+            const bool in = k < ce;
+            cc[u] = in ? col[k] : -1;
+            vv[u] = in ? TY(val[k]) : TY(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (cc[u] >= 0) {
+                const int32_t t = u * 256 + (int32_t)threadIdx.x;
+                sv[t] = vv[u];
+                sx[t] = TY(x[cc[u]]);
+            }
+        }
+        __syncthreads();
+        const int32_t lo = max(k0, cs), hi = min(k1, ce);
+        if (lo < hi) {
+            for (int32_t g = k0 + ((lo - k0) & ~7); g < hi; g += 8) {
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int32_t k = g + j;
+                    if (k >= lo && k < hi) acc[j] = fmadd(sv[k - cs], sx[k - cs], acc[j]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (live) {
+        const TY v = ((acc[0] + acc[4]) + (acc[2] + acc[6])) + ((acc[1] + acc[5]) + (acc[3] + acc[7]));
+        y[row] = ADD ? b[row] + v : (b ? b[row] - v : v);
+    }
+}
+
+// Variants of the canonical spmv (all give the same bits): 1, 2, 4, 8 lanes per row
+// (spmv_kernel), or LDS staging with chunks of 512, 1024, 2048 or 4096 entries
+// (spmv_lds_kernel).
+enum { kSpmvLds512 = 16, kSpmvLds1K = 32, kSpmvLds2K = 64, kSpmvLds4K = 128 };
+static constexpr int kSpmvVariants[] = {1, 2, 4, 8, kSpmvLds512, kSpmvLds1K, kSpmvLds2K, kSpmvLds4K};
+static inline bool spmv_variant_ok(int v) {
+    for (int w : kSpmvVariants)
+        if (v == w) return true;
+    return false;
+}
+
 static inline unsigned spmv_grid(int64_t n) {
     const int64_t nb = std::max<int64_t>(1, (n + 255) / 256);
     return (unsigned)(((nb + 7) / 8) * 8);
 }
 
 template <typename TV, typename TX, typename TY>
-static void launch_spmv(int lpr, hipStream_t st, int64_t n, const int32_t *rs, const int32_t *col,
+static void launch_spmv(int variant, hipStream_t st, int64_t n, const int32_t *rs, const int32_t *col,
                         const TV *vals, const TX *x, TY *y, const TY *b, const int32_t *rsub = nullptr,
                         const int32_t *act = nullptr) {
     const dim3 g(spmv_grid(n)), t(256);
-    switch (lpr) {
+    switch (variant) {
     case 1: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 1>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
     case 2: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 2>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
     case 4: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 4>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
+    case kSpmvLds512:
+        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 512, false>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
+                           act);
+        break;
+    case kSpmvLds1K:
+        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 1024, false>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
+                           act);
+        break;
+    case kSpmvLds2K:
+        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 2048, false>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
+                           act);
+        break;
+    case kSpmvLds4K:
+        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 4096, false>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
+                           act);
+        break;
     default: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 8>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
     }
 }
 
 template <typename TV, typename TX, typename TY>
-static void launch_spmv_add(int lpr, hipStream_t st, int64_t n, const int32_t *rs, const int32_t *col,
+static void launch_spmv_add(int variant, hipStream_t st, int64_t n, const int32_t *rs, const int32_t *col,
                             const TV *vals, const TX *x, TY *y, const TY *b, const int32_t *rsub = nullptr,
                             const int32_t *act = nullptr) {
     const dim3 g(spmv_grid(n)), t(256);
-    switch (lpr) {
+    switch (variant) {
     case 1: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 1>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
     case 2: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 2>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
     case 4: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 4>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
+    case kSpmvLds512:
+        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 512, true>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
+                           act);
+        break;
+    case kSpmvLds1K:
+        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 1024, true>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
+                           act);
+        break;
+    case kSpmvLds2K:
+        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 2048, true>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
+                           act);
+        break;
+    case kSpmvLds4K:
+        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 4096, true>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
+                           act);
+        break;
     default: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 8>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
     }
 }

@@ -144,11 +144,15 @@ def test_batch_equals_alone_bitwise(gpu):
     ("bratu", dict(N=12, space="C2", lam=2.0), dict(krylov="cg", preconditioner="jacobi")),
     ("rd", dict(N=3), dict(krylov="cg", preconditioner="jacobi")),
     ("ns", dict(N=5), dict(krylov="gmres", preconditioner="ilu0")),
+    ("ns", dict(N=9), dict(krylov="gmres", preconditioner="ilu0", gmres_restart=100)),
+    ("bratu", dict(N=40, space="C2", lam=2.0), dict(krylov="cg", preconditioner="amg")),
+    ("rd", dict(N=6), dict(krylov="cg", preconditioner="amg")),
 ])
 def test_spmv_lane_count_keeps_bits(gpu, monkeypatch, kind, kw, opts):
-    """Every spmv lane count (1, 2, 4, 8 lanes per row) sums a row in the same order."""
+    """Every spmv variant (1, 2, 4, 8 lanes per row; LDS staging in chunks of 512, 1024,
+    2048 or 4096 entries, with rows that span chunks) sums a row in the same order."""
     xs = []
-    for lpr in ("1", "2", "4", "8"):
+    for lpr in ("1", "2", "4", "8", "16", "32", "64", "128"):
         monkeypatch.setenv("PYHIP_SPMV_LPR", lpr)
         P, op = build(kind, kw)
         for k, v in opts.items():
