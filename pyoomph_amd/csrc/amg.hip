@@ -136,55 +136,91 @@ __device__ __forceinline__ int32_t find_col(const int32_t *__restrict__ cols, in
     return -1;
 }
 
-// Q[i][J] = sum over k in row i (increasing) of A[i][k] P[k][J]; kQLanes lanes per row,
-// one entry per lane at a time
-constexpr int kQLanes = 16, kCLanes = 32;
-
-__global__ __launch_bounds__(256) void amg_qvals_kernel(int64_t n, const int32_t *__restrict__ rs,
-                                                        const int32_t *__restrict__ col,
-                                                        const double *__restrict__ vals,
-                                                        const int32_t *__restrict__ prs,
-                                                        const int32_t *__restrict__ pcol,
-                                                        const double *__restrict__ pv,
-                                                        const int32_t *__restrict__ qrs,
-                                                        const int32_t *__restrict__ qcol, double *__restrict__ qv) {
-    const int64_t i = (int64_t)blockIdx.x * (256 / kQLanes) + threadIdx.x / kQLanes;
-    if (i >= n) return;
-    for (int32_t q = qrs[i] + threadIdx.x % kQLanes; q < qrs[i + 1]; q += kQLanes) {
-        const int32_t J = qcol[q];
-        double s = 0.0;
-        for (int32_t k = rs[i]; k < rs[i + 1]; k++) {
-            const int32_t c = col[k];
-            const int32_t e = find_col(pcol, prs[c], prs[c + 1], J);
-            if (e >= 0) s += vals[k] * pv[e];
-        }
-        qv[q] = s;
-    }
+// Sparse row products with fixed patterns, for Q = A P and A_{l+1} = P^T Q:
+//   out[I][J] = sum over the terms t of row I (increasing t) of tval[t] * src[trow[t]][J]
+// (Q: terms = the entries k of A's row i, tval = A[i][k], source rows = P's rows col[k];
+// Galerkin: terms = the entries of P^T's row I, tval = P[i][I], source rows = Q's rows i).
+// Every row pattern is column-sorted, and the output pattern holds every column a term's
+// source row has. G lanes own one output row: its columns and sums sit in LDS; the terms
+// are fetched G at a time (row, value, source row bounds, one per lane) and walked in
+// order, the lanes spreading over the source row's entries, each finding its column in
+// the output row by binary search in LDS and adding its product with a fused multiply-add
+// (a source row has distinct columns, so no two lanes touch one sum in a step). A sum thus
+// receives its terms in increasing t whatever G, alone or in a batch. Output rows longer
+// than CAPR take the same sums entry by entry (binary search in the source rows).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// A_{l+1}[I][J] = sum over the fine rows i of P^T row I (increasing) of P[i][I] Q[i][J];
-// kCLanes lanes per coarse row, one entry per lane at a time
-__global__ __launch_bounds__(256) void amg_galerkin_kernel(int64_t nc, const int32_t *__restrict__ trs,
-                                                           const int32_t *__restrict__ trow,
-                                                           const double *__restrict__ ptv,
-                                                           const int32_t *__restrict__ qrs,
-                                                           const int32_t *__restrict__ qcol,
-                                                           const double *__restrict__ qv,
-                                                           const int32_t *__restrict__ crs,
-                                                           const int32_t *__restrict__ ccol, double *__restrict__ cv) {
-    const int64_t I = (int64_t)blockIdx.x * (256 / kCLanes) + threadIdx.x / kCLanes;
-    if (I >= nc) return;
-    for (int32_t q = crs[I] + threadIdx.x % kCLanes; q < crs[I + 1]; q += kCLanes) {
-        const int32_t J = ccol[q];
-        double s = 0.0;
-        for (int32_t t = trs[I]; t < trs[I + 1]; t++) {
-            const int32_t i = trow[t];
-            const int32_t e = find_col(qcol, qrs[i], qrs[i + 1], J);
-            if (e >= 0) s += ptv[t] * qv[e];
+template <int G, int CAPR>
+__global__ __launch_bounds__(256) void amg_rowprod_kernel(int64_t nout, const int32_t *__restrict__ ors,
+                                                          const int32_t *__restrict__ ocol, double *__restrict__ oval,
+                                                          const int32_t *__restrict__ trs,
+                                                          const int32_t *__restrict__ trow,
+                                                          const double *__restrict__ tval,
+                                                          const int32_t *__restrict__ srs,
+                                                          const int32_t *__restrict__ scol,
+                                                          const double *__restrict__ sval) {
+    static_assert(G == 16 || G == 32 || G == 64, "lanes per row");
+    constexpr int NG = 256 / G;
+    __shared__ int32_t lc[NG][CAPR];
+    __shared__ double la[NG][CAPR];
+    const int g = threadIdx.x / G, lane = threadIdx.x % G;
+    const int64_t I = (int64_t)blockIdx.x * NG + g;
+    if (I >= nout) return;   // a whole group; no workgroup barrier below
+    const int32_t o0 = ors[I], R = ors[I + 1] - o0, t0 = trs[I], t1 = trs[I + 1];
+    if (R > CAPR) {
+        for (int32_t q = o0 + lane; q < o0 + R; q += G) {
+            const int32_t J = ocol[q];
+            double s = 0.0;
+            for (int32_t t = t0; t < t1; t++) {
+                const int32_t r = trow[t];
+                const int32_t e = find_col(scol, srs[r], srs[r + 1], J);
+                if (e >= 0) s = __builtin_fma(tval[t], sval[e], s);
+            }
+            oval[q] = s;
         }
-        cv[q] = s;
+        return;
     }
+    int32_t *mc = lc[g];
+    double *ma = la[g];
+    for (int32_t k = lane; k < R; k += G) {
+        mc[k] = ocol[o0 + k];
+        ma[k] = 0.0;
+    }
+    wave_lds_sync();
+    for (int32_t tb = t0; tb < t1; tb += G) {
+        const int32_t t = tb + lane;
+        int32_t sb = 0, se = 0;
+        double tv = 0.0;
+        if (t < t1) {
+            const int32_t r = trow[t];
+            tv = tval[t];
+            sb = srs[r];
+            se = srs[r + 1];
+        }
+        const int nt = min(G, t1 - tb);
+        for (int u = 0; u < nt; u++) {
+            const double p = __shfl(tv, u, G);
+            const int32_t b = __shfl(sb, u, G), e = __shfl(se, u, G);
+            for (int32_t x = b + lane; x < e; x += G) {
+                const int32_t J = scol[x];
+                int32_t lo = 0, hi = R;
+                while (lo < hi) {
+                    const int32_t m = (lo + hi) >> 1;
+                    if (mc[m] < J) lo = m + 1;
+                    else hi = m;
+                }
+                if (lo < R && mc[lo] == J) ma[lo] = __builtin_fma(p, sval[x], ma[lo]);
+            }
+            wave_lds_sync();
+        }
+    }
+    for (int32_t k = lane; k < R; k += G) oval[o0 + k] = ma[k];
 }
+constexpr int kQLanes = 16, kQCap = 64, kCLanes = 32, kCCap = 128;
 
 // P^T values in P^T order
 __global__ __launch_bounds__(256) void amg_ptvals_kernel(int64_t nnzp, const int32_t *__restrict__ tsrc,
@@ -513,12 +549,14 @@ int amg_setup(AmgDev &A, hipStream_t st, const double *vals0, KernelTimer *tm) {
         hipLaunchKernelGGL(amg_pvals_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.prs, L.pcol, L.pcptr,
                            L.pcidx, L.agg, vals, L.w, L.pv);
         hipLaunchKernelGGL(amg_ptvals_kernel, dim3(ablk(L.nnzp)), dim3(256), 0, st, L.nnzp, L.tsrc, L.pv, L.ptv);
-        hipLaunchKernelGGL(amg_qvals_kernel, dim3(ablk(L.n, 256 / kQLanes)), dim3(256), 0, st, L.n, L.rs, L.col,
-                           vals, L.prs, L.pcol, L.pv, L.qrs, L.qcol, L.qv);
+        // Q = A_l P (terms: A's row entries; source rows: P's rows)
+        hipLaunchKernelGGL((amg_rowprod_kernel<kQLanes, kQCap>), dim3(ablk(L.n, 256 / kQLanes)), dim3(256), 0, st,
+                           L.n, L.qrs, L.qcol, L.qv, L.rs, L.col, vals, L.prs, L.pcol, L.pv);
         AmgDevLevel &C = (l + 1 < A.lv.size()) ? A.lv[l + 1] : A.bot;
+        // A_{l+1} = P^T Q (terms: P^T's row entries; source rows: Q's rows)
         if (L.nc)
-            hipLaunchKernelGGL(amg_galerkin_kernel, dim3(ablk(L.nc, 256 / kCLanes)), dim3(256), 0, st, L.nc, L.trs,
-                               L.trow, L.ptv, L.qrs, L.qcol, L.qv, C.rs, C.col, C.own_vals);
+            hipLaunchKernelGGL((amg_rowprod_kernel<kCLanes, kCCap>), dim3(ablk(L.nc, 256 / kCLanes)), dim3(256), 0,
+                               st, L.nc, C.rs, C.col, C.own_vals, L.trs, L.trow, L.ptv, L.qrs, L.qcol, L.qv);
         ACHECK();
     }
     A.bot.vals = A.lv.empty() ? vals0 : A.bot.own_vals;
