@@ -22,6 +22,9 @@ It restates, in plain Python / numpy / C, the reference algorithms on the hot pa
                     (oomph problem.cc:5459-5659).
 * ``assembly``   -- element loop, residual assembly and CSR (oomph problem.cc:3810-3900,
                     4038-4170).
+* ``faces``      -- boundary-face (interface) elements carrying NeumannBC(u=alpha*u + beta)
+                    on P1 / P2 triangles (pyoomph/meshes/bcs.py:71-89; src/mesh.cpp:615-667;
+                    oomph Telements.h TElementShape<1,2>, <1,3>; integral.cc:303-316).
 * ``newton``     -- oomph's Newton loop with the SuperLU solve called the way pyoomph calls
                     it (oomph problem.cc:8986-9345; double_vector.cc:611-640;
                     pyoomph/solvers/scipy.py:47-104).

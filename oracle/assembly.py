@@ -75,9 +75,13 @@ class OracleProblem:
     params    -- form parameters (Poisson: 4 per Gaussian cx,cy,w,a; Bratu: lambda;
                  Navier-Stokes: mu, rho)
     aux       -- nodal auxiliary field (reaction-diffusion source f) or None
+    faces     -- [nface, 2|3] boundary-face nodes (``faces.boundary_faces``) carrying the
+                 flux face_alpha*u + face_beta (``oracle/faces.py``), or None; the face
+                 elements follow the bulk elements in element order
     """
 
-    def __init__(self, mesh, form, nvalue, pinned, bc, params, aux=None):
+    def __init__(self, mesh, form, nvalue, pinned, bc, params, aux=None, faces=None, face_alpha=0.0,
+                 face_beta=0.0):
         from .meshes import number_equations
         self.mesh, self.form, self.nvalue = mesh, form, nvalue
         self.pinned = np.asarray(pinned, dtype=bool)
@@ -106,6 +110,22 @@ class OracleProblem:
         self.geqn = np.full((E.shape[0], ns), -1, dtype=np.int64)
         rows = np.nonzero(free)
         self.geqn[rows[0], self.leqn[rows]] = g[rows]
+        self.faces = None
+        self.face_alpha, self.face_beta = float(face_alpha), float(face_beta)
+        if faces is not None and len(faces):
+            if nvalue != 1:
+                raise ValueError("boundary-face fluxes are built for the scalar forms")
+            self.faces = np.asarray(faces, dtype=np.int64)
+            fg = self.eqn[self.faces, 0]                      # [nface, nn]
+            ffree = fg >= 0
+            self.face_leqn = np.full(fg.shape, -1, dtype=np.int64)
+            self.face_leqn[ffree] = (np.cumsum(ffree, axis=1) - 1)[ffree]
+            fnv = ffree.sum(axis=1).astype(np.int64)
+            fgeqn = np.full((fg.shape[0], ns), -1, dtype=np.int64)
+            r = np.nonzero(ffree)
+            fgeqn[r[0], self.face_leqn[r]] = fg[r]
+            self.nvar = np.concatenate([self.nvar, fnv])
+            self.geqn = np.concatenate([self.geqn, fgeqn])
 
     # -- time stepping (scalar forms) --------------------------------------------------
     # du/dt = tw[0] u + (tw[1] u(t-1) + tw[2] u(t-2)), nodal history combined per node;
@@ -168,7 +188,40 @@ class OracleProblem:
             len(self.params), int(flag), int(deriv), self.maxdof, _dp(H), tw0, _dp(Y), _dp(R), _dp(J))
         if rc != 0:
             raise RuntimeError("oracle_elements failed: %d" % rc)
+        if self.faces is not None:
+            R, J = self._append_faces(U, R, J, flag, deriv, y)
         return R, J
+
+    def _append_faces(self, U, R, J, flag, deriv, y):
+        """Face element blocks after the bulk ones, in the same compact local numbering
+        (residual stride maxdof, Jacobian block e at e*maxdof^2 with row stride nvar)."""
+        from .faces import face_outputs
+        F = self.faces
+        nf, nn = F.shape
+        Xf = np.ascontiguousarray(self.mesh.coords[F][:, :, :2])
+        Uf = np.ascontiguousarray(U[F, 0])
+        Rs, Js = face_outputs(Xf, Uf, self.face_alpha, self.face_beta, 1 if flag == 1 else 0)
+        if deriv:
+            Rs[:] = 0.0
+        if deriv or flag == 2 or y is not None:
+            Js[:] = 0.0
+        md = self.maxdof
+        Rf = np.zeros((nf, md))
+        Jf = np.zeros((nf, md * md))
+        nel = self.mesh.elements.shape[0]
+        for f in range(nf):
+            loc = self.face_leqn[f]
+            fr = np.nonzero(loc >= 0)[0]
+            nv = len(fr)
+            Rf[f, loc[fr]] = Rs[f, fr]
+            blk = np.zeros((nv, nv))
+            blk[np.ix_(loc[fr], loc[fr])] = Js[f][np.ix_(fr, fr)]
+            Jf[f, :nv * nv] = blk.ravel()
+            assert nv == self.nvar[nel + f]
+        R = np.concatenate([R, Rf])
+        if J is not None:
+            J = np.concatenate([J, Jf.reshape(nf, md, md)])
+        return np.ascontiguousarray(R), None if J is None else np.ascontiguousarray(J)
 
     def get_residuals(self, x, deriv=0):
         """oomph Problem::get_residuals: residuals[eqn] += el_res[i] in element order."""

@@ -1,0 +1,90 @@
+"""Boundary-face (interface) elements of the scalar forms: the flux of pyoomph's
+``NeumannBC(u=flux) @ boundary`` with flux = alpha*u + beta.
+
+Test infrastructure only (see ``oracle/__init__``).
+
+Restates, for P1 / P2 triangle meshes:
+
+* which faces exist: ``Mesh`` interface elements are built on the bulk elements' faces
+  that lie on the named boundary (src/mesh.cpp:615-660; ``InterfaceTElementLine1dC1`` /
+  ``C2`` for ``BulkElementTri2dC1`` / ``C2``), one boundary after the other, bulk
+  elements in element order, faces of an element in edge order (0-1, 1-2, 2-0);
+* the face element: a ``TElement<1,2>`` / ``TElement<1,3>`` line (nodes at s = 0, 1 or
+  0, 1/2, 1; oomph Telements.h ``TElementShape<1,2>``, ``<1,3>`` shape and dshape_local)
+  with its default ``TGauss<1,2>`` / ``TGauss<1,3>`` rule (oomph integral.cc:303-316;
+  the generated functable's integration order is 0, so the element default is kept,
+  src/mesh.cpp:664-667);
+* the residual ``weak(flux, testfunction(u))`` (pyoomph/meshes/bcs.py:84-89): per point
+  W = w * |dx/ds|, R_i += W * (flux * psi_i), J_ij += W * ((alpha * psi_j) * psi_i).
+
+Pinned by closed forms in tests/test_oracle_faces.py: the edge integrals of 1 and of
+linear data (P1: L/2 per node; P2: L * (1/6, 2/3, 1/6)), the 1D mass matrices
+(L/6 [[2,1],[1,2]] and L/30 [[4,2,-1],[2,16,2],[-1,2,4]]), the Robin Jacobian as the
+derivative of the residual (finite differences), and a Poisson problem whose exact
+solution with a Neumann side is known.
+"""
+import numpy as np
+
+# TGauss<1,2> / TGauss<1,3>: knots on [0, 1] and weights, as the reference writes them
+_RULES = {
+    2: (np.array([0.5 * (-0.577350269189626 + 1.0), 0.5 * (0.577350269189626 + 1.0)]), np.array([0.5, 0.5])),
+    3: (np.array([0.5 * (-0.774596669241483 + 1.0), 0.5, 0.5 * (0.774596669241483 + 1.0)]),
+        np.array([5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0])),
+}
+
+# triangle edges in local nodes, ends and (P2) midside node in face order
+_EDGES = {"tri_p1": [(0, 1), (1, 2), (2, 0)], "tri_p2": [(0, 3, 1), (1, 4, 2), (2, 5, 0)]}
+
+
+def shape(nn, s):
+    """TElementShape<1,nn>: psi, dpsi/ds at local coordinate s."""
+    if nn == 2:
+        return np.array([1.0 - s, s]), np.array([-1.0, 1.0])
+    psi = np.array([2.0 * (s - 1.0) * (s - 0.5), 4.0 * (1.0 - s) * s, 2.0 * (s - 0.5) * s])
+    dpsi = np.array([4.0 * s - 3.0, 4.0 - 8.0 * s, 4.0 * s - 1.0])
+    return psi, dpsi
+
+
+def boundary_faces(mesh, names):
+    """[nface, 2|3] node ids of the faces on each named boundary (in name order; bulk
+    elements in order, edges 0-1, 1-2, 2-0)."""
+    if mesh.kind not in _EDGES:
+        raise ValueError("boundary faces are built for triangle meshes")
+    faces = []
+    for name in names:
+        on = mesh.on_boundary([name])
+        for e in range(mesh.nelement):
+            el = mesh.elements[e]
+            for edge in _EDGES[mesh.kind]:
+                nodes = [int(el[k]) for k in edge]
+                if all(on[n] for n in nodes):
+                    faces.append(nodes)
+    nn = len(_EDGES[mesh.kind][0])
+    return np.array(faces, dtype=np.int64).reshape(-1, nn)
+
+
+def face_outputs(X, U, alpha, beta, flag):
+    """Element residuals [nf, nn] and Jacobians [nf, nn, nn] in face-node numbering.
+
+    X [nf, nn, 2] node coordinates, U [nf, nn] nodal values."""
+    nf, nn = U.shape
+    knots, weights = _RULES[nn]
+    R = np.zeros((nf, nn))
+    J = np.zeros((nf, nn, nn))
+    for g in range(nn):
+        psi, dpsi = shape(nn, knots[g])
+        tx = np.zeros(nf)
+        ty = np.zeros(nf)
+        u = np.zeros(nf)
+        for l in range(nn):
+            tx = tx + X[:, l, 0] * dpsi[l]
+            ty = ty + X[:, l, 1] * dpsi[l]
+            u = u + U[:, l] * psi[l]
+        W = weights[g] * np.sqrt(tx * tx + ty * ty)
+        flux = alpha * u + beta
+        for i in range(nn):
+            R[:, i] = R[:, i] + W * (flux * psi[i])
+            if flag == 1:
+                for j in range(nn):
+                    J[:, i, j] = J[:, i, j] + W * ((alpha * psi[j]) * psi[i])
+    return R, J

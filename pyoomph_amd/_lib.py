@@ -36,7 +36,9 @@ class SubproblemDesc(ctypes.Structure):
                 ("nnode", ctypes.c_int64), ("nelem", ctypes.c_int64),
                 ("coords", c_dp), ("conn", c_i32p), ("nvalue", ctypes.c_int),
                 ("eqn", c_i32p), ("pinned_values", c_dp), ("ndof", ctypes.c_int64),
-                ("params", c_dp), ("nparams", ctypes.c_int), ("aux", c_dp)]
+                ("params", c_dp), ("nparams", ctypes.c_int), ("aux", c_dp),
+                ("nface", ctypes.c_int64), ("face_conn", c_i32p), ("face_alpha", ctypes.c_double),
+                ("face_beta", ctypes.c_double)]
 
 
 class ElementTable(ctypes.Structure):

@@ -112,7 +112,18 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         PYHIP_REQUIRE(d.form != PYHIP_FORM_REACTION_DIFFUSION || d.aux,
                       "reaction-diffusion needs the nodal source field (aux)");
         PYHIP_REQUIRE(d.ndof >= 0, "negative ndof");
+        PYHIP_REQUIRE(d.nface >= 0 && (d.nface == 0 || d.face_conn), "faces without connectivity");
+        PYHIP_REQUIRE(d.nface == 0 || ((d.elem_kind == PYHIP_ELEM_TRI_P1 || d.elem_kind == PYHIP_ELEM_TRI_P2) &&
+                                       d.nvalue == 1),
+                      "boundary-face flux elements: scalar forms on P1 / P2 triangles");
+        PYHIP_REQUIRE(d.nface == 0 || d.nparams <= kFaceAlpha, "too many parameters for a problem with faces");
         std::pair<int, int> k(d.form, d.elem_kind);
+        if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
+    }
+    // face groups after the bulk groups, one per face kind
+    for (int s = 0; s < nsub; s++) {
+        if (subs[s].nface == 0) continue;
+        std::pair<int, int> k(kFormFlux, subs[s].elem_kind == PYHIP_ELEM_TRI_P1 ? kKindLine2 : kKindLine3);
         if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
     }
     P->groups.resize(gkeys.size());
@@ -120,6 +131,11 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         Group &G = P->groups[g];
         G.form = gkeys[g].first;
         G.kind = gkeys[g].second;
+        if (G.form == kFormFlux) {
+            G.ns = G.nn = G.kind == kKindLine2 ? 2 : 3;
+            G.dim = 2;
+            continue;
+        }
         G.ns = form_nslot(G.form, G.kind);
         G.nn = elem_nnode(G.kind);
         G.dim = elem_dim(G.kind);
@@ -142,6 +158,13 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         S.el_off = P->groups[S.group].nel;
         S.nel = d.nelem;
         P->groups[S.group].nel += d.nelem;
+        if (d.nface > 0) {
+            const std::pair<int, int> fk(kFormFlux, d.elem_kind == PYHIP_ELEM_TRI_P1 ? kKindLine2 : kKindLine3);
+            S.fgroup = (int)(std::find(gkeys.begin(), gkeys.end(), fk) - gkeys.begin());
+            S.fel_off = P->groups[S.fgroup].nel;
+            S.nface = d.nface;
+            P->groups[S.fgroup].nel += d.nface;
+        }
         P->dof_off_h[s + 1] = S.dof_off + d.ndof;
         node_off += d.nnode;
     }
@@ -186,6 +209,24 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
             if (d.aux) aux_h[S.node_off + n] = d.aux[n];
         }
         for (int k = 0; k < d.nparams; k++) P->params_h[(size_t)s * kParamStride + k] = d.params[k];
+        if (S.nface > 0) {
+            P->params_h[(size_t)s * kParamStride + kFaceAlpha] = d.face_alpha;
+            P->params_h[(size_t)s * kParamStride + kFaceBeta] = d.face_beta;
+            const Group &F = P->groups[S.fgroup];
+            for (int64_t f = 0; f < d.nface; f++) {
+                const int64_t ge = S.fel_off + f;
+                sub_g[S.fgroup][ge] = s;
+                for (int l = 0; l < F.nn; l++) {
+                    const int32_t n = d.face_conn[f * F.nn + l];
+                    PYHIP_REQUIRE(n >= 0 && n < d.nnode, "face connectivity out of range");
+                    conn_sm[S.fgroup][(int64_t)l * F.nel + ge] = (int32_t)(S.node_off + n);
+                    const int32_t q = d.eqn[n];
+                    const int32_t gq = q >= 0 ? (int32_t)(q + S.dof_off) : -1;
+                    seqn_em[S.fgroup][ge * F.ns + l] = gq;
+                    seqn_sm[S.fgroup][(int64_t)l * F.nel + ge] = gq;
+                }
+            }
+        }
         std::vector<int32_t> &sem = seqn_em[S.group];
         for (int64_t e = 0; e < d.nelem; e++) {
             const int64_t ge = S.el_off + e;
@@ -218,6 +259,17 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         B.er_off = G.er_off + S.el_off * G.ns;
         B.ej_off = G.ej_off + S.el_off * G.ns * G.ns;
         blocks.push_back(B);
+        if (S.nface > 0) {
+            // the interface elements follow the bulk elements of their mesh
+            const Group &F = P->groups[S.fgroup];
+            ElementBlock FB;
+            FB.ns = F.ns;
+            FB.nel = S.nface;
+            FB.seqn = seqn_em[S.fgroup].data() + S.fel_off * F.ns;
+            FB.er_off = F.er_off + S.fel_off * F.ns;
+            FB.ej_off = F.ej_off + S.fel_off * F.ns * F.ns;
+            blocks.push_back(FB);
+        }
     }
     HostPattern hp;
     std::string err;
@@ -252,7 +304,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         UP(G.conn, conn_sm[g].data(), conn_sm[g].size());
         UP(G.seqn, seqn_sm[g].data(), seqn_sm[g].size());
         UP(G.sub, sub_g[g].data(), sub_g[g].size());
-        if (!P->tab64[G.kind]) {
+        if (G.form != kFormFlux && !P->tab64[G.kind]) {
             rc = upload_tables<double>(G.kind, &P->tab64[G.kind], B);
             if (rc) return rc;
             rc = upload_tables<float>(G.kind, &P->tab32[G.kind], B);
@@ -344,7 +396,7 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
     PYHIP_REQUIRE(flag != 2 || deriv_param < 0, "no mass matrix of a parameter derivative");
     PYHIP_REQUIRE(deriv_param == -1 || deriv_param == 0, "unknown global parameter");
     for (auto &G : p->groups)
-        PYHIP_REQUIRE(deriv_param < 0 || G.form == PYHIP_FORM_BRATU,
+        PYHIP_REQUIRE(deriv_param < 0 || G.form == PYHIP_FORM_BRATU || G.form == kFormFlux,
                       "parameter derivative only defined for the Bratu form");
     int rc;
     if (precision == PYHIP_F64) {
@@ -369,7 +421,7 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.conn = G.conn; a.seqn = G.seqn; a.sub = G.sub;
             a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
             a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
-            a.tab = p->tab64[G.kind];
+            a.tab = G.form == kFormFlux ? nullptr : p->tab64[G.kind];
             a.Er = p->Er64 + G.er_off;
             a.Ej = flag ? p->Ej64 + G.ej_off : nullptr;
             a.flag = flag; a.deriv = deriv_param >= 0;
@@ -378,14 +430,17 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.hist1 = p->hist1; a.hist2 = p->hist2;
             a.hvec = p->hvec;
             TimeScope ts(&p->timer, "element", p->st);
-            if ((rc = launch_elements_f64(G.kind, G.form, a, p->st))) return rc;
+            if (G.form == kFormFlux) {
+                if ((rc = launch_faces_f64(G.kind, a, p->st))) return rc;
+            } else if ((rc = launch_elements_f64(G.kind, G.form, a, p->st)))
+                return rc;
         } else {
             ElemArgs<float> a{};
             a.nel = (int)G.nel;
             a.conn = G.conn; a.seqn = G.seqn; a.sub = G.sub;
             a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
             a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
-            a.tab = p->tab32[G.kind];
+            a.tab = G.form == kFormFlux ? nullptr : p->tab32[G.kind];
             a.Er = p->Er32 + G.er_off;
             a.Ej = flag ? p->Ej32 + G.ej_off : nullptr;
             a.flag = flag; a.deriv = deriv_param >= 0;
@@ -394,7 +449,10 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.hist1 = p->hist1; a.hist2 = p->hist2;
             a.hvec = p->hvec;
             TimeScope ts(&p->timer, "element", p->st);
-            if ((rc = launch_elements_f32(G.kind, G.form, a, p->st))) return rc;
+            if (G.form == kFormFlux) {
+                if ((rc = launch_faces_f32(G.kind, a, p->st))) return rc;
+            } else if ((rc = launch_elements_f32(G.kind, G.form, a, p->st)))
+                return rc;
         }
     }
     const GatherMap m = p->gmap();

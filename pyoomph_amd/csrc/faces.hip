@@ -1,0 +1,126 @@
+// Boundary-face (interface) elements of the scalar forms: the flux term of pyoomph's
+// NeumannBC(u=flux) @ boundary (pyoomph/meshes/bcs.py:71-89: residual weak(flux, test)),
+// flux = alpha*u + beta, on the faces of P1 / P2 triangles.
+//
+// A face element is the reference's InterfaceTElementLine1dC1 / C2 (src/mesh.cpp:627-635):
+// a TElement<1,2> / TElement<1,3> line (nodes at s = 0, 1 or 0, 1/2, 1; shapes
+// Telements.h TElementShape<1,2>, <1,3>) with its default TGauss<1,2> / TGauss<1,3> scheme
+// (integral.cc:303-316; the functable's integration order is 0, so the element default
+// stays, src/mesh.cpp:664-667). Per point: tangent t = sum_l x_l dpsi_l/ds,
+// W = w |t|, u = sum_l u_l psi_l;
+//   R_i += W * (flux * psi_i),   J_ij += W * ((alpha * psi_j) * psi_i).
+// The flux does not depend on lambda, on du/dt or on second derivatives: dR/dlambda, the
+// mass matrix and the Hessian-vector product of a face are zero. One thread per face;
+// the element vector / matrix layout is the bulk kernels' ([face][i], [face][i][j]).
+#include <cmath>
+
+#include "element_kernels.h"
+#include "problem.h"
+
+namespace pyhip {
+
+template <int NN> struct FaceRule;
+template <> struct FaceRule<2> {
+    __device__ static void point(int g, double &s, double &w) {
+        s = g == 0 ? 0.5 * (-0.577350269189626 + 1.0) : 0.5 * (0.577350269189626 + 1.0);
+        w = 0.5;
+    }
+    template <typename T> __device__ static void shape(T s, T (&psi)[2], T (&dpsi)[2]) {
+        psi[0] = T(1.0) - s;
+        psi[1] = s;
+        dpsi[0] = T(-1.0);
+        dpsi[1] = T(1.0);
+    }
+};
+template <> struct FaceRule<3> {
+    __device__ static void point(int g, double &s, double &w) {
+        s = g == 0 ? 0.5 * (-0.774596669241483 + 1.0) : (g == 1 ? 0.5 : 0.5 * (0.774596669241483 + 1.0));
+        w = g == 1 ? (8.0 / 18.0) : (5.0 / 18.0);
+    }
+    template <typename T> __device__ static void shape(T s, T (&psi)[3], T (&dpsi)[3]) {
+        psi[0] = T(2.0) * (s - T(1.0)) * (s - T(0.5));
+        psi[1] = T(4.0) * (T(1.0) - s) * s;
+        psi[2] = T(2.0) * (s - T(0.5)) * s;
+        dpsi[0] = T(4.0) * s - T(3.0);
+        dpsi[1] = T(4.0) - T(8.0) * s;
+        dpsi[2] = T(4.0) * s - T(1.0);
+    }
+};
+
+template <typename T, int NN>
+__global__ __launch_bounds__(256) void flux_face_kernel(ElemArgs<T> a) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= a.nel) return;
+    const int s = a.sub[e];
+    const T alpha = (T)a.params[(int64_t)s * a.pstride + kFaceAlpha];
+    const T beta = (T)a.params[(int64_t)s * a.pstride + kFaceBeta];
+    T X[NN][2], U[NN];
+#pragma unroll
+    for (int l = 0; l < NN; l++) {
+        const int32_t n = a.conn[(int64_t)l * a.nel + e];
+        X[l][0] = (T)a.coords[(int64_t)n * 3];
+        X[l][1] = (T)a.coords[(int64_t)n * 3 + 1];
+        const int32_t q = a.seqn[(int64_t)l * a.nel + e];
+        U[l] = (T)(q >= 0 ? a.dofs[q] : a.pinned[(int64_t)n * a.nvalue]);
+    }
+    T R[NN], J[NN][NN];
+#pragma unroll
+    for (int i = 0; i < NN; i++) {
+        R[i] = T(0);
+#pragma unroll
+        for (int j = 0; j < NN; j++) J[i][j] = T(0);
+    }
+#pragma unroll
+    for (int g = 0; g < NN; g++) {
+        double sd, wd;
+        FaceRule<NN>::point(g, sd, wd);
+        const T sq = (T)sd, wq = (T)wd;
+        T psi[NN], dpsi[NN];
+        FaceRule<NN>::shape(sq, psi, dpsi);
+        T tx = T(0), ty = T(0), u = T(0);
+#pragma unroll
+        for (int l = 0; l < NN; l++) {
+            tx = tx + X[l][0] * dpsi[l];
+            ty = ty + X[l][1] * dpsi[l];
+            u = u + U[l] * psi[l];
+        }
+        const T W = wq * sqrt(tx * tx + ty * ty);
+        const T flux = alpha * u + beta;
+#pragma unroll
+        for (int i = 0; i < NN; i++) {
+            R[i] = R[i] + W * (flux * psi[i]);
+#pragma unroll
+            for (int j = 0; j < NN; j++) J[i][j] = J[i][j] + W * ((alpha * psi[j]) * psi[i]);
+        }
+    }
+    // dR/dlambda, the mass matrix and the Hessian-vector product of a face vanish
+    const bool zero_r = a.deriv != 0;
+    const bool zero_j = a.deriv != 0 || a.flag == 2 || a.hvec != nullptr;
+#pragma unroll
+    for (int i = 0; i < NN; i++) {
+        a.Er[(int64_t)e * NN + i] = zero_r ? T(0) : R[i];
+        if (a.Ej) {
+#pragma unroll
+            for (int j = 0; j < NN; j++) a.Ej[((int64_t)e * NN + i) * NN + j] = zero_j ? T(0) : J[i][j];
+        }
+    }
+}
+
+template <typename T>
+static int launch_faces(int kind, const ElemArgs<T> &a, hipStream_t st) {
+    if (a.nel <= 0) return PYHIP_OK;
+    const dim3 g((a.nel + 255) / 256), t(256);
+    if (kind == kKindLine2) hipLaunchKernelGGL((flux_face_kernel<T, 2>), g, t, 0, st, a);
+    else if (kind == kKindLine3) hipLaunchKernelGGL((flux_face_kernel<T, 3>), g, t, 0, st, a);
+    else {
+        set_error("unknown face element kind");
+        return PYHIP_ERR_INVALID;
+    }
+    PYHIP_CHECK_HIP(hipGetLastError());
+    return PYHIP_OK;
+}
+
+int launch_faces_f64(int kind, const ElemArgs<double> &a, hipStream_t st) { return launch_faces<double>(kind, a, st); }
+int launch_faces_f32(int kind, const ElemArgs<float> &a, hipStream_t st) { return launch_faces<float>(kind, a, st); }
+
+}  // namespace pyhip

@@ -1767,7 +1767,8 @@ int pyhip_arclength_step(pyhip_problem *p, double *x, int param, const pyhip_new
     PYHIP_REQUIRE(o->jacobian_precision == PYHIP_F64 && o->linear.precision == PYHIP_F64,
                   "arclength continuation runs FP64 solves");
     for (auto &G : p->groups)
-        PYHIP_REQUIRE(G.form == PYHIP_FORM_BRATU, "arclength continuation in lambda needs Bratu sub-problems");
+        PYHIP_REQUIRE(G.form == PYHIP_FORM_BRATU || G.form == kFormFlux,
+                      "arclength continuation in lambda needs Bratu sub-problems");
     int rc;
     if ((rc = krylov_setup(p, o->linear.restart))) return rc;
     KrylovWork *W = p->kw;

@@ -13,12 +13,21 @@ namespace pyhip {
 
 constexpr int kParamStride = 64;  // doubles of parameters per sub-problem
 
+// Boundary-face (interface) elements: internal form and element kinds of their groups;
+// their flux coefficients sit at the end of the sub-problem's parameter block.
+constexpr int kFormFlux = 16;          // flux = alpha*u + beta on the faces
+constexpr int kKindLine2 = 16;         // 2-node faces of P1 triangles (TGauss<1,2>)
+constexpr int kKindLine3 = 17;         // 3-node faces of P2 triangles (TGauss<1,3>)
+constexpr int kFaceAlpha = kParamStride - 2, kFaceBeta = kParamStride - 1;
+
 struct SubInfo {
     int form, kind, nvalue;
     int64_t dof_off, ndof, node_off, nnode;
     int group;               // element group holding its elements
     int64_t el_off;          // first element inside that group
     int64_t nel;
+    int fgroup = -1;         // face group (-1: no faces)
+    int64_t fel_off = 0, nface = 0;
 };
 
 struct Group {
@@ -98,6 +107,8 @@ struct TimeScope {
 
 int launch_elements_f64(int kind, int form, const ElemArgs<double> &a, hipStream_t s);
 int launch_elements_f32(int kind, int form, const ElemArgs<float> &a, hipStream_t s);
+int launch_faces_f64(int kind, const ElemArgs<double> &a, hipStream_t s);
+int launch_faces_f32(int kind, const ElemArgs<float> &a, hipStream_t s);
 int launch_gather_values_f64(const GatherMap &m, const double *Ej, const double *Er, double *vals,
                              double *res, hipStream_t st);
 int launch_gather_values_f32(const GatherMap &m, const float *Ej, const float *Er, float *vals,

@@ -16,7 +16,7 @@ search. A node keeps the coordinates computed by its first request (vertices
 """
 import numpy as np
 
-__all__ = ["StructuredMesh", "RectangularQuadMesh", "CuboidBrickMesh"]
+__all__ = ["StructuredMesh", "RectangularQuadMesh", "CuboidBrickMesh", "boundary_faces"]
 
 
 class StructuredMesh:
@@ -64,6 +64,36 @@ class StructuredMesh:
         for n in names:
             m |= self.boundary_nodes(n)
         return m
+
+
+# triangle edges in local nodes: ends and (P2) midside node, in face-node order
+_TRI_EDGES = {"tri_p1": ((0, 1), (1, 2), (2, 0)), "tri_p2": ((0, 3, 1), (1, 4, 2), (2, 5, 0))}
+
+
+def boundary_faces(mesh, names):
+    """Nodes of the bulk faces on the named boundaries, for the interface elements that
+    carry a boundary flux (the reference builds one interface element per bulk face on
+    the boundary, src/mesh.cpp:615-660): int32 [nface, 2] (P1 triangles: end, end) or
+    [nface, 3] (P2: end, midside, end); one boundary after the other, bulk elements in
+    element order, edges 0-1, 1-2, 2-0 of each."""
+    if mesh.kind not in _TRI_EDGES:
+        raise ValueError("boundary-face fluxes are built for P1 / P2 triangle meshes")
+    E = mesh.elements
+    edges = _TRI_EDGES[mesh.kind]
+    out = []
+    for name in names:
+        on = mesh.boundary_nodes(name)
+        sel = np.stack([np.all(on[E[:, list(ed)]], axis=1) for ed in edges], axis=1)   # [nel, 3]
+        el, k = np.nonzero(sel)                                                        # element order, edge order
+        f = np.stack([E[el, ed] for ed in np.array(edges)[k].T], axis=1) if len(el) else \
+            np.zeros((0, len(edges[0])), dtype=np.int64)
+        out.append(f)
+    faces = np.ascontiguousarray(np.concatenate(out), dtype=np.int32)
+    if faces.shape[1] == 3 and len(faces):
+        c = mesh.coords
+        if not np.allclose(c[faces[:, 1]], 0.5 * (c[faces[:, 0]] + c[faces[:, 2]]), rtol=0, atol=1e-12):
+            raise ValueError("midside node of a boundary face is not at the edge midpoint")
+    return faces
 
 
 def _number(keys, coords):

@@ -40,9 +40,12 @@ class MeshProblem:
     dirichlet: list of (boundary names, {value_index: value}) applied in order (a node on
     several boundaries keeps the value of the last condition that names it), as
     successive ``DirichletBC(...) @ boundary`` do in pyoomph.
+    flux: (boundary names, alpha, beta) -- ``NeumannBC(u=alpha*u + beta) @ boundary`` for
+    each name (pyoomph/meshes/bcs.py:71-89: the residual gains the face integral of
+    (alpha*u + beta) * test), on P1 / P2 triangle meshes of the scalar forms; None: none.
     """
 
-    def __init__(self, mesh, equations, dirichlet=(), pin_extra=()):
+    def __init__(self, mesh, equations, dirichlet=(), pin_extra=(), flux=None):
         if not isinstance(equations, Equations):
             raise TypeError("equations must be an Equations instance")
         self.mesh = mesh
@@ -69,6 +72,14 @@ class MeshProblem:
         eqn[free] = np.arange(int(free.sum()))
         self.eqn = eqn.reshape(mesh.nnode, nv).astype(np.int32)
         self.ndof = int(free.sum())
+        self.faces, self.face_alpha, self.face_beta = None, 0.0, 0.0
+        if flux is not None:
+            from .meshes import boundary_faces
+            names, alpha, beta = flux
+            if nv != 1:
+                raise ValueError("boundary-face fluxes are built for the scalar forms")
+            self.faces = boundary_faces(mesh, list(names))
+            self.face_alpha, self.face_beta = float(alpha), float(beta)
 
     def nodal_values(self, dofs):
         U = self.values.copy()
@@ -143,6 +154,12 @@ class Problem:
             d.params = par.ctypes.data_as(_lib.c_dp) if par.size else None
             d.nparams = int(par.size)
             d.aux = aux.ctypes.data_as(_lib.c_dp) if aux is not None else None
+            if mp.faces is not None and len(mp.faces):
+                fc = np.ascontiguousarray(mp.faces, dtype=np.int32)
+                keep.append(fc)
+                d.nface = fc.shape[0]
+                d.face_conn = fc.ctypes.data_as(_lib.c_i32p)
+                d.face_alpha, d.face_beta = mp.face_alpha, mp.face_beta
         torch.cuda.set_device(self.device)
         # the problem runs on the stream current at creation; every host wait below is on
         # this stream only, so problems on other streams keep running

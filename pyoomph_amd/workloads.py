@@ -84,6 +84,15 @@ def bratu(N, space, lam):
     return MeshProblem(mesh, eq, dirichlet=[(ALL2D, {0: 0.0})])
 
 
+def bratu_flux(N, space, lam, alpha, beta, flux_sides=("right", "top"), fixed_sides=("left", "bottom")):
+    """Bratu with u = 0 on ``fixed_sides`` and the boundary flux alpha*u + beta
+    (``NeumannBC(u=alpha*var("u") + beta) @ side``) on ``flux_sides``."""
+    mesh = RectangularQuadMesh(N, split_in_tris="left", space=space)
+    eq = BratuEquation("u", space=space, lambda_=lam)
+    return MeshProblem(mesh, eq, dirichlet=[(tuple(fixed_sides), {0: 0.0})],
+                       flux=(tuple(flux_sides), alpha, beta))
+
+
 def bratu_sweep_specs(nprob=512, seed=0, nmin=16, nmax=128):
     rng = np.random.default_rng(seed)
     specs = []

@@ -1,0 +1,106 @@
+"""GPU parity of the boundary-face flux elements (NeumannBC(u=alpha*u + beta) on the
+faces of P1 / P2 triangles) against the oracle: CSR pattern bit-exact, values and
+residuals within 1e-12, Newton solutions within 1e-8, the exact linear solution of a
+Laplace problem with a flux side, and batch = alone bitwise."""
+import numpy as np
+import pytest
+
+from oracle import assembly as oa
+from oracle import faces as of
+from oracle import meshes as om
+from oracle import newton as on
+from oracle import timestepping as ots
+
+pytestmark = pytest.mark.gpu
+
+FLUX, FIXED = ("right", "top"), ("left", "bottom")
+
+
+def build(N, space, lam, alpha, beta):
+    from pyoomph_amd import workloads as wl
+    from pyoomph_amd.problem import Problem
+    mp = wl.bratu_flux(N, space, lam, alpha, beta, FLUX, FIXED)
+    m = om.rectangular_quad_mesh(N, "left", space)
+    op = oa.OracleProblem(m, oa.FORM_BRATU, 1, m.on_boundary(set(FIXED))[:, None], np.zeros((m.nnode, 1)),
+                          [lam], faces=of.boundary_faces(m, list(FLUX)), face_alpha=alpha, face_beta=beta)
+    assert np.array_equal(mp.eqn, op.eqn) and np.array_equal(mp.faces, op.faces)
+    return Problem(mp), op
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("space", ["C1", "C2"])
+@pytest.mark.parametrize("alpha,beta", [(0.0, 0.6), (1.3, -0.4)])
+def test_flux_assembly_parity(gpu, space, alpha, beta):
+    P, op = build(7, space, 1.2, alpha, beta)
+    for seed in (1, 2):
+        x = np.random.default_rng(seed).uniform(-0.4, 0.4, P.ndof)
+        P.set_current_dofs(x)
+        R, J = P.assemble_jacobian()
+        r, rs, col, val = op.get_jacobian(x)
+        assert np.array_equal(J.indptr, rs) and np.array_equal(J.indices, col)
+        assert rel(J.data, val) <= 1e-12 and rel(R, r) <= 1e-12
+        assert np.array_equal(P.get_residuals(), R)
+        d = P.get_residuals(dparameter="lambda")
+        assert rel(d, op.get_residuals(x, deriv=1)) <= 1e-12
+
+
+def test_flux_mass_matrix_has_no_face_term(gpu):
+    P, op = build(6, "C2", 1.0, 2.0, 0.5)
+    x = np.random.default_rng(3).uniform(-0.2, 0.2, P.ndof)
+    P.set_current_dofs(x)
+    M = P.assemble_mass_matrix()
+    op.set_time(ots.bdf1_weights(1.0), x, x)
+    _, rs, col, val = op.get_mass_matrix(x)
+    op.set_time(None, None, None)
+    assert np.array_equal(M.indptr, rs) and np.array_equal(M.indices, col)
+    assert np.array_equal(M.data, val)
+
+
+@pytest.mark.parametrize("space", ["C1", "C2"])
+@pytest.mark.parametrize("alpha,beta", [(0.0, 0.7), (2.0, 0.7)])
+def test_laplace_with_flux_side_has_linear_solution(gpu, space, alpha, beta):
+    """u = 0 on the left, flux on the right only: u = -beta / (1 + alpha) * x."""
+    from pyoomph_amd import workloads as wl
+    from pyoomph_amd.problem import Problem
+    mp = wl.bratu_flux(9, space, 0.0, alpha, beta, ("right",), ("left",))
+    P = Problem(mp)
+    P.krylov, P.preconditioner = "cg", "jacobi"
+    P.newton_solver_tolerance = 1e-12
+    st = P.solve()
+    assert st[0]["converged"]
+    u = mp.nodal_values(P.get_current_dofs())[:, 0]
+    c = -beta / (1.0 + alpha)
+    assert np.max(np.abs(u - c * mp.mesh.coords[:, 0])) <= 1e-10
+
+
+@pytest.mark.parametrize("opts", [dict(krylov="cg", preconditioner="jacobi"),
+                                  dict(krylov="cg", preconditioner="amg"),
+                                  dict(krylov="gmres", preconditioner="ilu0")])
+@pytest.mark.parametrize("space", ["C1", "C2"])
+def test_flux_newton_matches_oracle(gpu, opts, space):
+    P, op = build(14, space, 1.8, 0.8, 0.3)
+    for k, v in opts.items():
+        setattr(P, k, v)
+    st = P.solve()
+    assert st[0]["converged"] and st[0]["linear_failures"] == 0
+    xo = on.newton_solve(op, np.zeros(op.ndof), tol=1e-8)
+    assert rel(P.get_current_dofs(), xo) <= 1e-8
+
+
+def test_flux_batch_equals_alone_bitwise(gpu):
+    from pyoomph_amd import workloads as wl
+    from pyoomph_amd.problem import Problem
+    parts = [lambda: wl.bratu_flux(11, "C2", 1.1, 0.5, 0.2), lambda: wl.bratu(9, "C1", 2.0),
+             lambda: wl.bratu_flux(13, "C1", 0.6, 0.0, -0.3), lambda: wl.bratu_flux(6, "C2", 2.2, 3.0, 0.0)]
+    batch = Problem([f() for f in parts])
+    batch.krylov, batch.preconditioner = "cg", "amg"
+    batch.solve()
+    xb = batch.get_current_dofs()
+    for i, f in enumerate(parts):
+        alone = Problem(f())
+        alone.krylov, alone.preconditioner = "cg", "amg"
+        alone.solve()
+        assert np.array_equal(alone.get_current_dofs(), batch.sub_dofs(i, xb)), i

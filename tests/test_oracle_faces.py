@@ -1,0 +1,113 @@
+"""Boundary-face flux elements of the oracle (NeumannBC(u=alpha*u + beta)), pinned by
+closed forms: edge integrals, the 1D mass matrices, the Jacobian as the derivative of the
+residual, and a Laplace problem with a Neumann / Robin side whose exact solution is
+linear (reproduced exactly by P1 and P2)."""
+import numpy as np
+import pytest
+
+from oracle import assembly as oa
+from oracle import faces as of
+from oracle import meshes as om
+from oracle import newton as on
+
+
+def _edge(nn, a, b):
+    """one straight face from a to b (P2: midside node at the midpoint)"""
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    pts = [a, b] if nn == 2 else [a, 0.5 * (a + b), b]
+    return np.array([pts])
+
+
+@pytest.mark.parametrize("nn", [2, 3])
+def test_edge_load_and_mass_closed_forms(nn):
+    X = _edge(nn, (0.3, -0.2), (1.1, 0.4))
+    L = np.hypot(0.8, 0.6)
+    R, J = of.face_outputs(X, np.zeros((1, nn)), 0.0, 1.0, 1)
+    load = [0.5, 0.5] if nn == 2 else [1 / 6, 2 / 3, 1 / 6]
+    assert np.allclose(R[0], L * np.array(load), rtol=1e-13, atol=0)
+    assert np.all(J == 0.0)
+    _, M = of.face_outputs(X, np.zeros((1, nn)), 1.0, 0.0, 1)
+    mass = np.array([[2, 1], [1, 2]]) / 6 if nn == 2 else np.array([[4, 2, -1], [2, 16, 2], [-1, 2, 4]]) / 30
+    assert np.allclose(M[0], L * mass, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("nn", [2, 3])
+def test_robin_residual_is_linear_and_jacobian_its_derivative(nn):
+    rng = np.random.default_rng(5)
+    X = _edge(nn, (0.0, 0.0), (0.25, 0.1))
+    U = rng.standard_normal((1, nn))
+    alpha, beta = 1.7, -0.4
+    R, J = of.face_outputs(X, U, alpha, beta, 1)
+    R0, _ = of.face_outputs(X, np.zeros((1, nn)), alpha, beta, 0)
+    assert np.allclose(R[0], J[0] @ U[0] + R0[0], rtol=1e-13, atol=1e-15)
+    h = 1e-6
+    for j in range(nn):
+        Up, Um = U.copy(), U.copy()
+        Up[0, j] += h
+        Um[0, j] -= h
+        fd = (of.face_outputs(X, Up, alpha, beta, 0)[0] - of.face_outputs(X, Um, alpha, beta, 0)[0]) / (2 * h)
+        assert np.allclose(fd[0], J[0][:, j], rtol=1e-7, atol=1e-10)
+
+
+@pytest.mark.parametrize("space", ["C1", "C2"])
+def test_boundary_faces_cover_each_side_once(space):
+    N = 5
+    m = om.rectangular_quad_mesh(N, "left", space)
+    for side in ("left", "right", "bottom", "top"):
+        F = of.boundary_faces(m, [side])
+        assert F.shape == (N, 2 if space == "C1" else 3)
+        ends = m.coords[F[:, [0, -1]]]
+        L = np.linalg.norm(ends[:, 1] - ends[:, 0], axis=1)
+        assert np.isclose(L.sum(), 1.0, rtol=1e-14)
+        on = m.on_boundary([side])
+        assert np.all(on[F])
+
+
+def test_product_faces_equal_oracle_faces():
+    from pyoomph_amd import meshes as pm
+    for space, kind in (("C1", "tri_p1"), ("C2", "tri_p2")):
+        m = om.rectangular_quad_mesh(6, "left", space)
+        pmesh = pm.RectangularQuadMesh(6, "left", space)
+        assert pmesh.kind == kind and np.array_equal(pmesh.elements, m.elements)
+        for names in (["right"], ["top", "left"]):
+            assert np.array_equal(pm.boundary_faces(pmesh, names), of.boundary_faces(m, names))
+
+
+@pytest.mark.parametrize("space", ["C1", "C2"])
+@pytest.mark.parametrize("alpha,beta", [(0.0, 0.7), (2.0, 0.7)])
+def test_laplace_with_flux_side_has_linear_solution(space, alpha, beta):
+    """-lap u = 0 (Bratu with lambda = 0), u = 0 on the left, flux alpha*u + beta on the
+    right (NeumannBC: du/dn = -(alpha*u + beta)), natural top and bottom: u = c x with
+    c = -beta / (1 + alpha), which P1 and P2 reproduce."""
+    m = om.rectangular_quad_mesh(4, "left", space)
+    pinned = m.on_boundary({"left"})[:, None]
+    F = of.boundary_faces(m, ["right"])
+    op = oa.OracleProblem(m, oa.FORM_BRATU, 1, pinned, np.zeros((m.nnode, 1)), [0.0], faces=F,
+                          face_alpha=alpha, face_beta=beta)
+    x = on.newton_solve(op, np.zeros(op.ndof), tol=1e-12)
+    u = op.nodal_values(x)[:, 0]
+    c = -beta / (1.0 + alpha)
+    assert np.allclose(u, c * m.coords[:, 0], rtol=0, atol=1e-12)
+
+
+def test_face_contributions_follow_the_bulk_in_the_csr():
+    """Face couplings lie inside their bulk element's couplings: the pattern is the bulk
+    pattern, and the values gain the face integrals (Robin: the face mass matrix)."""
+    m = om.rectangular_quad_mesh(3, "left", "C2")
+    pinned = m.on_boundary({"left"})[:, None]
+    F = of.boundary_faces(m, ["right", "top"])
+    kw = dict(mesh=m, form=oa.FORM_BRATU, nvalue=1, pinned=pinned, bc=np.zeros((m.nnode, 1)), params=[1.0])
+    bulk = oa.OracleProblem(**kw)
+    face = oa.OracleProblem(**kw, faces=F, face_alpha=3.0, face_beta=0.0)
+    x = np.random.default_rng(2).standard_normal(bulk.ndof) * 0.1
+    rb, rsb, cb, vb = bulk.get_jacobian(x)
+    rf, rsf, cf, vf = face.get_jacobian(x)
+    assert np.array_equal(rsb, rsf) and np.array_equal(cb, cf)
+    # the difference is alpha times the assembled face mass matrix, and R changes by J_face x
+    import scipy.sparse as sp
+    n = bulk.ndof
+    Jb = sp.csr_matrix((vb, cb, rsb), shape=(n, n)).toarray()
+    Jf = sp.csr_matrix((vf, cf, rsf), shape=(n, n)).toarray()
+    D = Jf - Jb
+    assert np.allclose(D, D.T, atol=1e-15) and np.all(np.diag(D) >= 0) and D.sum() > 0
+    assert np.allclose(rf - rb, D @ x, rtol=1e-12, atol=1e-15)
