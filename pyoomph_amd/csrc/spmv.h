@@ -159,10 +159,11 @@ __global__ __launch_bounds__(256) void spmv_lds_kernel(int64_t n, const int32_t 
     TY acc[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) acc[j] = TY(0);
-    for (int32_t cs = kb0; cs < kb1; cs += CAP) {
+    // values and columns of the next chunk are loaded while the current one is summed
+    int32_t cc[U];
+    TY vv[U];
+    auto load_chunk = [&](int32_t cs) {
         const int32_t ce = min(cs + CAP, kb1);
-        int32_t cc[U];
-        TY vv[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int32_t k = cs + u * 256 + (int32_t)threadIdx.x;
@@ -170,6 +171,10 @@ __global__ __launch_bounds__(256) void spmv_lds_kernel(int64_t n, const int32_t 
             cc[u] = in ? col[k] : -1;
             vv[u] = in ? TY(val[k]) : TY(0);
         }
+    };
+    if (kb0 < kb1) load_chunk(kb0);
+    for (int32_t cs = kb0; cs < kb1; cs += CAP) {
+        const int32_t ce = min(cs + CAP, kb1);
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (cc[u] >= 0) {
@@ -179,6 +184,7 @@ __global__ __launch_bounds__(256) void spmv_lds_kernel(int64_t n, const int32_t 
             }
         }
         __syncthreads();
+        if (cs + CAP < kb1) load_chunk(cs + CAP);
         const int32_t lo = max(k0, cs), hi = min(k1, ce);
         if (lo < hi) {
             for (int32_t g = k0 + ((lo - k0) & ~7); g < hi; g += 8) {
