@@ -242,16 +242,6 @@ __global__ __launch_bounds__(256) void amg_wb_kernel(int64_t n, const double *__
     x[i] = w[i] * b[i];
 }
 
-// out = x + W r
-__global__ __launch_bounds__(256) void amg_update_kernel(int64_t n, const double *__restrict__ w,
-                                                         const double *__restrict__ r, const double *x,
-                                                         double *out, const int32_t *__restrict__ rsub,
-                                                         const int32_t *__restrict__ act) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n || (act && !act[rsub[i]])) return;
-    out[i] = x[i] + w[i] * r[i];
-}
-
 // Dense Cholesky factor of the listed sub-problems' matrices (<= kBottomMaxRows rows),
 // one workgroup per sub-problem, right-looking, fixed order. A non-positive pivot is
 // replaced by 1 and flagged.
@@ -434,20 +424,39 @@ int amg_build(AmgDev &A, int64_t n, const int32_t *rs_h, const int32_t *col_h, c
 // lanes per row of the canonical spmv for a matrix with this mean row length
 static inline int lanes_for(int64_t nnz, int64_t n) { return (n > 0 && nnz <= 12 * n) ? 4 : 8; }
 
-// One level spmv, y = A x, b - A x (SUB) or b + A x (ADD), timed as an "spmv" launch
-// with its algorithmic bytes (values and column indices, row starts, x, y, b once each).
-enum { kSpmvPlain, kSpmvSub, kSpmvAdd };
+// One level spmv, timed as an "spmv" launch with its algorithmic bytes (values and column
+// indices, row starts, the input entries and every row-indexed vector read or written, once
+// each). Modes: y = A x (PLAIN), y = b - A x (SUB), y = b + A x (ADD), the weighted-Jacobi
+// step out = x + W (b - A x) (JACOBI), the restriction y = A x with the next level's
+// out = W y (SCALE).
+enum { kSpmvPlain, kSpmvSub, kSpmvAdd, kSpmvJacobi, kSpmvScale };
 // Rows filtered by (rsub, act): the bytes count the rows, entries and input entries of the
 // visited sub-problems (nnz, rows, ncols are those counts).
 static void level_spmv(KernelTimer *tm, int mode, int lpr, hipStream_t st, int64_t n, int64_t nnz, int64_t rows,
                        int64_t ncols, const int32_t *rs, const int32_t *col, const double *vals, const double *x,
-                       double *y, const double *b, const int32_t *rsub = nullptr, const int32_t *act = nullptr) {
+                       double *y, const double *b, const int32_t *rsub = nullptr, const int32_t *act = nullptr,
+                       const double *w = nullptr, double *out = nullptr) {
     TimeScope ts(tm, "spmv", st);
-    if (mode == kSpmvAdd) launch_spmv_add<double, double, double>(lpr, st, n, rs, col, vals, x, y, b, rsub, act);
-    else
-        launch_spmv<double, double, double>(lpr, st, n, rs, col, vals, x, y, mode == kSpmvSub ? b : nullptr, rsub,
-                                            act);
-    if (tm) tm->add_work("spmv", 12 * nnz + 4 * (rows + 1) + 8 * ncols + 8 * rows + (mode != kSpmvPlain ? 8 * rows : 0));
+    int64_t vec = 8 * rows;   // y
+    switch (mode) {
+    case kSpmvAdd:
+        launch_spmv_epi<double, double, double, kEpiAdd>(lpr, st, n, rs, col, vals, x, y, b, w, out, rsub, act);
+        vec = 16 * rows;
+        break;
+    case kSpmvJacobi:
+        launch_spmv_epi<double, double, double, kEpiJacobi>(lpr, st, n, rs, col, vals, x, y, b, w, out, rsub, act);
+        vec = 32 * rows;   // b, w, own x, out
+        break;
+    case kSpmvScale:
+        launch_spmv_epi<double, double, double, kEpiScale>(lpr, st, n, rs, col, vals, x, y, b, w, out, rsub, act);
+        vec = 24 * rows;   // y, w, out
+        break;
+    default:
+        launch_spmv_epi<double, double, double, kEpiPlain>(lpr, st, n, rs, col, vals, x, y,
+                                                           mode == kSpmvSub ? b : nullptr, w, out, rsub, act);
+        if (mode == kSpmvSub) vec = 16 * rows;
+    }
+    if (tm) tm->add_work("spmv", 12 * nnz + 4 * (rows + 1) + 8 * ncols + vec);
 }
 
 // sum of v[s] over the sub-problems with act_h[s] != 0 (all without act_h)
@@ -605,11 +614,17 @@ int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer
         AmgDevLevel &L = A.lv[l];
         const double *bl = l == 0 ? b : L.b;
         AmgDevLevel &C = (l + 1 < nl) ? A.lv[l + 1] : A.bot;
-        hipLaunchKernelGGL(amg_wb_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.w, bl, L.x, L.row_sub, act);
+        // x = W b (on coarser levels the restriction above has written it)
+        if (l == 0)
+            hipLaunchKernelGGL(amg_wb_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.w, bl, L.x, L.row_sub, act);
         const int la = L.lpr_a ? L.lpr_a : lanes_for(L.nnz, L.n);
         level_spmv(tm, kSpmvSub, la, st, L.n, nnz[l], rows[l], rows[l], L.rs, L.col, L.vals, L.x, L.r, bl,
                    L.row_sub, act);
-        if (L.nc)
+        // b_c = P^T r, and x_c = W_c b_c where the coarse level is not the coarsest
+        if (L.nc && l + 1 < nl)
+            level_spmv(tm, kSpmvScale, L.lpr_t ? L.lpr_t : 8, st, L.nc, nnzp[l], rows[l + 1], rows[l], L.trs, L.trow,
+                       L.ptv, L.r, C.b, nullptr, C.row_sub, act, C.w, C.x);
+        else if (L.nc)
             level_spmv(tm, kSpmvPlain, L.lpr_t ? L.lpr_t : 8, st, L.nc, nnzp[l], rows[l + 1], rows[l], L.trs, L.trow,
                        L.ptv, L.r, C.b, nullptr, C.row_sub, act);
     }
@@ -625,11 +640,10 @@ int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer
         if (L.nc)
             level_spmv(tm, kSpmvAdd, L.lpr_p ? L.lpr_p : 4, st, L.n, nnzp[l], rows[l], rows[l + 1], L.prs, L.pcol,
                        L.pv, C.x, L.x, L.x, L.row_sub, act);
-        level_spmv(tm, kSpmvSub, L.lpr_a ? L.lpr_a : lanes_for(L.nnz, L.n), st, L.n, nnz[l], rows[l], rows[l], L.rs,
-                   L.col, L.vals, L.x, L.r, bl, L.row_sub, act);
+        // post-sweep out = x + W (b - A x) in the spmv's epilogue
         double *out = l == 0 ? z : L.x2;
-        hipLaunchKernelGGL(amg_update_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.w, L.r, L.x, out, L.row_sub,
-                           act);
+        level_spmv(tm, kSpmvJacobi, L.lpr_a ? L.lpr_a : lanes_for(L.nnz, L.n), st, L.n, nnz[l], rows[l], rows[l],
+                   L.rs, L.col, L.vals, L.x, nullptr, bl, L.row_sub, act, L.w, out);
         if (l > 0) std::swap(L.x, L.x2);   // the level's result is in L.x for the prolongation above
         if (L.nends)
             hipLaunchKernelGGL(amg_chol_solve_kernel, dim3(L.nends), dim3(kCholThreads), 0, st, L.ends, L.sub_off,

@@ -80,14 +80,33 @@ __device__ __forceinline__ void rows_dot(int64_t r0, int64_t r1, const int32_t *
     }
 }
 
-// y = A x (or b - A x); 256 rows per block (rows_dot); the grid is a multiple of 8
-// (spmv_grid). Rows the filter skips keep their y.
-template <typename TV, typename TX, typename TY, int LPR>
+// What a launch writes for row i with row sum v = (A x)_i (every variant):
+//   kEpiPlain   y_i = v, or b_i - v when b is given
+//   kEpiAdd     y_i = b_i + v            (y may alias b)
+//   kEpiJacobi  out_i = x_i + w_i (b_i - v)   (a weighted-Jacobi step; y is not written)
+//   kEpiScale   y_i = v and out_i = w_i v     (a restriction and the next level's x = W b)
+enum { kEpiPlain = 0, kEpiAdd = 1, kEpiJacobi = 2, kEpiScale = 3 };
+
+template <int EPI, typename TX, typename TY>
+__device__ __forceinline__ void spmv_store(int64_t row, TY v, const TX *x, TY *y, const TY *b, const TY *w,
+                                           TY *out) {
+    if constexpr (EPI == kEpiPlain) y[row] = b ? b[row] - v : v;
+    else if constexpr (EPI == kEpiAdd) y[row] = b[row] + v;
+    else if constexpr (EPI == kEpiJacobi) out[row] = TY(x[row]) + w[row] * (b[row] - v);
+    else {
+        y[row] = v;
+        out[row] = w[row] * v;
+    }
+}
+
+// 256 rows per block (rows_dot); the grid is a multiple of 8 (spmv_grid). Rows the filter
+// skips keep their outputs.
+template <typename TV, typename TX, typename TY, int LPR, int EPI>
 __global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__restrict__ rs,
                                                    const int32_t *__restrict__ col,
                                                    const TV *__restrict__ val,
-                                                   const TX *__restrict__ x, TY *__restrict__ y,
-                                                   const TY *__restrict__ b,
+                                                   const TX *__restrict__ x, TY *y, const TY *b,
+                                                   const TY *__restrict__ w, TY *__restrict__ out,
                                                    const int32_t *__restrict__ rsub,
                                                    const int32_t *__restrict__ act) {
     constexpr int RPP = 256 / LPR;
@@ -103,32 +122,7 @@ __global__ __launch_bounds__(256) void spmv_kernel(int64_t n, const int32_t *__r
 #pragma unroll
         for (int p = 0; p < LPR; p++) {
             const int64_t row = r0 + p * RPP + threadIdx.x / LPR;
-            if (live[p]) y[row] = b ? b[row] - v[p] : v[p];
-        }
-    }
-}
-
-// y = b + A x (rows_dot), same grid as spmv_kernel; y may alias b
-template <typename TV, typename TX, typename TY, int LPR>
-__global__ __launch_bounds__(256) void spmv_add_kernel(int64_t n, const int32_t *__restrict__ rs,
-                                                       const int32_t *__restrict__ col,
-                                                       const TV *__restrict__ val,
-                                                       const TX *__restrict__ x, TY *y,
-                                                       const TY *b, const int32_t *__restrict__ rsub,
-                                                       const int32_t *__restrict__ act) {
-    constexpr int RPP = 256 / LPR;
-    const int64_t nb = (n + 255) / 256, per = (nb + 7) / 8, k = blockIdx.x / 8;
-    const int64_t lb = (int64_t)(blockIdx.x % 8) * per + k;
-    if (k >= per || lb >= nb) return;
-    const int64_t r0 = lb * 256;
-    TY v[LPR];
-    bool live[LPR];
-    rows_dot<TV, TX, TY, LPR>(r0, n, rs, col, val, x, rsub, act, v, live);
-    if (threadIdx.x % LPR == 0) {
-#pragma unroll
-        for (int p = 0; p < LPR; p++) {
-            const int64_t row = r0 + p * RPP + threadIdx.x / LPR;
-            if (live[p]) y[row] = b[row] + v[p];
+            if (live[p]) spmv_store<EPI>(row, v[p], x, y, b, w, out);
         }
     }
 }
@@ -138,13 +132,14 @@ __global__ __launch_bounds__(256) void spmv_add_kernel(int64_t n, const int32_t 
 // the value and x[col], both kept in LDS), then one thread per row adds the chunk's entries
 // of its row to its eight stripe sums (entry k0+8g+j to stripe j, in increasing k), so the
 // result has rows_dot's bits. Rows longer than a chunk span several chunks; their stripe
-// sums stay in registers. ADD: y = b + A x; otherwise y = A x, or b - A x when b is given.
-// Rows the filter skips keep their y; a block none of whose rows is live returns at once.
-template <typename TV, typename TX, typename TY, int CAP, bool ADD>
+// sums stay in registers. Outputs as spmv_kernel (EPI). Rows the filter skips keep their
+// outputs; a block none of whose rows is live returns at once.
+template <typename TV, typename TX, typename TY, int CAP, int EPI>
 __global__ __launch_bounds__(256) void spmv_lds_kernel(int64_t n, const int32_t *__restrict__ rs,
                                                        const int32_t *__restrict__ col,
                                                        const TV *__restrict__ val, const TX *__restrict__ x,
-                                                       TY *y, const TY *b, const int32_t *__restrict__ rsub,
+                                                       TY *y, const TY *b, const TY *__restrict__ w,
+                                                       TY *__restrict__ out, const int32_t *__restrict__ rsub,
                                                        const int32_t *__restrict__ act) {
     constexpr int U = (CAP + 255) / 256;
     __shared__ TY sv[CAP], sx[CAP];
@@ -199,7 +194,7 @@ __global__ __launch_bounds__(256) void spmv_lds_kernel(int64_t n, const int32_t 
     }
     if (live) {
         const TY v = ((acc[0] + acc[4]) + (acc[2] + acc[6])) + ((acc[1] + acc[5]) + (acc[3] + acc[7]));
-        y[row] = ADD ? b[row] + v : (b ? b[row] - v : v);
+        spmv_store<EPI>(row, v, x, y, b, w, out);
     }
 }
 
@@ -219,62 +214,44 @@ static inline unsigned spmv_grid(int64_t n) {
     return (unsigned)(((nb + 7) / 8) * 8);
 }
 
+template <typename TV, typename TX, typename TY, int EPI>
+static void launch_spmv_epi(int variant, hipStream_t st, int64_t n, const int32_t *rs, const int32_t *col,
+                            const TV *vals, const TX *x, TY *y, const TY *b, const TY *w, TY *out,
+                            const int32_t *rsub, const int32_t *act) {
+    const dim3 g(spmv_grid(n)), t(256);
+#define PYHIP_SPMV_LANES(L)                                                                                   \
+    hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, L, EPI>), g, t, 0, st, n, rs, col, vals, x, y, b, w, out, rsub, act)
+#define PYHIP_SPMV_LDS(C)                                                                                     \
+    hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, C, EPI>), g, t, 0, st, n, rs, col, vals, x, y, b, w, out, rsub, \
+                       act)
+    switch (variant) {
+    case 1: PYHIP_SPMV_LANES(1); break;
+    case 2: PYHIP_SPMV_LANES(2); break;
+    case 4: PYHIP_SPMV_LANES(4); break;
+    case kSpmvLds512: PYHIP_SPMV_LDS(512); break;
+    case kSpmvLds1K: PYHIP_SPMV_LDS(1024); break;
+    case kSpmvLds2K: PYHIP_SPMV_LDS(2048); break;
+    case kSpmvLds4K: PYHIP_SPMV_LDS(4096); break;
+    default: PYHIP_SPMV_LANES(8); break;
+    }
+#undef PYHIP_SPMV_LANES
+#undef PYHIP_SPMV_LDS
+}
+
+// y = A x, or y = b - A x when b is given
 template <typename TV, typename TX, typename TY>
 static void launch_spmv(int variant, hipStream_t st, int64_t n, const int32_t *rs, const int32_t *col,
                         const TV *vals, const TX *x, TY *y, const TY *b, const int32_t *rsub = nullptr,
                         const int32_t *act = nullptr) {
-    const dim3 g(spmv_grid(n)), t(256);
-    switch (variant) {
-    case 1: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 1>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
-    case 2: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 2>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
-    case 4: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 4>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
-    case kSpmvLds512:
-        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 512, false>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
-                           act);
-        break;
-    case kSpmvLds1K:
-        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 1024, false>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
-                           act);
-        break;
-    case kSpmvLds2K:
-        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 2048, false>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
-                           act);
-        break;
-    case kSpmvLds4K:
-        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 4096, false>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
-                           act);
-        break;
-    default: hipLaunchKernelGGL((spmv_kernel<TV, TX, TY, 8>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
-    }
+    launch_spmv_epi<TV, TX, TY, kEpiPlain>(variant, st, n, rs, col, vals, x, y, b, nullptr, nullptr, rsub, act);
 }
 
+// y = b + A x (y may alias b)
 template <typename TV, typename TX, typename TY>
 static void launch_spmv_add(int variant, hipStream_t st, int64_t n, const int32_t *rs, const int32_t *col,
                             const TV *vals, const TX *x, TY *y, const TY *b, const int32_t *rsub = nullptr,
                             const int32_t *act = nullptr) {
-    const dim3 g(spmv_grid(n)), t(256);
-    switch (variant) {
-    case 1: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 1>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
-    case 2: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 2>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
-    case 4: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 4>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
-    case kSpmvLds512:
-        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 512, true>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
-                           act);
-        break;
-    case kSpmvLds1K:
-        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 1024, true>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
-                           act);
-        break;
-    case kSpmvLds2K:
-        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 2048, true>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
-                           act);
-        break;
-    case kSpmvLds4K:
-        hipLaunchKernelGGL((spmv_lds_kernel<TV, TX, TY, 4096, true>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub,
-                           act);
-        break;
-    default: hipLaunchKernelGGL((spmv_add_kernel<TV, TX, TY, 8>), g, t, 0, st, n, rs, col, vals, x, y, b, rsub, act); break;
-    }
+    launch_spmv_epi<TV, TX, TY, kEpiAdd>(variant, st, n, rs, col, vals, x, y, b, nullptr, nullptr, rsub, act);
 }
 
 }  // namespace pyhip
