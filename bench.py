@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nprob", type=int, default=512)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=16,
+    ap.add_argument("--cpu-sample", type=int, default=64,
                     help="problems in the stratified CPU-baseline sample")
     ap.add_argument("--cpu-seconds", type=float, default=40.0,
                     help="bound on the CPU-baseline sample (seconds of oracle work)")
@@ -519,8 +519,8 @@ def main():
                 errs.append(err_of(i, on.newton_solve(op, np.zeros(op.ndof))))
             print("# accuracy sample: %s" % [(specs[i][0], specs[i][1]) for i in picks], file=sys.stderr,
                   flush=True)
-        # timed sample: a stratified slice of the sweep (every len/16-th problem in order of
-        # unknowns), so its solves/s is that of the sweep's size mix
+        # timed sample: a stratified slice of the sweep (every len/k-th problem in order of
+        # unknowns, k = --cpu-sample), so its solves/s is that of the sweep's size mix
         k = min(args.cpu_sample, len(order))
         strat = [order[int((j + 0.5) * len(order) / k)] for j in range(k)]
         n_cpu, t_cpu = 0, 0.0
