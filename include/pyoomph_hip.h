@@ -90,17 +90,18 @@ typedef struct {
                                     Bratu: lambda; Navier-Stokes: mu, rho               */
     int nparams;
     const double *aux;           /* [nnode] auxiliary nodal field, or NULL              */
-    /* Flux on boundary faces: pyoomph's NeumannBC(u=flux) @ boundary (an
+    /* Fluxes on boundary faces: pyoomph's NeumannBC(u=flux) @ boundary (an
      * InterfaceEquations residual weak(flux, testfunction(u)), pyoomph/meshes/bcs.py:71-89,
      * assembled by interface elements built on the bulk elements' faces,
-     * src/mesh.cpp:615-660) with flux = face_alpha*u + face_beta (Neumann: alpha = 0;
-     * Robin: alpha != 0). Scalar forms on P1 / P2 triangles only (faces: TElement<1,2> /
+     * src/mesh.cpp:615-660) with flux = alpha*u + beta per face (Neumann: alpha = 0;
+     * Robin: alpha != 0), so several conditions on several boundaries of one mesh are
+     * one face list. Scalar forms on P1 / P2 triangles only (faces: TElement<1,2> /
      * TElement<1,3> with TGauss<1,2> / TGauss<1,3>, src/elements.hpp:676-730). The faces
      * follow the bulk elements in element order. nface = 0: none. */
     int64_t nface;
     const int32_t *face_conn;    /* [nface][2] (P1: end, end) or [nface][3] (P2: end,
                                     midside, end) mesh nodes of each face                */
-    double face_alpha, face_beta;
+    const double *face_coef;     /* [nface][2] alpha, beta of each face's flux           */
 } pyhip_subproblem_desc;
 
 typedef struct pyhip_problem pyhip_problem;

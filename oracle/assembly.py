@@ -76,8 +76,9 @@ class OracleProblem:
                  Navier-Stokes: mu, rho)
     aux       -- nodal auxiliary field (reaction-diffusion source f) or None
     faces     -- [nface, 2|3] boundary-face nodes (``faces.boundary_faces``) carrying the
-                 flux face_alpha*u + face_beta (``oracle/faces.py``), or None; the face
-                 elements follow the bulk elements in element order
+                 flux face_alpha*u + face_beta (``oracle/faces.py``; scalars, or one value
+                 per face), or None; the face elements follow the bulk elements in
+                 element order
     """
 
     def __init__(self, mesh, form, nvalue, pinned, bc, params, aux=None, faces=None, face_alpha=0.0,
@@ -111,8 +112,10 @@ class OracleProblem:
         rows = np.nonzero(free)
         self.geqn[rows[0], self.leqn[rows]] = g[rows]
         self.faces = None
-        self.face_alpha, self.face_beta = float(face_alpha), float(face_beta)
         if faces is not None and len(faces):
+            nf = len(faces)
+            self.face_alpha = np.broadcast_to(np.asarray(face_alpha, dtype=np.float64), (nf,)).copy()
+            self.face_beta = np.broadcast_to(np.asarray(face_beta, dtype=np.float64), (nf,)).copy()
             if nvalue != 1:
                 raise ValueError("boundary-face fluxes are built for the scalar forms")
             self.faces = np.asarray(faces, dtype=np.int64)

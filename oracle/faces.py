@@ -66,7 +66,8 @@ def boundary_faces(mesh, names):
 def face_outputs(X, U, alpha, beta, flag):
     """Element residuals [nf, nn] and Jacobians [nf, nn, nn] in face-node numbering.
 
-    X [nf, nn, 2] node coordinates, U [nf, nn] nodal values."""
+    X [nf, nn, 2] node coordinates, U [nf, nn] nodal values, alpha / beta scalars or one
+    value per face."""
     nf, nn = U.shape
     knots, weights = _RULES[nn]
     R = np.zeros((nf, nn))

@@ -112,11 +112,11 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         PYHIP_REQUIRE(d.form != PYHIP_FORM_REACTION_DIFFUSION || d.aux,
                       "reaction-diffusion needs the nodal source field (aux)");
         PYHIP_REQUIRE(d.ndof >= 0, "negative ndof");
-        PYHIP_REQUIRE(d.nface >= 0 && (d.nface == 0 || d.face_conn), "faces without connectivity");
+        PYHIP_REQUIRE(d.nface >= 0 && (d.nface == 0 || (d.face_conn && d.face_coef)),
+                      "faces without connectivity or flux coefficients");
         PYHIP_REQUIRE(d.nface == 0 || ((d.elem_kind == PYHIP_ELEM_TRI_P1 || d.elem_kind == PYHIP_ELEM_TRI_P2) &&
                                        d.nvalue == 1),
                       "boundary-face flux elements: scalar forms on P1 / P2 triangles");
-        PYHIP_REQUIRE(d.nface == 0 || d.nparams <= kFaceAlpha, "too many parameters for a problem with faces");
         std::pair<int, int> k(d.form, d.elem_kind);
         if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
     }
@@ -184,12 +184,14 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
     // host staging: element-major global equations per group, slot-major copies
     std::vector<std::vector<int32_t>> seqn_em(P->groups.size()), conn_sm(P->groups.size()),
         seqn_sm(P->groups.size()), sub_g(P->groups.size());
+    std::vector<std::vector<double>> coef_g(P->groups.size());
     for (size_t g = 0; g < P->groups.size(); g++) {
         const Group &G = P->groups[g];
         seqn_em[g].resize(G.nel * G.ns);
         conn_sm[g].resize(G.nn * G.nel);
         seqn_sm[g].resize(G.ns * G.nel);
         sub_g[g].resize(G.nel);
+        if (G.form == kFormFlux) coef_g[g].resize(2 * G.nel);
     }
     std::vector<double> coords_h(P->nnode * 3, 0.0), pinned_h(P->nnode * 3, 0.0),
         aux_h(P->nnode, 0.0);
@@ -210,12 +212,12 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         }
         for (int k = 0; k < d.nparams; k++) P->params_h[(size_t)s * kParamStride + k] = d.params[k];
         if (S.nface > 0) {
-            P->params_h[(size_t)s * kParamStride + kFaceAlpha] = d.face_alpha;
-            P->params_h[(size_t)s * kParamStride + kFaceBeta] = d.face_beta;
             const Group &F = P->groups[S.fgroup];
             for (int64_t f = 0; f < d.nface; f++) {
                 const int64_t ge = S.fel_off + f;
                 sub_g[S.fgroup][ge] = s;
+                coef_g[S.fgroup][2 * ge] = d.face_coef[2 * f];
+                coef_g[S.fgroup][2 * ge + 1] = d.face_coef[2 * f + 1];
                 for (int l = 0; l < F.nn; l++) {
                     const int32_t n = d.face_conn[f * F.nn + l];
                     PYHIP_REQUIRE(n >= 0 && n < d.nnode, "face connectivity out of range");
@@ -304,6 +306,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         UP(G.conn, conn_sm[g].data(), conn_sm[g].size());
         UP(G.seqn, seqn_sm[g].data(), seqn_sm[g].size());
         UP(G.sub, sub_g[g].data(), sub_g[g].size());
+        if (G.form == kFormFlux) UP(G.coef, coef_g[g].data(), coef_g[g].size());
         if (G.form != kFormFlux && !P->tab64[G.kind]) {
             rc = upload_tables<double>(G.kind, &P->tab64[G.kind], B);
             if (rc) return rc;
@@ -337,6 +340,7 @@ int pyhip_problem_destroy(pyhip_problem *p) {
         dfree(G.conn);
         dfree(G.seqn);
         dfree(G.sub);
+        dfree(G.coef);
     }
     void *ptrs[] = {p->coords, p->pinned, p->aux, p->params, p->row_start, p->col, p->diag,
                     p->occ_ptr, p->occ_roff, p->cptr, p->cidx,
@@ -429,6 +433,7 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
             a.hist1 = p->hist1; a.hist2 = p->hist2;
             a.hvec = p->hvec;
+            a.fcoef = G.coef;
             TimeScope ts(&p->timer, "element", p->st);
             if (G.form == kFormFlux) {
                 if ((rc = launch_faces_f64(G.kind, a, p->st))) return rc;
@@ -448,6 +453,7 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
             a.hist1 = p->hist1; a.hist2 = p->hist2;
             a.hvec = p->hvec;
+            a.fcoef = G.coef;
             TimeScope ts(&p->timer, "element", p->st);
             if (G.form == kFormFlux) {
                 if ((rc = launch_faces_f32(G.kind, a, p->st))) return rc;

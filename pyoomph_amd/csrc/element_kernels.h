@@ -61,6 +61,8 @@ struct ElemArgs {
     const double *hist1, *hist2;
     // Hessian-vector product: Ej receives sum_k dJ/du_k y_k (y dof-indexed, 0 at pinned)
     const double *hvec;
+    // boundary faces: [nel][2] alpha, beta of each face's flux alpha*u + beta
+    const double *fcoef;
 };
 
 template <int KIND> struct ElemTraits;

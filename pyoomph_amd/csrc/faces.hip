@@ -1,6 +1,6 @@
 // Boundary-face (interface) elements of the scalar forms: the flux term of pyoomph's
 // NeumannBC(u=flux) @ boundary (pyoomph/meshes/bcs.py:71-89: residual weak(flux, test)),
-// flux = alpha*u + beta, on the faces of P1 / P2 triangles.
+// flux = alpha*u + beta with alpha, beta per face, on the faces of P1 / P2 triangles.
 //
 // A face element is the reference's InterfaceTElementLine1dC1 / C2 (src/mesh.cpp:627-635):
 // a TElement<1,2> / TElement<1,3> line (nodes at s = 0, 1 or 0, 1/2, 1; shapes
@@ -51,9 +51,7 @@ template <typename T, int NN>
 __global__ __launch_bounds__(256) void flux_face_kernel(ElemArgs<T> a) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= a.nel) return;
-    const int s = a.sub[e];
-    const T alpha = (T)a.params[(int64_t)s * a.pstride + kFaceAlpha];
-    const T beta = (T)a.params[(int64_t)s * a.pstride + kFaceBeta];
+    const T alpha = (T)a.fcoef[2 * (int64_t)e], beta = (T)a.fcoef[2 * (int64_t)e + 1];
     T X[NN][2], U[NN];
 #pragma unroll
     for (int l = 0; l < NN; l++) {

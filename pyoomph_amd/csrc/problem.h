@@ -13,12 +13,11 @@ namespace pyhip {
 
 constexpr int kParamStride = 64;  // doubles of parameters per sub-problem
 
-// Boundary-face (interface) elements: internal form and element kinds of their groups;
-// their flux coefficients sit at the end of the sub-problem's parameter block.
+// Boundary-face (interface) elements: internal form and element kinds of their groups
+// (each face carries its own flux coefficients, Group::coef).
 constexpr int kFormFlux = 16;          // flux = alpha*u + beta on the faces
 constexpr int kKindLine2 = 16;         // 2-node faces of P1 triangles (TGauss<1,2>)
 constexpr int kKindLine3 = 17;         // 3-node faces of P2 triangles (TGauss<1,3>)
-constexpr int kFaceAlpha = kParamStride - 2, kFaceBeta = kParamStride - 1;
 
 struct SubInfo {
     int form, kind, nvalue;
@@ -37,6 +36,7 @@ struct Group {
     int32_t *conn = nullptr;          // [nn][nel]
     int32_t *seqn = nullptr;          // [ns][nel]
     int32_t *sub = nullptr;           // [nel]
+    double *coef = nullptr;           // face groups: [nel][2] alpha, beta of each face's flux
 };
 
 struct KrylovWork;   // krylov.hip

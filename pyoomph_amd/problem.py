@@ -40,9 +40,10 @@ class MeshProblem:
     dirichlet: list of (boundary names, {value_index: value}) applied in order (a node on
     several boundaries keeps the value of the last condition that names it), as
     successive ``DirichletBC(...) @ boundary`` do in pyoomph.
-    flux: (boundary names, alpha, beta) -- ``NeumannBC(u=alpha*u + beta) @ boundary`` for
-    each name (pyoomph/meshes/bcs.py:71-89: the residual gains the face integral of
-    (alpha*u + beta) * test), on P1 / P2 triangle meshes of the scalar forms; None: none.
+    flux: list of (boundary names, alpha, beta) -- ``NeumannBC(u=alpha*u + beta) @ boundary``
+    for each name (pyoomph/meshes/bcs.py:71-89: the residual gains the face integral of
+    (alpha*u + beta) * test), in order, on P1 / P2 triangle meshes of the scalar forms; one
+    (names, alpha, beta) triple is accepted for a single condition; None: none.
     """
 
     def __init__(self, mesh, equations, dirichlet=(), pin_extra=(), flux=None):
@@ -72,14 +73,21 @@ class MeshProblem:
         eqn[free] = np.arange(int(free.sum()))
         self.eqn = eqn.reshape(mesh.nnode, nv).astype(np.int32)
         self.ndof = int(free.sum())
-        self.faces, self.face_alpha, self.face_beta = None, 0.0, 0.0
+        self.faces, self.face_coef = None, None
         if flux is not None:
             from .meshes import boundary_faces
-            names, alpha, beta = flux
             if nv != 1:
                 raise ValueError("boundary-face fluxes are built for the scalar forms")
-            self.faces = boundary_faces(mesh, list(names))
-            self.face_alpha, self.face_beta = float(alpha), float(beta)
+            if len(flux) == 3 and np.isscalar(flux[1]):
+                flux = [flux]
+            faces, coef = [], []
+            for names, alpha, beta in flux:
+                names = [names] if isinstance(names, str) else list(names)
+                f = boundary_faces(mesh, names)
+                faces.append(f)
+                coef.append(np.tile([float(alpha), float(beta)], (len(f), 1)))
+            self.faces = np.ascontiguousarray(np.concatenate(faces), dtype=np.int32)
+            self.face_coef = np.ascontiguousarray(np.concatenate(coef), dtype=np.float64)
 
     def nodal_values(self, dofs):
         U = self.values.copy()
@@ -156,10 +164,11 @@ class Problem:
             d.aux = aux.ctypes.data_as(_lib.c_dp) if aux is not None else None
             if mp.faces is not None and len(mp.faces):
                 fc = np.ascontiguousarray(mp.faces, dtype=np.int32)
-                keep.append(fc)
+                cf = np.ascontiguousarray(mp.face_coef, dtype=np.float64)
+                keep += [fc, cf]
                 d.nface = fc.shape[0]
                 d.face_conn = fc.ctypes.data_as(_lib.c_i32p)
-                d.face_alpha, d.face_beta = mp.face_alpha, mp.face_beta
+                d.face_coef = cf.ctypes.data_as(_lib.c_dp)
         torch.cuda.set_device(self.device)
         # the problem runs on the stream current at creation; every host wait below is on
         # this stream only, so problems on other streams keep running

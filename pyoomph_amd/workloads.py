@@ -90,7 +90,7 @@ def bratu_flux(N, space, lam, alpha, beta, flux_sides=("right", "top"), fixed_si
     mesh = RectangularQuadMesh(N, split_in_tris="left", space=space)
     eq = BratuEquation("u", space=space, lambda_=lam)
     return MeshProblem(mesh, eq, dirichlet=[(tuple(fixed_sides), {0: 0.0})],
-                       flux=(tuple(flux_sides), alpha, beta))
+                       flux=[(tuple(flux_sides), alpha, beta)])
 
 
 def bratu_sweep_specs(nprob=512, seed=0, nmin=16, nmax=128):

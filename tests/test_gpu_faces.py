@@ -104,3 +104,35 @@ def test_flux_batch_equals_alone_bitwise(gpu):
         alone.krylov, alone.preconditioner = "cg", "amg"
         alone.solve()
         assert np.array_equal(alone.get_current_dofs(), batch.sub_dofs(i, xb)), i
+
+
+@pytest.mark.parametrize("space", ["C1", "C2"])
+def test_several_fluxes_on_one_mesh(gpu, space):
+    """Neumann on the right, Robin on the top, u = 0 on the left and bottom: per-face flux
+    coefficients against the oracle (assembly and Newton)."""
+    from pyoomph_amd.equations import BratuEquation
+    from pyoomph_amd.meshes import RectangularQuadMesh
+    from pyoomph_amd.problem import MeshProblem, Problem
+    N, lam = 10, 1.4
+    mesh = RectangularQuadMesh(N, split_in_tris="left", space=space)
+    mp = MeshProblem(mesh, BratuEquation("u", space=space, lambda_=lam), dirichlet=[(FIXED, {0: 0.0})],
+                     flux=[(("right",), 0.0, 0.45), (("top",), 1.7, -0.25)])
+    m = om.rectangular_quad_mesh(N, "left", space)
+    Fr, Ft = of.boundary_faces(m, ["right"]), of.boundary_faces(m, ["top"])
+    op = oa.OracleProblem(m, oa.FORM_BRATU, 1, m.on_boundary(set(FIXED))[:, None], np.zeros((m.nnode, 1)), [lam],
+                          faces=np.concatenate([Fr, Ft]),
+                          face_alpha=np.r_[np.zeros(len(Fr)), np.full(len(Ft), 1.7)],
+                          face_beta=np.r_[np.full(len(Fr), 0.45), np.full(len(Ft), -0.25)])
+    assert np.array_equal(mp.faces, op.faces)
+    P = Problem(mp)
+    x = np.random.default_rng(6).uniform(-0.3, 0.3, P.ndof)
+    P.set_current_dofs(x)
+    R, J = P.assemble_jacobian()
+    r, rs, col, val = op.get_jacobian(x)
+    assert np.array_equal(J.indptr, rs) and np.array_equal(J.indices, col)
+    assert rel(J.data, val) <= 1e-12 and rel(R, r) <= 1e-12
+    P.set_current_dofs(np.zeros(P.ndof))
+    P.krylov, P.preconditioner = "cg", "amg"
+    st = P.solve()
+    assert st[0]["converged"]
+    assert rel(P.get_current_dofs(), on.newton_solve(op, np.zeros(op.ndof), tol=1e-8)) <= 1e-8

@@ -111,3 +111,29 @@ def test_face_contributions_follow_the_bulk_in_the_csr():
     D = Jf - Jb
     assert np.allclose(D, D.T, atol=1e-15) and np.all(np.diag(D) >= 0) and D.sum() > 0
     assert np.allclose(rf - rb, D @ x, rtol=1e-12, atol=1e-15)
+
+
+def test_several_fluxes_on_one_mesh_superpose():
+    """Per-face coefficients: Neumann on one side and Robin on another give the sum of
+    the two conditions' face terms (residual and Jacobian)."""
+    m = om.rectangular_quad_mesh(4, "left", "C1")
+    pinned = m.on_boundary({"left"})[:, None]
+    kw = dict(mesh=m, form=oa.FORM_BRATU, nvalue=1, pinned=pinned, bc=np.zeros((m.nnode, 1)), params=[0.8])
+    Fr, Ft = of.boundary_faces(m, ["right"]), of.boundary_faces(m, ["top"])
+    both = oa.OracleProblem(**kw, faces=np.concatenate([Fr, Ft]),
+                            face_alpha=np.r_[np.zeros(len(Fr)), np.full(len(Ft), 2.5)],
+                            face_beta=np.r_[np.full(len(Fr), 0.4), np.full(len(Ft), -0.3)])
+    right = oa.OracleProblem(**kw, faces=Fr, face_alpha=0.0, face_beta=0.4)
+    top = oa.OracleProblem(**kw, faces=Ft, face_alpha=2.5, face_beta=-0.3)
+    none = oa.OracleProblem(**kw)
+    x = np.random.default_rng(4).standard_normal(none.ndof) * 0.2
+    import scipy.sparse as sp
+    n = none.ndof
+
+    def dense(op):
+        r, rs, c, v = op.get_jacobian(x)
+        return r, sp.csr_matrix((v, c, rs), shape=(n, n)).toarray()
+    (rb, Jb), (rr, Jr), (rt, Jt), (r0, J0) = dense(both), dense(right), dense(top), dense(none)
+    assert np.allclose(rb - r0, (rr - r0) + (rt - r0), rtol=1e-12, atol=1e-14)
+    assert np.allclose(Jb - J0, (Jr - J0) + (Jt - J0), rtol=1e-12, atol=1e-14)
+    assert np.array_equal(Jr, J0)            # a Neumann flux adds nothing to the Jacobian
