@@ -244,15 +244,20 @@ __global__ __launch_bounds__(256) void amg_wb_kernel(int64_t n, const double *__
 
 // Dense Cholesky factor of the listed sub-problems' matrices (<= kBottomMaxRows rows),
 // one workgroup per sub-problem, right-looking, fixed order. A non-positive pivot is
-// replaced by 1 and flagged.
+// replaced by 1 and flagged. The workgroup then writes the explicit inverse
+// X = L^-T L^-1 (thread c solves L y = e_c, L^T x = y for column c, 32 columns per pass)
+// as a symmetric matrix (entry (i, j) from the column max(i, j)), so the coarsest solve is
+// one dense symmetric product and the V-cycle stays a symmetric operator.
+static_assert(kCholThreads == kBottomMaxRows, "one thread per coarsest row");
 __global__ __launch_bounds__(kCholThreads) void amg_chol_factor_kernel(const int32_t *__restrict__ subs,
                                                                        const int64_t *__restrict__ sub_off,
                                                                        const int32_t *__restrict__ rs,
                                                                        const int32_t *__restrict__ col,
                                                                        const double *__restrict__ vals,
-                                                                       double *__restrict__ chol,
+                                                                       double *__restrict__ cinv,
                                                                        int32_t *__restrict__ fail) {
     __shared__ double a[kBottomMaxRows][kBottomMaxRows + 1];
+    __shared__ double w[kBottomMaxRows][kCholThreads / 2 + 1];
     const int s = subs[blockIdx.x], t = threadIdx.x;
     const int64_t r0 = sub_off[s];
     const int m = (int)(sub_off[s + 1] - r0);
@@ -278,39 +283,61 @@ __global__ __launch_bounds__(kCholThreads) void amg_chol_factor_kernel(const int
             for (int j = k + 1; j <= i; j++) a[i][j] = a[i][j] - a[i][k] * a[j][k];
         __syncthreads();
     }
-    double *L = chol + (int64_t)s * kBottomMaxRows * kBottomMaxRows;
-    for (int i = t; i < m; i += kCholThreads)
-        for (int j = 0; j <= i; j++) L[i * kBottomMaxRows + j] = a[i][j];
+    double *X = cinv + (int64_t)s * kBottomMaxRows * kBottomMaxRows;
+    constexpr int H = kCholThreads / 2;
+    for (int c0 = 0; c0 < m; c0 += H) {
+        const int c = c0 + t;
+        if (t < H && c < m) {
+            for (int i = 0; i < m; i++) {          // L y = e_c (y_i = 0 for i < c)
+                if (i < c) {
+                    w[i][t] = 0.0;
+                    continue;
+                }
+                double v = i == c ? 1.0 : 0.0;
+                for (int k = c; k < i; k++) v = v - a[i][k] * w[k][t];
+                w[i][t] = v / a[i][i];
+            }
+            for (int i = m - 1; i >= 0; i--) {     // L^T x = y
+                double v = w[i][t];
+                for (int k = i + 1; k < m; k++) v = v - a[k][i] * w[k][t];
+                w[i][t] = v / a[i][i];
+            }
+        }
+        __syncthreads();
+        // columns c0 .. c0+H-1: entries (i, c) with i <= c, mirrored
+        for (int q = t; q < H * m; q += kCholThreads) {
+            const int cc = q / m, i = q % m, c = c0 + cc;
+            if (c < m && i <= c) {
+                const double v = w[i][cc];
+                X[i * kBottomMaxRows + c] = v;
+                X[c * kBottomMaxRows + i] = v;
+            }
+        }
+        __syncthreads();
+    }
 }
 
-// x = (L L^T)^-1 b for the listed sub-problems (column-oriented triangular solves)
+// x = X b on the listed sub-problems' coarsest rows (X the symmetric inverse above): thread
+// i sums X[j][i] b[j] over j in increasing order (X[j][i] = X[i][j]; the reads of one j are
+// contiguous over the threads).
 __global__ __launch_bounds__(kCholThreads) void amg_chol_solve_kernel(const int32_t *__restrict__ subs,
                                                                       const int64_t *__restrict__ sub_off,
-                                                                      const double *__restrict__ chol,
-                                                                      const double *__restrict__ b,
-                                                                      double *__restrict__ x,
+                                                                      const double *__restrict__ cinv,
+                                                                      const double *b, double *x,
                                                                       const int32_t *__restrict__ act) {
-    __shared__ double y[kBottomMaxRows];
+    __shared__ double sb[kBottomMaxRows];
     const int s = subs[blockIdx.x], t = threadIdx.x;
     if (act && !act[s]) return;   // the whole workgroup leaves together
     const int64_t r0 = sub_off[s];
     const int m = (int)(sub_off[s + 1] - r0);
-    const double *L = chol + (int64_t)s * kBottomMaxRows * kBottomMaxRows;
-    for (int i = t; i < m; i += kCholThreads) y[i] = b[r0 + i];
+    const double *X = cinv + (int64_t)s * kBottomMaxRows * kBottomMaxRows;
+    if (t < m) sb[t] = b[r0 + t];
     __syncthreads();
-    for (int k = 0; k < m; k++) {           // L y = b
-        if (t == 0) y[k] = y[k] / L[k * kBottomMaxRows + k];
-        __syncthreads();
-        for (int i = k + 1 + t; i < m; i += kCholThreads) y[i] = y[i] - L[i * kBottomMaxRows + k] * y[k];
-        __syncthreads();
+    if (t < m) {
+        double acc = 0.0;
+        for (int j = 0; j < m; j++) acc = __builtin_fma(X[j * kBottomMaxRows + t], sb[j], acc);
+        x[r0 + t] = acc;
     }
-    for (int k = m - 1; k >= 0; k--) {      // L^T x = y
-        if (t == 0) y[k] = y[k] / L[k * kBottomMaxRows + k];
-        __syncthreads();
-        for (int i = t; i < k; i += kCholThreads) y[i] = y[i] - L[k * kBottomMaxRows + i] * y[k];
-        __syncthreads();
-    }
-    for (int i = t; i < m; i += kCholThreads) x[r0 + i] = y[i];
 }
 
 // ---- host ------------------------------------------------------------------------------------
