@@ -57,8 +57,13 @@ int amg_set_fine(AmgDev &A, const double *vals);
 // z = M^-1 b. With a timer, every level spmv is timed and counted as an "spmv" launch.
 // With act (device [nsub], host copy act_h) only the rows of sub-problems with act[s] != 0
 // are computed; the other rows of z keep their values.
+// x0_ready: the caller has already written the finest level's first smoothing x = W b
+// into amg_fine_x (see amg_fine_w), e.g. fused into the update that produced b.
 int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer *tm = nullptr,
-              const int32_t *act = nullptr, const int32_t *act_h = nullptr);
+              const int32_t *act = nullptr, const int32_t *act_h = nullptr, bool x0_ready = false);
+// the finest level's smoothing weights W and first-sweep vector (nullptr without levels)
+inline const double *amg_fine_w(const AmgDev &A) { return A.lv.empty() ? nullptr : A.lv[0].w; }
+inline double *amg_fine_x(const AmgDev &A) { return A.lv.empty() ? nullptr : A.lv[0].x; }
 void amg_free(AmgDev &A);
 
 }  // namespace pyhip

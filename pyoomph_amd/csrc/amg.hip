@@ -616,7 +616,7 @@ int amg_set_fine(AmgDev &A, const double *vals) {
 
 // z = M^-1 b (both [n0])
 int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer *tm, const int32_t *act,
-              const int32_t *act_h) {
+              const int32_t *act_h, bool x0_ready) {
     const int nl = (int)A.lv.size();
     AmgDevLevel &Bt = A.bot;
     if (nl == 0) {
@@ -642,7 +642,7 @@ int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer
         const double *bl = l == 0 ? b : L.b;
         AmgDevLevel &C = (l + 1 < nl) ? A.lv[l + 1] : A.bot;
         // x = W b (on coarser levels the restriction above has written it)
-        if (l == 0)
+        if (l == 0 && !x0_ready)
             hipLaunchKernelGGL(amg_wb_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.w, bl, L.x, L.row_sub, act);
         const int la = L.lpr_a ? L.lpr_a : lanes_for(L.nnz, L.n);
         level_spmv(tm, kSpmvSub, la, st, L.n, nnz[l], rows[l], rows[l], L.rs, L.col, L.vals, L.x, L.r, bl,

@@ -758,12 +758,12 @@ template <typename T>
 // act[s] != 0 when act (device, host copy act_h) is given; the other preconditioners
 // compute every row.
 static int precond_apply(pyhip_problem *p, KrylovWork *W, int pc, const T *v, T *z, const int32_t *act = nullptr,
-                         const int32_t *act_h = nullptr) {
+                         const int32_t *act_h = nullptr, bool x0_ready = false) {
     TimeScope ts(&p->timer, "precond", p->st);
     if (pc == PYHIP_PRECOND_ILU0) return ilu_apply<T>(p, W, v, z);
     if (pc == PYHIP_PRECOND_AMG) {
         if constexpr (std::is_same<T, double>::value)
-            return amg_apply(W->amg, p->st, v, z, &p->timer, act, act_h);
+            return amg_apply(W->amg, p->st, v, z, &p->timer, act, act_h, x0_ready);
         set_error("the multigrid preconditioner runs FP64 solves only");
         return PYHIP_ERR_INVALID;
     }
@@ -979,14 +979,18 @@ __global__ __launch_bounds__(256) void cg_xr_kernel(int64_t n, const T *__restri
                                                     T *__restrict__ r, const double *__restrict__ rz,
                                                     const double *__restrict__ pq,
                                                     const int32_t *__restrict__ dsub,
-                                                    const int32_t *__restrict__ active) {
+                                                    const int32_t *__restrict__ active,
+                                                    const double *__restrict__ w0, double *__restrict__ x0) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int s = dsub[i];
     if (!active[s]) return;
     const double a = rz[s] / pq[s];
     x[i] = T(double(x[i]) + a * double(p[i]));
-    r[i] = T(double(r[i]) - a * double(q[i]));
+    const T rn = T(double(r[i]) - a * double(q[i]));
+    r[i] = rn;
+    // the multigrid V-cycle's first smoothing x0 = W r of the new residual
+    if (x0) x0[i] = w0[i] * double(rn);
 }
 
 // convergence test on |r|, then beta = (r,z)_new / (r,z)_old
@@ -1076,6 +1080,14 @@ static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *
     KCHECK();
     int32_t nact = 0;
     if ((rc = read_counter(p, W, W->nactive_d, &nact, W->active, act_h))) return rc;
+    // with the multigrid preconditioner the x/r update also writes the V-cycle's first
+    // smoothing W r of the finest level
+    const double *w0 = nullptr;
+    double *x0 = nullptr;
+    if (std::is_same<T, double>::value && o.precond == PYHIP_PRECOND_AMG) {
+        w0 = amg_fine_w(W->amg);
+        x0 = amg_fine_x(W->amg);
+    }
     int it = 0;
     while (nact > 0 && it < o.max_iters) {
         if ((rc = spmv<TV, T, T>(p, vals, pv, qv, (const T *)nullptr, W->active, act_h))) return rc;
@@ -1085,10 +1097,10 @@ static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *
         {
             TimeScope ts(&p->timer, "vector", p->st);
             hipLaunchKernelGGL((cg_xr_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, pv, qv, x, r,
-                               W->cg_rz, W->cg_pq, p->dof_sub, W->active);
+                               W->cg_rz, W->cg_pq, p->dof_sub, W->active, w0, x0);
         }
         KCHECK();
-        if ((rc = precond_apply<T>(p, W, o.precond, r, z, W->active, act_h))) return rc;
+        if ((rc = precond_apply<T>(p, W, o.precond, r, z, W->active, act_h, x0 != nullptr))) return rc;
         if ((rc = batched_dot2<T>(p, W, r, r, r, z, d2))) return rc;
         PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
         hipLaunchKernelGGL(cg_check_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2, W->bnorm, rtol,
