@@ -17,6 +17,7 @@ struct AmgDevLevel {
     int32_t *rs = nullptr, *col = nullptr, *diag = nullptr, *agg = nullptr;
     int32_t *prs = nullptr, *pcol = nullptr, *pcptr = nullptr, *pcidx = nullptr;
     int32_t *trs = nullptr, *trow = nullptr, *tsrc = nullptr, *qrs = nullptr, *qcol = nullptr;
+    int32_t *prow = nullptr, *tinv = nullptr;   // row of each P entry; its position in P^T
     int32_t *row_sub = nullptr;
     int64_t *sub_off = nullptr;
     int32_t *cont = nullptr;           // [nsub] 1: the sub-problem continues below this level

@@ -105,23 +105,25 @@ __global__ __launch_bounds__(256) void amg_weights_kernel(int64_t n, const int32
     w[i] = (d[i] != 0.0 && isfinite(d[i])) ? om / d[i] : 0.0;
 }
 
-// P[q] = [pcol(q) == agg(i)] - w_i * (sum of the A entries listed for q)
-__global__ __launch_bounds__(256) void amg_pvals_kernel(int64_t n, const int32_t *__restrict__ prs,
+// P[q] = [pcol(q) == agg(i)] - w_i * (sum of the A entries listed for q), i = prow(q): one
+// thread per entry of P, which also writes the value to its place in P^T (tinv)
+__global__ __launch_bounds__(256) void amg_pvals_kernel(int64_t nnzp, const int32_t *__restrict__ prow,
                                                         const int32_t *__restrict__ pcol,
                                                         const int32_t *__restrict__ pcptr,
                                                         const int32_t *__restrict__ pcidx,
                                                         const int32_t *__restrict__ agg,
                                                         const double *__restrict__ vals,
-                                                        const double *__restrict__ w, double *__restrict__ pv) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const double wi = w[i];
-    const int32_t ai = agg[i];
-    for (int32_t q = prs[i]; q < prs[i + 1]; q++) {
-        double s = 0.0;
-        for (int32_t c = pcptr[q]; c < pcptr[q + 1]; c++) s += vals[pcidx[c]];
-        pv[q] = (pcol[q] == ai ? 1.0 : 0.0) - wi * s;
-    }
+                                                        const double *__restrict__ w,
+                                                        const int32_t *__restrict__ tinv, double *__restrict__ pv,
+                                                        double *__restrict__ ptv) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= nnzp) return;
+    const int32_t i = prow[q];
+    double s = 0.0;
+    for (int32_t c = pcptr[q]; c < pcptr[q + 1]; c++) s += vals[pcidx[c]];
+    const double v = (pcol[q] == agg[i] ? 1.0 : 0.0) - w[i] * s;
+    pv[q] = v;
+    ptv[tinv[q]] = v;
 }
 
 // position of column j in the sorted row [b, e) of cols, or -1
@@ -221,13 +223,6 @@ __global__ __launch_bounds__(256) void amg_rowprod_kernel(int64_t nout, const in
     for (int32_t k = lane; k < R; k += G) oval[o0 + k] = ma[k];
 }
 constexpr int kQLanes = 16, kQCap = 64, kCLanes = 32, kCCap = 128;
-
-// P^T values in P^T order
-__global__ __launch_bounds__(256) void amg_ptvals_kernel(int64_t nnzp, const int32_t *__restrict__ tsrc,
-                                                         const double *__restrict__ pv, double *__restrict__ ptv) {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t < nnzp) ptv[t] = pv[tsrc[t]];
-}
 
 // ---- V-cycle kernels -----------------------------------------------------------------------
 
@@ -363,7 +358,7 @@ void amg_free(AmgDev &A) {
     auto fl = [](AmgDevLevel &L) {
         void *ptrs[] = {L.rs, L.col, L.diag, L.agg, L.prs, L.pcol, L.pcptr, L.pcidx, L.trs, L.trow, L.tsrc,
                         L.qrs, L.qcol, L.row_sub, L.sub_off, L.own_vals, L.d, L.w, L.pv, L.qv, L.b, L.x,
-                        L.x2, L.r, L.pw, L.py, L.cont, L.ends, L.ptv};
+                        L.x2, L.r, L.pw, L.py, L.cont, L.ends, L.ptv, L.prow, L.tinv};
         for (void *q : ptrs) dfree(q);
         L = AmgDevLevel();
     };
@@ -427,6 +422,12 @@ int amg_build(AmgDev &A, int64_t n, const int32_t *rs_h, const int32_t *col_h, c
         for (size_t q = 0; q + 1 < P.sub_off.size(); q++)
             D.h_nnzp[q] = (int64_t)P.prs[P.sub_off[q + 1]] - P.prs[P.sub_off[q]];
         D.nnzq = P.qrs.back();
+        // row of every P entry, and the place of every P entry in P^T
+        std::vector<int32_t> prow(D.nnzp), tinv(D.nnzp);
+        for (int64_t i = 0; i + 1 < (int64_t)P.prs.size(); i++)
+            for (int32_t q = P.prs[i]; q < P.prs[i + 1]; q++) prow[q] = (int32_t)i;
+        for (int64_t t = 0; t < D.nnzp; t++) tinv[P.tsrc[t]] = (int32_t)t;
+        if ((rc = aup(&D.prow, prow, B)) || (rc = aup(&D.tinv, tinv, B))) return rc;
         if ((rc = aup(&D.agg, P.agg, B)) || (rc = aup(&D.prs, P.prs, B)) || (rc = aup(&D.pcol, P.pcol, B)) ||
             (rc = aup(&D.pcptr, P.pcptr, B)) || (rc = aup(&D.pcidx, P.pcidx, B)) || (rc = aup(&D.trs, P.trs, B)) ||
             (rc = aup(&D.trow, P.trow, B)) || (rc = aup(&D.tsrc, P.tsrc, B)) || (rc = aup(&D.qrs, P.qrs, B)) ||
@@ -582,9 +583,9 @@ int amg_setup(AmgDev &A, hipStream_t st, const double *vals0, KernelTimer *tm) {
         if (L.nends)
             hipLaunchKernelGGL(amg_chol_factor_kernel, dim3(L.nends), dim3(kCholThreads), 0, st, L.ends, L.sub_off,
                                L.rs, L.col, vals, A.chol, A.chol_fail);
-        hipLaunchKernelGGL(amg_pvals_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.prs, L.pcol, L.pcptr,
-                           L.pcidx, L.agg, vals, L.w, L.pv);
-        hipLaunchKernelGGL(amg_ptvals_kernel, dim3(ablk(L.nnzp)), dim3(256), 0, st, L.nnzp, L.tsrc, L.pv, L.ptv);
+        if (L.nnzp > 0)
+            hipLaunchKernelGGL(amg_pvals_kernel, dim3(ablk(L.nnzp)), dim3(256), 0, st, L.nnzp, L.prow, L.pcol,
+                               L.pcptr, L.pcidx, L.agg, vals, L.w, L.tinv, L.pv, L.ptv);
         // Q = A_l P (terms: A's row entries; source rows: P's rows)
         hipLaunchKernelGGL((amg_rowprod_kernel<kQLanes, kQCap>), dim3(ablk(L.n, 256 / kQLanes)), dim3(256), 0, st,
                            L.n, L.qrs, L.qcol, L.qv, L.rs, L.col, vals, L.prs, L.pcol, L.pv);
