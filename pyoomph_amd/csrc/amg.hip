@@ -2,7 +2,7 @@
 //
 // Symbolic plan: amg_plan.h (host, once per pattern). For every new matrix (each Newton
 // step) the device computes, level by level:
-//   d = diag(A_l); lambda_s ~ lambda_max(D^-1 A_l) of every sub-problem s by ten power
+//   d = diag(A_l); lambda_s ~ lambda_max(D^-1 A_l) of every sub-problem s by six power
 //   iterations from a fixed oscillating start; w_i = 4 / (3 * 1.1 * lambda_s) / d_i;
 //   P = (I - W A_l) P_t; Q = A_l P; A_{l+1} = P^T Q   (fixed summation orders).
 // One application z = M^-1 b is a V-cycle: on each level one weighted-Jacobi sweep from
@@ -23,7 +23,7 @@
 
 namespace pyhip {
 
-constexpr int kPowerIters = 10;
+constexpr int kPowerIters = 6;
 constexpr double kLambdaSafety = 1.1;
 constexpr int kCholThreads = 64;
 
