@@ -203,7 +203,8 @@ class OracleProblem:
         nf, nn = F.shape
         Xf = np.ascontiguousarray(self.mesh.coords[F][:, :, :2])
         Uf = np.ascontiguousarray(U[F, 0])
-        Rs, Js = face_outputs(Xf, Uf, self.face_alpha, self.face_beta, 1 if flag == 1 else 0)
+        Rs, Js = face_outputs(Xf, Uf, self.face_alpha, self.face_beta, 1 if flag == 1 else 0,
+                              quad=self.mesh.kind == "quad_q2")
         if deriv:
             Rs[:] = 0.0
         if deriv or flag == 2 or y is not None:

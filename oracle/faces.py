@@ -3,15 +3,18 @@
 
 Test infrastructure only (see ``oracle/__init__``).
 
-Restates, for P1 / P2 triangle meshes:
+Restates, for P1 / P2 triangle and Q2 quadrilateral meshes:
 
 * which faces exist: ``Mesh`` interface elements are built on the bulk elements' faces
   that lie on the named boundary (src/mesh.cpp:615-660; ``InterfaceTElementLine1dC1`` /
   ``C2`` for ``BulkElementTri2dC1`` / ``C2``), one boundary after the other, bulk
-  elements in element order, faces of an element in edge order (0-1, 1-2, 2-0);
+  elements in element order, faces of an element in edge order (triangles 0-1, 1-2, 2-0;
+  Q2 quads 0-1-2, 2-5-8, 6-7-8, 0-3-6) -- ``InterfaceElementLine1dC2`` for
+  ``BulkElementQuad2dC2``;
 * the face element: a ``TElement<1,2>`` / ``TElement<1,3>`` line (nodes at s = 0, 1 or
   0, 1/2, 1; oomph Telements.h ``TElementShape<1,2>``, ``<1,3>`` shape and dshape_local)
-  with its default ``TGauss<1,2>`` / ``TGauss<1,3>`` rule (oomph integral.cc:303-316;
+  with its default ``TGauss<1,2>`` / ``TGauss<1,3>`` rule (oomph integral.cc:303-316; Q2
+  quad faces: ``QElement<1,3>``, OneDimLagrange shapes on [-1, 1], ``Gauss<1,3>``;
   the generated functable's integration order is 0, so the element default is kept,
   src/mesh.cpp:664-667);
 * the residual ``weak(flux, testfunction(u))`` (pyoomph/meshes/bcs.py:84-89): per point
@@ -32,12 +35,20 @@ _RULES = {
         np.array([5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0])),
 }
 
-# triangle edges in local nodes, ends and (P2) midside node in face order
-_EDGES = {"tri_p1": [(0, 1), (1, 2), (2, 0)], "tri_p2": [(0, 3, 1), (1, 4, 2), (2, 5, 0)]}
+# Gauss<1,3> on [-1, 1] (integral.cc:50-53), the rule of QElement<1,3> faces of Q2 quads
+_RULE_Q3 = (np.array([-0.774596669241483, 0.0, 0.774596669241483]), np.array([5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0]))
+
+# element edges in local nodes, ends and (P2, Q2) midside node in face order
+_EDGES = {"tri_p1": [(0, 1), (1, 2), (2, 0)], "tri_p2": [(0, 3, 1), (1, 4, 2), (2, 5, 0)],
+          "quad_q2": [(0, 1, 2), (2, 5, 8), (6, 7, 8), (0, 3, 6)]}
 
 
-def shape(nn, s):
-    """TElementShape<1,nn>: psi, dpsi/ds at local coordinate s."""
+def shape(nn, s, quad=False):
+    """TElementShape<1,nn> (quad: OneDimLagrange shape<3>, dshape<3> of QElement<1,3>,
+    shape.h:634-649): psi, dpsi/ds at local coordinate s."""
+    if quad:
+        return (np.array([0.5 * s * (s - 1.0), 1.0 - s * s, 0.5 * s * (s + 1.0)]),
+                np.array([s - 0.5, -2.0 * s, s + 0.5]))
     if nn == 2:
         return np.array([1.0 - s, s]), np.array([-1.0, 1.0])
     psi = np.array([2.0 * (s - 1.0) * (s - 0.5), 4.0 * (1.0 - s) * s, 2.0 * (s - 0.5) * s])
@@ -49,7 +60,7 @@ def boundary_faces(mesh, names):
     """[nface, 2|3] node ids of the faces on each named boundary (in name order; bulk
     elements in order, edges 0-1, 1-2, 2-0)."""
     if mesh.kind not in _EDGES:
-        raise ValueError("boundary faces are built for triangle meshes")
+        raise ValueError("boundary faces are built for triangle and Q2 quadrilateral meshes")
     faces = []
     for name in names:
         on = mesh.on_boundary([name])
@@ -63,17 +74,17 @@ def boundary_faces(mesh, names):
     return np.array(faces, dtype=np.int64).reshape(-1, nn)
 
 
-def face_outputs(X, U, alpha, beta, flag):
+def face_outputs(X, U, alpha, beta, flag, quad=False):
     """Element residuals [nf, nn] and Jacobians [nf, nn, nn] in face-node numbering.
 
     X [nf, nn, 2] node coordinates, U [nf, nn] nodal values, alpha / beta scalars or one
-    value per face."""
+    value per face; quad: the faces of Q2 quadrilaterals."""
     nf, nn = U.shape
-    knots, weights = _RULES[nn]
+    knots, weights = _RULE_Q3 if quad else _RULES[nn]
     R = np.zeros((nf, nn))
     J = np.zeros((nf, nn, nn))
     for g in range(nn):
-        psi, dpsi = shape(nn, knots[g])
+        psi, dpsi = shape(nn, knots[g], quad)
         tx = np.zeros(nf)
         ty = np.zeros(nf)
         u = np.zeros(nf)

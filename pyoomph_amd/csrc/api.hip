@@ -90,6 +90,11 @@ int pyhip_element_init(int form, int elem_kind, pyhip_element_table *t) {
     return PYHIP_OK;
 }
 
+// face element kind of a bulk element kind
+static int face_kind(int elem_kind) {
+    return elem_kind == PYHIP_ELEM_TRI_P1 ? kKindLine2 : (elem_kind == PYHIP_ELEM_QUAD_Q2 ? kKindLine3Q : kKindLine3);
+}
+
 int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stream,
                          pyhip_problem **out) {
     PYHIP_REQUIRE(subs && nsub > 0 && out, "invalid arguments");
@@ -114,16 +119,17 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         PYHIP_REQUIRE(d.ndof >= 0, "negative ndof");
         PYHIP_REQUIRE(d.nface >= 0 && (d.nface == 0 || (d.face_conn && d.face_coef)),
                       "faces without connectivity or flux coefficients");
-        PYHIP_REQUIRE(d.nface == 0 || ((d.elem_kind == PYHIP_ELEM_TRI_P1 || d.elem_kind == PYHIP_ELEM_TRI_P2) &&
+        PYHIP_REQUIRE(d.nface == 0 || ((d.elem_kind == PYHIP_ELEM_TRI_P1 || d.elem_kind == PYHIP_ELEM_TRI_P2 ||
+                                        d.elem_kind == PYHIP_ELEM_QUAD_Q2) &&
                                        d.nvalue == 1),
-                      "boundary-face flux elements: scalar forms on P1 / P2 triangles");
+                      "boundary-face flux elements: scalar forms on P1 / P2 triangles and Q2 quadrilaterals");
         std::pair<int, int> k(d.form, d.elem_kind);
         if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
     }
     // face groups after the bulk groups, one per face kind
     for (int s = 0; s < nsub; s++) {
         if (subs[s].nface == 0) continue;
-        std::pair<int, int> k(kFormFlux, subs[s].elem_kind == PYHIP_ELEM_TRI_P1 ? kKindLine2 : kKindLine3);
+        std::pair<int, int> k(kFormFlux, face_kind(subs[s].elem_kind));
         if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
     }
     P->groups.resize(gkeys.size());
@@ -159,7 +165,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         S.nel = d.nelem;
         P->groups[S.group].nel += d.nelem;
         if (d.nface > 0) {
-            const std::pair<int, int> fk(kFormFlux, d.elem_kind == PYHIP_ELEM_TRI_P1 ? kKindLine2 : kKindLine3);
+            const std::pair<int, int> fk(kFormFlux, face_kind(d.elem_kind));
             S.fgroup = (int)(std::find(gkeys.begin(), gkeys.end(), fk) - gkeys.begin());
             S.fel_off = P->groups[S.fgroup].nel;
             S.nface = d.nface;

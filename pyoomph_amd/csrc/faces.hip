@@ -1,12 +1,16 @@
 // Boundary-face (interface) elements of the scalar forms: the flux term of pyoomph's
 // NeumannBC(u=flux) @ boundary (pyoomph/meshes/bcs.py:71-89: residual weak(flux, test)),
-// flux = alpha*u + beta with alpha, beta per face, on the faces of P1 / P2 triangles.
+// flux = alpha*u + beta with alpha, beta per face, on the faces of P1 / P2 triangles and
+// Q2 quadrilaterals.
 //
-// A face element is the reference's InterfaceTElementLine1dC1 / C2 (src/mesh.cpp:627-635):
-// a TElement<1,2> / TElement<1,3> line (nodes at s = 0, 1 or 0, 1/2, 1; shapes
-// Telements.h TElementShape<1,2>, <1,3>) with its default TGauss<1,2> / TGauss<1,3> scheme
-// (integral.cc:303-316; the functable's integration order is 0, so the element default
-// stays, src/mesh.cpp:664-667). Per point: tangent t = sum_l x_l dpsi_l/ds,
+// A face element of a triangle is the reference's InterfaceTElementLine1dC1 / C2
+// (src/mesh.cpp:627-635): a TElement<1,2> / TElement<1,3> line (nodes at s = 0, 1 or
+// 0, 1/2, 1; shapes Telements.h TElementShape<1,2>, <1,3>) with its default TGauss<1,2> /
+// TGauss<1,3> scheme (integral.cc:303-316); a face element of a Q2 quadrilateral is
+// InterfaceElementLine1dC2 (src/mesh.cpp:618-621): a QElement<1,3> line (nodes at
+// s = -1, 0, 1; shape.h OneDimLagrange shape<3>, dshape<3>) with Gauss<1,3>
+// (integral.cc:50-53). The functable's integration order is 0, so the element default
+// scheme stays (src/mesh.cpp:664-667). Per point: tangent t = sum_l x_l dpsi_l/ds,
 // W = w |t|, u = sum_l u_l psi_l;
 //   R_i += W * (flux * psi_i),   J_ij += W * ((alpha * psi_j) * psi_i).
 // The flux does not depend on lambda, on du/dt or on second derivatives: dR/dlambda, the
@@ -47,7 +51,23 @@ template <> struct FaceRule<3> {
     }
 };
 
-template <typename T, int NN>
+// Q2-quadrilateral faces: QElement<1,3>, Gauss<1,3> on [-1, 1]
+struct FaceRuleQ3 {
+    __device__ static void point(int g, double &s, double &w) {
+        s = g == 0 ? -0.774596669241483 : (g == 1 ? 0.0 : 0.774596669241483);
+        w = g == 1 ? (8.0 / 9.0) : (5.0 / 9.0);
+    }
+    template <typename T> __device__ static void shape(T s, T (&psi)[3], T (&dpsi)[3]) {
+        psi[0] = T(0.5) * s * (s - T(1.0));
+        psi[1] = T(1.0) - s * s;
+        psi[2] = T(0.5) * s * (s + T(1.0));
+        dpsi[0] = s - T(0.5);
+        dpsi[1] = T(-2.0) * s;
+        dpsi[2] = s + T(0.5);
+    }
+};
+
+template <typename T, int NN, typename RULE>
 __global__ __launch_bounds__(256) void flux_face_kernel(ElemArgs<T> a) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= a.nel) return;
@@ -71,10 +91,10 @@ __global__ __launch_bounds__(256) void flux_face_kernel(ElemArgs<T> a) {
 #pragma unroll
     for (int g = 0; g < NN; g++) {
         double sd, wd;
-        FaceRule<NN>::point(g, sd, wd);
+        RULE::point(g, sd, wd);
         const T sq = (T)sd, wq = (T)wd;
         T psi[NN], dpsi[NN];
-        FaceRule<NN>::shape(sq, psi, dpsi);
+        RULE::shape(sq, psi, dpsi);
         T tx = T(0), ty = T(0), u = T(0);
 #pragma unroll
         for (int l = 0; l < NN; l++) {
@@ -108,8 +128,9 @@ template <typename T>
 static int launch_faces(int kind, const ElemArgs<T> &a, hipStream_t st) {
     if (a.nel <= 0) return PYHIP_OK;
     const dim3 g((a.nel + 255) / 256), t(256);
-    if (kind == kKindLine2) hipLaunchKernelGGL((flux_face_kernel<T, 2>), g, t, 0, st, a);
-    else if (kind == kKindLine3) hipLaunchKernelGGL((flux_face_kernel<T, 3>), g, t, 0, st, a);
+    if (kind == kKindLine2) hipLaunchKernelGGL((flux_face_kernel<T, 2, FaceRule<2>>), g, t, 0, st, a);
+    else if (kind == kKindLine3) hipLaunchKernelGGL((flux_face_kernel<T, 3, FaceRule<3>>), g, t, 0, st, a);
+    else if (kind == kKindLine3Q) hipLaunchKernelGGL((flux_face_kernel<T, 3, FaceRuleQ3>), g, t, 0, st, a);
     else {
         set_error("unknown face element kind");
         return PYHIP_ERR_INVALID;

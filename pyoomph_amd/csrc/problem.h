@@ -18,6 +18,7 @@ constexpr int kParamStride = 64;  // doubles of parameters per sub-problem
 constexpr int kFormFlux = 16;          // flux = alpha*u + beta on the faces
 constexpr int kKindLine2 = 16;         // 2-node faces of P1 triangles (TGauss<1,2>)
 constexpr int kKindLine3 = 17;         // 3-node faces of P2 triangles (TGauss<1,3>)
+constexpr int kKindLine3Q = 18;        // 3-node faces of Q2 quadrilaterals (Gauss<1,3>)
 
 struct SubInfo {
     int form, kind, nvalue;

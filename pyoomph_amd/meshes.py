@@ -66,20 +66,21 @@ class StructuredMesh:
         return m
 
 
-# triangle edges in local nodes: ends and (P2) midside node, in face-node order
-_TRI_EDGES = {"tri_p1": ((0, 1), (1, 2), (2, 0)), "tri_p2": ((0, 3, 1), (1, 4, 2), (2, 5, 0))}
+# element edges in local nodes: ends and (P2, Q2) midside node, in face-node order
+_FACE_EDGES = {"tri_p1": ((0, 1), (1, 2), (2, 0)), "tri_p2": ((0, 3, 1), (1, 4, 2), (2, 5, 0)),
+               "quad_q2": ((0, 1, 2), (2, 5, 8), (6, 7, 8), (0, 3, 6))}
 
 
 def boundary_faces(mesh, names):
     """Nodes of the bulk faces on the named boundaries, for the interface elements that
     carry a boundary flux (the reference builds one interface element per bulk face on
     the boundary, src/mesh.cpp:615-660): int32 [nface, 2] (P1 triangles: end, end) or
-    [nface, 3] (P2: end, midside, end); one boundary after the other, bulk elements in
-    element order, edges 0-1, 1-2, 2-0 of each."""
-    if mesh.kind not in _TRI_EDGES:
-        raise ValueError("boundary-face fluxes are built for P1 / P2 triangle meshes")
+    [nface, 3] (P2 triangles, Q2 quadrilaterals: end, midside, end); one boundary after the
+    other, bulk elements in element order, edges of each in the order of ``_FACE_EDGES``."""
+    if mesh.kind not in _FACE_EDGES:
+        raise ValueError("boundary-face fluxes are built for P1 / P2 triangle and Q2 quadrilateral meshes")
     E = mesh.elements
-    edges = _TRI_EDGES[mesh.kind]
+    edges = _FACE_EDGES[mesh.kind]
     out = []
     for name in names:
         on = mesh.boundary_nodes(name)

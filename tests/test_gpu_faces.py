@@ -16,11 +16,14 @@ pytestmark = pytest.mark.gpu
 FLUX, FIXED = ("right", "top"), ("left", "bottom")
 
 
-def build(N, space, lam, alpha, beta):
-    from pyoomph_amd import workloads as wl
-    from pyoomph_amd.problem import Problem
-    mp = wl.bratu_flux(N, space, lam, alpha, beta, FLUX, FIXED)
-    m = om.rectangular_quad_mesh(N, "left", space)
+def build(N, space, lam, alpha, beta, split="left"):
+    from pyoomph_amd.equations import BratuEquation
+    from pyoomph_amd.meshes import RectangularQuadMesh
+    from pyoomph_amd.problem import MeshProblem, Problem
+    mesh = RectangularQuadMesh(N, split_in_tris=split, space=space)
+    mp = MeshProblem(mesh, BratuEquation("u", space=space, lambda_=lam), dirichlet=[(FIXED, {0: 0.0})],
+                     flux=[(FLUX, alpha, beta)])
+    m = om.rectangular_quad_mesh(N, split, space)
     op = oa.OracleProblem(m, oa.FORM_BRATU, 1, m.on_boundary(set(FIXED))[:, None], np.zeros((m.nnode, 1)),
                           [lam], faces=of.boundary_faces(m, list(FLUX)), face_alpha=alpha, face_beta=beta)
     assert np.array_equal(mp.eqn, op.eqn) and np.array_equal(mp.faces, op.faces)
@@ -31,10 +34,10 @@ def rel(a, b):
     return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
 
 
-@pytest.mark.parametrize("space", ["C1", "C2"])
+@pytest.mark.parametrize("space,split", [("C1", "left"), ("C2", "left"), ("C2", False)])
 @pytest.mark.parametrize("alpha,beta", [(0.0, 0.6), (1.3, -0.4)])
-def test_flux_assembly_parity(gpu, space, alpha, beta):
-    P, op = build(7, space, 1.2, alpha, beta)
+def test_flux_assembly_parity(gpu, space, split, alpha, beta):
+    P, op = build(7, space, 1.2, alpha, beta, split)
     for seed in (1, 2):
         x = np.random.default_rng(seed).uniform(-0.4, 0.4, P.ndof)
         P.set_current_dofs(x)
@@ -59,13 +62,16 @@ def test_flux_mass_matrix_has_no_face_term(gpu):
     assert np.array_equal(M.data, val)
 
 
-@pytest.mark.parametrize("space", ["C1", "C2"])
+@pytest.mark.parametrize("space,split", [("C1", "left"), ("C2", "left"), ("C2", False)])
 @pytest.mark.parametrize("alpha,beta", [(0.0, 0.7), (2.0, 0.7)])
-def test_laplace_with_flux_side_has_linear_solution(gpu, space, alpha, beta):
+def test_laplace_with_flux_side_has_linear_solution(gpu, space, split, alpha, beta):
     """u = 0 on the left, flux on the right only: u = -beta / (1 + alpha) * x."""
-    from pyoomph_amd import workloads as wl
-    from pyoomph_amd.problem import Problem
-    mp = wl.bratu_flux(9, space, 0.0, alpha, beta, ("right",), ("left",))
+    from pyoomph_amd.equations import BratuEquation
+    from pyoomph_amd.meshes import RectangularQuadMesh
+    from pyoomph_amd.problem import MeshProblem, Problem
+    mp = MeshProblem(RectangularQuadMesh(9, split_in_tris=split, space=space),
+                     BratuEquation("u", space=space, lambda_=0.0), dirichlet=[(("left",), {0: 0.0})],
+                     flux=[(("right",), alpha, beta)])
     P = Problem(mp)
     P.krylov, P.preconditioner = "cg", "jacobi"
     P.newton_solver_tolerance = 1e-12
@@ -73,15 +79,16 @@ def test_laplace_with_flux_side_has_linear_solution(gpu, space, alpha, beta):
     assert st[0]["converged"]
     u = mp.nodal_values(P.get_current_dofs())[:, 0]
     c = -beta / (1.0 + alpha)
-    assert np.max(np.abs(u - c * mp.mesh.coords[:, 0])) <= 1e-10
+    # Q2: the reference's Gauss<2,3> positive knot is mistyped (~6e-9), see the oracle test
+    assert np.max(np.abs(u - c * mp.mesh.coords[:, 0])) <= (1e-10 if split else 5e-9)
 
 
 @pytest.mark.parametrize("opts", [dict(krylov="cg", preconditioner="jacobi"),
                                   dict(krylov="cg", preconditioner="amg"),
                                   dict(krylov="gmres", preconditioner="ilu0")])
-@pytest.mark.parametrize("space", ["C1", "C2"])
-def test_flux_newton_matches_oracle(gpu, opts, space):
-    P, op = build(14, space, 1.8, 0.8, 0.3)
+@pytest.mark.parametrize("space,split", [("C1", "left"), ("C2", "left"), ("C2", False)])
+def test_flux_newton_matches_oracle(gpu, opts, space, split):
+    P, op = build(14, space, 1.8, 0.8, 0.3, split)
     for k, v in opts.items():
         setattr(P, k, v)
     st = P.solve()
