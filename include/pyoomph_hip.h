@@ -96,12 +96,13 @@ typedef struct {
      * src/mesh.cpp:615-660) with flux = alpha*u + beta per face (Neumann: alpha = 0;
      * Robin: alpha != 0), so several conditions on several boundaries of one mesh are
      * one face list. Scalar forms on P1 / P2 triangles (faces: TElement<1,2> /
-     * TElement<1,3> with TGauss<1,2> / TGauss<1,3>, src/elements.hpp:676-730) and Q2
-     * quadrilaterals (QElement<1,3> with Gauss<1,3>). The faces follow the bulk elements
-     * in element order. nface = 0: none. */
+     * TElement<1,3> with TGauss<1,2> / TGauss<1,3>, src/elements.hpp:676-730), Q2
+     * quadrilaterals (QElement<1,3> with Gauss<1,3>) and Q2 bricks (QElement<2,3> with
+     * Gauss<2,3>). The faces follow the bulk elements in element order. nface = 0: none. */
     int64_t nface;
-    const int32_t *face_conn;    /* [nface][2] (P1: end, end) or [nface][3] (P2, Q2: end,
-                                    midside, end) mesh nodes of each face                */
+    const int32_t *face_conn;    /* [nface][2] (P1: end, end), [nface][3] (P2, Q2 quads:
+                                    end, midside, end) or [nface][9] (Q2 bricks: the
+                                    face's 3x3 node lattice) mesh nodes of each face     */
     const double *face_coef;     /* [nface][2] alpha, beta of each face's flux           */
 } pyhip_subproblem_desc;
 

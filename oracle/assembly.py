@@ -198,13 +198,17 @@ class OracleProblem:
     def _append_faces(self, U, R, J, flag, deriv, y):
         """Face element blocks after the bulk ones, in the same compact local numbering
         (residual stride maxdof, Jacobian block e at e*maxdof^2 with row stride nvar)."""
-        from .faces import face_outputs
+        from .faces import face_outputs, face_outputs_quad9
         F = self.faces
         nf, nn = F.shape
-        Xf = np.ascontiguousarray(self.mesh.coords[F][:, :, :2])
         Uf = np.ascontiguousarray(U[F, 0])
-        Rs, Js = face_outputs(Xf, Uf, self.face_alpha, self.face_beta, 1 if flag == 1 else 0,
-                              quad=self.mesh.kind == "quad_q2")
+        if nn == 9:
+            Xf = np.ascontiguousarray(self.mesh.coords[F][:, :, :3])
+            Rs, Js = face_outputs_quad9(Xf, Uf, self.face_alpha, self.face_beta, 1 if flag == 1 else 0)
+        else:
+            Xf = np.ascontiguousarray(self.mesh.coords[F][:, :, :2])
+            Rs, Js = face_outputs(Xf, Uf, self.face_alpha, self.face_beta, 1 if flag == 1 else 0,
+                                  quad=self.mesh.kind == "quad_q2")
         if deriv:
             Rs[:] = 0.0
         if deriv or flag == 2 or y is not None:

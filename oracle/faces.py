@@ -3,14 +3,15 @@
 
 Test infrastructure only (see ``oracle/__init__``).
 
-Restates, for P1 / P2 triangle and Q2 quadrilateral meshes:
+Restates, for P1 / P2 triangle, Q2 quadrilateral and Q2 brick meshes:
 
 * which faces exist: ``Mesh`` interface elements are built on the bulk elements' faces
   that lie on the named boundary (src/mesh.cpp:615-660; ``InterfaceTElementLine1dC1`` /
   ``C2`` for ``BulkElementTri2dC1`` / ``C2``), one boundary after the other, bulk
   elements in element order, faces of an element in edge order (triangles 0-1, 1-2, 2-0;
-  Q2 quads 0-1-2, 2-5-8, 6-7-8, 0-3-6) -- ``InterfaceElementLine1dC2`` for
-  ``BulkElementQuad2dC2``;
+  Q2 quads 0-1-2, 2-5-8, 6-7-8, 0-3-6; Q2 bricks k=0, k=2, j=0, j=2, i=0, i=2) --
+  ``InterfaceElementLine1dC2`` for ``BulkElementQuad2dC2``, ``InterfaceElementQuad2dC2``
+  for ``BulkElementBrick3dC2`` (``face_outputs_quad9``);
 * the face element: a ``TElement<1,2>`` / ``TElement<1,3>`` line (nodes at s = 0, 1 or
   0, 1/2, 1; oomph Telements.h ``TElementShape<1,2>``, ``<1,3>`` shape and dshape_local)
   with its default ``TGauss<1,2>`` / ``TGauss<1,3>`` rule (oomph integral.cc:303-316; Q2
@@ -40,7 +41,14 @@ _RULE_Q3 = (np.array([-0.774596669241483, 0.0, 0.774596669241483]), np.array([5.
 
 # element edges in local nodes, ends and (P2, Q2) midside node in face order
 _EDGES = {"tri_p1": [(0, 1), (1, 2), (2, 0)], "tri_p2": [(0, 3, 1), (1, 4, 2), (2, 5, 0)],
-          "quad_q2": [(0, 1, 2), (2, 5, 8), (6, 7, 8), (0, 3, 6)]}
+          "quad_q2": [(0, 1, 2), (2, 5, 8), (6, 7, 8), (0, 3, 6)],
+          # Q2 brick: local node i + 3j + 9k; a face's nodes a + 3b over its two free axes
+          "hex_q2": [tuple(i + 3 * j for j in range(3) for i in range(3)),
+                     tuple(18 + i + 3 * j for j in range(3) for i in range(3)),
+                     tuple(i + 9 * k for k in range(3) for i in range(3)),
+                     tuple(6 + i + 9 * k for k in range(3) for i in range(3)),
+                     tuple(3 * j + 9 * k for k in range(3) for j in range(3)),
+                     tuple(2 + 3 * j + 9 * k for k in range(3) for j in range(3))]}
 
 
 def shape(nn, s, quad=False):
@@ -60,7 +68,7 @@ def boundary_faces(mesh, names):
     """[nface, 2|3] node ids of the faces on each named boundary (in name order; bulk
     elements in order, edges 0-1, 1-2, 2-0)."""
     if mesh.kind not in _EDGES:
-        raise ValueError("boundary faces are built for triangle and Q2 quadrilateral meshes")
+        raise ValueError("boundary faces are built for triangle, Q2 quadrilateral and Q2 brick meshes")
     faces = []
     for name in names:
         on = mesh.on_boundary([name])
@@ -72,6 +80,48 @@ def boundary_faces(mesh, names):
                     faces.append(nodes)
     nn = len(_EDGES[mesh.kind][0])
     return np.array(faces, dtype=np.int64).reshape(-1, nn)
+
+
+def face_outputs_quad9(X, U, alpha, beta, flag):
+    """Faces of Q2 bricks (``InterfaceElementQuad2dC2``: ``QElement<2,3>`` with the Q2
+    quadrilateral's ``Gauss<2,3>`` table, its mistyped knot included): X [nf, 9, 3], U [nf, 9].
+    Per point W = w |t_0 x t_1| and the flux come first; each entry then sums its nine point
+    terms in point order."""
+    from .tables import tables
+    T = tables("quad_q2")
+    w, psi, dpsi = T["w"], T["psi"], T["dpsi"]
+    nf = U.shape[0]
+    nq = len(w)
+    W = np.zeros((nf, nq))
+    Fl = np.zeros((nf, nq))
+    for q in range(nq):
+        t0 = np.zeros((nf, 3))
+        t1 = np.zeros((nf, 3))
+        u = np.zeros(nf)
+        for l in range(9):
+            for d in range(3):
+                t0[:, d] = t0[:, d] + X[:, l, d] * dpsi[q, l, 0]
+                t1[:, d] = t1[:, d] + X[:, l, d] * dpsi[q, l, 1]
+            u = u + U[:, l] * psi[q, l]
+        n0 = t0[:, 1] * t1[:, 2] - t0[:, 2] * t1[:, 1]
+        n1 = t0[:, 2] * t1[:, 0] - t0[:, 0] * t1[:, 2]
+        n2 = t0[:, 0] * t1[:, 1] - t0[:, 1] * t1[:, 0]
+        W[:, q] = w[q] * np.sqrt((n0 * n0 + n1 * n1) + n2 * n2)
+        Fl[:, q] = alpha * u + beta
+    R = np.zeros((nf, 9))
+    J = np.zeros((nf, 9, 9))
+    for i in range(9):
+        r = np.zeros(nf)
+        for q in range(nq):
+            r = r + W[:, q] * (Fl[:, q] * psi[q, i])
+        R[:, i] = r
+        if flag == 1:
+            for j in range(9):
+                v = np.zeros(nf)
+                for q in range(nq):
+                    v = v + W[:, q] * ((alpha * psi[q, j]) * psi[q, i])
+                J[:, i, j] = v
+    return R, J
 
 
 def face_outputs(X, U, alpha, beta, flag, quad=False):

@@ -92,7 +92,12 @@ int pyhip_element_init(int form, int elem_kind, pyhip_element_table *t) {
 
 // face element kind of a bulk element kind
 static int face_kind(int elem_kind) {
-    return elem_kind == PYHIP_ELEM_TRI_P1 ? kKindLine2 : (elem_kind == PYHIP_ELEM_QUAD_Q2 ? kKindLine3Q : kKindLine3);
+    switch (elem_kind) {
+    case PYHIP_ELEM_TRI_P1: return kKindLine2;
+    case PYHIP_ELEM_QUAD_Q2: return kKindLine3Q;
+    case PYHIP_ELEM_HEX_Q2: return kKindQuad9;
+    default: return kKindLine3;
+    }
 }
 
 int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stream,
@@ -119,10 +124,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         PYHIP_REQUIRE(d.ndof >= 0, "negative ndof");
         PYHIP_REQUIRE(d.nface >= 0 && (d.nface == 0 || (d.face_conn && d.face_coef)),
                       "faces without connectivity or flux coefficients");
-        PYHIP_REQUIRE(d.nface == 0 || ((d.elem_kind == PYHIP_ELEM_TRI_P1 || d.elem_kind == PYHIP_ELEM_TRI_P2 ||
-                                        d.elem_kind == PYHIP_ELEM_QUAD_Q2) &&
-                                       d.nvalue == 1),
-                      "boundary-face flux elements: scalar forms on P1 / P2 triangles and Q2 quadrilaterals");
+        PYHIP_REQUIRE(d.nface == 0 || d.nvalue == 1, "boundary-face flux elements: scalar forms only");
         std::pair<int, int> k(d.form, d.elem_kind);
         if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
     }
@@ -138,8 +140,8 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         G.form = gkeys[g].first;
         G.kind = gkeys[g].second;
         if (G.form == kFormFlux) {
-            G.ns = G.nn = G.kind == kKindLine2 ? 2 : 3;
-            G.dim = 2;
+            G.ns = G.nn = G.kind == kKindLine2 ? 2 : (G.kind == kKindQuad9 ? 9 : 3);
+            G.dim = G.kind == kKindQuad9 ? 3 : 2;
             continue;
         }
         G.ns = form_nslot(G.form, G.kind);
@@ -313,10 +315,12 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         UP(G.seqn, seqn_sm[g].data(), seqn_sm[g].size());
         UP(G.sub, sub_g[g].data(), sub_g[g].size());
         if (G.form == kFormFlux) UP(G.coef, coef_g[g].data(), coef_g[g].size());
-        if (G.form != kFormFlux && !P->tab64[G.kind]) {
-            rc = upload_tables<double>(G.kind, &P->tab64[G.kind], B);
+        // the faces of Q2 bricks use the Q2 quadrilateral tables
+        const int tk = G.form != kFormFlux ? G.kind : (G.kind == kKindQuad9 ? (int)ELEM_QUAD_Q2 : -1);
+        if (tk >= 0 && !P->tab64[tk]) {
+            rc = upload_tables<double>(tk, &P->tab64[tk], B);
             if (rc) return rc;
-            rc = upload_tables<float>(G.kind, &P->tab32[G.kind], B);
+            rc = upload_tables<float>(tk, &P->tab32[tk], B);
             if (rc) return rc;
         }
     }
@@ -431,7 +435,7 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.conn = G.conn; a.seqn = G.seqn; a.sub = G.sub;
             a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
             a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
-            a.tab = G.form == kFormFlux ? nullptr : p->tab64[G.kind];
+            a.tab = G.form != kFormFlux ? p->tab64[G.kind] : (G.kind == kKindQuad9 ? p->tab64[ELEM_QUAD_Q2] : nullptr);
             a.Er = p->Er64 + G.er_off;
             a.Ej = flag ? p->Ej64 + G.ej_off : nullptr;
             a.flag = flag; a.deriv = deriv_param >= 0;
@@ -451,7 +455,7 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.conn = G.conn; a.seqn = G.seqn; a.sub = G.sub;
             a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
             a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
-            a.tab = G.form == kFormFlux ? nullptr : p->tab32[G.kind];
+            a.tab = G.form != kFormFlux ? p->tab32[G.kind] : (G.kind == kKindQuad9 ? p->tab32[ELEM_QUAD_Q2] : nullptr);
             a.Er = p->Er32 + G.er_off;
             a.Ej = flag ? p->Ej32 + G.ej_off : nullptr;
             a.flag = flag; a.deriv = deriv_param >= 0;

@@ -1,7 +1,7 @@
 // Boundary-face (interface) elements of the scalar forms: the flux term of pyoomph's
 // NeumannBC(u=flux) @ boundary (pyoomph/meshes/bcs.py:71-89: residual weak(flux, test)),
-// flux = alpha*u + beta with alpha, beta per face, on the faces of P1 / P2 triangles and
-// Q2 quadrilaterals.
+// flux = alpha*u + beta with alpha, beta per face, on the faces of P1 / P2 triangles, Q2
+// quadrilaterals and Q2 bricks.
 //
 // A face element of a triangle is the reference's InterfaceTElementLine1dC1 / C2
 // (src/mesh.cpp:627-635): a TElement<1,2> / TElement<1,3> line (nodes at s = 0, 1 or
@@ -124,6 +124,67 @@ __global__ __launch_bounds__(256) void flux_face_kernel(ElemArgs<T> a) {
     }
 }
 
+// Faces of Q2 bricks: InterfaceElementQuad2dC2 (src/mesh.cpp:650-653), a QElement<2,3>
+// with its Gauss<2,3> scheme and shapes -- the Q2 quadrilateral tables (a.tab), mistyped
+// knot included -- on a face in 3D: t_d = sum_l x_l dpsi_l/ds_d, W = w |t_0 x t_1|. Per
+// face the point weights and fluxes come first; then every residual entry and Jacobian
+// entry sums its nine point terms in point order:
+//   R_i = sum_q W_q (flux_q psi_qi),   J_ij = sum_q W_q ((alpha psi_qj) psi_qi).
+template <typename T>
+__global__ __launch_bounds__(256) void flux_face_quad9_kernel(ElemArgs<T> a) {
+    constexpr int NN = 9, NQ = 9;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= a.nel) return;
+    const T alpha = (T)a.fcoef[2 * (int64_t)e], beta = (T)a.fcoef[2 * (int64_t)e + 1];
+    T X[NN][3], U[NN];
+#pragma unroll
+    for (int l = 0; l < NN; l++) {
+        const int32_t n = a.conn[(int64_t)l * a.nel + e];
+#pragma unroll
+        for (int d = 0; d < 3; d++) X[l][d] = (T)a.coords[(int64_t)n * 3 + d];
+        const int32_t q = a.seqn[(int64_t)l * a.nel + e];
+        U[l] = (T)(q >= 0 ? a.dofs[q] : a.pinned[(int64_t)n * a.nvalue]);
+    }
+    T W[NQ], F[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const T *P = a.tab->psi + q * NN;
+        const T *D = a.tab->dpsi + q * NN * 2;
+        T t0[3] = {T(0), T(0), T(0)}, t1[3] = {T(0), T(0), T(0)}, u = T(0);
+#pragma unroll
+        for (int l = 0; l < NN; l++) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                t0[d] = t0[d] + X[l][d] * D[l * 2];
+                t1[d] = t1[d] + X[l][d] * D[l * 2 + 1];
+            }
+            u = u + U[l] * P[l];
+        }
+        const T n0 = t0[1] * t1[2] - t0[2] * t1[1];
+        const T n1 = t0[2] * t1[0] - t0[0] * t1[2];
+        const T n2 = t0[0] * t1[1] - t0[1] * t1[0];
+        W[q] = a.tab->w[q] * sqrt((n0 * n0 + n1 * n1) + n2 * n2);
+        F[q] = alpha * u + beta;
+    }
+    const bool zero_r = a.deriv != 0;
+    const bool zero_j = a.deriv != 0 || a.flag == 2 || a.hvec != nullptr;
+    for (int i = 0; i < NN; i++) {
+        T r = T(0);
+#pragma unroll
+        for (int q = 0; q < NQ; q++) r = r + W[q] * (F[q] * a.tab->psi[q * NN + i]);
+        a.Er[(int64_t)e * NN + i] = zero_r ? T(0) : r;
+        if (a.Ej) {
+            for (int j = 0; j < NN; j++) {
+                T v = T(0);
+#pragma unroll
+                for (int q = 0; q < NQ; q++)
+                    v = v + W[q] * ((alpha * a.tab->psi[q * NN + j]) * a.tab->psi[q * NN + i]);
+                a.Ej[((int64_t)e * NN + i) * NN + j] = zero_j ? T(0) : v;
+            }
+        }
+    }
+}
+
 template <typename T>
 static int launch_faces(int kind, const ElemArgs<T> &a, hipStream_t st) {
     if (a.nel <= 0) return PYHIP_OK;
@@ -131,6 +192,7 @@ static int launch_faces(int kind, const ElemArgs<T> &a, hipStream_t st) {
     if (kind == kKindLine2) hipLaunchKernelGGL((flux_face_kernel<T, 2, FaceRule<2>>), g, t, 0, st, a);
     else if (kind == kKindLine3) hipLaunchKernelGGL((flux_face_kernel<T, 3, FaceRule<3>>), g, t, 0, st, a);
     else if (kind == kKindLine3Q) hipLaunchKernelGGL((flux_face_kernel<T, 3, FaceRuleQ3>), g, t, 0, st, a);
+    else if (kind == kKindQuad9) hipLaunchKernelGGL((flux_face_quad9_kernel<T>), g, t, 0, st, a);
     else {
         set_error("unknown face element kind");
         return PYHIP_ERR_INVALID;

@@ -19,6 +19,7 @@ constexpr int kFormFlux = 16;          // flux = alpha*u + beta on the faces
 constexpr int kKindLine2 = 16;         // 2-node faces of P1 triangles (TGauss<1,2>)
 constexpr int kKindLine3 = 17;         // 3-node faces of P2 triangles (TGauss<1,3>)
 constexpr int kKindLine3Q = 18;        // 3-node faces of Q2 quadrilaterals (Gauss<1,3>)
+constexpr int kKindQuad9 = 19;         // 9-node faces of Q2 bricks (Gauss<2,3>)
 
 struct SubInfo {
     int form, kind, nvalue;

@@ -68,17 +68,22 @@ class StructuredMesh:
 
 # element edges in local nodes: ends and (P2, Q2) midside node, in face-node order
 _FACE_EDGES = {"tri_p1": ((0, 1), (1, 2), (2, 0)), "tri_p2": ((0, 3, 1), (1, 4, 2), (2, 5, 0)),
-               "quad_q2": ((0, 1, 2), (2, 5, 8), (6, 7, 8), (0, 3, 6))}
+               "quad_q2": ((0, 1, 2), (2, 5, 8), (6, 7, 8), (0, 3, 6)),
+               # Q2 brick faces (local node i + 3j + 9k), face node a + 3b over the face's two axes
+               "hex_q2": ((0, 1, 2, 3, 4, 5, 6, 7, 8), (18, 19, 20, 21, 22, 23, 24, 25, 26),
+                          (0, 1, 2, 9, 10, 11, 18, 19, 20), (6, 7, 8, 15, 16, 17, 24, 25, 26),
+                          (0, 3, 6, 9, 12, 15, 18, 21, 24), (2, 5, 8, 11, 14, 17, 20, 23, 26))}
 
 
 def boundary_faces(mesh, names):
     """Nodes of the bulk faces on the named boundaries, for the interface elements that
     carry a boundary flux (the reference builds one interface element per bulk face on
     the boundary, src/mesh.cpp:615-660): int32 [nface, 2] (P1 triangles: end, end) or
-    [nface, 3] (P2 triangles, Q2 quadrilaterals: end, midside, end); one boundary after the
-    other, bulk elements in element order, edges of each in the order of ``_FACE_EDGES``."""
+    [nface, 3] (P2 triangles, Q2 quadrilaterals: end, midside, end) or [nface, 9] (Q2 bricks:
+    the face's 3x3 node lattice); one boundary after the other, bulk elements in element
+    order, faces of each in the order of ``_FACE_EDGES``."""
     if mesh.kind not in _FACE_EDGES:
-        raise ValueError("boundary-face fluxes are built for P1 / P2 triangle and Q2 quadrilateral meshes")
+        raise ValueError("boundary-face fluxes are built for P1 / P2 triangle, Q2 quadrilateral and Q2 brick meshes")
     E = mesh.elements
     edges = _FACE_EDGES[mesh.kind]
     out = []
@@ -90,10 +95,16 @@ def boundary_faces(mesh, names):
             np.zeros((0, len(edges[0])), dtype=np.int64)
         out.append(f)
     faces = np.ascontiguousarray(np.concatenate(out), dtype=np.int32)
+    c = mesh.coords
     if faces.shape[1] == 3 and len(faces):
-        c = mesh.coords
         if not np.allclose(c[faces[:, 1]], 0.5 * (c[faces[:, 0]] + c[faces[:, 2]]), rtol=0, atol=1e-12):
             raise ValueError("midside node of a boundary face is not at the edge midpoint")
+    if faces.shape[1] == 9 and len(faces):
+        ok = np.allclose(c[faces[:, 1]], 0.5 * (c[faces[:, 0]] + c[faces[:, 2]]), rtol=0, atol=1e-12) and \
+            np.allclose(c[faces[:, 4]], 0.25 * (c[faces[:, 0]] + c[faces[:, 2]] + c[faces[:, 6]] + c[faces[:, 8]]),
+                        rtol=0, atol=1e-12)
+        if not ok:
+            raise ValueError("boundary face nodes of a Q2 brick are not in face-lattice order")
     return faces
 
 

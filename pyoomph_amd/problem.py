@@ -42,8 +42,8 @@ class MeshProblem:
     successive ``DirichletBC(...) @ boundary`` do in pyoomph.
     flux: list of (boundary names, alpha, beta) -- ``NeumannBC(u=alpha*u + beta) @ boundary``
     for each name (pyoomph/meshes/bcs.py:71-89: the residual gains the face integral of
-    (alpha*u + beta) * test), in order, on P1 / P2 triangle and Q2 quadrilateral meshes of
-    the scalar forms; one
+    (alpha*u + beta) * test), in order, on P1 / P2 triangle, Q2 quadrilateral and Q2 brick
+    meshes of the scalar forms; one
     (names, alpha, beta) triple is accepted for a single condition; None: none.
     """
 

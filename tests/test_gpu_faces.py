@@ -143,3 +143,49 @@ def test_several_fluxes_on_one_mesh(gpu, space):
     st = P.solve()
     assert st[0]["converged"]
     assert rel(P.get_current_dofs(), on.newton_solve(op, np.zeros(op.ndof), tol=1e-8)) <= 1e-8
+
+
+@pytest.mark.parametrize("alpha,beta", [(0.0, 0.6), (1.3, -0.4)])
+def test_brick_flux_parity_and_newton(gpu, alpha, beta):
+    """Faces of Q2 bricks (9-node quads in 3D): assembly against the oracle, then Newton."""
+    from pyoomph_amd.equations import BratuEquation
+    from pyoomph_amd.meshes import CuboidBrickMesh
+    from pyoomph_amd.problem import MeshProblem, Problem
+    N, lam = 4, 1.1
+    mp = MeshProblem(CuboidBrickMesh(N), BratuEquation("u", space="C2", lambda_=lam),
+                     dirichlet=[(("left", "bottom"), {0: 0.0})],
+                     flux=[(("right", "top"), alpha, beta), (("front",), 0.5 * alpha, 0.3)])
+    m = om.cuboid_brick_mesh(N)
+    Fa, Fb = of.boundary_faces(m, ["right", "top"]), of.boundary_faces(m, ["front"])
+    op = oa.OracleProblem(m, oa.FORM_BRATU, 1, m.on_boundary({"left", "bottom"})[:, None],
+                          np.zeros((m.nnode, 1)), [lam], faces=np.concatenate([Fa, Fb]),
+                          face_alpha=np.r_[np.full(len(Fa), alpha), np.full(len(Fb), 0.5 * alpha)],
+                          face_beta=np.r_[np.full(len(Fa), beta), np.full(len(Fb), 0.3)])
+    assert np.array_equal(mp.eqn, op.eqn) and np.array_equal(mp.faces, op.faces)
+    P = Problem(mp)
+    x = np.random.default_rng(9).uniform(-0.3, 0.3, P.ndof)
+    P.set_current_dofs(x)
+    R, J = P.assemble_jacobian()
+    r, rs, col, val = op.get_jacobian(x)
+    assert np.array_equal(J.indptr, rs) and np.array_equal(J.indices, col)
+    assert rel(J.data, val) <= 1e-12 and rel(R, r) <= 1e-12
+    P.set_current_dofs(np.zeros(P.ndof))
+    P.krylov, P.preconditioner = "cg", "amg"
+    st = P.solve()
+    assert st[0]["converged"]
+    assert rel(P.get_current_dofs(), on.newton_solve(op, np.zeros(op.ndof), tol=1e-8)) <= 1e-8
+
+
+def test_brick_laplace_with_flux_side_has_linear_solution(gpu):
+    from pyoomph_amd.equations import BratuEquation
+    from pyoomph_amd.meshes import CuboidBrickMesh
+    from pyoomph_amd.problem import MeshProblem, Problem
+    mp = MeshProblem(CuboidBrickMesh(3), BratuEquation("u", space="C2", lambda_=0.0),
+                     dirichlet=[(("left",), {0: 0.0})], flux=[(("right",), 2.0, 0.7)])
+    P = Problem(mp)
+    P.krylov, P.preconditioner = "cg", "jacobi"
+    P.newton_solver_tolerance = 1e-12
+    assert P.solve()[0]["converged"]
+    u = mp.nodal_values(P.get_current_dofs())[:, 0]
+    # the reference's mistyped Gauss<2,3> knot on the faces bounds the exactness (~1e-8)
+    assert np.max(np.abs(u - (-0.7 / 3.0) * mp.mesh.coords[:, 0])) <= 1e-7

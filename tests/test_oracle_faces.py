@@ -140,3 +140,49 @@ def test_several_fluxes_on_one_mesh_superpose():
     assert np.allclose(rb - r0, (rr - r0) + (rt - r0), rtol=1e-12, atol=1e-14)
     assert np.allclose(Jb - J0, (Jr - J0) + (Jt - J0), rtol=1e-12, atol=1e-14)
     assert np.array_equal(Jr, J0)            # a Neumann flux adds nothing to the Jacobian
+
+
+# The reference's Gauss<2,3> (the rule of Q2 quadrilaterals and of the QElement<2,3> faces of
+# Q2 bricks) has a mistyped positive knot (0.774596662941483): its integrals of the exact
+# polynomials are off by up to ~1e-7 relative (entry by entry), which bounds these checks.
+Q9_TOL = 2e-7
+
+
+def test_brick_face_load_and_mass_closed_forms():
+    """a planar square face (side h, tilted out of the coordinate planes): load
+    h^2 (1/6, 2/3, 1/6) x (1/6, 2/3, 1/6), mass h^2 M1 x M1, M1 = [[4,2,-1],[2,16,2],[-1,2,4]]/30"""
+    h, th = 0.4, 0.3
+    X = np.array([[[0.1 + a * h / 2, 0.2 + b * h / 2 * np.cos(th), b * h / 2 * np.sin(th)]
+                   for b in range(3) for a in range(3)]])
+    v = np.array([1 / 6, 2 / 3, 1 / 6])
+    R, J = of.face_outputs_quad9(X, np.zeros((1, 9)), 0.0, 1.0, 1)
+    assert np.allclose(R[0], h * h * np.kron(v, v), rtol=Q9_TOL, atol=0) and np.all(J == 0.0)
+    _, M = of.face_outputs_quad9(X, np.zeros((1, 9)), 1.0, 0.0, 1)
+    M1 = np.array([[4, 2, -1], [2, 16, 2], [-1, 2, 4]]) / 30
+    assert np.allclose(M[0], h * h * np.kron(M1, M1), rtol=0, atol=Q9_TOL * h * h)
+    U = np.random.default_rng(8).standard_normal((1, 9))
+    Rr, Jr = of.face_outputs_quad9(X, U, 1.3, -0.2, 1)
+    R0, _ = of.face_outputs_quad9(X, np.zeros((1, 9)), 1.3, -0.2, 0)
+    assert np.allclose(Rr[0], Jr[0] @ U[0] + R0[0], rtol=1e-13, atol=1e-15)
+
+
+def test_brick_boundary_faces():
+    from pyoomph_amd import meshes as pm
+    N = 3
+    m = om.cuboid_brick_mesh(N)
+    for side in ("left", "right", "bottom", "top", "back", "front"):
+        F = of.boundary_faces(m, [side])
+        assert F.shape == (N * N, 9) and np.all(m.on_boundary([side])[F])
+    assert np.array_equal(pm.boundary_faces(pm.CuboidBrickMesh(N), ["left", "front"]),
+                          of.boundary_faces(m, ["left", "front"]))
+
+
+@pytest.mark.parametrize("alpha,beta", [(0.0, 0.7), (2.0, 0.7)])
+def test_laplace_in_a_cube_with_flux_side_has_linear_solution(alpha, beta):
+    m = om.cuboid_brick_mesh(2)
+    pinned = m.on_boundary({"left"})[:, None]
+    op = oa.OracleProblem(m, oa.FORM_BRATU, 1, pinned, np.zeros((m.nnode, 1)), [0.0],
+                          faces=of.boundary_faces(m, ["right"]), face_alpha=alpha, face_beta=beta)
+    x = on.newton_solve(op, np.zeros(op.ndof), tol=1e-12)
+    u = op.nodal_values(x)[:, 0]
+    assert np.allclose(u, -beta / (1.0 + alpha) * m.coords[:, 0], rtol=0, atol=1e-8)
