@@ -98,12 +98,15 @@ typedef struct {
      * one face list. Scalar forms on P1 / P2 triangles (faces: TElement<1,2> /
      * TElement<1,3> with TGauss<1,2> / TGauss<1,3>, src/elements.hpp:676-730), Q2
      * quadrilaterals (QElement<1,3> with Gauss<1,3>) and Q2 bricks (QElement<2,3> with
-     * Gauss<2,3>). The faces follow the bulk elements in element order. nface = 0: none. */
+     * Gauss<2,3>); for Taylor-Hood the traction NeumannBC(velocity_x=alpha*u_x + beta_x,
+     * velocity_y=alpha*u_y + beta_y) on Q2 quad faces. The faces follow the bulk elements
+     * in element order. nface = 0: none. */
     int64_t nface;
     const int32_t *face_conn;    /* [nface][2] (P1: end, end), [nface][3] (P2, Q2 quads:
                                     end, midside, end) or [nface][9] (Q2 bricks: the
                                     face's 3x3 node lattice) mesh nodes of each face     */
-    const double *face_coef;     /* [nface][2] alpha, beta of each face's flux           */
+    const double *face_coef;     /* [nface][2] alpha, beta of each face's flux (Taylor-
+                                    Hood: [nface][3] alpha, beta_x, beta_y)               */
 } pyhip_subproblem_desc;
 
 typedef struct pyhip_problem pyhip_problem;

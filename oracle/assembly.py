@@ -114,12 +114,15 @@ class OracleProblem:
         self.faces = None
         if faces is not None and len(faces):
             nf = len(faces)
+            nc = 1 if nvalue == 1 else 2          # velocity components of a Taylor-Hood traction
             self.face_alpha = np.broadcast_to(np.asarray(face_alpha, dtype=np.float64), (nf,)).copy()
-            self.face_beta = np.broadcast_to(np.asarray(face_beta, dtype=np.float64), (nf,)).copy()
-            if nvalue != 1:
-                raise ValueError("boundary-face fluxes are built for the scalar forms")
+            fb = np.asarray(face_beta, dtype=np.float64)
+            self.face_beta = np.broadcast_to(fb.reshape(-1, nc) if fb.ndim else fb, (nf, nc)).copy()
+            if nvalue not in (1, 3):
+                raise ValueError("boundary-face fluxes are built for the scalar forms and Taylor-Hood")
             self.faces = np.asarray(faces, dtype=np.int64)
-            fg = self.eqn[self.faces, 0]                      # [nface, nn]
+            # face slots: (node l, component c) = l*nc + c
+            fg = self.eqn[np.repeat(self.faces, nc, axis=1), np.tile(np.arange(nc), self.faces.shape[1])]
             ffree = fg >= 0
             self.face_leqn = np.full(fg.shape, -1, dtype=np.int64)
             self.face_leqn[ffree] = (np.cumsum(ffree, axis=1) - 1)[ffree]
@@ -201,14 +204,22 @@ class OracleProblem:
         from .faces import face_outputs, face_outputs_quad9
         F = self.faces
         nf, nn = F.shape
-        Uf = np.ascontiguousarray(U[F, 0])
+        nc = self.face_beta.shape[1]
+        fl = 1 if flag == 1 else 0
         if nn == 9:
             Xf = np.ascontiguousarray(self.mesh.coords[F][:, :, :3])
-            Rs, Js = face_outputs_quad9(Xf, Uf, self.face_alpha, self.face_beta, 1 if flag == 1 else 0)
+            Rs, Js = face_outputs_quad9(Xf, np.ascontiguousarray(U[F, 0]), self.face_alpha, self.face_beta[:, 0], fl)
         else:
             Xf = np.ascontiguousarray(self.mesh.coords[F][:, :, :2])
-            Rs, Js = face_outputs(Xf, Uf, self.face_alpha, self.face_beta, 1 if flag == 1 else 0,
-                                  quad=self.mesh.kind == "quad_q2")
+            quad = self.mesh.kind == "quad_q2"
+            Rs = np.zeros((nf, nn * nc))
+            Js = np.zeros((nf, nn * nc, nn * nc))
+            for c in range(nc):
+                # each velocity component's rows and columns: slots l*nc + c
+                Rc, Jc = face_outputs(Xf, np.ascontiguousarray(U[F, c]), self.face_alpha, self.face_beta[:, c], fl,
+                                      quad=quad)
+                Rs[:, c::nc] = Rc
+                Js[:, c::nc, c::nc] = Jc
         if deriv:
             Rs[:] = 0.0
         if deriv or flag == 2 or y is not None:

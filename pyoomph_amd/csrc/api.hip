@@ -91,7 +91,8 @@ int pyhip_element_init(int form, int elem_kind, pyhip_element_table *t) {
 }
 
 // face element kind of a bulk element kind
-static int face_kind(int elem_kind) {
+static int face_kind(int elem_kind, int form) {
+    if (form == PYHIP_FORM_NAVIER_STOKES_TH) return kKindLine3TH;
     switch (elem_kind) {
     case PYHIP_ELEM_TRI_P1: return kKindLine2;
     case PYHIP_ELEM_QUAD_Q2: return kKindLine3Q;
@@ -124,14 +125,15 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         PYHIP_REQUIRE(d.ndof >= 0, "negative ndof");
         PYHIP_REQUIRE(d.nface >= 0 && (d.nface == 0 || (d.face_conn && d.face_coef)),
                       "faces without connectivity or flux coefficients");
-        PYHIP_REQUIRE(d.nface == 0 || d.nvalue == 1, "boundary-face flux elements: scalar forms only");
+        PYHIP_REQUIRE(d.nface == 0 || d.nvalue == 1 || d.form == PYHIP_FORM_NAVIER_STOKES_TH,
+                      "boundary-face flux elements: scalar forms and Taylor-Hood tractions");
         std::pair<int, int> k(d.form, d.elem_kind);
         if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
     }
     // face groups after the bulk groups, one per face kind
     for (int s = 0; s < nsub; s++) {
         if (subs[s].nface == 0) continue;
-        std::pair<int, int> k(kFormFlux, face_kind(subs[s].elem_kind));
+        std::pair<int, int> k(kFormFlux, face_kind(subs[s].elem_kind, subs[s].form));
         if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
     }
     P->groups.resize(gkeys.size());
@@ -140,7 +142,8 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         G.form = gkeys[g].first;
         G.kind = gkeys[g].second;
         if (G.form == kFormFlux) {
-            G.ns = G.nn = G.kind == kKindLine2 ? 2 : (G.kind == kKindQuad9 ? 9 : 3);
+            G.nn = G.kind == kKindLine2 ? 2 : (G.kind == kKindQuad9 ? 9 : 3);
+            G.ns = G.kind == kKindLine3TH ? 2 * G.nn : G.nn;
             G.dim = G.kind == kKindQuad9 ? 3 : 2;
             continue;
         }
@@ -167,7 +170,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         S.nel = d.nelem;
         P->groups[S.group].nel += d.nelem;
         if (d.nface > 0) {
-            const std::pair<int, int> fk(kFormFlux, face_kind(d.elem_kind));
+            const std::pair<int, int> fk(kFormFlux, face_kind(d.elem_kind, d.form));
             S.fgroup = (int)(std::find(gkeys.begin(), gkeys.end(), fk) - gkeys.begin());
             S.fel_off = P->groups[S.fgroup].nel;
             S.nface = d.nface;
@@ -199,7 +202,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         conn_sm[g].resize(G.nn * G.nel);
         seqn_sm[g].resize(G.ns * G.nel);
         sub_g[g].resize(G.nel);
-        if (G.form == kFormFlux) coef_g[g].resize(2 * G.nel);
+        if (G.form == kFormFlux) coef_g[g].resize((size_t)face_ncoef(G.kind) * G.nel);
     }
     std::vector<double> coords_h(P->nnode * 3, 0.0), pinned_h(P->nnode * 3, 0.0),
         aux_h(P->nnode, 0.0);
@@ -221,19 +224,22 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         for (int k = 0; k < d.nparams; k++) P->params_h[(size_t)s * kParamStride + k] = d.params[k];
         if (S.nface > 0) {
             const Group &F = P->groups[S.fgroup];
+            const int nc = face_ncoef(F.kind), vps = F.ns / F.nn;   // values per face node
             for (int64_t f = 0; f < d.nface; f++) {
                 const int64_t ge = S.fel_off + f;
                 sub_g[S.fgroup][ge] = s;
-                coef_g[S.fgroup][2 * ge] = d.face_coef[2 * f];
-                coef_g[S.fgroup][2 * ge + 1] = d.face_coef[2 * f + 1];
+                for (int k = 0; k < nc; k++) coef_g[S.fgroup][nc * ge + k] = d.face_coef[nc * f + k];
                 for (int l = 0; l < F.nn; l++) {
                     const int32_t n = d.face_conn[f * F.nn + l];
                     PYHIP_REQUIRE(n >= 0 && n < d.nnode, "face connectivity out of range");
                     conn_sm[S.fgroup][(int64_t)l * F.nel + ge] = (int32_t)(S.node_off + n);
-                    const int32_t q = d.eqn[n];
-                    const int32_t gq = q >= 0 ? (int32_t)(q + S.dof_off) : -1;
-                    seqn_em[S.fgroup][ge * F.ns + l] = gq;
-                    seqn_sm[S.fgroup][(int64_t)l * F.nel + ge] = gq;
+                    for (int c = 0; c < vps; c++) {
+                        const int32_t q = d.eqn[(int64_t)n * d.nvalue + c];
+                        const int32_t gq = q >= 0 ? (int32_t)(q + S.dof_off) : -1;
+                        const int sl = l * vps + c;
+                        seqn_em[S.fgroup][ge * F.ns + sl] = gq;
+                        seqn_sm[S.fgroup][(int64_t)sl * F.nel + ge] = gq;
+                    }
                 }
             }
         }

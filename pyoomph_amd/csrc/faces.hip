@@ -185,6 +185,78 @@ __global__ __launch_bounds__(256) void flux_face_quad9_kernel(ElemArgs<T> a) {
     }
 }
 
+// Traction faces of Taylor-Hood Q2 quadrilaterals: NeumannBC(velocity_x=f_x,
+// velocity_y=f_y) with f_c = alpha*u_c + beta_c (coefficients alpha, beta_x, beta_y per
+// face) on QElement<1,3> faces with Gauss<1,3>; the face's slots are (node l, component c)
+// = 2l + c. Per point, in the order of flux_face_kernel:
+//   R_{lc} += W * (f_c * psi_l),   J_{lc,mc} += W * ((alpha * psi_m) * psi_l).
+template <typename T>
+__global__ __launch_bounds__(256) void traction_face_kernel(ElemArgs<T> a) {
+    constexpr int NN = 3, NS = 6;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= a.nel) return;
+    const T alpha = (T)a.fcoef[3 * (int64_t)e];
+    const T beta[2] = {(T)a.fcoef[3 * (int64_t)e + 1], (T)a.fcoef[3 * (int64_t)e + 2]};
+    T X[NN][2], U[NN][2];
+#pragma unroll
+    for (int l = 0; l < NN; l++) {
+        const int32_t n = a.conn[(int64_t)l * a.nel + e];
+        X[l][0] = (T)a.coords[(int64_t)n * 3];
+        X[l][1] = (T)a.coords[(int64_t)n * 3 + 1];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const int32_t q = a.seqn[(int64_t)(2 * l + c) * a.nel + e];
+            U[l][c] = (T)(q >= 0 ? a.dofs[q] : a.pinned[(int64_t)n * a.nvalue + c]);
+        }
+    }
+    T R[NS], J[NN][NN];
+#pragma unroll
+    for (int i = 0; i < NS; i++) R[i] = T(0);
+#pragma unroll
+    for (int i = 0; i < NN; i++)
+#pragma unroll
+        for (int j = 0; j < NN; j++) J[i][j] = T(0);
+#pragma unroll
+    for (int g = 0; g < NN; g++) {
+        double sd, wd;
+        FaceRuleQ3::point(g, sd, wd);
+        const T sq = (T)sd, wq = (T)wd;
+        T psi[NN], dpsi[NN];
+        FaceRuleQ3::shape(sq, psi, dpsi);
+        T tx = T(0), ty = T(0), u0 = T(0), u1 = T(0);
+#pragma unroll
+        for (int l = 0; l < NN; l++) {
+            tx = tx + X[l][0] * dpsi[l];
+            ty = ty + X[l][1] * dpsi[l];
+            u0 = u0 + U[l][0] * psi[l];
+            u1 = u1 + U[l][1] * psi[l];
+        }
+        const T W = wq * sqrt(tx * tx + ty * ty);
+        const T f[2] = {alpha * u0 + beta[0], alpha * u1 + beta[1]};
+#pragma unroll
+        for (int l = 0; l < NN; l++) {
+#pragma unroll
+            for (int c = 0; c < 2; c++) R[2 * l + c] = R[2 * l + c] + W * (f[c] * psi[l]);
+#pragma unroll
+            for (int m = 0; m < NN; m++) J[l][m] = J[l][m] + W * ((alpha * psi[m]) * psi[l]);
+        }
+    }
+    const bool zero_r = a.deriv != 0;
+    const bool zero_j = a.deriv != 0 || a.flag == 2 || a.hvec != nullptr;
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+        a.Er[(int64_t)e * NS + i] = zero_r ? T(0) : R[i];
+        if (a.Ej) {
+#pragma unroll
+            for (int j = 0; j < NS; j++) {
+                // velocity components do not couple through the face
+                const T v = (i % 2 == j % 2) ? J[i / 2][j / 2] : T(0);
+                a.Ej[((int64_t)e * NS + i) * NS + j] = zero_j ? T(0) : v;
+            }
+        }
+    }
+}
+
 template <typename T>
 static int launch_faces(int kind, const ElemArgs<T> &a, hipStream_t st) {
     if (a.nel <= 0) return PYHIP_OK;
@@ -193,6 +265,7 @@ static int launch_faces(int kind, const ElemArgs<T> &a, hipStream_t st) {
     else if (kind == kKindLine3) hipLaunchKernelGGL((flux_face_kernel<T, 3, FaceRule<3>>), g, t, 0, st, a);
     else if (kind == kKindLine3Q) hipLaunchKernelGGL((flux_face_kernel<T, 3, FaceRuleQ3>), g, t, 0, st, a);
     else if (kind == kKindQuad9) hipLaunchKernelGGL((flux_face_quad9_kernel<T>), g, t, 0, st, a);
+    else if (kind == kKindLine3TH) hipLaunchKernelGGL((traction_face_kernel<T>), g, t, 0, st, a);
     else {
         set_error("unknown face element kind");
         return PYHIP_ERR_INVALID;

@@ -20,6 +20,9 @@ constexpr int kKindLine2 = 16;         // 2-node faces of P1 triangles (TGauss<1
 constexpr int kKindLine3 = 17;         // 3-node faces of P2 triangles (TGauss<1,3>)
 constexpr int kKindLine3Q = 18;        // 3-node faces of Q2 quadrilaterals (Gauss<1,3>)
 constexpr int kKindQuad9 = 19;         // 9-node faces of Q2 bricks (Gauss<2,3>)
+constexpr int kKindLine3TH = 20;       // traction faces of Taylor-Hood Q2 quads (2 slots per node)
+// flux coefficients per face: alpha, beta (scalar forms) or alpha, beta_x, beta_y (Taylor-Hood)
+inline int face_ncoef(int face_kind) { return face_kind == kKindLine3TH ? 3 : 2; }
 
 struct SubInfo {
     int form, kind, nvalue;
@@ -38,7 +41,7 @@ struct Group {
     int32_t *conn = nullptr;          // [nn][nel]
     int32_t *seqn = nullptr;          // [ns][nel]
     int32_t *sub = nullptr;           // [nel]
-    double *coef = nullptr;           // face groups: [nel][2] alpha, beta of each face's flux
+    double *coef = nullptr;           // face groups: [nel][face_ncoef] flux coefficients of each face
 };
 
 struct KrylovWork;   // krylov.hip

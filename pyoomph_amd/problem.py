@@ -43,7 +43,8 @@ class MeshProblem:
     flux: list of (boundary names, alpha, beta) -- ``NeumannBC(u=alpha*u + beta) @ boundary``
     for each name (pyoomph/meshes/bcs.py:71-89: the residual gains the face integral of
     (alpha*u + beta) * test), in order, on P1 / P2 triangle, Q2 quadrilateral and Q2 brick
-    meshes of the scalar forms; one
+    meshes of the scalar forms; for Taylor-Hood, beta = (beta_x, beta_y) and the condition is
+    ``NeumannBC(velocity_x=alpha*u_x + beta_x, velocity_y=alpha*u_y + beta_y)``; one
     (names, alpha, beta) triple is accepted for a single condition; None: none.
     """
 
@@ -77,8 +78,8 @@ class MeshProblem:
         self.faces, self.face_coef = None, None
         if flux is not None:
             from .meshes import boundary_faces
-            if nv != 1:
-                raise ValueError("boundary-face fluxes are built for the scalar forms")
+            if nv not in (1, 3):
+                raise ValueError("boundary-face fluxes are built for the scalar forms and Taylor-Hood")
             if len(flux) == 3 and np.isscalar(flux[1]):
                 flux = [flux]
             faces, coef = [], []
@@ -86,7 +87,9 @@ class MeshProblem:
                 names = [names] if isinstance(names, str) else list(names)
                 f = boundary_faces(mesh, names)
                 faces.append(f)
-                coef.append(np.tile([float(alpha), float(beta)], (len(f), 1)))
+                # Taylor-Hood: beta = (beta_x, beta_y), the traction on the velocity components
+                b = [float(beta)] if nv == 1 else [float(beta[0]), float(beta[1])]
+                coef.append(np.tile([float(alpha)] + b, (len(f), 1)))
             self.faces = np.ascontiguousarray(np.concatenate(faces), dtype=np.int32)
             self.face_coef = np.ascontiguousarray(np.concatenate(coef), dtype=np.float64)
 

@@ -189,3 +189,33 @@ def test_brick_laplace_with_flux_side_has_linear_solution(gpu):
     u = mp.nodal_values(P.get_current_dofs())[:, 0]
     # the reference's mistyped Gauss<2,3> knot on the faces bounds the exactness (~1e-8)
     assert np.max(np.abs(u - (-0.7 / 3.0) * mp.mesh.coords[:, 0])) <= 1e-7
+
+
+@pytest.mark.parametrize("alpha,beta", [(0.0, (0.1, -0.05)), (0.8, (0.0, 0.2))])
+def test_taylor_hood_traction_parity_and_newton(gpu, alpha, beta):
+    """Taylor-Hood with a traction face (NeumannBC on the velocity components) on the right
+    of the driven cavity: assembly against the oracle, then Newton (GMRES + ILU(0))."""
+    from pyoomph_amd.equations import NavierStokesEquations
+    from pyoomph_amd.meshes import RectangularQuadMesh
+    from pyoomph_amd.problem import MeshProblem, Problem
+    from tests.test_oracle_faces import _ns_traction_problem
+    N = 6
+    mp = MeshProblem(RectangularQuadMesh(N, split_in_tris=False, space="C2"),
+                     NavierStokesEquations(dynamic_viscosity=0.05, mass_density=1.0),
+                     dirichlet=[(("top",), {0: 1.0, 1: 0.0}), (("left", "bottom"), {0: 0.0, 1: 0.0})],
+                     pin_extra=[(0, 2, 0.0)], flux=[(("right",), alpha, beta)])
+    _, op = _ns_traction_problem(N, alpha, beta)
+    assert np.array_equal(mp.eqn, op.eqn) and np.array_equal(mp.faces, op.faces)
+    P = Problem(mp)
+    x = np.random.default_rng(4).uniform(-0.3, 0.3, P.ndof)
+    P.set_current_dofs(x)
+    R, J = P.assemble_jacobian()
+    r, rs, col, val = op.get_jacobian(x)
+    assert np.array_equal(J.indptr, rs) and np.array_equal(J.indices, col)
+    assert np.array_equal(J.data, val) and np.array_equal(R, r)
+    P.set_current_dofs(np.zeros(P.ndof))
+    P.krylov, P.preconditioner, P.gmres_restart = "gmres", "ilu0", 100
+    P.newton_solver_tolerance = 1e-10
+    st = P.solve()
+    assert st[0]["converged"]
+    assert rel(P.get_current_dofs(), on.newton_solve(op, np.zeros(op.ndof), tol=1e-10)) <= 1e-8

@@ -186,3 +186,49 @@ def test_laplace_in_a_cube_with_flux_side_has_linear_solution(alpha, beta):
     x = on.newton_solve(op, np.zeros(op.ndof), tol=1e-12)
     u = op.nodal_values(x)[:, 0]
     assert np.allclose(u, -beta / (1.0 + alpha) * m.coords[:, 0], rtol=0, atol=1e-8)
+
+
+def _ns_traction_problem(N, alpha, beta, faces=True):
+    """Taylor-Hood on the unit square: u = (1, 0) on the top, u = 0 on the left and the
+    bottom, the traction alpha*u + beta on the right (NeumannBC on the velocity components),
+    the pressure pinned at node 0 (a one-sided outflow would fix its level anyway)."""
+    m = om.rectangular_quad_mesh(N, False, "C2")
+    nn = m.nnode
+    vert = np.zeros(nn, bool)
+    vert[m.elements[:, [0, 2, 6, 8]].ravel()] = True
+    top, walls = m.on_boundary({"top"}), m.on_boundary({"left", "bottom"})
+    pinned = np.zeros((nn, 3), bool)
+    bc = np.zeros((nn, 3))
+    pinned[top | walls, :2] = True
+    bc[top & ~walls, 0] = 1.0
+    pinned[~vert, 2] = True
+    pinned[0, 2] = True
+    F = of.boundary_faces(m, ["right"]) if faces else None
+    return m, oa.OracleProblem(m, oa.FORM_NS, 3, pinned, bc, [0.05, 1.0], faces=F, face_alpha=alpha,
+                               face_beta=np.asarray(beta, float))
+
+
+def test_taylor_hood_traction_load_closed_form():
+    """alpha = 0: the residual gains the traction load; summed over the free x- (y-) momentum
+    rows it is beta_x (beta_y) times the right side's length less the two corner nodes'
+    shares (h/6 each: the top and bottom corners are Dirichlet nodes)."""
+    N = 4
+    beta = (0.3, -0.2)
+    m, op = _ns_traction_problem(N, 0.0, beta)
+    _, op0 = _ns_traction_problem(N, 0.0, beta, faces=False)
+    x = np.random.default_rng(1).standard_normal(op.ndof) * 0.1
+    d = op.get_residuals(x) - op0.get_residuals(x)
+    for c in range(2):
+        rows = op.eqn[:, c][op.eqn[:, c] >= 0]
+        assert np.isclose(d[rows].sum(), beta[c] * (1.0 - 2 * (1.0 / N) / 6), rtol=1e-12)
+    assert np.allclose(d[op.eqn[:, 2][op.eqn[:, 2] >= 0]], 0.0)     # no pressure rows
+    # alpha != 0: the Jacobian gains alpha times the face mass matrix on each component
+    _, opa = _ns_traction_problem(N, 2.0, beta)
+    _, rs, col, val = opa.get_jacobian(x)
+    _, rs0, col0, val0 = op.get_jacobian(x)
+    import scipy.sparse as sp
+    n = op.ndof
+    D = sp.csr_matrix((val, col, rs), shape=(n, n)).toarray() - sp.csr_matrix((val0, col0, rs0), shape=(n, n)).toarray()
+    ux, uy = op.eqn[:, 0], op.eqn[:, 1]
+    assert np.allclose(D, D.T, atol=1e-15)
+    assert np.all(D[np.ix_(ux[ux >= 0], uy[uy >= 0])] == 0.0)       # components do not couple
