@@ -7,7 +7,7 @@
 
 namespace pyhip {
 
-constexpr int kAggregateCap = 12;      // neighbours an aggregate seed takes at most
+constexpr int kAggregateCap = 16;      // neighbours an aggregate seed takes at most
 constexpr int kBottomRows = 48;        // a sub-problem stops coarsening at this many rows
 constexpr int kBottomMaxRows = 64;     // dense Cholesky size of the coarsest solve (LDS)
 constexpr int kMaxLevels = 16;
