@@ -42,8 +42,13 @@ __global__ __launch_bounds__(256) void multidot_kernel(int nvec, const T *__rest
                                                       int64_t ldv, const T *__restrict__ w,
                                                       const int64_t *__restrict__ cb,
                                                       const int64_t *__restrict__ ce,
-                                                      double *__restrict__ partial, int pstride) {
+                                                      double *__restrict__ partial, int pstride,
+                                                      const int32_t *__restrict__ csub,
+                                                      const int32_t *__restrict__ act) {
     const int64_t c = blockIdx.x;
+    // chunks of sub-problems that stopped (act[csub] == 0) are skipped: their partials are
+    // left as they are and nothing reads their sums
+    if (act && !act[csub[c]]) return;
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int64_t b0 = cb[c], b1 = ce[c];
     for (int jj = q; jj < kDotVecs; jj += 4) {
@@ -513,14 +518,16 @@ static int ensure_basis(KrylovWork *W, int64_t n, int nvec) {
 
 // Per-sub-problem dot products of nvec vectors with w: out[j][sub].
 template <typename T>
+// With act (device [nsub]) the chunks of sub-problems with act[s] == 0 are skipped and
+// their sums are not defined.
 static int batched_dots(pyhip_problem *p, KrylovWork *W, int nvec, const T *V, int64_t ldv,
-                        const T *w, double *out) {
+                        const T *w, double *out, const int32_t *act = nullptr) {
     TimeScope ts(&p->timer, "vector", p->st);
     const int pstride = W->m + 2;
     if (W->nchunk == 0) return PYHIP_OK;
     hipLaunchKernelGGL((multidot_kernel<T>), dim3((unsigned)W->nchunk, (unsigned)((nvec + kDotVecs - 1) / kDotVecs)),
                        dim3(256), 0, p->st, nvec,
-                       V, ldv, w, W->chunk_begin, W->chunk_end, W->partial, pstride);
+                       V, ldv, w, W->chunk_begin, W->chunk_end, W->partial, pstride, W->chunk_sub, act);
     KCHECK();
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)(nvec * W->nsub)), dim3(64), 0, p->st, nvec,
                        W->nsub, W->sub_chunk_ptr, W->partial, pstride, out, 0);
@@ -922,9 +929,12 @@ __global__ __launch_bounds__(256) void dot2_kernel(const T *__restrict__ a, cons
                                                    const T *__restrict__ c, const T *__restrict__ d,
                                                    const int64_t *__restrict__ cb,
                                                    const int64_t *__restrict__ ce,
-                                                   double *__restrict__ partial, int pstride) {
+                                                   double *__restrict__ partial, int pstride,
+                                                   const int32_t *__restrict__ csub,
+                                                   const int32_t *__restrict__ act) {
     __shared__ double r0[256], r1[256];
     const int64_t ch = blockIdx.x;
+    if (act && !act[csub[ch]]) return;   // a stopped sub-problem's chunk (whole block)
     double s0 = 0.0, s1 = 0.0;
     for (int64_t i = cb[ch] + threadIdx.x; i < ce[ch]; i += 256) {
         s0 += double(a[i]) * double(b[i]);
@@ -1024,12 +1034,12 @@ __global__ __launch_bounds__(256) void cg_p_kernel(int64_t n, const T *__restric
 
 template <typename T>
 static int batched_dot2(pyhip_problem *p, KrylovWork *W, const T *a, const T *b, const T *c,
-                        const T *d, double *out) {
+                        const T *d, double *out, const int32_t *act = nullptr) {
     TimeScope ts(&p->timer, "vector", p->st);
     const int pstride = W->m + 2;
     if (W->nchunk == 0) return PYHIP_OK;
     hipLaunchKernelGGL((dot2_kernel<T>), dim3((unsigned)W->nchunk), dim3(256), 0, p->st, a, b, c, d,
-                       W->chunk_begin, W->chunk_end, W->partial, pstride);
+                       W->chunk_begin, W->chunk_end, W->partial, pstride, W->chunk_sub, act);
     KCHECK();
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)(2 * W->nsub)), dim3(64), 0, p->st, 2,
                        W->nsub, W->sub_chunk_ptr, W->partial, pstride, out, 0);
@@ -1091,7 +1101,7 @@ static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *
     int it = 0;
     while (nact > 0 && it < o.max_iters) {
         if ((rc = spmv<TV, T, T>(p, vals, pv, qv, (const T *)nullptr, W->active, act_h))) return rc;
-        if ((rc = batched_dots<T>(p, W, 1, pv, n, qv, d2 + 2 * ns))) return rc;
+        if ((rc = batched_dots<T>(p, W, 1, pv, n, qv, d2 + 2 * ns, W->active))) return rc;
         hipLaunchKernelGGL(cg_alpha_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2 + 2 * ns,
                            W->cg_pq, W->active, W->nfail_d);
         {
@@ -1101,7 +1111,7 @@ static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *
         }
         KCHECK();
         if ((rc = precond_apply<T>(p, W, o.precond, r, z, W->active, act_h, x0 != nullptr))) return rc;
-        if ((rc = batched_dot2<T>(p, W, r, r, r, z, d2))) return rc;
+        if ((rc = batched_dot2<T>(p, W, r, r, r, z, d2, W->active))) return rc;
         PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
         hipLaunchKernelGGL(cg_check_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2, W->bnorm, rtol,
                            W->cg_rz, W->cg_beta, W->active, W->nactive_d);
