@@ -860,19 +860,21 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
             if ((rc = precond_apply<T>(p, W, o.precond, vk, t))) return rc;
             if ((rc = spmv<TV, T, T>(p, vals, t, w, (const T *)nullptr))) return rc;
             // CGS2
-            if ((rc = batched_dots<T>(p, W, k + 1, V, n, w, d1))) return rc;
+            // (the Gram-Schmidt dots skip the sub-problems that have stopped: every kernel
+            // that reads them skips those too)
+            if ((rc = batched_dots<T>(p, W, k + 1, V, n, w, d1, W->active))) return rc;
             {
                 TimeScope tsu(&p->timer, "vector", p->st);
                 hipLaunchKernelGGL((cgs_update_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, k + 1, ns, V,
                                    n, d1, p->dof_sub, W->active, w);
             }
-            if ((rc = batched_dots<T>(p, W, k + 1, V, n, w, d2))) return rc;
+            if ((rc = batched_dots<T>(p, W, k + 1, V, n, w, d2, W->active))) return rc;
             {
                 TimeScope tsu(&p->timer, "vector", p->st);
                 hipLaunchKernelGGL((cgs_update_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, k + 1, ns, V,
                                    n, d2, p->dof_sub, W->active, w);
             }
-            if ((rc = batched_dots<T>(p, W, 1, w, n, w, d3))) return rc;
+            if ((rc = batched_dots<T>(p, W, 1, w, n, w, d3, W->active))) return rc;
             {
                 TimeScope tsu(&p->timer, "vector", p->st);
                 hipLaunchKernelGGL((scale_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, w, d3, p->dof_sub,
