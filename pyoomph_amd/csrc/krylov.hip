@@ -841,7 +841,12 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
                            W->bnorm, rtol, stag, cyc == 0 ? 1 : 0, need, W->g, W->active, W->kcols, W->beta0,
                            W->nactive_d);
         KCHECK();
+        // the running flags come to the host with the count: between reads they only lose
+        // sub-problems, so the host copy stays a superset (it sizes the byte counts of the
+        // filtered launches; the device flags decide which rows are computed)
+        int32_t *act_h = W->act_h;
         PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+        PYHIP_CHECK_HIP(hipMemcpyAsync(act_h, W->active, ns * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
         PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
         if (debug_enabled()) {
             double b0 = 0, bn = 0;
@@ -857,8 +862,10 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
         int k = 0;
         for (; k < m; k++) {
             T *vk = V + (int64_t)k * n;
-            if ((rc = precond_apply<T>(p, W, o.precond, vk, t))) return rc;
-            if ((rc = spmv<TV, T, T>(p, vals, t, w, (const T *)nullptr))) return rc;
+            // the preconditioner (multigrid) and the spmv skip the rows of stopped
+            // sub-problems: nothing reads their t and w
+            if ((rc = precond_apply<T>(p, W, o.precond, vk, t, W->active, act_h))) return rc;
+            if ((rc = spmv<TV, T, T>(p, vals, t, w, (const T *)nullptr, W->active, act_h))) return rc;
             // CGS2
             // (the Gram-Schmidt dots skip the sub-problems that have stopped: every kernel
             // that reads them skips those too)
@@ -892,6 +899,7 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
             if ((k + 1) % kGmresCheck == 0 || k + 1 == m) {
                 PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost,
                                                p->st));
+                PYHIP_CHECK_HIP(hipMemcpyAsync(act_h, W->active, ns * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
                 PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
                 if (*W->nactive_h == 0) break;
             }
