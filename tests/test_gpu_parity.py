@@ -46,7 +46,12 @@ def test_assembly_parity(gpu, kind, kw):
         assert rel(J.data, val) <= 1e-12
         assert rel(R, r) <= 1e-12
         if kind != "bratu":
-            assert np.array_equal(J.data, val) and np.array_equal(R, r)
+            # no transcendental function in the Jacobians: bitwise
+            assert np.array_equal(J.data, val)
+        if kind not in ("bratu", "poisson"):
+            # the Poisson residual holds exp of the Gaussian sources, where device exp and
+            # glibc exp may round differently; everything else is bitwise
+            assert np.array_equal(R, r)
         assert np.array_equal(P.get_residuals(), R)
 
 
