@@ -222,6 +222,12 @@ typedef struct {
     double relaxation;               /* Relaxation_factor (1.0)                        */
     int jacobian_precision;          /* element kernels for J: PYHIP_F64 / PYHIP_F32   */
     pyhip_linear_opts linear;
+    /* 0 (default): the reference's test max|R| < tol alone. > 0: a sub-problem also needs
+     * max|dx| <= correction_tol * max|x| of its last Newton update (FP64 norms). With an
+     * FP32 Jacobian Newton converges linearly, and max|R| < tol does not bound the error
+     * of the solution; the update does (error ~ contraction * |dx|). No counterpart in
+     * the reference, whose Jacobians are FP64.                                         */
+    double correction_tol;
 } pyhip_newton_opts;
 typedef struct {
     int newton_iters;       /* Newton steps taken                                     */
@@ -264,6 +270,12 @@ typedef struct {
     double parameter;            /* out: the parameter after the step                   */
     int newton_iters;            /* out: Newton iterations of the accepted solve        */
     int rejected;                /* out: retries with a shorter step in this call      */
+    int linear_failed;           /* out: 1 if a linear solve of this sub-problem ended
+                                    above its tolerance; the step is then not taken (its
+                                    dofs, parameter and state are kept) and the call
+                                    returns PYHIP_ERR_NOT_CONVERGED after finishing the
+                                    other sub-problems (the reference throws,
+                                    problem.cc:10905-10915)                              */
 } pyhip_arclength_state;
 int pyhip_arclength_step(pyhip_problem *p, double *dofs_dev, int param, const pyhip_newton_opts *opts,
                          const int32_t *active, const double *ds, pyhip_arclength_state *state,

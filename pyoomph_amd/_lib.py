@@ -64,7 +64,7 @@ class NewtonOpts(ctypes.Structure):
                 ("max_residuals", ctypes.c_double),
                 ("always_take_one_newton_step", ctypes.c_int),
                 ("relaxation", ctypes.c_double), ("jacobian_precision", ctypes.c_int),
-                ("linear", LinearOpts)]
+                ("linear", LinearOpts), ("correction_tol", ctypes.c_double)]
 
 
 class NewtonStats(ctypes.Structure):
@@ -78,7 +78,8 @@ class ArclengthState(ctypes.Structure):
     _fields_ = [("theta_squared", ctypes.c_double), ("parameter_derivative", ctypes.c_double),
                 ("parameter_current", ctypes.c_double), ("ds_current", ctypes.c_double),
                 ("step_taken", ctypes.c_int), ("parameter", ctypes.c_double),
-                ("newton_iters", ctypes.c_int), ("rejected", ctypes.c_int)]
+                ("newton_iters", ctypes.c_int), ("rejected", ctypes.c_int),
+                ("linear_failed", ctypes.c_int)]
 
 
 # every symbol declared in include/pyoomph_hip.h, with (restype, argtypes)
@@ -96,6 +97,11 @@ SIGNATURES = {
     "pyhip_pattern_copy": (ctypes.c_int, [ctypes.c_void_p, c_i32p, c_i32p]),
     "pyhip_assemble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "pyhip_jit_element_init": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "pyhip_jit_set_form_params": (ctypes.c_int, [ctypes.c_void_p, c_dp, ctypes.c_int]),
+    "pyhip_superlu": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)] * 4 + [c_dp] + [ctypes.POINTER(ctypes.c_int)] * 2 +
+                      [c_dp] + [ctypes.POINTER(ctypes.c_int)] * 3 + [ctypes.POINTER(ctypes.c_void_p),
+                                                                     ctypes.POINTER(ctypes.c_int)]),
     "pyhip_export_reference_csr": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                   ctypes.c_void_p, ctypes.c_void_p, c_i64p]),
     "pyhip_linear_opts_default": (None, [ctypes.POINTER(LinearOpts)]),

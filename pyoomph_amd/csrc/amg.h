@@ -33,6 +33,10 @@ struct AmgDevLevel {
     // spmv variant (spmv.h) of the level's launches (A_l, restriction P^T, prolongation P):
     // 0 until tuned (all variants give the same bits)
     int lpr_a = 0, lpr_t = 0, lpr_p = 0;
+    // FP32 V-cycle (amg_make_f32): values rounded from the FP64 set-up, FP32 work vectors
+    const float *vals32 = nullptr;     // A_l values (level 0: the caller's FP32 matrix)
+    float *own_vals32 = nullptr, *w32 = nullptr, *pv32 = nullptr, *ptv32 = nullptr;
+    float *b32 = nullptr, *x32 = nullptr, *x2_32 = nullptr, *r32 = nullptr;
 };
 
 struct AmgDev {
@@ -44,6 +48,7 @@ struct AmgDev {
     double *nrm = nullptr;             // [2*nsub] power-iteration norms
     double *chol = nullptr;            // [nsub][kBottomMaxRows^2] coarsest-level Cholesky factors
     int32_t *chol_fail = nullptr;      // [nsub] 1 if a pivot was not positive
+    float *chol32 = nullptr;           // FP32 copy of the coarsest inverses
 };
 
 // plan (host) and device buffers, once per pattern; sizes counted in `bytes`
@@ -65,6 +70,13 @@ int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer
 // the finest level's smoothing weights W and first-sweep vector (nullptr without levels)
 inline const double *amg_fine_w(const AmgDev &A) { return A.lv.empty() ? nullptr : A.lv[0].w; }
 inline double *amg_fine_x(const AmgDev &A) { return A.lv.empty() ? nullptr : A.lv[0].x; }
+// FP32 V-cycle: after amg_setup (FP64), FP32 copies of every level's values, weights, P,
+// P^T and the coarsest inverses; then the finest FP32 matrix and z = M^-1 b in FP32 (the
+// same cycle as amg_apply, each operation rounded to FP32).
+int amg_make_f32(AmgDev &A, hipStream_t st, int64_t &bytes);
+int amg_set_fine32(AmgDev &A, const float *vals);
+int amg_apply32(AmgDev &A, hipStream_t st, const float *b, float *z, KernelTimer *tm = nullptr,
+                const int32_t *act = nullptr, const int32_t *act_h = nullptr);
 void amg_free(AmgDev &A);
 
 }  // namespace pyhip

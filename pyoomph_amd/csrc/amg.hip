@@ -228,8 +228,9 @@ constexpr int kQLanes = 16, kQCap = 64, kCLanes = 32, kCCap = 128;
 
 // x = W b
 // (rows of sub-problems with act[row_sub] == 0 are skipped when act is given)
-__global__ __launch_bounds__(256) void amg_wb_kernel(int64_t n, const double *__restrict__ w,
-                                                     const double *__restrict__ b, double *__restrict__ x,
+template <typename T>
+__global__ __launch_bounds__(256) void amg_wb_kernel(int64_t n, const T *__restrict__ w,
+                                                     const T *__restrict__ b, T *__restrict__ x,
                                                      const int32_t *__restrict__ rsub,
                                                      const int32_t *__restrict__ act) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -315,24 +316,32 @@ __global__ __launch_bounds__(kCholThreads) void amg_chol_factor_kernel(const int
 // x = X b on the listed sub-problems' coarsest rows (X the symmetric inverse above): thread
 // i sums X[j][i] b[j] over j in increasing order (X[j][i] = X[i][j]; the reads of one j are
 // contiguous over the threads).
+template <typename T>
 __global__ __launch_bounds__(kCholThreads) void amg_chol_solve_kernel(const int32_t *__restrict__ subs,
                                                                       const int64_t *__restrict__ sub_off,
-                                                                      const double *__restrict__ cinv,
-                                                                      const double *b, double *x,
+                                                                      const T *__restrict__ cinv,
+                                                                      const T *b, T *x,
                                                                       const int32_t *__restrict__ act) {
-    __shared__ double sb[kBottomMaxRows];
+    __shared__ T sb[kBottomMaxRows];
     const int s = subs[blockIdx.x], t = threadIdx.x;
     if (act && !act[s]) return;   // the whole workgroup leaves together
     const int64_t r0 = sub_off[s];
     const int m = (int)(sub_off[s + 1] - r0);
-    const double *X = cinv + (int64_t)s * kBottomMaxRows * kBottomMaxRows;
+    const T *X = cinv + (int64_t)s * kBottomMaxRows * kBottomMaxRows;
     if (t < m) sb[t] = b[r0 + t];
     __syncthreads();
     if (t < m) {
-        double acc = 0.0;
-        for (int j = 0; j < m; j++) acc = __builtin_fma(X[j * kBottomMaxRows + t], sb[j], acc);
+        T acc = T(0);
+        for (int j = 0; j < m; j++) acc = fmadd(X[j * kBottomMaxRows + t], sb[j], acc);
         x[r0 + t] = acc;
     }
+}
+
+// FP32 copy of FP64 values (round to nearest)
+__global__ __launch_bounds__(256) void amg_to_f32_kernel(int64_t n, const double *__restrict__ a,
+                                                         float *__restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
 }
 
 // ---- host ------------------------------------------------------------------------------------
@@ -358,7 +367,8 @@ void amg_free(AmgDev &A) {
     auto fl = [](AmgDevLevel &L) {
         void *ptrs[] = {L.rs, L.col, L.diag, L.agg, L.prs, L.pcol, L.pcptr, L.pcidx, L.trs, L.trow, L.tsrc,
                         L.qrs, L.qcol, L.row_sub, L.sub_off, L.own_vals, L.d, L.w, L.pv, L.qv, L.b, L.x,
-                        L.x2, L.r, L.pw, L.py, L.cont, L.ends, L.ptv, L.prow, L.tinv};
+                        L.x2, L.r, L.pw, L.py, L.cont, L.ends, L.ptv, L.prow, L.tinv, L.own_vals32, L.w32,
+                        L.pv32, L.ptv32, L.b32, L.x32, L.x2_32, L.r32};
         for (void *q : ptrs) dfree(q);
         L = AmgDevLevel();
     };
@@ -367,6 +377,7 @@ void amg_free(AmgDev &A) {
     dfree(A.nrm);
     dfree(A.chol);
     dfree(A.chol_fail);
+    dfree(A.chol32);
     A = AmgDev();
 }
 
@@ -460,31 +471,33 @@ static inline int lanes_for(int64_t nnz, int64_t n) { return (n > 0 && nnz <= 12
 enum { kSpmvPlain, kSpmvSub, kSpmvAdd, kSpmvJacobi, kSpmvScale };
 // Rows filtered by (rsub, act): the bytes count the rows, entries and input entries of the
 // visited sub-problems (nnz, rows, ncols are those counts).
+template <typename T>
 static void level_spmv(KernelTimer *tm, int mode, int lpr, hipStream_t st, int64_t n, int64_t nnz, int64_t rows,
-                       int64_t ncols, const int32_t *rs, const int32_t *col, const double *vals, const double *x,
-                       double *y, const double *b, const int32_t *rsub = nullptr, const int32_t *act = nullptr,
-                       const double *w = nullptr, double *out = nullptr) {
+                       int64_t ncols, const int32_t *rs, const int32_t *col, const T *vals, const T *x,
+                       T *y, const T *b, const int32_t *rsub = nullptr, const int32_t *act = nullptr,
+                       const T *w = nullptr, T *out = nullptr) {
     TimeScope ts(tm, "spmv", st);
-    int64_t vec = 8 * rows;   // y
+    constexpr int64_t E = sizeof(T);
+    int64_t vec = E * rows;   // y
     switch (mode) {
     case kSpmvAdd:
-        launch_spmv_epi<double, double, double, kEpiAdd>(lpr, st, n, rs, col, vals, x, y, b, w, out, rsub, act);
-        vec = 16 * rows;
+        launch_spmv_epi<T, T, T, kEpiAdd>(lpr, st, n, rs, col, vals, x, y, b, w, out, rsub, act);
+        vec = 2 * E * rows;
         break;
     case kSpmvJacobi:
-        launch_spmv_epi<double, double, double, kEpiJacobi>(lpr, st, n, rs, col, vals, x, y, b, w, out, rsub, act);
-        vec = 32 * rows;   // b, w, own x, out
+        launch_spmv_epi<T, T, T, kEpiJacobi>(lpr, st, n, rs, col, vals, x, y, b, w, out, rsub, act);
+        vec = 4 * E * rows;   // b, w, own x, out
         break;
     case kSpmvScale:
-        launch_spmv_epi<double, double, double, kEpiScale>(lpr, st, n, rs, col, vals, x, y, b, w, out, rsub, act);
-        vec = 24 * rows;   // y, w, out
+        launch_spmv_epi<T, T, T, kEpiScale>(lpr, st, n, rs, col, vals, x, y, b, w, out, rsub, act);
+        vec = 3 * E * rows;   // y, w, out
         break;
     default:
-        launch_spmv_epi<double, double, double, kEpiPlain>(lpr, st, n, rs, col, vals, x, y,
-                                                           mode == kSpmvSub ? b : nullptr, w, out, rsub, act);
-        if (mode == kSpmvSub) vec = 16 * rows;
+        launch_spmv_epi<T, T, T, kEpiPlain>(lpr, st, n, rs, col, vals, x, y, mode == kSpmvSub ? b : nullptr, w, out,
+                                            rsub, act);
+        if (mode == kSpmvSub) vec = 2 * E * rows;
     }
-    if (tm) tm->add_work("spmv", 12 * nnz + 4 * (rows + 1) + 8 * ncols + vec);
+    if (tm) tm->add_work("spmv", (E + 4) * nnz + 4 * (rows + 1) + E * ncols + vec);
 }
 
 // sum of v[s] over the sub-problems with act_h[s] != 0 (all without act_h)
@@ -556,7 +569,7 @@ static int level_weights(hipStream_t st, AmgDevLevel &L, const double *vals, dou
     hipLaunchKernelGGL(amg_diag_kernel, dim3(ablk(n)), dim3(256), 0, st, n, L.diag, vals, L.d);
     hipLaunchKernelGGL(amg_power_start_kernel, dim3(ablk(n)), dim3(256), 0, st, n, L.row_sub, L.sub_off, L.pw);
     for (int it = 0; it < kPowerIters; it++) {
-        level_spmv(tm, kSpmvPlain, lpr, st, n, L.nnz, n, n, L.rs, L.col, vals, L.pw, L.py, nullptr);
+        level_spmv<double>(tm, kSpmvPlain, lpr, st, n, L.nnz, n, n, L.rs, L.col, vals, L.pw, L.py, nullptr);
         hipLaunchKernelGGL(amg_sub_norms_kernel, dim3(nsub), dim3(256), 0, st, L.sub_off, L.d, L.py, L.pw, nrm);
         if (it + 1 < kPowerIters)
             hipLaunchKernelGGL(amg_power_scale_kernel, dim3(ablk(n)), dim3(256), 0, st, n, L.row_sub, nrm, L.d, L.py,
@@ -601,6 +614,19 @@ int amg_setup(AmgDev &A, hipStream_t st, const double *vals0, KernelTimer *tm) {
         hipLaunchKernelGGL(amg_chol_factor_kernel, dim3(A.bot.nends), dim3(kCholThreads), 0, st, A.bot.ends,
                            A.bot.sub_off, A.bot.rs, A.bot.col, A.bot.vals, A.chol, A.chol_fail);
     ACHECK();
+    // A coarsest matrix that is not positive definite (an indefinite Jacobian, e.g. past a
+    // fold) makes the V-cycle an invalid preconditioner: report it instead of iterating on.
+    std::vector<int32_t> fail_h(std::max(1, A.nsub), 0);
+    PYHIP_CHECK_HIP(hipMemcpyAsync(fail_h.data(), A.chol_fail, fail_h.size() * sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, st));
+    PYHIP_CHECK_HIP(hipStreamSynchronize(st));
+    for (int s = 0; s < A.nsub; s++)
+        if (fail_h[s]) {
+            set_error("multigrid: the coarsest matrix of sub-problem " + std::to_string(s) +
+                      " is not positive definite (indefinite Jacobian, e.g. past a fold); use the "
+                      "ILU(0) or Jacobi preconditioner there");
+            return PYHIP_ERR_BREAKDOWN;
+        }
     if (!A.lv.empty() && !A.lv[0].lpr_a && (rc = tune_levels(A, st))) return rc;
     return PYHIP_OK;
 }
@@ -615,15 +641,43 @@ int amg_set_fine(AmgDev &A, const double *vals) {
     return PYHIP_OK;
 }
 
-// z = M^-1 b (both [n0])
-int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer *tm, const int32_t *act,
-              const int32_t *act_h, bool x0_ready) {
+// Per-precision views of a level's values and work vectors (the FP64 hierarchy, or its
+// FP32 copy made by amg_make_f32).
+template <typename T> struct AmgView;
+template <> struct AmgView<double> {
+    static const double *vals(const AmgDevLevel &L) { return L.vals; }
+    static const double *w(const AmgDevLevel &L) { return L.w; }
+    static const double *pv(const AmgDevLevel &L) { return L.pv; }
+    static const double *ptv(const AmgDevLevel &L) { return L.ptv; }
+    static double *b(AmgDevLevel &L) { return L.b; }
+    static double *&x(AmgDevLevel &L) { return L.x; }
+    static double *&x2(AmgDevLevel &L) { return L.x2; }
+    static double *r(AmgDevLevel &L) { return L.r; }
+    static const double *chol(const AmgDev &A) { return A.chol; }
+};
+template <> struct AmgView<float> {
+    static const float *vals(const AmgDevLevel &L) { return L.vals32; }
+    static const float *w(const AmgDevLevel &L) { return L.w32; }
+    static const float *pv(const AmgDevLevel &L) { return L.pv32; }
+    static const float *ptv(const AmgDevLevel &L) { return L.ptv32; }
+    static float *b(AmgDevLevel &L) { return L.b32; }
+    static float *&x(AmgDevLevel &L) { return L.x32; }
+    static float *&x2(AmgDevLevel &L) { return L.x2_32; }
+    static float *r(AmgDevLevel &L) { return L.r32; }
+    static const float *chol(const AmgDev &A) { return A.chol32; }
+};
+
+// z = M^-1 b (both [n0]) in precision T
+template <typename T>
+static int amg_apply_t(AmgDev &A, hipStream_t st, const T *b, T *z, KernelTimer *tm, const int32_t *act,
+                       const int32_t *act_h, bool x0_ready) {
+    using V = AmgView<T>;
     const int nl = (int)A.lv.size();
     AmgDevLevel &Bt = A.bot;
     if (nl == 0) {
         if (Bt.nends)
-            hipLaunchKernelGGL(amg_chol_solve_kernel, dim3(Bt.nends), dim3(kCholThreads), 0, st, Bt.ends, Bt.sub_off,
-                               A.chol, b, z, act);
+            hipLaunchKernelGGL((amg_chol_solve_kernel<T>), dim3(Bt.nends), dim3(kCholThreads), 0, st, Bt.ends,
+                               Bt.sub_off, V::chol(A), b, z, act);
         ACHECK();
         return PYHIP_OK;
     }
@@ -640,45 +694,103 @@ int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer
     // then overwritten by their coarsest solve.
     for (int l = 0; l < nl; l++) {
         AmgDevLevel &L = A.lv[l];
-        const double *bl = l == 0 ? b : L.b;
+        const T *bl = l == 0 ? b : V::b(L);
         AmgDevLevel &C = (l + 1 < nl) ? A.lv[l + 1] : A.bot;
         // x = W b (on coarser levels the restriction above has written it)
         if (l == 0 && !x0_ready)
-            hipLaunchKernelGGL(amg_wb_kernel, dim3(ablk(L.n)), dim3(256), 0, st, L.n, L.w, bl, L.x, L.row_sub, act);
+            hipLaunchKernelGGL((amg_wb_kernel<T>), dim3(ablk(L.n)), dim3(256), 0, st, L.n, V::w(L), bl, V::x(L),
+                               L.row_sub, act);
         const int la = L.lpr_a ? L.lpr_a : lanes_for(L.nnz, L.n);
-        level_spmv(tm, kSpmvSub, la, st, L.n, nnz[l], rows[l], rows[l], L.rs, L.col, L.vals, L.x, L.r, bl,
-                   L.row_sub, act);
+        level_spmv<T>(tm, kSpmvSub, la, st, L.n, nnz[l], rows[l], rows[l], L.rs, L.col, V::vals(L), V::x(L),
+                      V::r(L), bl, L.row_sub, act);
         // b_c = P^T r, and x_c = W_c b_c where the coarse level is not the coarsest
         if (L.nc && l + 1 < nl)
-            level_spmv(tm, kSpmvScale, L.lpr_t ? L.lpr_t : 8, st, L.nc, nnzp[l], rows[l + 1], rows[l], L.trs, L.trow,
-                       L.ptv, L.r, C.b, nullptr, C.row_sub, act, C.w, C.x);
+            level_spmv<T>(tm, kSpmvScale, L.lpr_t ? L.lpr_t : 8, st, L.nc, nnzp[l], rows[l + 1], rows[l], L.trs,
+                          L.trow, V::ptv(L), V::r(L), V::b(C), nullptr, C.row_sub, act, V::w(C), V::x(C));
         else if (L.nc)
-            level_spmv(tm, kSpmvPlain, L.lpr_t ? L.lpr_t : 8, st, L.nc, nnzp[l], rows[l + 1], rows[l], L.trs, L.trow,
-                       L.ptv, L.r, C.b, nullptr, C.row_sub, act);
+            level_spmv<T>(tm, kSpmvPlain, L.lpr_t ? L.lpr_t : 8, st, L.nc, nnzp[l], rows[l + 1], rows[l], L.trs,
+                          L.trow, V::ptv(L), V::r(L), V::b(C), nullptr, C.row_sub, act);
     }
     if (Bt.nends)
-        hipLaunchKernelGGL(amg_chol_solve_kernel, dim3(Bt.nends), dim3(kCholThreads), 0, st, Bt.ends, Bt.sub_off,
-                           A.chol, Bt.b, Bt.x, act);
+        hipLaunchKernelGGL((amg_chol_solve_kernel<T>), dim3(Bt.nends), dim3(kCholThreads), 0, st, Bt.ends,
+                           Bt.sub_off, V::chol(A), V::b(Bt), V::x(Bt), act);
     // up: coarse correction and post-sweep; sub-problems ending on a level get their
     // coarsest solve there
     for (int l = nl - 1; l >= 0; l--) {
         AmgDevLevel &L = A.lv[l];
-        const double *bl = l == 0 ? b : L.b;
+        const T *bl = l == 0 ? b : V::b(L);
         AmgDevLevel &C = (l + 1 < nl) ? A.lv[l + 1] : A.bot;
         if (L.nc)
-            level_spmv(tm, kSpmvAdd, L.lpr_p ? L.lpr_p : 4, st, L.n, nnzp[l], rows[l], rows[l + 1], L.prs, L.pcol,
-                       L.pv, C.x, L.x, L.x, L.row_sub, act);
+            level_spmv<T>(tm, kSpmvAdd, L.lpr_p ? L.lpr_p : 4, st, L.n, nnzp[l], rows[l], rows[l + 1], L.prs, L.pcol,
+                          V::pv(L), V::x(C), V::x(L), V::x(L), L.row_sub, act);
         // post-sweep out = x + W (b - A x) in the spmv's epilogue
-        double *out = l == 0 ? z : L.x2;
-        level_spmv(tm, kSpmvJacobi, L.lpr_a ? L.lpr_a : lanes_for(L.nnz, L.n), st, L.n, nnz[l], rows[l], rows[l],
-                   L.rs, L.col, L.vals, L.x, nullptr, bl, L.row_sub, act, L.w, out);
-        if (l > 0) std::swap(L.x, L.x2);   // the level's result is in L.x for the prolongation above
+        T *out = l == 0 ? z : V::x2(L);
+        level_spmv<T>(tm, kSpmvJacobi, L.lpr_a ? L.lpr_a : lanes_for(L.nnz, L.n), st, L.n, nnz[l], rows[l], rows[l],
+                      L.rs, L.col, V::vals(L), V::x(L), nullptr, bl, L.row_sub, act, V::w(L), out);
+        if (l > 0) std::swap(V::x(L), V::x2(L));   // the level's result is in x for the prolongation above
         if (L.nends)
-            hipLaunchKernelGGL(amg_chol_solve_kernel, dim3(L.nends), dim3(kCholThreads), 0, st, L.ends, L.sub_off,
-                               A.chol, bl, l == 0 ? z : L.x, act);
+            hipLaunchKernelGGL((amg_chol_solve_kernel<T>), dim3(L.nends), dim3(kCholThreads), 0, st, L.ends,
+                               L.sub_off, V::chol(A), bl, l == 0 ? z : V::x(L), act);
     }
     ACHECK();
     return PYHIP_OK;
+}
+
+int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer *tm, const int32_t *act,
+              const int32_t *act_h, bool x0_ready) {
+    return amg_apply_t<double>(A, st, b, z, tm, act, act_h, x0_ready);
+}
+
+int amg_make_f32(AmgDev &A, hipStream_t st, int64_t &B) {
+    if (!A.built) {
+        set_error("multigrid plan not built");
+        return PYHIP_ERR_INVALID;
+    }
+    int rc;
+    auto conv = [&](const double *src, float *&dst, int64_t n) -> int {
+        if (!dst && (rc = dev_alloc(&dst, n, B))) return rc;
+        if (n > 0) hipLaunchKernelGGL(amg_to_f32_kernel, dim3(ablk(n)), dim3(256), 0, st, n, src, dst);
+        return PYHIP_OK;
+    };
+    auto vecs = [&](AmgDevLevel &L) -> int {
+        if (!L.b32 && ((rc = dev_alloc(&L.b32, L.n, B)) || (rc = dev_alloc(&L.x32, L.n, B)) ||
+                       (rc = dev_alloc(&L.x2_32, L.n, B)) || (rc = dev_alloc(&L.r32, L.n, B))))
+            return rc;
+        return PYHIP_OK;
+    };
+    for (size_t l = 0; l < A.lv.size(); l++) {
+        AmgDevLevel &L = A.lv[l];
+        if (l > 0) {
+            if ((rc = conv(L.own_vals, L.own_vals32, L.nnz))) return rc;
+            L.vals32 = L.own_vals32;
+        }
+        if ((rc = conv(L.w, L.w32, L.n)) || (rc = conv(L.pv, L.pv32, L.nnzp)) || (rc = conv(L.ptv, L.ptv32, L.nnzp)) ||
+            (rc = vecs(L)))
+            return rc;
+    }
+    if (!A.lv.empty()) {
+        if ((rc = conv(A.bot.own_vals, A.bot.own_vals32, A.bot.nnz))) return rc;
+        A.bot.vals32 = A.bot.own_vals32;
+    }
+    if ((rc = vecs(A.bot))) return rc;
+    if ((rc = conv(A.chol, A.chol32, (int64_t)std::max(1, A.nsub) * kBottomMaxRows * kBottomMaxRows))) return rc;
+    ACHECK();
+    return PYHIP_OK;
+}
+
+int amg_set_fine32(AmgDev &A, const float *vals) {
+    if (!A.built) {
+        set_error("multigrid plan not built");
+        return PYHIP_ERR_INVALID;
+    }
+    if (A.lv.empty()) A.bot.vals32 = vals;
+    else A.lv[0].vals32 = vals;
+    return PYHIP_OK;
+}
+
+int amg_apply32(AmgDev &A, hipStream_t st, const float *b, float *z, KernelTimer *tm, const int32_t *act,
+                const int32_t *act_h) {
+    return amg_apply_t<float>(A, st, b, z, tm, act, act_h, false);
 }
 
 }  // namespace pyhip

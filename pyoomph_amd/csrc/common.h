@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/pyoomph_hip.h"
+#include "element_common.h"
 
 namespace pyhip {
 
@@ -35,13 +36,11 @@ void set_error(const std::string &msg);
         }                                                                             \
     } while (0)
 
-// Element families on the hot path (a row of the reference's BulkElement* classes).
-enum ElemKind : int {
-    ELEM_TRI_P1 = PYHIP_ELEM_TRI_P1,   // BulkElementTri2dC1, TGauss<2,2>
-    ELEM_TRI_P2 = PYHIP_ELEM_TRI_P2,   // BulkElementTri2dC2, TGauss<2,3>
-    ELEM_QUAD_Q2 = PYHIP_ELEM_QUAD_Q2, // BulkElementQuad2dC2, Gauss<2,3>
-    ELEM_HEX_Q2 = PYHIP_ELEM_HEX_Q2,   // BulkElementBrick3dC2, Gauss<3,3>
-};
+// Element families on the hot path (a row of the reference's BulkElement* classes):
+// ElemKind in element_common.h, with the values of PYHIP_ELEM_*.
+static_assert(ELEM_TRI_P1 == PYHIP_ELEM_TRI_P1 && ELEM_TRI_P2 == PYHIP_ELEM_TRI_P2 &&
+                  ELEM_QUAD_Q2 == PYHIP_ELEM_QUAD_Q2 && ELEM_HEX_Q2 == PYHIP_ELEM_HEX_Q2,
+              "element family ids");
 
 // Reference-element tables at the integration points of the default scheme.
 struct RefTables {

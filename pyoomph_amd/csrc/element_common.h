@@ -1,0 +1,169 @@
+// Device-side element helpers shared by the compiled element kernels (element_kernels.h)
+// and the kernels generated at run time (pyoomph_amd/codegen, compiled with hiprtc, which
+// receives this file as a header): the reference-element tables, the kernel arguments,
+// geometry per integration point and small vector helpers. Device code only, no host
+// library headers.
+#pragma once
+#if defined(__HIPCC_RTC__)
+typedef __hip_internal::int8_t int8_t;
+typedef __hip_internal::int32_t int32_t;
+typedef __hip_internal::int64_t int64_t;
+#else
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#endif
+
+namespace pyhip {
+
+// element families (the values of PYHIP_ELEM_*)
+enum ElemKind : int {
+    ELEM_TRI_P1 = 1,    // BulkElementTri2dC1, TGauss<2,2>
+    ELEM_TRI_P2 = 2,    // BulkElementTri2dC2, TGauss<2,3>
+    ELEM_QUAD_Q2 = 3,   // BulkElementQuad2dC2, Gauss<2,3>
+    ELEM_HEX_Q2 = 4,    // BulkElementBrick3dC2, Gauss<3,3>
+};
+
+template <typename T>
+struct DevTables {
+    T w[27];
+    T psi[27 * 27];
+    T dpsi[27 * 27 * 3];
+    T psi1[27 * 4];
+};
+
+template <typename T>
+struct ElemArgs {
+    int nel;                  // elements in this launch (one element group)
+    const int32_t *conn;      // [nnode_el][nel] global node (slot-major)
+    const int32_t *seqn;      // [nslot][nel] global equation, -1 pinned
+    const int32_t *sub;       // [nel] sub-problem id
+    const double *coords;     // [nnode_total][dim]
+    const double *dofs;       // [ndof_total]
+    const double *pinned;     // [nnode_total][nvalue]
+    const double *aux;        // [nnode_total] or nullptr
+    const double *params;     // [nsub][pstride]; Poisson: n, then (cx, cy, w, a) * n
+    int pstride;
+    int nvalue;
+    const DevTables<T> *tab;
+    T *Er;                    // [nel][nslot]
+    T *Ej;                    // [nel][nslot][nslot] or nullptr
+    int flag;                 // 0 residual, 1 residual + Jacobian, 2 residual + mass matrix
+    int deriv;                // 1: d/dlambda (Bratu)
+    // time derivative of the scalar forms: du/dt = tw0 u + (tw1 u(t-1) + tw2 u(t-2)),
+    // history dofs [ndof_total] (pinned nodes keep their boundary value)
+    int transient;
+    double tw0, tw1, tw2;
+    const double *hist1, *hist2;
+    // Hessian-vector product: Ej receives sum_k dJ/du_k y_k (y dof-indexed, 0 at pinned)
+    const double *hvec;
+    // boundary faces: [nel][2] alpha, beta of each face's flux alpha*u + beta
+    const double *fcoef;
+};
+
+template <int KIND> struct ElemTraits;
+template <> struct ElemTraits<ELEM_TRI_P1> { static constexpr int NN = 3, DIM = 2, NQ = 3; };
+template <> struct ElemTraits<ELEM_TRI_P2> { static constexpr int NN = 6, DIM = 2, NQ = 7; };
+template <> struct ElemTraits<ELEM_QUAD_Q2> { static constexpr int NN = 9, DIM = 2, NQ = 9; };
+template <> struct ElemTraits<ELEM_HEX_Q2> { static constexpr int NN = 27, DIM = 3, NQ = 27; };
+
+__device__ __forceinline__ double texp(double x) { return ::exp(x); }
+__device__ __forceinline__ float texp(float x) { return ::expf(x); }
+__device__ __forceinline__ double tsqrt(double x) { return ::sqrt(x); }
+__device__ __forceinline__ float tsqrt(float x) { return ::sqrtf(x); }
+
+// Metric inverse and g^{ab} g_a (src/elements.cpp:3655-3700 / 3800-3840), from the
+// tangents t[a][i]. Returns w*sqrt(det).
+template <typename T, int DIM>
+__device__ __forceinline__ T metric(const T (&t)[DIM][DIM], T wq, T (&g)[DIM][DIM]) {
+    T am[DIM][DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; a++)
+#pragma unroll
+        for (int b = 0; b < DIM; b++) {
+            T s = T(0);
+#pragma unroll
+            for (int i = 0; i < DIM; i++) s = s + t[a][i] * t[b][i];
+            am[a][b] = s;
+        }
+    T au[DIM][DIM];
+    T det_a;
+    if constexpr (DIM == 2) {
+        det_a = am[0][0] * am[1][1] - am[0][1] * am[1][0];
+        au[0][0] = am[1][1] / det_a;
+        au[0][1] = -am[0][1] / det_a;
+        au[1][0] = -am[1][0] / det_a;
+        au[1][1] = am[0][0] / det_a;
+    } else {
+        det_a = am[0][0] * am[1][1] * am[2][2] + am[0][1] * am[1][2] * am[2][0] +
+                am[0][2] * am[1][0] * am[2][1] - am[0][0] * am[1][2] * am[2][1] -
+                am[0][1] * am[1][0] * am[2][2] - am[0][2] * am[1][1] * am[2][0];
+        au[0][0] = (am[1][1] * am[2][2] - am[1][2] * am[2][1]) / det_a;
+        au[0][1] = -(am[0][1] * am[2][2] - am[0][2] * am[2][1]) / det_a;
+        au[0][2] = (am[0][1] * am[1][2] - am[0][2] * am[1][1]) / det_a;
+        au[1][0] = -(am[1][0] * am[2][2] - am[1][2] * am[2][0]) / det_a;
+        au[1][1] = (am[0][0] * am[2][2] - am[0][2] * am[2][0]) / det_a;
+        au[1][2] = -(am[0][0] * am[1][2] - am[0][2] * am[1][0]) / det_a;
+        au[2][0] = (am[1][0] * am[2][1] - am[1][1] * am[2][0]) / det_a;
+        au[2][1] = -(am[0][0] * am[2][1] - am[0][1] * am[2][0]) / det_a;
+        au[2][2] = (am[0][0] * am[1][1] - am[0][1] * am[1][0]) / det_a;
+    }
+#pragma unroll
+    for (int b = 0; b < DIM; b++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            T s = au[0][b] * t[0][i] + au[1][b] * t[1][i];
+            if constexpr (DIM == 3) s = s + au[2][b] * t[2][i];
+            g[b][i] = s;
+        }
+    return wq * tsqrt(det_a);
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ T dotv(const T *a, const T *b) {
+    T s = a[0] * b[0] + a[1] * b[1];
+    if constexpr (DIM == 3) s = s + a[2] * b[2];
+    return s;
+}
+
+// Per-integration-point geometry and field values, computed once per (element, point)
+// by one thread and shared through LDS: tangents, metric inverse, w*det, dx_shape of
+// every node, and the interpolated fields. Same operation order as the CPU restatement
+// of the generated code (fill_shape_info_at_s, then the interpolation loops).
+template <typename T, int NN, int DIM>
+__device__ __forceinline__ T point_geometry(const T (*X)[DIM], const T *__restrict__ D, T wq,
+                                            T (*dx)[DIM]) {
+    T t[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+        for (int j = 0; j < DIM; j++) t[i][j] = T(0);
+#pragma unroll
+    for (int l = 0; l < NN; l++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++)
+#pragma unroll
+            for (int j = 0; j < DIM; j++) t[j][i] = t[j][i] + X[l][i] * D[l * DIM + j];
+    T g[DIM][DIM];
+    const T W = metric<T, DIM>(t, wq, g);
+#pragma unroll
+    for (int l = 0; l < NN; l++)
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+            T s = T(0);
+#pragma unroll
+            for (int b = 0; b < DIM; b++) s = s + g[b][i] * D[l * DIM + b];
+            dx[l][i] = s;
+        }
+    return W;
+}
+
+// Elements per 256-thread workgroup of the scalar-form kernels: every (element, slot row)
+// pair of phase B and every (element, point) pair of a phase-A chunk gets a thread.
+template <int KIND> struct ScalarChunk { static constexpr int QC = ElemTraits<KIND>::NQ; };
+template <> struct ScalarChunk<ELEM_HEX_Q2> { static constexpr int QC = 9; };
+template <int KIND>
+constexpr int scalar_epb() {
+    return 256 / (ElemTraits<KIND>::NN > ScalarChunk<KIND>::QC ? ElemTraits<KIND>::NN : ScalarChunk<KIND>::QC);
+}
+
+}  // namespace pyhip

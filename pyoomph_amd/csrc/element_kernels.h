@@ -25,51 +25,9 @@
 // deterministic gather (scatter.hip) sums them into the residual and the CSR values.
 #pragma once
 #include "common.h"
+#include "element_common.h"
 
 namespace pyhip {
-
-template <typename T>
-struct DevTables {
-    T w[27];
-    T psi[27 * 27];
-    T dpsi[27 * 27 * 3];
-    T psi1[27 * 4];
-};
-
-template <typename T>
-struct ElemArgs {
-    int nel;                  // elements in this launch (one element group)
-    const int32_t *conn;      // [nnode_el][nel] global node (slot-major)
-    const int32_t *seqn;      // [nslot][nel] global equation, -1 pinned
-    const int32_t *sub;       // [nel] sub-problem id
-    const double *coords;     // [nnode_total][dim]
-    const double *dofs;       // [ndof_total]
-    const double *pinned;     // [nnode_total][nvalue]
-    const double *aux;        // [nnode_total] or nullptr
-    const double *params;     // [nsub][pstride]; Poisson: n, then (cx, cy, w, a) * n
-    int pstride;
-    int nvalue;
-    const DevTables<T> *tab;
-    T *Er;                    // [nel][nslot]
-    T *Ej;                    // [nel][nslot][nslot] or nullptr
-    int flag;                 // 0 residual, 1 residual + Jacobian, 2 residual + mass matrix
-    int deriv;                // 1: d/dlambda (Bratu)
-    // time derivative of the scalar forms: du/dt = tw0 u + (tw1 u(t-1) + tw2 u(t-2)),
-    // history dofs [ndof_total] (pinned nodes keep their boundary value)
-    int transient;
-    double tw0, tw1, tw2;
-    const double *hist1, *hist2;
-    // Hessian-vector product: Ej receives sum_k dJ/du_k y_k (y dof-indexed, 0 at pinned)
-    const double *hvec;
-    // boundary faces: [nel][2] alpha, beta of each face's flux alpha*u + beta
-    const double *fcoef;
-};
-
-template <int KIND> struct ElemTraits;
-template <> struct ElemTraits<ELEM_TRI_P1> { static constexpr int NN = 3, DIM = 2, NQ = 3; };
-template <> struct ElemTraits<ELEM_TRI_P2> { static constexpr int NN = 6, DIM = 2, NQ = 7; };
-template <> struct ElemTraits<ELEM_QUAD_Q2> { static constexpr int NN = 9, DIM = 2, NQ = 9; };
-template <> struct ElemTraits<ELEM_HEX_Q2> { static constexpr int NN = 27, DIM = 3, NQ = 27; };
 
 // Taylor-Hood Q2/Q1 slot layout = oomph local dof order (node-major, value-minor):
 // node 0: ux uy p | 1: ux uy | 2: ux uy p | 3,4,5: ux uy | 6: ux uy p | 7: ux uy | 8: ux uy p
@@ -82,110 +40,11 @@ __device__ __constant__ const int8_t kTHSlotP[22] = {-1, -1, 0,  -1, -1, -1, -1,
 // velocity-x slot of each node (velocity-y is the next slot)
 __device__ __constant__ const int8_t kTHNodeSlot[9] = {0, 3, 5, 8, 10, 12, 14, 17, 19};
 
-__device__ __forceinline__ double texp(double x) { return ::exp(x); }
-__device__ __forceinline__ float texp(float x) { return ::expf(x); }
-__device__ __forceinline__ double tsqrt(double x) { return ::sqrt(x); }
-__device__ __forceinline__ float tsqrt(float x) { return ::sqrtf(x); }
-
-// Metric inverse and g^{ab} g_a (src/elements.cpp:3655-3700 / 3800-3840), from the
-// tangents t[a][i]. Returns w*sqrt(det).
-template <typename T, int DIM>
-__device__ __forceinline__ T metric(const T (&t)[DIM][DIM], T wq, T (&g)[DIM][DIM]) {
-    T am[DIM][DIM];
-#pragma unroll
-    for (int a = 0; a < DIM; a++)
-#pragma unroll
-        for (int b = 0; b < DIM; b++) {
-            T s = T(0);
-#pragma unroll
-            for (int i = 0; i < DIM; i++) s = s + t[a][i] * t[b][i];
-            am[a][b] = s;
-        }
-    T au[DIM][DIM];
-    T det_a;
-    if constexpr (DIM == 2) {
-        det_a = am[0][0] * am[1][1] - am[0][1] * am[1][0];
-        au[0][0] = am[1][1] / det_a;
-        au[0][1] = -am[0][1] / det_a;
-        au[1][0] = -am[1][0] / det_a;
-        au[1][1] = am[0][0] / det_a;
-    } else {
-        det_a = am[0][0] * am[1][1] * am[2][2] + am[0][1] * am[1][2] * am[2][0] +
-                am[0][2] * am[1][0] * am[2][1] - am[0][0] * am[1][2] * am[2][1] -
-                am[0][1] * am[1][0] * am[2][2] - am[0][2] * am[1][1] * am[2][0];
-        au[0][0] = (am[1][1] * am[2][2] - am[1][2] * am[2][1]) / det_a;
-        au[0][1] = -(am[0][1] * am[2][2] - am[0][2] * am[2][1]) / det_a;
-        au[0][2] = (am[0][1] * am[1][2] - am[0][2] * am[1][1]) / det_a;
-        au[1][0] = -(am[1][0] * am[2][2] - am[1][2] * am[2][0]) / det_a;
-        au[1][1] = (am[0][0] * am[2][2] - am[0][2] * am[2][0]) / det_a;
-        au[1][2] = -(am[0][0] * am[1][2] - am[0][2] * am[1][0]) / det_a;
-        au[2][0] = (am[1][0] * am[2][1] - am[1][1] * am[2][0]) / det_a;
-        au[2][1] = -(am[0][0] * am[2][1] - am[0][1] * am[2][0]) / det_a;
-        au[2][2] = (am[0][0] * am[1][1] - am[0][1] * am[1][0]) / det_a;
-    }
-#pragma unroll
-    for (int b = 0; b < DIM; b++)
-#pragma unroll
-        for (int i = 0; i < DIM; i++) {
-            T s = au[0][b] * t[0][i] + au[1][b] * t[1][i];
-            if constexpr (DIM == 3) s = s + au[2][b] * t[2][i];
-            g[b][i] = s;
-        }
-    return wq * tsqrt(det_a);
-}
-
-template <typename T, int DIM>
-__device__ __forceinline__ T dotv(const T *a, const T *b) {
-    T s = a[0] * b[0] + a[1] * b[1];
-    if constexpr (DIM == 3) s = s + a[2] * b[2];
-    return s;
-}
-
-// Per-integration-point geometry and field values, computed once per (element, point)
-// by one thread and shared through LDS: tangents, metric inverse, w*det, dx_shape of
-// every node, and the interpolated fields. Same operation order as the CPU restatement
-// of the generated code (fill_shape_info_at_s, then the interpolation loops).
-template <typename T, int NN, int DIM>
-__device__ __forceinline__ T point_geometry(const T (*X)[DIM], const T *__restrict__ D, T wq,
-                                            T (*dx)[DIM]) {
-    T t[DIM][DIM];
-#pragma unroll
-    for (int i = 0; i < DIM; i++)
-#pragma unroll
-        for (int j = 0; j < DIM; j++) t[i][j] = T(0);
-#pragma unroll
-    for (int l = 0; l < NN; l++)
-#pragma unroll
-        for (int i = 0; i < DIM; i++)
-#pragma unroll
-            for (int j = 0; j < DIM; j++) t[j][i] = t[j][i] + X[l][i] * D[l * DIM + j];
-    T g[DIM][DIM];
-    const T W = metric<T, DIM>(t, wq, g);
-#pragma unroll
-    for (int l = 0; l < NN; l++)
-#pragma unroll
-        for (int i = 0; i < DIM; i++) {
-            T s = T(0);
-#pragma unroll
-            for (int b = 0; b < DIM; b++) s = s + g[b][i] * D[l * DIM + b];
-            dx[l][i] = s;
-        }
-    return W;
-}
-
 // Scalar weak forms (Poisson / Bratu / reaction-diffusion): slot = node.
 // Phase A (per chunk of integration points): thread t of an element computes points
 // q = t, t + NS, ... of the chunk: geometry, dx_shape of all nodes, u, grad u and the
 // source term and its derivative. Phase B: thread = test row i, accumulates its residual
 // and Jacobian row over the chunk's points in increasing point order.
-template <int KIND> struct ScalarChunk { static constexpr int QC = ElemTraits<KIND>::NQ; };
-template <> struct ScalarChunk<ELEM_HEX_Q2> { static constexpr int QC = 9; };
-// Elements per 256-thread workgroup: every (element, slot row) pair of phase B and every
-// (element, point) pair of a phase-A chunk gets a thread, so each phase is one round.
-template <int KIND>
-constexpr int scalar_epb() {
-    return 256 / (ElemTraits<KIND>::NN > ScalarChunk<KIND>::QC ? ElemTraits<KIND>::NN : ScalarChunk<KIND>::QC);
-}
 
 // TR: the form has a time derivative (a.transient); HV: the matrix output is the
 // Hessian-vector product sum_k dJ_ij/du_k y_k = int d2src (sum_k y_k psi_k) psi_j psi_i

@@ -95,6 +95,11 @@ struct KrylovWork {
     int ilu_order = 0;                // PYHIP_ILU_ORDER_* of the current solve
     AmgDev amg;
     bool amg_stale = true;            // multigrid values to be rebuilt at the next set-up
+    const double *amg_src64 = nullptr;   // FP64 matrix the FP32 V-cycle's set-up reads (or null)
+    double *amg_conv64 = nullptr;     // [nnz] FP32 matrix widened for that set-up
+    // per sub-problem: 1 if the last Krylov solve (or refinement) ended above its tolerance
+    // or broke down; written by every solver before it returns
+    std::vector<int32_t> lfail_h;
     int64_t bytes = 0;
 };
 

@@ -166,7 +166,9 @@ __global__ void gmres_start_kernel(int nsub, int m, const double *__restrict__ n
 }
 
 // Column k of the Hessenberg: h[0..k] = dots (CGS pass 1 + pass 2), h[k+1] = |w|.
-__global__ void gmres_givens_kernel(int nsub, int m, int k, const double *__restrict__ d1,
+// m: stride of the per-sub-problem Hessenberg / Givens arrays (the workspace's restart
+// capacity); mr <= m: the restart length of this solve (the cycle ends after column mr-1).
+__global__ void gmres_givens_kernel(int nsub, int m, int mr, int k, const double *__restrict__ d1,
                                     const double *__restrict__ d2, const double *__restrict__ wn2,
                                     double *__restrict__ H, double *__restrict__ cs,
                                     double *__restrict__ sn, double *__restrict__ g,
@@ -199,7 +201,7 @@ __global__ void gmres_givens_kernel(int nsub, int m, int k, const double *__rest
     gs[k] = c * gs[k];
     kcols[s] = k + 1;
     const bool done = !(fabs(gs[k + 1]) > tol * bnorm[s]) || hk1 == 0.0 || den == 0.0 ||
-                      !isfinite(gs[k + 1]) || (k + 1 == m);
+                      !isfinite(gs[k + 1]) || (k + 1 == mr);
     if (done) active[s] = 0;
     else atomicAdd(nactive, 1);
 }
@@ -771,8 +773,8 @@ static int precond_apply(pyhip_problem *p, KrylovWork *W, int pc, const T *v, T 
     if (pc == PYHIP_PRECOND_AMG) {
         if constexpr (std::is_same<T, double>::value)
             return amg_apply(W->amg, p->st, v, z, &p->timer, act, act_h, x0_ready);
-        set_error("the multigrid preconditioner runs FP64 solves only");
-        return PYHIP_ERR_INVALID;
+        else
+            return amg_apply32(W->amg, p->st, v, z, &p->timer, act, act_h);
     }
     const T *jd = std::is_same<T, float>::value ? (const T *)W->jdiag32 : (const T *)W->jdiag64;
     hipLaunchKernelGGL((jacobi_apply_kernel<T>), dim3(nblk(p->ndof)), dim3(256), 0, p->st, p->ndof, jd, v, z);
@@ -791,10 +793,30 @@ static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals
             // first matrix of a solve and kept for its later Newton steps; the finest level
             // always uses the current matrix
             if (!W->amg_stale) return amg_set_fine(W->amg, vals);
-            W->amg_stale = false;
-            return amg_setup(W->amg, p->st, vals, &p->timer);
+            rc = amg_setup(W->amg, p->st, vals, &p->timer);
+            W->amg_stale = rc != PYHIP_OK;   // a failed set-up is redone by the next solve
+            return rc;
+        } else if constexpr (std::is_same<T, float>::value && std::is_same<TV, float>::value) {
+            // FP32 V-cycle: the hierarchy is set up in FP64 (from the FP64 Jacobian when the
+            // caller has one, else from the FP32 values widened), then rounded to FP32
+            int rc = amg_build(W->amg, p->ndof, p->row_start_h.data(), p->col_h.data(), p->dof_off_h, W->bytes);
+            if (rc) return rc;
+            if (W->amg_stale) {
+                const double *src = W->amg_src64;
+                if (!src) {
+                    if (!W->amg_conv64 && (rc = dev_alloc(&W->amg_conv64, p->nnz, W->bytes))) return rc;
+                    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(nblk(p->nnz)), dim3(256), 0, p->st,
+                                       p->nnz, vals, W->amg_conv64);
+                    KCHECK();
+                    src = W->amg_conv64;
+                }
+                rc = amg_setup(W->amg, p->st, src, &p->timer);
+                W->amg_stale = rc != PYHIP_OK;
+                if (rc || (rc = amg_make_f32(W->amg, p->st, W->bytes))) return rc;
+            }
+            return amg_set_fine32(W->amg, vals);
         }
-        set_error("the multigrid preconditioner runs FP64 solves only");
+        set_error("the multigrid preconditioner runs with matching solve and matrix precision");
         return PYHIP_ERR_INVALID;
     }
     if (pc == PYHIP_PRECOND_ILU0) {
@@ -816,10 +838,14 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
                  const pyhip_linear_opts &o, double rtol, double stag, const int32_t *need, int *iters,
                  int *failed) {
     const int64_t n = p->ndof;
+    // m: layout stride of the workspace (sized for the longest restart asked so far);
+    // mr: this solve's restart length, so the result does not depend on earlier solves
     const int m = W->m, ns = W->nsub;
+    const int mr = o.restart;
+    PYHIP_REQUIRE(mr >= 1 && mr <= m, "GMRES restart longer than the workspace");
     const int prec = std::is_same<T, float>::value ? PYHIP_F32 : PYHIP_F64;
     int rc;
-    if ((rc = ensure_basis<T>(W, n, m + 1))) return rc;
+    if ((rc = ensure_basis<T>(W, n, mr + 1))) return rc;
     T *V = (T *)W->V;
     T *w = std::is_same<T, float>::value ? (T *)W->w32 : (T *)W->w64;
     T *t = std::is_same<T, float>::value ? (T *)W->t32 : (T *)W->t64;
@@ -831,7 +857,7 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
     // x0 = 0
     PYHIP_CHECK_HIP(hipMemsetAsync(x, 0, n * sizeof(T), p->st));
     int total = 0;
-    const int max_cycles = std::max(1, (o.max_iters + m - 1) / m);
+    const int max_cycles = std::max(1, (o.max_iters + mr - 1) / mr);
     for (int cyc = 0; cyc < max_cycles; cyc++) {
         // r = b - A x  -> V_0 (unnormalised)
         if ((rc = spmv<TV, T, T>(p, vals, x, V, b))) return rc;
@@ -860,7 +886,7 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
                            W->active, V);
         KCHECK();
         int k = 0;
-        for (; k < m; k++) {
+        for (; k < mr; k++) {
             T *vk = V + (int64_t)k * n;
             // the preconditioner (multigrid) and the spmv skip the rows of stopped
             // sub-problems: nothing reads their t and w
@@ -888,7 +914,7 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
                                    W->active, V + (int64_t)(k + 1) * n);
             }
             PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
-            hipLaunchKernelGGL(gmres_givens_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, m, k, d1, d2,
+            hipLaunchKernelGGL(gmres_givens_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, m, mr, k, d1, d2,
                                d3, W->H, W->cs, W->sn, W->g, W->bnorm, rtol, W->active, W->kcols,
                                W->nactive_d);
             KCHECK();
@@ -896,7 +922,7 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
             // the host reads the number of sub-problems still iterating every kGmresCheck
             // iterations (and at the end of the cycle); in between, a sub-problem that
             // has converged keeps its state (every kernel skips it) while the others go on
-            if ((k + 1) % kGmresCheck == 0 || k + 1 == m) {
+            if ((k + 1) % kGmresCheck == 0 || k + 1 == mr) {
                 PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost,
                                                p->st));
                 PYHIP_CHECK_HIP(hipMemcpyAsync(act_h, W->active, ns * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
@@ -924,7 +950,10 @@ static int gmres(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *b, T 
                        rtol * 1.01, 0.0, 1, need, W->g, W->active, W->kcols, W->beta0, W->nactive_d);
     KCHECK();
     PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+    PYHIP_CHECK_HIP(hipMemcpyAsync(W->act_h, W->active, ns * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
     PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    // a sub-problem still marked active is above its tolerance
+    W->lfail_h.assign(W->act_h, W->act_h + ns);
     *failed = *W->nactive_h;
     *iters = total;
     return PYHIP_OK;
@@ -982,13 +1011,15 @@ __global__ void cg_start_kernel(int nsub, const double *__restrict__ d2,
 
 // alpha = rz / (p, Ap); a non-positive curvature stops the sub-problem (breakdown)
 __global__ void cg_alpha_kernel(int nsub, const double *__restrict__ pq_in, double *__restrict__ pq,
-                                int32_t *__restrict__ active, int32_t *__restrict__ nfail) {
+                                int32_t *__restrict__ active, int32_t *__restrict__ nfail,
+                                int32_t *__restrict__ fail) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsub || !active[s]) return;
     const double v = pq_in[s];
     pq[s] = v;
     if (!(v > 0.0) || !isfinite(v)) {
         active[s] = 0;
+        fail[s] = 1;
         atomicAdd(nfail, 1);
     }
 }
@@ -1094,6 +1125,7 @@ static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *
     if ((rc = batched_dot2<T>(p, W, r, r, r, z, d2))) return rc;
     PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
     PYHIP_CHECK_HIP(hipMemsetAsync(W->nfail_d, 0, sizeof(int32_t), p->st));
+    PYHIP_CHECK_HIP(hipMemsetAsync(W->cg_fail, 0, ns * sizeof(int32_t), p->st));
     hipLaunchKernelGGL(cg_start_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2, need, W->bnorm,
                        W->cg_rz, W->active, W->nactive_d);
     hipLaunchKernelGGL((convert_kernel<T, T>), dim3(nblk(n)), dim3(256), 0, p->st, n, z, pv);
@@ -1113,7 +1145,7 @@ static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *
         if ((rc = spmv<TV, T, T>(p, vals, pv, qv, (const T *)nullptr, W->active, act_h))) return rc;
         if ((rc = batched_dots<T>(p, W, 1, pv, n, qv, d2 + 2 * ns, W->active))) return rc;
         hipLaunchKernelGGL(cg_alpha_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2 + 2 * ns,
-                           W->cg_pq, W->active, W->nfail_d);
+                           W->cg_pq, W->active, W->nfail_d, W->cg_fail);
         {
             TimeScope ts(&p->timer, "vector", p->st);
             hipLaunchKernelGGL((cg_xr_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, pv, qv, x, r,
@@ -1135,7 +1167,11 @@ static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *
         if ((rc = read_counter(p, W, W->nactive_d, &nact, W->active, act_h))) return rc;
     }
     int32_t nf = 0;
-    if ((rc = read_counter(p, W, W->nfail_d, &nf))) return rc;
+    PYHIP_CHECK_HIP(hipMemcpyAsync(W->flags_h, W->cg_fail, ns * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+    if ((rc = read_counter(p, W, W->nfail_d, &nf, W->active, act_h))) return rc;
+    // per sub-problem: broke down, or still above tolerance
+    W->lfail_h.assign(ns, 0);
+    for (int s = 0; s < ns; s++) W->lfail_h[s] = (W->flags_h[s] != 0 || act_h[s] != 0) ? 1 : 0;
     *iters = it;
     *failed = nf + nact;   // breakdowns + sub-problems still above tolerance
     return PYHIP_OK;
@@ -1448,6 +1484,7 @@ static int cg_worklist(pyhip_problem *p, KrylovWork *W, const TV *vals, const T 
     int rc;
     *iters = 0;
     *failed = 0;
+    W->lfail_h.assign(ns, 0);
     if (n == 0 || W->nchunk == 0) return PYHIP_OK;
     PYHIP_REQUIRE(W->achunk && W->achunk_h && W->gpart && W->cg_iters && W->cg_fail && W->flags_h &&
                       W->active && need && W->sub_chunk_ptr && W->chunk_begin && W->chunk_end &&
@@ -1509,6 +1546,8 @@ static int cg_worklist(pyhip_problem *p, KrylovWork *W, const TV *vals, const T 
         nnz_done += sp * W->sub_nnz_h[s];
     }
     p->timer.add_work("spmv", (int64_t)(sizeof(TV) + 4) * nnz_done + (int64_t)(4 + 2 * sizeof(T)) * rows_done);
+    W->lfail_h.assign(ns, 0);
+    for (int s = 0; s < ns; s++) W->lfail_h[s] = (W->flags_h[ns + s] != 0 || W->flags_h[s] != 0) ? 1 : 0;
     if (debug_enabled())
         fprintf(stderr, "[pyhip] cg: %d iterations (longest sub-problem %d), %d still active, %d failed\n", it,
                 maxit, nactive, nfail);
@@ -1567,6 +1606,7 @@ static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts
         KCHECK();
     }
     W->ilu_order = o.ilu_order;
+    W->amg_src64 = jac_prec == PYHIP_F64 ? W->vals64 : nullptr;
     if ((rc = precond_setup<float, float>(p, W, o.precond, W->vals32))) return rc;
     PYHIP_CHECK_HIP(hipMemsetAsync(d, 0, n * sizeof(double), p->st));
     hipLaunchKernelGGL((convert_kernel<double, double>), dim3(nblk(n)), dim3(256), 0, p->st, n, r, W->r64);
@@ -1608,6 +1648,7 @@ static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts
         if (!left) break;
         PYHIP_CHECK_HIP(hipMemcpyAsync(W->need, need_h.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, p->st));
     }
+    W->lfail_h = need_h;   // still above the refinement tolerance
     *failed = left;
     return PYHIP_OK;
 }
@@ -1665,6 +1706,7 @@ void pyhip_newton_opts_default(pyhip_newton_opts *o) {
     o->relaxation = 1.0;
     o->jacobian_precision = PYHIP_F64;
     pyhip_linear_opts_default(&o->linear);
+    o->correction_tol = 0.0;
 }
 
 int64_t pyhip_krylov_bytes(const pyhip_problem *p) { return p->kw ? p->kw->bytes : 0; }
@@ -1681,7 +1723,7 @@ void pyhip_krylov_free(pyhip_problem *p) {
                     W->ilu.src, W->ilu.diag, W->ilu.lo_rows, W->ilu.up_rows, W->ilu.lu,
                     W->ilu.tmp64, W->ilu.tmp64b, W->ilu.tmp32, W->ilu.tmp32b, W->achunk,
                     W->gpart, W->cg_iters, W->cg_fail, W->ac_cur, W->ac_der, W->ac_y, W->ac_z,
-                    W->ac_t, W->ac_zs, W->ac_alpha, W->ac_mask};
+                    W->ac_t, W->ac_zs, W->ac_alpha, W->ac_mask, W->amg_conv64};
     for (void *q : ptrs)
         dfree(q);
     if (W->nactive_h) (void)hipHostFree(W->nactive_h);
@@ -1718,6 +1760,7 @@ int pyhip_newton_solve(pyhip_problem *p, double *x, const pyhip_newton_opts *o,
     };
     std::vector<int32_t> running(ns, 1);
     std::vector<int> count(ns, 0);
+    std::vector<double> dmax(ns, 0.0), xmax(ns, 0.0);
     auto record = [&](int s, double v) {
         auto &S = stats[s];
         if (S.nhistory < 16) S.history[S.nhistory++] = v;
@@ -1760,18 +1803,29 @@ int pyhip_newton_solve(pyhip_problem *p, double *x, const pyhip_newton_opts *o,
         for (int s = 0; s < ns; s++)
             if (running[s]) {
                 stats[s].linear_iters += its;
-                stats[s].linear_failures += lfail > 0;
+                stats[s].linear_failures += (s < (int)W->lfail_h.size() && W->lfail_h[s]) ? 1 : 0;
             }
         hipLaunchKernelGGL((axpy_kernel<double, double>), dim3(nblk(n)), dim3(256), 0, p->st, n, W->d64, x,
                            -o->relaxation, p->dof_sub, W->dof_mask);
         KCHECK();
+        if (o->correction_tol > 0.0) {
+            // max |dx| and max |x| per sub-problem (FP64)
+            if ((rc = batched_maxabs(p, W->d64, W->sub_scalar))) return rc;
+            PYHIP_CHECK_HIP(hipMemcpyAsync(dmax.data(), W->sub_scalar, ns * sizeof(double), hipMemcpyDeviceToHost,
+                                           p->st));
+            if ((rc = batched_maxabs(p, x, W->sub_scalar))) return rc;
+            PYHIP_CHECK_HIP(hipMemcpyAsync(xmax.data(), W->sub_scalar, ns * sizeof(double), hipMemcpyDeviceToHost,
+                                           p->st));
+        }
         if ((rc = residual_norms())) return rc;
         for (int s = 0; s < ns; s++) {
             if (!running[s]) continue;
             const double mr = W->maxres_h[s];
             record(s, mr);
             stats[s].newton_iters = count[s];
-            if (mr < o->tol) {
+            const bool small_update = o->correction_tol <= 0.0 ||
+                                      dmax[s] * std::fabs(o->relaxation) <= o->correction_tol * xmax[s];
+            if (mr < o->tol && small_update) {
                 running[s] = 0;
                 stats[s].converged = 1;
             } else if (mr > o->max_residuals || count[s] >= o->max_iter) {
@@ -1884,8 +1938,14 @@ int pyhip_arclength_step(pyhip_problem *p, double *x, int param, const pyhip_new
         if ((rc = upload(act, alpha)) || (rc = vec(4, x, nullptr, W->ac_cur))) return rc;
         if ((rc = upload(fresh, alpha)) || (rc = vec(5, nullptr, nullptr, W->ac_der))) return rc;
     }
-    std::vector<int32_t> pending = act, failed_sub(ns, 0);
+    std::vector<int32_t> pending = act, failed_sub(ns, 0), lin_failed(ns, 0);
     std::vector<int> count(ns, 0);
+    // a linear solve that ends above its tolerance is an error for its sub-problem, not a
+    // rejected step (problem.cc:10905-10915 throws on a linear-solver error)
+    auto note_linear_failures = [&](std::vector<int32_t> &run) {
+        for (int s = 0; s < ns; s++)
+            if (run[s] && s < (int)W->lfail_h.size() && W->lfail_h[s]) lin_failed[s] = 1;
+    };
     while (true) {
         int npend = 0;
         for (int s = 0; s < ns; s++) npend += pending[s];
@@ -1917,9 +1977,11 @@ int pyhip_arclength_step(pyhip_problem *p, double *x, int param, const pyhip_new
             int its = 0, refs = 0, lfail = 0;
             if ((rc = linear_solve(p, W, o->linear, PYHIP_F64, W->res64, W->ac_y, W->dof_mask, &its, &refs, &lfail)))
                 return rc;
+            note_linear_failures(run);
             if ((rc = pyhip_assemble(p, x, 0, param, PYHIP_F64, W->ac_t, nullptr))) return rc;
             if ((rc = linear_solve(p, W, o->linear, PYHIP_F64, W->ac_t, W->ac_z, W->dof_mask, &its, &refs, &lfail)))
                 return rc;
+            note_linear_failures(run);
             if ((rc = sub_dot(W->ac_der, W->ac_y, dots.data())) || (rc = sub_dot(W->ac_der, W->ac_z, dots.data() + ns)))
                 return rc;
             for (int s = 0; s < ns; s++) {
@@ -1937,14 +1999,17 @@ int pyhip_arclength_step(pyhip_problem *p, double *x, int param, const pyhip_new
             if ((rc = residuals(run))) return rc;
             for (int s = 0; s < ns; s++) {
                 if (!run[s]) continue;
-                if (maxres[s] < o->tol) run[s] = 0;
+                if (lin_failed[s]) run[s] = 0;
+                else if (maxres[s] < o->tol) run[s] = 0;
                 else if (maxres[s] > o->max_residuals || count[s] >= o->max_iter) run[s] = 0, failed_sub[s] = 1;
             }
         }
         // failed Newton solves retry with a shorter step (problem.cc:10918-10928)
         for (int s = 0; s < ns; s++) {
             if (!pending[s]) continue;
-            if (failed_sub[s]) {
+            if (lin_failed[s]) {
+                pending[s] = 0;
+            } else if (failed_sub[s]) {
                 st[s].ds_current *= (2.0 / 3.0);
                 st[s].rejected++;
             } else {
@@ -1964,6 +2029,23 @@ int pyhip_arclength_step(pyhip_problem *p, double *x, int param, const pyhip_new
         if (pd * cdir < 0.0) pd = -pd;
         st[s].parameter_derivative = pd;
     };
+    // a sub-problem whose linear solve failed keeps its state: dofs back to Dof_current,
+    // parameter back to Parameter_current
+    int nlin = 0;
+    std::string lin_msg;
+    for (int s = 0; s < ns; s++) {
+        st[s].linear_failed = lin_failed[s];
+        if (!lin_failed[s]) continue;
+        act[s] = 0;
+        lam[s] = st[s].parameter_current;
+        ds_next[s] = ds[s];
+        if (nlin++ < 8) lin_msg += (nlin > 1 ? ", " : " ") + std::to_string(s);
+    }
+    if (nlin) {
+        std::vector<int32_t> back(ns, 0);
+        for (int s = 0; s < ns; s++) back[s] = lin_failed[s];
+        if ((rc = set_lambda()) || (rc = upload(back, alpha)) || (rc = vec(4, W->ac_cur, nullptr, x))) return rc;
+    }
     for (int s = 0; s < ns; s++) {
         if (!act[s]) continue;
         helper(s, dz[s]);
@@ -1983,6 +2065,11 @@ int pyhip_arclength_step(pyhip_problem *p, double *x, int param, const pyhip_new
     if ((rc = upload(act, alpha)) || (rc = vec(2, W->ac_zs, nullptr, W->ac_der))) return rc;
     for (int s = 0; s < ns; s++) st[s].parameter = lam[s];
     PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    if (nlin) {
+        set_error("ERROR IN THE LINEAR SOLVER during an arclength step of sub-problem(s)" + lin_msg +
+                  (nlin > 8 ? ", ..." : "") + " (state kept; see linear_failed)");
+        return PYHIP_ERR_NOT_CONVERGED;
+    }
     return PYHIP_OK;
 }
 

@@ -115,6 +115,8 @@ class Problem:
         self.max_residuals = 1e10
         self.always_take_one_newton_step = True
         self.newton_relaxation_factor = 1.0
+        # > 0: Newton also stops only once max|dx| <= this * max|x| (FP32 Jacobians)
+        self.newton_correction_tolerance = 0.0
         self.jacobian_precision = "f64"
         self.linear_precision = "f64"
         self.preconditioner = "ilu0"
@@ -315,6 +317,7 @@ class Problem:
         o.linear.refine_rtol = self.refine_rtol
         o.linear.ilu_order = {"auto": _lib.ILU_ORDER_AUTO, "natural": _lib.ILU_ORDER_NATURAL,
                               "multicolor": _lib.ILU_ORDER_MULTICOLOR}[self.ilu_order]
+        o.correction_tol = self.newton_correction_tolerance
         return o
 
     # -- time stepping ------------------------------------------------------------------
@@ -465,7 +468,8 @@ class Problem:
 
     def arclength_stats(self):
         return [{"newton_iters": a.newton_iters, "rejected": a.rejected, "theta_squared": a.theta_squared,
-                 "parameter_derivative": a.parameter_derivative} for a in self._arc]
+                 "parameter_derivative": a.parameter_derivative, "linear_failed": bool(a.linear_failed)}
+                for a in self._arc]
 
     def solve(self, raise_on_failure=True):
         """Steady Newton solve of every sub-problem; returns per-sub-problem stats."""
