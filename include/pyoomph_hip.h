@@ -163,6 +163,22 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs_dev, int flag, int deriv
 int pyhip_set_time_stepping(pyhip_problem *p, int enabled, const double *weights,
                             const double *hist1_dev, const double *hist2_dev);
 
+/* Allocation-free variant of pyhip_assemble (same arguments, same results bit for bit).
+ * It has no counterpart in the reference: it is for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc. The element vectors and matrices that pyhip_assemble keeps in buffers it
+ * allocates on first use live here in one static device array of
+ * PYHIP_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends). The call allocates
+ * nothing and rejects, with PYHIP_ERR_INVALID, a problem whose element buffers
+ * (elements x (slots + slots^2) values of the precision) exceed that size. Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (other streams, other
+ * threads) race. Nothing in the array has to start at zero (every entry the gather reads
+ * is written first). After it, pyhip_export_reference_csr reports that no FP64 Jacobian
+ * is held (the element matrices are not in the problem's own buffer). */
+#define PYHIP_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int pyhip_assemble_noalloc(pyhip_problem *p, const double *dofs_dev, int flag, int deriv_param,
+                           int precision, double *residual_dev, void *values_dev);
+
 /* Hessian-vector product of the scalar forms: values_dev [nnz] (structural pattern)
  * receives sum_k dJ_ij/du_k y_k at dofs_dev, residual_dev the residual; the reference's
  * HessianVectorProduct with one vector (src/codegen.cpp:3646-3910, assembled by

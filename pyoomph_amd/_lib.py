@@ -97,6 +97,8 @@ SIGNATURES = {
     "pyhip_pattern_copy": (ctypes.c_int, [ctypes.c_void_p, c_i32p, c_i32p]),
     "pyhip_assemble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "pyhip_assemble_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "pyhip_jit_element_init": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "pyhip_jit_set_form_params": (ctypes.c_int, [ctypes.c_void_p, c_dp, ctypes.c_int]),
     "pyhip_superlu": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)] * 4 + [c_dp] + [ctypes.POINTER(ctypes.c_int)] * 2 +

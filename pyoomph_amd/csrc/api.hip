@@ -407,8 +407,12 @@ int pyhip_pattern_copy(const pyhip_problem *p, int32_t *row_start, int32_t *col_
     return PYHIP_OK;
 }
 
-int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_param,
-                   int precision, double *residual, void *values) {
+}  // extern "C"
+
+namespace pyhip {
+
+static int check_assemble_args(pyhip_problem *p, const double *dofs, int flag, int deriv_param, int precision,
+                               double *residual, void *values) {
     PYHIP_REQUIRE(p && dofs && residual, "null argument");
     PYHIP_REQUIRE(flag == 0 || flag == 1 || flag == 2, "flag must be 0, 1 or 2");
     PYHIP_REQUIRE(precision == PYHIP_F64 || precision == PYHIP_F32, "bad precision");
@@ -418,22 +422,14 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
     for (auto &G : p->groups)
         PYHIP_REQUIRE(deriv_param < 0 || G.form == PYHIP_FORM_BRATU || G.form == kFormFlux,
                       "parameter derivative only defined for the Bratu form");
+    return PYHIP_OK;
+}
+
+// Element kernels into the element buffers Er / Ej (FP64 or FP32 by precision; Ej may be
+// null when flag == 0), then the deterministic gather into residual / values.
+static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int deriv_param, int precision,
+                         double *residual, void *values, void *Er, void *Ej) {
     int rc;
-    if (precision == PYHIP_F64) {
-        if (!p->Er64) {
-            if ((rc = dev_alloc(&p->Er64, p->n_er, p->device_bytes))) return rc;
-        }
-        if (flag && !p->Ej64) {
-            if ((rc = dev_alloc(&p->Ej64, p->n_ej, p->device_bytes))) return rc;
-        }
-    } else {
-        if (!p->Er32) {
-            if ((rc = dev_alloc(&p->Er32, p->n_er, p->device_bytes))) return rc;
-        }
-        if (flag && !p->Ej32) {
-            if ((rc = dev_alloc(&p->Ej32, p->n_ej, p->device_bytes))) return rc;
-        }
-    }
     for (auto &G : p->groups) {
         if (precision == PYHIP_F64) {
             ElemArgs<double> a{};
@@ -442,8 +438,8 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
             a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
             a.tab = G.form != kFormFlux ? p->tab64[G.kind] : (G.kind == kKindQuad9 ? p->tab64[ELEM_QUAD_Q2] : nullptr);
-            a.Er = p->Er64 + G.er_off;
-            a.Ej = flag ? p->Ej64 + G.ej_off : nullptr;
+            a.Er = (double *)Er + G.er_off;
+            a.Ej = flag ? (double *)Ej + G.ej_off : nullptr;
             a.flag = flag; a.deriv = deriv_param >= 0;
             a.transient = p->transient ? 1 : 0;
             a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
@@ -462,8 +458,8 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
             a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
             a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
             a.tab = G.form != kFormFlux ? p->tab32[G.kind] : (G.kind == kKindQuad9 ? p->tab32[ELEM_QUAD_Q2] : nullptr);
-            a.Er = p->Er32 + G.er_off;
-            a.Ej = flag ? p->Ej32 + G.ej_off : nullptr;
+            a.Er = (float *)Er + G.er_off;
+            a.Ej = flag ? (float *)Ej + G.ej_off : nullptr;
             a.flag = flag; a.deriv = deriv_param >= 0;
             a.transient = p->transient ? 1 : 0;
             a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
@@ -481,25 +477,110 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
     if (precision == PYHIP_F64) {
         if (flag) {
             TimeScope ts(&p->timer, "gather_values", p->st);
-            if ((rc = launch_gather_values_f64(m, p->Ej64, p->Er64, (double *)values, residual, p->st)))
+            if ((rc = launch_gather_values_f64(m, (const double *)Ej, (const double *)Er, (double *)values,
+                                               residual, p->st)))
                 return rc;
-            p->have_ej64 = (deriv_param < 0);   // the last FP64 matrix (Jacobian or mass)
         } else {
             TimeScope ts(&p->timer, "gather_residual", p->st);
-            if ((rc = launch_gather_residual_f64(p->ndof, p->occ_ptr, p->occ_roff, p->Er64, residual, p->st)))
+            if ((rc = launch_gather_residual_f64(p->ndof, p->occ_ptr, p->occ_roff, (const double *)Er, residual,
+                                                 p->st)))
                 return rc;
         }
     } else {
         if (flag) {
             TimeScope ts(&p->timer, "gather_values", p->st);
-            if ((rc = launch_gather_values_f32(m, p->Ej32, p->Er32, (float *)values, residual, p->st)))
+            if ((rc = launch_gather_values_f32(m, (const float *)Ej, (const float *)Er, (float *)values, residual,
+                                               p->st)))
                 return rc;
         } else {
             TimeScope ts(&p->timer, "gather_residual", p->st);
-            if ((rc = launch_gather_residual_f32(p->ndof, p->occ_ptr, p->occ_roff, p->Er32, residual, p->st)))
+            if ((rc = launch_gather_residual_f32(p->ndof, p->occ_ptr, p->occ_roff, (const float *)Er, residual,
+                                                 p->st)))
                 return rc;
         }
     }
+    return PYHIP_OK;
+}
+
+// ---- the allocation-free assembly ------------------------------------------------------
+// Element buffers of pyhip_assemble_noalloc: one array per device, alive until the process
+// ends. Host code never uses the array's name as an address (on the host it is only a
+// stand-in for the symbol); the device address comes from hipGetSymbolAddress, once per
+// device, and is kept below.
+static __device__ __align__(256) unsigned char scratch[PYHIP_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_address(void **out) {
+    int dev = -1;
+    PYHIP_CHECK_HIP(hipGetDevice(&dev));
+    PYHIP_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        PYHIP_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        PYHIP_REQUIRE(ptr != nullptr, "no device address for the scratch array");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return PYHIP_OK;
+}
+
+}  // namespace pyhip
+
+extern "C" {
+
+int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_param,
+                   int precision, double *residual, void *values) {
+    int rc = check_assemble_args(p, dofs, flag, deriv_param, precision, residual, values);
+    if (rc) return rc;
+    void *Er, *Ej;
+    if (precision == PYHIP_F64) {
+        if (!p->Er64) {
+            if ((rc = dev_alloc(&p->Er64, p->n_er, p->device_bytes))) return rc;
+        }
+        if (flag && !p->Ej64) {
+            if ((rc = dev_alloc(&p->Ej64, p->n_ej, p->device_bytes))) return rc;
+        }
+        Er = p->Er64;
+        Ej = p->Ej64;
+    } else {
+        if (!p->Er32) {
+            if ((rc = dev_alloc(&p->Er32, p->n_er, p->device_bytes))) return rc;
+        }
+        if (flag && !p->Ej32) {
+            if ((rc = dev_alloc(&p->Ej32, p->n_ej, p->device_bytes))) return rc;
+        }
+        Er = p->Er32;
+        Ej = p->Ej32;
+    }
+    if ((rc = assemble_into(p, dofs, flag, deriv_param, precision, residual, values, Er, Ej))) return rc;
+    if (precision == PYHIP_F64 && flag) p->have_ej64 = (deriv_param < 0);   // the last FP64 matrix
+    return PYHIP_OK;
+}
+
+int pyhip_assemble_noalloc(pyhip_problem *p, const double *dofs, int flag, int deriv_param, int precision,
+                           double *residual, void *values) {
+    int rc = check_assemble_args(p, dofs, flag, deriv_param, precision, residual, values);
+    if (rc) return rc;
+    const size_t esz = precision == PYHIP_F64 ? sizeof(double) : sizeof(float);
+    const size_t er_bytes = ((size_t)p->n_er * esz + 255) / 256 * 256;
+    const size_t need = er_bytes + (flag ? (size_t)p->n_ej * esz : 0);
+    if (need > (size_t)PYHIP_NOALLOC_SCRATCH_BYTES) {
+        set_error("pyhip_assemble_noalloc: the element buffers of this problem need " + std::to_string(need) +
+                  " bytes, more than the " + std::to_string((long long)PYHIP_NOALLOC_SCRATCH_BYTES) +
+                  "-byte static scratch; use pyhip_assemble");
+        return PYHIP_ERR_INVALID;
+    }
+    void *base = nullptr;
+    if ((rc = scratch_address(&base))) return rc;
+    void *Er = base;
+    void *Ej = flag ? (void *)((unsigned char *)base + er_bytes) : nullptr;
+    if ((rc = assemble_into(p, dofs, flag, deriv_param, precision, residual, values, Er, Ej))) return rc;
+    // the last FP64 element matrices are not in the problem's own buffer
+    if (precision == PYHIP_F64 && flag) p->have_ej64 = false;
     return PYHIP_OK;
 }
 
