@@ -4,17 +4,22 @@
 Workload (headline): the batched sweep of independent 2D Bratu problems
 -lap u = lambda e^u on the unit square, 512 problems, N ~ U{16..128} per side, P1 or P2
 50/50, lambda ~ U(0.5, 3.0), u = 0 on the boundary, all device-resident in one batch.
-A step is one full steady Newton solve of every problem from u = 0 (pyoomph's Newton
-semantics: tolerance 1e-8 on max |R|, always one step); the Jacobians are assembled by
-the FP64 element kernels and the linear systems solved by batched preconditioned CG
-(smoothed-aggregation multigrid V-cycle by default, Jacobi with --precond jacobi), each
-problem with its own scalars and convergence test.
+A step is one full steady Newton solve of every problem from u = 0: pyoomph's Newton
+(max |R| < 1e-8, always one step) with, in addition, an FP64 update below 1e-6 of the
+solution (on fine meshes a small residual alone does not make a small error), inexact
+linear solves (batched preconditioned CG to 1e-3 relative residual per Newton step:
+smoothed-aggregation multigrid V-cycle by default, Jacobi with --precond jacobi), each
+problem with its own scalars and convergence tests. FP64 (the value), and mixed precision
+beside it: FP32 element kernels for the Jacobian, one FP32 CG solve with the FP32 V-cycle
+per Newton step, FP64 residual and norms (the Newton loop is the FP64 refinement).
 
 Reported: solves/s over the timed steps (value), p50/p99 of the per-problem completion
-time inside a step, the max relative error against the FP64 CPU oracle on a sample,
-FP64 assembly GB/s against the 8 TB/s HBM peak, the roofline of the dominant kernel
-(HIP events on the problem's stream around every launch), the plain FP64
-single-problem GPU path on a sample, and the oracle timed on the host (cpu_baseline).
+time inside a step, the max relative error against the converged FP64 CPU oracle
+(SuperLU Newton to max|R| < 1e-13) on problems across the size range, FP64 assembly GB/s
+against the 8 TB/s HBM peak, the roofline of the dominant kernel (HIP events on the
+problem's stream around every launch), the single-problem GPU paths on a sample (plain
+FP64 CG + Jacobi, and the batch's own solver), and the reference's CPU path as a per-core
+process farm on the host (cpu_baseline).
 """
 import argparse
 import json
@@ -40,24 +45,27 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=64,
                     help="problems in the stratified CPU-baseline sample")
-    ap.add_argument("--cpu-seconds", type=float, default=40.0,
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="bound on the CPU-baseline sample (seconds of oracle work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--no-other", action="store_true", help="skip the run in the other precision")
     ap.add_argument("--precision", choices=("f64", "mixed"), default="f64",
                     help="f64: FP64 Jacobian and Krylov; mixed: FP32 element kernels for the "
                          "Jacobian and FP32 inner CG inside FP64 iterative refinement (FP64 "
                          "residuals, updates and norms)")
     ap.add_argument("--precond", choices=("jacobi", "amg", "ilu0"), default="amg",
                     help="CG preconditioner: Jacobi, or the smoothed-aggregation multigrid V-cycle")
+    ap.add_argument("--correction-tol", type=float, default=None,
+                    help="Newton also stops only once max|dx| <= this * max|x| (default: per precision)")
     ap.add_argument("--err-sample", type=int, default=6,
                     help="problems (spread over the size range, largest included) checked against "
                          "the FP64 oracle")
     ap.add_argument("--serving", default=None, metavar="RATES",
                     help="instead of the sweep: serve a seeded Poisson-arrival stream of mixed requests "
                          "at each of these comma-separated rates (req/s) and print one JSON line per rate")
-    ap.add_argument("--serving-requests", type=int, default=48)
-    ap.add_argument("--serving-precond", choices=("jacobi", "amg"), default="jacobi",
+    ap.add_argument("--serving-requests", type=int, default=500)
+    ap.add_argument("--serving-precond", choices=("jacobi", "amg"), default="amg",
                     help="preconditioner of the CG class of served requests")
     ap.add_argument("--full", choices=("cavity256", "rd64", "cavity_startup"), default=None,
                     help="instead of the sweep: one full-size single-problem solve, the 256x256 "
@@ -140,78 +148,105 @@ def wl_ndof(spec):
     return m * m
 
 
-# mixed precision: Newton stops once its FP64 update is below this fraction of the
-# solution as well (an FP32 Jacobian converges linearly; the update bounds the error)
-MIXED_CORRECTION_TOL = 1e-10
+# Newton stopping: pyoomph's max|R| < 1e-8 and, since a small residual does not make a small
+# error on fine meshes (a smooth error e leaves a residual ~h^2 e), an FP64 update below
+# CORRECTION_TOL of the solution. Inexact Newton: each linear solve stops at KRYLOV_RTOL
+# relative residual (FP64 CG, or one FP32 CG solve whose FP64 refinement is the Newton loop).
+CORRECTION_TOL = 1e-6
+KRYLOV_RTOL = 1e-3
+# accuracy reference: the oracle's Newton run to max|R| < 1e-13 (the converged FP64 solution)
+ORACLE_TOL = 1e-13
 
 
-def configure(P, precision="f64", precond="jacobi"):
+def configure(P, precision="f64", precond="jacobi", correction_tol=CORRECTION_TOL, inexact=True):
+    """Sweep solver settings. inexact=False keeps pyoomph's Newton (max|R| < 1e-8 alone)
+    with tight linear solves: the transient and continuation modes compare step by step
+    with the oracle's Newton iterates."""
     P.krylov = "cg"
     P.preconditioner = precond
     P.jacobian_precision = "f32" if precision == "mixed" else "f64"
     P.linear_precision = "f32" if precision == "mixed" else "f64"
     P.gmres_max_iters = 20000
-    if precision == "mixed":
-        P.newton_correction_tolerance = MIXED_CORRECTION_TOL
+    if inexact:
+        P.gmres_rtol = KRYLOV_RTOL
+        P.rtol_f32 = KRYLOV_RTOL
+        P.max_refinements = 1
+        P.newton_correction_tolerance = correction_tol
         P.max_newton_iterations = 20
 
 
 def serving_main(args):
     """Served Newton solves: requests (2D P2 Poisson, 3D Q2 reaction-diffusion, 2D
-    Taylor-Hood cavity) of 5 k - 600 k unknowns (cavity up to 12 k) arrive as a seeded
-    Poisson process, each with its mesh; admission under a 200 GB HBM budget, continuous
-    batching per solver class on 4 worker streams (pattern set-up, upload, Newton solve
-    and download inside the latency). Per rate: solves/s over the stream (first arrival
-    to last completion), p50/p99 latency (arrival to result), and the oracle's error on
-    the smallest request of each kind."""
+    Taylor-Hood cavity) on the served mesh sizes (5 k - 490 k unknowns, cavity 6 k - 11 k)
+    arrive as a seeded Poisson process, each with its own data. The server starts with one
+    slot pool per mesh class (mesh, pattern, solver plans and workspace on the device,
+    within a 200 GB HBM budget; start-up reported, not timed); requests join their pool at
+    Newton-step boundaries and leave when converged (slot load, Newton steps and download
+    inside the latency), at most 4 pools stepping at once. Per rate: solves/s over the
+    stream (first arrival to last completion), p50/p99 latency (arrival to result), mean
+    slot occupancy, and the error against the oracle's converged solution on a sample of
+    requests (the smallest and one mid-size request of each kind)."""
     import torch
-    from pyoomph_amd.serving import GpuExecutor, ServingLoop, poisson_stream, summarize
+    from pyoomph_amd.serving import (SIZES, GpuPools, ServingLoop, attach_payloads, ns_reynolds, poisson_stream,
+                                     summarize)
     from oracle import newton as on
     from tests.oracle_bridge import oracle_problem
     torch.cuda.set_device(0)
-    ex = GpuExecutor(cg_precond=args.serving_precond)
-    # warm the library and the element kernels once (not timed)
-    ServingLoop(ex, workers=1).run(poisson_stream(1000.0, 3, seed=99, dof_range=(2000, 4000),
-                                                  ns_max_dofs=3000), realtime=False)
+    ex = GpuPools(cg_precond=args.serving_precond)
+    loop = ServingLoop(ex, hbm_budget=200e9, workers=4)
+    t0 = time.perf_counter()
+    sigs = [(k, N) for k, v in SIZES.items() for N in v]
+    for sig in sigs:
+        loop.prepare([sig])
+        print("pool %s-%d ready (%d slots, %.2f GB)" % (sig[0], sig[1], loop.pool_of[sig].slots,
+                                                       loop.pool_of[sig].device_bytes / 1e9), file=sys.stderr, flush=True)
+    startup = time.perf_counter() - t0
     for rate in [float(r) for r in args.serving.split(",")]:
-        reqs = poisson_stream(rate, args.serving_requests, seed=args.seed,
-                              dof_range=(5_000, 600_000), ns_max_dofs=12_000)
-        for q in reqs:      # requests arrive with their meshes (built before the clock starts)
-            q.attach_payload()
-        loop = ServingLoop(ex, hbm_budget=200e9, workers=4, max_batch_dofs=4_000_000)
+        reqs = attach_payloads(poisson_stream(rate, args.serving_requests, seed=args.seed))
+        loop.steps = {sig: [] for sig in loop.pool_of}
         torch.cuda.synchronize()
         res = loop.run(reqs, realtime=True)
         s = summarize(res)
         errs = {}
+        byrid = sorted(reqs, key=lambda q: q.rid)
         for kind in ("poisson", "rd", "ns"):
-            cand = [(q, r) for q, r in zip(sorted(reqs, key=lambda q: q.rid), res) if q.kind == kind]
-            if not cand:
-                continue
-            q, r = min(cand, key=lambda qr: qr[0].ndof)
-            mp = q.mesh_problem()
-            if kind == "poisson":
-                op, tol = oracle_problem("poisson", N=q.N, seed=q.seed), 1e-8
-            elif kind == "rd":
-                op, tol = oracle_problem("rd", N=q.N, f=mp.equations.source_field), 1e-8
-            else:
-                op, tol = oracle_problem("ns", N=q.N, Re=100.0), 1e-10
-            xo = on.newton_solve(op, np.zeros(op.ndof), tol=tol)
-            errs[kind] = float(np.max(np.abs(r.dofs - xo)) / max(np.max(np.abs(xo)), 1e-300))
+            cand = sorted([(q.ndof, q.rid) for q in byrid if q.kind == kind])
+            pick = {cand[0][1]} if cand else set()
+            mid = [rid for nd, rid in cand if nd <= 35_000]
+            if mid:
+                pick.add(mid[-1])
+            for rid in sorted(pick):
+                q, r = byrid[rid], res[rid]
+                if kind == "poisson":
+                    op = oracle_problem("poisson", N=q.N, seed=q.seed)
+                elif kind == "rd":
+                    op = oracle_problem("rd", N=q.N, f=q.payload[1])
+                else:
+                    op = oracle_problem("ns", N=q.N, Re=ns_reynolds(q.seed))
+                xo = on.newton_solve(op, np.zeros(op.ndof), tol=1e-12, max_iter=30)
+                e = float(np.max(np.abs(r.dofs - xo)) / max(np.max(np.abs(xo)), 1e-300))
+                errs["%s-%d" % (kind, q.N)] = max(e, errs.get("%s-%d" % (kind, q.N), 0.0))
         kinds = {k: sum(q.kind == k for q in reqs) for k in ("poisson", "rd", "ns")}
+        occ = loop.occupancy()
+        busy = sum(v["mean_busy"] * v["steps"] for v in occ.values())
+        steps = sum(v["steps"] for v in occ.values())
         out = {"metric": "served Newton solves/s & p99 latency at 1e-8 rel err", "value": s["solves_per_s"],
                "unit": "solves/s", "n_gpus": 1, "higher_is_better": True, "rate_req_per_s": rate,
                "requests": s["requests"], "p50_ms": s["p50_s"] * 1e3, "p99_ms": s["p99_s"] * 1e3,
-               "mean_batch": s["mean_batch"], "batches": len(loop.batches),
-               "peak_reserved_GB": loop.peak_reserved / 1e9,
-               "converged": int(sum(bool(r.stats.get("converged")) for r in res)),
-               "max_rel_err_vs_oracle": max(errs.values()) if errs else None, "err_by_kind": errs,
+               "mean_newton_steps": s["mean_newton_steps"], "converged": s["converged"],
+               "pool_steps": steps, "mean_busy_slots_per_step": busy / max(steps, 1), "pools": occ,
+               "max_rel_err_vs_oracle": max(errs.values()) if errs else None, "err_sample": errs,
                "dtype": "f64", "data": "synthetic",
                "config": {"workload": "serving: Poisson arrivals, P2 Poisson / Q2 reaction-diffusion / "
-                                      "Taylor-Hood cavity requests, 5k-600k unknowns (cavity <= 12k)",
+                                      "Taylor-Hood cavity requests on 14 mesh classes, 5k-490k unknowns",
                           "mix": kinds, "total_dofs": int(sum(q.ndof for q in reqs)), "seed": args.seed,
-                          "hbm_budget_GB": 200, "workers": 4, "max_batch_dofs": 4_000_000,
-                          "cg_precond": args.serving_precond}}
+                          "hbm_budget_GB": 200, "pools": len(loop.pool_of),
+                          "pool_GB": round(loop.reserved / 1e9, 2), "startup_s": round(startup, 1),
+                          "workers": loop.workers, "cg_precond": args.serving_precond,
+                          "solver": "inexact Newton (Krylov rtol %g, stop at max|R| < 1e-8 (cavity 1e-10) and "
+                                    "max|dx| <= %g max|x|)" % (ex.krylov_rtol, ex.correction_tol)}}
         print(json.dumps(out), flush=True)
+    loop.close()
 
 
 def full_main(args):
@@ -305,7 +340,7 @@ def transient_main(args):
     torch.cuda.set_device(0)
     specs = wl.bratu_sweep_specs(args.nprob, seed=args.seed)
     P = Problem([wl.bratu(N, sp, lam) for N, sp, lam in specs])
-    configure(P, "f64", args.precond)
+    configure(P, "f64", args.precond, inexact=False)
     dts = [0.02] * args.transient
     P.set_initial_condition(np.zeros(P.ndof))     # warmup: plans, tuning
     P.unsteady_newton_solve(dts[0])
@@ -423,33 +458,47 @@ def main():
     specs = wl.bratu_sweep_specs(args.nprob, seed=args.seed)
     parts = [wl.bratu(N, sp, lam) for N, sp, lam in specs]
     P = Problem(parts)
-    configure(P, args.precision, args.precond)
+    ctol = CORRECTION_TOL if args.correction_tol is None else args.correction_tol
     print("# batch: %d problems, %d dofs, %d nnz" % (len(parts), P.ndof, P.nnz), file=sys.stderr, flush=True)
+    order = sorted(range(len(specs)), key=lambda i: wl_ndof(specs[i]))
 
-    def step():
-        P.dofs.zero_()
-        return P.solve()
+    def timed_sweep(precision, with_timing):
+        """warmup + timed steps of the whole sweep in one precision; returns
+        (solves/s, ms/step, latencies, stats, kernel families, spmv bytes, dofs)."""
+        configure(P, precision, args.precond, correction_tol=ctol)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    _lib.check(_lib.load().pyhip_timing_enable(P._handle, 1))
-    lat = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = step()
-        lat += [s["done_ms"] for s in st]
-        assert all(s["converged"] for s in st)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    fams = {f: timing(P, f) for f in ("element", "gather_values", "gather_residual", "spmv",
-                                       "precond", "vector")}
-    spmv_work = timing_bytes(P, "spmv")
-    _lib.check(_lib.load().pyhip_timing_enable(P._handle, 0))
-    x_gpu = P.get_current_dofs()
-    solves_per_s = args.nprob * args.steps / elapsed
-    ms_per_step = elapsed / args.steps * 1e3
+        def step():
+            P.dofs.zero_()
+            return P.solve()
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        if with_timing:
+            _lib.check(_lib.load().pyhip_timing_enable(P._handle, 1))
+        lat = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = step()
+            lat += [s["done_ms"] for s in st]
+            assert all(s["converged"] for s in st)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        fams, work = None, None
+        if with_timing:
+            fams = {f: timing(P, f) for f in ("element", "gather_values", "gather_residual", "spmv",
+                                               "precond", "vector")}
+            work = timing_bytes(P, "spmv")
+            _lib.check(_lib.load().pyhip_timing_enable(P._handle, 0))
+        return (args.nprob * args.steps / elapsed, elapsed / args.steps * 1e3, np.array(lat), st, fams, work,
+                P.get_current_dofs())
+
+    # the headline precision, timed with the kernel-family events; then the other precision
+    sps, ms_per_step, lat, st, fams, spmv_work, x_gpu = timed_sweep(args.precision, True)
+    other = "mixed" if args.precision == "f64" else "f64"
+    other_run = None if args.no_other else timed_sweep(other, False)
+    configure(P, args.precision, args.precond, correction_tol=ctol)
 
     # FP64 assembly (element kernels + gathers), timed with the same events
     _lib.check(_lib.load().pyhip_timing_enable(P._handle, 1))
@@ -469,7 +518,7 @@ def main():
     avg_ms = ms_l / max(n_l, 1)
     if dom == "spmv":
         # the CG visits only the rows of problems still iterating, and the multigrid levels
-        # are smaller than the matrix: the library counts the bytes each launch processed
+        # are smaller than the matrix: the library counts the bytes the launches processed
         alg = spmv_work / max(n_l, 1) if spmv_work else spmv_bytes(P)
     elif dom in ("element", "gather_values", "gather_residual"):
         alg = assembly_bytes(P)
@@ -485,17 +534,17 @@ def main():
             if line.startswith("traffic_bytes_per_launch="):
                 traffic = float(line.split("=", 1)[1])
 
-    # plain FP64 single-problem GPU path on a sample (one problem at a time, CG + Jacobi),
-    # and the same sample one problem at a time with the batch's own solver, so that the
-    # batching gain and the preconditioner gain are reported apart
+    # single-problem GPU paths on a sample (one problem per Problem, pattern and plan set up
+    # outside the clock, the same stopping rules): the plain FP64 path (CG + Jacobi) and the
+    # batch's own solver -- the batching gain and the preconditioner gain apart
     single = single_same = None
     if not args.no_single:
         def one_at_a_time(precision, precond):
             t_single, n_single = 0.0, 0
             for N, sp, lam in specs[:8]:
                 Q = Problem(wl.bratu(N, sp, lam))
-                configure(Q, precision, precond)
-                Q.solve()                      # pattern / plan set-up outside the clock
+                configure(Q, precision, precond, correction_tol=ctol)
+                Q.solve()
                 Q.dofs.zero_()
                 torch.cuda.synchronize()
                 t = time.perf_counter()
@@ -508,57 +557,29 @@ def main():
         single = one_at_a_time("f64", "jacobi")
         single_same = one_at_a_time(args.precision, args.precond)
 
-    # CPU-baseline leg: the oracle checks accuracy (problems spread over the size range,
-    # largest included, plus the timed sample) and is timed on the smallest problems
-    cpu = None
-    errs = []
-    if not args.no_cpu_baseline:
+    # accuracy: every sampled problem against the converged FP64 oracle (SuperLU Newton to
+    # max|R| < ORACLE_TOL), problems spread over the size range, the largest included
+    errs, errs_other = [], []
+    if args.err_sample > 0:
         from oracle import newton as on
         from tests.oracle_bridge import oracle_problem
-        order = sorted(range(len(specs)), key=lambda i: (specs[i][0] * (2 if specs[i][1] == "C2" else 1)))
-
-        def err_of(i, xo):
-            xg = P.sub_dofs(i, x_gpu)
-            return float(np.max(np.abs(xg - xo)) / max(np.max(np.abs(xo)), 1e-300))
-
-        if args.err_sample > 0:
-            picks = sorted(set(order[int(round(k * (len(order) - 1) / max(args.err_sample - 1, 1)))]
-                               for k in range(args.err_sample)))
-            for i in picks:
-                N, sp, lam = specs[i]
-                op = oracle_problem("bratu", N=N, space=sp, lam=lam)
-                errs.append(err_of(i, on.newton_solve(op, np.zeros(op.ndof))))
-            print("# accuracy sample: %s" % [(specs[i][0], specs[i][1]) for i in picks], file=sys.stderr,
-                  flush=True)
-        # timed sample: a stratified slice of the sweep (every len/k-th problem in order of
-        # unknowns, k = --cpu-sample), so its solves/s is that of the sweep's size mix
-        k = min(args.cpu_sample, len(order))
-        strat = [order[int((j + 0.5) * len(order) / k)] for j in range(k)]
-        n_cpu, t_cpu = 0, 0.0
-        for i in strat:
+        picks = sorted(set(order[int(round(k * (len(order) - 1) / max(args.err_sample - 1, 1)))]
+                           for k in range(args.err_sample)))
+        for i in picks:
             N, sp, lam = specs[i]
             op = oracle_problem("bratu", N=N, space=sp, lam=lam)
-            t = time.perf_counter()
-            xo = on.newton_solve(op, np.zeros(op.ndof))
-            t_cpu += time.perf_counter() - t
-            n_cpu += 1
-            errs.append(err_of(i, xo))
-            if t_cpu > args.cpu_seconds:
-                break
-        done = strat[:n_cpu]
-        cpu = {"value": n_cpu / t_cpu, "unit": "solves/s", "cores": 1, "kind": "port",
-               "host_cores": os.cpu_count(),
-               "sample": "%d of %d problems of a stratified slice of the sweep (every %d-th by unknowns, "
-                         "%d..%d unknowns), oracle C element code + vectors_of_pairs CSR + scipy SuperLU, "
-                         "one thread, %.1f s" % (n_cpu, k, len(order) // k,
-                                                 min(wl_ndof(specs[i]) for i in done),
-                                                 max(wl_ndof(specs[i]) for i in done), t_cpu)}
+            xo = on.newton_solve(op, np.zeros(op.ndof), tol=ORACLE_TOL, max_iter=30)
+            nrm = max(np.max(np.abs(xo)), 1e-300)
+            errs.append(float(np.max(np.abs(P.sub_dofs(i, x_gpu) - xo)) / nrm))
+            if other_run is not None:
+                errs_other.append(float(np.max(np.abs(P.sub_dofs(i, other_run[6]) - xo)) / nrm))
+        print("# accuracy sample: %s" % [(specs[i][0], specs[i][1]) for i in picks], file=sys.stderr, flush=True)
 
-    max_err = max(errs) if errs else None
-    lat = np.array(lat)
+    cpu = None if args.no_cpu_baseline else cpu_farm_baseline(specs, order, args)
+
     out = {
         "metric": METRIC,
-        "value": solves_per_s,
+        "value": sps,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -572,14 +593,13 @@ def main():
         "config": {"workload": "bratu_sweep: %d independent -lap u = lambda e^u problems, "
                                "N~U{16..128}, P1/P2 50/50, lambda~U(0.5,3)" % args.nprob,
                    "problems": args.nprob, "dofs": P.ndof, "nnz": P.nnz,
-                   "solver": ("Newton + batched CG/%s, FP64" % args.precond if args.precision == "f64" else
-                              "Newton (FP64 residual, stops at max|R| < 1e-8 and max|dx| <= %g max|x|) + "
-                              "FP32 element kernels for J, FP32 batched CG/%s inside FP64 iterative refinement"
-                              % (MIXED_CORRECTION_TOL, args.precond)),
+                   "solver": solver_text(args.precision, args.precond, ctol),
                    "precision": args.precision, "preconditioner": args.precond, "seed": args.seed},
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
-        "max_rel_err_vs_oracle": max_err,
+        "max_rel_err_vs_oracle": max(errs) if errs else None,
+        "oracle": "SuperLU Newton (the oracle) to max|R| < %g on %d problems across the size range" %
+                  (ORACLE_TOL, len(errs)),
         "assembly_fp64_ms": asm_ms,
         "assembly_element_kernel_ms": asm_el,
         "assembly_GBps": asm_gbs,
@@ -587,18 +607,87 @@ def main():
         "copy_kernel_GBps": copy_gbs,
         "assembly_frac_of_copy": asm_gbs / copy_gbs,
         "single_problem_fp64_solves_per_s": single,
-        "batched_speedup_vs_single": (solves_per_s / single) if single else None,
+        "batched_speedup_vs_single": (sps / single) if single else None,
         "single_problem_same_solver_solves_per_s": single_same,
-        "batched_speedup_vs_single_same_solver": (solves_per_s / single_same) if single_same else None,
+        "batched_speedup_vs_single_same_solver": (sps / single_same) if single_same else None,
         "kernel_ms": {f: {"launches": v[0], "ms": v[1]} for f, v in fams.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg_ms,
-                     "share_of_step": ms_l / (elapsed * 1e3), "frac_of_copy": achieved / copy_gbs,
-                     "full_matrix_bytes": spmv_bytes(P)},
+                     "share_of_step": ms_l / (ms_per_step * args.steps),
+                     "frac_of_copy": achieved / copy_gbs, "full_matrix_bytes": spmv_bytes(P)},
         "cpu_baseline": cpu,
     }
+    if other_run is not None:
+        o_sps, o_ms, o_lat = other_run[0], other_run[1], other_run[2]
+        out[other + "_precision" if other == "mixed" else "fp64_precision"] = {
+            "value": o_sps, "ms_per_step": o_ms, "p50_ms": float(np.percentile(o_lat, 50)),
+            "p99_ms": float(np.percentile(o_lat, 99)),
+            "max_rel_err_vs_oracle": max(errs_other) if errs_other else None,
+            "speedup_vs_single_problem_fp64": (o_sps / single) if single else None,
+            "solver": solver_text(other, args.precond, ctol)}
     print(json.dumps(out))
+
+
+def solver_text(precision, precond, ctol):
+    stop = "Newton stops at max|R| < 1e-8 and max|dx| <= %g max|x| (FP64 norms)" % ctol
+    if precision == "f64":
+        return "%s; batched CG/%s in FP64 to %g relative residual per Newton step" % (stop, precond, KRYLOV_RTOL)
+    return ("%s; FP32 element kernels for J, one batched FP32 CG/%s solve to %g per Newton step (FP32 "
+            "V-cycle), FP64 residual: the Newton loop is the FP64 refinement" % (stop, precond, KRYLOV_RTOL))
+
+
+def _farm_worker(job):
+    """One oracle solve (CPU baseline farm): the native build of the oracle's element code."""
+    os.environ["ORACLE_BUILD"] = "native"
+    from oracle import newton as on
+    from tests.oracle_bridge import oracle_problem
+    N, sp, lam = job
+    op = oracle_problem("bratu", N=N, space=sp, lam=lam)
+    on.newton_solve(op, np.zeros(op.ndof), tol=ORACLE_TOL, max_iter=30)
+    return 1
+
+
+def cpu_farm_baseline(specs, order, args):
+    """The reference's CPU path timed as its parameter-scan farm (pyoomph/utils/
+    paramscan.py:249-277): one single-threaded process per core, each running whole
+    Newton solves (oracle element code built -O3 -march=native on this host, vectors_of_pairs
+    CSR, SuperLU), over a stratified slice of the sweep, for about --cpu-seconds."""
+    import multiprocessing as mp
+    import subprocess
+    subprocess.check_call(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "native"])
+    cores = max(1, min(16, os.cpu_count() or 1))
+    k = min(args.cpu_sample, len(order))
+    strat = [order[int((j + 0.5) * len(order) / k)] for j in range(k)]
+    # a seeded shuffle of the slice, repeated: whatever the time bound cuts off, the solves
+    # done are a mix of sizes, not the small end of the size order
+    rng = np.random.default_rng(1)
+    jobs = [specs[strat[j]] for j in np.concatenate([rng.permutation(len(strat)) for _ in range(4)])]
+    env_threads = {v: os.environ.get(v) for v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    for v in env_threads:
+        os.environ[v] = "1"
+    ctx = mp.get_context("spawn")
+    done = 0
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        it = pool.imap_unordered(_farm_worker, jobs)
+        for _ in it:
+            done += 1
+            if time.perf_counter() - t0 > args.cpu_seconds and done >= cores:
+                break
+        elapsed = time.perf_counter() - t0
+        pool.terminate()
+    for v, val in env_threads.items():
+        if val is None:
+            os.environ.pop(v, None)
+        else:
+            os.environ[v] = val
+    return {"value": done / elapsed, "unit": "solves/s", "cores": cores, "kind": "port",
+            "host_cores": os.cpu_count(),
+            "sample": "%d solves in %.1f s from a stratified slice of the sweep (%d problems, every %d-th by "
+                      "unknowns, in seeded random order), %d single-threaded processes (the parameter-scan farm); oracle "
+                      "C element code built -O3 -march=native + vectors_of_pairs CSR + scipy SuperLU, "
+                      "Newton to max|R| < %g" % (done, elapsed, k, len(order) // k, cores, ORACLE_TOL)}
 
 
 if __name__ == "__main__":

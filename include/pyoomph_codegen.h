@@ -1,0 +1,45 @@
+/*
+ * Run-time code generation: element kernels of weak forms generated as HIP source
+ * (pyoomph_amd/codegen) and compiled for gfx950 with hiprtc in the running process.
+ *
+ * The reference generates C for each weak form (src/codegen.cpp:4684-4810 write_code,
+ * :3912-4109 write_generic_RJM), compiles it with the system compiler and loads it with
+ * dlopen (src/ccompiler.cpp:191-248; compiler registry pyoomph/generic/ccompiler.py:82-104).
+ * Here the generated source is a batched element kernel, hiprtc compiles it and the code
+ * object is loaded as a HIP module; the form then gets an id that pyhip_subproblem_desc
+ * accepts like the built-in forms (one value per node, the element family it was
+ * generated for), and pyhip_assemble / pyhip_newton_solve / pyhip_arclength_step launch
+ * its kernels. The source must define four extern "C" kernels <name>_f64, <name>_f64_tr,
+ * <name>_f32, <name>_f32_tr taking the library's element-kernel arguments by value (the
+ * shape the generator emits). FP64 kernels are compiled with -ffp-contract=off, like the
+ * compiled FP64 element kernels.
+ */
+#ifndef PYOOMPH_CODEGEN_H
+#define PYOOMPH_CODEGEN_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYHIP_GENERATED_FORM_BASE 100   /* ids of generated forms: 100, 101, ...       */
+
+/* Compiles `source` (with the given headers, as hiprtc takes them) and, when `load` is
+ * non-zero, loads it on the current device and registers the form for element family
+ * `elem_kind`: symmetric != 0 marks a symmetric positive definite Jacobian (conjugate
+ * gradients apply), has_param_derivative != 0 a global parameter 0 with dR/dp. *form_id
+ * receives the new id (load only); *code_size the size of the compiled code object. The
+ * compiler log of a failure is in pyhip_last_error(). With load == 0 nothing touches the
+ * GPU (a compile check). */
+int pyhip_codegen_compile(const char *name, const char *source, int nheaders, const char *const *header_sources,
+                          const char *const *header_names, int elem_kind, int symmetric, int has_param_derivative,
+                          int load, int *form_id, size_t *code_size);
+
+/* Registered generated forms (0 .. count-1 are ids PYHIP_GENERATED_FORM_BASE + k). */
+int pyhip_codegen_count(int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -262,6 +262,31 @@ void pyhip_newton_opts_default(pyhip_newton_opts *o);
 int pyhip_newton_solve(pyhip_problem *p, double *dofs_dev, const pyhip_newton_opts *opts,
                        pyhip_newton_stats *stats);
 
+/* One Newton iteration of pyhip_newton_solve for the sub-problems with mask[s] != 0 that
+ * have not converged or failed yet; state[s] (host, [nsub]) carries each sub-problem's
+ * progress between calls, newton_iters = 0 starting a new solve (the caller puts the
+ * start values into its dofs). Convergence and failure follow pyhip_newton_solve. Every
+ * sub-problem's iteration is its own (per-sub-problem assembly rows, Krylov scalars and
+ * tests; the multigrid hierarchy is rebuilt from every call's Jacobian), so a
+ * sub-problem gives the same iterates whichever others run in the same calls.
+ * Sub-problems can thus start and finish at any call: the slots of a serving loop. No counterpart in the reference, which solves one
+ * problem per process (pyoomph/utils/paramscan.py:249-277). */
+typedef struct {
+    int newton_iters;       /* in/out: Newton steps taken in this solve; 0 starts one      */
+    int converged;          /* in/out: 1 once converged (the sub-problem is then skipped)   */
+    int failed;             /* in/out: 1 once NewtonSolverError (max_residuals / max_iter) */
+    double maxres;          /* out: max |R| after the last step                            */
+    double max_update;      /* out: max |dx| of the last step                              */
+    int linear_iters;       /* in/out: accumulated inner Krylov iterations                  */
+    int linear_failures;    /* in/out: accumulated linear solves above their tolerance      */
+} pyhip_newton_step_state;
+int pyhip_newton_step(pyhip_problem *p, double *dofs_dev, const pyhip_newton_opts *opts, const int32_t *mask,
+                      pyhip_newton_step_state *state);
+
+/* Replaces one sub-problem's auxiliary nodal field (reaction-diffusion source f): aux
+ * [nnode] host values, nnode the sub-problem's node count. */
+int pyhip_problem_set_aux(pyhip_problem *p, int sub, const double *aux, int64_t nnode);
+
 /* ---- arclength continuation ------------------------------------------------------------------
  * One pseudo-arclength step in global parameter `param` (0: the Bratu lambda) for every
  * sub-problem with active[s] != 0, each with its own step length ds[s] and state:

@@ -30,12 +30,17 @@ TH_SLOTS = [(0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2), (3, 
 
 
 def lib():
+    """The oracle's C library: the exact build (-ffp-contract=off, the checker), or with
+    ORACLE_BUILD=native the build with the reference's compiler defaults (-O3
+    -march=native, contraction on; ccompiler.py:213-221), used only to time the CPU
+    baseline."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "_build", "liboracle.so")
+        native = os.environ.get("ORACLE_BUILD") == "native"
+        path = os.path.join(_HERE, "_build", "liboracle_native.so" if native else "liboracle.so")
         if not os.path.exists(path):
             import subprocess
-            subprocess.check_call(["make", "-s", "-C", _HERE])
+            subprocess.check_call(["make", "-s", "-C", _HERE] + (["native"] if native else []))
         L = ctypes.CDLL(path)
         dp = ctypes.POINTER(ctypes.c_double)
         lp = ctypes.POINTER(ctypes.c_long)

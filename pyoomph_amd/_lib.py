@@ -74,6 +74,12 @@ class NewtonStats(ctypes.Structure):
                 ("done_ms", ctypes.c_double)]
 
 
+class NewtonStepState(ctypes.Structure):
+    _fields_ = [("newton_iters", ctypes.c_int), ("converged", ctypes.c_int), ("failed", ctypes.c_int),
+                ("maxres", ctypes.c_double), ("max_update", ctypes.c_double), ("linear_iters", ctypes.c_int),
+                ("linear_failures", ctypes.c_int)]
+
+
 class ArclengthState(ctypes.Structure):
     _fields_ = [("theta_squared", ctypes.c_double), ("parameter_derivative", ctypes.c_double),
                 ("parameter_current", ctypes.c_double), ("ds_current", ctypes.c_double),
@@ -99,6 +105,14 @@ SIGNATURES = {
                                       ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "pyhip_assemble_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "pyhip_codegen_compile": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)]),
+    "pyhip_codegen_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "pyhip_newton_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
+                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(NewtonStepState)]),
+    "pyhip_problem_set_aux": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dp, ctypes.c_int64]),
     "pyhip_jit_element_init": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "pyhip_jit_set_form_params": (ctypes.c_int, [ctypes.c_void_p, c_dp, ctypes.c_int]),
     "pyhip_superlu": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)] * 4 + [c_dp] + [ctypes.POINTER(ctypes.c_int)] * 2 +

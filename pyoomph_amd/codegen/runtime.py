@@ -1,0 +1,104 @@
+"""Compiling generated element kernels into the running library, and using them.
+
+``GeneratedEquation(form, space=..., params=...)`` is an equation object like the built-in
+ones (pyoomph_amd.equations): a ``MeshProblem`` on a mesh of the form's element family
+with it assembles and solves through the generated kernels. The first use of a (form,
+element family) pair emits the kernel source (``scalar.emit_kernels``), compiles it with
+hiprtc inside libpyoomph_hip.so and loads it on the GPU (``pyhip_codegen_compile``); later
+uses take the registered id -- the reference likewise compiles each element's code once
+and keeps the loaded library (src/ccompiler.cpp:191-248).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .. import _lib
+from ..equations import Equations
+from . import scalar
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+COMMON_HEADER = os.path.join(os.path.dirname(_HERE), "csrc", "element_common.h")
+
+_registry = {}   # (form name, kind id) -> form id
+
+
+def _signatures(L):
+    L.pyhip_codegen_compile.restype = ctypes.c_int
+    L.pyhip_codegen_compile.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)]
+
+
+def compile_form(form, kind, load=True):
+    """Compiles the form's kernels for element family ``kind`` with hiprtc; with load, also
+    loads them on the current device and returns the form id (cached per form name and
+    family). Without load (no GPU needed) returns the code-object size."""
+    kid = scalar.KIND_IDS[kind] if isinstance(kind, str) else int(kind)
+    key = (form.name, kid)
+    if load and key in _registry:
+        return _registry[key]
+    src, name, info = scalar.emit_kernels(form, kid)
+    L = _lib.load()
+    _signatures(L)
+    with open(COMMON_HEADER) as f:
+        hdr = f.read()
+    hs = (ctypes.c_char_p * 1)(hdr.encode())
+    hn = (ctypes.c_char_p * 1)(b"element_common.h")
+    fid = ctypes.c_int(-1)
+    size = ctypes.c_size_t(0)
+    _lib.check(L.pyhip_codegen_compile(name.encode(), src.encode(), 1, hs, hn, kid, int(info["symmetric"]),
+                                       int(info["has_param_derivative"]), int(bool(load)), ctypes.byref(fid),
+                                       ctypes.byref(size)))
+    if not load:
+        return size.value
+    _registry[key] = fid.value
+    return fid.value
+
+
+class GeneratedEquation(Equations):
+    """A generated scalar weak form on one element family.
+
+    space: "C1" (P1 triangles) or "C2" (P2 triangles, Q2 quadrilaterals, Q2 bricks); the
+    element family is the mesh's, known when the problem is built (``check_mesh``).
+    params: the form's parameter values (``form.nparams`` of them); aux: the nodal field f
+    (forms with aux=True)."""
+
+    def __init__(self, form, *, space="C2", params=(), aux=None):
+        self.gen_form = form
+        self.space = space
+        self._params = np.asarray(params, dtype=np.float64).reshape(-1)
+        if self._params.size != form.nparams:
+            raise ValueError("the form takes %d parameters" % form.nparams)
+        self._aux = None if aux is None else np.asarray(aux, dtype=np.float64)
+        if form.aux and self._aux is None:
+            raise ValueError("the form reads the auxiliary field: pass aux")
+        self.global_parameters = (form.global_param_name,) if form.global_param is not None else ()
+        self._kind = None
+        self.symmetric = False
+
+    def check_mesh(self, mesh):
+        super().check_mesh(mesh)
+        if self._kind is not None and self._kind != scalar.KIND_IDS[mesh.kind]:
+            raise ValueError("one GeneratedEquation object per element family")
+        self._kind = scalar.KIND_IDS[mesh.kind]
+        self.symmetric = self.gen_form.symmetric(scalar.KIND_DIM[self._kind])
+
+    @property
+    def form(self):
+        """The registered form id (compiles and loads the kernels on first use, on the
+        current device)."""
+        if self._kind is None:
+            raise RuntimeError("the equation has not been put on a mesh yet")
+        return compile_form(self.gen_form, self._kind)
+
+    def params(self):
+        return self._params.copy()
+
+    def aux(self, mesh):
+        if self._aux is None:
+            return None
+        if self._aux.shape != (mesh.nnode,):
+            raise ValueError("aux must have one value per node")
+        return self._aux

@@ -78,5 +78,7 @@ int amg_set_fine32(AmgDev &A, const float *vals);
 int amg_apply32(AmgDev &A, hipStream_t st, const float *b, float *z, KernelTimer *tm = nullptr,
                 const int32_t *act = nullptr, const int32_t *act_h = nullptr);
 void amg_free(AmgDev &A);
+// row-dependent spmv-family bytes of one V-cycle per sub-problem (element size esize)
+std::vector<int64_t> amg_vcycle_bytes(const AmgDev &A, int esize);
 
 }  // namespace pyhip

@@ -497,7 +497,7 @@ static void level_spmv(KernelTimer *tm, int mode, int lpr, hipStream_t st, int64
                                             rsub, act);
         if (mode == kSpmvSub) vec = 2 * E * rows;
     }
-    if (tm) tm->add_work("spmv", (E + 4) * nnz + 4 * (rows + 1) + E * ncols + vec);
+    if (tm) tm->add_work("spmv", (E + 4) * nnz + 4 * (rows + 1) + E * ncols + vec, 4);
 }
 
 // sum of v[s] over the sub-problems with act_h[s] != 0 (all without act_h)
@@ -734,6 +734,28 @@ static int amg_apply_t(AmgDev &A, hipStream_t st, const T *b, T *z, KernelTimer 
     }
     ACHECK();
     return PYHIP_OK;
+}
+
+// Row-dependent spmv-family bytes one V-cycle moves for each sub-problem (the terms
+// level_spmv counts, less the 4 bytes per launch), element size esize.
+std::vector<int64_t> amg_vcycle_bytes(const AmgDev &A, int esize) {
+    std::vector<int64_t> out(std::max(0, A.nsub), 0);
+    const int nl = (int)A.lv.size();
+    const int64_t E = esize;
+    for (int s = 0; s < A.nsub; s++) {
+        int64_t b = 0;
+        for (int l = 0; l < nl; l++) {
+            const AmgDevLevel &L = A.lv[l];
+            const AmgDevLevel &C = (l + 1 < nl) ? A.lv[l + 1] : A.bot;
+            const int64_t r = L.h_rows[s], z = L.h_nnz[s], zp = L.h_nnzp[s], rc = C.h_rows[s];
+            b += (E + 4) * z + 4 * r + E * r + 2 * E * r;             // r = b - A x
+            if (L.nc) b += (E + 4) * zp + 4 * rc + E * r + (l + 1 < nl ? 3 : 1) * E * rc;   // restriction
+            if (L.nc) b += (E + 4) * zp + 4 * r + E * rc + 2 * E * r; // prolongation x += P x_c
+            b += (E + 4) * z + 4 * r + E * r + 4 * E * r;             // post-sweep
+        }
+        out[s] = b;
+    }
+    return out;
 }
 
 int amg_apply(AmgDev &A, hipStream_t st, const double *b, double *z, KernelTimer *tm, const int32_t *act,

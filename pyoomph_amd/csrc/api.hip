@@ -113,7 +113,9 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
     for (int s = 0; s < nsub; s++) {
         const auto &d = subs[s];
         PYHIP_REQUIRE(elem_nnode(d.elem_kind) > 0, "unknown element family");
-        PYHIP_REQUIRE(d.form >= 1 && d.form <= 4, "unknown weak form");
+        PYHIP_REQUIRE((d.form >= 1 && d.form <= 4) || gen_form_known(d.form), "unknown weak form");
+        PYHIP_REQUIRE(!gen_form_known(d.form) || gen_form_kind(d.form) == d.elem_kind,
+                      "generated form compiled for another element family");
         PYHIP_REQUIRE(d.form != PYHIP_FORM_NAVIER_STOKES_TH || d.elem_kind == PYHIP_ELEM_QUAD_Q2,
                       "Taylor-Hood needs Q2 quadrilaterals");
         PYHIP_REQUIRE(d.nvalue == form_nvalue(d.form), "values per node do not match the form");
@@ -400,6 +402,16 @@ int pyhip_problem_set_params(pyhip_problem *p, int sub, const double *params, in
     return PYHIP_OK;
 }
 
+int pyhip_problem_set_aux(pyhip_problem *p, int sub, const double *aux, int64_t nnode) {
+    PYHIP_REQUIRE(p && sub >= 0 && sub < p->nsub && aux, "bad arguments");
+    const SubInfo &S = p->subs[sub];
+    PYHIP_REQUIRE(nnode == S.nnode, "aux needs one value per node of the sub-problem's mesh");
+    PYHIP_CHECK_HIP(hipMemcpyAsync(p->aux + S.node_off, aux, (size_t)nnode * sizeof(double), hipMemcpyHostToDevice,
+                                   p->st));
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    return PYHIP_OK;
+}
+
 int pyhip_pattern_copy(const pyhip_problem *p, int32_t *row_start, int32_t *col_index) {
     PYHIP_REQUIRE(p && row_start && col_index, "null argument");
     memcpy(row_start, p->row_start_h.data(), p->row_start_h.size() * sizeof(int32_t));
@@ -420,8 +432,9 @@ static int check_assemble_args(pyhip_problem *p, const double *dofs, int flag, i
     PYHIP_REQUIRE(flag != 2 || deriv_param < 0, "no mass matrix of a parameter derivative");
     PYHIP_REQUIRE(deriv_param == -1 || deriv_param == 0, "unknown global parameter");
     for (auto &G : p->groups)
-        PYHIP_REQUIRE(deriv_param < 0 || G.form == PYHIP_FORM_BRATU || G.form == kFormFlux,
-                      "parameter derivative only defined for the Bratu form");
+        PYHIP_REQUIRE(deriv_param < 0 || G.form == PYHIP_FORM_BRATU || G.form == kFormFlux ||
+                          gen_form_has_pderiv(G.form),
+                      "parameter derivative only defined for forms with a global parameter");
     return PYHIP_OK;
 }
 
@@ -449,6 +462,8 @@ static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int der
             TimeScope ts(&p->timer, "element", p->st);
             if (G.form == kFormFlux) {
                 if ((rc = launch_faces_f64(G.kind, a, p->st))) return rc;
+            } else if (G.form >= PYHIP_GENERATED_FORM_BASE) {
+                if ((rc = launch_generated_f64(G.form, a, p->st))) return rc;
             } else if ((rc = launch_elements_f64(G.kind, G.form, a, p->st)))
                 return rc;
         } else {
@@ -469,6 +484,8 @@ static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int der
             TimeScope ts(&p->timer, "element", p->st);
             if (G.form == kFormFlux) {
                 if ((rc = launch_faces_f32(G.kind, a, p->st))) return rc;
+            } else if (G.form >= PYHIP_GENERATED_FORM_BASE) {
+                if ((rc = launch_generated_f32(G.form, a, p->st))) return rc;
             } else if ((rc = launch_elements_f32(G.kind, G.form, a, p->st)))
                 return rc;
         }

@@ -1,0 +1,161 @@
+// Run-time compiled element kernels (include/pyoomph_codegen.h): hiprtc compilation,
+// module loading, the registry of generated forms and their launches.
+#include <hip/hiprtc.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pyoomph_codegen.h"
+#include "problem.h"
+
+namespace pyhip {
+
+namespace {
+
+struct GenForm {
+    std::string name;
+    int kind = 0;
+    bool symmetric = false, has_pderiv = false;
+    hipModule_t mod = nullptr;
+    hipFunction_t f64 = nullptr, f64_tr = nullptr, f32 = nullptr, f32_tr = nullptr;
+};
+
+std::mutex g_mu;
+std::vector<GenForm> g_forms;
+
+// elements per 256-thread workgroup of the scalar element kernels (element_common.h
+// scalar_epb): 256 / max(nodes, points per phase-A chunk)
+int gen_epb(int kind) {
+    switch (kind) {
+    case ELEM_TRI_P1: return 256 / 3;
+    case ELEM_TRI_P2: return 256 / 7;
+    case ELEM_QUAD_Q2: return 256 / 9;
+    case ELEM_HEX_Q2: return 256 / 27;
+    }
+    return 0;
+}
+
+#define PYHIP_CHECK_RTC(expr)                                                         \
+    do {                                                                              \
+        hiprtcResult _r = (expr);                                                     \
+        if (_r != HIPRTC_SUCCESS) {                                                   \
+            set_error(std::string(#expr) + ": " + hiprtcGetErrorString(_r));          \
+            return PYHIP_ERR_INVALID;                                                 \
+        }                                                                             \
+    } while (0)
+
+int compile(const char *source, const char *name, int nh, const char *const *hs, const char *const *hn,
+            std::vector<char> &code) {
+    hiprtcProgram prog;
+    PYHIP_CHECK_RTC(hiprtcCreateProgram(&prog, source, (std::string(name) + ".hip").c_str(), nh, hs, hn));
+    // FP64 results must follow the written operation order (no contraction), and packed
+    // FP32 instructions are not generated
+    const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    if (r != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        std::string log;
+        if (hiprtcGetProgramLogSize(prog, &ls) == HIPRTC_SUCCESS && ls > 1) {
+            log.resize(ls);
+            (void)hiprtcGetProgramLog(prog, &log[0]);
+        }
+        (void)hiprtcDestroyProgram(&prog);
+        set_error(std::string("hiprtc: ") + hiprtcGetErrorString(r) + "\n" + log);
+        return PYHIP_ERR_INVALID;
+    }
+    size_t cs = 0;
+    PYHIP_CHECK_RTC(hiprtcGetCodeSize(prog, &cs));
+    code.resize(cs);
+    PYHIP_CHECK_RTC(hiprtcGetCode(prog, code.data()));
+    (void)hiprtcDestroyProgram(&prog);
+    return PYHIP_OK;
+}
+
+}  // namespace
+
+bool gen_form_known(int form) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    const int k = form - PYHIP_GENERATED_FORM_BASE;
+    return k >= 0 && k < (int)g_forms.size();
+}
+
+int gen_form_kind(int form) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    return g_forms[form - PYHIP_GENERATED_FORM_BASE].kind;
+}
+
+bool gen_form_has_pderiv(int form) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    const int k = form - PYHIP_GENERATED_FORM_BASE;
+    return k >= 0 && k < (int)g_forms.size() && g_forms[k].has_pderiv;
+}
+
+template <typename T>
+static int launch_gen(int form, const ElemArgs<T> &a, hipStream_t st) {
+    hipFunction_t f;
+    int kind;
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        const GenForm &G = g_forms[form - PYHIP_GENERATED_FORM_BASE];
+        const bool f64 = std::is_same<T, double>::value;
+        f = f64 ? (a.transient ? G.f64_tr : G.f64) : (a.transient ? G.f32_tr : G.f32);
+        kind = G.kind;
+    }
+    PYHIP_REQUIRE(a.hvec == nullptr, "Hessian-vector products are not generated for run-time forms");
+    const int epb = gen_epb(kind);
+    const unsigned blocks = (unsigned)((a.nel + epb - 1) / epb);
+    if (blocks == 0) return PYHIP_OK;
+    ElemArgs<T> arg = a;
+    void *params[] = {&arg};
+    PYHIP_CHECK_HIP(hipModuleLaunchKernel(f, blocks, 1, 1, 256, 1, 1, 0, st, params, nullptr));
+    return PYHIP_OK;
+}
+
+int launch_generated_f64(int form, const ElemArgs<double> &a, hipStream_t st) { return launch_gen(form, a, st); }
+int launch_generated_f32(int form, const ElemArgs<float> &a, hipStream_t st) { return launch_gen(form, a, st); }
+
+}  // namespace pyhip
+
+using namespace pyhip;
+
+extern "C" {
+
+int pyhip_codegen_compile(const char *name, const char *source, int nheaders, const char *const *header_sources,
+                          const char *const *header_names, int elem_kind, int symmetric, int has_param_derivative,
+                          int load, int *form_id, size_t *code_size) {
+    PYHIP_REQUIRE(name && source && nheaders >= 0 && (nheaders == 0 || (header_sources && header_names)),
+                  "null argument");
+    PYHIP_REQUIRE(gen_epb(elem_kind) > 0, "unknown element family");
+    std::vector<char> code;
+    int rc = compile(source, name, nheaders, header_sources, header_names, code);
+    if (rc) return rc;
+    if (code_size) *code_size = code.size();
+    if (!load) return PYHIP_OK;
+    PYHIP_REQUIRE(form_id, "null form id");
+    GenForm G;
+    G.name = name;
+    G.kind = elem_kind;
+    G.symmetric = symmetric != 0;
+    G.has_pderiv = has_param_derivative != 0;
+    PYHIP_CHECK_HIP(hipModuleLoadData(&G.mod, code.data()));
+    const std::string n(name);
+    PYHIP_CHECK_HIP(hipModuleGetFunction(&G.f64, G.mod, (n + "_f64").c_str()));
+    PYHIP_CHECK_HIP(hipModuleGetFunction(&G.f64_tr, G.mod, (n + "_f64_tr").c_str()));
+    PYHIP_CHECK_HIP(hipModuleGetFunction(&G.f32, G.mod, (n + "_f32").c_str()));
+    PYHIP_CHECK_HIP(hipModuleGetFunction(&G.f32_tr, G.mod, (n + "_f32_tr").c_str()));
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_forms.push_back(G);
+    *form_id = PYHIP_GENERATED_FORM_BASE + (int)g_forms.size() - 1;
+    return PYHIP_OK;
+}
+
+int pyhip_codegen_count(int *count) {
+    PYHIP_REQUIRE(count, "null argument");
+    std::lock_guard<std::mutex> lock(g_mu);
+    *count = (int)g_forms.size();
+    return PYHIP_OK;
+}
+
+}  // extern "C"

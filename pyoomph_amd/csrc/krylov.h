@@ -80,6 +80,7 @@ struct KrylovWork {
     double *gpart = nullptr;          // [3][nchunk*kWlBlocksPerChunk] block partials
     int32_t *cg_iters = nullptr;      // [nsub] iterations done in the current solve
     int32_t *cg_fail = nullptr;       // [nsub] breakdown / non-finite residual
+    int32_t *cg_iters2 = nullptr;     // [nsub] preconditioned CG: preconditioner applies
     int32_t *flags_h = nullptr;       // pinned [3][nsub]: active, fail, iterations
     std::vector<int64_t> sub_nnz_h;   // [nsub] pattern entries per sub-problem
     std::vector<int32_t> chunk_ptr_h; // [nsub+1] host copy of sub_chunk_ptr
@@ -95,6 +96,7 @@ struct KrylovWork {
     int ilu_order = 0;                // PYHIP_ILU_ORDER_* of the current solve
     AmgDev amg;
     bool amg_stale = true;            // multigrid values to be rebuilt at the next set-up
+    bool step_res_valid = false;      // res64 holds the residual pyhip_newton_step ended with
     const double *amg_src64 = nullptr;   // FP64 matrix the FP32 V-cycle's set-up reads (or null)
     double *amg_conv64 = nullptr;     // [nnz] FP32 matrix widened for that set-up
     // per sub-problem: 1 if the last Krylov solve (or refinement) ended above its tolerance

@@ -466,6 +466,7 @@ static int krylov_setup(pyhip_problem *p, int m) {
         const size_t ngp = (size_t)3 * W->nchunk * kWlBlocksPerChunk;
         if ((rc = kalloc(&W->achunk, std::max<int64_t>(1, W->nchunk), W)) ||
             (rc = kalloc(&W->gpart, ngp, W)) || (rc = kalloc(&W->cg_iters, ns, W)) ||
+            (rc = kalloc(&W->cg_iters2, ns, W)) ||
             (rc = kalloc(&W->cg_fail, ns, W)))
             return rc;
         if (ngp) PYHIP_CHECK_HIP(hipMemset(W->gpart, 0, ngp * sizeof(double)));
@@ -619,7 +620,7 @@ static int spmv(pyhip_problem *p, const TV *vals, const TX *x, TY *y, const TY *
                 }
         }
         p->timer.add_work("spmv", (int64_t)(sizeof(TV) + 4) * nnz +
-                                      (int64_t)(4 + sizeof(TX) + sizeof(TY) + (b ? sizeof(TY) : 0)) * rows + 4);
+                                      (int64_t)(4 + sizeof(TX) + sizeof(TY) + (b ? sizeof(TY) : 0)) * rows + 4, 4);
     }
     return PYHIP_OK;
 }
@@ -995,6 +996,28 @@ __global__ __launch_bounds__(256) void dot2_kernel(const T *__restrict__ a, cons
     }
 }
 
+// nactive = number of sub-problems with active[s] != 0 (one block, fixed order)
+__global__ __launch_bounds__(256) void count_active_kernel(int nsub, const int32_t *__restrict__ active,
+                                                           int32_t *__restrict__ nactive) {
+    __shared__ int32_t c[256];
+    int32_t k = 0;
+    for (int s = threadIdx.x; s < nsub; s += 256) k += active[s] != 0;
+    c[threadIdx.x] = k;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) c[threadIdx.x] += c[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *nactive = c[0];
+}
+
+constexpr int kPcgRead = 4;   // preconditioned-CG iterations between host reads of the flags
+
+__global__ void zero_i32_kernel(int n, int32_t *__restrict__ v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = 0;
+}
+
 // start: bnorm = |r0|, rz = (r0, z0); active where needed and |b| > 0
 __global__ void cg_start_kernel(int nsub, const double *__restrict__ d2,
                                 const int32_t *__restrict__ need, double *__restrict__ bnorm,
@@ -1010,11 +1033,13 @@ __global__ void cg_start_kernel(int nsub, const double *__restrict__ d2,
 }
 
 // alpha = rz / (p, Ap); a non-positive curvature stops the sub-problem (breakdown)
+// (also counts, per sub-problem, the iterations whose spmv visited it)
 __global__ void cg_alpha_kernel(int nsub, const double *__restrict__ pq_in, double *__restrict__ pq,
                                 int32_t *__restrict__ active, int32_t *__restrict__ nfail,
-                                int32_t *__restrict__ fail) {
+                                int32_t *__restrict__ fail, int32_t *__restrict__ nspmv) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsub || !active[s]) return;
+    nspmv[s] += 1;
     const double v = pq_in[s];
     pq[s] = v;
     if (!(v > 0.0) || !isfinite(v)) {
@@ -1044,13 +1069,15 @@ __global__ __launch_bounds__(256) void cg_xr_kernel(int64_t n, const T *__restri
     if (x0) x0[i] = w0[i] * double(rn);
 }
 
-// convergence test on |r|, then beta = (r,z)_new / (r,z)_old
+// convergence test on |r|, then beta = (r,z)_new / (r,z)_old (also counts, per
+// sub-problem, the preconditioner applies that visited it)
 __global__ void cg_check_kernel(int nsub, const double *__restrict__ d2,
                                 const double *__restrict__ bnorm, double rtol,
                                 double *__restrict__ rz, double *__restrict__ beta,
-                                int32_t *__restrict__ active, int32_t *__restrict__ nactive) {
+                                int32_t *__restrict__ active, int32_t *__restrict__ nprec) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsub || !active[s]) return;
+    nprec[s] += 1;
     const double rr = d2[s], rzn = d2[nsub + s];
     if (!(sqrt(rr) > rtol * bnorm[s]) || !isfinite(rr)) {
         active[s] = 0;
@@ -1058,7 +1085,6 @@ __global__ void cg_check_kernel(int nsub, const double *__restrict__ d2,
     }
     beta[s] = rzn / rz[s];
     rz[s] = rzn;
-    atomicAdd(nactive, 1);
 }
 
 template <typename T>
@@ -1140,39 +1166,75 @@ static int cg_generic(pyhip_problem *p, KrylovWork *W, const TV *vals, const T *
         w0 = amg_fine_w(W->amg);
         x0 = amg_fine_x(W->amg);
     }
+    // The host reads the running count every kPcgRead iterations: in between, a converged
+    // sub-problem's flag is cleared on the device and every kernel skips its rows, so the
+    // host keeps enqueueing instead of waiting. The spmv-family bytes of those launches
+    // depend on which sub-problems were live: the launches add their per-launch part now
+    // and the rows visited are counted per sub-problem on the device (spmv: nspmv, V-cycle:
+    // nprec) and added at the end.
+    int32_t *nspmv = W->cg_iters, *nprec = W->cg_iters2;
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, nspmv);
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, nprec);
+    const bool defer_saved = p->timer.defer;
+    p->timer.defer = true;
     int it = 0;
     while (nact > 0 && it < o.max_iters) {
-        if ((rc = spmv<TV, T, T>(p, vals, pv, qv, (const T *)nullptr, W->active, act_h))) return rc;
-        if ((rc = batched_dots<T>(p, W, 1, pv, n, qv, d2 + 2 * ns, W->active))) return rc;
-        hipLaunchKernelGGL(cg_alpha_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2 + 2 * ns,
-                           W->cg_pq, W->active, W->nfail_d, W->cg_fail);
-        {
-            TimeScope ts(&p->timer, "vector", p->st);
-            hipLaunchKernelGGL((cg_xr_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, pv, qv, x, r,
-                               W->cg_rz, W->cg_pq, p->dof_sub, W->active, w0, x0);
+        const int batch = std::min(kPcgRead, o.max_iters - it);
+        for (int k = 0; k < batch; k++, it++) {
+            if ((rc = spmv<TV, T, T>(p, vals, pv, qv, (const T *)nullptr, W->active, act_h))) return rc;
+            if ((rc = batched_dots<T>(p, W, 1, pv, n, qv, d2 + 2 * ns, W->active))) return rc;
+            hipLaunchKernelGGL(cg_alpha_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2 + 2 * ns,
+                               W->cg_pq, W->active, W->nfail_d, W->cg_fail, nspmv);
+            {
+                TimeScope ts(&p->timer, "vector", p->st);
+                hipLaunchKernelGGL((cg_xr_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, pv, qv, x, r,
+                                   W->cg_rz, W->cg_pq, p->dof_sub, W->active, w0, x0);
+            }
+            KCHECK();
+            if ((rc = precond_apply<T>(p, W, o.precond, r, z, W->active, act_h, x0 != nullptr))) return rc;
+            if ((rc = batched_dot2<T>(p, W, r, r, r, z, d2, W->active))) return rc;
+            hipLaunchKernelGGL(cg_check_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2, W->bnorm, rtol,
+                               W->cg_rz, W->cg_beta, W->active, nprec);
+            {
+                TimeScope ts(&p->timer, "vector", p->st);
+                hipLaunchKernelGGL((cg_p_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, z, pv, W->cg_beta,
+                                   p->dof_sub, W->active);
+            }
+            KCHECK();
         }
-        KCHECK();
-        if ((rc = precond_apply<T>(p, W, o.precond, r, z, W->active, act_h, x0 != nullptr))) return rc;
-        if ((rc = batched_dot2<T>(p, W, r, r, r, z, d2, W->active))) return rc;
-        PYHIP_CHECK_HIP(hipMemsetAsync(W->nactive_d, 0, sizeof(int32_t), p->st));
-        hipLaunchKernelGGL(cg_check_kernel, dim3(nblk(ns, 64)), dim3(64), 0, p->st, ns, d2, W->bnorm, rtol,
-                           W->cg_rz, W->cg_beta, W->active, W->nactive_d);
-        {
-            TimeScope ts(&p->timer, "vector", p->st);
-            hipLaunchKernelGGL((cg_p_kernel<T>), dim3(nblk(n)), dim3(256), 0, p->st, n, z, pv, W->cg_beta,
-                               p->dof_sub, W->active);
-        }
-        KCHECK();
-        it++;
+        // the running count of this read: recount from the flags
+        hipLaunchKernelGGL(count_active_kernel, dim3(1), dim3(256), 0, p->st, ns, W->active, W->nactive_d);
         if ((rc = read_counter(p, W, W->nactive_d, &nact, W->active, act_h))) return rc;
     }
+    p->timer.defer = defer_saved;
     int32_t nf = 0;
     PYHIP_CHECK_HIP(hipMemcpyAsync(W->flags_h, W->cg_fail, ns * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+    PYHIP_CHECK_HIP(hipMemcpyAsync(W->flags_h + ns, nspmv, ns * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+    PYHIP_CHECK_HIP(hipMemcpyAsync(W->flags_h + 2 * ns, nprec, ns * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
     if ((rc = read_counter(p, W, W->nfail_d, &nf, W->active, act_h))) return rc;
-    // per sub-problem: broke down, or still above tolerance
+    if (p->timer.on) {
+        // rows visited: each spmv of the loop (y = A p: matrix, row starts, p, q) and each
+        // V-cycle, per sub-problem
+        std::vector<int64_t> vb;
+        if (o.precond == PYHIP_PRECOND_AMG) vb = amg_vcycle_bytes(W->amg, (int)sizeof(T));
+        int64_t bytes = 0;
+        for (int s = 0; s < ns; s++) {
+            const int64_t rows = p->dof_off_h[s + 1] - p->dof_off_h[s];
+            bytes += (int64_t)W->flags_h[ns + s] *
+                     ((int64_t)(sizeof(TV) + 4) * W->sub_nnz_h[s] + (int64_t)(4 + 2 * sizeof(T)) * rows);
+            if (!vb.empty()) bytes += (int64_t)W->flags_h[2 * ns + s] * vb[s];
+        }
+        p->timer.add_work("spmv", bytes);
+    }
+    // per sub-problem: broke down, or still above tolerance; iterations of the longest-
+    // running sub-problem (the loop may have run a few no-op iterations past the last one)
     W->lfail_h.assign(ns, 0);
-    for (int s = 0; s < ns; s++) W->lfail_h[s] = (W->flags_h[s] != 0 || act_h[s] != 0) ? 1 : 0;
-    *iters = it;
+    int maxit = 0;
+    for (int s = 0; s < ns; s++) {
+        W->lfail_h[s] = (W->flags_h[s] != 0 || act_h[s] != 0) ? 1 : 0;
+        maxit = std::max(maxit, (int)W->flags_h[ns + s]);
+    }
+    *iters = maxit;
     *failed = nf + nact;   // breakdowns + sub-problems still above tolerance
     return PYHIP_OK;
 }
@@ -1722,7 +1784,7 @@ void pyhip_krylov_free(pyhip_problem *p) {
                     W->cg_beta, W->nfail_d, W->ilu.perm, W->ilu.rs, W->ilu.col,
                     W->ilu.src, W->ilu.diag, W->ilu.lo_rows, W->ilu.up_rows, W->ilu.lu,
                     W->ilu.tmp64, W->ilu.tmp64b, W->ilu.tmp32, W->ilu.tmp32b, W->achunk,
-                    W->gpart, W->cg_iters, W->cg_fail, W->ac_cur, W->ac_der, W->ac_y, W->ac_z,
+                    W->gpart, W->cg_iters, W->cg_iters2, W->cg_fail, W->ac_cur, W->ac_der, W->ac_y, W->ac_z,
                     W->ac_t, W->ac_zs, W->ac_alpha, W->ac_mask, W->amg_conv64};
     for (void *q : ptrs)
         dfree(q);
@@ -1747,6 +1809,7 @@ int pyhip_newton_solve(pyhip_problem *p, double *x, const pyhip_newton_opts *o,
     if ((rc = krylov_setup(p, o->linear.restart))) return rc;
     KrylovWork *W = p->kw;
     W->amg_stale = true;
+    W->step_res_valid = false;
     const int ns = p->nsub;
     const int64_t n = p->ndof;
     if (o->jacobian_precision == PYHIP_F64 && !W->vals64)
@@ -1843,6 +1906,103 @@ int pyhip_newton_solve(pyhip_problem *p, double *x, const pyhip_newton_opts *o,
     return PYHIP_OK;
 }
 
+// One Newton iteration (declared in pyoomph_hip.h) for the sub-problems with mask[s] that
+// have neither converged nor failed: the body of pyhip_newton_solve's loop, with each
+// sub-problem's progress kept by the caller between calls, so that sub-problems can start
+// and finish at any call (slots of a serving loop).
+int pyhip_newton_step(pyhip_problem *p, double *x, const pyhip_newton_opts *o, const int32_t *mask,
+                      pyhip_newton_step_state *st) {
+    PYHIP_REQUIRE(p && x && o && mask && st, "null argument");
+    PYHIP_REQUIRE(o->linear.restart >= 1 && o->linear.restart <= kMaxRestart, "restart must be in 1..1000");
+    PYHIP_REQUIRE(o->jacobian_precision == PYHIP_F64 || o->jacobian_precision == PYHIP_F32,
+                  "bad Jacobian precision");
+    int rc;
+    if ((rc = krylov_setup(p, o->linear.restart))) return rc;
+    KrylovWork *W = p->kw;
+    const int ns = p->nsub;
+    const int64_t n = p->ndof;
+    if (o->jacobian_precision == PYHIP_F64 && !W->vals64)
+        if ((rc = dev_alloc(&W->vals64, p->nnz, W->bytes))) return rc;
+    if (o->linear.precision == PYHIP_F32 && !W->vals32)
+        if ((rc = dev_alloc(&W->vals32, p->nnz, W->bytes))) return rc;
+    std::vector<int32_t> running(ns, 0);
+    int nrun = 0;
+    bool fresh = false;
+    for (int s = 0; s < ns; s++) {
+        const bool empty = p->dof_off_h[s + 1] == p->dof_off_h[s];
+        if (!mask[s] || st[s].converged || st[s].failed) continue;
+        if (empty) {
+            st[s].converged = 1;
+            continue;
+        }
+        running[s] = 1;
+        nrun++;
+        fresh = fresh || st[s].newton_iters == 0;
+    }
+    auto residual_norms = [&]() -> int {
+        int r2;
+        if ((r2 = pyhip_assemble(p, x, 0, -1, PYHIP_F64, W->res64, nullptr))) return r2;
+        if ((r2 = batched_maxabs(p, W->res64, W->maxres))) return r2;
+        PYHIP_CHECK_HIP(hipMemcpyAsync(W->maxres_h, W->maxres, ns * sizeof(double), hipMemcpyDeviceToHost, p->st));
+        PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+        return PYHIP_OK;
+    };
+    if (!nrun) return PYHIP_OK;
+    // a sub-problem starting its solve: the first residual (count 1: without
+    // always_take_one_newton_step a small enough start is already converged); otherwise
+    // the residual the last call ended with is still the one at the running dofs
+    if (fresh || !W->step_res_valid) {
+        if ((rc = residual_norms())) return rc;
+    }
+    W->step_res_valid = false;
+    for (int s = 0; s < ns; s++) {
+        if (!running[s] || st[s].newton_iters) continue;
+        st[s].maxres = W->maxres_h[s];
+        if (W->maxres_h[s] < o->tol && !o->always_take_one_newton_step) {
+            st[s].converged = 1;
+            running[s] = 0;
+            nrun--;
+        }
+    }
+    if (!nrun) return PYHIP_OK;
+    // the multigrid hierarchy comes from this step's Jacobian, so that no sub-problem's
+    // preconditioner depends on when the others started their solves
+    W->amg_stale = true;
+    PYHIP_CHECK_HIP(hipMemcpyAsync(W->dof_mask, running.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, p->st));
+    if (o->jacobian_precision == PYHIP_F64) {
+        if ((rc = pyhip_assemble(p, x, 1, -1, PYHIP_F64, W->w64, W->vals64))) return rc;
+    } else {
+        if ((rc = pyhip_assemble(p, x, 1, -1, PYHIP_F32, W->w64, W->vals32))) return rc;
+    }
+    int its = 0, refs = 0, lfail = 0;
+    if ((rc = linear_solve(p, W, o->linear, o->jacobian_precision, W->res64, W->d64, W->dof_mask, &its, &refs,
+                           &lfail)))
+        return rc;
+    hipLaunchKernelGGL((axpy_kernel<double, double>), dim3(nblk(n)), dim3(256), 0, p->st, n, W->d64, x,
+                       -o->relaxation, p->dof_sub, W->dof_mask);
+    KCHECK();
+    std::vector<double> dmax(ns, 0.0), xmax(ns, 0.0);
+    if ((rc = batched_maxabs(p, W->d64, W->sub_scalar))) return rc;
+    PYHIP_CHECK_HIP(hipMemcpyAsync(dmax.data(), W->sub_scalar, ns * sizeof(double), hipMemcpyDeviceToHost, p->st));
+    if ((rc = batched_maxabs(p, x, W->sub_scalar))) return rc;
+    PYHIP_CHECK_HIP(hipMemcpyAsync(xmax.data(), W->sub_scalar, ns * sizeof(double), hipMemcpyDeviceToHost, p->st));
+    if ((rc = residual_norms())) return rc;
+    for (int s = 0; s < ns; s++) {
+        if (!running[s]) continue;
+        auto &S = st[s];
+        S.newton_iters++;
+        S.linear_iters += its;
+        S.linear_failures += (s < (int)W->lfail_h.size() && W->lfail_h[s]) ? 1 : 0;
+        S.maxres = W->maxres_h[s];
+        S.max_update = dmax[s] * std::fabs(o->relaxation);
+        const bool small_update = o->correction_tol <= 0.0 || S.max_update <= o->correction_tol * xmax[s];
+        if (S.maxres < o->tol && small_update) S.converged = 1;
+        else if (S.maxres > o->max_residuals || S.newton_iters >= o->max_iter) S.failed = 1;
+    }
+    W->step_res_valid = true;
+    return PYHIP_OK;
+}
+
 // Arclength continuation (declared in pyoomph_hip.h): the reference's scalar recursion per
 // sub-problem on the host, the vectors and solves batched on the device.
 int pyhip_arclength_step(pyhip_problem *p, double *x, int param, const pyhip_newton_opts *o,
@@ -1853,12 +2013,13 @@ int pyhip_arclength_step(pyhip_problem *p, double *x, int param, const pyhip_new
     PYHIP_REQUIRE(o->jacobian_precision == PYHIP_F64 && o->linear.precision == PYHIP_F64,
                   "arclength continuation runs FP64 solves");
     for (auto &G : p->groups)
-        PYHIP_REQUIRE(G.form == PYHIP_FORM_BRATU || G.form == kFormFlux,
-                      "arclength continuation in lambda needs Bratu sub-problems");
+        PYHIP_REQUIRE(G.form == PYHIP_FORM_BRATU || G.form == kFormFlux || gen_form_has_pderiv(G.form),
+                      "arclength continuation needs sub-problems with a global parameter (Bratu lambda)");
     int rc;
     if ((rc = krylov_setup(p, o->linear.restart))) return rc;
     KrylovWork *W = p->kw;
     W->amg_stale = true;
+    W->step_res_valid = false;
     const int ns = p->nsub;
     const int64_t n = p->ndof;
     if (!W->vals64 && (rc = dev_alloc(&W->vals64, p->nnz, W->bytes))) return rc;

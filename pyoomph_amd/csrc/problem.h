@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/pyoomph_codegen.h"
 #include "common.h"
 #include "element_kernels.h"
 #include "pattern.h"
@@ -55,8 +56,11 @@ struct KernelTimer {
     std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::map<std::string, std::pair<int64_t, double>> total;
     std::map<std::string, int64_t> work;   // algorithmic bytes moved by the family's kernels
-    void add_work(const char *fam, int64_t bytes) {
-        if (on) work[fam] += bytes;
+    // While deferred, launches add only their per-launch part; the caller adds the bytes
+    // that depend on the rows visited once it knows them exactly (per-sub-problem counts).
+    bool defer = false;
+    void add_work(const char *fam, int64_t bytes, int64_t per_launch = 0) {
+        if (on) work[fam] += defer ? per_launch : bytes;
     }
     hipEvent_t take() {
         if (used == pool.size()) {
@@ -112,6 +116,12 @@ struct TimeScope {
 
 int launch_elements_f64(int kind, int form, const ElemArgs<double> &a, hipStream_t s);
 int launch_elements_f32(int kind, int form, const ElemArgs<float> &a, hipStream_t s);
+// run-time generated forms (codegen_rt.hip)
+bool gen_form_known(int form);
+int gen_form_kind(int form);
+bool gen_form_has_pderiv(int form);
+int launch_generated_f64(int form, const ElemArgs<double> &a, hipStream_t s);
+int launch_generated_f32(int form, const ElemArgs<float> &a, hipStream_t s);
 int launch_faces_f64(int kind, const ElemArgs<double> &a, hipStream_t s);
 int launch_faces_f32(int kind, const ElemArgs<float> &a, hipStream_t s);
 int launch_gather_values_f64(const GatherMap &m, const double *Ej, const double *Er, double *vals,

@@ -238,7 +238,9 @@ class Problem:
         return self._host(res[:self.ndof])
 
     def _param_index(self, name):
-        if name != "lambda":
+        """The device index of a global parameter: every form has at most one (index 0)."""
+        names = {n for mp in self.parts for n in getattr(mp.equations, "global_parameters", ())}
+        if name not in names:
             raise ValueError("unknown global parameter %r" % name)
         return 0
 
@@ -459,6 +461,36 @@ class Problem:
         """Sets the continued parameter (the Bratu lambda) of one sub-problem."""
         par = np.array([value], dtype=np.float64)
         _lib.check(_lib.load().pyhip_problem_set_params(self._handle, int(sub), par.ctypes.data_as(_lib.c_dp), 1))
+
+    def set_sub_data(self, sub, params=None, aux=None):
+        """Replaces one sub-problem's parameters and / or auxiliary nodal field (a new
+        request in a serving slot on the same mesh)."""
+        L = _lib.load()
+        if params is not None:
+            par = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+            _lib.check(L.pyhip_problem_set_params(self._handle, int(sub), par.ctypes.data_as(_lib.c_dp),
+                                                  int(par.size)))
+        if aux is not None:
+            a = np.ascontiguousarray(aux, dtype=np.float64).reshape(-1)
+            _lib.check(L.pyhip_problem_set_aux(self._handle, int(sub), a.ctypes.data_as(_lib.c_dp), int(a.size)))
+
+    def current_device_bytes(self):
+        """Device bytes held now (mesh, pattern, element buffers, solver workspace)."""
+        nb = ctypes.c_int64()
+        _lib.check(_lib.load().pyhip_problem_info(self._handle, None, None, None, ctypes.byref(nb)))
+        return nb.value
+
+    def newton_step(self, mask, state, opts=None):
+        """One Newton iteration for the sub-problems with mask != 0 that have neither
+        converged nor failed (``pyhip_newton_step``): ``state`` is a ctypes array of
+        ``_lib.NewtonStepState`` [nsub] carried between calls, newton_iters = 0 starting a
+        sub-problem's solve from its current dofs."""
+        mask = np.ascontiguousarray(mask, dtype=np.int32)
+        if mask.shape != (len(self.parts),) or len(state) != len(self.parts):
+            raise ValueError("mask and state need one entry per sub-problem")
+        o = self.newton_options() if opts is None else opts
+        _lib.check(_lib.load().pyhip_newton_step(self._handle, _lib.ptr(self.dofs), ctypes.byref(o),
+                                                 mask.ctypes.data_as(_lib.c_i32p), state))
 
     def global_parameters(self):
         """Current value of the continued parameter per sub-problem (after a step)."""

@@ -2,37 +2,43 @@
 
 pyoomph has no request loop; its closest analogue is the process farm of parameter scans
 (pyoomph/utils/paramscan.py:249-277), which runs one independent Problem per process.
-Here requests share one GPU:
+Here requests share one GPU, batched continuously at the granularity of a Newton step:
 
-* Arrivals. A request names a problem (its kind, mesh size and seed) and an arrival
-  time; ``poisson_stream`` draws a seeded Poisson-arrival stream mixing 2D P2 Poisson,
-  3D Q2 reaction-diffusion and 2D Taylor-Hood Navier-Stokes requests.
-* Admission. A request is admitted, first come first served, when its estimated device
-  footprint fits in what is left of the HBM budget; the head of the queue is never
-  overtaken, so no request starves.
-* Continuous batching. Admitted requests wait in one queue per solver class (conjugate
-  gradients for the symmetric forms, GMRES otherwise). Whenever one of the ``workers``
-  is free it takes the oldest admitted request and every admitted request of the same
-  class behind it, up to ``max_batch_dofs``, builds them into one batched problem on its
-  own HIP stream and solves it. Requests that arrive while batches run are admitted as
-  memory frees up and start as soon as a worker is free, so batches of different
-  arrivals overlap on the device.
-* Each sub-problem of a batch has its own Newton and Krylov scalars and reductions, so a
-  request's result does not depend on the batch it ran in (``tests/test_gpu_serving.py``
-  checks alone against in the mix).
+* Mesh classes. A request names a problem kind (2D P2 Poisson, 3D Q2 reaction-diffusion,
+  2D Taylor-Hood Navier-Stokes), one of the kind's mesh sizes (``SIZES``: 5k to 500k
+  unknowns) and its own data: the Poisson source's eight Gaussians, the
+  reaction-diffusion source field, the Navier-Stokes Reynolds number, all drawn from the
+  request's seed. ``poisson_stream`` draws a seeded Poisson-arrival stream of them.
+* Slot pools. Every (kind, mesh size) has a pool: one device-resident batched problem
+  whose sub-problems are ``slots`` copies of that mesh. Mesh, equation maps, sparsity
+  pattern, solver plans (ILU order, multigrid aggregates) and Krylov workspace are built
+  once, when the server starts, and kept; pools are created while their measured device
+  bytes fit in the HBM budget. A request occupies one slot: its parameters / source
+  field are copied into the slot and the slot's dofs set to the start value.
+* Continuous batching. Each pool advances its busy slots one Newton step at a time
+  (``pyhip_newton_step``). Requests that arrive while a pool is stepping wait in the
+  pool's queue and take free slots at the next step boundary, first come first served;
+  a request whose solve has converged (or failed) leaves at once -- its dofs copied to
+  the host, its slot freed -- while the others go on. At most ``workers`` pools step at
+  a time, each on its own HIP stream, so different pools' work overlaps on the device;
+  pools wanting to step are served in the order they asked.
+* Each slot's Newton and Krylov iteration is its own (per-sub-problem element rows,
+  scalars, reductions and convergence tests; Jacobi and ILU(0) per sub-problem block; the
+  multigrid hierarchy rebuilt from every step's Jacobian), so a request gives the same
+  bits alone and in the mix (``tests/test_gpu_serving.py``).
 
-The executor is pluggable: ``GpuExecutor`` runs batches with the HIP library;
-the scheduler itself is plain host code and is tested with a stand-in executor.
+The executor is pluggable: ``GpuPools`` runs pools with the HIP library; the scheduler is
+plain host code, tested with a stand-in executor (``tests/test_serving_scheduler.py``).
 """
-import math
 import threading
 import time
+from collections import deque
 from dataclasses import dataclass, field
 
 import numpy as np
 
-__all__ = ["Request", "Result", "poisson_stream", "ServingLoop", "GpuExecutor", "estimate_bytes",
-           "SOLVER_CLASS"]
+__all__ = ["Request", "Result", "poisson_stream", "attach_payloads", "request_data", "ServingLoop",
+           "GpuPools", "SlotPool", "summarize", "slots_for", "SIZES", "SOLVER_CLASS"]
 
 # solver class of each request kind (symmetric positive definite Jacobian -> CG)
 SOLVER_CLASS = {"poisson": "cg", "rd": "cg", "ns": "gmres"}
@@ -53,22 +59,38 @@ def _ndof_ns(N):
 
 _NDOF = {"poisson": _ndof_poisson, "rd": _ndof_rd, "ns": _ndof_ns}
 
-
-def size_for_dofs(kind, ndof):
-    """Smallest mesh size N of a request kind with at least ``ndof`` unknowns."""
-    N = 2
-    while _NDOF[kind](N) < ndof:
-        N += 1
-    return N
+# the mesh sizes served per kind, about log-uniform in unknowns: Poisson 5k..450k, reaction-
+# diffusion 7k..490k, Taylor-Hood 6k..11k (its GMRES / ILU(0) solves grow fastest)
+SIZES = {"poisson": (36, 56, 88, 136, 212, 336),
+         "rd": (10, 14, 20, 28, 40),
+         "ns": (26, 30, 36)}
 
 
-# device bytes per unknown of one request, measured on the library's layout (pattern,
-# element buffers, Krylov workspace) and rounded up; used only for admission
-_BYTES_PER_DOF = {"poisson": 1400, "rd": 5200, "ns": 4200}
+def slots_for(kind, N, target_dofs=2_000_000, lo=2, hi=16):
+    """Slots of a pool: about ``target_dofs`` unknowns per pool, within [lo, hi]."""
+    return int(min(hi, max(lo, target_dofs // _NDOF[kind](N))))
 
 
-def estimate_bytes(kind, ndof):
-    return int(_BYTES_PER_DOF[kind] * ndof) + (1 << 20)
+def ns_reynolds(seed):
+    """Reynolds number of a Navier-Stokes request: 50..150 from its seed."""
+    return 50.0 + float(seed % 101)
+
+
+def request_data(kind, N, seed, mesh=None):
+    """(parameters, auxiliary nodal field or None) of a request; ``mesh`` is the request's
+    mesh (needed for the reaction-diffusion field)."""
+    from . import workloads as wl
+    from .equations import NavierStokesEquations, PoissonEquation
+    if kind == "poisson":
+        return PoissonEquation("u", space="C2", source=wl.gaussian_source(seed)).params(), None
+    if kind == "rd":
+        if mesh is None:
+            from .meshes import CuboidBrickMesh
+            mesh = CuboidBrickMesh(N)
+        return np.zeros(0), wl.smoothed_noise(mesh, seed)
+    if kind == "ns":
+        return NavierStokesEquations(dynamic_viscosity=1.0 / ns_reynolds(seed), mass_density=1.0).params(), None
+    raise KeyError(kind)
 
 
 @dataclass
@@ -76,9 +98,9 @@ class Request:
     rid: int
     kind: str          # "poisson" | "rd" | "ns"
     N: int             # mesh size (elements per side)
-    seed: int
+    seed: int          # the request's data
     arrival: float     # seconds after the start of the stream
-    payload: object = field(default=None, repr=False, compare=False)   # the request's mesh problem
+    payload: object = field(default=None, repr=False, compare=False)   # (params, aux)
 
     @property
     def ndof(self):
@@ -88,32 +110,47 @@ class Request:
     def solver(self):
         return SOLVER_CLASS[self.kind]
 
-    def attach_payload(self):
-        """Builds the request's mesh problem now (the client side of a request: it
-        arrives with its mesh, fields and parameters)."""
-        self.payload = self.mesh_problem()
-        return self
+    @property
+    def signature(self):
+        return (self.kind, self.N)
+
+    def data(self, mesh=None):
+        if self.payload is None:
+            self.payload = request_data(self.kind, self.N, self.seed, mesh)
+        return self.payload
 
     def mesh_problem(self):
-        if self.payload is not None:
-            return self.payload
+        """The request as a stand-alone mesh problem (its own mesh and data)."""
         from . import workloads as wl
         if self.kind == "poisson":
             return wl.poisson_gaussians(self.N, seed=self.seed)
         if self.kind == "rd":
             return wl.reaction_diffusion_cube(self.N, seed=self.seed)
-        return wl.lid_driven_cavity(self.N, Re=100.0)
+        return wl.lid_driven_cavity(self.N, Re=ns_reynolds(self.seed))
+
+
+def attach_payloads(requests):
+    """Computes every request's data (the client side: a request arrives with its fields
+    and parameters), one mesh per mesh class."""
+    meshes = {}
+    for r in requests:
+        if r.payload is not None:
+            continue
+        if r.kind == "rd" and r.signature not in meshes:
+            from .meshes import CuboidBrickMesh
+            meshes[r.signature] = CuboidBrickMesh(r.N)
+        r.data(meshes.get(r.signature))
+    return requests
 
 
 @dataclass
 class Result:
     rid: int
     arrival: float
-    admitted: float
-    started: float
-    done: float
-    batch: int
-    batch_size: int
+    admitted: float    # loaded into a slot
+    done: float        # converged and copied to the host
+    pool: tuple
+    slot: int
     dofs: object = field(repr=False, default=None)
     stats: dict = field(default_factory=dict)
 
@@ -122,12 +159,10 @@ class Result:
         return self.done - self.arrival
 
 
-def poisson_stream(rate, n, seed=0, mix=(("poisson", 0.45), ("rd", 0.35), ("ns", 0.20)),
-                   dof_range=(5_000, 600_000), ns_max_dofs=12_000):
-    """``n`` requests with exponential inter-arrival times of mean 1/rate (seeded). Sizes
-    are log-uniform in ``dof_range``; Navier-Stokes requests are capped at
-    ``ns_max_dofs`` unknowns (the GMRES/ILU(0) solve of larger Taylor-Hood systems is
-    slow)."""
+def poisson_stream(rate, n, seed=0, mix=(("poisson", 0.45), ("rd", 0.35), ("ns", 0.20)), sizes=None):
+    """``n`` requests with exponential inter-arrival times of mean 1/rate (seeded); kinds
+    drawn with the ``mix`` weights, mesh sizes uniformly from the kind's ``sizes``."""
+    sizes = sizes or SIZES
     rng = np.random.default_rng(seed)
     kinds = [k for k, _ in mix]
     p = np.array([w for _, w in mix], dtype=float)
@@ -137,121 +172,151 @@ def poisson_stream(rate, n, seed=0, mix=(("poisson", 0.45), ("rd", 0.35), ("ns",
     for i in range(n):
         t += float(rng.exponential(1.0 / rate))
         kind = kinds[int(rng.choice(len(kinds), p=p))]
-        lo, hi = dof_range
-        if kind == "ns":
-            hi = min(hi, ns_max_dofs)
-        nd = math.exp(rng.uniform(math.log(lo), math.log(hi)))
-        out.append(Request(i, kind, size_for_dofs(kind, nd), int(rng.integers(1 << 30)), t))
+        N = int(sizes[kind][int(rng.integers(len(sizes[kind])))])
+        out.append(Request(i, kind, N, int(rng.integers(1 << 30)), t))
     return out
 
 
 class ServingLoop:
-    """Admission by HBM budget, continuous batching per solver class, ``workers`` batches
-    in flight (one HIP stream each)."""
+    """Slot pools per mesh class, Newton-step-granular continuous batching, at most
+    ``workers`` pools stepping at a time.
 
-    def __init__(self, executor, hbm_budget=200e9, workers=4, max_batch_dofs=4_000_000,
-                 estimator=None):
+    ``executor``: ``create(signature, slots) -> pool`` and ``make_stream()``; a pool has
+    ``slots``, ``device_bytes``, ``load(slot, request)``, ``step() -> [finished slots]``
+    and ``take(slot) -> (dofs, stats)``."""
+
+    def __init__(self, executor, hbm_budget=200e9, workers=4, slots=None):
         self.executor = executor
         self.budget = float(hbm_budget)
         self.workers = int(workers)
-        self.max_batch_dofs = int(max_batch_dofs)
-        self.estimator = estimator or (lambda r: estimate_bytes(r.kind, r.ndof))
-        self.peak_reserved = 0.0
-        self.batches = []        # (batch id, [rid...], solver)
+        self.slots = slots or slots_for
+        self.reserved = 0
+        self.pool_of = {}
+        self.steps = {}          # signature -> [busy slots per step]
+
+    def prepare(self, signatures):
+        """Creates the pools of these mesh classes (server start, before any request)."""
+        for sig in signatures:
+            sig = tuple(sig)
+            if sig in self.pool_of:
+                continue
+            pool = self.executor.create(sig, self.slots(*sig))
+            if self.reserved + pool.device_bytes > self.budget:
+                pool.close()
+                raise MemoryError("the pool of %s (%d bytes) does not fit in the HBM budget" % (sig, pool.device_bytes))
+            self.reserved += pool.device_bytes
+            self.pool_of[sig] = pool
+            self.steps[sig] = []
+
+    def close(self):
+        for pool in self.pool_of.values():
+            pool.close()
+        self.pool_of.clear()
+        self.reserved = 0
 
     def run(self, requests, realtime=True):
-        """Serve ``requests`` (sorted by arrival). With ``realtime`` the arrivals are
-        released at their times on the wall clock; otherwise all are present at t = 0.
-        Returns the results in request order."""
+        """Serves ``requests``; with ``realtime`` each arrives at its time on the wall
+        clock, otherwise all at t = 0. Returns the results in request order."""
         reqs = sorted(requests, key=lambda r: r.arrival)
-        for r in reqs:
-            if self.estimator(r) > self.budget:
-                raise ValueError("request %d needs more than the whole HBM budget" % r.rid)
+        missing = {r.signature for r in reqs} - set(self.pool_of)
+        if missing:
+            raise ValueError("no pool serves the mesh classes %s" % sorted(missing))
         cv = threading.Condition()
-        state = {"reserved": 0.0, "idle": self.workers, "next": 0, "error": None}
-        pending, queues, results = [], {}, {}
-        admitted_at = {}
-        work = []                 # batches handed to workers
-        stop = [False]
+        queue = {sig: deque() for sig in self.pool_of}
+        results = {}
+        turn = deque()                     # pools waiting to step, in the order they asked
+        state = {"released": 0, "tokens": self.workers, "error": None, "closing": False}
         t0 = time.perf_counter()
 
         def now():
             return time.perf_counter() - t0
 
-        def worker(wid):
-            stream = self.executor.make_stream()
+        def pool_worker(sig):
+            pool = self.pool_of[sig]
+            busy = {}                      # slot -> (request, admitted)
+            free = deque(range(pool.slots))
             while True:
                 with cv:
-                    while not work and not stop[0]:
+                    while not busy and not queue[sig] and not state["closing"]:
                         cv.wait()
-                    if not work:
+                    if not busy and not queue[sig]:
                         return
-                    bid, batch = work.pop(0)
-                t_start = now()
+                    turn.append(sig)
+                    while (turn[0] != sig or state["tokens"] == 0) and state["error"] is None:
+                        cv.wait()
+                    if state["error"] is not None:
+                        return
+                    turn.popleft()
+                    state["tokens"] -= 1
+                    cv.notify_all()
+                    admit = []
+                    while free and queue[sig]:
+                        admit.append((free.popleft(), queue[sig].popleft()))
                 try:
-                    outs = self.executor.run_batch(batch, stream)
-                except Exception as e:   # surfaced to run()
+                    t_adm = now()
+                    for slot, r in admit:
+                        pool.load(slot, r)
+                        busy[slot] = (r, t_adm)
+                    t_s = time.perf_counter()
+                    finished = pool.step()
+                    self.steps[sig].append((len(busy), time.perf_counter() - t_s))
+                    outs = [(slot, pool.take(slot)) for slot in finished]
+                except BaseException as e:   # surfaced by run()
                     with cv:
                         state["error"] = e
+                        state["tokens"] += 1
                         cv.notify_all()
                     return
                 t_done = now()
                 with cv:
-                    for r, (x, st) in zip(batch, outs):
-                        results[r.rid] = Result(r.rid, r.arrival, admitted_at[r.rid], t_start, t_done,
-                                                bid, len(batch), x, st)
-                        state["reserved"] -= self.estimator(r)
-                    state["idle"] += 1
+                    state["tokens"] += 1
+                    for slot, (x, st) in outs:
+                        r, t_a = busy.pop(slot)
+                        free.append(slot)
+                        results[r.rid] = Result(r.rid, r.arrival, t_a, t_done, sig, slot, x, st)
                     cv.notify_all()
 
-        threads = [threading.Thread(target=worker, args=(w,), daemon=True) for w in range(self.workers)]
+        threads = [threading.Thread(target=pool_worker, args=(sig,), daemon=True) for sig in self.pool_of]
         for th in threads:
             th.start()
-        nb = 0
         try:
             with cv:
                 while len(results) < len(reqs):
                     if state["error"] is not None:
                         raise state["error"]
                     t = now()
-                    # arrivals
-                    while state["next"] < len(reqs) and (not realtime or reqs[state["next"]].arrival <= t):
-                        pending.append(reqs[state["next"]])
-                        state["next"] += 1
-                    # admission: first come first served under the budget
-                    while pending and state["reserved"] + self.estimator(pending[0]) <= self.budget:
-                        r = pending.pop(0)
-                        state["reserved"] += self.estimator(r)
-                        self.peak_reserved = max(self.peak_reserved, state["reserved"])
-                        admitted_at[r.rid] = now()
-                        queues.setdefault(r.solver, []).append(r)
-                    # batching: the oldest admitted request and its class mates
-                    while state["idle"] > 0 and any(queues.values()):
-                        cls = min((q[0].arrival, c) for c, q in queues.items() if q)[1]
-                        q = queues[cls]
-                        batch = [q.pop(0)]
-                        dofs = batch[0].ndof
-                        while q and dofs + q[0].ndof <= self.max_batch_dofs:
-                            dofs += q[0].ndof
-                            batch.append(q.pop(0))
-                        work.append((nb, batch))
-                        self.batches.append((nb, [r.rid for r in batch], cls))
-                        nb += 1
-                        state["idle"] -= 1
+                    while state["released"] < len(reqs) and (not realtime or reqs[state["released"]].arrival <= t):
+                        r = reqs[state["released"]]
+                        queue[r.signature].append(r)
+                        state["released"] += 1
                         cv.notify_all()
-                    if len(results) == len(reqs):
-                        break
-                    timeout = None
-                    if state["next"] < len(reqs) and realtime:
-                        timeout = max(0.0, reqs[state["next"]].arrival - now())
-                    cv.wait(timeout=timeout if timeout is not None else 0.5)
+                    timeout = 0.5
+                    if state["released"] < len(reqs) and realtime:
+                        timeout = min(timeout, max(0.0, reqs[state["released"]].arrival - now()))
+                    cv.wait(timeout=timeout)
+                if state["error"] is not None:
+                    raise state["error"]
         finally:
             with cv:
-                stop[0] = True
+                state["closing"] = True
+                if state["error"] is None and len(results) < len(reqs):
+                    state["error"] = RuntimeError("serving stopped")
                 cv.notify_all()
             for th in threads:
-                th.join(timeout=60)
+                th.join(timeout=120)
         return [results[r.rid] for r in sorted(reqs, key=lambda r: r.rid)]
+
+    def occupancy(self):
+        """Per pool: Newton steps taken, mean busy slots per step, mean wall time of a step
+        (load, step and take of the finished slots)."""
+        out = {}
+        for sig, v in self.steps.items():
+            b = [x[0] for x in v]
+            t = [x[1] for x in v]
+            out["%s-%d" % sig] = {"slots": self.pool_of[sig].slots if sig in self.pool_of else None,
+                                  "steps": len(v), "mean_busy": float(np.mean(b)) if b else 0.0,
+                                  "ms_per_step": 1e3 * float(np.mean(t)) if t else 0.0}
+        return out
 
 
 def summarize(results):
@@ -261,46 +326,107 @@ def summarize(results):
     last = max(r.done for r in results)
     return {"requests": len(results), "solves_per_s": len(results) / max(last - first, 1e-9),
             "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "mean_batch": float(np.mean([r.batch_size for r in results]))}
+            "mean_newton_steps": float(np.mean([r.stats.get("newton_iters", 0) for r in results])),
+            "converged": int(sum(bool(r.stats.get("converged", False)) for r in results))}
 
 
-class GpuExecutor:
-    """Runs a batch with the HIP library: one batched ``Problem`` on the worker's stream,
-    FP64 Newton; CG for the symmetric class (Jacobi or the multigrid V-cycle,
-    ``cg_precond``), GMRES(``gmres_restart``) + ILU(0) otherwise."""
+class SlotPool:
+    """One mesh class's device-resident batched problem (``slots`` copies of the mesh) on
+    its own HIP stream."""
 
-    def __init__(self, device="cuda:0", cg_precond="jacobi", gmres_restart=200):
+    def __init__(self, executor, sig, slots, warm=True):
+        from . import _lib
+        from .problem import Problem
+        torch = executor.torch
+        self.executor, self.sig, self.slots = executor, tuple(sig), int(slots)
+        kind, N = self.sig
+        self.stream = torch.cuda.Stream(device=executor.device)
+        proto = Request(-1, kind, N, 0, 0.0)
+        mp = proto.mesh_problem()
+        self.mesh = mp.mesh
+        with torch.cuda.stream(self.stream):
+            self.P = Problem([mp] * self.slots, device=executor.device)
+        executor.configure(self.P, kind)
+        self.opts = self.P.newton_options()
+        self._lib = _lib
+        self.state = (_lib.NewtonStepState * self.slots)()
+        self.mask = np.zeros(self.slots, dtype=np.int32)
+        if warm:
+            # one solve in slot 0 allocates the solver workspace and builds its plans
+            self.load(0, proto)
+            while not self.step():
+                pass
+            self.take(0)
+        self.device_bytes = self.P.current_device_bytes()
+
+    def load(self, slot, r):
+        P = self.P
+        params, aux = r.data(self.mesh)
+        P.set_sub_data(slot, params if len(params) else None, aux)
+        a, b = int(P.dof_offsets[slot]), int(P.dof_offsets[slot + 1])
+        with self.executor.torch.cuda.stream(P._stream):
+            P.dofs[a:b].zero_()
+        self.state[slot] = self._lib.NewtonStepState()
+        self.mask[slot] = 1
+
+    def step(self):
+        self.P.newton_step(self.mask, self.state, self.opts)
+        return [s for s in range(self.slots) if self.mask[s] and (self.state[s].converged or self.state[s].failed)]
+
+    def take(self, slot):
+        P = self.P
+        a, b = int(P.dof_offsets[slot]), int(P.dof_offsets[slot + 1])
+        x = P._host(P.dofs[a:b]).copy()
+        s = self.state[slot]
+        self.mask[slot] = 0
+        return x, {"newton_iters": s.newton_iters, "converged": bool(s.converged), "final_maxres": s.maxres,
+                   "max_update": s.max_update, "linear_iters": s.linear_iters,
+                   "linear_failures": s.linear_failures}
+
+    def close(self):
+        self.P = None
+
+
+class GpuPools:
+    """Slot pools on the GPU, FP64: CG for the symmetric kinds (``cg_precond``: Jacobi or
+    the multigrid V-cycle), GMRES(``gmres_restart``) + ILU(0) for Taylor-Hood. Inexact
+    Newton: each linear solve to ``krylov_rtol``, Newton done once max|R| is below
+    ``tol`` (``ns_tol`` for Taylor-Hood) and the last update below ``correction_tol`` of
+    the solution."""
+
+    def __init__(self, device="cuda:0", cg_precond="amg", gmres_restart=200, krylov_rtol=1e-3,
+                 correction_tol=1e-6, tol=1e-8, ns_tol=1e-10, max_newton=30):
         import torch
         self.torch = torch
         self.device = torch.device(device)
         self.cg_precond = cg_precond
         self.gmres_restart = gmres_restart
+        self.krylov_rtol, self.correction_tol = krylov_rtol, correction_tol
+        self.tol, self.ns_tol, self.max_newton = tol, ns_tol, max_newton
 
     def make_stream(self):
         return self.torch.cuda.Stream(device=self.device)
 
-    def configure(self, P, cls):
-        if cls == "cg":
+    def configure(self, P, kind):
+        if SOLVER_CLASS[kind] == "cg":
             P.krylov, P.preconditioner = "cg", self.cg_precond
+            P.newton_solver_tolerance = self.tol
         else:
             P.krylov, P.preconditioner = "gmres", "ilu0"
             P.gmres_restart = self.gmres_restart
-            P.newton_solver_tolerance = 1e-10
+            P.newton_solver_tolerance = self.ns_tol
         P.gmres_max_iters = 20000
+        P.gmres_rtol = self.krylov_rtol
+        P.newton_correction_tolerance = self.correction_tol
+        P.max_newton_iterations = self.max_newton
 
-    def run_batch(self, batch, stream):
-        from .problem import Problem
-        torch = self.torch
-        with torch.cuda.stream(stream):
-            P = Problem([r.mesh_problem() for r in batch], device=self.device)
-            self.configure(P, batch[0].solver)
-            st = P.solve(raise_on_failure=False)
-            x = P.get_current_dofs()
-            outs = [(P.sub_dofs(i, x).copy(), dict(st[i], device_bytes=P.device_bytes))
-                    for i in range(len(batch))]
-            del P
-        return outs
+    def create(self, sig, slots):
+        return SlotPool(self, sig, slots)
 
-    def run_alone(self, request):
-        """The same request solved by itself (for the alone-vs-mix check)."""
-        return self.run_batch([request], self.torch.cuda.current_stream(self.device))[0]
+    def solve_alone(self, request):
+        """The request by itself in a pool of one slot (the alone-versus-mix check)."""
+        pool = SlotPool(self, request.signature, 1, warm=False)
+        pool.load(0, request)
+        while not pool.step():
+            pass
+        return pool.take(0)

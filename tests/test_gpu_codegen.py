@@ -1,0 +1,171 @@
+"""Run-time generated element kernels (pyoomph_amd/codegen) on the GPU.
+
+* The generated Poisson, Bratu and reaction-diffusion kernels give the compiled kernels'
+  FP64 residuals, Jacobian values and dR/dlambda bit for bit (same operations in the same
+  order), steady and with the time derivative; FP32 within 1e-6. Newton solves through the
+  generated kernels give the compiled path's bits and the oracle's solution (1e-8).
+* A form with no compiled counterpart, -div((1 + k u^2) grad u) = s0 with u = 0 on the unit
+  square: by the Kirchhoff transform v = u + k u^3 / 3 solves -lap v = s0, whose solution is
+  the double sine series 16 s0 / (pi^4 m n (m^2 + n^2)) sin(m pi x) sin(n pi y), m, n odd;
+  the P2 solution converges to it, its Jacobian equals central differences of its residual,
+  and Newton converges quadratically.
+"""
+import numpy as np
+import pytest
+
+from oracle import newton as on
+from pyoomph_amd import workloads as wl
+from pyoomph_amd.codegen import library
+from pyoomph_amd.codegen.runtime import GeneratedEquation
+from pyoomph_amd.meshes import CuboidBrickMesh, RectangularQuadMesh
+from tests.oracle_bridge import oracle_problem
+
+pytestmark = pytest.mark.gpu
+
+ALL2D = ("left", "right", "bottom", "top")
+ALL3D = ALL2D + ("back", "front")
+
+
+def pair(kind, **kw):
+    """(compiled-form MeshProblem, generated-form MeshProblem) on the same mesh."""
+    from pyoomph_amd.problem import MeshProblem
+    if kind == "bratu":
+        a = wl.bratu(kw["N"], kw["space"], kw["lam"])
+        mesh = RectangularQuadMesh(kw["N"], split_in_tris="left", space=kw["space"])
+        eq = GeneratedEquation(library.bratu(), space=kw["space"], params=[kw["lam"]])
+        return a, MeshProblem(mesh, eq, dirichlet=[(ALL2D, {0: 0.0})])
+    if kind == "poisson":
+        a = wl.poisson_gaussians(kw["N"], kw.get("seed", 0))
+        mesh = RectangularQuadMesh(kw["N"], split_in_tris="left", space="C2")
+        eq = GeneratedEquation(library.poisson(8), space="C2", params=a.equations.params())
+        return a, MeshProblem(mesh, eq, dirichlet=[(ALL2D, {0: 0.0})])
+    if kind == "rd":
+        a = wl.reaction_diffusion_cube(kw["N"], kw.get("seed", 0))
+        mesh = CuboidBrickMesh(kw["N"])
+        eq = GeneratedEquation(library.reaction_diffusion(), space="C2", aux=a.equations.source_field)
+        return a, MeshProblem(mesh, eq, dirichlet=[(ALL3D, {0: 0.0})])
+    raise KeyError(kind)
+
+
+CASES = [("bratu", dict(N=9, space="C1", lam=1.5)), ("bratu", dict(N=7, space="C2", lam=2.5)),
+         ("poisson", dict(N=8, seed=3)), ("rd", dict(N=3))]
+
+
+@pytest.mark.parametrize("kind,kw", CASES)
+def test_generated_kernels_equal_compiled_bitwise(gpu, kind, kw):
+    from pyoomph_amd.problem import Problem
+    mp_a, mp_b = pair(kind, **kw)
+    A, B = Problem(mp_a), Problem(mp_b)
+    assert A.ndof == B.ndof and A.nnz == B.nnz
+    rng = np.random.default_rng(1)
+    for _ in range(2):
+        x = rng.uniform(-0.4, 0.4, A.ndof)
+        A.set_current_dofs(x)
+        B.set_current_dofs(x)
+        ra, va = A.assemble_device("f64")
+        rb, vb = B.assemble_device("f64")
+        assert np.array_equal(ra.cpu().numpy(), rb.cpu().numpy())
+        assert np.array_equal(va.cpu().numpy(), vb.cpu().numpy())
+        if kind == "bratu":
+            da, dva = A.assemble_device("f64", dparameter="lambda")
+            db, dvb = B.assemble_device("f64", dparameter="lambda")
+            assert np.array_equal(da.cpu().numpy(), db.cpu().numpy())
+            assert np.array_equal(dva.cpu().numpy(), dvb.cpu().numpy())
+        # FP32 kernels (the compiled ones contract to FMA, the generated ones do not)
+        r32a, v32a = A.assemble_device("f32")
+        r32b, v32b = B.assemble_device("f32")
+        scale = np.max(np.abs(va.cpu().numpy()))
+        assert np.max(np.abs(v32a.cpu().numpy().astype(float) - v32b.cpu().numpy())) <= 1e-6 * scale
+        # the time derivative (transient instantiations)
+        h1, h2 = (A.torch.tensor(rng.uniform(-0.3, 0.3, A.ndof), device=A.device) for _ in range(2))
+        w = A.bdf_weights(0.1, 0.07)
+        A.set_time_state(w, h1, h2)
+        B.set_time_state(w, h1, h2)
+        ta, tva = A.assemble_device("f64")
+        tb, tvb = B.assemble_device("f64")
+        A.set_time_state(None, None, None)
+        B.set_time_state(None, None, None)
+        assert np.array_equal(ta.cpu().numpy(), tb.cpu().numpy())
+        assert np.array_equal(tva.cpu().numpy(), tvb.cpu().numpy())
+
+
+@pytest.mark.parametrize("kind,kw,precond", [("bratu", dict(N=20, space="C2", lam=2.0), "amg"),
+                                             ("bratu", dict(N=17, space="C1", lam=0.9), "jacobi"),
+                                             ("poisson", dict(N=16, seed=2), "amg"),
+                                             ("rd", dict(N=4), "ilu0")])
+def test_generated_newton_equals_compiled_and_oracle(gpu, kind, kw, precond):
+    from pyoomph_amd.problem import Problem
+    mp_a, mp_b = pair(kind, **kw)
+    out = []
+    for mp in (mp_a, mp_b):
+        P = Problem(mp)
+        P.krylov, P.preconditioner = "cg", precond
+        st = P.solve()
+        assert st[0]["converged"]
+        out.append(P.get_current_dofs())
+    assert np.array_equal(out[0], out[1])
+    okw = dict(kw)
+    if kind == "rd":
+        okw["f"] = mp_a.equations.source_field
+    op = oracle_problem(kind, **okw)
+    xo = on.newton_solve(op, np.zeros(op.ndof))
+    assert np.max(np.abs(out[1] - xo)) <= 1e-8 * np.max(np.abs(xo))
+
+
+def _series_v(x, y, s0, M=301):
+    """Solution of -lap v = s0 on the unit square, v = 0 on the boundary."""
+    m = np.arange(1, M + 1, 2, dtype=np.float64)
+    v = np.zeros_like(x)
+    sx = np.sin(np.pi * np.outer(x, m))          # [npts, nm]
+    sy = np.sin(np.pi * np.outer(y, m))
+    coef = 16.0 * s0 / (np.pi ** 4 * np.outer(m, m) * (m[:, None] ** 2 + m[None, :] ** 2))
+    v = np.einsum("pi,ij,pj->p", sx, coef, sy)
+    return v
+
+
+def _solve_nd(N, k, s0):
+    from pyoomph_amd.problem import MeshProblem, Problem
+    mesh = RectangularQuadMesh(N, split_in_tris="left", space="C2")
+    eq = GeneratedEquation(library.nonlinear_diffusion(), space="C2", params=[k, s0])
+    mp = MeshProblem(mesh, eq, dirichlet=[(ALL2D, {0: 0.0})])
+    P = Problem(mp)
+    P.krylov, P.preconditioner, P.gmres_restart = "gmres", "ilu0", 100
+    st = P.solve()
+    return mp, P, st
+
+
+def test_nonlinear_diffusion_matches_the_kirchhoff_transform(gpu):
+    k, s0 = 2.0, 8.0
+    errs = []
+    for N in (8, 16):
+        mp, P, st = _solve_nd(N, k, s0)
+        assert st[0]["converged"] and st[0]["newton_iters"] <= 6
+        # quadratic convergence: each residual at most ~ C * previous^2 once small
+        h = st[0]["history"]
+        assert h[-1] < 1e-8 and any(h[i + 1] < 10 * h[i] ** 2 for i in range(len(h) - 1) if h[i] < 1e-2)
+        U = mp.nodal_values(P.get_current_dofs())[:, 0]
+        c = mp.mesh.coords
+        v = _series_v(c[:, 0], c[:, 1], s0)
+        u = v.copy()
+        for _ in range(50):     # u + k u^3 / 3 = v, pointwise Newton
+            u = u - (u + k * u ** 3 / 3 - v) / (1 + k * u * u)
+        errs.append(np.max(np.abs(U - u)) / np.max(np.abs(u)))
+    assert errs[1] < 2e-3, errs
+    assert errs[1] < errs[0] / 3.0, errs     # converging with the mesh
+
+
+def test_nonlinear_diffusion_jacobian_is_the_residual_derivative(gpu):
+    mp, P, st = _solve_nd(6, 2.0, 8.0)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-0.5, 0.5, P.ndof)
+    P.set_current_dofs(x)
+    R, J = P.assemble_jacobian()
+    d = rng.normal(size=P.ndof)
+    eps = 1e-6
+    P.set_current_dofs(x + eps * d)
+    rp = P.get_residuals()
+    P.set_current_dofs(x - eps * d)
+    rm = P.get_residuals()
+    fd = (rp - rm) / (2 * eps)
+    assert np.max(np.abs(J @ d - fd)) <= 1e-6 * np.max(np.abs(fd))
+    assert not np.allclose(J.toarray(), J.toarray().T)   # dF/du makes it non-symmetric

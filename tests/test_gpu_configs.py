@@ -214,24 +214,38 @@ def test_multigrid_on_indefinite_jacobian_reports_failure(gpu):
         P.solve()
 
 
-def test_mixed_precision_sweep_slice_matches_oracle(gpu):
-    """The mixed path on a slice of the sweep: FP32 element kernels for the Jacobian, FP32
-    CG with the FP32 multigrid V-cycle inside FP64 iterative refinement, FP64 residual,
-    Newton stopping on max|R| < 1e-8 and an FP64 update below 1e-10 of the solution: every
-    sub-problem within 1e-8 of the FP64 oracle."""
-    from pyoomph_amd.problem import Problem
+@pytest.fixture(scope="module")
+def converged_slice():
+    """The slice and the oracle's converged solutions (SuperLU Newton to max|R| < 1e-13)."""
     specs, idx = sweep_slice(16)
     sl = [specs[i] for i in idx]
+    ref = []
+    for N, sp, lam in sl:
+        op = oracle_problem("bratu", N=N, space=sp, lam=lam)
+        ref.append(on.newton_solve(op, np.zeros(op.ndof), tol=1e-13, max_iter=30))
+    return idx, sl, ref
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_benchmark_settings_reach_the_converged_solution(gpu, converged_slice, precision):
+    """The benchmark's solver settings on a slice of the sweep: inexact Newton (CG to 1e-3
+    relative residual per step; in mixed precision FP32 element kernels for J and one FP32
+    CG solve with the FP32 V-cycle per step, FP64 residual) stopping at max|R| < 1e-8 and
+    an FP64 update below 1e-6 of the solution. Every sub-problem is within 1e-8 of the
+    converged FP64 oracle solution (pyoomph's residual test alone is not: on 128 x 128 P2
+    meshes it stops ~1e-6 away)."""
+    from pyoomph_amd.problem import Problem
+    idx, sl, ref = converged_slice
     P = Problem([wl.bratu(*s) for s in sl])
     P.krylov, P.preconditioner = "cg", "amg"
-    P.jacobian_precision, P.linear_precision = "f32", "f32"
-    P.newton_correction_tolerance = 1e-10
+    P.jacobian_precision = P.linear_precision = "f32" if precision == "mixed" else "f64"
+    P.gmres_rtol = P.rtol_f32 = 1e-3
+    P.max_refinements = 1
+    P.newton_correction_tolerance = 1e-6
     P.max_newton_iterations = 20
     st = P.solve()
     x = P.get_current_dofs()
     for i, (N, sp, lam) in enumerate(sl):
-        print("mixed sweep problem", idx[i], N, sp, "%.4f" % lam, st[i]["newton_iters"], flush=True)
+        print(precision, "sweep problem", idx[i], N, sp, "%.4f" % lam, st[i]["newton_iters"], flush=True)
         assert st[i]["converged"]
-        op = oracle_problem("bratu", N=N, space=sp, lam=lam)
-        xo = on.newton_solve(op, np.zeros(op.ndof))
-        assert rel(P.sub_dofs(i, x), xo) <= 1e-8, (idx[i], N, sp, lam)
+        assert rel(P.sub_dofs(i, x), ref[i]) <= 1e-8, (idx[i], N, sp, lam)

@@ -1,95 +1,150 @@
-"""Serving-loop scheduler on the CPU: admission under the HBM budget, first-come order,
-batches of one solver class, every request served once (stand-in executor, no GPU)."""
+"""Serving-loop scheduler on the CPU with a stand-in executor (no GPU): every request is
+served once by its mesh class's pool; slots are never over-filled; requests join a running
+pool at step boundaries in arrival order and leave as soon as their solve is done; at most
+``workers`` pools step at once; pools are created within the HBM budget; a failing step
+surfaces."""
 import threading
 import time
 
 import numpy as np
 import pytest
 
-from pyoomph_amd.serving import (Request, ServingLoop, poisson_stream, size_for_dofs, summarize,
-                                 _NDOF)
+from pyoomph_amd.serving import SIZES, Request, ServingLoop, poisson_stream, slots_for, summarize, _NDOF
+
+
+class FakePool:
+    """A request needs 1 + seed % 4 Newton steps; a step sleeps in proportion to the busy
+    unknowns."""
+
+    def __init__(self, ex, sig, slots):
+        self.ex, self.sig, self.slots = ex, sig, slots
+        self.device_bytes = ex.bytes_per_slot * slots
+        self.busy = {}
+        self.log = []            # (step index, frozenset of rids in the step)
+        self.loads = []          # rids in load order
+
+    def load(self, slot, r):
+        assert slot not in self.busy and 0 <= slot < self.slots
+        self.busy[slot] = [r, 0]
+        self.loads.append(r.rid)
+
+    def step(self):
+        ex = self.ex
+        with ex.lock:
+            ex.running += 1
+            ex.max_running = max(ex.max_running, ex.running)
+        self.log.append(frozenset(r.rid for r, _ in self.busy.values()))
+        if ex.fail_after is not None and sum(len(p.log) for p in ex.pools) > ex.fail_after:
+            raise RuntimeError("device failure")
+        time.sleep(0.0005 + ex.per_dof * sum(r.ndof for r, _ in self.busy.values()))
+        done = []
+        for slot, st in self.busy.items():
+            st[1] += 1
+            if st[1] >= 1 + st[0].seed % 4:
+                done.append(slot)
+        with ex.lock:
+            ex.running -= 1
+        return done
+
+    def take(self, slot):
+        r, n = self.busy.pop(slot)
+        return np.full(3, float(r.rid)), {"newton_iters": n, "converged": True}
+
+    def close(self):
+        pass
 
 
 class FakeExecutor:
-    """Sleeps in proportion to the batch's unknowns; records what ran together."""
-
-    def __init__(self, per_dof=2e-9):
-        self.per_dof = per_dof
+    def __init__(self, per_dof=2e-9, bytes_per_slot=1000, fail_after=None):
+        self.per_dof, self.bytes_per_slot, self.fail_after = per_dof, bytes_per_slot, fail_after
         self.lock = threading.Lock()
-        self.running = 0
-        self.max_running = 0
-        self.batches = []
+        self.running = self.max_running = 0
+        self.pools = []
 
     def make_stream(self):
         return None
 
-    def run_batch(self, batch, stream):
-        with self.lock:
-            self.running += 1
-            self.max_running = max(self.max_running, self.running)
-            self.batches.append([r.rid for r in batch])
-        time.sleep(0.001 + self.per_dof * sum(r.ndof for r in batch))
-        with self.lock:
-            self.running -= 1
-        return [(np.full(3, float(r.rid)), {"converged": True}) for r in batch]
+    def create(self, sig, slots):
+        p = FakePool(self, sig, slots)
+        self.pools.append(p)
+        return p
 
 
-def test_sizes_and_stream_are_seeded():
+def all_sigs():
+    return [(k, N) for k, v in SIZES.items() for N in v]
+
+
+def test_stream_is_seeded_and_sizes_are_the_served_ones():
     a = poisson_stream(500, 40, seed=3)
     b = poisson_stream(500, 40, seed=3)
     assert [(r.kind, r.N, r.seed, r.arrival) for r in a] == [(r.kind, r.N, r.seed, r.arrival) for r in b]
     assert all(x.arrival < y.arrival for x, y in zip(a, a[1:]))
-    for r in a:
-        assert r.ndof >= 5_000 * 0.999
-        lim = 12_000 if r.kind == "ns" else 600_000
-        # the smallest size with at least the drawn count of unknowns
-        assert _NDOF[r.kind](r.N - 1) < lim
-    for kind in ("poisson", "rd", "ns"):
-        N = size_for_dofs(kind, 10_000)
-        assert _NDOF[kind](N) >= 10_000 > _NDOF[kind](N - 1)
+    assert all(r.N in SIZES[r.kind] for r in a)
+    for k, sizes in SIZES.items():
+        nd = [_NDOF[k](N) for N in sizes]
+        assert nd == sorted(nd) and nd[0] >= 5_000 and nd[-1] <= 600_000
+    assert slots_for("poisson", 36) == 16 and slots_for("rd", 40) == 4
 
 
-def test_every_request_served_once_with_class_pure_batches():
-    reqs = poisson_stream(2000, 60, seed=1)
+def test_every_request_served_once_by_its_pool_in_arrival_order():
+    reqs = poisson_stream(3000, 200, seed=1)
     ex = FakeExecutor()
-    loop = ServingLoop(ex, hbm_budget=1e12, workers=3, max_batch_dofs=2_000_000)
+    loop = ServingLoop(ex, hbm_budget=1e12, workers=3, slots=lambda k, N: 3)
+    loop.prepare(all_sigs())
     res = loop.run(reqs)
-    assert [r.rid for r in res] == [r.rid for r in sorted(reqs, key=lambda r: r.rid)]
+    assert [r.rid for r in res] == list(range(len(reqs)))
     assert all(float(r.dofs[0]) == r.rid for r in res)
-    seen = [rid for b in ex.batches for rid in b]
-    assert sorted(seen) == list(range(len(reqs)))
-    kinds = {r.rid: r.solver for r in reqs}
-    for b in ex.batches:
-        assert len({kinds[i] for i in b}) == 1
-        assert sum(reqs[i].ndof for i in b) <= 2_000_000 or len(b) == 1
+    assert all(r.pool == q.signature for r, q in zip(res, sorted(reqs, key=lambda q: q.rid)))
     assert ex.max_running <= 3
+    by_rid = {q.rid: q for q in reqs}
+    for p in ex.pools:
+        # never more busy slots than the pool has; FIFO within the pool
+        assert all(len(s) <= p.slots for s in p.log)
+        assert p.loads == sorted(p.loads, key=lambda i: by_rid[i].arrival)
+        # each request stays for exactly its step count, in consecutive steps of the pool
+        for rid in p.loads:
+            idx = [i for i, s in enumerate(p.log) if rid in s]
+            assert idx == list(range(idx[0], idx[0] + 1 + by_rid[rid].seed % 4))
     for r in res:
-        assert r.arrival <= r.admitted + 1e-6 and r.admitted <= r.started + 1e-6 and r.started <= r.done
+        assert r.arrival <= r.admitted + 1e-6 and r.admitted <= r.done
+        assert r.stats["newton_iters"] == 1 + by_rid[r.rid].seed % 4
     s = summarize(res)
     assert s["requests"] == len(reqs) and s["p99_s"] >= s["p50_s"] > 0
 
 
-def test_budget_is_never_exceeded_and_admission_is_fifo():
-    reqs = [Request(i, "poisson" if i % 2 else "rd", 20, i, 0.0) for i in range(24)]
-    est = lambda r: 1000.0 if r.kind == "rd" else 400.0
-    ex = FakeExecutor(per_dof=1e-8)
-    loop = ServingLoop(ex, hbm_budget=2500.0, workers=4, max_batch_dofs=10**9, estimator=est)
-    res = loop.run(reqs, realtime=False)
-    assert loop.peak_reserved <= 2500.0
-    adm = sorted(res, key=lambda r: r.admitted)
-    assert [r.rid for r in adm] == sorted(r.rid for r in res)
-    assert len(res) == 24
+def test_newcomers_join_a_running_pool_and_finished_ones_leave():
+    # one pool, 4 slots; a long request (4 steps) and short ones arriving later
+    reqs = [Request(0, "poisson", 36, 3, 0.0)] + [Request(i, "poisson", 36, 4 * i, 0.002 * i) for i in range(1, 9)]
+    ex = FakeExecutor(per_dof=1e-6)
+    loop = ServingLoop(ex, hbm_budget=1e12, workers=1, slots=lambda k, N: 4)
+    loop.prepare([("poisson", 36)])
+    res = loop.run(reqs)
+    (p,) = ex.pools
+    # some step ran the long request together with a request admitted after it started
+    first = [i for i, s in enumerate(p.log) if 0 in s]
+    assert any(len(p.log[i] - {0}) > 0 and any(x not in p.log[first[0]] for x in p.log[i]) for i in first[1:])
+    # the short requests (1 step each) finished before the long one
+    assert all(r.done <= res[0].done for r in res[1:3])
 
 
-def test_request_larger_than_budget_is_rejected():
-    loop = ServingLoop(FakeExecutor(), hbm_budget=10.0)
+def test_pools_are_created_within_the_budget():
+    loop = ServingLoop(FakeExecutor(bytes_per_slot=100), hbm_budget=1000.0, slots=lambda k, N: 4)
+    loop.prepare([("poisson", 36), ("rd", 10)])
+    assert loop.reserved == 800
+    with pytest.raises(MemoryError):
+        loop.prepare([("ns", 26)])
+    assert loop.reserved == 800 and ("ns", 26) not in loop.pool_of
+
+
+def test_request_without_a_pool_is_rejected():
+    loop = ServingLoop(FakeExecutor())
+    loop.prepare([("poisson", 36)])
     with pytest.raises(ValueError):
-        loop.run([Request(0, "rd", 30, 0, 0.0)])
+        loop.run([Request(0, "rd", 10, 0, 0.0)])
 
 
-def test_executor_failure_surfaces():
-    class Bad(FakeExecutor):
-        def run_batch(self, batch, stream):
-            raise RuntimeError("device failure")
-    with pytest.raises(RuntimeError):
-        ServingLoop(Bad(), workers=2).run(poisson_stream(1000, 5, seed=0))
+def test_step_failure_surfaces():
+    loop = ServingLoop(FakeExecutor(fail_after=5), workers=2, slots=lambda k, N: 2)
+    loop.prepare(all_sigs())
+    with pytest.raises(RuntimeError, match="device failure"):
+        loop.run(poisson_stream(1000, 30, seed=0))
