@@ -265,7 +265,9 @@ int pyhip_newton_solve(pyhip_problem *p, double *dofs_dev, const pyhip_newton_op
 /* One Newton iteration of pyhip_newton_solve for the sub-problems with mask[s] != 0 that
  * have not converged or failed yet; state[s] (host, [nsub]) carries each sub-problem's
  * progress between calls, newton_iters = 0 starting a new solve (the caller puts the
- * start values into its dofs). Convergence and failure follow pyhip_newton_solve. Every
+ * start values into its dofs). Convergence and failure follow pyhip_newton_solve, and a
+ * sub-problem whose start residual is not finite fails at once. Element kernels, the
+ * preconditioner set-up and the solves cover the stepped sub-problems only. Every
  * sub-problem's iteration is its own (per-sub-problem assembly rows, Krylov scalars and
  * tests; the multigrid hierarchy is rebuilt from every call's Jacobian), so a
  * sub-problem gives the same iterates whichever others run in the same calls.

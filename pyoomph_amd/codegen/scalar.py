@@ -192,6 +192,9 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
     const int le = threadIdx.x / NS, row = threadIdx.x % NS;
     const int e = blockIdx.x * EPB + le;
     const bool active = (le < EPB) && (e < a.nel);
+    // a block none of whose elements belongs to a sub-problem being assembled exits
+    // (the same decision for every thread, before any barrier)
+    if (a.sub_on && !__syncthreads_or(active && a.sub_on[a.sub[e]])) return;
     if (active) {{
         const int node = a.conn[(int64_t)row * a.nel + e];
         const int eq = a.seqn[(int64_t)row * a.nel + e];

@@ -57,7 +57,10 @@ int amg_build(AmgDev &A, int64_t n, const int32_t *rs_h, const int32_t *col_h,
 struct KernelTimer;
 // numeric set-up for the matrix values `vals` (level-0 pattern); with a timer, the
 // power-iteration spmv launches are timed and counted as "spmv" launches
-int amg_setup(AmgDev &A, hipStream_t st, const double *vals, KernelTimer *tm = nullptr);
+// need_h: host flags [nsub] of the sub-problems whose hierarchy is used (null: all); a
+// coarsest matrix that is not positive definite is reported only for those
+int amg_setup(AmgDev &A, hipStream_t st, const double *vals, KernelTimer *tm = nullptr,
+              const int32_t *need_h = nullptr);
 // the finest level's values (the current matrix; the rest of the hierarchy is kept)
 int amg_set_fine(AmgDev &A, const double *vals);
 // z = M^-1 b. With a timer, every level spmv is timed and counted as an "spmv" launch.

@@ -580,7 +580,7 @@ static int level_weights(hipStream_t st, AmgDevLevel &L, const double *vals, dou
     return PYHIP_OK;
 }
 
-int amg_setup(AmgDev &A, hipStream_t st, const double *vals0, KernelTimer *tm) {
+int amg_setup(AmgDev &A, hipStream_t st, const double *vals0, KernelTimer *tm, const int32_t *need_h) {
     if (!A.built) {
         set_error("multigrid plan not built");
         return PYHIP_ERR_INVALID;
@@ -621,7 +621,7 @@ int amg_setup(AmgDev &A, hipStream_t st, const double *vals0, KernelTimer *tm) {
                                    hipMemcpyDeviceToHost, st));
     PYHIP_CHECK_HIP(hipStreamSynchronize(st));
     for (int s = 0; s < A.nsub; s++)
-        if (fail_h[s]) {
+        if (fail_h[s] && (!need_h || need_h[s])) {
             set_error("multigrid: the coarsest matrix of sub-problem " + std::to_string(s) +
                       " is not positive definite (indefinite Jacobian, e.g. past a fold); use the "
                       "ILU(0) or Jacobi preconditioner there");

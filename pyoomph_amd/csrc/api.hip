@@ -441,7 +441,7 @@ static int check_assemble_args(pyhip_problem *p, const double *dofs, int flag, i
 // Element kernels into the element buffers Er / Ej (FP64 or FP32 by precision; Ej may be
 // null when flag == 0), then the deterministic gather into residual / values.
 static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int deriv_param, int precision,
-                         double *residual, void *values, void *Er, void *Ej) {
+                         double *residual, void *values, void *Er, void *Ej, const int32_t *sub_on = nullptr) {
     int rc;
     for (auto &G : p->groups) {
         if (precision == PYHIP_F64) {
@@ -459,6 +459,7 @@ static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int der
             a.hist1 = p->hist1; a.hist2 = p->hist2;
             a.hvec = p->hvec;
             a.fcoef = G.coef;
+            a.sub_on = sub_on;
             TimeScope ts(&p->timer, "element", p->st);
             if (G.form == kFormFlux) {
                 if ((rc = launch_faces_f64(G.kind, a, p->st))) return rc;
@@ -481,6 +482,7 @@ static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int der
             a.hist1 = p->hist1; a.hist2 = p->hist2;
             a.hvec = p->hvec;
             a.fcoef = G.coef;
+            a.sub_on = sub_on;
             TimeScope ts(&p->timer, "element", p->st);
             if (G.form == kFormFlux) {
                 if ((rc = launch_faces_f32(G.kind, a, p->st))) return rc;
@@ -551,6 +553,17 @@ extern "C" {
 
 int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_param,
                    int precision, double *residual, void *values) {
+    return pyhip::assemble_subset(p, dofs, flag, deriv_param, precision, residual, values, nullptr);
+}
+
+}  // extern "C"
+
+namespace pyhip {
+// pyhip_assemble over the sub-problems with sub_on[s] != 0 (device array [nsub], or null for
+// all): the element kernels skip the blocks of the others, whose residual and CSR rows are
+// then unspecified (computed from element rows left by earlier calls).
+int assemble_subset(pyhip_problem *p, const double *dofs, int flag, int deriv_param, int precision,
+                    double *residual, void *values, const int32_t *sub_on) {
     int rc = check_assemble_args(p, dofs, flag, deriv_param, precision, residual, values);
     if (rc) return rc;
     void *Er, *Ej;
@@ -573,10 +586,14 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
         Er = p->Er32;
         Ej = p->Ej32;
     }
-    if ((rc = assemble_into(p, dofs, flag, deriv_param, precision, residual, values, Er, Ej))) return rc;
-    if (precision == PYHIP_F64 && flag) p->have_ej64 = (deriv_param < 0);   // the last FP64 matrix
+    if ((rc = assemble_into(p, dofs, flag, deriv_param, precision, residual, values, Er, Ej, sub_on))) return rc;
+    // the export reads the element matrices of the last complete FP64 Jacobian
+    if (precision == PYHIP_F64 && flag) p->have_ej64 = (deriv_param < 0) && !sub_on;
     return PYHIP_OK;
 }
+}  // namespace pyhip
+
+extern "C" {
 
 int pyhip_assemble_noalloc(pyhip_problem *p, const double *dofs, int flag, int deriv_param, int precision,
                            double *residual, void *values) {

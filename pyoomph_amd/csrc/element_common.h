@@ -58,6 +58,9 @@ struct ElemArgs {
     const double *hvec;
     // boundary faces: [nel][2] alpha, beta of each face's flux alpha*u + beta
     const double *fcoef;
+    // [nsub] or nullptr: elements of sub-problems with sub_on[s] == 0 may be skipped (their
+    // element rows keep what they held)
+    const int32_t *sub_on;
 };
 
 template <int KIND> struct ElemTraits;

@@ -73,6 +73,9 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
     const int row = threadIdx.x % NS;       // slot row of this thread
     const int e = blockIdx.x * EPB + le;
     const bool active = (le < EPB) && (e < a.nel);
+    // a block none of whose elements belongs to a sub-problem being assembled exits
+    // (the same decision for every thread, before any barrier)
+    if (a.sub_on && !__syncthreads_or(active && a.sub_on[a.sub[e]])) return;
 
     if (active) {
         const int node = a.conn[(int64_t)row * a.nel + e];
@@ -257,6 +260,9 @@ __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
     const int slot = threadIdx.x % NS;
     const int e = blockIdx.x * EPB + le;
     const bool active = (le < EPB) && (e < a.nel);
+    // a block none of whose elements belongs to a sub-problem being assembled exits
+    // (the same decision for every thread, before any barrier)
+    if (a.sub_on && !__syncthreads_or(active && a.sub_on[a.sub[e]])) return;
     if (active) {
         const int ln = kTHSlotNode[slot];
         const int kind = kTHSlotKind[slot];

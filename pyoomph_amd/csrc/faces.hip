@@ -70,7 +70,7 @@ struct FaceRuleQ3 {
 template <typename T, int NN, typename RULE>
 __global__ __launch_bounds__(256) void flux_face_kernel(ElemArgs<T> a) {
     const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= a.nel) return;
+    if (e >= a.nel || (a.sub_on && !a.sub_on[a.sub[e]])) return;
     const T alpha = (T)a.fcoef[2 * (int64_t)e], beta = (T)a.fcoef[2 * (int64_t)e + 1];
     T X[NN][2], U[NN];
 #pragma unroll
@@ -134,7 +134,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void flux_face_quad9_kernel(ElemArgs<T> a) {
     constexpr int NN = 9, NQ = 9;
     const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= a.nel) return;
+    if (e >= a.nel || (a.sub_on && !a.sub_on[a.sub[e]])) return;
     const T alpha = (T)a.fcoef[2 * (int64_t)e], beta = (T)a.fcoef[2 * (int64_t)e + 1];
     T X[NN][3], U[NN];
 #pragma unroll
@@ -194,7 +194,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void traction_face_kernel(ElemArgs<T> a) {
     constexpr int NN = 3, NS = 6;
     const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= a.nel) return;
+    if (e >= a.nel || (a.sub_on && !a.sub_on[a.sub[e]])) return;
     const T alpha = (T)a.fcoef[3 * (int64_t)e];
     const T beta[2] = {(T)a.fcoef[3 * (int64_t)e + 1], (T)a.fcoef[3 * (int64_t)e + 2]};
     T X[NN][2], U[NN][2];

@@ -96,7 +96,8 @@ struct KrylovWork {
     int ilu_order = 0;                // PYHIP_ILU_ORDER_* of the current solve
     AmgDev amg;
     bool amg_stale = true;            // multigrid values to be rebuilt at the next set-up
-    bool step_res_valid = false;      // res64 holds the residual pyhip_newton_step ended with
+    std::vector<int32_t> need_h;      // host copy of the sub-problems a preconditioner set-up serves
+    std::vector<uint8_t> step_res_ok; // [nsub] res64 / maxres_h hold the sub-problem's residual at its dofs
     const double *amg_src64 = nullptr;   // FP64 matrix the FP32 V-cycle's set-up reads (or null)
     double *amg_conv64 = nullptr;     // [nnz] FP32 matrix widened for that set-up
     // per sub-problem: 1 if the last Krylov solve (or refinement) ended above its tolerance

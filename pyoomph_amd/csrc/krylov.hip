@@ -305,11 +305,17 @@ __global__ __launch_bounds__(256) void ilu_factor_level_kernel(int64_t nrows,
                                                                const int32_t *__restrict__ col,
                                                                const int32_t *__restrict__ diag,
                                                                T *__restrict__ lu,
-                                                               int32_t *__restrict__ err) {
+                                                               int32_t *__restrict__ err,
+                                                               const int32_t *__restrict__ perm,
+                                                               const int32_t *__restrict__ row_sub,
+                                                               const int32_t *__restrict__ need) {
     const int64_t w = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
     const int lane = threadIdx.x % 64;
     if (w >= nrows) return;
     const int32_t i = rows[w];
+    // rows of sub-problems that are not being solved keep their old factor (sub-problems
+    // never couple, so no needed row reads them)
+    if (need && !need[row_sub[perm[i]]]) return;
     const int32_t r0 = rs[i], r1 = rs[i + 1], di = diag[i];
     for (int32_t kk = r0; kk < di; kk++) {
         const int32_t k = col[kk];
@@ -710,7 +716,7 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
 }
 
 template <typename TV, typename T>
-static int ilu_factor(pyhip_problem *p, KrylovWork *W, const TV *vals) {
+static int ilu_factor(pyhip_problem *p, KrylovWork *W, const TV *vals, const int32_t *need) {
     IluPlan &L = W->ilu;
     int rc;
     const int prec = std::is_same<T, float>::value ? PYHIP_F32 : PYHIP_F64;
@@ -728,7 +734,8 @@ static int ilu_factor(pyhip_problem *p, KrylovWork *W, const TV *vals) {
     for (size_t l = 0; l + 1 < L.lo_ptr.size(); l++) {
         const int64_t nr = L.lo_ptr[l + 1] - L.lo_ptr[l];
         hipLaunchKernelGGL((ilu_factor_level_kernel<T>), dim3(nblk(nr, 4)), dim3(256), 0, p->st, nr,
-                           L.lo_rows + L.lo_ptr[l], L.rs, L.col, L.diag, lu, W->nactive_d);
+                           L.lo_rows + L.lo_ptr[l], L.rs, L.col, L.diag, lu, W->nactive_d, L.perm, p->dof_sub,
+                           need);
     }
     KCHECK();
     PYHIP_CHECK_HIP(hipMemcpyAsync(W->nactive_h, W->nactive_d, sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
@@ -783,8 +790,17 @@ static int precond_apply(pyhip_problem *p, KrylovWork *W, int pc, const T *v, T 
     return PYHIP_OK;
 }
 
+// W->need_h = the device flags need[nsub] (waits for the stream)
+static int need_on_host(pyhip_problem *p, KrylovWork *W, const int32_t *need) {
+    if (!need) return PYHIP_OK;
+    W->need_h.resize(W->nsub);
+    PYHIP_CHECK_HIP(hipMemcpyAsync(W->need_h.data(), need, W->nsub * sizeof(int32_t), hipMemcpyDeviceToHost, p->st));
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    return PYHIP_OK;
+}
+
 template <typename TV, typename T>
-static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals) {
+static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals, const int32_t *need = nullptr) {
     TimeScope ts(&p->timer, "precond", p->st);
     if (pc == PYHIP_PRECOND_AMG) {
         if constexpr (std::is_same<T, double>::value && std::is_same<TV, double>::value) {
@@ -794,7 +810,8 @@ static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals
             // first matrix of a solve and kept for its later Newton steps; the finest level
             // always uses the current matrix
             if (!W->amg_stale) return amg_set_fine(W->amg, vals);
-            rc = amg_setup(W->amg, p->st, vals, &p->timer);
+            if ((rc = need_on_host(p, W, need))) return rc;
+            rc = amg_setup(W->amg, p->st, vals, &p->timer, need ? W->need_h.data() : nullptr);
             W->amg_stale = rc != PYHIP_OK;   // a failed set-up is redone by the next solve
             return rc;
         } else if constexpr (std::is_same<T, float>::value && std::is_same<TV, float>::value) {
@@ -811,7 +828,8 @@ static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals
                     KCHECK();
                     src = W->amg_conv64;
                 }
-                rc = amg_setup(W->amg, p->st, src, &p->timer);
+                if ((rc = need_on_host(p, W, need))) return rc;
+                rc = amg_setup(W->amg, p->st, src, &p->timer, need ? W->need_h.data() : nullptr);
                 W->amg_stale = rc != PYHIP_OK;
                 if (rc || (rc = amg_make_f32(W->amg, p->st, W->bytes))) return rc;
             }
@@ -823,7 +841,7 @@ static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals
     if (pc == PYHIP_PRECOND_ILU0) {
         int rc = ilu_plan_build(p, W);
         if (rc) return rc;
-        return ilu_factor<TV, T>(p, W, vals);
+        return ilu_factor<TV, T>(p, W, vals, need);
     }
     T *jd = std::is_same<T, float>::value ? (T *)W->jdiag32 : (T *)W->jdiag64;
     hipLaunchKernelGGL((jacobi_setup_kernel<TV, T>), dim3(nblk(p->ndof)), dim3(256), 0, p->st, p->ndof,
@@ -1657,7 +1675,7 @@ static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts
     if (o.precision == PYHIP_F64) {
         PYHIP_REQUIRE(jac_prec == PYHIP_F64, "an FP64 Krylov solve needs the FP64 Jacobian");
         W->ilu_order = o.ilu_order;
-        if ((rc = precond_setup<double, double>(p, W, o.precond, W->vals64))) return rc;
+        if ((rc = precond_setup<double, double>(p, W, o.precond, W->vals64, mask))) return rc;
         return krylov<double, double>(p, W, W->vals64, r, d, o, o.rtol, 0.0, mask, iters, failed);
     }
     // FP32 inner solves inside FP64 iterative refinement:
@@ -1669,7 +1687,7 @@ static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts
     }
     W->ilu_order = o.ilu_order;
     W->amg_src64 = jac_prec == PYHIP_F64 ? W->vals64 : nullptr;
-    if ((rc = precond_setup<float, float>(p, W, o.precond, W->vals32))) return rc;
+    if ((rc = precond_setup<float, float>(p, W, o.precond, W->vals32, mask))) return rc;
     PYHIP_CHECK_HIP(hipMemsetAsync(d, 0, n * sizeof(double), p->st));
     hipLaunchKernelGGL((convert_kernel<double, double>), dim3(nblk(n)), dim3(256), 0, p->st, n, r, W->r64);
     KCHECK();
@@ -1809,7 +1827,7 @@ int pyhip_newton_solve(pyhip_problem *p, double *x, const pyhip_newton_opts *o,
     if ((rc = krylov_setup(p, o->linear.restart))) return rc;
     KrylovWork *W = p->kw;
     W->amg_stale = true;
-    W->step_res_valid = false;
+    W->step_res_ok.assign(W->step_res_ok.size(), 0);
     const int ns = p->nsub;
     const int64_t n = p->ndof;
     if (o->jacobian_precision == PYHIP_F64 && !W->vals64)
@@ -1927,7 +1945,8 @@ int pyhip_newton_step(pyhip_problem *p, double *x, const pyhip_newton_opts *o, c
         if ((rc = dev_alloc(&W->vals32, p->nnz, W->bytes))) return rc;
     std::vector<int32_t> running(ns, 0);
     int nrun = 0;
-    bool fresh = false;
+    bool need_res = false;
+    W->step_res_ok.resize(ns, 0);
     for (int s = 0; s < ns; s++) {
         const bool empty = p->dof_off_h[s + 1] == p->dof_off_h[s];
         if (!mask[s] || st[s].converged || st[s].failed) continue;
@@ -1937,42 +1956,57 @@ int pyhip_newton_step(pyhip_problem *p, double *x, const pyhip_newton_opts *o, c
         }
         running[s] = 1;
         nrun++;
-        fresh = fresh || st[s].newton_iters == 0;
+        need_res = need_res || st[s].newton_iters == 0 || !W->step_res_ok[s];
     }
+    if (!nrun) return PYHIP_OK;
+    // Residuals, Jacobians and solves cover the running sub-problems only (W->dof_mask):
+    // the element kernels skip the others' blocks, the Krylov kernels their rows.
+    auto upload_running = [&]() -> int {
+        PYHIP_CHECK_HIP(hipMemcpyAsync(W->dof_mask, running.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice,
+                                       p->st));
+        return PYHIP_OK;
+    };
     auto residual_norms = [&]() -> int {
         int r2;
-        if ((r2 = pyhip_assemble(p, x, 0, -1, PYHIP_F64, W->res64, nullptr))) return r2;
+        if ((r2 = assemble_subset(p, x, 0, -1, PYHIP_F64, W->res64, nullptr, W->dof_mask))) return r2;
         if ((r2 = batched_maxabs(p, W->res64, W->maxres))) return r2;
         PYHIP_CHECK_HIP(hipMemcpyAsync(W->maxres_h, W->maxres, ns * sizeof(double), hipMemcpyDeviceToHost, p->st));
         PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+        for (int s = 0; s < ns; s++) W->step_res_ok[s] = (uint8_t)running[s];
         return PYHIP_OK;
     };
-    if (!nrun) return PYHIP_OK;
+    if ((rc = upload_running())) return rc;
     // a sub-problem starting its solve: the first residual (count 1: without
     // always_take_one_newton_step a small enough start is already converged); otherwise
-    // the residual the last call ended with is still the one at the running dofs
-    if (fresh || !W->step_res_valid) {
-        if ((rc = residual_norms())) return rc;
-    }
-    W->step_res_valid = false;
+    // the residual the call that last stepped it ended with is still the one at its dofs
+    if (need_res && (rc = residual_norms())) return rc;
+    bool dropped = false;
     for (int s = 0; s < ns; s++) {
         if (!running[s] || st[s].newton_iters) continue;
         st[s].maxres = W->maxres_h[s];
         if (W->maxres_h[s] < o->tol && !o->always_take_one_newton_step) {
             st[s].converged = 1;
-            running[s] = 0;
-            nrun--;
+        } else if (!std::isfinite(W->maxres_h[s])) {
+            // a non-finite start residual (non-finite data) cannot converge; failing it here
+            // keeps its Jacobian out of the shared preconditioner set-up, whose breakdown
+            // would stop every sub-problem of the call
+            st[s].failed = 1;
+        } else {
+            continue;
         }
+        running[s] = 0;
+        nrun--;
+        dropped = true;
     }
     if (!nrun) return PYHIP_OK;
     // the multigrid hierarchy comes from this step's Jacobian, so that no sub-problem's
     // preconditioner depends on when the others started their solves
     W->amg_stale = true;
-    PYHIP_CHECK_HIP(hipMemcpyAsync(W->dof_mask, running.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, p->st));
+    if (dropped && (rc = upload_running())) return rc;
     if (o->jacobian_precision == PYHIP_F64) {
-        if ((rc = pyhip_assemble(p, x, 1, -1, PYHIP_F64, W->w64, W->vals64))) return rc;
+        if ((rc = assemble_subset(p, x, 1, -1, PYHIP_F64, W->w64, W->vals64, W->dof_mask))) return rc;
     } else {
-        if ((rc = pyhip_assemble(p, x, 1, -1, PYHIP_F32, W->w64, W->vals32))) return rc;
+        if ((rc = assemble_subset(p, x, 1, -1, PYHIP_F32, W->w64, W->vals32, W->dof_mask))) return rc;
     }
     int its = 0, refs = 0, lfail = 0;
     if ((rc = linear_solve(p, W, o->linear, o->jacobian_precision, W->res64, W->d64, W->dof_mask, &its, &refs,
@@ -1999,7 +2033,6 @@ int pyhip_newton_step(pyhip_problem *p, double *x, const pyhip_newton_opts *o, c
         if (S.maxres < o->tol && small_update) S.converged = 1;
         else if (S.maxres > o->max_residuals || S.newton_iters >= o->max_iter) S.failed = 1;
     }
-    W->step_res_valid = true;
     return PYHIP_OK;
 }
 
@@ -2019,7 +2052,7 @@ int pyhip_arclength_step(pyhip_problem *p, double *x, int param, const pyhip_new
     if ((rc = krylov_setup(p, o->linear.restart))) return rc;
     KrylovWork *W = p->kw;
     W->amg_stale = true;
-    W->step_res_valid = false;
+    W->step_res_ok.assign(W->step_res_ok.size(), 0);
     const int ns = p->nsub;
     const int64_t n = p->ndof;
     if (!W->vals64 && (rc = dev_alloc(&W->vals64, p->nnz, W->bytes))) return rc;

@@ -114,6 +114,10 @@ struct TimeScope {
     }
 };
 
+// pyhip_assemble restricted to the sub-problems with sub_on[s] != 0 (device [nsub]; null:
+// all of them); api.hip
+int assemble_subset(pyhip_problem *p, const double *dofs, int flag, int deriv_param, int precision,
+                    double *residual, void *values, const int32_t *sub_on);
 int launch_elements_f64(int kind, int form, const ElemArgs<double> &a, hipStream_t s);
 int launch_elements_f32(int kind, int form, const ElemArgs<float> &a, hipStream_t s);
 // run-time generated forms (codegen_rt.hip)

@@ -387,19 +387,27 @@ class SlotPool:
         self.P = None
 
 
+# CG preconditioner per symmetric kind: the multigrid V-cycle for the P2 Poisson requests;
+# Jacobi for the Q2 bricks, whose Galerkin products make the per-step multigrid set-up cost
+# more than the iterations it saves
+DEFAULT_CG_PRECOND = {"poisson": "amg", "rd": "jacobi"}
+
+
 class GpuPools:
-    """Slot pools on the GPU, FP64: CG for the symmetric kinds (``cg_precond``: Jacobi or
-    the multigrid V-cycle), GMRES(``gmres_restart``) + ILU(0) for Taylor-Hood. Inexact
+    """Slot pools on the GPU, FP64: CG for the symmetric kinds (``cg_precond``: "jacobi",
+    "amg", or a dict per kind), GMRES(``gmres_restart``) + ILU(0) for Taylor-Hood. Inexact
     Newton: each linear solve to ``krylov_rtol``, Newton done once max|R| is below
     ``tol`` (``ns_tol`` for Taylor-Hood) and the last update below ``correction_tol`` of
     the solution."""
 
-    def __init__(self, device="cuda:0", cg_precond="amg", gmres_restart=200, krylov_rtol=1e-3,
-                 correction_tol=1e-6, tol=1e-8, ns_tol=1e-10, max_newton=30):
+    def __init__(self, device="cuda:0", cg_precond=None, gmres_restart=200, krylov_rtol=1e-4,
+                 correction_tol=1e-7, tol=1e-8, ns_tol=1e-10, max_newton=30):
         import torch
         self.torch = torch
         self.device = torch.device(device)
-        self.cg_precond = cg_precond
+        cg_precond = cg_precond or DEFAULT_CG_PRECOND
+        self.cg_precond = dict(cg_precond) if isinstance(cg_precond, dict) else \
+            {k: cg_precond for k, c in SOLVER_CLASS.items() if c == "cg"}
         self.gmres_restart = gmres_restart
         self.krylov_rtol, self.correction_tol = krylov_rtol, correction_tol
         self.tol, self.ns_tol, self.max_newton = tol, ns_tol, max_newton
@@ -409,7 +417,7 @@ class GpuPools:
 
     def configure(self, P, kind):
         if SOLVER_CLASS[kind] == "cg":
-            P.krylov, P.preconditioner = "cg", self.cg_precond
+            P.krylov, P.preconditioner = "cg", self.cg_precond[kind]
             P.newton_solver_tolerance = self.tol
         else:
             P.krylov, P.preconditioner = "gmres", "ilu0"

@@ -222,7 +222,7 @@ def converged_slice():
     ref = []
     for N, sp, lam in sl:
         op = oracle_problem("bratu", N=N, space=sp, lam=lam)
-        ref.append(on.newton_solve(op, np.zeros(op.ndof), tol=1e-13, max_iter=30))
+        ref.append(on.converged_solve(op, np.zeros(op.ndof), tol=1e-13))
     return idx, sl, ref
 
 

@@ -23,7 +23,7 @@ def _oracle(req):
         op = oracle_problem("rd", N=req.N, f=req.payload[1])
     else:
         op = oracle_problem("ns", N=req.N, Re=ns_reynolds(req.seed))
-    return on.newton_solve(op, np.zeros(op.ndof), tol=1e-12, max_iter=30)
+    return on.converged_solve(op, np.zeros(op.ndof), tol=1e-12)
 
 
 @pytest.mark.parametrize("cg_precond", ["amg", "jacobi"])
@@ -49,3 +49,42 @@ def test_served_results_equal_alone_and_oracle(gpu, cg_precond):
     assert s["requests"] == len(reqs) and s["converged"] == len(reqs)
     assert loop.reserved <= 8e9
     loop.close()
+
+
+@pytest.mark.parametrize("kind,N,precond", [("poisson", 8, "amg"), ("rd", 3, "jacobi"), ("ns", 6, None)])
+def test_a_failed_request_leaves_its_pool_usable(gpu, kind, N, precond):
+    """A request whose data is NaN fails (its Newton residual is not finite) while the
+    request beside it converges; the slot it leaves holds NaN element rows and factors,
+    and the next request in that slot converges with the bits it has alone."""
+    from pyoomph_amd.serving import GpuPools, Request, SlotPool
+    ex = GpuPools(cg_precond=precond)
+    pool = SlotPool(ex, (kind, N), 2, warm=False)
+    good = [Request(i, kind, N, 11 + i, 0.0) for i in range(2)]
+    bad = Request(9, kind, N, 99, 0.0)
+    params, aux = bad.data(pool.mesh)
+    if kind == "rd":
+        aux = aux.copy()
+        aux[len(aux) // 2] = np.nan
+    else:
+        params = params.copy()
+        params[-1] = np.nan
+    bad.payload = (params, aux)
+
+    def run(loads):
+        for s, r in loads:
+            pool.load(s, r)
+        out = {}
+        for _ in range(40):
+            for s in pool.step():
+                out[s] = pool.take(s)
+            if len(out) == len(loads):
+                return out
+        raise AssertionError("pool did not finish")
+
+    out = run([(0, bad), (1, good[0])])
+    assert not out[0][1]["converged"]
+    assert out[1][1]["converged"]
+    assert np.array_equal(out[1][0], ex.solve_alone(good[0])[0])
+    out = run([(0, good[1])])
+    assert out[0][1]["converged"]
+    assert np.array_equal(out[0][0], ex.solve_alone(good[1])[0])

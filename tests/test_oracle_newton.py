@@ -92,3 +92,21 @@ def test_superlu_solve_leaves_inputs_untouched():
     c0, v0 = col.copy(), val.copy()
     on.superlu_solve(op.ndof, rs, col, val, r)
     assert np.array_equal(col, c0) and np.array_equal(val, v0)
+
+
+def test_converged_solve_reaches_roundoff_where_the_absolute_test_stops_early():
+    """A small source (residual scale 1e-6) on the Q2 cube: the absolute test max|R| < 1e-12
+    accepts the first Newton iterate of the cubic problem, whose error is still ~1e-6 of the
+    solution; one more step (converged_solve) lands on the fixed point: a further Newton
+    step moves it by roundoff only."""
+    m = om.cuboid_brick_mesh(4)
+    rng = np.random.default_rng(7)
+    f = 40.0 * rng.uniform(-1.0, 1.0, m.nnode)
+    op = oa.OracleProblem(m, oa.FORM_RD, 1, m.on_boundary()[:, None], np.zeros((m.nnode, 1)), [], aux=f)
+    hist = []
+    x1 = on.newton_solve(op, np.zeros(op.ndof), tol=1e-6, history=hist)
+    xc = on.converged_solve(op, np.zeros(op.ndof), tol=1e-6)
+    x_next = xc - on.superlu_solve(op.ndof, *op.get_jacobian(xc)[1:], op.get_residuals(xc))
+    scale = np.max(np.abs(xc))
+    assert np.max(np.abs(x_next - xc)) <= 1e-14 * scale
+    assert np.max(np.abs(x1 - xc)) > 1e3 * np.max(np.abs(x_next - xc))
