@@ -199,6 +199,9 @@ int pyhip_export_reference_csr(pyhip_problem *p, const double *values_dev,
 #define PYHIP_PRECOND_JACOBI 0
 #define PYHIP_PRECOND_ILU0 1
 #define PYHIP_PRECOND_AMG 2    /* smoothed-aggregation multigrid V-cycle (FP64 solves) */
+#define PYHIP_PRECOND_BLOCK 3  /* Taylor-Hood: block-triangular, multigrid V-cycle on the velocity
+                                  components and the lumped pressure mass / viscosity for the
+                                  Schur complement (FP64 solves; scalar forms: multigrid) */
 /* ILU(0) row order within a sub-problem (Taylor-Hood: velocity unknowns, then pressure).
  * NATURAL keeps the equation order; MULTICOLOR sorts each group by a greedy colouring,
  * so the level-scheduled triangular solves need one launch per colour instead of one per

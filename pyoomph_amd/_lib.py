@@ -22,7 +22,7 @@ ELEM_KIND = {"tri_p1": ELEM_TRI_P1, "tri_p2": ELEM_TRI_P2, "quad_q2": ELEM_QUAD_
              "hex_q2": ELEM_HEX_Q2}
 FORM_POISSON, FORM_BRATU, FORM_REACTION_DIFFUSION, FORM_NAVIER_STOKES_TH = 1, 2, 3, 4
 F64, F32 = 0, 1
-PRECOND_JACOBI, PRECOND_ILU0, PRECOND_AMG = 0, 1, 2
+PRECOND_JACOBI, PRECOND_ILU0, PRECOND_AMG, PRECOND_BLOCK = 0, 1, 2, 3
 ILU_ORDER_AUTO, ILU_ORDER_NATURAL, ILU_ORDER_MULTICOLOR = 0, 1, 2
 KRYLOV_GMRES, KRYLOV_CG = 0, 1
 

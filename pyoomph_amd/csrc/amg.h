@@ -41,6 +41,9 @@ struct AmgDevLevel {
 
 struct AmgDev {
     bool built = false;
+    // non-symmetric operators (a convection-diffusion velocity block): the coarsest solve
+    // uses a Gauss-Jordan inverse instead of the Cholesky factor (set before amg_build)
+    bool nonsym = false;
     int nsub = 0;
     int max_bot_rows = 0;
     std::vector<AmgDevLevel> lv;       // levels with a coarser level

@@ -313,6 +313,50 @@ __global__ __launch_bounds__(kCholThreads) void amg_chol_factor_kernel(const int
     }
 }
 
+// Non-symmetric coarsest matrices: in-place Gauss-Jordan inverse without pivoting (the
+// operators are diagonally dominant velocity blocks), thread t owning row t; a zero or
+// non-finite pivot is replaced by 1 and flagged. Written in the layout the solve below
+// reads: X[j][i] = (A^-1)_{ij}.
+__global__ __launch_bounds__(kCholThreads) void amg_gj_inverse_kernel(const int32_t *__restrict__ subs,
+                                                                      const int64_t *__restrict__ sub_off,
+                                                                      const int32_t *__restrict__ rs,
+                                                                      const int32_t *__restrict__ col,
+                                                                      const double *__restrict__ vals,
+                                                                      double *__restrict__ cinv,
+                                                                      int32_t *__restrict__ fail) {
+    __shared__ double a[kBottomMaxRows][kBottomMaxRows + 1];
+    const int s = subs[blockIdx.x], t = threadIdx.x;
+    const int64_t r0 = sub_off[s];
+    const int m = (int)(sub_off[s + 1] - r0);
+    for (int j = 0; j < kBottomMaxRows; j++) a[t][j] = 0.0;
+    __syncthreads();
+    if (t < m)
+        for (int32_t e = rs[r0 + t]; e < rs[r0 + t + 1]; e++) a[t][col[e] - r0] = vals[e];
+    __syncthreads();
+    for (int k = 0; k < m; k++) {
+        double pv = a[k][k];
+        if (!(fabs(pv) > 0.0) || !isfinite(pv)) {
+            if (t == 0) fail[s] = 1;
+            pv = 1.0;
+        }
+        const double inv = 1.0 / pv;
+        __syncthreads();   // every thread has read the pivot
+        if (t < m && t != k) a[k][t] = a[k][t] * inv;
+        __syncthreads();   // row k scaled
+        if (t < m && t != k) {
+            const double f = a[t][k];
+            for (int j = 0; j < m; j++)
+                if (j != k) a[t][j] = a[t][j] - f * a[k][j];
+            a[t][k] = -f * inv;
+        }
+        if (t == k) a[k][k] = inv;
+        __syncthreads();
+    }
+    double *X = cinv + (int64_t)s * kBottomMaxRows * kBottomMaxRows;
+    if (t < m)
+        for (int j = 0; j < m; j++) X[j * kBottomMaxRows + t] = a[t][j];
+}
+
 // x = X b on the listed sub-problems' coarsest rows (X the symmetric inverse above): thread
 // i sums X[j][i] b[j] over j in increasing order (X[j][i] = X[i][j]; the reads of one j are
 // contiguous over the threads).
@@ -594,8 +638,8 @@ int amg_setup(AmgDev &A, hipStream_t st, const double *vals0, KernelTimer *tm, c
         L.vals = vals;
         if ((rc = level_weights(st, L, vals, A.nrm, A.nsub, tm))) return rc;
         if (L.nends)
-            hipLaunchKernelGGL(amg_chol_factor_kernel, dim3(L.nends), dim3(kCholThreads), 0, st, L.ends, L.sub_off,
-                               L.rs, L.col, vals, A.chol, A.chol_fail);
+            hipLaunchKernelGGL(A.nonsym ? amg_gj_inverse_kernel : amg_chol_factor_kernel, dim3(L.nends),
+                               dim3(kCholThreads), 0, st, L.ends, L.sub_off, L.rs, L.col, vals, A.chol, A.chol_fail);
         if (L.nnzp > 0)
             hipLaunchKernelGGL(amg_pvals_kernel, dim3(ablk(L.nnzp)), dim3(256), 0, st, L.nnzp, L.prow, L.pcol,
                                L.pcptr, L.pcidx, L.agg, vals, L.w, L.tinv, L.pv, L.ptv);
@@ -611,8 +655,9 @@ int amg_setup(AmgDev &A, hipStream_t st, const double *vals0, KernelTimer *tm, c
     }
     A.bot.vals = A.lv.empty() ? vals0 : A.bot.own_vals;
     if (A.bot.nends)
-        hipLaunchKernelGGL(amg_chol_factor_kernel, dim3(A.bot.nends), dim3(kCholThreads), 0, st, A.bot.ends,
-                           A.bot.sub_off, A.bot.rs, A.bot.col, A.bot.vals, A.chol, A.chol_fail);
+        hipLaunchKernelGGL(A.nonsym ? amg_gj_inverse_kernel : amg_chol_factor_kernel, dim3(A.bot.nends),
+                           dim3(kCholThreads), 0, st, A.bot.ends, A.bot.sub_off, A.bot.rs, A.bot.col, A.bot.vals,
+                           A.chol, A.chol_fail);
     ACHECK();
     // A coarsest matrix that is not positive definite (an indefinite Jacobian, e.g. past a
     // fold) makes the V-cycle an invalid preconditioner: report it instead of iterating on.
@@ -623,8 +668,9 @@ int amg_setup(AmgDev &A, hipStream_t st, const double *vals0, KernelTimer *tm, c
     for (int s = 0; s < A.nsub; s++)
         if (fail_h[s] && (!need_h || need_h[s])) {
             set_error("multigrid: the coarsest matrix of sub-problem " + std::to_string(s) +
-                      " is not positive definite (indefinite Jacobian, e.g. past a fold); use the "
-                      "ILU(0) or Jacobi preconditioner there");
+                      (A.nonsym ? std::string(" has a zero pivot") :
+                                  std::string(" is not positive definite (indefinite Jacobian, e.g. past a fold); use "
+                                              "the ILU(0) or Jacobi preconditioner there")));
             return PYHIP_ERR_BREAKDOWN;
         }
     if (!A.lv.empty() && !A.lv[0].lpr_a && (rc = tune_levels(A, st))) return rc;

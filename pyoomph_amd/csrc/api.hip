@@ -1,5 +1,6 @@
 // C ABI: element table, problem set-up, assembly and reference-CSR export.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 
@@ -209,6 +210,8 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
     std::vector<double> coords_h(P->nnode * 3, 0.0), pinned_h(P->nnode * 3, 0.0),
         aux_h(P->nnode, 0.0);
     P->params_h.assign((size_t)nsub * kParamStride, 0.0);
+    P->dof_val_h.assign(P->ndof, 0);
+    P->pmass_h.assign(P->ndof, 0.0);
     for (int s = 0; s < nsub; s++) {
         const auto &d = subs[s];
         const SubInfo &S = P->subs[s];
@@ -220,8 +223,26 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
                 pinned_h[(S.node_off + n) * 3 + v] = d.pinned_values[n * d.nvalue + v];
                 const int32_t q = d.eqn[n * d.nvalue + v];
                 PYHIP_REQUIRE(q >= -1 && q < d.ndof, "equation number out of range");
+                if (q >= 0) P->dof_val_h[S.dof_off + q] = (int8_t)v;
             }
             if (d.aux) aux_h[S.node_off + n] = d.aux[n];
+        }
+        if (d.form == PYHIP_FORM_NAVIER_STOKES_TH) {
+            // lumped Q1 pressure mass: a quarter of each element's area (corners 0, 2, 8, 6
+            // of the 3x3 node block, counter-clockwise) to each corner's pressure unknown
+            static const int corner[4] = {0, 2, 8, 6};
+            for (int64_t e = 0; e < d.nelem; e++) {
+                double area2 = 0.0;
+                for (int c = 0; c < 4; c++) {
+                    const int32_t a = d.conn[e * G.nn + corner[c]], b = d.conn[e * G.nn + corner[(c + 1) % 4]];
+                    area2 += d.coords[a * dim] * d.coords[b * dim + 1] - d.coords[b * dim] * d.coords[a * dim + 1];
+                }
+                const double quarter = 0.125 * std::fabs(area2);
+                for (int c = 0; c < 4; c++) {
+                    const int32_t q = d.eqn[(int64_t)d.conn[e * G.nn + corner[c]] * d.nvalue + 2];
+                    if (q >= 0) P->pmass_h[S.dof_off + q] += quarter;
+                }
+            }
         }
         for (int k = 0; k < d.nparams; k++) P->params_h[(size_t)s * kParamStride + k] = d.params[k];
         if (S.nface > 0) {

@@ -33,6 +33,30 @@ struct IluPlan {
     float *tmp32b = nullptr;
 };
 
+// Block-triangular preconditioner of Taylor-Hood Jacobians [A G; D 0] (block_precond.hip):
+//   z_p = (mu / m_p) r_p per pressure unknown (the lumped Q1 pressure mass scaled by the
+//   viscosity approximating the Schur complement), z_u = V-cycle(A_d; r_u - G z_p), with A_d
+//   the velocity block without the coupling of different velocity components (rows of
+//   scalar forms: their whole matrix), on its own multigrid hierarchy.
+struct BlockPlan {
+    bool built = false;
+    int64_t nv = 0, np = 0, nnzv = 0;
+    int32_t *vrows = nullptr;      // [nv] problem row of each velocity-block row
+    int32_t *vsrc = nullptr;       // [nnzv] index of each A_d entry in the problem's CSR values
+    int32_t *prow = nullptr;       // [np] problem row of each pressure unknown
+    int32_t *psub = nullptr;       // [np] its sub-problem
+    int32_t *vsub = nullptr;       // [nv] sub-problem of each velocity-block row
+    double *pmass = nullptr;       // [np] lumped pressure mass
+    double *sinv = nullptr;        // [np] mu / pmass (set-up)
+    double *vals = nullptr;        // [nnzv] A_d values
+    double *tv = nullptr, *zv = nullptr;   // [nv]
+    double *pz = nullptr, *t = nullptr;    // [n]: z_p in pressure rows (0 elsewhere); G z_p
+    const double *jvals = nullptr;         // the Jacobian values of the last set-up
+    std::vector<int32_t> rs_h, col_h;      // A_d pattern
+    std::vector<int64_t> off_h;            // [nsub+1] A_d rows per sub-problem
+    AmgDev amg;
+};
+
 struct KrylovWork {
     int64_t n = 0;
     int nsub = 0;
@@ -95,6 +119,7 @@ struct KrylovWork {
     IluPlan ilu;
     int ilu_order = 0;                // PYHIP_ILU_ORDER_* of the current solve
     AmgDev amg;
+    BlockPlan blk;
     bool amg_stale = true;            // multigrid values to be rebuilt at the next set-up
     std::vector<int32_t> need_h;      // host copy of the sub-problems a preconditioner set-up serves
     std::vector<uint8_t> step_res_ok; // [nsub] res64 / maxres_h hold the sub-problem's residual at its dofs

@@ -770,6 +770,176 @@ static int ilu_apply(pyhip_problem *p, KrylovWork *W, const T *v, T *z) {
     return PYHIP_OK;
 }
 
+static int need_on_host(pyhip_problem *p, KrylovWork *W, const int32_t *need);
+
+// ---- block-triangular preconditioner of Taylor-Hood Jacobians (BlockPlan in krylov.h) ----
+//
+// J = [A G; D 0] (velocity rows / pressure rows). The upper block-triangular
+// [A G; 0 -S]^-1 with the Schur complement S = -D A^-1 G approximated by the lumped Q1
+// pressure mass over the viscosity (a mesh-independent equivalent of the Stokes Schur
+// complement, here with the sign the Navier-Stokes Jacobian's Schur diagonal has) and A^-1
+// by one multigrid V-cycle on A_d, the velocity block without the coupling of the two
+// velocity components. Sub-problems of scalar forms have no pressure rows: all of their
+// matrix is in A_d, so for them this is the multigrid preconditioner.
+
+__global__ void blk_gather_kernel(int64_t n, const int32_t *__restrict__ src, const double *__restrict__ a,
+                                  double *__restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = a[src[i]];
+}
+
+// sinv = mu / m_p per pressure unknown (mu: the first parameter of its sub-problem)
+__global__ void blk_sinv_kernel(int64_t np, const int32_t *__restrict__ psub, const double *__restrict__ pmass,
+                                const double *__restrict__ params, int pstride, double *__restrict__ sinv) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < np) sinv[i] = params[(int64_t)psub[i] * pstride] / pmass[i];
+}
+
+// z_p = sinv * r_p, into z and into pz (whose velocity rows stay 0)
+__global__ void blk_pressure_kernel(int64_t np, const int32_t *__restrict__ prow, const int32_t *__restrict__ psub,
+                                    const double *__restrict__ sinv, const double *__restrict__ r,
+                                    double *__restrict__ pz, double *__restrict__ z,
+                                    const int32_t *__restrict__ act) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= np || (act && !act[psub[i]])) return;
+    const int32_t row = prow[i];
+    const double v = r[row] * sinv[i];
+    pz[row] = v;
+    z[row] = v;
+}
+
+// tv = r_u - (G z_p)_u (t = J pz; without pressure rows t is not read)
+__global__ void blk_vgather_kernel(int64_t nv, const int32_t *__restrict__ vrows, const int32_t *__restrict__ vsub,
+                                   const double *__restrict__ r, const double *__restrict__ t,
+                                   double *__restrict__ tv, const int32_t *__restrict__ act) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nv || (act && !act[vsub[i]])) return;
+    const int32_t row = vrows[i];
+    tv[i] = t ? r[row] - t[row] : r[row];
+}
+
+__global__ void blk_vscatter_kernel(int64_t nv, const int32_t *__restrict__ vrows, const int32_t *__restrict__ vsub,
+                                    const double *__restrict__ zv, double *__restrict__ z,
+                                    const int32_t *__restrict__ act) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nv || (act && !act[vsub[i]])) return;
+    z[vrows[i]] = zv[i];
+}
+
+// host plan, once per pattern: the velocity / pressure split, A_d's pattern with the source
+// of each entry, and A_d's multigrid plan
+static int blk_plan_build(pyhip_problem *p, KrylovWork *W) {
+    BlockPlan &B = W->blk;
+    if (B.built) return PYHIP_OK;
+    const int64_t n = p->ndof;
+    const int ns = p->nsub;
+    PYHIP_REQUIRE((int64_t)p->dof_val_h.size() == n && (int64_t)p->pmass_h.size() == n,
+                  "block preconditioner: unknown kinds missing");
+    std::vector<int32_t> vmap(n, -1), vrows, vsub, prow, psub, vsrc;
+    std::vector<double> pmass;
+    B.off_h.assign(ns + 1, 0);
+    for (int s = 0; s < ns; s++) {
+        const bool th = p->subs[s].form == PYHIP_FORM_NAVIER_STOKES_TH;
+        for (int64_t r = p->dof_off_h[s]; r < p->dof_off_h[s + 1]; r++) {
+            if (th && p->dof_val_h[r] == 2) {
+                PYHIP_REQUIRE(p->pmass_h[r] > 0.0, "block preconditioner: pressure unknown with no element");
+                prow.push_back((int32_t)r);
+                psub.push_back(s);
+                pmass.push_back(p->pmass_h[r]);
+            } else {
+                vmap[r] = (int32_t)vrows.size();
+                vrows.push_back((int32_t)r);
+                vsub.push_back(s);
+            }
+        }
+        B.off_h[s + 1] = (int64_t)vrows.size();
+    }
+    B.rs_h.assign(1, 0);
+    B.col_h.clear();
+    for (const int32_t r : vrows) {
+        for (int32_t e = p->row_start_h[r]; e < p->row_start_h[r + 1]; e++) {
+            const int32_t c = p->col_h[e];
+            if (vmap[c] >= 0 && p->dof_val_h[c] == p->dof_val_h[r]) {
+                B.col_h.push_back(vmap[c]);
+                vsrc.push_back(e);
+            }
+        }
+        B.rs_h.push_back((int32_t)B.col_h.size());
+    }
+    B.nv = (int64_t)vrows.size();
+    B.np = (int64_t)prow.size();
+    B.nnzv = (int64_t)B.col_h.size();
+    int rc;
+    auto up = [&](int32_t **dst, const std::vector<int32_t> &v) -> int {
+        if ((rc = dev_alloc(dst, std::max<size_t>(1, v.size()), W->bytes))) return rc;
+        if (!v.empty()) PYHIP_CHECK_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        return PYHIP_OK;
+    };
+    if ((rc = up(&B.vrows, vrows)) || (rc = up(&B.vsub, vsub)) || (rc = up(&B.vsrc, vsrc)) ||
+        (rc = up(&B.prow, prow)) || (rc = up(&B.psub, psub)))
+        return rc;
+    if ((rc = dev_alloc(&B.pmass, std::max<int64_t>(1, B.np), W->bytes)) ||
+        (rc = dev_alloc(&B.sinv, std::max<int64_t>(1, B.np), W->bytes)) ||
+        (rc = dev_alloc(&B.vals, std::max<int64_t>(1, B.nnzv), W->bytes)) ||
+        (rc = dev_alloc(&B.tv, std::max<int64_t>(1, B.nv), W->bytes)) ||
+        (rc = dev_alloc(&B.zv, std::max<int64_t>(1, B.nv), W->bytes)) ||
+        (rc = dev_alloc(&B.pz, std::max<int64_t>(1, n), W->bytes)) ||
+        (rc = dev_alloc(&B.t, std::max<int64_t>(1, n), W->bytes)))
+        return rc;
+    if (B.np) PYHIP_CHECK_HIP(hipMemcpy(B.pmass, pmass.data(), B.np * sizeof(double), hipMemcpyHostToDevice));
+    PYHIP_CHECK_HIP(hipMemset(B.pz, 0, std::max<int64_t>(1, n) * sizeof(double)));
+    B.amg.nonsym = true;
+    if ((rc = amg_build(B.amg, B.nv, B.rs_h.data(), B.col_h.data(), B.off_h, W->bytes))) return rc;
+    B.built = true;
+    return PYHIP_OK;
+}
+
+static void blk_free(KrylovWork *W) {
+    BlockPlan &B = W->blk;
+    void *ptrs[] = {B.vrows, B.vsrc, B.prow, B.psub, B.vsub, B.pmass, B.sinv, B.vals, B.tv, B.zv, B.pz, B.t};
+    for (void *q : ptrs) dfree(q);
+    amg_free(B.amg);
+    W->blk = BlockPlan();
+}
+
+// numeric set-up from the FP64 Jacobian values: A_d's values and hierarchy, mu / m_p
+static int blk_setup(pyhip_problem *p, KrylovWork *W, const double *vals, const int32_t *need) {
+    int rc = blk_plan_build(p, W);
+    if (rc) return rc;
+    BlockPlan &B = W->blk;
+    B.jvals = vals;
+    if (B.nnzv)
+        hipLaunchKernelGGL(blk_gather_kernel, dim3(nblk(B.nnzv)), dim3(256), 0, p->st, B.nnzv, B.vsrc, vals, B.vals);
+    if (B.np)
+        hipLaunchKernelGGL(blk_sinv_kernel, dim3(nblk(B.np)), dim3(256), 0, p->st, B.np, B.psub, B.pmass, p->params,
+                           kParamStride, B.sinv);
+    KCHECK();
+    if ((rc = need_on_host(p, W, need))) return rc;
+    return amg_setup(B.amg, p->st, B.vals, &p->timer, need ? W->need_h.data() : nullptr);
+}
+
+static int blk_apply(pyhip_problem *p, KrylovWork *W, const double *r, double *z, const int32_t *act,
+                     const int32_t *act_h) {
+    BlockPlan &B = W->blk;
+    int rc;
+    if (B.np) {
+        hipLaunchKernelGGL(blk_pressure_kernel, dim3(nblk(B.np)), dim3(256), 0, p->st, B.np, B.prow, B.psub, B.sinv,
+                           r, B.pz, z, act);
+        KCHECK();
+        if ((rc = spmv<double, double, double>(p, B.jvals, B.pz, B.t, (const double *)nullptr, act, act_h))) return rc;
+    }
+    if (B.nv) {
+        hipLaunchKernelGGL(blk_vgather_kernel, dim3(nblk(B.nv)), dim3(256), 0, p->st, B.nv, B.vrows, B.vsub, r,
+                           B.np ? (const double *)B.t : nullptr, B.tv, act);
+        KCHECK();
+        if ((rc = amg_apply(B.amg, p->st, B.tv, B.zv, &p->timer, act, act_h))) return rc;
+        hipLaunchKernelGGL(blk_vscatter_kernel, dim3(nblk(B.nv)), dim3(256), 0, p->st, B.nv, B.vrows, B.vsub, B.zv, z,
+                           act);
+        KCHECK();
+    }
+    return PYHIP_OK;
+}
+
 template <typename T>
 // z = M^-1 v. The multigrid V-cycle computes only the rows of sub-problems with
 // act[s] != 0 when act (device, host copy act_h) is given; the other preconditioners
@@ -778,6 +948,11 @@ static int precond_apply(pyhip_problem *p, KrylovWork *W, int pc, const T *v, T 
                          const int32_t *act_h = nullptr, bool x0_ready = false) {
     TimeScope ts(&p->timer, "precond", p->st);
     if (pc == PYHIP_PRECOND_ILU0) return ilu_apply<T>(p, W, v, z);
+    if (pc == PYHIP_PRECOND_BLOCK) {
+        if constexpr (std::is_same<T, double>::value) return blk_apply(p, W, v, z, act, act_h);
+        set_error("the block preconditioner runs FP64 solves");
+        return PYHIP_ERR_INVALID;
+    }
     if (pc == PYHIP_PRECOND_AMG) {
         if constexpr (std::is_same<T, double>::value)
             return amg_apply(W->amg, p->st, v, z, &p->timer, act, act_h, x0_ready);
@@ -802,6 +977,12 @@ static int need_on_host(pyhip_problem *p, KrylovWork *W, const int32_t *need) {
 template <typename TV, typename T>
 static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals, const int32_t *need = nullptr) {
     TimeScope ts(&p->timer, "precond", p->st);
+    if (pc == PYHIP_PRECOND_BLOCK) {
+        if constexpr (std::is_same<T, double>::value && std::is_same<TV, double>::value)
+            return blk_setup(p, W, vals, need);
+        set_error("the block preconditioner runs FP64 solves");
+        return PYHIP_ERR_INVALID;
+    }
     if (pc == PYHIP_PRECOND_AMG) {
         if constexpr (std::is_same<T, double>::value && std::is_same<TV, double>::value) {
             int rc = amg_build(W->amg, p->ndof, p->row_start_h.data(), p->col_h.data(), p->dof_off_h, W->bytes);
@@ -1810,6 +1991,7 @@ void pyhip_krylov_free(pyhip_problem *p) {
     if (W->act_h) (void)hipHostFree(W->act_h);
     if (W->maxres_h) (void)hipHostFree(W->maxres_h);
     amg_free(W->amg);
+    blk_free(W);
     if (W->achunk_h) (void)hipHostFree(W->achunk_h);
     if (W->flags_h) (void)hipHostFree(W->flags_h);
     delete W;

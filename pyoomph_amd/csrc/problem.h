@@ -166,6 +166,8 @@ struct pyhip_problem {
     int32_t *dof_sub = nullptr;             // [ndof] sub-problem of each row
     int64_t *dof_off = nullptr;             // [nsub+1]
     std::vector<int32_t> row_start_h, col_h;
+    std::vector<int8_t> dof_val_h;           // [ndof] nodal value index of each unknown (Taylor-Hood: 0 ux, 1 uy, 2 p)
+    std::vector<double> pmass_h;             // [ndof] lumped Q1 pressure mass of Taylor-Hood pressure unknowns
     // element buffers (lazy)
     double *Er64 = nullptr, *Ej64 = nullptr;
     float *Er32 = nullptr, *Ej32 = nullptr;

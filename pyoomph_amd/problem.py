@@ -306,7 +306,7 @@ class Problem:
         o.jacobian_precision = _lib.F64 if self.jacobian_precision == "f64" else _lib.F32
         o.linear.precision = _lib.F64 if self.linear_precision == "f64" else _lib.F32
         o.linear.precond = {"ilu0": _lib.PRECOND_ILU0, "jacobi": _lib.PRECOND_JACOBI,
-                            "amg": _lib.PRECOND_AMG}[self.preconditioner]
+                            "amg": _lib.PRECOND_AMG, "block": _lib.PRECOND_BLOCK}[self.preconditioner]
         kry = self.krylov
         if kry == "auto":
             kry = "cg" if all(getattr(mp.equations, "symmetric", False) for mp in self.parts) else "gmres"

@@ -5,7 +5,7 @@ pyoomph has no request loop; its closest analogue is the process farm of paramet
 Here requests share one GPU, batched continuously at the granularity of a Newton step:
 
 * Mesh classes. A request names a problem kind (2D P2 Poisson, 3D Q2 reaction-diffusion,
-  2D Taylor-Hood Navier-Stokes), one of the kind's mesh sizes (``SIZES``: 5k to 500k
+  2D Taylor-Hood Navier-Stokes), one of the kind's mesh sizes (``SIZES``: 5k to 490k
   unknowns) and its own data: the Poisson source's eight Gaussians, the
   reaction-diffusion source field, the Navier-Stokes Reynolds number, all drawn from the
   request's seed. ``poisson_stream`` draws a seeded Poisson-arrival stream of them.
@@ -23,8 +23,8 @@ Here requests share one GPU, batched continuously at the granularity of a Newton
   a time, each on its own HIP stream, so different pools' work overlaps on the device;
   pools wanting to step are served in the order they asked.
 * Each slot's Newton and Krylov iteration is its own (per-sub-problem element rows,
-  scalars, reductions and convergence tests; Jacobi and ILU(0) per sub-problem block; the
-  multigrid hierarchy rebuilt from every step's Jacobian), so a request gives the same
+  scalars, reductions and convergence tests; Jacobi per row; the multigrid hierarchies,
+  per sub-problem, rebuilt from every step's Jacobian), so a request gives the same
   bits alone and in the mix (``tests/test_gpu_serving.py``).
 
 The executor is pluggable: ``GpuPools`` runs pools with the HIP library; the scheduler is
@@ -60,13 +60,13 @@ def _ndof_ns(N):
 _NDOF = {"poisson": _ndof_poisson, "rd": _ndof_rd, "ns": _ndof_ns}
 
 # the mesh sizes served per kind, about log-uniform in unknowns: Poisson 5k..450k, reaction-
-# diffusion 7k..490k, Taylor-Hood 6k..11k (its GMRES / ILU(0) solves grow fastest)
+# diffusion 7k..490k, Taylor-Hood 6k..360k
 SIZES = {"poisson": (36, 56, 88, 136, 212, 336),
          "rd": (10, 14, 20, 28, 40),
-         "ns": (26, 30, 36)}
+         "ns": (26, 44, 76, 128, 200)}
 
 
-def slots_for(kind, N, target_dofs=2_000_000, lo=2, hi=16):
+def slots_for(kind, N, target_dofs=4_000_000, lo=2, hi=16):
     """Slots of a pool: about ``target_dofs`` unknowns per pool, within [lo, hi]."""
     return int(min(hi, max(lo, target_dofs // _NDOF[kind](N))))
 
@@ -395,12 +395,14 @@ DEFAULT_CG_PRECOND = {"poisson": "amg", "rd": "jacobi"}
 
 class GpuPools:
     """Slot pools on the GPU, FP64: CG for the symmetric kinds (``cg_precond``: "jacobi",
-    "amg", or a dict per kind), GMRES(``gmres_restart``) + ILU(0) for Taylor-Hood. Inexact
+    "amg", or a dict per kind), GMRES(``gmres_restart``) with the block-triangular
+    preconditioner (multigrid on the velocity components, lumped pressure mass for the
+    Schur complement) for Taylor-Hood. Inexact
     Newton: each linear solve to ``krylov_rtol``, Newton done once max|R| is below
     ``tol`` (``ns_tol`` for Taylor-Hood) and the last update below ``correction_tol`` of
     the solution."""
 
-    def __init__(self, device="cuda:0", cg_precond=None, gmres_restart=200, krylov_rtol=1e-4,
+    def __init__(self, device="cuda:0", cg_precond=None, gmres_restart=100, krylov_rtol=1e-4,
                  correction_tol=1e-7, tol=1e-8, ns_tol=1e-10, max_newton=30):
         import torch
         self.torch = torch
@@ -420,7 +422,7 @@ class GpuPools:
             P.krylov, P.preconditioner = "cg", self.cg_precond[kind]
             P.newton_solver_tolerance = self.tol
         else:
-            P.krylov, P.preconditioner = "gmres", "ilu0"
+            P.krylov, P.preconditioner = "gmres", "block"
             P.gmres_restart = self.gmres_restart
             P.newton_solver_tolerance = self.ns_tol
         P.gmres_max_iters = 20000

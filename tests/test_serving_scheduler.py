@@ -83,7 +83,7 @@ def test_stream_is_seeded_and_sizes_are_the_served_ones():
     for k, sizes in SIZES.items():
         nd = [_NDOF[k](N) for N in sizes]
         assert nd == sorted(nd) and nd[0] >= 5_000 and nd[-1] <= 600_000
-    assert slots_for("poisson", 36) == 16 and slots_for("rd", 40) == 4
+    assert slots_for("poisson", 36) == 16 and slots_for("rd", 40) == 8
 
 
 def test_every_request_served_once_by_its_pool_in_arrival_order():
