@@ -7,7 +7,7 @@ Workload (headline): the batched sweep of independent 2D Bratu problems
 A step is one full steady Newton solve of every problem from u = 0: pyoomph's Newton
 (max |R| < 1e-8, always one step) with, in addition, an FP64 update below 1e-6 of the
 solution (on fine meshes a small residual alone does not make a small error), inexact
-linear solves (batched preconditioned CG to 1e-3 relative residual per Newton step:
+linear solves (batched preconditioned CG to 3e-3 relative residual per Newton step:
 smoothed-aggregation multigrid V-cycle by default, Jacobi with --precond jacobi), each
 problem with its own scalars and convergence tests. FP64 (the value), and mixed precision
 beside it: FP32 element kernels for the Jacobian, one FP32 CG solve with the FP32 V-cycle
@@ -15,7 +15,8 @@ per Newton step, FP64 residual and norms (the Newton loop is the FP64 refinement
 
 Reported: solves/s over the timed steps (value), p50/p99 of the per-problem completion
 time inside a step, the max relative error against the converged FP64 CPU oracle
-(SuperLU Newton to max|R| < 1e-13) on problems across the size range, FP64 assembly GB/s
+(SuperLU Newton to max|R| < 1e-13 and one step more) on problems across the size range, FP64
+assembly GB/s
 against the 8 TB/s HBM peak, the roofline of the dominant kernel (HIP events on the
 problem's stream around every launch), the single-problem GPU paths on a sample (plain
 FP64 CG + Jacobi, and the batch's own solver), and the reference's CPU path as a per-core
@@ -56,6 +57,8 @@ def parse():
                          "residuals, updates and norms)")
     ap.add_argument("--precond", choices=("jacobi", "amg", "ilu0"), default="amg",
                     help="CG preconditioner: Jacobi, or the smoothed-aggregation multigrid V-cycle")
+    ap.add_argument("--krylov-rtol", type=float, default=None,
+                    help="relative tolerance of each linear solve of the sweep (default 3e-3)")
     ap.add_argument("--correction-tol", type=float, default=None,
                     help="Newton also stops only once max|dx| <= this * max|x| (default: per precision)")
     ap.add_argument("--err-sample", type=int, default=6,
@@ -157,7 +160,7 @@ def wl_ndof(spec):
 # CORRECTION_TOL of the solution. Inexact Newton: each linear solve stops at KRYLOV_RTOL
 # relative residual (FP64 CG, or one FP32 CG solve whose FP64 refinement is the Newton loop).
 CORRECTION_TOL = 1e-6
-KRYLOV_RTOL = 1e-3
+KRYLOV_RTOL = 3e-3
 # accuracy reference: the oracle's Newton run to max|R| < 1e-13 and one step more (converged)
 ORACLE_TOL = 1e-13
 
@@ -330,12 +333,21 @@ def full_main(args):
         el64, st64, k64 = run(P)
         x64 = P.get_current_dofs()
         diff = float(np.max(np.abs(x_mixed - x64)) / max(np.max(np.abs(x64)), 1e-300))
+        # the same mixed precision with CG + Jacobi as the FP32 inner solver
+        P.krylov, P.preconditioner = "cg", "jacobi"
+        P.jacobian_precision, P.linear_precision = "f32", "f32"
+        P.newton_solver_tolerance = 1e-15
+        el_cg, st_cg, _ = run(P)
+        x_cg = P.get_current_dofs()
+        diff_cg = float(np.max(np.abs(x_cg - x64)) / max(np.max(np.abs(x64)), 1e-300))
         out = {"workload": "3D reaction-diffusion -lap u + u^3 = f, 64^3 Q2 bricks, f = smoothed U(-1,1) "
                            "noise, Newton from 0 to max|R| < 1e-15", "ndof": P.ndof, "nnz": P.nnz, "setup_s": setup,
                "solve_s": el, "solver": "FP32 element kernels + FP32 ILU(0)-GMRES(50) inside FP64 iterative "
                                         "refinement, FP64 residual", "stats": st, "kernel_ms": k,
                "fp64_cg_jacobi_solve_s": el64, "fp64_stats": st64,
-               "max_rel_diff_vs_fp64_solve": diff}
+               "max_rel_diff_vs_fp64_solve": diff,
+               "mixed_cg_jacobi_solve_s": el_cg, "mixed_cg_jacobi_stats": st_cg,
+               "mixed_cg_jacobi_max_rel_diff_vs_fp64_solve": diff_cg}
     print(json.dumps(out), flush=True)
 
 
@@ -432,7 +444,10 @@ def continuation_main(args):
 
 
 def main():
+    global KRYLOV_RTOL
     args = parse()
+    if args.krylov_rtol is not None:
+        KRYLOV_RTOL = args.krylov_rtol
     if args.continuation:
         continuation_main(args)
         return

@@ -36,6 +36,13 @@ int pyhip_codegen_compile(const char *name, const char *source, int nheaders, co
                           const char *const *header_names, int elem_kind, int symmetric, int has_param_derivative,
                           int load, int *form_id, size_t *code_size);
 
+/* Names of a generated form's nodal fields, as its JIT element table lists them
+ * (pyhip_jit_element_init): the unknown `field` and, for forms that read the auxiliary
+ * nodal field, `aux_field` (NULL: none); `global_param` is the index of the form parameter
+ * its dR/dp slot differentiates by (the table's global parameter 0). Defaults: "u", none,
+ * 0. */
+int pyhip_codegen_set_fields(int form_id, const char *field, const char *aux_field, int global_param);
+
 /* Registered generated forms (0 .. count-1 are ids PYHIP_GENERATED_FORM_BASE + k). */
 int pyhip_codegen_count(int *count);
 

@@ -279,13 +279,18 @@ typedef struct JITFuncSpec_Table_FiniteElement {
 typedef void (*JIT_ELEMENT_init_SPEC)(JITFuncSpec_Table_FiniteElement_t *functable);
 
 /* Fills a zeroed table (the host's `new` + zero, src/problem.cpp:110-111) for weak form
- * `form` (PYHIP_FORM_*) on element family `elem_kind` (PYHIP_ELEM_*), like a generated
- * JIT_ELEMENT_init. Fields: nodal_dim; the velocity / pressure or scalar field counts and
- * names in the space of the family (Taylor-Hood: "velocity_x", "velocity_y" in C2,
- * "pressure" in C1; scalar forms: "u"); one residual "" with its ResidualAndJacobian,
- * ResidualAndJacobianSteady and ResidualAndJacobian_NoHang slots; Bratu: one global
- * parameter "lambda" with ParameterDerivative[0][0]; the required shapes; integration
- * order 0 (the family's default scheme); clean_up, which frees what the init allocated.
+ * `form` (PYHIP_FORM_*, or a run-time generated form id from pyhip_codegen_compile, on the
+ * family it was generated for) on element family `elem_kind` (PYHIP_ELEM_*), like a
+ * generated JIT_ELEMENT_init. Fields: nodal_dim; the velocity / pressure or scalar field
+ * counts and names in the space of the family (Taylor-Hood: "velocity_x", "velocity_y" in
+ * C2, "pressure" in C1; scalar forms: "u", reaction-diffusion and generated forms with an
+ * auxiliary field add it second; generated forms: the names pyhip_codegen_set_fields
+ * gave); one residual "" with its ResidualAndJacobian, ResidualAndJacobianSteady and
+ * ResidualAndJacobian_NoHang slots; Bratu and generated forms with dR/dp: one global
+ * parameter with ParameterDerivative[0][0]; the required shapes; integration order 0 (the
+ * family's default scheme); clean_up, which frees what the init allocated. The slots of a
+ * generated form run its hiprtc-compiled kernel on the element (at most 16 generated forms
+ * have tables at a time).
  * The host points global_parameters[i] at its parameter values before the slots run.
  * Returns PYHIP_OK or PYHIP_ERR_INVALID (unknown form / family). */
 int pyhip_jit_element_init(int form, int elem_kind, JITFuncSpec_Table_FiniteElement_t *functable);

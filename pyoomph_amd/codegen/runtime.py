@@ -53,6 +53,9 @@ def compile_form(form, kind, load=True):
                                        ctypes.byref(size)))
     if not load:
         return size.value
+    # the names its JIT element table lists: the unknown "u", the auxiliary field "f"
+    _lib.check(L.pyhip_codegen_set_fields(fid.value, b"u", b"f" if form.aux else None,
+                                          int(form.global_param or 0)))
     _registry[key] = fid.value
     return fid.value
 

@@ -190,11 +190,8 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
     __shared__ T sDx[EPB][QC][NN][DIM];
     __shared__ T sQ[EPB][QC][NQS];
     const int le = threadIdx.x / NS, row = threadIdx.x % NS;
-    const int e = blockIdx.x * EPB + le;
-    const bool active = (le < EPB) && (e < a.nel);
-    // a block none of whose elements belongs to a sub-problem being assembled exits
-    // (the same decision for every thread, before any barrier)
-    if (a.sub_on && !__syncthreads_or(active && a.sub_on[a.sub[e]])) return;
+    const int e = a.e0 + (int)blockIdx.x * EPB + le;
+    const bool active = (le < EPB) && (e < a.e1);
     if (active) {{
         const int node = a.conn[(int64_t)row * a.nel + e];
         const int eq = a.seqn[(int64_t)row * a.nel + e];
@@ -211,8 +208,8 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
     }}
     __syncthreads();
     const int la = threadIdx.x / QC, qa = threadIdx.x % QC;
-    const int ea = blockIdx.x * EPB + la;
-    const bool active_a = (la < EPB) && (ea < a.nel);
+    const int ea = a.e0 + (int)blockIdx.x * EPB + la;
+    const bool active_a = (la < EPB) && (ea < a.e1);
     const double *par = nullptr;
     if (active_a) par = a.params + (int64_t)a.sub[ea] * a.pstride;
     const bool myrow = active && sEq[le][row] >= 0;

@@ -459,58 +459,85 @@ static int check_assemble_args(pyhip_problem *p, const double *dofs, int flag, i
     return PYHIP_OK;
 }
 
+// Element ranges [e0, e1) of group g to compute: all of it, or (sub_on_h, host [nsub]) the
+// elements of the sub-problems with sub_on_h[s] != 0, adjacent ones merged
+static std::vector<std::pair<int64_t, int64_t>> element_runs(const pyhip_problem *p, int g,
+                                                             const int32_t *sub_on_h) {
+    std::vector<std::pair<int64_t, int64_t>> runs;
+    if (!sub_on_h) {
+        runs.push_back({0, p->groups[g].nel});
+        return runs;
+    }
+    for (int s = 0; s < p->nsub; s++) {
+        if (!sub_on_h[s]) continue;
+        const SubInfo &S = p->subs[s];
+        int64_t b = -1, n = 0;
+        if (S.group == g) b = S.el_off, n = S.nel;
+        else if (S.fgroup == g) b = S.fel_off, n = S.nface;
+        if (b < 0 || n == 0) continue;
+        if (!runs.empty() && runs.back().second == b) runs.back().second = b + n;
+        else runs.push_back({b, b + n});
+    }
+    return runs;
+}
+
 // Element kernels into the element buffers Er / Ej (FP64 or FP32 by precision; Ej may be
 // null when flag == 0), then the deterministic gather into residual / values.
 static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int deriv_param, int precision,
-                         double *residual, void *values, void *Er, void *Ej, const int32_t *sub_on = nullptr) {
+                         double *residual, void *values, void *Er, void *Ej, const int32_t *sub_on_h = nullptr) {
     int rc;
-    for (auto &G : p->groups) {
-        if (precision == PYHIP_F64) {
-            ElemArgs<double> a{};
-            a.nel = (int)G.nel;
-            a.conn = G.conn; a.seqn = G.seqn; a.sub = G.sub;
-            a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
-            a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
-            a.tab = G.form != kFormFlux ? p->tab64[G.kind] : (G.kind == kKindQuad9 ? p->tab64[ELEM_QUAD_Q2] : nullptr);
-            a.Er = (double *)Er + G.er_off;
-            a.Ej = flag ? (double *)Ej + G.ej_off : nullptr;
-            a.flag = flag; a.deriv = deriv_param >= 0;
-            a.transient = p->transient ? 1 : 0;
-            a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
-            a.hist1 = p->hist1; a.hist2 = p->hist2;
-            a.hvec = p->hvec;
-            a.fcoef = G.coef;
-            a.sub_on = sub_on;
-            TimeScope ts(&p->timer, "element", p->st);
-            if (G.form == kFormFlux) {
-                if ((rc = launch_faces_f64(G.kind, a, p->st))) return rc;
-            } else if (G.form >= PYHIP_GENERATED_FORM_BASE) {
-                if ((rc = launch_generated_f64(G.form, a, p->st))) return rc;
-            } else if ((rc = launch_elements_f64(G.kind, G.form, a, p->st)))
-                return rc;
-        } else {
-            ElemArgs<float> a{};
-            a.nel = (int)G.nel;
-            a.conn = G.conn; a.seqn = G.seqn; a.sub = G.sub;
-            a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
-            a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
-            a.tab = G.form != kFormFlux ? p->tab32[G.kind] : (G.kind == kKindQuad9 ? p->tab32[ELEM_QUAD_Q2] : nullptr);
-            a.Er = (float *)Er + G.er_off;
-            a.Ej = flag ? (float *)Ej + G.ej_off : nullptr;
-            a.flag = flag; a.deriv = deriv_param >= 0;
-            a.transient = p->transient ? 1 : 0;
-            a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
-            a.hist1 = p->hist1; a.hist2 = p->hist2;
-            a.hvec = p->hvec;
-            a.fcoef = G.coef;
-            a.sub_on = sub_on;
-            TimeScope ts(&p->timer, "element", p->st);
-            if (G.form == kFormFlux) {
-                if ((rc = launch_faces_f32(G.kind, a, p->st))) return rc;
-            } else if (G.form >= PYHIP_GENERATED_FORM_BASE) {
-                if ((rc = launch_generated_f32(G.form, a, p->st))) return rc;
-            } else if ((rc = launch_elements_f32(G.kind, G.form, a, p->st)))
-                return rc;
+    for (size_t g = 0; g < p->groups.size(); g++) {
+        const Group &G = p->groups[g];
+        for (const auto &run : element_runs(p, (int)g, sub_on_h)) {
+            if (precision == PYHIP_F64) {
+                ElemArgs<double> a{};
+                a.nel = (int)G.nel;
+                a.e0 = (int)run.first;
+                a.e1 = (int)run.second;
+                a.conn = G.conn; a.seqn = G.seqn; a.sub = G.sub;
+                a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
+                a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
+                a.tab = G.form != kFormFlux ? p->tab64[G.kind] : (G.kind == kKindQuad9 ? p->tab64[ELEM_QUAD_Q2] : nullptr);
+                a.Er = (double *)Er + G.er_off;
+                a.Ej = flag ? (double *)Ej + G.ej_off : nullptr;
+                a.flag = flag; a.deriv = deriv_param >= 0;
+                a.transient = p->transient ? 1 : 0;
+                a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
+                a.hist1 = p->hist1; a.hist2 = p->hist2;
+                a.hvec = p->hvec;
+                a.fcoef = G.coef;
+                TimeScope ts(&p->timer, "element", p->st);
+                if (G.form == kFormFlux) {
+                    if ((rc = launch_faces_f64(G.kind, a, p->st))) return rc;
+                } else if (G.form >= PYHIP_GENERATED_FORM_BASE) {
+                    if ((rc = launch_generated_f64(G.form, a, p->st))) return rc;
+                } else if ((rc = launch_elements_f64(G.kind, G.form, a, p->st)))
+                    return rc;
+            } else {
+                ElemArgs<float> a{};
+                a.nel = (int)G.nel;
+                a.e0 = (int)run.first;
+                a.e1 = (int)run.second;
+                a.conn = G.conn; a.seqn = G.seqn; a.sub = G.sub;
+                a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
+                a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
+                a.tab = G.form != kFormFlux ? p->tab32[G.kind] : (G.kind == kKindQuad9 ? p->tab32[ELEM_QUAD_Q2] : nullptr);
+                a.Er = (float *)Er + G.er_off;
+                a.Ej = flag ? (float *)Ej + G.ej_off : nullptr;
+                a.flag = flag; a.deriv = deriv_param >= 0;
+                a.transient = p->transient ? 1 : 0;
+                a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
+                a.hist1 = p->hist1; a.hist2 = p->hist2;
+                a.hvec = p->hvec;
+                a.fcoef = G.coef;
+                TimeScope ts(&p->timer, "element", p->st);
+                if (G.form == kFormFlux) {
+                    if ((rc = launch_faces_f32(G.kind, a, p->st))) return rc;
+                } else if (G.form >= PYHIP_GENERATED_FORM_BASE) {
+                    if ((rc = launch_generated_f32(G.form, a, p->st))) return rc;
+                } else if ((rc = launch_elements_f32(G.kind, G.form, a, p->st)))
+                    return rc;
+            }
         }
     }
     const GatherMap m = p->gmap();
@@ -580,11 +607,12 @@ int pyhip_assemble(pyhip_problem *p, const double *dofs, int flag, int deriv_par
 }  // extern "C"
 
 namespace pyhip {
-// pyhip_assemble over the sub-problems with sub_on[s] != 0 (device array [nsub], or null for
-// all): the element kernels skip the blocks of the others, whose residual and CSR rows are
-// then unspecified (computed from element rows left by earlier calls).
+// pyhip_assemble over the sub-problems with sub_on_h[s] != 0 (host array [nsub], or null for
+// all): the element kernels run over those sub-problems' elements only; the residual and
+// CSR rows of the others are then unspecified (gathered from element rows left by earlier
+// calls).
 int assemble_subset(pyhip_problem *p, const double *dofs, int flag, int deriv_param, int precision,
-                    double *residual, void *values, const int32_t *sub_on) {
+                    double *residual, void *values, const int32_t *sub_on_h) {
     int rc = check_assemble_args(p, dofs, flag, deriv_param, precision, residual, values);
     if (rc) return rc;
     void *Er, *Ej;
@@ -607,9 +635,9 @@ int assemble_subset(pyhip_problem *p, const double *dofs, int flag, int deriv_pa
         Er = p->Er32;
         Ej = p->Ej32;
     }
-    if ((rc = assemble_into(p, dofs, flag, deriv_param, precision, residual, values, Er, Ej, sub_on))) return rc;
+    if ((rc = assemble_into(p, dofs, flag, deriv_param, precision, residual, values, Er, Ej, sub_on_h))) return rc;
     // the export reads the element matrices of the last complete FP64 Jacobian
-    if (precision == PYHIP_F64 && flag) p->have_ej64 = (deriv_param < 0) && !sub_on;
+    if (precision == PYHIP_F64 && flag) p->have_ej64 = (deriv_param < 0) && !sub_on_h;
     return PYHIP_OK;
 }
 }  // namespace pyhip

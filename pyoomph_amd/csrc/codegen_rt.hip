@@ -17,6 +17,8 @@ struct GenForm {
     std::string name;
     int kind = 0;
     bool symmetric = false, has_pderiv = false;
+    std::string field = "u", aux_field;   // nodal field names (JIT table)
+    int global_param = 0;                 // the parameter dR/dp is taken in
     hipModule_t mod = nullptr;
     hipFunction_t f64 = nullptr, f64_tr = nullptr, f32 = nullptr, f32_tr = nullptr;
 };
@@ -92,6 +94,16 @@ bool gen_form_has_pderiv(int form) {
     return k >= 0 && k < (int)g_forms.size() && g_forms[k].has_pderiv;
 }
 
+bool gen_form_fields(int form, std::string &field, std::string &aux_field, int &global_param) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    const int k = form - PYHIP_GENERATED_FORM_BASE;
+    if (k < 0 || k >= (int)g_forms.size()) return false;
+    field = g_forms[k].field;
+    aux_field = g_forms[k].aux_field;
+    global_param = g_forms[k].global_param;
+    return true;
+}
+
 template <typename T>
 static int launch_gen(int form, const ElemArgs<T> &a, hipStream_t st) {
     hipFunction_t f;
@@ -105,8 +117,8 @@ static int launch_gen(int form, const ElemArgs<T> &a, hipStream_t st) {
     }
     PYHIP_REQUIRE(a.hvec == nullptr, "Hessian-vector products are not generated for run-time forms");
     const int epb = gen_epb(kind);
-    const unsigned blocks = (unsigned)((a.nel + epb - 1) / epb);
-    if (blocks == 0) return PYHIP_OK;
+    const unsigned blocks = (unsigned)((a.e1 - a.e0 + epb - 1) / epb);
+    if (a.e1 <= a.e0) return PYHIP_OK;
     ElemArgs<T> arg = a;
     void *params[] = {&arg};
     PYHIP_CHECK_HIP(hipModuleLaunchKernel(f, blocks, 1, 1, 256, 1, 1, 0, st, params, nullptr));
@@ -148,6 +160,18 @@ int pyhip_codegen_compile(const char *name, const char *source, int nheaders, co
     std::lock_guard<std::mutex> lock(g_mu);
     g_forms.push_back(G);
     *form_id = PYHIP_GENERATED_FORM_BASE + (int)g_forms.size() - 1;
+    return PYHIP_OK;
+}
+
+int pyhip_codegen_set_fields(int form_id, const char *field, const char *aux_field, int global_param) {
+    PYHIP_REQUIRE(field && field[0], "null field name");
+    PYHIP_REQUIRE(global_param >= 0 && global_param < kParamStride, "parameter index out of range");
+    std::lock_guard<std::mutex> lock(g_mu);
+    const int k = form_id - PYHIP_GENERATED_FORM_BASE;
+    PYHIP_REQUIRE(k >= 0 && k < (int)g_forms.size(), "unknown generated form");
+    g_forms[k].field = field;
+    g_forms[k].aux_field = aux_field ? aux_field : "";
+    g_forms[k].global_param = global_param;
     return PYHIP_OK;
 }
 

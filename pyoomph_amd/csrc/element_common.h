@@ -58,9 +58,10 @@ struct ElemArgs {
     const double *hvec;
     // boundary faces: [nel][2] alpha, beta of each face's flux alpha*u + beta
     const double *fcoef;
-    // [nsub] or nullptr: elements of sub-problems with sub_on[s] == 0 may be skipped (their
-    // element rows keep what they held)
-    const int32_t *sub_on;
+    // elements [e0, e1) of the group are computed by this launch (the slot-major arrays
+    // keep the stride nel): the assembly of a subset of the sub-problems launches once per
+    // contiguous run of their elements
+    int e0, e1;
 };
 
 template <int KIND> struct ElemTraits;

@@ -71,11 +71,8 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
 
     const int le = threadIdx.x / NS;        // element within block
     const int row = threadIdx.x % NS;       // slot row of this thread
-    const int e = blockIdx.x * EPB + le;
-    const bool active = (le < EPB) && (e < a.nel);
-    // a block none of whose elements belongs to a sub-problem being assembled exits
-    // (the same decision for every thread, before any barrier)
-    if (a.sub_on && !__syncthreads_or(active && a.sub_on[a.sub[e]])) return;
+    const int e = a.e0 + (int)blockIdx.x * EPB + le;
+    const bool active = (le < EPB) && (e < a.e1);
 
     if (active) {
         const int node = a.conn[(int64_t)row * a.nel + e];
@@ -96,8 +93,8 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
 
     // phase-A thread: point qa of the current chunk of element la
     const int la = threadIdx.x / QC, qa = threadIdx.x % QC;
-    const int ea = blockIdx.x * EPB + la;
-    const bool active_a = (la < EPB) && (ea < a.nel);
+    const int ea = a.e0 + (int)blockIdx.x * EPB + la;
+    const bool active_a = (la < EPB) && (ea < a.e1);
     T par0 = T(0);
     const double *par = nullptr;
     if (active_a) {
@@ -258,11 +255,8 @@ __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
 
     const int le = threadIdx.x / NS;
     const int slot = threadIdx.x % NS;
-    const int e = blockIdx.x * EPB + le;
-    const bool active = (le < EPB) && (e < a.nel);
-    // a block none of whose elements belongs to a sub-problem being assembled exits
-    // (the same decision for every thread, before any barrier)
-    if (a.sub_on && !__syncthreads_or(active && a.sub_on[a.sub[e]])) return;
+    const int e = a.e0 + (int)blockIdx.x * EPB + le;
+    const bool active = (le < EPB) && (e < a.e1);
     if (active) {
         const int ln = kTHSlotNode[slot];
         const int kind = kTHSlotKind[slot];

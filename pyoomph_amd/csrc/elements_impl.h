@@ -7,7 +7,7 @@ namespace pyhip {
 template <typename T, int KIND, int FORM>
 static int launch_scalar(const ElemArgs<T> &a, hipStream_t s) {
     constexpr int EPB = scalar_epb<KIND>();
-    const int blocks = (a.nel + EPB - 1) / EPB;
+    const int blocks = (a.e1 - a.e0 + EPB - 1) / EPB;
     if (blocks == 0) return PYHIP_OK;
     if (a.hvec)
         hipLaunchKernelGGL((scalar_element_kernel<T, KIND, FORM, false, true>), dim3(blocks), dim3(256), 0, s, a);
@@ -39,7 +39,7 @@ static int launch_elements_impl(int kind, int form, const ElemArgs<T> &a, hipStr
             return PYHIP_ERR_INVALID;
         }
         constexpr int EPB = 256 / 22;
-        const int blocks = (a.nel + EPB - 1) / EPB;
+        const int blocks = (a.e1 - a.e0 + EPB - 1) / EPB;
         if (blocks == 0) return PYHIP_OK;
         if (a.hvec) {
             set_error("Hessian-vector products cover the scalar forms");

@@ -69,8 +69,8 @@ struct FaceRuleQ3 {
 
 template <typename T, int NN, typename RULE>
 __global__ __launch_bounds__(256) void flux_face_kernel(ElemArgs<T> a) {
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= a.nel || (a.sub_on && !a.sub_on[a.sub[e]])) return;
+    const int e = a.e0 + (int)(blockIdx.x * 256 + threadIdx.x);
+    if (e >= a.e1) return;
     const T alpha = (T)a.fcoef[2 * (int64_t)e], beta = (T)a.fcoef[2 * (int64_t)e + 1];
     T X[NN][2], U[NN];
 #pragma unroll
@@ -133,8 +133,8 @@ __global__ __launch_bounds__(256) void flux_face_kernel(ElemArgs<T> a) {
 template <typename T>
 __global__ __launch_bounds__(256) void flux_face_quad9_kernel(ElemArgs<T> a) {
     constexpr int NN = 9, NQ = 9;
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= a.nel || (a.sub_on && !a.sub_on[a.sub[e]])) return;
+    const int e = a.e0 + (int)(blockIdx.x * 256 + threadIdx.x);
+    if (e >= a.e1) return;
     const T alpha = (T)a.fcoef[2 * (int64_t)e], beta = (T)a.fcoef[2 * (int64_t)e + 1];
     T X[NN][3], U[NN];
 #pragma unroll
@@ -193,8 +193,8 @@ __global__ __launch_bounds__(256) void flux_face_quad9_kernel(ElemArgs<T> a) {
 template <typename T>
 __global__ __launch_bounds__(256) void traction_face_kernel(ElemArgs<T> a) {
     constexpr int NN = 3, NS = 6;
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= a.nel || (a.sub_on && !a.sub_on[a.sub[e]])) return;
+    const int e = a.e0 + (int)(blockIdx.x * 256 + threadIdx.x);
+    if (e >= a.e1) return;
     const T alpha = (T)a.fcoef[3 * (int64_t)e];
     const T beta[2] = {(T)a.fcoef[3 * (int64_t)e + 1], (T)a.fcoef[3 * (int64_t)e + 2]};
     T X[NN][2], U[NN][2];
@@ -259,8 +259,8 @@ __global__ __launch_bounds__(256) void traction_face_kernel(ElemArgs<T> a) {
 
 template <typename T>
 static int launch_faces(int kind, const ElemArgs<T> &a, hipStream_t st) {
-    if (a.nel <= 0) return PYHIP_OK;
-    const dim3 g((a.nel + 255) / 256), t(256);
+    if (a.e1 <= a.e0) return PYHIP_OK;
+    const dim3 g((a.e1 - a.e0 + 255) / 256), t(256);
     if (kind == kKindLine2) hipLaunchKernelGGL((flux_face_kernel<T, 2, FaceRule<2>>), g, t, 0, st, a);
     else if (kind == kKindLine3) hipLaunchKernelGGL((flux_face_kernel<T, 3, FaceRule<3>>), g, t, 0, st, a);
     else if (kind == kKindLine3Q) hipLaunchKernelGGL((flux_face_kernel<T, 3, FaceRuleQ3>), g, t, 0, st, a);

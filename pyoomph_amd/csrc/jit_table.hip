@@ -14,6 +14,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/pyoomph_jit_table.h"
@@ -60,6 +61,11 @@ struct ElemBlock {
 // device buffers (made on the first slot call).
 struct Instance {
     int form = 0, kind = 0, nn = 0, ns = 0, nq = 0;
+    // run-time generated forms (include/pyoomph_codegen.h): kernels from the form's module,
+    // the auxiliary field in nodal field 1 when the form reads one, the global parameter's
+    // index among the form parameters
+    bool generated = false, has_aux = false, pderiv = false;
+    int gparam = 0;
     JITFuncSpec_Table_FiniteElement_t *table = nullptr;
     double form_params[kParamStride] = {};
     int nform_params = 0;
@@ -69,6 +75,11 @@ struct Instance {
 };
 
 Instance g_inst[5][5];   // [form][kind]
+
+// generated forms: a fixed set of instances, each with its own slot functions (the slot
+// signature carries no user pointer, so every table needs distinct function addresses)
+constexpr int kGenInstances = 16;
+Instance g_gen[kGenInstances];
 
 int ensure_device(Instance &I) {
     if (I.dev) return PYHIP_OK;
@@ -118,17 +129,21 @@ int run_element(Instance &I, const JITElementInfo_t *ei, const JITShapeInfo_t *s
             h.hist2[s] = ei->nodal_data[l][v][2];
         }
     }
-    if (I.form == PYHIP_FORM_REACTION_DIFFUSION)
+    if (I.form == PYHIP_FORM_REACTION_DIFFUSION || I.has_aux)
         for (int l = 0; l < I.nn; l++) h.aux[l] = ei->nodal_data[l][1][0];   // the source field f
+    const JITFuncSpec_Table_FiniteElement_t *T = I.table;
+    const double gp = (T && T->global_parameters && T->global_parameters[0]) ? *T->global_parameters[0] : 0.0;
     if (I.form == PYHIP_FORM_BRATU) {
-        const JITFuncSpec_Table_FiniteElement_t *T = I.table;
-        h.params[0] = (T && T->global_parameters && T->global_parameters[0]) ? *T->global_parameters[0] : 0.0;
+        h.params[0] = gp;
     } else {
         for (int k = 0; k < I.nform_params; k++) h.params[k] = I.form_params[k];
+        if (I.generated && I.pderiv) h.params[I.gparam] = gp;
     }
     PYHIP_CHECK_HIP(hipMemcpy(I.dev, &h, offsetof(ElemBlock, Er), hipMemcpyHostToDevice));
     ElemArgs<double> a{};
     a.nel = 1;
+    a.e0 = 0;
+    a.e1 = 1;
     a.conn = I.dev->conn;
     a.seqn = I.dev->seqn;
     a.sub = I.dev->sub;
@@ -152,7 +167,8 @@ int run_element(Instance &I, const JITElementInfo_t *ei, const JITShapeInfo_t *s
         a.hist1 = I.dev->hist1;
         a.hist2 = I.dev->hist2;
     }
-    if ((rc = launch_elements_f64(I.kind, I.form, a, nullptr))) return rc;
+    if ((rc = I.generated ? launch_generated_f64(I.form, a, nullptr) : launch_elements_f64(I.kind, I.form, a, nullptr)))
+        return rc;
     PYHIP_CHECK_HIP(hipMemcpy(out_R, I.dev->Er, I.ns * sizeof(double), hipMemcpyDeviceToHost));
     if (flag) PYHIP_CHECK_HIP(hipMemcpy(out_J, I.dev->Ej, (size_t)I.ns * I.ns * sizeof(double), hipMemcpyDeviceToHost));
     return PYHIP_OK;
@@ -186,13 +202,11 @@ void report(int rc) {
 
 // The reference signature: (eleminfo, shapeinfo, residuals, jacobian, mass_matrix, flag)
 // with flag 0: R, 1: R and J, 2: R, J and M.
-template <int FORM, int KIND, bool STEADY>
-void slot_res_jac(const JITElementInfo_t *ei, const JITShapeInfo_t *si, double *R, double *J, double *M,
-                  unsigned flag) {
-    Instance &I = g_inst[FORM][KIND];
+void res_jac(Instance &I, bool steady, const JITElementInfo_t *ei, const JITShapeInfo_t *si, double *R, double *J,
+             double *M, unsigned flag) {
     std::lock_guard<std::mutex> lock(I.mu);
     // time derivatives exist in the Taylor-Hood momentum only (the other forms are steady)
-    const bool tr = !STEADY && FORM == PYHIP_FORM_NAVIER_STOKES_TH && si && si->timestepper_weights_dt_BDF2;
+    const bool tr = !steady && I.form == PYHIP_FORM_NAVIER_STOKES_TH && si && si->timestepper_weights_dt_BDF2;
     double er[kMaxSlots], ej[kMaxSlots * kMaxSlots];
     int rc = run_element(I, ei, si, flag >= 1 ? 1 : 0, 0, tr, er, ej);
     if (rc) return report(rc);
@@ -204,16 +218,39 @@ void slot_res_jac(const JITElementInfo_t *ei, const JITShapeInfo_t *si, double *
     }
 }
 
-template <int FORM, int KIND>
-void slot_dparam(const JITElementInfo_t *ei, const JITShapeInfo_t *si, double *R, double *J, double *M,
-                 unsigned flag) {
-    (void)M;
-    Instance &I = g_inst[FORM][KIND];
+void dparam(Instance &I, const JITElementInfo_t *ei, const JITShapeInfo_t *si, double *R, double *J,
+            unsigned flag) {
     std::lock_guard<std::mutex> lock(I.mu);
     double er[kMaxSlots], ej[kMaxSlots * kMaxSlots];
     int rc = run_element(I, ei, si, flag >= 1 ? 1 : 0, 1, false, er, ej);
     if (rc) return report(rc);
     scatter(I, ei, si, er, flag >= 1 ? ej : nullptr, R, flag >= 1 ? J : nullptr);
+}
+
+template <int FORM, int KIND, bool STEADY>
+void slot_res_jac(const JITElementInfo_t *ei, const JITShapeInfo_t *si, double *R, double *J, double *M,
+                  unsigned flag) {
+    res_jac(g_inst[FORM][KIND], STEADY, ei, si, R, J, M, flag);
+}
+
+template <int FORM, int KIND>
+void slot_dparam(const JITElementInfo_t *ei, const JITShapeInfo_t *si, double *R, double *J, double *M,
+                 unsigned flag) {
+    (void)M;
+    dparam(g_inst[FORM][KIND], ei, si, R, J, flag);
+}
+
+template <int IDX, bool STEADY>
+void gen_slot_res_jac(const JITElementInfo_t *ei, const JITShapeInfo_t *si, double *R, double *J, double *M,
+                      unsigned flag) {
+    res_jac(g_gen[IDX], STEADY, ei, si, R, J, M, flag);
+}
+
+template <int IDX>
+void gen_slot_dparam(const JITElementInfo_t *ei, const JITShapeInfo_t *si, double *R, double *J, double *M,
+                     unsigned flag) {
+    (void)M;
+    dparam(g_gen[IDX], ei, si, R, J, flag);
 }
 
 using Slot = JITFuncSpec_ResidualAndJacobian_FiniteElement;
@@ -246,6 +283,20 @@ bool pick_slots(int form, int kind, Slot &u, Slot &s, Slot &d) {
     PYHIP_SLOT_CASE(PYHIP_FORM_NAVIER_STOKES_TH, PYHIP_ELEM_QUAD_Q2)
 #undef PYHIP_SLOT_CASE
     return false;
+}
+
+template <int... IDX>
+void gen_slots_table(std::integer_sequence<int, IDX...>, int k, Slot &u, Slot &st, Slot &d) {
+    static const Slot su[] = {&gen_slot_res_jac<IDX, false>...};
+    static const Slot ss[] = {&gen_slot_res_jac<IDX, true>...};
+    static const Slot sd[] = {&gen_slot_dparam<IDX>...};
+    u = su[k];
+    st = ss[k];
+    d = sd[k];
+}
+
+void pick_gen_slots(int k, Slot &u, Slot &s, Slot &d) {
+    gen_slots_table(std::make_integer_sequence<int, kGenInstances>{}, k, u, s, d);
 }
 
 char *dupstr(const char *s) {
@@ -297,6 +348,8 @@ void clean_up(JITFuncSpec_Table_FiniteElement_t *t) {
     for (auto &row : g_inst)
         for (auto &I : row)
             if (I.table == t) I.table = nullptr;
+    for (auto &I : g_gen)
+        if (I.table == t) I.table = nullptr;
 }
 
 }  // namespace
@@ -309,24 +362,54 @@ extern "C" {
 
 int pyhip_jit_element_init(int form, int elem_kind, JITFuncSpec_Table_FiniteElement_t *t) {
     PYHIP_REQUIRE(t, "null table");
-    PYHIP_REQUIRE(form >= 1 && form <= 4 && elem_nnode(elem_kind) > 0, "unknown weak form or element family");
+    const bool gen = gen_form_known(form);
+    PYHIP_REQUIRE(((form >= 1 && form <= 4) || gen) && elem_nnode(elem_kind) > 0,
+                  "unknown weak form or element family");
     Slot su = nullptr, ss = nullptr, sd = nullptr;
-    PYHIP_REQUIRE(pick_slots(form, elem_kind, su, ss, sd), "weak form not available on this element family");
-    Instance &I = g_inst[form][elem_kind];
+    std::string field = "u", aux_field;
+    int gparam = 0;
+    Instance *ip = nullptr;
+    if (gen) {
+        PYHIP_REQUIRE(gen_form_kind(form) == elem_kind, "generated form compiled for another element family");
+        (void)gen_form_fields(form, field, aux_field, gparam);
+        // the instance of this form, else a free one
+        int k = -1;
+        for (int i = 0; i < kGenInstances && k < 0; i++)
+            if (g_gen[i].generated && g_gen[i].form == form) k = i;
+        for (int i = 0; i < kGenInstances && k < 0; i++)
+            if (!g_gen[i].generated) k = i;
+        PYHIP_REQUIRE(k >= 0, "JIT tables of at most 16 generated forms");
+        pick_gen_slots(k, su, ss, sd);
+        ip = &g_gen[k];
+    } else {
+        PYHIP_REQUIRE(pick_slots(form, elem_kind, su, ss, sd), "weak form not available on this element family");
+        ip = &g_inst[form][elem_kind];
+    }
+    Instance &I = *ip;
     std::lock_guard<std::mutex> lock(I.mu);
     I.form = form;
     I.kind = elem_kind;
     I.nn = elem_nnode(elem_kind);
     I.ns = form == PYHIP_FORM_NAVIER_STOKES_TH ? 22 : I.nn;
     I.table = t;
+    I.generated = gen;
+    I.has_aux = gen && !aux_field.empty();
+    I.pderiv = gen && gen_form_has_pderiv(form);
+    I.gparam = gparam;
     const bool th = form == PYHIP_FORM_NAVIER_STOKES_TH;
     const bool p1 = elem_kind == PYHIP_ELEM_TRI_P1;
     t->nodal_dim = (unsigned)elem_dim(elem_kind);
     t->lagr_dim = 0;
     // fields and their nodal-data order: C2 fields first, then C1 (Taylor-Hood: velocity in
     // C2 at offset 0, pressure in C1 at offset 2; scalar forms: "u" in the family's space,
-    // reaction-diffusion adds its source field "f", pinned, after it)
-    const bool rd = form == PYHIP_FORM_REACTION_DIFFUSION;
+    // reaction-diffusion adds its source field "f", pinned, after it; generated forms their
+    // field and auxiliary field names)
+    const bool rd = form == PYHIP_FORM_REACTION_DIFFUSION || I.has_aux;
+    if (gen) {
+        if (aux_field.empty()) aux_field = "f";
+    } else {
+        aux_field = "f";
+    }
     if (th) {
         t->numfields_C2 = t->numfields_C2_bulk = t->numfields_C2_basebulk = t->numfields_C2_new = 2;
         t->fieldnames_C2 = names({"velocity_x", "velocity_y"});
@@ -339,11 +422,11 @@ int pyhip_jit_element_init(int form, int elem_kind, JITFuncSpec_Table_FiniteElem
     } else if (p1) {
         const unsigned nf = rd ? 2 : 1;
         t->numfields_C1 = t->numfields_C1_bulk = t->numfields_C1_basebulk = t->numfields_C1_new = nf;
-        t->fieldnames_C1 = rd ? names({"u", "f"}) : names({"u"});
+        t->fieldnames_C1 = rd ? names({field.c_str(), aux_field.c_str()}) : names({field.c_str()});
     } else {
         const unsigned nf = rd ? 2 : 1;
         t->numfields_C2 = t->numfields_C2_bulk = t->numfields_C2_basebulk = t->numfields_C2_new = nf;
-        t->fieldnames_C2 = rd ? names({"u", "f"}) : names({"u"});
+        t->fieldnames_C2 = rd ? names({field.c_str(), aux_field.c_str()}) : names({field.c_str()});
     }
     t->num_res_jacs = 1;
     t->current_res_jac = 0;
@@ -364,7 +447,7 @@ int pyhip_jit_element_init(int form, int elem_kind, JITFuncSpec_Table_FiniteElem
     if (th) rs.psi_C1 = true;
     rs.psi_Pos = rs.dx_psi_Pos = true;
     t->merged_required_shapes = rs;
-    if (form == PYHIP_FORM_BRATU) {
+    if (form == PYHIP_FORM_BRATU || I.pderiv) {
         t->numglobal_params = 1;
         t->global_paramindices = (unsigned *)calloc(1, sizeof(unsigned));
         t->global_parameters = (double **)calloc(1, sizeof(double *));
@@ -385,14 +468,22 @@ int pyhip_jit_element_init(int form, int elem_kind, JITFuncSpec_Table_FiniteElem
 int pyhip_jit_set_form_params(JITFuncSpec_Table_FiniteElement_t *t, const double *params, int nparams) {
     PYHIP_REQUIRE(t && (params || nparams == 0), "null argument");
     PYHIP_REQUIRE(nparams >= 0 && nparams <= kParamStride, "too many parameters");
+    auto set = [&](Instance &I) {
+        std::lock_guard<std::mutex> lock(I.mu);
+        for (int k = 0; k < nparams; k++) I.form_params[k] = params[k];
+        I.nform_params = nparams;
+    };
     for (auto &row : g_inst)
         for (auto &I : row)
             if (I.table == t) {
-                std::lock_guard<std::mutex> lock(I.mu);
-                for (int k = 0; k < nparams; k++) I.form_params[k] = params[k];
-                I.nform_params = nparams;
+                set(I);
                 return PYHIP_OK;
             }
+    for (auto &I : g_gen)
+        if (I.table == t) {
+            set(I);
+            return PYHIP_OK;
+        }
     set_error("table not filled by pyhip_jit_element_init (or cleaned up)");
     return PYHIP_ERR_INVALID;
 }

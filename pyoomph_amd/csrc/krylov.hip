@@ -2150,7 +2150,7 @@ int pyhip_newton_step(pyhip_problem *p, double *x, const pyhip_newton_opts *o, c
     };
     auto residual_norms = [&]() -> int {
         int r2;
-        if ((r2 = assemble_subset(p, x, 0, -1, PYHIP_F64, W->res64, nullptr, W->dof_mask))) return r2;
+        if ((r2 = assemble_subset(p, x, 0, -1, PYHIP_F64, W->res64, nullptr, running.data()))) return r2;
         if ((r2 = batched_maxabs(p, W->res64, W->maxres))) return r2;
         PYHIP_CHECK_HIP(hipMemcpyAsync(W->maxres_h, W->maxres, ns * sizeof(double), hipMemcpyDeviceToHost, p->st));
         PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
@@ -2186,9 +2186,9 @@ int pyhip_newton_step(pyhip_problem *p, double *x, const pyhip_newton_opts *o, c
     W->amg_stale = true;
     if (dropped && (rc = upload_running())) return rc;
     if (o->jacobian_precision == PYHIP_F64) {
-        if ((rc = assemble_subset(p, x, 1, -1, PYHIP_F64, W->w64, W->vals64, W->dof_mask))) return rc;
+        if ((rc = assemble_subset(p, x, 1, -1, PYHIP_F64, W->w64, W->vals64, running.data()))) return rc;
     } else {
-        if ((rc = assemble_subset(p, x, 1, -1, PYHIP_F32, W->w64, W->vals32, W->dof_mask))) return rc;
+        if ((rc = assemble_subset(p, x, 1, -1, PYHIP_F32, W->w64, W->vals32, running.data()))) return rc;
     }
     int its = 0, refs = 0, lfail = 0;
     if ((rc = linear_solve(p, W, o->linear, o->jacobian_precision, W->res64, W->d64, W->dof_mask, &its, &refs,

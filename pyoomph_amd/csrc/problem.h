@@ -114,16 +114,17 @@ struct TimeScope {
     }
 };
 
-// pyhip_assemble restricted to the sub-problems with sub_on[s] != 0 (device [nsub]; null:
+// pyhip_assemble restricted to the sub-problems with sub_on_h[s] != 0 (host [nsub]; null:
 // all of them); api.hip
 int assemble_subset(pyhip_problem *p, const double *dofs, int flag, int deriv_param, int precision,
-                    double *residual, void *values, const int32_t *sub_on);
+                    double *residual, void *values, const int32_t *sub_on_h);
 int launch_elements_f64(int kind, int form, const ElemArgs<double> &a, hipStream_t s);
 int launch_elements_f32(int kind, int form, const ElemArgs<float> &a, hipStream_t s);
 // run-time generated forms (codegen_rt.hip)
 bool gen_form_known(int form);
 int gen_form_kind(int form);
 bool gen_form_has_pderiv(int form);
+bool gen_form_fields(int form, std::string &field, std::string &aux_field, int &global_param);
 int launch_generated_f64(int form, const ElemArgs<double> &a, hipStream_t s);
 int launch_generated_f32(int form, const ElemArgs<float> &a, hipStream_t s);
 int launch_faces_f64(int kind, const ElemArgs<double> &a, hipStream_t s);

@@ -164,3 +164,83 @@ def test_superlu_entry_solves_like_splu(gpu):
     assert L.pyhip_superlu(ctypes.byref(op_), ctypes.byref(n_), None, None, None, None, None, None, None, None, None,
                            ctypes.byref(fac), ctypes.byref(info)) == 0
     assert not fac.value
+
+
+GEN_CASES = [("bratu", dict(N=4, space="C2", lam=2.3), _lib.ELEM_TRI_P2),
+             ("bratu", dict(N=5, space="C1", lam=1.7), _lib.ELEM_TRI_P1),
+             ("rd", dict(N=2), _lib.ELEM_HEX_Q2)]
+
+
+@pytest.mark.parametrize("kind,kw,elem", GEN_CASES)
+def test_generated_form_tables_equal_the_compiled_ones(gpu, kind, kw, elem):
+    """A run-time generated form's JIT table: fields, global parameter and slots; its slot
+    outputs equal the compiled form's slot outputs bit for bit (R, J, dR/dlambda)."""
+    from pyoomph_amd.codegen import library
+    from pyoomph_amd.codegen.runtime import compile_form
+    mp = product_problem(kind, **kw)
+    okw = dict(kw)
+    aux = None
+    if kind == "rd":
+        okw["f"] = aux = mp.equations.source_field
+    op = oracle_problem(kind, **okw)
+    gform = library.bratu() if kind == "bratu" else library.reaction_diffusion()
+    gid = compile_form(gform, {_lib.ELEM_TRI_P1: "tri_p1", _lib.ELEM_TRI_P2: "tri_p2",
+                               _lib.ELEM_HEX_Q2: "hex_q2"}[elem])
+    tc, tg = jt.JitTable(mp.equations.form, elem), jt.JitTable(gid, elem)
+    try:
+        space = "numfields_C1" if elem == _lib.ELEM_TRI_P1 else "numfields_C2"
+        names = "fieldnames_C1" if elem == _lib.ELEM_TRI_P1 else "fieldnames_C2"
+        nf = tc.uint(space)
+        assert tg.uint(space) == nf and tg.names(names, nf) == tc.names(names, nf)
+        assert tg.uint("numglobal_params") == tc.uint("numglobal_params")
+        if kind == "bratu":
+            tc.set_global_parameter(0, kw["lam"])
+            tg.set_global_parameter(0, kw["lam"])
+        x = np.random.default_rng(8).uniform(-0.4, 0.4, op.ndof)
+        nel = op.mesh.elements.shape[0]
+        for e in sorted({0, nel // 2, nel - 1}):
+            info = element_info(op, e, x, aux=aux)
+            nv = int(op.nvar[e])
+            for name in ("ResidualAndJacobianSteady", "ResidualAndJacobian"):
+                a = call_slot(tc.slot(name), info, nv, 1)
+                b = call_slot(tg.slot(name), info, nv, 1)
+                assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (e, name)
+            if kind == "bratu":
+                a = call_slot(tc.parameter_derivative(), info, nv, 1)
+                b = call_slot(tg.parameter_derivative(), info, nv, 1)
+                assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), e
+    finally:
+        tc.close()
+        tg.close()
+
+
+def test_generated_form_without_a_compiled_counterpart(gpu):
+    """Nonlinear diffusion -div((1 + k u^2) grad u) = s0 through its JIT table: the slot's
+    Jacobian is the central difference of the slot's residual (and not symmetric)."""
+    from pyoomph_amd.codegen import library
+    from pyoomph_amd.codegen.runtime import compile_form
+    form = library.nonlinear_diffusion()
+    gid = compile_form(form, "tri_p2")
+    op = oracle_problem("bratu", N=3, space="C2", lam=1.0)     # mesh, numbering, slots
+    t = jt.JitTable(gid, _lib.ELEM_TRI_P2)
+    try:
+        t.set_form_params([2.0, 8.0])
+        rng = np.random.default_rng(2)
+        x = rng.uniform(-0.5, 0.5, op.ndof)
+        e = 4
+        nv = int(op.nvar[e])
+        R, J, _ = call_slot(t.slot("ResidualAndJacobianSteady"), element_info(op, e, x), nv, 1)
+        d = rng.normal(size=op.ndof)
+        eps = 1e-6
+        Rp, _, _ = call_slot(t.slot("ResidualAndJacobianSteady"), element_info(op, e, x + eps * d), nv, 0)
+        Rm, _, _ = call_slot(t.slot("ResidualAndJacobianSteady"), element_info(op, e, x - eps * d), nv, 0)
+        leq = op.leqn[e]
+        dl = np.zeros(nv)
+        for s in range(len(leq)):
+            if leq[s] >= 0:
+                dl[leq[s]] = d[op.slot_geqn[e, s]]
+        fd = (Rp - Rm) / (2 * eps)
+        assert np.max(np.abs(J @ dl - fd)) <= 1e-6 * max(np.max(np.abs(fd)), 1e-12)
+        assert not np.allclose(J, J.T)
+    finally:
+        t.close()
