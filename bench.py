@@ -488,29 +488,34 @@ def main():
 
     def timed_sweep(precision, with_timing):
         """warmup + timed steps of the whole sweep in one precision; returns
-        (solves/s, ms/step, latencies, stats, kernel families, spmv bytes, dofs)."""
+        (solves/s, ms/step, latencies, stats, kernel families, spmv bytes, dofs). With
+        with_timing, the timed steps are repeated once more with HIP events on the problem's
+        stream around every kernel family (the per-launch durations of the roofline; the
+        events cost ~8 % of a step, so the solves/s come from the first, event-free pass)."""
         configure(P, precision, args.precond, correction_tol=ctol)
 
         def step():
             P.dofs.zero_()
             return P.solve()
 
+        def timed():
+            lat = []
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                st = step()
+                lat += [s["done_ms"] for s in st]
+                assert all(s["converged"] for s in st)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0, lat, st
+
         for _ in range(args.warmup):
             step()
-        torch.cuda.synchronize()
-        if with_timing:
-            _lib.check(_lib.load().pyhip_timing_enable(P._handle, 1))
-        lat = []
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            st = step()
-            lat += [s["done_ms"] for s in st]
-            assert all(s["converged"] for s in st)
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
+        elapsed, lat, st = timed()
         fams, work = None, None
         if with_timing:
+            _lib.check(_lib.load().pyhip_timing_enable(P._handle, 1))
+            timed()
             fams = {f: timing(P, f) for f in ("element", "gather_values", "gather_residual", "spmv",
                                                "precond", "vector")}
             work = timing_bytes(P, "spmv")

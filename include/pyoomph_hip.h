@@ -179,11 +179,13 @@ int pyhip_set_time_stepping(pyhip_problem *p, int enabled, const double *weights
 int pyhip_assemble_noalloc(pyhip_problem *p, const double *dofs_dev, int flag, int deriv_param,
                            int precision, double *residual_dev, void *values_dev);
 
-/* Hessian-vector product of the scalar forms: values_dev [nnz] (structural pattern)
- * receives sum_k dJ_ij/du_k y_k at dofs_dev, residual_dev the residual; the reference's
- * HessianVectorProduct with one vector (src/codegen.cpp:3646-3910, assembled by
- * src/elements.cpp:4964-4995, flag 1; for these forms the transposed product, flag 4, is
- * the same matrix). pyhip_export_reference_csr then gives its reference CSR. */
+/* Hessian-vector product of the scalar forms and of Taylor-Hood: values_dev [nnz]
+ * (structural pattern) receives sum_k dJ_ij/du_k y_k at dofs_dev, residual_dev the
+ * residual; the reference's HessianVectorProduct with one vector (src/codegen.cpp:
+ * 3646-3910, assembled by src/elements.cpp:4964-4995, flag 1; for the scalar forms the
+ * transposed product, flag 4, is the same matrix). Taylor-Hood: the convection block with
+ * the velocity replaced by y (the only non-linear term). Several vectors: one call per
+ * vector. pyhip_export_reference_csr then gives its reference CSR. */
 int pyhip_assemble_hessian_vector(pyhip_problem *p, const double *dofs_dev, const double *y_dev,
                                   double *residual_dev, double *values_dev);
 

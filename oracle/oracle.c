@@ -218,14 +218,19 @@ static const int TH_slot_pidx[22] = {-1, -1, 0, -1, -1, -1, -1, 1, -1, -1, -1, -
 
 /* Time derivative (H non-NULL, H[slot] = w1 v(t-1) + w2 v(t-2)): du_a/dt = w0 u_a + sum_l
  * H_a[l] psi_l; the momentum rows gain (rho du_a/dt) psi_i and the Jacobian w0 rho psi_j psi_i
- * on the diagonal velocity blocks; flag 2 writes the mass matrix rho psi_j psi_i there. */
+ * on the diagonal velocity blocks; flag 2 writes the mass matrix rho psi_j psi_i there.
+ * Hessian-vector product (Y non-NULL, slot values of the vector, flag 1): only the
+ * convection term rho (u . grad) u_a is not linear, so J receives its Jacobian with the
+ * velocity u replaced by y: rho (psi_j d_c y_a + delta_ac (y . grad psi_j)) psi_i on the
+ * velocity-velocity blocks, nothing elsewhere (src/codegen.cpp:3646-3910). */
 static void th_element(const ref_t *r, const double *X, const double *V, const long *leqn,
-                       const double *par, int flag, int nvar, const double *H, double tw0, double *R, double *J)
+                       const double *par, int flag, int nvar, const double *H, double tw0, const double *Y,
+                       double *R, double *J)
 {
     const double mu = par[0], rho = par[1];
     const int nn = 9;
     double dx[MAXN][MAXD];
-    double Ux[9], Uy[9], Pp[4], Hx[9] = {0}, Hy[9] = {0};
+    double Ux[9], Uy[9], Pp[4], Hx[9] = {0}, Hy[9] = {0}, Yx[9] = {0}, Yy[9] = {0};
     for (int s = 0; s < 22; s++) {
         int k = TH_slot_kind[s], n = TH_slot_node[s];
         if (k == 0) Ux[n] = V[s];
@@ -233,6 +238,8 @@ static void th_element(const ref_t *r, const double *X, const double *V, const l
         else Pp[TH_slot_pidx[s]] = V[s];
         if (H && k == 0) Hx[n] = H[s];
         if (H && k == 1) Hy[n] = H[s];
+        if (Y && k == 0) Yx[n] = Y[s];
+        if (Y && k == 1) Yy[n] = Y[s];
     }
     for (int q = 0; q < r->nqp; q++) {
         const double W = geometry(r, q, X, dx);
@@ -259,6 +266,18 @@ static void th_element(const ref_t *r, const double *X, const double *V, const l
         double conv[2];
         for (int a = 0; a < 2; a++) conv[a] = rho * (u[0] * du[a][0] + u[1] * du[a][1]);
         const double divu = du[0][0] + du[1][1];
+        double yv[2] = {0.0, 0.0}, dy[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+        if (Y) {
+            for (int l = 0; l < nn; l++) yv[0] = yv[0] + Yx[l] * P[l];
+            for (int l = 0; l < nn; l++) yv[1] = yv[1] + Yy[l] * P[l];
+            for (int b = 0; b < 2; b++) {
+                double s0 = 0.0, s1 = 0.0;
+                for (int l = 0; l < nn; l++) s0 = s0 + Yx[l] * dx[l][b];
+                for (int l = 0; l < nn; l++) s1 = s1 + Yy[l] * dx[l][b];
+                dy[0][b] = s0;
+                dy[1][b] = s1;
+            }
+        }
         double ud[2] = {0.0, 0.0};
         if (H) {
             double h0 = 0.0, h1 = 0.0;
@@ -288,7 +307,16 @@ static void th_element(const ref_t *r, const double *X, const double *V, const l
                 if (iu < 0) continue;
                 int ks = TH_slot_kind[ss], ls = TH_slot_node[ss];
                 double v;
-                if (flag == 2) {
+                if (Y) {
+                    if (kt < 2 && ks < 2) {
+                        int a = kt, c = ks;
+                        double cvy = P[ls] * dy[a][c];
+                        if (a == c) cvy = cvy + (yv[0] * dx[ls][0] + yv[1] * dx[ls][1]);
+                        v = W * ((rho * cvy) * P[lt]);
+                    } else {
+                        continue;
+                    }
+                } else if (flag == 2) {
                     if (kt < 2 && kt == ks) v = W * ((rho * P[ls]) * P[lt]);
                     else continue;
                 } else if (kt < 2 && ks < 2) {
@@ -342,9 +370,10 @@ int oracle_elements(int form, int nel, int dim, int nnode, int nqp, int nnode1,
         double *Re = R + (size_t)e * maxdof;
         double *Je = J ? J + (size_t)e * maxdof * maxdof : NULL;
         if (form == 4) {
-            if (deriv || Yv) return -2;
+            if (deriv) return -2;
             const double *He = Hist ? Hist + (size_t)e * nslot : NULL;
-            th_element(&r, Xe, Ve, Le, par, flag, (int)nvar[e], He, tw0, Re, Je);
+            const double *Ye = Yv ? Yv + (size_t)e * nslot : NULL;
+            th_element(&r, Xe, Ve, Le, par, flag, (int)nvar[e], He, tw0, Ye, Re, Je);
         } else {
             const double *Ae = aux ? aux + (size_t)e * nnode : NULL;
             const double *He = Hist ? Hist + (size_t)e * nslot : NULL;

@@ -670,9 +670,6 @@ int pyhip_assemble_noalloc(pyhip_problem *p, const double *dofs, int flag, int d
 int pyhip_assemble_hessian_vector(pyhip_problem *p, const double *dofs, const double *y, double *residual,
                                   double *values) {
     PYHIP_REQUIRE(p && dofs && y && residual && values, "null argument");
-    for (auto &G : p->groups)
-        PYHIP_REQUIRE(G.form != PYHIP_FORM_NAVIER_STOKES_TH,
-                      "Hessian-vector products cover the scalar forms (Poisson, Bratu, reaction-diffusion)");
     p->hvec = y;
     const int rc = pyhip_assemble(p, dofs, 1, -1, PYHIP_F64, residual, values);
     p->hvec = nullptr;

@@ -241,17 +241,20 @@ __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
 // TR: time derivative of the velocities (a.transient): du/dt = tw0 u + sum_l H psi_l with
 // H = tw1 u(t-1) + tw2 u(t-2) per slot; momentum rows gain (rho du/dt) psi_i, the diagonal
 // velocity blocks tw0 rho psi_j psi_i; flag 2 writes the mass matrix rho psi_j psi_i there.
-template <typename T, bool TR>
+template <typename T, bool TR, bool HV = false>
 __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
     constexpr int NN = 9, DIM = 2, NQ = 9, NS = 22;
     constexpr int EPB = 256 / NS;  // 11 elements, 242 threads
+    // per point: u0, u1, du00, du01, du10, du11, p[, du0/dt, du1/dt][, y0, y1, dy00, dy01, dy10, dy11]
+    constexpr int QY = TR ? 9 : 7;
     __shared__ T sX[EPB][NN][DIM];
     __shared__ T sV[EPB][NS];
     __shared__ int sEq[EPB][NS];
     __shared__ T sW[EPB][NQ];
     __shared__ T sDx[EPB][NQ][NN][DIM];
-    __shared__ T sQ[EPB][NQ][TR ? 9 : 7];   // u0, u1, du00, du01, du10, du11, p[, du0/dt, du1/dt]
+    __shared__ T sQ[EPB][NQ][QY + (HV ? 6 : 0)];
     __shared__ T sH[TR ? EPB : 1][NS];
+    __shared__ T sY[HV ? EPB : 1][NS];   // Hessian-vector product direction per slot
 
     const int le = threadIdx.x / NS;
     const int slot = threadIdx.x % NS;
@@ -273,6 +276,7 @@ __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
             const double u1 = eq >= 0 ? a.hist1[eq] : pin, u2 = eq >= 0 ? a.hist2[eq] : pin;
             sH[le][slot] = T(a.tw1 * u1 + a.tw2 * u2);
         }
+        if constexpr (HV) sY[le][slot] = T(eq >= 0 ? a.hvec[eq] : 0.0);
         sEq[le][slot] = eq;
     }
     __syncthreads();
@@ -319,6 +323,25 @@ __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
             for (int l = 0; l < NN; l++) h1 = h1 + sH[le][kTHNodeSlot[l] + 1] * P[l];
             sQ[le][q][7] = T(a.tw0) * u0 + h0;
             sQ[le][q][8] = T(a.tw0) * u1 + h1;
+        }
+        if constexpr (HV) {
+            T y0 = T(0), y1 = T(0);
+#pragma unroll
+            for (int l = 0; l < NN; l++) y0 = y0 + sY[le][kTHNodeSlot[l]] * P[l];
+#pragma unroll
+            for (int l = 0; l < NN; l++) y1 = y1 + sY[le][1 + kTHNodeSlot[l]] * P[l];
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                T s0 = T(0), s1 = T(0);
+#pragma unroll
+                for (int l = 0; l < NN; l++) s0 = s0 + sY[le][kTHNodeSlot[l]] * sDx[le][q][l][b];
+#pragma unroll
+                for (int l = 0; l < NN; l++) s1 = s1 + sY[le][kTHNodeSlot[l] + 1] * sDx[le][q][l][b];
+                sQ[le][q][QY + 2 + b] = s0;   // dy[0][b]
+                sQ[le][q][QY + 4 + b] = s1;   // dy[1][b]
+            }
+            sQ[le][q][QY] = y0;
+            sQ[le][q][QY + 1] = y1;
         }
     }
     __syncthreads();
@@ -367,7 +390,17 @@ __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
                 if (sEq[le][ss] < 0) continue;
                 const int ks = kTHSlotKind[ss], ls = kTHSlotNode[ss];
                 T v;
-                if (a.flag == 2) {
+                if constexpr (HV) {
+                    // d/du of the convection block in the direction y: the block with u -> y
+                    if (kt < 2 && ks < 2) {
+                        const T *dxs = sDx[le][q][ls];
+                        T cvy = P[ls] * sQ[le][q][QY + 2 + 2 * kt + ks];
+                        if (kt == ks) cvy = cvy + (sQ[le][q][QY] * dxs[0] + sQ[le][q][QY + 1] * dxs[1]);
+                        v = W * ((rho * cvy) * P[lt]);
+                    } else {
+                        continue;
+                    }
+                } else if (a.flag == 2) {
                     if (kt < 2 && kt == ks) v = W * ((rho * P[ls]) * P[lt]);
                     else continue;
                 } else if (kt < 2 && ks < 2) {

@@ -42,10 +42,9 @@ static int launch_elements_impl(int kind, int form, const ElemArgs<T> &a, hipStr
         const int blocks = (a.e1 - a.e0 + EPB - 1) / EPB;
         if (blocks == 0) return PYHIP_OK;
         if (a.hvec) {
-            set_error("Hessian-vector products cover the scalar forms");
-            return PYHIP_ERR_INVALID;
-        }
-        if (a.transient)
+            // the product of the steady Jacobian (the time term is linear)
+            hipLaunchKernelGGL((th_element_kernel<T, false, true>), dim3(blocks), dim3(256), 0, s, a);
+        } else if (a.transient)
             hipLaunchKernelGGL((th_element_kernel<T, true>), dim3(blocks), dim3(256), 0, s, a);
         else
             hipLaunchKernelGGL((th_element_kernel<T, false>), dim3(blocks), dim3(256), 0, s, a);

@@ -408,8 +408,11 @@ class Problem:
         return scipy.sparse.csr_matrix((val, col, rs), shape=(self.ndof, self.ndof))
 
     def assemble_hessian_vector_product(self, y):
-        """sum_k dJ/du_k y_k at the current dofs (the reference's HessianVectorProduct
-        with one vector) in the reference's CSR form."""
+        """sum_k dJ/du_k y_k at the current dofs (the reference's HessianVectorProduct) in
+        the reference's CSR form; y of shape [nvec, ndof] gives one matrix per vector."""
+        y = np.asarray(y, dtype=np.float64)
+        if y.ndim == 2:
+            return [self.assemble_hessian_vector_product(v) for v in y]
         import scipy.sparse
         torch = self.torch
         with torch.cuda.stream(self._stream):

@@ -285,10 +285,11 @@ def test_solver_plugin_matches_superlu(gpu):
 
 @pytest.mark.parametrize("kind,kw", [("bratu", dict(N=7, space="C1", lam=1.5)),
                                      ("bratu", dict(N=5, space="C2", lam=2.5)),
-                                     ("rd", dict(N=3)), ("poisson", dict(N=4))])
+                                     ("rd", dict(N=3)), ("poisson", dict(N=4)), ("ns", dict(N=4, Re=70.0))])
 def test_hessian_vector_product_parity(gpu, kind, kw):
     """sum_k dJ/du_k y_k: reference CSR (first-touch, exact zeros dropped) and values
-    against the oracle; Poisson's product is identically zero, so no entry survives."""
+    against the oracle; Poisson's product is identically zero, so no entry survives;
+    Taylor-Hood's is the convection block with u -> y."""
     P, op = build(kind, kw)
     rng = np.random.default_rng(11)
     x, y = rng.uniform(-0.4, 0.4, P.ndof), rng.uniform(-1, 1, P.ndof)
@@ -299,3 +300,20 @@ def test_hessian_vector_product_parity(gpu, kind, kw):
     assert rel(H.data, val) <= 1e-12 if val.size else H.nnz == 0
     if kind != "bratu":
         assert np.array_equal(H.data, val)
+
+
+def test_hessian_products_of_several_vectors(gpu):
+    """Y [nvec, ndof] gives one product per vector, each equal to its single-vector call."""
+    P, op = build("ns", dict(N=3, Re=40.0))
+    rng = np.random.default_rng(13)
+    P.set_current_dofs(rng.uniform(-0.4, 0.4, P.ndof))
+    Y = rng.uniform(-1, 1, (3, P.ndof))
+    Hs = P.assemble_hessian_vector_product(Y)
+    assert len(Hs) == 3
+    for k in range(3):
+        H1 = P.assemble_hessian_vector_product(Y[k])
+        assert np.array_equal(Hs[k].indptr, H1.indptr) and np.array_equal(Hs[k].data, H1.data)
+    # linear in the vector: H(y0 + 2 y1) = H(y0) + 2 H(y1) on the structural entries
+    Hl = P.assemble_hessian_vector_product(Y[0] + 2.0 * Y[1])
+    D = (Hl - (Hs[0] + 2.0 * Hs[1])).toarray()
+    assert np.max(np.abs(D)) <= 1e-12 * np.max(np.abs(Hl.toarray()))

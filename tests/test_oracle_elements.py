@@ -257,3 +257,27 @@ def test_rd_hessian_vector_product_matches_finite_differences():
     Jm = op.element_outputs(x - 1e-5 * y, 1)[1][0]
     H = op.element_outputs(x, 1, y=y)[1][0]
     assert np.max(np.abs(H - (Jp - Jm) / 2e-5)) < 1e-7 * max(1.0, np.max(np.abs(H)))
+
+
+def test_taylor_hood_hessian_vector_product_matches_finite_differences():
+    """Taylor-Hood: the Jacobian is linear in the velocity (the convection term is the only
+    non-linearity), so (J(x + h y) - J(x - h y)) / 2h equals sum_k dJ/du_k y_k up to
+    rounding; pressure rows and columns of the product vanish."""
+    op = oracle_problem("ns", N=2, Re=50.0)
+    rng = np.random.default_rng(12)
+    x, y = rng.uniform(-0.5, 0.5, op.ndof), rng.uniform(-1, 1, op.ndof)
+
+    def dense_j(v, **kw2):
+        _, rs, col, val = op.get_jacobian(v, **kw2)
+        M = np.zeros((op.ndof, op.ndof))
+        for i in range(op.ndof):
+            M[i, col[rs[i]:rs[i + 1]]] = val[rs[i]:rs[i + 1]]
+        return M
+
+    H = dense_j(x, y=y)
+    h = 1e-3
+    Hfd = (dense_j(x + h * y) - dense_j(x - h * y)) / (2 * h)
+    assert np.max(np.abs(H)) > 1e-3
+    assert np.max(np.abs(H - Hfd)) < 1e-9 * max(1.0, np.max(np.abs(H)))
+    p = op.eqn[:, 2][op.eqn[:, 2] >= 0]
+    assert not H[p].any() and not H[:, p].any()
