@@ -290,6 +290,11 @@ typedef struct {
 int pyhip_newton_step(pyhip_problem *p, double *dofs_dev, const pyhip_newton_opts *opts, const int32_t *mask,
                       pyhip_newton_step_state *state);
 
+/* Moves the problem to another HIP stream (waits for the work queued on the old one); its
+ * later calls enqueue there. Lets a serving loop run whichever pool steps on one of a few
+ * streams, each on its own hardware queue. */
+int pyhip_problem_set_stream(pyhip_problem *p, void *stream);
+
 /* Replaces one sub-problem's auxiliary nodal field (reaction-diffusion source f): aux
  * [nnode] host values, nnode the sub-problem's node count. */
 int pyhip_problem_set_aux(pyhip_problem *p, int sub, const double *aux, int64_t nnode);

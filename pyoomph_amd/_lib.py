@@ -114,6 +114,7 @@ SIGNATURES = {
     "pyhip_newton_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(NewtonStepState)]),
     "pyhip_problem_set_aux": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dp, ctypes.c_int64]),
+    "pyhip_problem_set_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "pyhip_jit_element_init": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "pyhip_jit_set_form_params": (ctypes.c_int, [ctypes.c_void_p, c_dp, ctypes.c_int]),
     "pyhip_superlu": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)] * 4 + [c_dp] + [ctypes.POINTER(ctypes.c_int)] * 2 +

@@ -423,6 +423,14 @@ int pyhip_problem_set_params(pyhip_problem *p, int sub, const double *params, in
     return PYHIP_OK;
 }
 
+int pyhip_problem_set_stream(pyhip_problem *p, void *stream) {
+    PYHIP_REQUIRE(p, "null problem");
+    // the work queued so far finishes on the old stream before the new one is used
+    PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
+    p->st = (hipStream_t)stream;
+    return PYHIP_OK;
+}
+
 int pyhip_problem_set_aux(pyhip_problem *p, int sub, const double *aux, int64_t nnode) {
     PYHIP_REQUIRE(p && sub >= 0 && sub < p->nsub && aux, "bad arguments");
     const SubInfo &S = p->subs[sub];

@@ -477,6 +477,14 @@ class Problem:
             a = np.ascontiguousarray(aux, dtype=np.float64).reshape(-1)
             _lib.check(L.pyhip_problem_set_aux(self._handle, int(sub), a.ctypes.data_as(_lib.c_dp), int(a.size)))
 
+    def set_stream(self, stream):
+        """Moves the problem (its library calls and its tensors' copies) to a torch stream
+        of the same device; the work queued on the old stream completes first."""
+        if stream is None or stream == self._stream:
+            return
+        _lib.check(_lib.load().pyhip_problem_set_stream(self._handle, ctypes.c_void_p(stream.cuda_stream)))
+        self._stream = stream
+
     def current_device_bytes(self):
         """Device bytes held now (mesh, pattern, element buffers, solver workspace)."""
         nb = ctypes.c_int64()

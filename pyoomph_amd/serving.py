@@ -20,8 +20,9 @@ Here requests share one GPU, batched continuously at the granularity of a Newton
   pool's queue and take free slots at the next step boundary, first come first served;
   a request whose solve has converged (or failed) leaves at once -- its dofs copied to
   the host, its slot freed -- while the others go on. At most ``workers`` pools step at
-  a time, each on its own HIP stream, so different pools' work overlaps on the device;
-  pools wanting to step are served in the order they asked.
+  a time, each on one of ``workers`` HIP streams (one hardware queue each), so different
+  pools' work overlaps on the device; pools wanting to step are served in the order they
+  asked.
 * Each slot's Newton and Krylov iteration is its own (per-sub-problem element rows,
   scalars, reductions and convergence tests; Jacobi per row; the multigrid hierarchies,
   per sub-problem, rebuilt from every step's Jacobian), so a request gives the same
@@ -182,8 +183,8 @@ class ServingLoop:
     ``workers`` pools stepping at a time.
 
     ``executor``: ``create(signature, slots) -> pool`` and ``make_stream()``; a pool has
-    ``slots``, ``device_bytes``, ``load(slot, request)``, ``step() -> [finished slots]``
-    and ``take(slot) -> (dofs, stats)``."""
+    ``slots``, ``device_bytes``, ``bind(stream)``, ``load(slot, request)``,
+    ``step() -> [finished slots]`` and ``take(slot) -> (dofs, stats)``."""
 
     def __init__(self, executor, hbm_budget=200e9, workers=4, slots=None):
         self.executor = executor
@@ -226,6 +227,10 @@ class ServingLoop:
         results = {}
         turn = deque()                     # pools waiting to step, in the order they asked
         state = {"released": 0, "tokens": self.workers, "error": None, "closing": False}
+        # one stream per step in flight: a stepping pool runs on a free one (more streams
+        # than hardware queues would put pools behind each other's kernels)
+        streams = [self.executor.make_stream() for _ in range(self.workers)]
+        free_streams = deque(range(self.workers))
         t0 = time.perf_counter()
 
         def now():
@@ -248,11 +253,13 @@ class ServingLoop:
                         return
                     turn.popleft()
                     state["tokens"] -= 1
+                    si = free_streams.popleft()
                     cv.notify_all()
                     admit = []
                     while free and queue[sig]:
                         admit.append((free.popleft(), queue[sig].popleft()))
                 try:
+                    pool.bind(streams[si])
                     t_adm = now()
                     for slot, r in admit:
                         pool.load(slot, r)
@@ -265,11 +272,13 @@ class ServingLoop:
                     with cv:
                         state["error"] = e
                         state["tokens"] += 1
+                        free_streams.append(si)
                         cv.notify_all()
                     return
                 t_done = now()
                 with cv:
                     state["tokens"] += 1
+                    free_streams.append(si)
                     for slot, (x, st) in outs:
                         r, t_a = busy.pop(slot)
                         free.append(slot)
@@ -358,6 +367,10 @@ class SlotPool:
                 pass
             self.take(0)
         self.device_bytes = self.P.current_device_bytes()
+
+    def bind(self, stream):
+        """Runs the pool's next calls on ``stream`` (a stream of the serving loop)."""
+        self.P.set_stream(stream)
 
     def load(self, slot, r):
         P = self.P

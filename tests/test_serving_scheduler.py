@@ -20,8 +20,12 @@ class FakePool:
         self.ex, self.sig, self.slots = ex, sig, slots
         self.device_bytes = ex.bytes_per_slot * slots
         self.busy = {}
+        self.stream = None
         self.log = []            # (step index, frozenset of rids in the step)
         self.loads = []          # rids in load order
+
+    def bind(self, stream):
+        self.stream = stream
 
     def load(self, slot, r):
         assert slot not in self.busy and 0 <= slot < self.slots
@@ -33,8 +37,13 @@ class FakePool:
         with ex.lock:
             ex.running += 1
             ex.max_running = max(ex.max_running, ex.running)
+            # no two pools step on one stream at the same time
+            assert self.stream is not None and self.stream not in ex.streams_in_use
+            ex.streams_in_use.add(self.stream)
         self.log.append(frozenset(r.rid for r, _ in self.busy.values()))
         if ex.fail_after is not None and sum(len(p.log) for p in ex.pools) > ex.fail_after:
+            with ex.lock:
+                ex.streams_in_use.discard(self.stream)
             raise RuntimeError("device failure")
         time.sleep(0.0005 + ex.per_dof * sum(r.ndof for r, _ in self.busy.values()))
         done = []
@@ -44,6 +53,7 @@ class FakePool:
                 done.append(slot)
         with ex.lock:
             ex.running -= 1
+            ex.streams_in_use.discard(self.stream)
         return done
 
     def take(self, slot):
@@ -60,9 +70,12 @@ class FakeExecutor:
         self.lock = threading.Lock()
         self.running = self.max_running = 0
         self.pools = []
+        self.streams_made = 0
+        self.streams_in_use = set()
 
     def make_stream(self):
-        return None
+        self.streams_made += 1
+        return "stream%d" % self.streams_made
 
     def create(self, sig, slots):
         p = FakePool(self, sig, slots)
