@@ -43,6 +43,12 @@ int pyhip_codegen_compile(const char *name, const char *source, int nheaders, co
  * 0. */
 int pyhip_codegen_set_fields(int form_id, const char *field, const char *aux_field, int global_param);
 
+/* Value layout of a generated form's elements: 0 (default) one value per node; 1 the
+ * Taylor-Hood layout of Q2 quadrilaterals (velocity_x, velocity_y at every node, pressure
+ * at the four corners: 22 slots in oomph's node-major order, sub-problems with nvalue 3),
+ * like PYHIP_FORM_NAVIER_STOKES_TH. Set before the form is used in a problem. */
+int pyhip_codegen_set_layout(int form_id, int layout);
+
 /* Registered generated forms (0 .. count-1 are ids PYHIP_GENERATED_FORM_BASE + k). */
 int pyhip_codegen_count(int *count);
 

@@ -111,6 +111,7 @@ SIGNATURES = {
                                              ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)]),
     "pyhip_codegen_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
     "pyhip_codegen_set_fields": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
+    "pyhip_codegen_set_layout": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "pyhip_newton_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(NewtonStepState)]),
     "pyhip_problem_set_aux": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dp, ctypes.c_int64]),

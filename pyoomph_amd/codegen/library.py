@@ -51,3 +51,52 @@ def nonlinear_diffusion():
         return [c * g for g in s.grad_u]
 
     return ScalarWeakForm("gen_nonlinear_diffusion", flux, lambda s: -s.p[1], nparams=2)
+
+
+def navier_stokes():
+    """Steady / transient incompressible Navier-Stokes in Taylor-Hood form, parameters
+    (mu, rho): momentum flux mu (grad u + grad u^T) - p I, convection rho (u . grad) u, mass
+    rho, continuity div u -- the terms of pyoomph's NavierStokesEquations
+    (pyoomph/equations/navier_stokes.py:303-347) written in the hand-written kernel's order."""
+    from .mixed import TaylorHoodWeakForm
+
+    def flux(s):
+        mu = s.par[0]
+        F = []
+        for a in range(2):
+            row = []
+            for d in range(2):
+                v = mu * (s.grad_u[a][d] + s.grad_u[d][a])
+                if a == d:
+                    v = v - s.p
+                row.append(v)
+            F.append(row)
+        return F
+
+    def source(s):
+        rho = s.par[1]
+        return [rho * (s.u[0] * s.grad_u[a][0] + s.u[1] * s.grad_u[a][1]) for a in range(2)]
+
+    def continuity(s):
+        return s.grad_u[0][0] + s.grad_u[1][1]
+
+    return TaylorHoodWeakForm("gen_navier_stokes", flux, source, continuity, mass=lambda s: s.par[1], nparams=2)
+
+
+def stokes_brinkman():
+    """A Taylor-Hood form with no hand-written counterpart: Stokes flow with a Brinkman drag,
+    mu (grad u + grad u^T) - p I, drag k u, body force (0, -g), parameters (mu, k, g)."""
+    from .mixed import TaylorHoodWeakForm
+
+    def flux(s):
+        mu = s.par[0]
+        return [[mu * (s.grad_u[a][d] + s.grad_u[d][a]) - (s.p if a == d else 0.0) for d in range(2)]
+                for a in range(2)]
+
+    def source(s):
+        return [s.par[1] * s.u[0], s.par[1] * s.u[1] + s.par[2]]
+
+    def continuity(s):
+        return s.grad_u[0][0] + s.grad_u[1][1]
+
+    return TaylorHoodWeakForm("gen_stokes_brinkman", flux, source, continuity, nparams=3)

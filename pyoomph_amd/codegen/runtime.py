@@ -15,7 +15,7 @@ import numpy as np
 
 from .. import _lib
 from ..equations import Equations
-from . import scalar
+from . import mixed, scalar
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 COMMON_HEADER = os.path.join(os.path.dirname(_HERE), "csrc", "element_common.h")
@@ -39,7 +39,13 @@ def compile_form(form, kind, load=True):
     key = (form.name, kid)
     if load and key in _registry:
         return _registry[key]
-    src, name, info = scalar.emit_kernels(form, kid)
+    th = isinstance(form, mixed.TaylorHoodWeakForm)
+    if th:
+        if kid != scalar.KIND_IDS["quad_q2"]:
+            raise ValueError("Taylor-Hood forms are generated for Q2 quadrilaterals")
+        src, name, info = mixed.emit_th_kernels(form)
+    else:
+        src, name, info = scalar.emit_kernels(form, kid)
     L = _lib.load()
     _signatures(L)
     with open(COMMON_HEADER) as f:
@@ -53,7 +59,11 @@ def compile_form(form, kind, load=True):
                                        ctypes.byref(size)))
     if not load:
         return size.value
+    if th:
+        # velocity_x, velocity_y at every node, pressure at the corners
+        _lib.check(L.pyhip_codegen_set_layout(fid.value, 1))
     # the names its JIT element table lists: the unknown "u", the auxiliary field "f"
+    # (Taylor-Hood tables list velocity and pressure)
     _lib.check(L.pyhip_codegen_set_fields(fid.value, b"u", b"f" if form.aux else None,
                                           int(form.global_param or 0)))
     _registry[key] = fid.value
@@ -61,7 +71,8 @@ def compile_form(form, kind, load=True):
 
 
 class GeneratedEquation(Equations):
-    """A generated scalar weak form on one element family.
+    """A generated weak form on one element family: scalar forms (ScalarWeakForm) on any
+    family, Taylor-Hood forms (mixed.TaylorHoodWeakForm) on Q2 quadrilaterals.
 
     space: "C1" (P1 triangles) or "C2" (P2 triangles, Q2 quadrilaterals, Q2 bricks); the
     element family is the mesh's, known when the problem is built (``check_mesh``).
@@ -71,6 +82,9 @@ class GeneratedEquation(Equations):
     def __init__(self, form, *, space="C2", params=(), aux=None):
         self.gen_form = form
         self.space = space
+        self.taylor_hood = isinstance(form, mixed.TaylorHoodWeakForm)
+        if self.taylor_hood:
+            self.nvalue = 3
         self._params = np.asarray(params, dtype=np.float64).reshape(-1)
         if self._params.size != form.nparams:
             raise ValueError("the form takes %d parameters" % form.nparams)
@@ -83,10 +97,23 @@ class GeneratedEquation(Equations):
 
     def check_mesh(self, mesh):
         super().check_mesh(mesh)
+        if self.taylor_hood and mesh.kind != "quad_q2":
+            raise ValueError("Taylor-Hood needs a Q2 quadrilateral mesh")
         if self._kind is not None and self._kind != scalar.KIND_IDS[mesh.kind]:
             raise ValueError("one GeneratedEquation object per element family")
         self._kind = scalar.KIND_IDS[mesh.kind]
         self.symmetric = self.gen_form.symmetric(scalar.KIND_DIM[self._kind])
+
+    def dummy_values(self, mesh):
+        """Taylor-Hood: the pressure lives on the element vertices; the other nodes carry a
+        pinned dummy (as NavierStokesEquations)."""
+        if not self.taylor_hood:
+            return None
+        vertex = np.zeros(mesh.nnode, dtype=bool)
+        vertex[mesh.elements[:, [0, 2, 6, 8]].reshape(-1)] = True
+        d = np.zeros((mesh.nnode, 3), dtype=bool)
+        d[~vertex, 2] = True
+        return d
 
     @property
     def form(self):

@@ -12,14 +12,14 @@ static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 
 static int form_nslot(int form, int kind) {
-    if (form == PYHIP_FORM_NAVIER_STOKES_TH) return 22;
+    if (th_layout(form)) return 22;
     return elem_nnode(kind);
 }
-static int form_nvalue(int form) { return form == PYHIP_FORM_NAVIER_STOKES_TH ? 3 : 1; }
+static int form_nvalue(int form) { return th_layout(form) ? 3 : 1; }
 
 // (node, value) of each slot (oomph local dof order). Taylor-Hood: see element_kernels.h.
 static void slot_map(int form, int kind, int s, int &node, int &val) {
-    if (form == PYHIP_FORM_NAVIER_STOKES_TH) {
+    if (th_layout(form)) {
         static const int n[22] = {0, 0, 0, 1, 1, 2, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6, 6, 7, 7, 8, 8, 8};
         static const int v[22] = {0, 1, 2, 0, 1, 0, 1, 2, 0, 1, 0, 1, 0, 1, 0, 1, 2, 0, 1, 0, 1, 2};
         node = n[s];
@@ -93,7 +93,7 @@ int pyhip_element_init(int form, int elem_kind, pyhip_element_table *t) {
 
 // face element kind of a bulk element kind
 static int face_kind(int elem_kind, int form) {
-    if (form == PYHIP_FORM_NAVIER_STOKES_TH) return kKindLine3TH;
+    if (th_layout(form)) return kKindLine3TH;
     switch (elem_kind) {
     case PYHIP_ELEM_TRI_P1: return kKindLine2;
     case PYHIP_ELEM_QUAD_Q2: return kKindLine3Q;
@@ -128,7 +128,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         PYHIP_REQUIRE(d.ndof >= 0, "negative ndof");
         PYHIP_REQUIRE(d.nface >= 0 && (d.nface == 0 || (d.face_conn && d.face_coef)),
                       "faces without connectivity or flux coefficients");
-        PYHIP_REQUIRE(d.nface == 0 || d.nvalue == 1 || d.form == PYHIP_FORM_NAVIER_STOKES_TH,
+        PYHIP_REQUIRE(d.nface == 0 || d.nvalue == 1 || th_layout(d.form),
                       "boundary-face flux elements: scalar forms and Taylor-Hood tractions");
         std::pair<int, int> k(d.form, d.elem_kind);
         if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
@@ -227,7 +227,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
             }
             if (d.aux) aux_h[S.node_off + n] = d.aux[n];
         }
-        if (d.form == PYHIP_FORM_NAVIER_STOKES_TH) {
+        if (th_layout(d.form)) {
             // lumped Q1 pressure mass: a quarter of each element's area (corners 0, 2, 8, 6
             // of the 3x3 node block, counter-clockwise) to each corner's pressure unknown
             static const int corner[4] = {0, 2, 8, 6};

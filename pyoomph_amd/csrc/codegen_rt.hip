@@ -19,6 +19,7 @@ struct GenForm {
     bool symmetric = false, has_pderiv = false;
     std::string field = "u", aux_field;   // nodal field names (JIT table)
     int global_param = 0;                 // the parameter dR/dp is taken in
+    int layout = 0;                       // 0 one value per node, 1 Taylor-Hood (22 slots)
     hipModule_t mod = nullptr;
     hipFunction_t f64 = nullptr, f64_tr = nullptr, f32 = nullptr, f32_tr = nullptr;
 };
@@ -94,6 +95,12 @@ bool gen_form_has_pderiv(int form) {
     return k >= 0 && k < (int)g_forms.size() && g_forms[k].has_pderiv;
 }
 
+int gen_form_layout(int form) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    const int k = form - PYHIP_GENERATED_FORM_BASE;
+    return (k >= 0 && k < (int)g_forms.size()) ? g_forms[k].layout : 0;
+}
+
 bool gen_form_fields(int form, std::string &field, std::string &aux_field, int &global_param) {
     std::lock_guard<std::mutex> lock(g_mu);
     const int k = form - PYHIP_GENERATED_FORM_BASE;
@@ -107,16 +114,18 @@ bool gen_form_fields(int form, std::string &field, std::string &aux_field, int &
 template <typename T>
 static int launch_gen(int form, const ElemArgs<T> &a, hipStream_t st) {
     hipFunction_t f;
-    int kind;
+    int kind, layout;
     {
         std::lock_guard<std::mutex> lock(g_mu);
         const GenForm &G = g_forms[form - PYHIP_GENERATED_FORM_BASE];
         const bool f64 = std::is_same<T, double>::value;
         f = f64 ? (a.transient ? G.f64_tr : G.f64) : (a.transient ? G.f32_tr : G.f32);
         kind = G.kind;
+        layout = G.layout;
     }
     PYHIP_REQUIRE(a.hvec == nullptr, "Hessian-vector products are not generated for run-time forms");
-    const int epb = gen_epb(kind);
+    // elements per workgroup: one thread per slot (Taylor-Hood: 22) or per node / point
+    const int epb = layout == 1 ? 256 / 22 : gen_epb(kind);
     const unsigned blocks = (unsigned)((a.e1 - a.e0 + epb - 1) / epb);
     if (a.e1 <= a.e0) return PYHIP_OK;
     ElemArgs<T> arg = a;
@@ -172,6 +181,16 @@ int pyhip_codegen_set_fields(int form_id, const char *field, const char *aux_fie
     g_forms[k].field = field;
     g_forms[k].aux_field = aux_field ? aux_field : "";
     g_forms[k].global_param = global_param;
+    return PYHIP_OK;
+}
+
+int pyhip_codegen_set_layout(int form_id, int layout) {
+    PYHIP_REQUIRE(layout == 0 || layout == 1, "layout must be 0 (one value per node) or 1 (Taylor-Hood)");
+    std::lock_guard<std::mutex> lock(g_mu);
+    const int k = form_id - PYHIP_GENERATED_FORM_BASE;
+    PYHIP_REQUIRE(k >= 0 && k < (int)g_forms.size(), "unknown generated form");
+    PYHIP_REQUIRE(layout == 0 || g_forms[k].kind == ELEM_QUAD_Q2, "the Taylor-Hood layout is for Q2 quadrilaterals");
+    g_forms[k].layout = layout;
     return PYHIP_OK;
 }
 

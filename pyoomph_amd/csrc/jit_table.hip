@@ -29,7 +29,7 @@ constexpr int kMaxSlots = 27;
 
 // (node, value) of slot s; value = the field index in nodal_data
 void jit_slot(int form, int s, int &node, int &val) {
-    if (form == PYHIP_FORM_NAVIER_STOKES_TH) {
+    if (th_layout(form)) {
         static const int n[22] = {0, 0, 0, 1, 1, 2, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6, 6, 7, 7, 8, 8, 8};
         static const int v[22] = {0, 1, 2, 0, 1, 0, 1, 2, 0, 1, 0, 1, 0, 1, 0, 1, 2, 0, 1, 0, 1, 2};
         node = n[s];
@@ -206,7 +206,7 @@ void res_jac(Instance &I, bool steady, const JITElementInfo_t *ei, const JITShap
              double *M, unsigned flag) {
     std::lock_guard<std::mutex> lock(I.mu);
     // time derivatives exist in the Taylor-Hood momentum only (the other forms are steady)
-    const bool tr = !steady && I.form == PYHIP_FORM_NAVIER_STOKES_TH && si && si->timestepper_weights_dt_BDF2;
+    const bool tr = !steady && th_layout(I.form) && si && si->timestepper_weights_dt_BDF2;
     double er[kMaxSlots], ej[kMaxSlots * kMaxSlots];
     int rc = run_element(I, ei, si, flag >= 1 ? 1 : 0, 0, tr, er, ej);
     if (rc) return report(rc);
@@ -390,13 +390,13 @@ int pyhip_jit_element_init(int form, int elem_kind, JITFuncSpec_Table_FiniteElem
     I.form = form;
     I.kind = elem_kind;
     I.nn = elem_nnode(elem_kind);
-    I.ns = form == PYHIP_FORM_NAVIER_STOKES_TH ? 22 : I.nn;
+    I.ns = th_layout(form) ? 22 : I.nn;
     I.table = t;
     I.generated = gen;
     I.has_aux = gen && !aux_field.empty();
     I.pderiv = gen && gen_form_has_pderiv(form);
     I.gparam = gparam;
-    const bool th = form == PYHIP_FORM_NAVIER_STOKES_TH;
+    const bool th = th_layout(form);
     const bool p1 = elem_kind == PYHIP_ELEM_TRI_P1;
     t->nodal_dim = (unsigned)elem_dim(elem_kind);
     t->lagr_dim = 0;

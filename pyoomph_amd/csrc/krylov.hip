@@ -671,7 +671,7 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
         if ((int64_t)p->ilu_last_h.size() == n) is_p = p->ilu_last_h;
         // pressure unknowns: eqn of the pressure slots (2, 7, 16, 21) of Taylor-Hood elements
         for (auto &G : p->groups) {
-            if (G.form != PYHIP_FORM_NAVIER_STOKES_TH) continue;
+            if (!th_layout(G.form)) continue;
             std::vector<int32_t> seqn((size_t)G.ns * G.nel);
             PYHIP_CHECK_HIP(hipMemcpy(seqn.data(), G.seqn, seqn.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
             const int ps[4] = {2, 7, 16, 21};
@@ -839,7 +839,7 @@ static int blk_plan_build(pyhip_problem *p, KrylovWork *W) {
     std::vector<double> pmass;
     B.off_h.assign(ns + 1, 0);
     for (int s = 0; s < ns; s++) {
-        const bool th = p->subs[s].form == PYHIP_FORM_NAVIER_STOKES_TH;
+        const bool th = th_layout(p->subs[s].form);
         for (int64_t r = p->dof_off_h[s]; r < p->dof_off_h[s + 1]; r++) {
             if (th && p->dof_val_h[r] == 2) {
                 PYHIP_REQUIRE(p->pmass_h[r] > 0.0, "block preconditioner: pressure unknown with no element");

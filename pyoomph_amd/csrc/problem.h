@@ -125,6 +125,11 @@ bool gen_form_known(int form);
 int gen_form_kind(int form);
 bool gen_form_has_pderiv(int form);
 bool gen_form_fields(int form, std::string &field, std::string &aux_field, int &global_param);
+int gen_form_layout(int form);
+// the Taylor-Hood value layout (22 slots: ux, uy at the nodes, p at the corners)
+inline bool th_layout(int form) {
+    return form == PYHIP_FORM_NAVIER_STOKES_TH || (form >= PYHIP_GENERATED_FORM_BASE && gen_form_layout(form) == 1);
+}
 int launch_generated_f64(int form, const ElemArgs<double> &a, hipStream_t s);
 int launch_generated_f32(int form, const ElemArgs<float> &a, hipStream_t s);
 int launch_faces_f64(int kind, const ElemArgs<double> &a, hipStream_t s);

@@ -137,6 +137,37 @@ def test_generated_kernels_compile_with_hiprtc(form, kind):
     assert size > 1000
 
 
+@pytest.mark.parametrize("form", [library.navier_stokes(), library.stokes_brinkman()], ids=lambda f: f.name)
+def test_taylor_hood_kernels_compile_with_hiprtc(form):
+    assert runtime.compile_form(form, "quad_q2", load=False) > 1000
+
+
+def test_taylor_hood_form_jacobian_blocks():
+    """The generated Navier-Stokes form's derivatives: the viscous block of the symmetric
+    gradient (2 mu on the own component's direction, mu across), the convection block rho
+    (u . grad psi_j + psi_j d_b u_a), the pressure column -phi d_a psi_i and the continuity
+    row d_b psi_j phi -- checked against sympy at a random point."""
+    from pyoomph_amd.codegen import mixed
+    form = library.navier_stokes()
+    s, F, S, Sp, m = form.build()
+    names = ([s.u[0].val, s.u[1].val, s.p.val] + [g.val for r in s.grad_u for g in r] +
+             [v.val for v in s.par])
+    env = {n: sp.Symbol("v%d" % i) for i, n in enumerate(names)}
+    rng = np.random.default_rng(5)
+    vals = {env[n]: float(rng.uniform(0.2, 0.9)) for n in names}
+    for e in [v for r in F for v in r] + S + [Sp]:
+        se = to_sympy(e, env)
+        for v in list(s.u) + [s.p] + [g for r in s.grad_u for g in r]:
+            mine = float(to_sympy(E.diff(e, v), env).subs(vals))
+            ref = float(sp.diff(se, env[v.val]).subs(vals))
+            assert math.isclose(mine, ref, rel_tol=1e-12, abs_tol=1e-12)
+    src, name, info = mixed.emit_th_kernels(form)
+    assert info["layout"] == 1 and info["nslot"] == 22 and not info["symmetric"]
+    # the residual entries in the hand-written kernel's operation order (element_kernels.h)
+    assert "((T(par[0]) * (g00 + g00)) - pp)" in src
+    assert "(T(par[1]) * ((u0 * g00) + (u1 * g01)))" in src
+
+
 def test_emitted_bratu_matches_the_compiled_operation_order():
     """The generated Bratu residual / Jacobian entries are the compiled kernel's expressions
     (element_kernels.h): W*((g.dxi) + src*P_i) and W*((dxj.dxi) + (dsrc*P_j)*P_i), with

@@ -169,3 +169,99 @@ def test_nonlinear_diffusion_jacobian_is_the_residual_derivative(gpu):
     fd = (rp - rm) / (2 * eps)
     assert np.max(np.abs(J @ d - fd)) <= 1e-6 * np.max(np.abs(fd))
     assert not np.allclose(J.toarray(), J.toarray().T)   # dF/du makes it non-symmetric
+
+
+def _th_pair(N, Re):
+    """(hand-written Navier-Stokes MeshProblem, generated Navier-Stokes MeshProblem)."""
+    from pyoomph_amd.problem import MeshProblem
+    a = wl.lid_driven_cavity(N, Re=Re)
+    mesh = RectangularQuadMesh(N, split_in_tris=False, space="C2")
+    eq = GeneratedEquation(library.navier_stokes(), space="C2", params=[1.0 / Re, 1.0])
+    bcs = [(("top",), {0: 1.0, 1: 0.0}), (("left", "right", "bottom"), {0: 0.0, 1: 0.0})]
+    return a, MeshProblem(mesh, eq, dirichlet=bcs, pin_extra=[(0, 2, 0.0)])
+
+
+def test_generated_taylor_hood_kernel_equals_the_hand_written_one(gpu):
+    """Generated Navier-Stokes (Taylor-Hood layout) against the hand-written kernel: the
+    residual bit for bit (same operations in the same order), the Jacobian to 1e-13 (the
+    generator groups the viscous and convection terms per derivative), steady and with the
+    BDF2 time term; the mass matrix; FP32 within 1e-5."""
+    from pyoomph_amd.problem import Problem
+    mp_a, mp_b = _th_pair(6, 80.0)
+    A, B = Problem(mp_a), Problem(mp_b)
+    assert A.ndof == B.ndof and A.nnz == B.nnz
+    assert np.array_equal(A.structural_pattern()[1], B.structural_pattern()[1])
+    rng = np.random.default_rng(3)
+    for transient in (False, True):
+        x = rng.uniform(-0.4, 0.4, A.ndof)
+        A.set_current_dofs(x)
+        B.set_current_dofs(x)
+        if transient:
+            h1, h2 = (A.torch.tensor(rng.uniform(-0.3, 0.3, A.ndof), device=A.device) for _ in range(2))
+            w = A.bdf_weights(0.05, 0.04)
+            A.set_time_state(w, h1, h2)
+            B.set_time_state(w, h1, h2)
+        ra, va = A.assemble_device("f64")
+        rb, vb = B.assemble_device("f64")
+        ra, rb, va, vb = (t.cpu().numpy() for t in (ra, rb, va, vb))
+        assert np.array_equal(ra, rb), transient
+        assert np.max(np.abs(va - vb)) <= 1e-13 * np.max(np.abs(va))
+        if transient:
+            Ma, Mb = A.assemble_mass_matrix(), B.assemble_mass_matrix()
+            assert np.max(np.abs((Ma - Mb).toarray())) <= 1e-14 * np.max(np.abs(Ma.toarray()))
+            A.set_time_state(None, None, None)
+            B.set_time_state(None, None, None)
+    _, v64 = B.assemble_device("f64")          # steady, at the last dofs
+    r32, v32 = B.assemble_device("f32")
+    v64 = v64.cpu().numpy()
+    assert np.max(np.abs(v32.cpu().numpy().astype(float) - v64)) <= 1e-5 * np.max(np.abs(v64))
+
+
+def test_generated_taylor_hood_newton_reaches_the_oracle(gpu):
+    from pyoomph_amd.problem import Problem
+    _, mp = _th_pair(16, 100.0)
+    P = Problem(mp)
+    P.krylov, P.preconditioner, P.gmres_restart = "gmres", "block", 100
+    P.newton_solver_tolerance, P.gmres_max_iters = 1e-12, 5000
+    st = P.solve()
+    assert st[0]["converged"] and st[0]["linear_failures"] == 0
+    op = oracle_problem("ns", N=16, Re=100.0)
+    xo = on.converged_solve(op, np.zeros(op.ndof), tol=1e-12)
+    assert np.max(np.abs(P.get_current_dofs() - xo)) <= 1e-8 * np.max(np.abs(xo))
+
+
+def test_generated_stokes_brinkman(gpu):
+    """A Taylor-Hood form with no hand-written counterpart: Stokes flow with a Brinkman drag
+    k u and a body force; the problem is linear, so Newton converges at once, the
+    Jacobian equals central differences of the residual, and with no body force and no
+    wall motion the solution is zero."""
+    from pyoomph_amd.problem import MeshProblem, Problem
+    mesh = RectangularQuadMesh(8, split_in_tris=False, space="C2")
+    eq = GeneratedEquation(library.stokes_brinkman(), space="C2", params=[0.3, 2.0, 1.5])
+    walls = [(("left", "right", "bottom", "top"), {0: 0.0, 1: 0.0})]
+    mp = MeshProblem(mesh, eq, dirichlet=walls, pin_extra=[(0, 2, 0.0)])
+    P = Problem(mp)
+    P.krylov, P.preconditioner, P.gmres_restart = "gmres", "ilu0", 200
+    P.newton_solver_tolerance = 1e-11
+    st = P.solve()
+    assert st[0]["converged"] and st[0]["newton_iters"] <= 2
+    rng = np.random.default_rng(6)
+    x = rng.uniform(-0.5, 0.5, P.ndof)
+    P.set_current_dofs(x)
+    R, J = P.assemble_jacobian()
+    d = rng.normal(size=P.ndof)
+    eps = 1e-6
+    P.set_current_dofs(x + eps * d)
+    rp = P.get_residuals()
+    P.set_current_dofs(x - eps * d)
+    rm = P.get_residuals()
+    assert np.max(np.abs(J @ d - (rp - rm) / (2 * eps))) <= 1e-7 * np.max(np.abs(J @ d))
+    # body force g acts on u_y only: the pressure -g y balances it and u = 0 is the discrete
+    # solution (up to the linear solver's tolerance)
+    P.dofs.zero_()
+    P.solve()
+    U = mp.nodal_values(P.get_current_dofs())
+    assert np.max(np.abs(U[:, :2])) <= 1e-8
+    corner = np.isin(np.arange(mesh.nnode), mesh.elements[:, [0, 2, 6, 8]].reshape(-1))
+    y = mesh.coords[corner, 1]
+    assert np.max(np.abs(U[corner, 2] - (-1.5 * y))) <= 1e-7
