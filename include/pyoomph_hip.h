@@ -145,7 +145,9 @@ int pyhip_pattern_copy(const pyhip_problem *p, int32_t *row_start, int32_t *col_
 int pyhip_assemble(pyhip_problem *p, const double *dofs_dev, int flag, int deriv_param,
                    int precision, double *residual_dev, void *values_dev);
 
-/* The Jacobian of the last FP64 flag=1 assembly in the reference's CSR form: exact zeros
+/* The matrix of the last complete FP64 assembly with a matrix output (the Jacobian, the
+ * mass matrix, dJ/dp or a Hessian-vector product; values_dev holds that assembly's values)
+ * in the reference's CSR form: exact zeros
  * dropped per element contribution (problem.cc:5524, Numerical_zero_for_sparse_assembly
  * = 0.0, :110) and columns in first-touch order (problem.cc:5618-5659).
  * row_start_dev [ndof+1], col_dev / val_dev [nnz] (structural nnz is an upper bound).
@@ -185,13 +187,67 @@ int pyhip_assemble_noalloc(pyhip_problem *p, const double *dofs_dev, int flag, i
  * 3646-3910, assembled by src/elements.cpp:4964-4995, flag 1; for the scalar forms the
  * transposed product, flag 4, is the same matrix). Taylor-Hood: the convection block with
  * the velocity replaced by y (the only non-linear term). Several vectors: one call per
- * vector. pyhip_export_reference_csr then gives its reference CSR. */
+ * vector; the transposed product: pyhip_assemble_multi. pyhip_export_reference_csr then
+ * gives its reference CSR. */
 int pyhip_assemble_hessian_vector(pyhip_problem *p, const double *dofs_dev, const double *y_dev,
                                   double *residual_dev, double *values_dev);
 
 int pyhip_export_reference_csr(pyhip_problem *p, const double *values_dev,
                                int32_t *row_start_dev, int32_t *col_dev, double *val_dev,
                                int64_t *nnz_out);
+
+/* ---- multi-assembly -------------------------------------------------------------------------
+ * Several quantities at the same dofs in one call: the reference's
+ * Problem::assemble_multiassembly (src/problem.cpp:2326-2368) with the requests of
+ * CustomMultiAssembleHandler (src/bifurcation.cpp:5354-5480), bound to Python as
+ * _assemble_multiassembly (src/pybind/problem.cpp:631-650) and used through
+ * MultiAssembleRequest (pyoomph/generic/bifurcation_tools.py:442-520). A device problem
+ * has one residual (the reference's contribution ""). Request kinds (the reference's
+ * "what" strings): */
+#define PYHIP_MA_RESIDUALS 0                           /* "residuals"                      */
+#define PYHIP_MA_JACOBIAN 1                            /* "jacobian"                       */
+#define PYHIP_MA_MASS_MATRIX 2                         /* "mass_matrix"                    */
+#define PYHIP_MA_DRESIDUALS_DPARAMETER 3               /* "dresiduals_dparameter"          */
+#define PYHIP_MA_DJACOBIAN_DPARAMETER 4                /* "djacobian_dparameter"           */
+#define PYHIP_MA_DMASS_MATRIX_DPARAMETER 5             /* "dmass_matrix_dparameter"        */
+#define PYHIP_MA_HESSIAN_VECTOR_PRODUCT 6              /* "hessian_vector_product[_transposed]" */
+#define PYHIP_MA_MASS_MATRIX_HESSIAN_VECTOR_PRODUCT 7  /* "mass_matrix_hessian_vector_product[_transposed]" */
+
+typedef struct {
+    int what;        /* PYHIP_MA_* */
+    int param;       /* global parameter index of the three parameter kinds (else ignored) */
+    int hvec;        /* row of the Hessian vectors for the two Hessian kinds (else ignored) */
+    int transposed;  /* Hessian kinds: 1 = entry (i, k) = sum_j y_j d^2 R_j/(du_i du_k)
+                        (jitbridge.h:678-691), 0 = sum_j y_j d^2 R_i/(du_j du_k) */
+} pyhip_multi_request;
+
+/* Numbering of the outputs (CustomMultiAssembleHandler, src/bifurcation.cpp:5393-5480):
+ * return_indices[i] = k >= 0 for the k-th vector, -1 - m for the m-th matrix, counted in
+ * request order. Errors as the reference's: the same quantity requested twice, a parameter
+ * or Hessian vector missing, a Hessian vector index >= nhvec, transposed and plain Hessian
+ * products in one call. Host-only: no device is touched. */
+int pyhip_multi_assembly_plan(int nreq, const pyhip_multi_request *req, int nhvec,
+                              int *return_indices, int *nvector, int *nmatrix);
+
+/* Runs the requests at dofs_dev and returns when the outputs are written.
+ *   hvecs_dev        [nhvec][ndof] the Hessian directions (device)
+ *   vectors          host array [nvector] of device pointers, each [ndof]
+ *   row_start_dev    host array [nmatrix] of device pointers, each [ndof+1]
+ *   col_dev, val_dev host arrays [nmatrix] of device pointers, each [nnz] (structural nnz
+ *                    of the problem is an upper bound)
+ *   nnz_out          host [nmatrix], entries written per matrix
+ * Matrices come as the reference's multi-assembly returns them: entries with at least one
+ * non-zero element contribution, each the element-order sum of those contributions
+ * (src/problem.cpp:2195-2203), columns ascending within a row (one std::map per row,
+ * :2232-2262). The mass matrix is dR/d(du/dt) (flag 2) whether or not time stepping is on;
+ * dM/dp and the mass-matrix Hessian products have no non-zero element contribution for
+ * these forms (their mass coefficients depend neither on the parameter nor on the dofs):
+ * empty matrices. Hessian products of run-time generated forms, and dM/dp of generated
+ * Taylor-Hood forms, are refused with PYHIP_ERR_INVALID. */
+int pyhip_assemble_multi(pyhip_problem *p, const double *dofs_dev, int nreq,
+                         const pyhip_multi_request *req, const double *hvecs_dev, int nhvec,
+                         double *const *vectors, int32_t *const *row_start_dev,
+                         int32_t *const *col_dev, double *const *val_dev, int64_t *nnz_out);
 
 /* ---- linear solver ------------------------------------------------------------------------
  * Preconditioned restarted GMRES on the device, batched over sub-problems (each has its

@@ -260,10 +260,12 @@ class OracleProblem:
         Jacobian: residual, row_start, column_index (first-touch order), values."""
         return self.get_jacobian(x, flag=2)
 
-    def get_hessian_vector_product(self, x, y):
+    def get_hessian_vector_product(self, x, y, transposed=False):
         """sum_k d J / d u_k y_k assembled like the Jacobian (the reference's
-        HessianVectorProduct with one vector): residual, row_start, column_index, values."""
-        return self.get_jacobian(x, y=y)
+        HessianVectorProduct with one vector, flag 1): residual, row_start, column_index,
+        values. transposed: entry (i, k) = sum_j y_j d^2 R_j/(du_i du_k) (flag 4,
+        SET_DIRECTIONAL_SYMMETRIC_HESSIAN_FROM_TRANSPOSED, src/jitbridge.h:678-691)."""
+        return self.get_jacobian(x, flag=4 if transposed else 1, y=y)
 
     def get_jacobian(self, x, deriv=0, flag=1, y=None):
         """oomph Problem::get_jacobian (vectors_of_pairs). Returns residual, row_start,
@@ -287,3 +289,22 @@ class OracleProblem:
         L.oracle_free(ctypes.cast(colp, ctypes.c_void_p))
         L.oracle_free(ctypes.cast(valp, ctypes.c_void_p))
         return res, row_start, col, val
+
+
+def map_order_csr(row_start, col, val):
+    """The rows of a vectors-of-pairs CSR with their columns in ascending order: the form in
+    which Problem::sparse_assemble_row_or_column_compressed_base_problem returns the
+    matrices of a multi-assembly (src/problem.cpp:2232-2262, one std::map<unsigned, double>
+    per row; each value is the same element-order sum of its non-zero element
+    contributions, only the order of a row's entries differs)."""
+    row_start = np.asarray(row_start)
+    col = np.asarray(col)
+    val = np.asarray(val)
+    oc = np.empty_like(col)
+    ov = np.empty_like(val)
+    for r in range(len(row_start) - 1):
+        a, b = row_start[r], row_start[r + 1]
+        order = np.argsort(col[a:b], kind="stable")
+        oc[a:b] = col[a:b][order]
+        ov[a:b] = val[a:b][order]
+    return row_start.copy(), oc, ov

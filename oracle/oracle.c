@@ -118,7 +118,9 @@ static double dot(int dim, const double *a, const double *b)
  * J receives sum_k d^2 R_i/(du_j du_k) Y_k = int d2src (sum_k Y_k psi_k) psi_j psi_i, the
  * product of the reference's HessianVectorProduct (src/codegen.cpp:3646-3910, called from
  * src/elements.cpp:4964-4995 with flag 1); d2src is the second u-derivative of the
- * source term (Bratu -lambda e^u, reaction-diffusion 6u, Poisson 0). */
+ * source term (Bratu -lambda e^u, reaction-diffusion 6u, Poisson 0). flag 4 asks for the
+ * transposed product sum_j Y_j d^2 R_j/(du_i du_k) (jitbridge.h:678-691) = int d2src
+ * (sum_j Y_j psi_j) psi_i psi_k: the same integrand, so the same matrix. */
 static void scalar_element(int form, const ref_t *r, const double *X, const double *U,
                            const double *aux, const long *leqn, const double *par, int npar,
                            int flag, int deriv, int nvar, const double *H, double tw0, const double *Y,
@@ -222,7 +224,10 @@ static const int TH_slot_pidx[22] = {-1, -1, 0, -1, -1, -1, -1, 1, -1, -1, -1, -
  * Hessian-vector product (Y non-NULL, slot values of the vector, flag 1): only the
  * convection term rho (u . grad) u_a is not linear, so J receives its Jacobian with the
  * velocity u replaced by y: rho (psi_j d_c y_a + delta_ac (y . grad psi_j)) psi_i on the
- * velocity-velocity blocks, nothing elsewhere (src/codegen.cpp:3646-3910). */
+ * velocity-velocity blocks, nothing elsewhere (src/codegen.cpp:3646-3910). flag 4 (with Y):
+ * the transposed product, entry (i, k) = sum_j Y_j d^2 R_j/(du_i du_k) (jitbridge.h:678-691),
+ * the Hessian of int rho ((u . grad) u) . y: for row (a, psi_i) and column (c, psi_j)
+ * rho (psi_i d_a psi_j y_c + psi_j d_c psi_i y_a). */
 static void th_element(const ref_t *r, const double *X, const double *V, const long *leqn,
                        const double *par, int flag, int nvar, const double *H, double tw0, const double *Y,
                        double *R, double *J)
@@ -307,7 +312,14 @@ static void th_element(const ref_t *r, const double *X, const double *V, const l
                 if (iu < 0) continue;
                 int ks = TH_slot_kind[ss], ls = TH_slot_node[ss];
                 double v;
-                if (Y) {
+                if (Y && flag == 4) {
+                    if (kt < 2 && ks < 2) {
+                        int a = kt, c = ks;
+                        v = W * (rho * (P[lt] * dx[ls][a] * yv[c] + P[ls] * dx[lt][c] * yv[a]));
+                    } else {
+                        continue;
+                    }
+                } else if (Y) {
                     if (kt < 2 && ks < 2) {
                         int a = kt, c = ks;
                         double cvy = P[ls] * dy[a][c];

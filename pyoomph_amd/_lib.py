@@ -88,6 +88,20 @@ class ArclengthState(ctypes.Structure):
                 ("linear_failed", ctypes.c_int)]
 
 
+class MultiRequest(ctypes.Structure):
+    _fields_ = [("what", ctypes.c_int), ("param", ctypes.c_int), ("hvec", ctypes.c_int),
+                ("transposed", ctypes.c_int)]
+
+
+# request kinds of the multi-assembly (PYHIP_MA_*), by the reference's "what" strings
+MULTI_WHAT = {"residuals": (0, 0), "jacobian": (1, 0), "mass_matrix": (2, 0),
+              "dresiduals_dparameter": (3, 0), "djacobian_dparameter": (4, 0),
+              "dmass_matrix_dparameter": (5, 0), "hessian_vector_product": (6, 0),
+              "hessian_vector_product_transposed": (6, 1),
+              "mass_matrix_hessian_vector_product": (7, 0),
+              "mass_matrix_hessian_vector_product_transposed": (7, 1)}
+
+
 # every symbol declared in include/pyoomph_hip.h, with (restype, argtypes)
 SIGNATURES = {
     "pyhip_last_error": (ctypes.c_char_p, []),
@@ -134,6 +148,14 @@ SIGNATURES = {
                                                      ctypes.c_void_p, ctypes.c_void_p]),
     "pyhip_set_time_stepping": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dp, ctypes.c_void_p,
                                                ctypes.c_void_p]),
+    "pyhip_multi_assembly_plan": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(MultiRequest), ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(ctypes.c_int)]),
+    "pyhip_assemble_multi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.POINTER(MultiRequest), ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+                                            ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+                                            c_i64p]),
     "pyhip_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "pyhip_timing_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, c_i64p,
                                          ctypes.POINTER(ctypes.c_double)]),

@@ -384,7 +384,7 @@ int pyhip_problem_destroy(pyhip_problem *p) {
     void *ptrs[] = {p->coords, p->pinned, p->aux, p->params, p->row_start, p->col, p->diag,
                     p->occ_ptr, p->occ_roff, p->cptr, p->cidx,
                     p->dof_sub, p->dof_off,
-                    p->Er64, p->Ej64, p->Er32, p->Ej32, p->keys, p->kept};
+                    p->Er64, p->Ej64, p->Er32, p->Ej32, p->keys, p->kept, p->ma_vals, p->ma_res};
     for (void *q : ptrs)
         dfree(q);
     for (int k = 0; k < 5; k++) {
@@ -513,6 +513,7 @@ static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int der
                 a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
                 a.hist1 = p->hist1; a.hist2 = p->hist2;
                 a.hvec = p->hvec;
+                a.htr = p->htr;
                 a.fcoef = G.coef;
                 TimeScope ts(&p->timer, "element", p->st);
                 if (G.form == kFormFlux) {
@@ -537,6 +538,7 @@ static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int der
                 a.tw0 = p->tw[0]; a.tw1 = p->tw[1]; a.tw2 = p->tw[2];
                 a.hist1 = p->hist1; a.hist2 = p->hist2;
                 a.hvec = p->hvec;
+                a.htr = p->htr;
                 a.fcoef = G.coef;
                 TimeScope ts(&p->timer, "element", p->st);
                 if (G.form == kFormFlux) {
@@ -644,8 +646,8 @@ int assemble_subset(pyhip_problem *p, const double *dofs, int flag, int deriv_pa
         Ej = p->Ej32;
     }
     if ((rc = assemble_into(p, dofs, flag, deriv_param, precision, residual, values, Er, Ej, sub_on_h))) return rc;
-    // the export reads the element matrices of the last complete FP64 Jacobian
-    if (precision == PYHIP_F64 && flag) p->have_ej64 = (deriv_param < 0) && !sub_on_h;
+    // the export reads the element matrices of the last complete FP64 matrix assembly
+    if (precision == PYHIP_F64 && flag) p->have_ej64 = !sub_on_h;
     return PYHIP_OK;
 }
 }  // namespace pyhip
@@ -704,11 +706,11 @@ int pyhip_set_time_stepping(pyhip_problem *p, int enabled, const double *weights
 int pyhip_export_reference_csr(pyhip_problem *p, const double *values, int32_t *rs_out,
                                int32_t *col_out, double *val_out, int64_t *nnz_out) {
     PYHIP_REQUIRE(p && values && rs_out && col_out && val_out && nnz_out, "null argument");
-    PYHIP_REQUIRE(p->have_ej64 && p->Ej64, "no FP64 Jacobian assembled yet");
+    PYHIP_REQUIRE(p->have_ej64 && p->Ej64, "no complete FP64 matrix assembled yet");
     int rc;
     if (!p->keys) {
         if ((rc = dev_alloc(&p->keys, p->nnz, p->device_bytes))) return rc;
-        if ((rc = dev_alloc(&p->kept, p->ndof + 1, p->device_bytes))) return rc;
+        if ((rc = dev_alloc(&p->kept, export_kept_ints(p->ndof), p->device_bytes))) return rc;
     }
     return export_reference_csr(p->gmap(), p->col, p->Ej64, values, p->keys, p->kept, rs_out,
                                 col_out, val_out, nnz_out, p->st);

@@ -54,8 +54,10 @@ struct ElemArgs {
     int transient;
     double tw0, tw1, tw2;
     const double *hist1, *hist2;
-    // Hessian-vector product: Ej receives sum_k dJ/du_k y_k (y dof-indexed, 0 at pinned)
+    // Hessian-vector product: Ej receives sum_k dJ/du_k y_k (y dof-indexed, 0 at pinned);
+    // htr = 1: the transposed product, entry (i, k) = sum_j y_j d^2 R_j / du_i du_k
     const double *hvec;
+    int htr;
     // boundary faces: [nel][2] alpha, beta of each face's flux alpha*u + beta
     const double *fcoef;
     // elements [e0, e1) of the group are computed by this launch (the slot-major arrays

@@ -48,8 +48,9 @@ __device__ __constant__ const int8_t kTHNodeSlot[9] = {0, 3, 5, 8, 10, 12, 14, 1
 
 // TR: the form has a time derivative (a.transient); HV: the matrix output is the
 // Hessian-vector product sum_k dJ_ij/du_k y_k = int d2src (sum_k y_k psi_k) psi_j psi_i
-// (d2src: Bratu -lambda e^u, reaction-diffusion 6u, Poisson 0). Separate instantiations
-// keep the steady kernels' LDS and registers.
+// (d2src: Bratu -lambda e^u, reaction-diffusion 6u, Poisson 0), symmetric in i, j and k,
+// so the transposed product (a.htr) is the same matrix. Separate instantiations keep the
+// steady kernels' LDS and registers.
 template <typename T, int KIND, int FORM, bool TR, bool HV>
 __global__ __launch_bounds__(256) void scalar_element_kernel(ElemArgs<T> a) {
     using Tr = ElemTraits<KIND>;
@@ -391,12 +392,19 @@ __global__ __launch_bounds__(256) void th_element_kernel(ElemArgs<T> a) {
                 const int ks = kTHSlotKind[ss], ls = kTHSlotNode[ss];
                 T v;
                 if constexpr (HV) {
-                    // d/du of the convection block in the direction y: the block with u -> y
                     if (kt < 2 && ks < 2) {
                         const T *dxs = sDx[le][q][ls];
-                        T cvy = P[ls] * sQ[le][q][QY + 2 + 2 * kt + ks];
-                        if (kt == ks) cvy = cvy + (sQ[le][q][QY] * dxs[0] + sQ[le][q][QY + 1] * dxs[1]);
-                        v = W * ((rho * cvy) * P[lt]);
+                        if (a.htr) {
+                            // Hessian of y . R (row kt,lt; column ks,ls): only the convection
+                            // term is not linear, rho (psi_lt d_kt psi_ls y_ks + psi_ls d_ks psi_lt y_kt)
+                            const T cvt = P[lt] * dxs[kt] * sQ[le][q][QY + ks] + P[ls] * dxl[ks] * sQ[le][q][QY + kt];
+                            v = W * (rho * cvt);
+                        } else {
+                            // d/du of the convection block in the direction y: the block with u -> y
+                            T cvy = P[ls] * sQ[le][q][QY + 2 + 2 * kt + ks];
+                            if (kt == ks) cvy = cvy + (sQ[le][q][QY] * dxs[0] + sQ[le][q][QY + 1] * dxs[1]);
+                            v = W * ((rho * cvy) * P[lt]);
+                        }
                     } else {
                         continue;
                     }

@@ -142,9 +142,13 @@ int launch_gather_residual_f64(int64_t nrow, const int64_t *occ_ptr, const int64
                                const double *Er, double *res, hipStream_t st);
 int launch_gather_residual_f32(int64_t nrow, const int64_t *occ_ptr, const int64_t *occ_roff,
                                const float *Er, double *res, hipStream_t st);
+// rows in first-touch column order, or (sorted) in ascending column order; kept holds
+// export_kept_ints(nrow) ints
 int export_reference_csr(const GatherMap &m, const int32_t *col, const double *Ej,
                          const double *vals, uint32_t *keys, int32_t *kept, int32_t *out_rs,
-                         int32_t *out_col, double *out_val, int64_t *nnz_out, hipStream_t st);
+                         int32_t *out_col, double *out_val, int64_t *nnz_out, hipStream_t st,
+                         bool sorted = false);
+int64_t export_kept_ints(int64_t nrow);
 
 
 }  // namespace pyhip
@@ -177,10 +181,13 @@ struct pyhip_problem {
     // element buffers (lazy)
     double *Er64 = nullptr, *Ej64 = nullptr;
     float *Er32 = nullptr, *Ej32 = nullptr;
-    bool have_ej64 = false;                  // Ej64 holds the last FP64 Jacobian
+    bool have_ej64 = false;                  // Ej64 holds the element matrices of the last complete FP64 matrix assembly
     // export scratch (lazy)
     uint32_t *keys = nullptr;
     int32_t *kept = nullptr;
+    // multi-assembly scratch (lazy): the structural values and the residual of a pass whose
+    // output was not requested
+    double *ma_vals = nullptr, *ma_res = nullptr;
     // Newton / Krylov workspace (lazy)
     pyhip::KrylovWork *kw = nullptr;
     // ILU(0) ordering: rows listed here go after all other rows of their sub-problem
@@ -192,6 +199,7 @@ struct pyhip_problem {
     double tw[3] = {0.0, 0.0, 0.0};
     const double *hist1 = nullptr, *hist2 = nullptr;
     const double *hvec = nullptr;            // set while a Hessian-vector product assembles
+    int htr = 0;                             // ... and 1 for the transposed product
     pyhip::GatherMap gmap() const {
         return pyhip::GatherMap{ndof, nnz, row_start, occ_ptr, occ_roff, cptr, cidx};
     }

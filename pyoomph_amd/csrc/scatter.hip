@@ -16,8 +16,6 @@
 #include "common.h"
 #include "pattern.h"
 
-#include <hipcub/hipcub.hpp>
-
 namespace pyhip {
 
 constexpr int kRowLanes = 32;
@@ -97,12 +95,15 @@ __global__ __launch_bounds__(256) void export_keys_kernel(
     if (lane == 0) kept[r] = n;
 }
 
-// Export, pass 2: place every kept entry at its first-touch rank within the row.
+// Export, pass 2: place every kept entry at its rank within the row: the first-touch rank
+// (keys in element order), or with sorted the rank among the kept entries in column order
+// (the std::map rows of Problem::sparse_assemble_row_or_column_compressed_base_problem,
+// src/problem.cpp:2232-2262).
 __global__ __launch_bounds__(256) void export_place_kernel(
     int64_t nrow, const int32_t *__restrict__ row_start, const int32_t *__restrict__ col,
     const uint32_t *__restrict__ keys, const double *__restrict__ vals,
     const int32_t *__restrict__ new_start, int32_t *__restrict__ out_col,
-    double *__restrict__ out_val) {
+    double *__restrict__ out_val, int sorted) {
     const int g = threadIdx.x / kRowLanes, lane = threadIdx.x % kRowLanes;
     const int64_t r = (int64_t)blockIdx.x * kRowsPerBlock + g;
     if (r >= nrow) return;
@@ -111,10 +112,97 @@ __global__ __launch_bounds__(256) void export_place_kernel(
         const uint32_t k = keys[s + p];
         if (k == UINT32_MAX) continue;
         int rank = 0;
-        for (int q = 0; q < len; q++) rank += (keys[s + q] < k);
+        if (sorted) {
+            for (int q = 0; q < p; q++) rank += (keys[s + q] != UINT32_MAX);
+        } else {
+            for (int q = 0; q < len; q++) rank += (keys[s + q] < k);
+        }
         out_col[new_start[r] + rank] = col[s + p];
         out_val[new_start[r] + rank] = vals[s + p];
     }
+}
+
+// Exclusive prefix sum of n int32 counts (row lengths of the export): per block of
+// kScanItems counts the block total, one block scans the totals, then every block scans
+// its counts from its offset. Three launches, no atomics, no library.
+constexpr int kScanThreads = 256, kScanPer = 8, kScanItems = kScanThreads * kScanPer;
+
+// exclusive scan of one value per thread over the block (Hillis-Steele in LDS)
+__device__ inline int32_t block_exclusive_scan(int32_t v, int32_t *sh, int32_t *total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int32_t add = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    const int32_t incl = sh[t];
+    *total = sh[kScanThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_block_totals_kernel(int64_t n, const int32_t *__restrict__ in,
+                                                                         int32_t *__restrict__ totals) {
+    __shared__ int32_t sh[kScanThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanItems + (int64_t)threadIdx.x * kScanPer;
+    int32_t v = 0;
+    for (int k = 0; k < kScanPer; k++)
+        if (base + k < n) v += in[base + k];
+    int32_t total;
+    block_exclusive_scan(v, sh, &total);
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
+}
+
+// one block: totals[0..nb) -> their exclusive prefix sums, in place
+__global__ __launch_bounds__(kScanThreads) void scan_totals_kernel(int64_t nb, int32_t *__restrict__ totals) {
+    __shared__ int32_t sh[kScanThreads];
+    int32_t carry = 0;
+    for (int64_t c0 = 0; c0 < nb; c0 += kScanThreads) {
+        const int64_t i = c0 + threadIdx.x;
+        const int32_t v = i < nb ? totals[i] : 0;
+        int32_t total;
+        const int32_t ex = block_exclusive_scan(v, sh, &total);
+        if (i < nb) totals[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(int64_t n, const int32_t *__restrict__ in,
+                                                                  const int32_t *__restrict__ offsets,
+                                                                  int32_t *__restrict__ out) {
+    __shared__ int32_t sh[kScanThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanItems + (int64_t)threadIdx.x * kScanPer;
+    int32_t v[kScanPer];
+    int32_t mine = 0;
+    for (int k = 0; k < kScanPer; k++) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        mine += v[k];
+    }
+    int32_t total;
+    int32_t run = offsets[blockIdx.x] + block_exclusive_scan(mine, sh, &total);
+    for (int k = 0; k < kScanPer; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+int64_t export_kept_ints(int64_t nrow) {
+    return nrow + 1 + (nrow + 1 + kScanItems - 1) / kScanItems;
+}
+
+static int exclusive_scan_i32(int64_t n, const int32_t *in, int32_t *totals, int32_t *out, hipStream_t st) {
+    const int64_t nb = (n + kScanItems - 1) / kScanItems;
+    if (nb == 0) return PYHIP_OK;
+    hipLaunchKernelGGL(scan_block_totals_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, st, n, in, totals);
+    PYHIP_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(scan_totals_kernel, dim3(1), dim3(kScanThreads), 0, st, nb, totals);
+    PYHIP_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, st, n, in, totals, out);
+    PYHIP_CHECK_HIP(hipGetLastError());
+    return PYHIP_OK;
 }
 
 template <typename T>
@@ -161,10 +249,11 @@ int launch_gather_residual_f32(int64_t nrow, const int64_t *occ_ptr, const int64
     return PYHIP_OK;
 }
 
-// Reference-form CSR export. Scratch: keys [nnz] int32, kept [nrow+1] int32.
+// Reference-form CSR export. Scratch: keys [nnz] uint32, kept [export_kept_ints(nrow)] int32.
 int export_reference_csr(const GatherMap &m, const int32_t *col, const double *Ej,
                          const double *vals, uint32_t *keys, int32_t *kept, int32_t *out_rs,
-                         int32_t *out_col, double *out_val, int64_t *nnz_out, hipStream_t st) {
+                         int32_t *out_col, double *out_val, int64_t *nnz_out, hipStream_t st,
+                         bool sorted) {
     const int64_t nrow = m.nrow;
     const int32_t *rs = m.rs;
     if (nrow == 0) {
@@ -176,18 +265,13 @@ int export_reference_csr(const GatherMap &m, const int32_t *col, const double *E
     hipLaunchKernelGGL(export_keys_kernel, dim3(blocks), dim3(256), 0, st, nrow, rs, m.cptr,
                        m.cidx, Ej, keys, kept);
     PYHIP_CHECK_HIP(hipGetLastError());
-    // exclusive scan of kept[0..nrow] -> out_rs[0..nrow] (kept[nrow] is a zero pad)
+    // exclusive scan of kept[0..nrow] -> out_rs[0..nrow] (kept[nrow] is a zero pad); the
+    // block totals of the scan follow the pad
     PYHIP_CHECK_HIP(hipMemsetAsync(kept + nrow, 0, sizeof(int32_t), st));
-    size_t tmp_bytes = 0;
-    PYHIP_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, kept, out_rs,
-                                                     (int)(nrow + 1), st));
-    void *tmp = nullptr;
-    PYHIP_CHECK_HIP(hipMallocAsync(&tmp, tmp_bytes, st));
-    PYHIP_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, kept, out_rs,
-                                                     (int)(nrow + 1), st));
-    PYHIP_CHECK_HIP(hipFreeAsync(tmp, st));
+    int rc = exclusive_scan_i32(nrow + 1, kept, kept + nrow + 1, out_rs, st);
+    if (rc) return rc;
     hipLaunchKernelGGL(export_place_kernel, dim3(blocks), dim3(256), 0, st, nrow, rs, col, keys,
-                       vals, out_rs, out_col, out_val);
+                       vals, out_rs, out_col, out_val, sorted ? 1 : 0);
     PYHIP_CHECK_HIP(hipGetLastError());
     int32_t total = 0;
     PYHIP_CHECK_HIP(hipMemcpyAsync(&total, out_rs + nrow, sizeof(int32_t), hipMemcpyDeviceToHost, st));

@@ -14,6 +14,9 @@ Reference interfaces mirrored here:
 * ``Problem.assemble_jacobian()`` (problem.py:711-717): residual vector and the CSR
   Jacobian in oomph's form (first-touch column order, exact zeros dropped, int32 /
   float64), as ``_assemble_residual_jacobian`` returns it (src/pybind/problem.cpp:572-602).
+* ``Problem._assemble_multiassembly`` (src/pybind/problem.cpp:631-639) and
+  ``MultiAssembleRequest`` (pyoomph/generic/bifurcation_tools.py:442-520): several
+  residual-like vectors and matrices at the same dofs in one call.
 * ``get_current_dofs`` / ``set_current_dofs``.
 * Equation numbering: nodes in mesh order, values in value order, pinned values skipped
   (oomph mesh.cc:686-708, nodes.cc:896-927).
@@ -27,7 +30,7 @@ import numpy as np
 from . import _lib
 from .equations import Equations, NavierStokesEquations
 
-__all__ = ["MeshProblem", "Problem", "NewtonSolverError"]
+__all__ = ["MeshProblem", "Problem", "NewtonSolverError", "MultiAssembleRequest"]
 
 
 class NewtonSolverError(RuntimeError):
@@ -265,18 +268,11 @@ class Problem:
         return rs, col[:self.nnz]
 
     def assemble_jacobian(self, dparameter=None):
-        """(residual, scipy.sparse.csr_matrix) in the reference's CSR form."""
+        """(residual, scipy.sparse.csr_matrix) in the reference's CSR form; with
+        ``dparameter`` (dR/dp, dJ/dp), dJ/dp's exact-zero element contributions dropped."""
         import scipy.sparse
-        torch = self.torch
         res, vals = self.assemble_device("f64", dparameter)
-        if dparameter is not None:
-            # the export needs the Jacobian itself: assemble it again (dR/dp kept above)
-            res_d = res
-            res, vals_j = self.assemble_device("f64")
-            rs, col, val = self._export(vals)
-            res = res_d
-        else:
-            rs, col, val = self._export(vals)
+        rs, col, val = self._export(vals)
         R = self._host(res[:self.ndof])
         J = scipy.sparse.csr_matrix((val, col, rs), shape=(self.ndof, self.ndof))
         return R, J
@@ -407,12 +403,16 @@ class Problem:
                 self.set_time_state(*keep)
         return scipy.sparse.csr_matrix((val, col, rs), shape=(self.ndof, self.ndof))
 
-    def assemble_hessian_vector_product(self, y):
+    def assemble_hessian_vector_product(self, y, transposed=False):
         """sum_k dJ/du_k y_k at the current dofs (the reference's HessianVectorProduct) in
-        the reference's CSR form; y of shape [nvec, ndof] gives one matrix per vector."""
+        the reference's CSR form; y of shape [nvec, ndof] gives one matrix per vector.
+        transposed: entry (i, k) = sum_j y_j d^2 R_j/(du_i du_k) (flag 4), in the
+        multi-assembly's CSR form (columns ascending)."""
         y = np.asarray(y, dtype=np.float64)
         if y.ndim == 2:
-            return [self.assemble_hessian_vector_product(v) for v in y]
+            return [self.assemble_hessian_vector_product(v, transposed) for v in y]
+        if transposed:
+            return self.start_multiassembly().dJdU(y, transposed=True).assemble()[0]
         import scipy.sparse
         torch = self.torch
         with torch.cuda.stream(self._stream):
@@ -423,6 +423,86 @@ class Problem:
                                                               _lib.ptr(res), _lib.ptr(vals)))
         rs, col, val = self._export(vals)
         return scipy.sparse.csr_matrix((val, col, rs), shape=(self.ndof, self.ndof))
+
+    # -- multi-assembly -------------------------------------------------------------------
+    def start_multiassembly(self):
+        """A request list run by ``assemble()`` (the reference's
+        AugmentedAssemblyHandler.start_multiassembly, bifurcation_tools.py:579-580)."""
+        return MultiAssembleRequest(self)
+
+    def _assemble_multiassembly(self, what, contributions, params, hessian_vectors, hessian_vector_indices):
+        """The reference's ``_assemble_multiassembly`` (src/pybind/problem.cpp:631-639):
+        returns (ndof, data, csrdata, return_indices) with data = the vectors, then the
+        values of each matrix; csrdata = row_start, column_index of each matrix;
+        return_indices[i] >= 0 a vector, -1 - m the m-th matrix. Parameters and Hessian
+        vector indices are taken in the order of the requests that need them
+        (src/bifurcation.cpp:5354-5392)."""
+        torch = self.torch
+        what = list(what)
+        contributions = list(contributions)
+        if len(what) != len(contributions):
+            raise RuntimeError("Number of what and contributions must match")
+        for c in contributions:
+            if c != "":
+                raise RuntimeError("unknown residual contribution %r: a device problem has one residual" % c)
+        params = list(params)
+        hidx = list(hessian_vector_indices)
+        reqs = (_lib.MultiRequest * max(len(what), 1))()
+        pi = hi = 0
+        for i, w in enumerate(what):
+            if w not in _lib.MULTI_WHAT:
+                raise RuntimeError("Unknown what type '%s'" % w)
+            kind, tr = _lib.MULTI_WHAT[w]
+            reqs[i].what, reqs[i].transposed, reqs[i].param, reqs[i].hvec = kind, tr, -1, -1
+            if kind in (3, 4, 5):
+                if pi >= len(params):
+                    raise RuntimeError("You have not provided enough parameters for the what type '%s'" % w)
+                reqs[i].param = self._param_index(params[pi])
+                pi += 1
+            if kind in (6, 7):
+                if hi >= len(hidx):
+                    raise RuntimeError("You have not provided enough hessian vector indices for the what type '%s'" % w)
+                reqs[i].hvec = int(hidx[hi])
+                hi += 1
+        if pi != len(params):
+            raise RuntimeError("You have provided too many parameters")
+        if hi != len(hidx):
+            raise RuntimeError("You have provided too many hessian vector indices")
+        L = _lib.load()
+        n = len(what)
+        ret = (ctypes.c_int * max(n, 1))()
+        nv, nm = ctypes.c_int(), ctypes.c_int()
+        nh = len(hessian_vectors)
+        _lib.check(L.pyhip_multi_assembly_plan(n, reqs, nh, ret, ctypes.byref(nv), ctypes.byref(nm)))
+        nv, nm = nv.value, nm.value
+        nd = self.ndof
+        with torch.cuda.stream(self._stream):
+            H = None
+            if nh:
+                H = torch.tensor(np.stack([np.asarray(v, dtype=np.float64).reshape(nd) for v in hessian_vectors]),
+                                 device=self.device)
+            vecs = [torch.empty(max(nd, 1), dtype=torch.float64, device=self.device) for _ in range(nv)]
+            rss = [torch.empty(nd + 1, dtype=torch.int32, device=self.device) for _ in range(nm)]
+            cols = [torch.empty(max(self.nnz, 1), dtype=torch.int32, device=self.device) for _ in range(nm)]
+            vals = [torch.empty(max(self.nnz, 1), dtype=torch.float64, device=self.device) for _ in range(nm)]
+
+        def parr(ts):
+            a = (ctypes.c_void_p * max(len(ts), 1))()
+            for k, t in enumerate(ts):
+                a[k] = t.data_ptr()
+            return a
+
+        nnz = (ctypes.c_int64 * max(nm, 1))()
+        _lib.check(L.pyhip_assemble_multi(self._handle, _lib.ptr(self.dofs), n, reqs,
+                                          _lib.ptr(H), nh, parr(vecs), parr(rss), parr(cols), parr(vals), nnz))
+        data = [self._host(v[:nd]) for v in vecs]
+        csrdata = []
+        for m in range(nm):
+            k = nnz[m]
+            data.append(self._host(vals[m][:k]))
+            csrdata.append(self._host(rss[m]))
+            csrdata.append(self._host(cols[m][:k]))
+        return nd, data, csrdata, [ret[i] for i in range(n)]
 
     # -- arclength continuation -----------------------------------------------------------
     _arc = None
@@ -531,3 +611,73 @@ class Problem:
             return out
         _lib.check(rc)
         return out
+
+
+class MultiAssembleRequest:
+    """The reference's MultiAssembleRequest (pyoomph/generic/bifurcation_tools.py:442-520) on
+    a device ``Problem``: R, J, M, dRdp, dJdp, dMdp, dJdU, dMdU append requests (returning
+    the request, so calls chain), ``assemble()`` runs them in one call and returns numpy
+    vectors and scipy CSR matrices in request order. A Hessian vector passed twice (the same
+    object) is sent once."""
+
+    def __init__(self, problem):
+        self._what = []
+        self._contributions = []
+        self._hessian_vectors = []
+        self._hessian_vector_indices = []
+        self._parameters = []
+        self.problem = problem
+
+    def _resolve_hessian_vector_index(self, V):
+        for i, w in enumerate(self._hessian_vectors):
+            if V is w:
+                return i
+        self._hessian_vectors.append(V)
+        return len(self._hessian_vectors) - 1
+
+    def _add(self, what, contribution):
+        self._what.append(what)
+        self._contributions.append(contribution)
+        return self
+
+    def R(self, contribution=""):
+        return self._add("residuals", contribution)
+
+    def J(self, contribution=""):
+        return self._add("jacobian", contribution)
+
+    def M(self, contribution=""):
+        return self._add("mass_matrix", contribution)
+
+    def dRdp(self, parameter, contribution=""):
+        self._parameters.append(parameter)
+        return self._add("dresiduals_dparameter", contribution)
+
+    def dJdp(self, parameter, contribution=""):
+        self._parameters.append(parameter)
+        return self._add("djacobian_dparameter", contribution)
+
+    def dMdp(self, parameter, contribution=""):
+        self._parameters.append(parameter)
+        return self._add("dmass_matrix_dparameter", contribution)
+
+    def dJdU(self, vector, contribution="", transposed=False):
+        self._hessian_vector_indices.append(self._resolve_hessian_vector_index(vector))
+        return self._add("hessian_vector_product_transposed" if transposed else "hessian_vector_product",
+                         contribution)
+
+    def dMdU(self, vector, contribution="", transposed=False):
+        self._hessian_vector_indices.append(self._resolve_hessian_vector_index(vector))
+        return self._add("mass_matrix_hessian_vector_product_transposed" if transposed
+                         else "mass_matrix_hessian_vector_product", contribution)
+
+    def assemble(self):
+        import scipy.sparse
+        n, vectors, csrdatas, return_indices = self.problem._assemble_multiassembly(
+            self._what, self._contributions, self._parameters, self._hessian_vectors,
+            self._hessian_vector_indices)
+        nmatrix = len(csrdatas) // 2
+        nvectors = len(vectors) - nmatrix
+        matrices = [scipy.sparse.csr_matrix((vectors[nvectors + i], csrdatas[2 * i + 1], csrdatas[2 * i]),
+                                            shape=(n, n)) for i in range(nmatrix)]
+        return [vectors[r] if r >= 0 else matrices[-(r + 1)] for r in return_indices]

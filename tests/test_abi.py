@@ -56,3 +56,56 @@ def test_default_options():
     _lib.load().pyhip_newton_opts_default(ctypes.byref(o))
     assert o.tol == 1e-8 and o.max_iter == 10 and o.max_residuals == 1e10
     assert o.always_take_one_newton_step == 1 and o.relaxation == 1.0
+
+
+def _plan(reqs, nhvec=0):
+    """pyhip_multi_assembly_plan over (what, param, hvec, transposed) tuples."""
+    L = _lib.load()
+    arr = (_lib.MultiRequest * max(len(reqs), 1))()
+    for i, (w, p, h, t) in enumerate(reqs):
+        arr[i].what, arr[i].param, arr[i].hvec, arr[i].transposed = w, p, h, t
+    ret = (ctypes.c_int * max(len(reqs), 1))()
+    nv, nm = ctypes.c_int(), ctypes.c_int()
+    rc = L.pyhip_multi_assembly_plan(len(reqs), arr, nhvec, ret, ctypes.byref(nv), ctypes.byref(nm))
+    if rc:
+        return rc, L.pyhip_last_error().decode()
+    return [ret[i] for i in range(len(reqs))], (nv.value, nm.value)
+
+
+def test_multi_assembly_numbering():
+    """Vectors count 0, 1, ..., matrices -1, -2, ... in request order, as
+    CustomMultiAssembleHandler numbers one contribution (src/bifurcation.cpp:5393-5480),
+    e.g. the Hopf tracker's R, dR/dp, J, J, M request (bifurcation_tools.py:1103)."""
+    W = _lib.MULTI_WHAT
+    reqs = [(W["residuals"][0], -1, -1, 0), (W["dresiduals_dparameter"][0], 0, -1, 0),
+            (W["jacobian"][0], -1, -1, 0), (W["mass_matrix"][0], -1, -1, 0),
+            (W["hessian_vector_product"][0], -1, 0, 0), (W["djacobian_dparameter"][0], 0, -1, 0),
+            (W["mass_matrix_hessian_vector_product"][0], -1, 0, 0),
+            (W["hessian_vector_product"][0], -1, 1, 0)]
+    ret, counts = _plan(reqs, nhvec=2)
+    assert ret == [0, 1, -1, -2, -3, -4, -5, -6] and counts == (2, 6)
+    assert _plan([]) == ([], (0, 0))
+
+
+@pytest.mark.parametrize("reqs,nh,msg", [
+    ([(0, -1, -1, 0), (0, -1, -1, 0)], 0, "multiple 'residuals'"),
+    ([(1, -1, -1, 0), (2, -1, -1, 0), (1, -1, -1, 0)], 0, "multiple 'jacobian'"),
+    ([(4, 0, -1, 0), (4, 0, -1, 0)], 0, "multiple 'djacobian_dparameter'"),
+    ([(6, -1, 0, 0), (6, -1, 0, 0)], 1, "multiple 'hessian_vector_product'"),
+    ([(6, -1, 0, 0), (6, -1, 1, 1)], 2, "transposed and non-transposed"),
+    ([(6, -1, 2, 0)], 2, "out of bounds"),
+    ([(3, -1, -1, 0)], 0, "no parameter"),
+    ([(7, -1, -1, 0)], 1, "no Hessian vector"),
+    ([(9, -1, -1, 0)], 0, "unknown multi-assembly request"),
+])
+def test_multi_assembly_request_errors(reqs, nh, msg):
+    rc, err = _plan(reqs, nh)
+    assert rc == _lib.PYHIP_ERR_INVALID and msg in err
+
+
+def test_multi_assembly_distinct_requests_pass():
+    """The same kind with another parameter, vector or transposition is a new quantity."""
+    ret, counts = _plan([(6, -1, 0, 0), (6, -1, 1, 0), (7, -1, 0, 0)], 2)
+    assert ret == [-1, -2, -3] and counts == (0, 3)
+    ret, counts = _plan([(6, -1, 0, 1), (7, -1, 0, 1)], 1)
+    assert ret == [-1, -2]
