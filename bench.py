@@ -86,6 +86,12 @@ def parse():
     ap.add_argument("--continuation", type=int, default=0, metavar="STEPS",
                     help="instead of the steady sweep: STEPS pseudo-arclength steps in lambda (ds 0.5, "
                          "max 0.3) of every problem of the sweep from lambda = 0, u = 0 (GMRES + --precond)")
+    ap.add_argument("--assembly-study", default=None, metavar="LIB",
+                    help="compare the FP64 assembly of this build with LIB, another build of the library "
+                         "(e.g. make -C pyoomph_amd/csrc EXACT=-ffp-contract=fast OUT=...): element-kernel "
+                         "time and the Jacobian's zero-drop decisions and values on the Poisson 64x64 P2 "
+                         "problem and the Bratu sweep")
+    ap.add_argument("--assembly-study-worker", default=None, metavar="DIR", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB read + 1 GiB write streaming kernel (three launches), the "
                          "reference for scaling HBM byte counters")
@@ -93,8 +99,9 @@ def parse():
 
 
 def spmv_bytes(P):
-    """Algorithmic bytes of one FP64 CSR SpMV over all rows: values + column indices,
-    row starts, x (once per dof) and y."""
+    """Algorithmic bytes of one FP64 CSR SpMV over all rows with int32 column indices:
+    values + column indices, row starts, x (once per dof) and y (the library counts its own
+    launches, with 2-byte column offsets where it uses them)."""
     return 12 * P.nnz + 4 * (P.ndof + 1) + 8 * P.ndof + 8 * P.ndof
 
 
@@ -443,9 +450,94 @@ def continuation_main(args):
     print(json.dumps(out), flush=True)
 
 
+def assembly_study_worker(args):
+    """One build's side of the assembly study, in a process of its own (the library comes
+    from PYOOMPH_HIP_LIB when set). Per case: the element kernels' time per FP64 Jacobian
+    assembly (mean of 5) and the Jacobian in the multi-assembly CSR (columns ascending,
+    exact-zero element contributions dropped). The first build saves its kept-entry mask and
+    values under DIR; a later build compares its own with them."""
+    import torch
+    from pyoomph_amd import _lib, workloads as wl
+    from pyoomph_amd.problem import Problem
+    torch.cuda.set_device(0)
+    out = args.assembly_study_worker
+    ref = os.path.exists(os.path.join(out, "poisson64_mask.npy"))
+    cases = [("poisson64", lambda: [wl.poisson_gaussians(64, 0)]),
+             ("bratu_sweep", lambda: [wl.bratu(N, sp, lam) for N, sp, lam in wl.bratu_sweep_specs(512, seed=0)])]
+    L = _lib.load()
+    for name, parts in cases:
+        P = Problem(parts())
+        P.set_current_dofs(np.random.default_rng(1).uniform(0.0, 0.5, P.ndof))
+        P.assemble_device("f64")
+        _lib.check(L.pyhip_timing_enable(P._handle, 1))
+        for _ in range(5):
+            P.assemble_device("f64")
+        el = timing(P, "element")
+        _lib.check(L.pyhip_timing_enable(P._handle, 0))
+        J, = P.start_multiassembly().J().assemble()
+        rs, col = P.structural_pattern()
+        rows = np.repeat(np.arange(P.ndof, dtype=np.int64), np.diff(rs))
+        jrows = np.repeat(np.arange(P.ndof, dtype=np.int64), np.diff(J.indptr))
+        mask = np.isin(rows * P.ndof + col, jrows * P.ndof + J.indices)
+        vals = np.zeros(P.nnz)
+        vals[mask] = J.data
+        r = {"case": name, "lib": _lib.LIB_PATH, "ndof": P.ndof, "structural_nnz": int(P.nnz),
+             "kept_nnz": int(J.nnz), "element_launches_per_assembly": el[0] / 5, "element_ms_per_assembly": el[1] / 5}
+        if not ref:
+            np.save(os.path.join(out, name + "_mask.npy"), np.packbits(mask))
+            np.save(os.path.join(out, name + "_vals.npy"), vals)
+        else:
+            m0 = np.unpackbits(np.load(os.path.join(out, name + "_mask.npy")))[:P.nnz].astype(bool)
+            v0 = np.load(os.path.join(out, name + "_vals.npy"), mmap_mode="r")
+            both = m0 & mask
+            d = np.abs(vals[both] - v0[both])
+            r.update({"decisions_changed": int(np.count_nonzero(m0 != mask)),
+                      "kept_here_only": int(np.count_nonzero(mask & ~m0)),
+                      "kept_there_only": int(np.count_nonzero(m0 & ~mask)),
+                      "values_differing": int(np.count_nonzero(d)),
+                      "max_rel_diff": float(d.max() / max(np.abs(v0[both]).max(), 1e-300)) if d.size else 0.0})
+        print(json.dumps(r), flush=True)
+        del P
+
+
+def assembly_study(args):
+    """Runs the worker on the in-tree build, then on args.assembly_study, and prints one
+    JSON line: element-kernel time of each and what changed in the Jacobian."""
+    import subprocess
+    import tempfile
+    lines = []
+    with tempfile.TemporaryDirectory() as d:
+        for lib in (None, args.assembly_study):
+            env = dict(os.environ)
+            env.pop("PYOOMPH_HIP_LIB", None)
+            if lib:
+                env["PYOOMPH_HIP_LIB"] = os.path.abspath(lib)
+            print("# assembly study: %s" % (lib or "in-tree build"), file=sys.stderr, flush=True)
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--assembly-study-worker", d],
+                               env=env, stdout=subprocess.PIPE, text=True)
+            if r.returncode:
+                sys.exit(r.returncode)
+            lines.append([json.loads(x) for x in r.stdout.splitlines() if x.startswith("{")])
+    out = {"study": "FP64 assembly: in-tree build vs %s" % args.assembly_study, "cases": []}
+    for a, b in zip(*lines):
+        out["cases"].append({"case": a["case"], "ndof": a["ndof"], "structural_nnz": a["structural_nnz"],
+                             "kept_nnz": [a["kept_nnz"], b["kept_nnz"]],
+                             "element_ms_per_assembly": [a["element_ms_per_assembly"], b["element_ms_per_assembly"]],
+                             "decisions_changed": b["decisions_changed"], "kept_by_other_only": b["kept_here_only"],
+                             "kept_by_in_tree_only": b["kept_there_only"], "values_differing": b["values_differing"],
+                             "max_rel_diff": b["max_rel_diff"]})
+    print(json.dumps(out), flush=True)
+
+
 def main():
     global KRYLOV_RTOL
     args = parse()
+    if args.assembly_study_worker:
+        assembly_study_worker(args)
+        return
+    if args.assembly_study:
+        assembly_study(args)
+        return
     if args.krylov_rtol is not None:
         KRYLOV_RTOL = args.krylov_rtol
     if args.continuation:

@@ -1,13 +1,15 @@
 """ctypes binding of libpyoomph_hip.so (the C ABI declared in include/pyoomph_hip.h).
 
 The library is built in-tree (``pyoomph_amd/libpyoomph_hip.so``). Loading fails loudly
-when it is missing: there is no CPU fallback.
+when it is missing: there is no CPU fallback. ``PYOOMPH_HIP_LIB`` names another build of
+the same sources instead (bench.py's assembly study loads one compiled with FP
+contraction this way, in a process of its own).
 """
 import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpyoomph_hip.so")
+LIB_PATH = os.environ.get("PYOOMPH_HIP_LIB") or os.path.join(_HERE, "libpyoomph_hip.so")
 
 PYHIP_OK = 0
 PYHIP_ERR_INVALID = -1

@@ -173,3 +173,30 @@ def test_each_sub_problem_gets_its_own_hierarchy():
             r0, r1 = (int(nxt.min()), int(nxt.max()) + 1) if len(nxt) else (0, 0)
         if len(alone) == len(batch):
             assert boff[s + 1] - boff[s] == aoff[1] - aoff[0]
+
+
+def test_tiny_sub_problems_leave_empty_prolongation_blocks():
+    """The batch of test_gpu_configs.test_multigrid_batch_with_empty_prolongation_blocks:
+    the finest prolongation has 256-row blocks without entries, starting at an odd entry:
+    every spmv variant must leave such a block without reading any entry."""
+    from pyoomph_amd import workloads as wl
+    specs = [(40, "C1", 1.0)] + [(7, "C1", 1.0 + 0.01 * i) for i in range(40)] + [(40, "C2", 1.2)]
+    rows, cols, off = [], [], [0]
+    for N, space, lam in specs:
+        mp = wl.bratu(N, space, lam)
+        E = mp.eqn[:, 0][mp.mesh.elements]
+        k = E.shape[1]
+        r, c = np.repeat(E, k, axis=1).ravel(), np.tile(E, (1, k)).ravel()
+        m = (r >= 0) & (c >= 0)
+        rows.append(r[m] + off[-1])
+        cols.append(c[m] + off[-1])
+        off.append(off[-1] + mp.ndof)
+    n = off[-1]
+    A = sp.csr_matrix((np.ones(sum(len(x) for x in rows)), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(n, n))
+    A.sort_indices()
+    levels, _ = _plan(A.indptr, A.indices, off, cap=16, bottom=48, maxlev=16)
+    prs = levels[0]["prs"]
+    empty_odd = [b for b in range((n + 255) // 256)
+                 if prs[b * 256] == prs[min(n, (b + 1) * 256)] and prs[b * 256] % 2 == 1]
+    assert len(empty_odd) >= 1

@@ -68,6 +68,25 @@ def test_bratu_sweep_slice_batch_matches_oracle(gpu):
         assert rel(P.sub_dofs(i, x), xo) <= 1e-8, (idx[i], N, sp, lam)
 
 
+def test_multigrid_batch_with_empty_prolongation_blocks(gpu):
+    """Forty tiny problems between two larger ones: their hierarchies end on the finest
+    level, so the finest prolongation has 256-row blocks without entries, starting at an
+    odd entry (checked on the host plan in test_amg_plan). Every spmv variant runs on that
+    matrix when the level's launches are tuned (a variant that read entries of such a block
+    would fault); the batch still matches the oracle."""
+    from pyoomph_amd.problem import Problem
+    specs = [(40, "C1", 1.0)] + [(7, "C1", 1.0 + 0.01 * i) for i in range(40)] + [(40, "C2", 1.2)]
+    P = Problem([wl.bratu(*s) for s in specs])
+    P.krylov, P.preconditioner = "cg", "amg"
+    st = P.solve()
+    x = P.get_current_dofs()
+    for i, (N, sp, lam) in enumerate(specs):
+        assert st[i]["converged"] and st[i]["linear_failures"] == 0
+        if i in (0, 1, 20, 40, 41):
+            op = oracle_problem("bratu", N=N, space=sp, lam=lam)
+            assert rel(P.sub_dofs(i, x), on.newton_solve(op, np.zeros(op.ndof))) <= 1e-8, (i, N, sp)
+
+
 def test_reaction_diffusion_mixed_path_matches_oracle(gpu):
     from pyoomph_amd.problem import Problem
     N = 10
