@@ -107,6 +107,20 @@ typedef struct {
                                     face's 3x3 node lattice) mesh nodes of each face     */
     const double *face_coef;     /* [nface][2] alpha, beta of each face's flux (Taylor-
                                     Hood: [nface][3] alpha, beta_x, beta_y)               */
+    /* Constraints enforced by Lagrange multipliers: pyoomph's EnforcedBC(u=constraint) @
+     * boundary (pyoomph/meshes/bcs.py:129-253) with constraint = alpha*u + beta per face,
+     * for the scalar forms on P1 / P2 triangles and Q2 quadrilaterals. The multiplier field
+     * l lives on the boundary in u's space as value 1 of the nodes (nvalue = 2: u, l; eqn
+     * and pinned_values hold both, l pinned to 0 where u is pinned and on nodes off the
+     * boundary, as EnforcedBC pins redundant multipliers, bcs.py:199-243). Each face adds
+     * the residuals weak(constraint, l_test) and weak(l, u_test) (bcs.py:185-189):
+     * R[u_i] += int l psi_i ds, R[l_i] += int (alpha u + beta) psi_i ds, on the face
+     * elements of the flux faces above, slots (u, l) per face node. The Jacobian is a
+     * saddle point (zero l-l block): GMRES with ILU(0), the multipliers ordered last.
+     * The faces follow the flux faces in element order. nenforced = 0: none. */
+    int64_t nenforced;
+    const int32_t *enforced_conn; /* [nenforced][2|3] mesh nodes of each face          */
+    const double *enforced_coef;  /* [nenforced][2] alpha, beta of each face's constraint */
 } pyhip_subproblem_desc;
 
 typedef struct pyhip_problem pyhip_problem;

@@ -84,10 +84,13 @@ class OracleProblem:
                  flux face_alpha*u + face_beta (``oracle/faces.py``; scalars, or one value
                  per face), or None; the face elements follow the bulk elements in
                  element order
+    enforced  -- [nenf, 2|3] boundary-face nodes of EnforcedBC(u=enf_alpha*u + enf_beta)
+                 (``faces.enforced_face_outputs``), or None; the scalar forms then have
+                 nvalue 2 (u, the multiplier); these faces follow the flux faces
     """
 
     def __init__(self, mesh, form, nvalue, pinned, bc, params, aux=None, faces=None, face_alpha=0.0,
-                 face_beta=0.0):
+                 face_beta=0.0, enforced=None, enf_alpha=1.0, enf_beta=0.0):
         from .meshes import number_equations
         self.mesh, self.form, self.nvalue = mesh, form, nvalue
         self.pinned = np.asarray(pinned, dtype=bool)
@@ -112,18 +115,20 @@ class OracleProblem:
         free = g >= 0
         self.leqn[free] = (np.cumsum(free, axis=1) - 1)[free]
         self.nvar = free.sum(axis=1).astype(np.int64)
-        self.maxdof = ns
-        self.geqn = np.full((E.shape[0], ns), -1, dtype=np.int64)
+        self.nslot = ns
+        nenf_slots = 0 if enforced is None or not len(enforced) else 2 * np.asarray(enforced).shape[1]
+        self.maxdof = max(ns, nenf_slots)
+        self.geqn = np.full((E.shape[0], self.maxdof), -1, dtype=np.int64)
         rows = np.nonzero(free)
         self.geqn[rows[0], self.leqn[rows]] = g[rows]
         self.faces = None
         if faces is not None and len(faces):
             nf = len(faces)
-            nc = 1 if nvalue == 1 else 2          # velocity components of a Taylor-Hood traction
+            nc = 2 if form == FORM_NS else 1      # velocity components of a Taylor-Hood traction
             self.face_alpha = np.broadcast_to(np.asarray(face_alpha, dtype=np.float64), (nf,)).copy()
             fb = np.asarray(face_beta, dtype=np.float64)
             self.face_beta = np.broadcast_to(fb.reshape(-1, nc) if fb.ndim else fb, (nf, nc)).copy()
-            if nvalue not in (1, 3):
+            if not (nvalue in (1, 2) and form != FORM_NS) and not (nvalue == 3 and form == FORM_NS):
                 raise ValueError("boundary-face fluxes are built for the scalar forms and Taylor-Hood")
             self.faces = np.asarray(faces, dtype=np.int64)
             # face slots: (node l, component c) = l*nc + c
@@ -132,11 +137,29 @@ class OracleProblem:
             self.face_leqn = np.full(fg.shape, -1, dtype=np.int64)
             self.face_leqn[ffree] = (np.cumsum(ffree, axis=1) - 1)[ffree]
             fnv = ffree.sum(axis=1).astype(np.int64)
-            fgeqn = np.full((fg.shape[0], ns), -1, dtype=np.int64)
+            fgeqn = np.full((fg.shape[0], self.maxdof), -1, dtype=np.int64)
             r = np.nonzero(ffree)
             fgeqn[r[0], self.face_leqn[r]] = fg[r]
             self.nvar = np.concatenate([self.nvar, fnv])
             self.geqn = np.concatenate([self.geqn, fgeqn])
+        self.enforced = None
+        if enforced is not None and len(enforced):
+            if nvalue != 2 or form == FORM_NS:
+                raise ValueError("EnforcedBC multipliers: scalar forms with nvalue 2 (u, multiplier)")
+            self.enforced = np.asarray(enforced, dtype=np.int64)
+            ne, enn = self.enforced.shape
+            self.enf_alpha = np.broadcast_to(np.asarray(enf_alpha, dtype=np.float64), (ne,)).copy()
+            self.enf_beta = np.broadcast_to(np.asarray(enf_beta, dtype=np.float64), (ne,)).copy()
+            # slots (node l, value c) = 2l + c: u and the multiplier
+            eg = self.eqn[np.repeat(self.enforced, 2, axis=1), np.tile(np.arange(2), enn)]
+            efree = eg >= 0
+            self.enf_leqn = np.full(eg.shape, -1, dtype=np.int64)
+            self.enf_leqn[efree] = (np.cumsum(efree, axis=1) - 1)[efree]
+            egeqn = np.full((ne, self.maxdof), -1, dtype=np.int64)
+            r = np.nonzero(efree)
+            egeqn[r[0], self.enf_leqn[r]] = eg[r]
+            self.nvar = np.concatenate([self.nvar, efree.sum(axis=1).astype(np.int64)])
+            self.geqn = np.concatenate([self.geqn, egeqn])
 
     # -- time stepping (scalar forms) --------------------------------------------------
     # du/dt = tw[0] u + (tw[1] u(t-1) + tw[2] u(t-2)), nodal history combined per node;
@@ -194,14 +217,47 @@ class OracleProblem:
         rc = lib().oracle_elements(
             self.form, nel, dim, nn, len(tab["w"]), 4 if psi1 is not None else 0,
             _dp(np.ascontiguousarray(tab["w"])), _dp(np.ascontiguousarray(tab["psi"])),
-            _dp(np.ascontiguousarray(tab["dpsi"])), _dp(psi1), self.maxdof, _dp(X), _dp(V),
+            _dp(np.ascontiguousarray(tab["dpsi"])), _dp(psi1), self.nslot, _dp(X), _dp(V),
             _dp(aux), _lp(np.ascontiguousarray(self.leqn)), _lp(self.nvar), _dp(self.params),
             len(self.params), int(flag), int(deriv), self.maxdof, _dp(H), tw0, _dp(Y), _dp(R), _dp(J))
         if rc != 0:
             raise RuntimeError("oracle_elements failed: %d" % rc)
         if self.faces is not None:
             R, J = self._append_faces(U, R, J, flag, deriv, y)
+        if self.enforced is not None:
+            R, J = self._append_enforced(U, R, J, flag, deriv, y)
         return R, J
+
+    def _append_enforced(self, U, R, J, flag, deriv, y):
+        """EnforcedBC face blocks after the flux faces, in the compact local numbering."""
+        from .faces import enforced_face_outputs
+        F = self.enforced
+        nf, nn = F.shape
+        quad = self.mesh.kind == "quad_q2"
+        Xf = np.ascontiguousarray(self.mesh.coords[F][:, :, :2])
+        Rs, Js = enforced_face_outputs(Xf, np.ascontiguousarray(U[F, 0]), np.ascontiguousarray(U[F, 1]),
+                                       self.enf_alpha, self.enf_beta, 1 if flag == 1 else 0, quad=quad)
+        if deriv:
+            Rs[:] = 0.0
+        if deriv or flag == 2 or y is not None:
+            Js[:] = 0.0
+        md = self.maxdof
+        Rf = np.zeros((nf, md))
+        Jf = np.zeros((nf, md * md))
+        nel = R.shape[0]
+        for f in range(nf):
+            loc = self.enf_leqn[f]
+            fr = np.nonzero(loc >= 0)[0]
+            nv = len(fr)
+            Rf[f, loc[fr]] = Rs[f, fr]
+            blk = np.zeros((nv, nv))
+            blk[np.ix_(loc[fr], loc[fr])] = Js[f][np.ix_(fr, fr)]
+            Jf[f, :nv * nv] = blk.ravel()
+            assert nv == self.nvar[nel + f]
+        R = np.concatenate([R, Rf])
+        if J is not None:
+            J = np.concatenate([J, Jf.reshape(nf, md, md)])
+        return np.ascontiguousarray(R), None if J is None else np.ascontiguousarray(J)
 
     def _append_faces(self, U, R, J, flag, deriv, y):
         """Face element blocks after the bulk ones, in the same compact local numbering

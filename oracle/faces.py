@@ -150,3 +150,39 @@ def face_outputs(X, U, alpha, beta, flag, quad=False):
                 for j in range(nn):
                     J[:, i, j] = J[:, i, j] + W * ((alpha * psi[j]) * psi[i])
     return R, J
+
+
+def enforced_face_outputs(X, U, Lm, alpha, beta, flag, quad=False):
+    """Faces of pyoomph's EnforcedBC(u=alpha*u + beta) @ boundary (pyoomph/meshes/bcs.py:
+    129-253): the Lagrange multiplier l (value 1 of the face nodes, u's space) adds the
+    residuals weak(alpha*u + beta, l_test) and weak(l, u_test) (bcs.py:185-189). Element
+    residuals [nf, 2nn] and Jacobians [nf, 2nn, 2nn] in the slot order (u_0, l_0, u_1, l_1,
+    ...) of the face nodes' values; X [nf, nn, 2], U / Lm [nf, nn] nodal u and l, alpha /
+    beta scalars or one value per face; the rules and geometry of face_outputs. Per point:
+    R[u_i] += W (l psi_i), R[l_i] += W ((alpha u + beta) psi_i), J[u_i][l_j] += W (psi_j
+    psi_i), J[l_i][u_j] += W ((alpha psi_j) psi_i); the u-u and l-l blocks stay zero."""
+    nf, nn = U.shape
+    knots, weights = _RULE_Q3 if quad else _RULES[nn]
+    R = np.zeros((nf, 2 * nn))
+    J = np.zeros((nf, 2 * nn, 2 * nn))
+    for g in range(nn):
+        psi, dpsi = shape(nn, knots[g], quad)
+        tx = np.zeros(nf)
+        ty = np.zeros(nf)
+        u = np.zeros(nf)
+        lm = np.zeros(nf)
+        for l in range(nn):
+            tx = tx + X[:, l, 0] * dpsi[l]
+            ty = ty + X[:, l, 1] * dpsi[l]
+            u = u + U[:, l] * psi[l]
+            lm = lm + Lm[:, l] * psi[l]
+        W = weights[g] * np.sqrt(tx * tx + ty * ty)
+        c = alpha * u + beta
+        for i in range(nn):
+            R[:, 2 * i] = R[:, 2 * i] + W * (lm * psi[i])
+            R[:, 2 * i + 1] = R[:, 2 * i + 1] + W * (c * psi[i])
+            if flag == 1:
+                for j in range(nn):
+                    J[:, 2 * i, 2 * j + 1] = J[:, 2 * i, 2 * j + 1] + W * (psi[j] * psi[i])
+                    J[:, 2 * i + 1, 2 * j] = J[:, 2 * i + 1, 2 * j] + W * ((alpha * psi[j]) * psi[i])
+    return R, J

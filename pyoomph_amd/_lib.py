@@ -39,7 +39,8 @@ class SubproblemDesc(ctypes.Structure):
                 ("coords", c_dp), ("conn", c_i32p), ("nvalue", ctypes.c_int),
                 ("eqn", c_i32p), ("pinned_values", c_dp), ("ndof", ctypes.c_int64),
                 ("params", c_dp), ("nparams", ctypes.c_int), ("aux", c_dp),
-                ("nface", ctypes.c_int64), ("face_conn", c_i32p), ("face_coef", c_dp)]
+                ("nface", ctypes.c_int64), ("face_conn", c_i32p), ("face_coef", c_dp),
+                ("nenforced", ctypes.c_int64), ("enforced_conn", c_i32p), ("enforced_coef", c_dp)]
 
 
 class ElementTable(ctypes.Structure):

@@ -102,6 +102,16 @@ static int face_kind(int elem_kind, int form) {
     }
 }
 
+// face element kind of the EnforcedBC faces of a bulk element kind (-1: not built)
+static int enforced_face_kind(int elem_kind) {
+    switch (elem_kind) {
+    case PYHIP_ELEM_TRI_P1: return kKindLine2E;
+    case PYHIP_ELEM_TRI_P2: return kKindLine3E;
+    case PYHIP_ELEM_QUAD_Q2: return kKindLine3QE;
+    }
+    return -1;
+}
+
 int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stream,
                          pyhip_problem **out) {
     PYHIP_REQUIRE(subs && nsub > 0 && out, "invalid arguments");
@@ -119,7 +129,12 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
                       "generated form compiled for another element family");
         PYHIP_REQUIRE(d.form != PYHIP_FORM_NAVIER_STOKES_TH || d.elem_kind == PYHIP_ELEM_QUAD_Q2,
                       "Taylor-Hood needs Q2 quadrilaterals");
-        PYHIP_REQUIRE(d.nvalue == form_nvalue(d.form), "values per node do not match the form");
+        PYHIP_REQUIRE(d.nenforced >= 0 && (d.nenforced == 0 || (d.enforced_conn && d.enforced_coef)),
+                      "EnforcedBC faces without connectivity or coefficients");
+        PYHIP_REQUIRE(d.nenforced == 0 || (!th_layout(d.form) && enforced_face_kind(d.elem_kind) > 0),
+                      "EnforcedBC multipliers: scalar forms on P1 / P2 triangles and Q2 quadrilaterals");
+        PYHIP_REQUIRE(d.nvalue == form_nvalue(d.form) + (d.nenforced > 0 ? 1 : 0),
+                      "values per node do not match the form (EnforcedBC adds the multiplier value)");
         PYHIP_REQUIRE(d.nnode > 0 && d.nelem > 0 && d.coords && d.conn && d.eqn && d.pinned_values,
                       "empty mesh or missing array");
         PYHIP_REQUIRE(d.nparams >= 0 && d.nparams <= kParamStride, "too many parameters");
@@ -128,15 +143,20 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         PYHIP_REQUIRE(d.ndof >= 0, "negative ndof");
         PYHIP_REQUIRE(d.nface >= 0 && (d.nface == 0 || (d.face_conn && d.face_coef)),
                       "faces without connectivity or flux coefficients");
-        PYHIP_REQUIRE(d.nface == 0 || d.nvalue == 1 || th_layout(d.form),
+        PYHIP_REQUIRE(d.nface == 0 || !th_layout(d.form) || d.nvalue == 3,
                       "boundary-face flux elements: scalar forms and Taylor-Hood tractions");
         std::pair<int, int> k(d.form, d.elem_kind);
         if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
     }
-    // face groups after the bulk groups, one per face kind
+    // face groups after the bulk groups, one per face kind (flux faces, then EnforcedBC faces)
     for (int s = 0; s < nsub; s++) {
         if (subs[s].nface == 0) continue;
         std::pair<int, int> k(kFormFlux, face_kind(subs[s].elem_kind, subs[s].form));
+        if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
+    }
+    for (int s = 0; s < nsub; s++) {
+        if (subs[s].nenforced == 0) continue;
+        std::pair<int, int> k(kFormFlux, enforced_face_kind(subs[s].elem_kind));
         if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
     }
     P->groups.resize(gkeys.size());
@@ -145,8 +165,8 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         G.form = gkeys[g].first;
         G.kind = gkeys[g].second;
         if (G.form == kFormFlux) {
-            G.nn = G.kind == kKindLine2 ? 2 : (G.kind == kKindQuad9 ? 9 : 3);
-            G.ns = G.kind == kKindLine3TH ? 2 * G.nn : G.nn;
+            G.nn = (G.kind == kKindLine2 || G.kind == kKindLine2E) ? 2 : (G.kind == kKindQuad9 ? 9 : 3);
+            G.ns = (G.kind == kKindLine3TH || enforced_kind(G.kind)) ? 2 * G.nn : G.nn;
             G.dim = G.kind == kKindQuad9 ? 3 : 2;
             continue;
         }
@@ -178,6 +198,13 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
             S.fel_off = P->groups[S.fgroup].nel;
             S.nface = d.nface;
             P->groups[S.fgroup].nel += d.nface;
+        }
+        if (d.nenforced > 0) {
+            const std::pair<int, int> ek(kFormFlux, enforced_face_kind(d.elem_kind));
+            S.egroup = (int)(std::find(gkeys.begin(), gkeys.end(), ek) - gkeys.begin());
+            S.eel_off = P->groups[S.egroup].nel;
+            S.nenf = d.nenforced;
+            P->groups[S.egroup].nel += d.nenforced;
         }
         P->dof_off_h[s + 1] = S.dof_off + d.ndof;
         node_off += d.nnode;
@@ -245,23 +272,30 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
             }
         }
         for (int k = 0; k < d.nparams; k++) P->params_h[(size_t)s * kParamStride + k] = d.params[k];
-        if (S.nface > 0) {
-            const Group &F = P->groups[S.fgroup];
+        // face sets: the flux faces, then the EnforcedBC faces (slots: the node's values 0
+        // and 1 per face node, u and the multiplier)
+        for (int fs = 0; fs < 2; fs++) {
+            const int fg = fs == 0 ? S.fgroup : S.egroup;
+            const int64_t nf = fs == 0 ? S.nface : S.nenf, foff = fs == 0 ? S.fel_off : S.eel_off;
+            const int32_t *fconn = fs == 0 ? d.face_conn : d.enforced_conn;
+            const double *fcoef = fs == 0 ? d.face_coef : d.enforced_coef;
+            if (nf == 0) continue;
+            const Group &F = P->groups[fg];
             const int nc = face_ncoef(F.kind), vps = F.ns / F.nn;   // values per face node
-            for (int64_t f = 0; f < d.nface; f++) {
-                const int64_t ge = S.fel_off + f;
-                sub_g[S.fgroup][ge] = s;
-                for (int k = 0; k < nc; k++) coef_g[S.fgroup][nc * ge + k] = d.face_coef[nc * f + k];
+            for (int64_t f = 0; f < nf; f++) {
+                const int64_t ge = foff + f;
+                sub_g[fg][ge] = s;
+                for (int k = 0; k < nc; k++) coef_g[fg][nc * ge + k] = fcoef[nc * f + k];
                 for (int l = 0; l < F.nn; l++) {
-                    const int32_t n = d.face_conn[f * F.nn + l];
+                    const int32_t n = fconn[f * F.nn + l];
                     PYHIP_REQUIRE(n >= 0 && n < d.nnode, "face connectivity out of range");
-                    conn_sm[S.fgroup][(int64_t)l * F.nel + ge] = (int32_t)(S.node_off + n);
+                    conn_sm[fg][(int64_t)l * F.nel + ge] = (int32_t)(S.node_off + n);
                     for (int c = 0; c < vps; c++) {
                         const int32_t q = d.eqn[(int64_t)n * d.nvalue + c];
                         const int32_t gq = q >= 0 ? (int32_t)(q + S.dof_off) : -1;
                         const int sl = l * vps + c;
-                        seqn_em[S.fgroup][ge * F.ns + sl] = gq;
-                        seqn_sm[S.fgroup][(int64_t)sl * F.nel + ge] = gq;
+                        seqn_em[fg][ge * F.ns + sl] = gq;
+                        seqn_sm[fg][(int64_t)sl * F.nel + ge] = gq;
                     }
                 }
             }
@@ -298,15 +332,19 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         B.er_off = G.er_off + S.el_off * G.ns;
         B.ej_off = G.ej_off + S.el_off * G.ns * G.ns;
         blocks.push_back(B);
-        if (S.nface > 0) {
-            // the interface elements follow the bulk elements of their mesh
-            const Group &F = P->groups[S.fgroup];
+        // the interface elements follow the bulk elements of their mesh: flux faces, then
+        // EnforcedBC faces
+        for (int fs = 0; fs < 2; fs++) {
+            const int fg = fs == 0 ? S.fgroup : S.egroup;
+            const int64_t nf = fs == 0 ? S.nface : S.nenf, foff = fs == 0 ? S.fel_off : S.eel_off;
+            if (nf == 0) continue;
+            const Group &F = P->groups[fg];
             ElementBlock FB;
             FB.ns = F.ns;
-            FB.nel = S.nface;
-            FB.seqn = seqn_em[S.fgroup].data() + S.fel_off * F.ns;
-            FB.er_off = F.er_off + S.fel_off * F.ns;
-            FB.ej_off = F.ej_off + S.fel_off * F.ns * F.ns;
+            FB.nel = nf;
+            FB.seqn = seqn_em[fg].data() + foff * F.ns;
+            FB.er_off = F.er_off + foff * F.ns;
+            FB.ej_off = F.ej_off + foff * F.ns * F.ns;
             blocks.push_back(FB);
         }
     }
@@ -482,6 +520,7 @@ static std::vector<std::pair<int64_t, int64_t>> element_runs(const pyhip_problem
         int64_t b = -1, n = 0;
         if (S.group == g) b = S.el_off, n = S.nel;
         else if (S.fgroup == g) b = S.fel_off, n = S.nface;
+        else if (S.egroup == g) b = S.eel_off, n = S.nenf;
         if (b < 0 || n == 0) continue;
         if (!runs.empty() && runs.back().second == b) runs.back().second = b + n;
         else runs.push_back({b, b + n});

@@ -257,6 +257,80 @@ __global__ __launch_bounds__(256) void traction_face_kernel(ElemArgs<T> a) {
     }
 }
 
+// Faces of pyoomph's EnforcedBC(u=constraint) @ boundary (pyoomph/meshes/bcs.py:129-253)
+// with constraint = alpha*u + beta per face: the Lagrange multiplier l is value 1 of the
+// face nodes, in u's space; slots (u_l, l_l) per face node l (slot 2l, 2l+1, the node-major
+// order of the nodes' values). The line elements, rules and geometry are the flux faces'.
+// Per point: u = sum u_l psi_l, l = sum l_l psi_l, c = alpha*u + beta;
+//   R[u_i] += W * (l * psi_i),               R[l_i] += W * (c * psi_i),
+//   J[u_i][l_j] += W * (psi_j * psi_i),      J[l_i][u_j] += W * ((alpha * psi_j) * psi_i),
+// (the residuals weak(l, u_test) and weak(constraint, l_test), bcs.py:185-189); the u-u and
+// l-l blocks are zero. No time derivative, no dependence on the global parameter and none
+// of second order: the mass matrix, dR/dp and the Hessian-vector products vanish.
+template <typename T, int NN, typename RULE>
+__global__ __launch_bounds__(256) void enforced_face_kernel(ElemArgs<T> a) {
+    constexpr int NS = 2 * NN;
+    const int e = a.e0 + (int)(blockIdx.x * 256 + threadIdx.x);
+    if (e >= a.e1) return;
+    const T alpha = (T)a.fcoef[2 * (int64_t)e], beta = (T)a.fcoef[2 * (int64_t)e + 1];
+    T X[NN][2], U[NN], Lm[NN];
+#pragma unroll
+    for (int l = 0; l < NN; l++) {
+        const int32_t n = a.conn[(int64_t)l * a.nel + e];
+        X[l][0] = (T)a.coords[(int64_t)n * 3];
+        X[l][1] = (T)a.coords[(int64_t)n * 3 + 1];
+        const int32_t qu = a.seqn[(int64_t)(2 * l) * a.nel + e];
+        const int32_t ql = a.seqn[(int64_t)(2 * l + 1) * a.nel + e];
+        U[l] = (T)(qu >= 0 ? a.dofs[qu] : a.pinned[(int64_t)n * a.nvalue]);
+        Lm[l] = (T)(ql >= 0 ? a.dofs[ql] : a.pinned[(int64_t)n * a.nvalue + 1]);
+    }
+    T R[NS], J[NS][NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+        R[i] = T(0);
+#pragma unroll
+        for (int j = 0; j < NS; j++) J[i][j] = T(0);
+    }
+#pragma unroll
+    for (int g = 0; g < NN; g++) {
+        double sd, wd;
+        RULE::point(g, sd, wd);
+        const T sq = (T)sd, wq = (T)wd;
+        T psi[NN], dpsi[NN];
+        RULE::shape(sq, psi, dpsi);
+        T tx = T(0), ty = T(0), u = T(0), lm = T(0);
+#pragma unroll
+        for (int l = 0; l < NN; l++) {
+            tx = tx + X[l][0] * dpsi[l];
+            ty = ty + X[l][1] * dpsi[l];
+            u = u + U[l] * psi[l];
+            lm = lm + Lm[l] * psi[l];
+        }
+        const T W = wq * sqrt(tx * tx + ty * ty);
+        const T c = alpha * u + beta;
+#pragma unroll
+        for (int i = 0; i < NN; i++) {
+            R[2 * i] = R[2 * i] + W * (lm * psi[i]);
+            R[2 * i + 1] = R[2 * i + 1] + W * (c * psi[i]);
+#pragma unroll
+            for (int j = 0; j < NN; j++) {
+                J[2 * i][2 * j + 1] = J[2 * i][2 * j + 1] + W * (psi[j] * psi[i]);
+                J[2 * i + 1][2 * j] = J[2 * i + 1][2 * j] + W * ((alpha * psi[j]) * psi[i]);
+            }
+        }
+    }
+    const bool zero_r = a.deriv != 0;
+    const bool zero_j = a.deriv != 0 || a.flag == 2 || a.hvec != nullptr;
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+        a.Er[(int64_t)e * NS + i] = zero_r ? T(0) : R[i];
+        if (a.Ej) {
+#pragma unroll
+            for (int j = 0; j < NS; j++) a.Ej[((int64_t)e * NS + i) * NS + j] = zero_j ? T(0) : J[i][j];
+        }
+    }
+}
+
 template <typename T>
 static int launch_faces(int kind, const ElemArgs<T> &a, hipStream_t st) {
     if (a.e1 <= a.e0) return PYHIP_OK;
@@ -266,6 +340,9 @@ static int launch_faces(int kind, const ElemArgs<T> &a, hipStream_t st) {
     else if (kind == kKindLine3Q) hipLaunchKernelGGL((flux_face_kernel<T, 3, FaceRuleQ3>), g, t, 0, st, a);
     else if (kind == kKindQuad9) hipLaunchKernelGGL((flux_face_quad9_kernel<T>), g, t, 0, st, a);
     else if (kind == kKindLine3TH) hipLaunchKernelGGL((traction_face_kernel<T>), g, t, 0, st, a);
+    else if (kind == kKindLine2E) hipLaunchKernelGGL((enforced_face_kernel<T, 2, FaceRule<2>>), g, t, 0, st, a);
+    else if (kind == kKindLine3E) hipLaunchKernelGGL((enforced_face_kernel<T, 3, FaceRule<3>>), g, t, 0, st, a);
+    else if (kind == kKindLine3QE) hipLaunchKernelGGL((enforced_face_kernel<T, 3, FaceRuleQ3>), g, t, 0, st, a);
     else {
         set_error("unknown face element kind");
         return PYHIP_ERR_INVALID;

@@ -665,7 +665,7 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
     const int64_t n = p->ndof;
     const auto &rs = p->row_start_h;
     const auto &col = p->col_h;
-    // group of each row: 0 velocity / other, 1 Taylor-Hood pressure
+    // group of each row: 0 velocity / other, 1 Taylor-Hood pressure or EnforcedBC multiplier
     std::vector<char> is_p(n, 0);
     {
         if ((int64_t)p->ilu_last_h.size() == n) is_p = p->ilu_last_h;
@@ -682,6 +682,14 @@ static int ilu_plan_build(pyhip_problem *p, KrylovWork *W) {
                 }
         }
     }
+    // EnforcedBC multipliers (value 1 of a scalar form's nodes) go last as well: their
+    // pivots come from the Schur complement of the rows before them
+    // (problems made from a bare CSR matrix, pyhip_solve_serial, carry no sub-problem data)
+    if ((int64_t)p->dof_val_h.size() == n)
+        for (size_t s = 0; s < p->subs.size(); s++)
+            if (p->subs[s].nenf > 0)
+                for (int64_t r = p->dof_off_h[s]; r < p->dof_off_h[s + 1]; r++)
+                    if (p->dof_val_h[r] == 1) is_p[r] = 1;
     IluHostPlan H;
     std::string err;
     if (ilu_plan_host(n, rs.data(), col.data(), p->dof_off_h, is_p, W->ilu_order, kIluAutoLevels, H, err)) {
@@ -977,6 +985,10 @@ static int need_on_host(pyhip_problem *p, KrylovWork *W, const int32_t *need) {
 template <typename TV, typename T>
 static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals, const int32_t *need = nullptr) {
     TimeScope ts(&p->timer, "precond", p->st);
+    if (pc == PYHIP_PRECOND_JACOBI || pc == PYHIP_PRECOND_AMG)
+        for (const auto &S : p->subs)
+            PYHIP_REQUIRE(S.nenf == 0, "EnforcedBC multipliers give rows with a zero diagonal: "
+                                       "use the ILU(0) preconditioner");
     if (pc == PYHIP_PRECOND_BLOCK) {
         if constexpr (std::is_same<T, double>::value && std::is_same<TV, double>::value)
             return blk_setup(p, W, vals, need);

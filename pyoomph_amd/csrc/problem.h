@@ -22,6 +22,11 @@ constexpr int kKindLine3 = 17;         // 3-node faces of P2 triangles (TGauss<1
 constexpr int kKindLine3Q = 18;        // 3-node faces of Q2 quadrilaterals (Gauss<1,3>)
 constexpr int kKindQuad9 = 19;         // 9-node faces of Q2 bricks (Gauss<2,3>)
 constexpr int kKindLine3TH = 20;       // traction faces of Taylor-Hood Q2 quads (2 slots per node)
+// faces of EnforcedBC constraints: slots (u, multiplier) per face node
+constexpr int kKindLine2E = 21;        // on P1 triangles
+constexpr int kKindLine3E = 22;        // on P2 triangles
+constexpr int kKindLine3QE = 23;       // on Q2 quadrilaterals
+inline bool enforced_kind(int face_kind) { return face_kind >= kKindLine2E && face_kind <= kKindLine3QE; }
 // flux coefficients per face: alpha, beta (scalar forms) or alpha, beta_x, beta_y (Taylor-Hood)
 inline int face_ncoef(int face_kind) { return face_kind == kKindLine3TH ? 3 : 2; }
 
@@ -33,6 +38,8 @@ struct SubInfo {
     int64_t nel;
     int fgroup = -1;         // face group (-1: no faces)
     int64_t fel_off = 0, nface = 0;
+    int egroup = -1;         // group of the EnforcedBC faces (-1: none)
+    int64_t eel_off = 0, nenf = 0;
 };
 
 struct Group {

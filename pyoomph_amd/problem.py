@@ -49,17 +49,30 @@ class MeshProblem:
     meshes of the scalar forms; for Taylor-Hood, beta = (beta_x, beta_y) and the condition is
     ``NeumannBC(velocity_x=alpha*u_x + beta_x, velocity_y=alpha*u_y + beta_y)``; one
     (names, alpha, beta) triple is accepted for a single condition; None: none.
+    enforced: list of (boundary names, alpha, beta) -- ``EnforcedBC(u=alpha*u + beta) @
+    boundary`` (pyoomph/meshes/bcs.py:129-253) for scalar forms on P1 / P2 triangle and Q2
+    quadrilateral meshes: a Lagrange-multiplier field l on those boundaries, in u's space,
+    held as value 1 of the nodes (``nvalue`` 2: u, l; numbered node by node after u, as
+    oomph numbers a node's values), pinned to 0 where u is pinned and off the boundaries
+    (EnforcedBC pins redundant multipliers, bcs.py:199-243); the residual gains
+    weak(alpha*u + beta, l_test) + weak(l, u_test) on every face. The Jacobian becomes a
+    saddle point: solve with GMRES and ILU(0).
     """
 
-    def __init__(self, mesh, equations, dirichlet=(), pin_extra=(), flux=None):
+    def __init__(self, mesh, equations, dirichlet=(), pin_extra=(), flux=None, enforced=None):
         if not isinstance(equations, Equations):
             raise TypeError("equations must be an Equations instance")
         self.mesh = mesh
         self.equations = equations
         equations.check_mesh(mesh)
         nv = equations.nvalue
-        self.pinned = np.zeros((mesh.nnode, nv), dtype=bool)
-        self.values = np.zeros((mesh.nnode, nv), dtype=np.float64)
+        if enforced is not None and len(enforced) == 3 and np.isscalar(enforced[1]):
+            enforced = [enforced]
+        if enforced and nv != 1:
+            raise ValueError("EnforcedBC multipliers are built for the scalar forms")
+        self.nvalue = nv + (1 if enforced else 0)
+        self.pinned = np.zeros((mesh.nnode, self.nvalue), dtype=bool)
+        self.values = np.zeros((mesh.nnode, self.nvalue), dtype=np.float64)
         for names, vals in dirichlet:
             m = mesh.on_boundary(list(names))
             for vi, v in vals.items():
@@ -73,10 +86,27 @@ class MeshProblem:
         for node, vi, v in pin_extra:
             self.pinned[node, vi] = True
             self.values[node, vi] = v
+        self.enforced, self.enforced_coef = None, None
+        if enforced:
+            from .meshes import boundary_faces
+            faces, coef = [], []
+            for names, alpha, beta in enforced:
+                names = [names] if isinstance(names, str) else list(names)
+                f = boundary_faces(mesh, names)
+                faces.append(f)
+                coef.append(np.tile([float(alpha), float(beta)], (len(f), 1)))
+            self.enforced = np.ascontiguousarray(np.concatenate(faces), dtype=np.int32)
+            self.enforced_coef = np.ascontiguousarray(np.concatenate(coef), dtype=np.float64)
+            on = np.zeros(mesh.nnode, dtype=bool)
+            on[self.enforced.reshape(-1)] = True
+            # the multiplier exists on the constrained boundaries only, and is redundant
+            # (pinned to 0) where u itself is pinned
+            self.pinned[:, 1] = ~on | self.pinned[:, 0]
+            self.values[:, 1] = 0.0
         free = ~self.pinned.reshape(-1)
         eqn = np.full(free.shape, -1, dtype=np.int64)
         eqn[free] = np.arange(int(free.sum()))
-        self.eqn = eqn.reshape(mesh.nnode, nv).astype(np.int32)
+        self.eqn = eqn.reshape(mesh.nnode, self.nvalue).astype(np.int32)
         self.ndof = int(free.sum())
         self.faces, self.face_coef = None, None
         if flux is not None:
@@ -164,7 +194,7 @@ class Problem:
             d.nelem = mp.mesh.nelement
             d.coords = coords.ctypes.data_as(_lib.c_dp)
             d.conn = conn.ctypes.data_as(_lib.c_i32p)
-            d.nvalue = eq.nvalue
+            d.nvalue = mp.nvalue
             d.eqn = eqn.ctypes.data_as(_lib.c_i32p)
             d.pinned_values = pv.ctypes.data_as(_lib.c_dp)
             d.ndof = mp.ndof
@@ -178,6 +208,11 @@ class Problem:
                 d.nface = fc.shape[0]
                 d.face_conn = fc.ctypes.data_as(_lib.c_i32p)
                 d.face_coef = cf.ctypes.data_as(_lib.c_dp)
+            if mp.enforced is not None and len(mp.enforced):
+                keep += [mp.enforced, mp.enforced_coef]
+                d.nenforced = mp.enforced.shape[0]
+                d.enforced_conn = mp.enforced.ctypes.data_as(_lib.c_i32p)
+                d.enforced_coef = mp.enforced_coef.ctypes.data_as(_lib.c_dp)
         torch.cuda.set_device(self.device)
         # the problem runs on the stream current at creation; every host wait below is on
         # this stream only, so problems on other streams keep running
@@ -304,8 +339,13 @@ class Problem:
         o.linear.precond = {"ilu0": _lib.PRECOND_ILU0, "jacobi": _lib.PRECOND_JACOBI,
                             "amg": _lib.PRECOND_AMG, "block": _lib.PRECOND_BLOCK}[self.preconditioner]
         kry = self.krylov
+        saddle = any(mp.enforced is not None for mp in self.parts)
         if kry == "auto":
-            kry = "cg" if all(getattr(mp.equations, "symmetric", False) for mp in self.parts) else "gmres"
+            kry = "cg" if all(getattr(mp.equations, "symmetric", False) for mp in self.parts) and not saddle \
+                else "gmres"
+        if saddle and (kry == "cg" or self.preconditioner != "ilu0"):
+            raise ValueError("EnforcedBC multipliers make the Jacobian a saddle point: "
+                             "use krylov='gmres' (or 'auto') with preconditioner='ilu0'")
         o.linear.krylov = _lib.KRYLOV_CG if kry == "cg" else _lib.KRYLOV_GMRES
         o.linear.rtol_f32 = self.rtol_f32
         o.linear.restart = self.gmres_restart

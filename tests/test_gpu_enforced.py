@@ -1,0 +1,130 @@
+"""GPU EnforcedBC (Lagrange-multiplier) faces against the oracle: the reference CSR of the
+saddle-point Jacobian (pattern bit-exact; values bitwise without the Bratu exponential,
+else within 1e-12), Newton with GMRES + ILU(0) (multipliers ordered last) against the
+oracle's SuperLU Newton and against the closed form of a linear solution, and a batch
+equal to each sub-problem alone."""
+import numpy as np
+import pytest
+
+from oracle import assembly as oa
+from oracle import faces as of
+from oracle import meshes as om
+from oracle import newton as on
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("C1", "left"), ("C2", "left"), ("C2", False)]
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def build(N, space, split, lam, enf=(1.0, -3.0), left=1.0, flux=None):
+    from pyoomph_amd.equations import BratuEquation
+    from pyoomph_amd.meshes import RectangularQuadMesh
+    from pyoomph_amd.problem import MeshProblem
+    mp = MeshProblem(RectangularQuadMesh(N, split_in_tris=split, space=space),
+                     BratuEquation("u", space=space, lambda_=lam), dirichlet=[(("left",), {0: left})],
+                     enforced=[(("right",), enf[0], enf[1])], flux=flux)
+    m = om.rectangular_quad_mesh(N, split, space)
+    faces = of.boundary_faces(m, ["right"])
+    on_f = np.zeros(m.nnode, bool)
+    on_f[faces.reshape(-1)] = True
+    pinned = np.zeros((m.nnode, 2), bool)
+    pinned[:, 0] = m.on_boundary(["left"])
+    pinned[:, 1] = ~on_f | pinned[:, 0]
+    bc = np.zeros((m.nnode, 2))
+    bc[pinned[:, 0], 0] = left
+    kw = {}
+    if flux is not None:
+        (names, a, b), = flux
+        kw = dict(faces=of.boundary_faces(m, list(names)), face_alpha=a, face_beta=b)
+    op = oa.OracleProblem(m, oa.FORM_BRATU, 2, pinned, bc, [lam], enforced=faces, enf_alpha=enf[0],
+                          enf_beta=enf[1], **kw)
+    assert np.array_equal(mp.eqn, op.eqn) and np.array_equal(mp.enforced, op.enforced)
+    return mp, op
+
+
+def solver(P):
+    P.krylov, P.preconditioner = "gmres", "ilu0"
+    P.gmres_restart, P.gmres_rtol = 100, 1e-12
+    P.newton_solver_tolerance = 1e-11
+    return P
+
+
+@pytest.mark.parametrize("space,split", CASES)
+@pytest.mark.parametrize("lam", [0.0, 1.5])
+def test_assembly_parity(gpu, space, split, lam):
+    from pyoomph_amd.problem import Problem
+    mp, op = build(6, space, split, lam, enf=(1.7, -0.4))
+    P = Problem(mp)
+    for seed in (1, 2):
+        x = np.random.default_rng(seed).uniform(-0.4, 0.4, P.ndof)
+        P.set_current_dofs(x)
+        R, J = P.assemble_jacobian()
+        r, rs, col, val = op.get_jacobian(x)
+        assert np.array_equal(J.indptr, rs) and np.array_equal(J.indices, col)
+        if lam == 0.0:
+            assert np.array_equal(J.data, val) and np.array_equal(R, r)
+        else:
+            assert rel(J.data, val) <= 1e-12 and rel(R, r) <= 1e-12
+    # no multiplier row keeps a diagonal entry: exact zeros are dropped as the reference drops them
+    lrow = mp.eqn[mp.eqn[:, 1] >= 0, 1]
+    assert not np.any(J.toarray()[lrow, lrow])
+
+
+@pytest.mark.parametrize("space,split", CASES)
+def test_linear_solution_and_multiplier(gpu, space, split):
+    """u = 1 on the left, EnforcedBC(u = u - 3) on the right, -lap u = 0: u = 1 + 2x and the
+    multiplier -du/dn = -2 on the right side (Q2: to the accuracy of the Gauss<2,3> knot)."""
+    from pyoomph_amd.problem import Problem
+    mp, op = build(8, space, split, 0.0)
+    P = solver(Problem(mp))
+    st = P.solve()
+    assert st[0]["converged"]
+    U = mp.nodal_values(P.get_current_dofs())
+    tol = 1e-10 if split else 1e-8
+    assert np.max(np.abs(U[:, 0] - (1.0 + 2.0 * mp.mesh.coords[:, 0]))) < tol
+    assert np.max(np.abs(U[mp.eqn[:, 1] >= 0, 1] + 2.0)) < 10 * tol
+
+
+@pytest.mark.parametrize("space,split", CASES)
+def test_newton_matches_oracle(gpu, space, split):
+    from pyoomph_amd.problem import Problem
+    mp, op = build(10, space, split, 1.2, enf=(1.0, -0.3), left=0.0)
+    P = solver(Problem(mp))
+    st = P.solve()
+    assert st[0]["converged"] and st[0]["linear_failures"] == 0
+    xo = on.newton_solve(op, np.zeros(op.ndof), tol=1e-12)
+    assert rel(P.get_current_dofs(), xo) <= 1e-8
+
+
+def test_with_flux_faces_and_batch_equals_alone(gpu):
+    """A Robin side and an enforced side on one mesh, batched with a plain problem and a
+    second enforced one: each sub-problem equals its own solve bit for bit and the oracle."""
+    from pyoomph_amd import workloads as wl
+    from pyoomph_amd.problem import Problem
+    mp1, op1 = build(7, "C2", "left", 1.0, enf=(1.0, -0.2), left=0.1,
+                     flux=[(("top",), 2.0, 0.5)])
+    mp2, op2 = build(5, "C1", "left", 0.7, enf=(2.0, 0.3), left=0.0)
+    plain = wl.bratu(9, "C2", 1.1)
+    parts = [mp1, plain, mp2]
+    B = solver(Problem(parts))
+    B.solve()
+    xb = B.get_current_dofs()
+    for i, mp in enumerate(parts):
+        A = solver(Problem(mp))
+        A.solve()
+        assert np.array_equal(B.sub_dofs(i, xb), A.get_current_dofs()), i
+    for i, op in ((0, op1), (2, op2)):
+        assert rel(B.sub_dofs(i, xb), on.newton_solve(op, np.zeros(op.ndof), tol=1e-12)) <= 1e-8
+
+
+def test_saddle_point_needs_ilu0(gpu):
+    from pyoomph_amd.problem import Problem
+    mp, _ = build(4, "C1", "left", 0.0)
+    P = Problem(mp)
+    P.preconditioner = "amg"
+    with pytest.raises(ValueError, match="saddle point"):
+        P.solve()
