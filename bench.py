@@ -650,10 +650,22 @@ def main():
     # measured HBM traffic of this kernel family on this configuration (two rocprofv3 PMC passes)
     tfile = os.path.join(ROOT, "profiles", "traffic_%s_bratu_sweep%s.txt"
                          % (dom, "" if args.precond == "amg" else "_" + args.precond))
+    traffic_note = None
     if os.path.exists(tfile) and args.precision == "f64":
+        # the profile names the spmv kernel source it was measured with; a figure taken with
+        # other kernels is not reported
+        import hashlib
+        src = hashlib.sha256(open(os.path.join(ROOT, "pyoomph_amd", "csrc", "spmv.h"), "rb").read()).hexdigest()
+        rec, tsrc = None, None
         for line in open(tfile):
             if line.startswith("traffic_bytes_per_launch="):
-                traffic = float(line.split("=", 1)[1])
+                rec = float(line.split("=", 1)[1])
+            if line.startswith("spmv_source_sha256="):
+                tsrc = line.split("=", 1)[1].strip()
+        if rec is not None and tsrc == src:
+            traffic = rec
+        elif rec is not None:
+            traffic_note = "the traffic profile was measured with other spmv kernels; not reported"
 
     # single-problem GPU paths on a sample (one problem per Problem, pattern and plan set up
     # outside the clock, the same stopping rules): the plain FP64 path (CG + Jacobi) and the
@@ -735,6 +747,7 @@ def main():
         "kernel_ms": {f: {"launches": v[0], "ms": v[1]} for f, v in fams.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": os.path.relpath(tfile, ROOT) if traffic is not None else traffic_note,
                      "algorithmic_bytes_per_launch": alg, "avg_launch_ms": avg_ms,
                      "share_of_step": ms_l / (ms_per_step * args.steps),
                      "frac_of_copy": achieved / copy_gbs, "full_matrix_bytes": spmv_bytes(P)},
