@@ -121,6 +121,11 @@ typedef struct {
     int64_t nenforced;
     const int32_t *enforced_conn; /* [nenforced][2|3] mesh nodes of each face          */
     const double *enforced_coef;  /* [nenforced][2] alpha, beta of each face's constraint */
+    /* 0: the faces above carry the built-in flux alpha*u + beta; else the id of a generated
+     * boundary-face form (pyhip_codegen_set_face_form, scalar forms on P1 / P2 triangles
+     * and Q2 quadrilaterals) whose flux expression replaces it, face_coef then holding its
+     * ncoef coefficients per face */
+    int face_form;
 } pyhip_subproblem_desc;
 
 typedef struct pyhip_problem pyhip_problem;

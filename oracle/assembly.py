@@ -84,13 +84,17 @@ class OracleProblem:
                  flux face_alpha*u + face_beta (``oracle/faces.py``; scalars, or one value
                  per face), or None; the face elements follow the bulk elements in
                  element order
+    face_expr -- (flux, dflux/du, dflux/dparam, d2flux/du dparam; the last two may be None)
+                 callables (u, x0, x1, C) replacing
+                 the flux alpha*u + beta of the faces by an expression, C = face_coef
+                 [nface, ncoef] (``faces.expression_face_outputs``)
     enforced  -- [nenf, 2|3] boundary-face nodes of EnforcedBC(u=enf_alpha*u + enf_beta)
                  (``faces.enforced_face_outputs``), or None; the scalar forms then have
                  nvalue 2 (u, the multiplier); these faces follow the flux faces
     """
 
     def __init__(self, mesh, form, nvalue, pinned, bc, params, aux=None, faces=None, face_alpha=0.0,
-                 face_beta=0.0, enforced=None, enf_alpha=1.0, enf_beta=0.0):
+                 face_beta=0.0, enforced=None, enf_alpha=1.0, enf_beta=0.0, face_expr=None, face_coef=None):
         from .meshes import number_equations
         self.mesh, self.form, self.nvalue = mesh, form, nvalue
         self.pinned = np.asarray(pinned, dtype=bool)
@@ -122,6 +126,8 @@ class OracleProblem:
         rows = np.nonzero(free)
         self.geqn[rows[0], self.leqn[rows]] = g[rows]
         self.faces = None
+        self.face_expr = face_expr
+        self.face_coef = None if face_coef is None else np.asarray(face_coef, dtype=np.float64)
         if faces is not None and len(faces):
             nf = len(faces)
             nc = 2 if form == FORM_NS else 1      # velocity components of a Taylor-Hood traction
@@ -267,7 +273,24 @@ class OracleProblem:
         nf, nn = F.shape
         nc = self.face_beta.shape[1]
         fl = 1 if flag == 1 else 0
-        if nn == 9:
+        if self.face_expr is not None:
+            from .faces import expression_face_outputs
+            flux, dflux, dflux_dp, d2flux_dudp = self.face_expr
+            Xf = np.ascontiguousarray(self.mesh.coords[F][:, :, :2])
+            quad = self.mesh.kind == "quad_q2"
+            Uf = np.ascontiguousarray(U[F, 0])
+            if deriv:
+                # dR/dp and dJ/dp: the flux's parameter derivative and its u-derivative
+                zero = lambda u, x0, x1, C: np.zeros_like(u)  # noqa: E731
+                f = dflux_dp if dflux_dp is not None else zero
+                d = d2flux_dudp if d2flux_dudp is not None else zero
+                Rs, Js = expression_face_outputs(Xf, Uf, self.face_coef, f, d, fl, quad=quad)
+            else:
+                Rs, Js = expression_face_outputs(Xf, Uf, self.face_coef, flux, dflux, fl, quad=quad)
+            if flag == 2 or y is not None:
+                Js[:] = 0.0
+            deriv = False   # handled: the derivative entries are the expression's
+        elif nn == 9:
             Xf = np.ascontiguousarray(self.mesh.coords[F][:, :, :3])
             Rs, Js = face_outputs_quad9(Xf, np.ascontiguousarray(U[F, 0]), self.face_alpha, self.face_beta[:, 0], fl)
         else:

@@ -186,3 +186,38 @@ def enforced_face_outputs(X, U, Lm, alpha, beta, flag, quad=False):
                     J[:, 2 * i, 2 * j + 1] = J[:, 2 * i, 2 * j + 1] + W * (psi[j] * psi[i])
                     J[:, 2 * i + 1, 2 * j] = J[:, 2 * i + 1, 2 * j] + W * ((alpha * psi[j]) * psi[i])
     return R, J
+
+
+def expression_face_outputs(X, U, C, flux, dflux, flag, quad=False):
+    """Faces of NeumannBC(u=flux) with a flux expression (the generated boundary-face forms):
+    flux(u, x0, x1, C) and its u-derivative dflux(...) evaluated per point for all faces at
+    once (C [nf, ncoef] the faces' coefficients); otherwise face_outputs' rules and order:
+    R_i += W (flux psi_i), J_ij += W ((dflux psi_j) psi_i)."""
+    nf, nn = U.shape
+    knots, weights = _RULE_Q3 if quad else _RULES[nn]
+    R = np.zeros((nf, nn))
+    J = np.zeros((nf, nn, nn))
+    for g in range(nn):
+        psi, dpsi = shape(nn, knots[g], quad)
+        tx = np.zeros(nf)
+        ty = np.zeros(nf)
+        u = np.zeros(nf)
+        x0 = np.zeros(nf)
+        x1 = np.zeros(nf)
+        for l in range(nn):
+            tx = tx + X[:, l, 0] * dpsi[l]
+            ty = ty + X[:, l, 1] * dpsi[l]
+            u = u + U[:, l] * psi[l]
+        for l in range(nn):
+            x0 = x0 + X[:, l, 0] * psi[l]
+        for l in range(nn):
+            x1 = x1 + X[:, l, 1] * psi[l]
+        W = weights[g] * np.sqrt(tx * tx + ty * ty)
+        F = flux(u, x0, x1, C)
+        D = dflux(u, x0, x1, C)
+        for i in range(nn):
+            R[:, i] = R[:, i] + W * (F * psi[i])
+            if flag == 1:
+                for j in range(nn):
+                    J[:, i, j] = J[:, i, j] + W * ((D * psi[j]) * psi[i])
+    return R, J

@@ -40,7 +40,8 @@ class SubproblemDesc(ctypes.Structure):
                 ("eqn", c_i32p), ("pinned_values", c_dp), ("ndof", ctypes.c_int64),
                 ("params", c_dp), ("nparams", ctypes.c_int), ("aux", c_dp),
                 ("nface", ctypes.c_int64), ("face_conn", c_i32p), ("face_coef", c_dp),
-                ("nenforced", ctypes.c_int64), ("enforced_conn", c_i32p), ("enforced_coef", c_dp)]
+                ("nenforced", ctypes.c_int64), ("enforced_conn", c_i32p), ("enforced_coef", c_dp),
+                ("face_form", ctypes.c_int)]
 
 
 class ElementTable(ctypes.Structure):
@@ -129,6 +130,7 @@ SIGNATURES = {
     "pyhip_codegen_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
     "pyhip_codegen_set_fields": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
     "pyhip_codegen_set_layout": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "pyhip_codegen_set_face_form": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "pyhip_newton_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(NewtonStepState)]),
     "pyhip_problem_set_aux": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dp, ctypes.c_int64]),

@@ -15,7 +15,7 @@ import numpy as np
 
 from .. import _lib
 from ..equations import Equations
-from . import mixed, scalar
+from . import face, mixed, scalar
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 COMMON_HEADER = os.path.join(os.path.dirname(_HERE), "csrc", "element_common.h")
@@ -66,6 +66,34 @@ def compile_form(form, kind, load=True):
     # (Taylor-Hood tables list velocity and pressure)
     _lib.check(L.pyhip_codegen_set_fields(fid.value, b"u", b"f" if form.aux else None,
                                           int(form.global_param or 0)))
+    _registry[key] = fid.value
+    return fid.value
+
+
+def compile_face_form(form, kind, load=True):
+    """Compiles a boundary-face form (face.FaceFluxForm) for the line faces of element family
+    ``kind`` and, with load, registers it as a face form (pyhip_codegen_set_face_form);
+    returns its id (cached per form name and family), or the code-object size without load."""
+    kid = scalar.KIND_IDS[kind] if isinstance(kind, str) else int(kind)
+    key = ("face", form.name, kid)
+    if load and key in _registry:
+        return _registry[key]
+    src, name, info = face.emit_face_kernels(form, kid)
+    L = _lib.load()
+    _signatures(L)
+    with open(COMMON_HEADER) as f:
+        hdr = f.read()
+    hs = (ctypes.c_char_p * 1)(hdr.encode())
+    hn = (ctypes.c_char_p * 1)(b"element_common.h")
+    fid = ctypes.c_int(-1)
+    size = ctypes.c_size_t(0)
+    _lib.check(L.pyhip_codegen_compile(name.encode(), src.encode(), 1, hs, hn, kid, 0,
+                                       int(info["has_param_derivative"]), int(bool(load)), ctypes.byref(fid),
+                                       ctypes.byref(size)))
+    if not load:
+        return size.value
+    _lib.check(L.pyhip_codegen_set_face_form(fid.value, form.ncoef))
+    _lib.check(L.pyhip_codegen_set_fields(fid.value, b"u", None, int(form.global_param or 0)))
     _registry[key] = fid.value
     return fid.value
 

@@ -124,7 +124,8 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
     for (int s = 0; s < nsub; s++) {
         const auto &d = subs[s];
         PYHIP_REQUIRE(elem_nnode(d.elem_kind) > 0, "unknown element family");
-        PYHIP_REQUIRE((d.form >= 1 && d.form <= 4) || gen_form_known(d.form), "unknown weak form");
+        PYHIP_REQUIRE((d.form >= 1 && d.form <= 4) || (gen_form_known(d.form) && !gen_face_form(d.form)),
+                      "unknown weak form");
         PYHIP_REQUIRE(!gen_form_known(d.form) || gen_form_kind(d.form) == d.elem_kind,
                       "generated form compiled for another element family");
         PYHIP_REQUIRE(d.form != PYHIP_FORM_NAVIER_STOKES_TH || d.elem_kind == PYHIP_ELEM_QUAD_Q2,
@@ -141,7 +142,11 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         PYHIP_REQUIRE(d.form != PYHIP_FORM_REACTION_DIFFUSION || d.aux,
                       "reaction-diffusion needs the nodal source field (aux)");
         PYHIP_REQUIRE(d.ndof >= 0, "negative ndof");
-        PYHIP_REQUIRE(d.nface >= 0 && (d.nface == 0 || (d.face_conn && d.face_coef)),
+        PYHIP_REQUIRE(d.face_form == 0 || (gen_face_form(d.face_form) && gen_form_kind(d.face_form) == d.elem_kind &&
+                                           !th_layout(d.form)),
+                      "face_form: a generated boundary-face form compiled for this element family, on a scalar form");
+        PYHIP_REQUIRE(d.nface >= 0 && (d.nface == 0 || (d.face_conn && (d.face_coef || (d.face_form != 0 &&
+                                                                                       gen_form_ncoef(d.face_form) == 0)))),
                       "faces without connectivity or flux coefficients");
         PYHIP_REQUIRE(d.nface == 0 || !th_layout(d.form) || d.nvalue == 3,
                       "boundary-face flux elements: scalar forms and Taylor-Hood tractions");
@@ -151,7 +156,8 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
     // face groups after the bulk groups, one per face kind (flux faces, then EnforcedBC faces)
     for (int s = 0; s < nsub; s++) {
         if (subs[s].nface == 0) continue;
-        std::pair<int, int> k(kFormFlux, face_kind(subs[s].elem_kind, subs[s].form));
+        std::pair<int, int> k(subs[s].face_form ? subs[s].face_form : kFormFlux,
+                              face_kind(subs[s].elem_kind, subs[s].form));
         if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
     }
     for (int s = 0; s < nsub; s++) {
@@ -164,7 +170,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         Group &G = P->groups[g];
         G.form = gkeys[g].first;
         G.kind = gkeys[g].second;
-        if (G.form == kFormFlux) {
+        if (face_group_form(G.form)) {
             G.nn = (G.kind == kKindLine2 || G.kind == kKindLine2E) ? 2 : (G.kind == kKindQuad9 ? 9 : 3);
             G.ns = (G.kind == kKindLine3TH || enforced_kind(G.kind)) ? 2 * G.nn : G.nn;
             G.dim = G.kind == kKindQuad9 ? 3 : 2;
@@ -193,7 +199,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         S.nel = d.nelem;
         P->groups[S.group].nel += d.nelem;
         if (d.nface > 0) {
-            const std::pair<int, int> fk(kFormFlux, face_kind(d.elem_kind, d.form));
+            const std::pair<int, int> fk(d.face_form ? d.face_form : kFormFlux, face_kind(d.elem_kind, d.form));
             S.fgroup = (int)(std::find(gkeys.begin(), gkeys.end(), fk) - gkeys.begin());
             S.fel_off = P->groups[S.fgroup].nel;
             S.nface = d.nface;
@@ -232,7 +238,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         conn_sm[g].resize(G.nn * G.nel);
         seqn_sm[g].resize(G.ns * G.nel);
         sub_g[g].resize(G.nel);
-        if (G.form == kFormFlux) coef_g[g].resize((size_t)face_ncoef(G.kind) * G.nel);
+        if (face_group_form(G.form)) coef_g[g].resize((size_t)group_ncoef(G.form, G.kind) * G.nel);
     }
     std::vector<double> coords_h(P->nnode * 3, 0.0), pinned_h(P->nnode * 3, 0.0),
         aux_h(P->nnode, 0.0);
@@ -281,7 +287,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
             const double *fcoef = fs == 0 ? d.face_coef : d.enforced_coef;
             if (nf == 0) continue;
             const Group &F = P->groups[fg];
-            const int nc = face_ncoef(F.kind), vps = F.ns / F.nn;   // values per face node
+            const int nc = group_ncoef(F.form, F.kind), vps = F.ns / F.nn;   // values per face node
             for (int64_t f = 0; f < nf; f++) {
                 const int64_t ge = foff + f;
                 sub_g[fg][ge] = s;
@@ -381,9 +387,9 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         UP(G.conn, conn_sm[g].data(), conn_sm[g].size());
         UP(G.seqn, seqn_sm[g].data(), seqn_sm[g].size());
         UP(G.sub, sub_g[g].data(), sub_g[g].size());
-        if (G.form == kFormFlux) UP(G.coef, coef_g[g].data(), coef_g[g].size());
+        if (face_group_form(G.form)) UP(G.coef, coef_g[g].data(), coef_g[g].size());
         // the faces of Q2 bricks use the Q2 quadrilateral tables
-        const int tk = G.form != kFormFlux ? G.kind : (G.kind == kKindQuad9 ? (int)ELEM_QUAD_Q2 : -1);
+        const int tk = !face_group_form(G.form) ? G.kind : (G.kind == kKindQuad9 ? (int)ELEM_QUAD_Q2 : -1);
         if (tk >= 0 && !P->tab64[tk]) {
             rc = upload_tables<double>(tk, &P->tab64[tk], B);
             if (rc) return rc;
@@ -499,7 +505,7 @@ static int check_assemble_args(pyhip_problem *p, const double *dofs, int flag, i
     PYHIP_REQUIRE(flag != 2 || deriv_param < 0, "no mass matrix of a parameter derivative");
     PYHIP_REQUIRE(deriv_param == -1 || deriv_param == 0, "unknown global parameter");
     for (auto &G : p->groups)
-        PYHIP_REQUIRE(deriv_param < 0 || G.form == PYHIP_FORM_BRATU || G.form == kFormFlux ||
+        PYHIP_REQUIRE(deriv_param < 0 || G.form == PYHIP_FORM_BRATU || face_group_form(G.form) ||
                           gen_form_has_pderiv(G.form),
                       "parameter derivative only defined for forms with a global parameter");
     return PYHIP_OK;
@@ -544,7 +550,8 @@ static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int der
                 a.conn = G.conn; a.seqn = G.seqn; a.sub = G.sub;
                 a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
                 a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
-                a.tab = G.form != kFormFlux ? p->tab64[G.kind] : (G.kind == kKindQuad9 ? p->tab64[ELEM_QUAD_Q2] : nullptr);
+                a.tab = !face_group_form(G.form) ? p->tab64[G.kind]
+                                                 : (G.kind == kKindQuad9 ? p->tab64[ELEM_QUAD_Q2] : nullptr);
                 a.Er = (double *)Er + G.er_off;
                 a.Ej = flag ? (double *)Ej + G.ej_off : nullptr;
                 a.flag = flag; a.deriv = deriv_param >= 0;
@@ -569,7 +576,8 @@ static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int der
                 a.conn = G.conn; a.seqn = G.seqn; a.sub = G.sub;
                 a.coords = p->coords; a.dofs = dofs; a.pinned = p->pinned; a.aux = p->aux;
                 a.params = p->params; a.pstride = kParamStride; a.nvalue = 3;
-                a.tab = G.form != kFormFlux ? p->tab32[G.kind] : (G.kind == kKindQuad9 ? p->tab32[ELEM_QUAD_Q2] : nullptr);
+                a.tab = !face_group_form(G.form) ? p->tab32[G.kind]
+                                                 : (G.kind == kKindQuad9 ? p->tab32[ELEM_QUAD_Q2] : nullptr);
                 a.Er = (float *)Er + G.er_off;
                 a.Ej = flag ? (float *)Ej + G.ej_off : nullptr;
                 a.flag = flag; a.deriv = deriv_param >= 0;

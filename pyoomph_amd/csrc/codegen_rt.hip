@@ -19,7 +19,8 @@ struct GenForm {
     bool symmetric = false, has_pderiv = false;
     std::string field = "u", aux_field;   // nodal field names (JIT table)
     int global_param = 0;                 // the parameter dR/dp is taken in
-    int layout = 0;                       // 0 one value per node, 1 Taylor-Hood (22 slots)
+    int layout = 0;                       // 0 one value per node, 1 Taylor-Hood (22 slots), 2 boundary faces
+    int ncoef = 0;                        // layout 2: coefficients per face
     hipModule_t mod = nullptr;
     hipFunction_t f64 = nullptr, f64_tr = nullptr, f32 = nullptr, f32_tr = nullptr;
 };
@@ -101,6 +102,12 @@ int gen_form_layout(int form) {
     return (k >= 0 && k < (int)g_forms.size()) ? g_forms[k].layout : 0;
 }
 
+int gen_form_ncoef(int form) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    const int k = form - PYHIP_GENERATED_FORM_BASE;
+    return (k >= 0 && k < (int)g_forms.size() && g_forms[k].layout == 2) ? g_forms[k].ncoef : -1;
+}
+
 bool gen_form_fields(int form, std::string &field, std::string &aux_field, int &global_param) {
     std::lock_guard<std::mutex> lock(g_mu);
     const int k = form - PYHIP_GENERATED_FORM_BASE;
@@ -124,8 +131,9 @@ static int launch_gen(int form, const ElemArgs<T> &a, hipStream_t st) {
         layout = G.layout;
     }
     PYHIP_REQUIRE(a.hvec == nullptr, "Hessian-vector products are not generated for run-time forms");
-    // elements per workgroup: one thread per slot (Taylor-Hood: 22) or per node / point
-    const int epb = layout == 1 ? 256 / 22 : gen_epb(kind);
+    // elements per workgroup: one thread per slot (Taylor-Hood: 22) or per node / point;
+    // boundary-face forms: one thread per face
+    const int epb = layout == 2 ? 256 : (layout == 1 ? 256 / 22 : gen_epb(kind));
     const unsigned blocks = (unsigned)((a.e1 - a.e0 + epb - 1) / epb);
     if (a.e1 <= a.e0) return PYHIP_OK;
     ElemArgs<T> arg = a;
@@ -191,6 +199,19 @@ int pyhip_codegen_set_layout(int form_id, int layout) {
     PYHIP_REQUIRE(k >= 0 && k < (int)g_forms.size(), "unknown generated form");
     PYHIP_REQUIRE(layout == 0 || g_forms[k].kind == ELEM_QUAD_Q2, "the Taylor-Hood layout is for Q2 quadrilaterals");
     g_forms[k].layout = layout;
+    return PYHIP_OK;
+}
+
+int pyhip_codegen_set_face_form(int form_id, int ncoef) {
+    PYHIP_REQUIRE(ncoef >= 0 && ncoef <= 16, "coefficients per face: 0..16");
+    std::lock_guard<std::mutex> lock(g_mu);
+    const int k = form_id - PYHIP_GENERATED_FORM_BASE;
+    PYHIP_REQUIRE(k >= 0 && k < (int)g_forms.size(), "unknown generated form");
+    const int kind = g_forms[k].kind;
+    PYHIP_REQUIRE(kind == ELEM_TRI_P1 || kind == ELEM_TRI_P2 || kind == ELEM_QUAD_Q2,
+                  "boundary-face forms are generated for the line faces of P1 / P2 triangles and Q2 quadrilaterals");
+    g_forms[k].layout = 2;
+    g_forms[k].ncoef = ncoef;
     return PYHIP_OK;
 }
 

@@ -66,6 +66,55 @@ struct ElemArgs {
     int e0, e1;
 };
 
+// Line faces of the 2D element families (the interface elements of the boundary-face
+// forms): TElement<1,2> / TElement<1,3> lines on s in [0, 1] with TGauss<1,2> / TGauss<1,3>
+// (faces of P1 / P2 triangles; integral.cc:303-316) and the QElement<1,3> line on [-1, 1]
+// with Gauss<1,3> (faces of Q2 quadrilaterals; integral.cc:50-53): knot and weight of point
+// g, Lagrange shapes and their s-derivatives.
+template <int NN> struct FaceRule;
+template <> struct FaceRule<2> {
+    __device__ static void point(int g, double &s, double &w) {
+        s = g == 0 ? 0.5 * (-0.577350269189626 + 1.0) : 0.5 * (0.577350269189626 + 1.0);
+        w = 0.5;
+    }
+    template <typename T> __device__ static void shape(T s, T (&psi)[2], T (&dpsi)[2]) {
+        psi[0] = T(1.0) - s;
+        psi[1] = s;
+        dpsi[0] = T(-1.0);
+        dpsi[1] = T(1.0);
+    }
+};
+template <> struct FaceRule<3> {
+    __device__ static void point(int g, double &s, double &w) {
+        s = g == 0 ? 0.5 * (-0.774596669241483 + 1.0) : (g == 1 ? 0.5 : 0.5 * (0.774596669241483 + 1.0));
+        w = g == 1 ? (8.0 / 18.0) : (5.0 / 18.0);
+    }
+    template <typename T> __device__ static void shape(T s, T (&psi)[3], T (&dpsi)[3]) {
+        psi[0] = T(2.0) * (s - T(1.0)) * (s - T(0.5));
+        psi[1] = T(4.0) * (T(1.0) - s) * s;
+        psi[2] = T(2.0) * (s - T(0.5)) * s;
+        dpsi[0] = T(4.0) * s - T(3.0);
+        dpsi[1] = T(4.0) - T(8.0) * s;
+        dpsi[2] = T(4.0) * s - T(1.0);
+    }
+};
+
+// Q2-quadrilateral faces: QElement<1,3>, Gauss<1,3> on [-1, 1]
+struct FaceRuleQ3 {
+    __device__ static void point(int g, double &s, double &w) {
+        s = g == 0 ? -0.774596669241483 : (g == 1 ? 0.0 : 0.774596669241483);
+        w = g == 1 ? (8.0 / 9.0) : (5.0 / 9.0);
+    }
+    template <typename T> __device__ static void shape(T s, T (&psi)[3], T (&dpsi)[3]) {
+        psi[0] = T(0.5) * s * (s - T(1.0));
+        psi[1] = T(1.0) - s * s;
+        psi[2] = T(0.5) * s * (s + T(1.0));
+        dpsi[0] = s - T(0.5);
+        dpsi[1] = T(-2.0) * s;
+        dpsi[2] = s + T(0.5);
+    }
+};
+
 template <int KIND> struct ElemTraits;
 template <> struct ElemTraits<ELEM_TRI_P1> { static constexpr int NN = 3, DIM = 2, NQ = 3; };
 template <> struct ElemTraits<ELEM_TRI_P2> { static constexpr int NN = 6, DIM = 2, NQ = 7; };

@@ -23,49 +23,7 @@
 
 namespace pyhip {
 
-template <int NN> struct FaceRule;
-template <> struct FaceRule<2> {
-    __device__ static void point(int g, double &s, double &w) {
-        s = g == 0 ? 0.5 * (-0.577350269189626 + 1.0) : 0.5 * (0.577350269189626 + 1.0);
-        w = 0.5;
-    }
-    template <typename T> __device__ static void shape(T s, T (&psi)[2], T (&dpsi)[2]) {
-        psi[0] = T(1.0) - s;
-        psi[1] = s;
-        dpsi[0] = T(-1.0);
-        dpsi[1] = T(1.0);
-    }
-};
-template <> struct FaceRule<3> {
-    __device__ static void point(int g, double &s, double &w) {
-        s = g == 0 ? 0.5 * (-0.774596669241483 + 1.0) : (g == 1 ? 0.5 : 0.5 * (0.774596669241483 + 1.0));
-        w = g == 1 ? (8.0 / 18.0) : (5.0 / 18.0);
-    }
-    template <typename T> __device__ static void shape(T s, T (&psi)[3], T (&dpsi)[3]) {
-        psi[0] = T(2.0) * (s - T(1.0)) * (s - T(0.5));
-        psi[1] = T(4.0) * (T(1.0) - s) * s;
-        psi[2] = T(2.0) * (s - T(0.5)) * s;
-        dpsi[0] = T(4.0) * s - T(3.0);
-        dpsi[1] = T(4.0) - T(8.0) * s;
-        dpsi[2] = T(4.0) * s - T(1.0);
-    }
-};
-
-// Q2-quadrilateral faces: QElement<1,3>, Gauss<1,3> on [-1, 1]
-struct FaceRuleQ3 {
-    __device__ static void point(int g, double &s, double &w) {
-        s = g == 0 ? -0.774596669241483 : (g == 1 ? 0.0 : 0.774596669241483);
-        w = g == 1 ? (8.0 / 9.0) : (5.0 / 9.0);
-    }
-    template <typename T> __device__ static void shape(T s, T (&psi)[3], T (&dpsi)[3]) {
-        psi[0] = T(0.5) * s * (s - T(1.0));
-        psi[1] = T(1.0) - s * s;
-        psi[2] = T(0.5) * s * (s + T(1.0));
-        dpsi[0] = s - T(0.5);
-        dpsi[1] = T(-2.0) * s;
-        dpsi[2] = s + T(0.5);
-    }
-};
+// face rules (FaceRule<2>, FaceRule<3>, FaceRuleQ3): element_common.h
 
 template <typename T, int NN, typename RULE>
 __global__ __launch_bounds__(256) void flux_face_kernel(ElemArgs<T> a) {

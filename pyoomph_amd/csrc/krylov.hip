@@ -2240,7 +2240,7 @@ int pyhip_arclength_step(pyhip_problem *p, double *x, int param, const pyhip_new
     PYHIP_REQUIRE(o->jacobian_precision == PYHIP_F64 && o->linear.precision == PYHIP_F64,
                   "arclength continuation runs FP64 solves");
     for (auto &G : p->groups)
-        PYHIP_REQUIRE(G.form == PYHIP_FORM_BRATU || G.form == kFormFlux || gen_form_has_pderiv(G.form),
+        PYHIP_REQUIRE(G.form == PYHIP_FORM_BRATU || face_group_form(G.form) || gen_form_has_pderiv(G.form),
                       "arclength continuation needs sub-problems with a global parameter (Bratu lambda)");
     int rc;
     if ((rc = krylov_setup(p, o->linear.restart))) return rc;

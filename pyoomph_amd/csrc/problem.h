@@ -137,6 +137,13 @@ int gen_form_layout(int form);
 inline bool th_layout(int form) {
     return form == PYHIP_FORM_NAVIER_STOKES_TH || (form >= PYHIP_GENERATED_FORM_BASE && gen_form_layout(form) == 1);
 }
+// per-face coefficients of a generated boundary-face form (layout 2), -1 for other forms
+int gen_form_ncoef(int form);
+inline bool gen_face_form(int form) { return form >= PYHIP_GENERATED_FORM_BASE && gen_form_layout(form) == 2; }
+// groups of boundary-face elements: the built-in flux / traction / EnforcedBC faces and the
+// generated boundary-face forms
+inline bool face_group_form(int form) { return form == kFormFlux || gen_face_form(form); }
+inline int group_ncoef(int form, int kind) { return form == kFormFlux ? face_ncoef(kind) : gen_form_ncoef(form); }
 int launch_generated_f64(int form, const ElemArgs<double> &a, hipStream_t s);
 int launch_generated_f32(int form, const ElemArgs<float> &a, hipStream_t s);
 int launch_faces_f64(int kind, const ElemArgs<double> &a, hipStream_t s);

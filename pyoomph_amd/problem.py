@@ -57,9 +57,12 @@ class MeshProblem:
     (EnforcedBC pins redundant multipliers, bcs.py:199-243); the residual gains
     weak(alpha*u + beta, l_test) + weak(l, u_test) on every face. The Jacobian becomes a
     saddle point: solve with GMRES and ILU(0).
+    flux_form: a generated boundary-face form (``codegen.face.FaceFluxForm``) whose flux
+    expression replaces alpha*u + beta on the flux faces; flux then lists (boundary names,
+    coefficients) with the form's ``ncoef`` coefficients per condition.
     """
 
-    def __init__(self, mesh, equations, dirichlet=(), pin_extra=(), flux=None, enforced=None):
+    def __init__(self, mesh, equations, dirichlet=(), pin_extra=(), flux=None, enforced=None, flux_form=None):
         if not isinstance(equations, Equations):
             raise TypeError("equations must be an Equations instance")
         self.mesh = mesh
@@ -109,7 +112,24 @@ class MeshProblem:
         self.eqn = eqn.reshape(mesh.nnode, self.nvalue).astype(np.int32)
         self.ndof = int(free.sum())
         self.faces, self.face_coef = None, None
-        if flux is not None:
+        self.flux_form = flux_form
+        if flux_form is not None:
+            from .meshes import boundary_faces
+            if nv != 1:
+                raise ValueError("generated boundary-face forms are built for the scalar forms")
+            faces, coef = [], []
+            for names, c in flux:
+                names = [names] if isinstance(names, str) else list(names)
+                c = np.asarray(c, dtype=np.float64).reshape(-1)
+                if c.size != flux_form.ncoef:
+                    raise ValueError("the face form takes %d coefficients per condition" % flux_form.ncoef)
+                f = boundary_faces(mesh, names)
+                faces.append(f)
+                coef.append(np.tile(c, (len(f), 1)))
+            self.faces = np.ascontiguousarray(np.concatenate(faces), dtype=np.int32)
+            self.face_coef = np.ascontiguousarray(np.concatenate(coef).reshape(len(self.faces), -1),
+                                                  dtype=np.float64)
+        elif flux is not None:
             from .meshes import boundary_faces
             if nv not in (1, 3):
                 raise ValueError("boundary-face fluxes are built for the scalar forms and Taylor-Hood")
@@ -207,7 +227,10 @@ class Problem:
                 keep += [fc, cf]
                 d.nface = fc.shape[0]
                 d.face_conn = fc.ctypes.data_as(_lib.c_i32p)
-                d.face_coef = cf.ctypes.data_as(_lib.c_dp)
+                d.face_coef = cf.ctypes.data_as(_lib.c_dp) if cf.size else None
+                if mp.flux_form is not None:
+                    from .codegen.runtime import compile_face_form
+                    d.face_form = compile_face_form(mp.flux_form, mp.mesh.kind)
             if mp.enforced is not None and len(mp.enforced):
                 keep += [mp.enforced, mp.enforced_coef]
                 d.nenforced = mp.enforced.shape[0]

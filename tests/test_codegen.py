@@ -177,3 +177,24 @@ def test_emitted_bratu_matches_the_compiled_operation_order():
     assert "W * ((((Q[0] * dxi[0]) + (Q[1] * dxi[1])) + (Q[2] * P[row])))" in src
     assert "W * ((((dxj[0] * dxi[0]) + (dxj[1] * dxi[1])) + ((Q[2] * P[j]) * P[row])))" in src
     assert "W * (((Q[0] * P[j]) * P[row]))" in src   # d/dlambda: -(exp(u)) psi_j psi_i
+
+
+def test_face_forms_compile_and_differentiate():
+    """Generated boundary-face forms: hiprtc compiles the kernels of every line-face family
+    (no GPU needed), and the printed point quantities are the flux and its u-derivative."""
+    from pyoomph_amd.codegen import expr as E
+    from pyoomph_amd.codegen.face import FaceFluxForm, emit_face_kernels
+    from pyoomph_amd.codegen.runtime import compile_face_form
+    f = FaceFluxForm("t_face_cubic", lambda s: (s.c[0] * s.u ** 3 + s.c[1] * s.x[1]) - s.p[0] * E.exp(s.u),
+                     ncoef=2, nparams=1, global_param=0)
+    for kind in ("tri_p1", "tri_p2", "quad_q2"):
+        assert compile_face_form(f, kind, load=False) > 0
+    src, name, info = emit_face_kernels(f, "tri_p2")
+    assert info["has_param_derivative"] and info["face_nodes"] == 3
+    s, F = f.build()
+    env = {"u": 0.7, "x0": 0.1, "x1": 0.4, "T(cf[0])": 2.0, "T(cf[1])": 0.3, "T(par[0])": 0.8}
+    import math
+    assert abs(E.evaluate(E.diff(F, s.u), env) - (2.0 * 3 * 0.7 ** 2 - 0.8 * math.exp(0.7))) < 1e-14
+    assert abs(E.evaluate(E.diff(F, s.p[0]), env) + math.exp(0.7)) < 1e-14
+    with pytest.raises(ValueError):
+        emit_face_kernels(f, "hex_q2")

@@ -265,3 +265,92 @@ def test_generated_stokes_brinkman(gpu):
     corner = np.isin(np.arange(mesh.nnode), mesh.elements[:, [0, 2, 6, 8]].reshape(-1))
     y = mesh.coords[corner, 1]
     assert np.max(np.abs(U[corner, 2] - (-1.5 * y))) <= 1e-7
+
+
+# ---- generated boundary-face forms --------------------------------------------------------
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _face_pair(space, split, flux_form=None, flux=None, lam=1.0, N=7):
+    from pyoomph_amd.equations import BratuEquation
+    from pyoomph_amd.problem import MeshProblem
+    mesh = RectangularQuadMesh(N, split_in_tris=split, space=space)
+    return MeshProblem(mesh, BratuEquation("u", space=space, lambda_=lam), dirichlet=[(("left",), {0: 0.0})],
+                       flux=flux, flux_form=flux_form)
+
+
+FACE_CASES = [("C1", "left"), ("C2", "left"), ("C2", False)]
+
+
+@pytest.mark.parametrize("space,split", FACE_CASES)
+def test_generated_linear_flux_equals_the_built_in_faces_bitwise(gpu, space, split):
+    """NeumannBC(u = c0*u + c1) generated from its expression reproduces the library's
+    flux faces bit for bit (residual, Jacobian pattern and values, dR/dlambda)."""
+    from pyoomph_amd.codegen.face import FaceFluxForm
+    from pyoomph_amd.problem import Problem
+    lin = FaceFluxForm("gen_robin", lambda s: s.c[0] * s.u + s.c[1], ncoef=2)
+    A = Problem(_face_pair(space, split, flux=[(("right", "top"), 1.3, -0.4)]))
+    B = Problem(_face_pair(space, split, flux_form=lin, flux=[(("right", "top"), (1.3, -0.4))]))
+    for seed in (1, 2):
+        x = np.random.default_rng(seed).uniform(-0.3, 0.3, A.ndof)
+        A.set_current_dofs(x)
+        B.set_current_dofs(x)
+        RA, JA = A.assemble_jacobian()
+        RB, JB = B.assemble_jacobian()
+        assert np.array_equal(RA, RB)
+        assert np.array_equal(JA.indptr, JB.indptr) and np.array_equal(JA.indices, JB.indices)
+        assert np.array_equal(JA.data, JB.data)
+        assert np.array_equal(A.get_residuals(dparameter="lambda"), B.get_residuals(dparameter="lambda"))
+
+
+def _cubic_form():
+    from pyoomph_amd.codegen import expr as E
+    from pyoomph_amd.codegen.face import FaceFluxForm
+    return FaceFluxForm("gen_cubic_flux", lambda s: (s.c[0] * s.u ** 3 + s.c[1] * s.x[1]) - s.p[0] * E.exp(s.u),
+                        ncoef=2, nparams=1, global_param=0)
+
+
+def _cubic_oracle(space, split, lam, c, N=7):
+    from oracle import assembly as oa
+    from oracle import faces as of
+    from oracle import meshes as om
+    m = om.rectangular_quad_mesh(N, split, space)
+    faces = of.boundary_faces(m, ["right", "top"])
+    C = np.tile(np.asarray(c, dtype=np.float64), (len(faces), 1))
+    flux = lambda u, x0, x1, C: (C[:, 0] * ((u * u) * u) + C[:, 1] * x1) - lam * np.exp(u)  # noqa: E731
+    dflux = lambda u, x0, x1, C: C[:, 0] * (3.0 * (u * u)) - lam * np.exp(u)  # noqa: E731
+    dp = lambda u, x0, x1, C: -np.exp(u)  # noqa: E731
+    pinned = m.on_boundary(["left"])[:, None]
+    return oa.OracleProblem(m, oa.FORM_BRATU, 1, pinned, np.zeros((m.nnode, 1)), [lam], faces=faces,
+                            face_expr=(flux, dflux, dp, dp), face_coef=C)
+
+
+@pytest.mark.parametrize("space,split", FACE_CASES)
+def test_generated_nonlinear_flux_matches_the_oracle(gpu, space, split):
+    """NeumannBC(u = c0 u^3 + c1 y - lambda e^u): assembly, dR/dlambda and dJ/dlambda within
+    1e-12 of the oracle's expression faces (exponentials may round differently), Newton
+    within 1e-8."""
+    from pyoomph_amd.problem import Problem
+    lam, c = 0.8, (2.0, 0.3)
+    P = Problem(_face_pair(space, split, flux_form=_cubic_form(), flux=[(("right", "top"), c)], lam=lam))
+    op = _cubic_oracle(space, split, lam, c)
+    assert P.ndof == op.ndof
+    x = np.random.default_rng(3).uniform(-0.3, 0.3, P.ndof)
+    P.set_current_dofs(x)
+    R, J = P.assemble_jacobian()
+    r, rs, col, val = op.get_jacobian(x)
+    assert np.array_equal(J.indptr, rs) and np.array_equal(J.indices, col)
+    assert rel(J.data, val) <= 1e-12 and rel(R, r) <= 1e-12
+    assert rel(P.get_residuals(dparameter="lambda"), op.get_residuals(x, deriv=1)) <= 1e-12
+    _, dJ = P.assemble_jacobian(dparameter="lambda")
+    _, drs, dcol, dval = op.get_jacobian(x, deriv=1)
+    assert np.array_equal(dJ.indptr, drs) and np.array_equal(dJ.indices, dcol) and rel(dJ.data, dval) <= 1e-12
+    # Newton from u = 0 where it converges (a milder flux and lambda)
+    lam, c = 0.2, (0.5, 0.3)
+    P = Problem(_face_pair(space, split, flux_form=_cubic_form(), flux=[(("right", "top"), c)], lam=lam))
+    op = _cubic_oracle(space, split, lam, c)
+    P.krylov, P.preconditioner = "gmres", "ilu0"
+    st = P.solve()
+    assert st[0]["converged"]
+    assert rel(P.get_current_dofs(), on.newton_solve(op, np.zeros(op.ndof), tol=1e-12)) <= 1e-8

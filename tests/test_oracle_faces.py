@@ -232,3 +232,31 @@ def test_taylor_hood_traction_load_closed_form():
     ux, uy = op.eqn[:, 0], op.eqn[:, 1]
     assert np.allclose(D, D.T, atol=1e-15)
     assert np.all(D[np.ix_(ux[ux >= 0], uy[uy >= 0])] == 0.0)       # components do not couple
+
+
+@pytest.mark.parametrize("nn,quad", [(2, False), (3, False), (3, True)])
+def test_expression_faces_closed_form_and_derivative(nn, quad):
+    """Expression fluxes (the generated boundary-face forms): a flux x0 on the edge from
+    (0, 0) to (L, 0) integrates to the closed-form loads int x psi_i ds; the Jacobian of a
+    nonlinear flux is the central difference of the residual."""
+    L = 1.3
+    X = _edge(nn, (0.0, 0.0), (L, 0.0))
+    C = np.zeros((1, 0))
+    R, _ = of.expression_face_outputs(X, np.zeros((1, nn)), C, lambda u, x0, x1, C: x0,
+                                      lambda u, x0, x1, C: 0.0 * u, 1, quad=quad)
+    exact = [L * L / 6, L * L / 3] if nn == 2 else [0.0, L * L / 3, L * L / 6]
+    assert np.allclose(R[0], exact, rtol=1e-13, atol=1e-15)
+    rng = np.random.default_rng(4)
+    X = _edge(nn, (0.2, 0.1), (0.9, 0.5))
+    C = np.array([[1.7, -0.3]])
+    F = lambda u, x0, x1, C: C[:, 0] * (u * u * u) + C[:, 1] * x1 * u  # noqa: E731
+    D = lambda u, x0, x1, C: C[:, 0] * (3.0 * (u * u)) + C[:, 1] * x1  # noqa: E731
+    U = rng.uniform(-0.5, 0.5, (1, nn))
+    R, J = of.expression_face_outputs(X, U, C, F, D, 1, quad=quad)
+    h = 1e-6
+    for k in range(nn):
+        e = np.zeros((1, nn))
+        e[0, k] = h
+        Rp, _ = of.expression_face_outputs(X, U + e, C, F, D, 0, quad=quad)
+        Rm, _ = of.expression_face_outputs(X, U - e, C, F, D, 0, quad=quad)
+        assert np.allclose(J[0][:, k], (Rp[0] - Rm[0]) / (2 * h), rtol=1e-8, atol=1e-10)
