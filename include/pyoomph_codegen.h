@@ -58,6 +58,15 @@ int pyhip_codegen_set_layout(int form_id, int layout);
  * src/codegen.cpp:3912-4109, pyoomph/meshes/bcs.py:71-89). Set before the form is used. */
 int pyhip_codegen_set_face_form(int form_id, int ncoef);
 
+/* Named residual tables of a generated bulk form: pyoomph's add_residual(...,
+ * destination=name) collects a second residual (and Jacobian) of the same equations under
+ * a name (pyoomph/generic/codegen.py:777-800; the functable's res_jac_names and
+ * ResidualAndJacobian[i], src/jitbridge.h:329-497), e.g. a mass matrix of the fields. The
+ * table's kernels are a generated form of their own (same element family and layout);
+ * this call attaches them to form_id under `name`. pyhip_problem_set_solved_residual
+ * switches between the tables. */
+int pyhip_codegen_add_residual_table(int form_id, const char *name, int table_form_id);
+
 /* Registered generated forms (0 .. count-1 are ids PYHIP_GENERATED_FORM_BASE + k). */
 int pyhip_codegen_count(int *count);
 

@@ -142,6 +142,14 @@ int pyhip_problem_info(const pyhip_problem *p, int64_t *ndof, int64_t *nnz,
 /* dof offset of each sub-problem, host [nsub+1] */
 int pyhip_problem_dof_offsets(const pyhip_problem *p, int64_t *offsets);
 int pyhip_problem_set_params(pyhip_problem *p, int sub, const double *params, int nparams);
+/* The residual the following assemblies and solves use: "" the default one, or a named
+ * residual table of the generated forms (pyhip_codegen_add_residual_table). Replaces
+ * Problem::_set_solved_residual (src/problem.cpp:621-633) -> the element codes'
+ * _set_solved_residual (:186-200): element groups without that table contribute nothing
+ * (the compiled forms and the boundary faces have the default residual only).
+ * *ngroups receives the number of bulk element groups that have it (the reference throws
+ * when none has it; pyoomph_amd.problem.Problem._set_solved_residual does). */
+int pyhip_problem_set_solved_residual(pyhip_problem *p, const char *name, int *ngroups);
 /* Column-sorted structural pattern to host buffers [ndof+1], [nnz]. */
 int pyhip_pattern_copy(const pyhip_problem *p, int32_t *row_start, int32_t *col_index);
 
@@ -238,11 +246,13 @@ typedef struct {
     int hvec;        /* row of the Hessian vectors for the two Hessian kinds (else ignored) */
     int transposed;  /* Hessian kinds: 1 = entry (i, k) = sum_j y_j d^2 R_j/(du_i du_k)
                         (jitbridge.h:678-691), 0 = sum_j y_j d^2 R_i/(du_j du_k) */
+    const char *contribution;  /* the residual table (NULL or "": the default residual; a
+                        name: a generated form's table, pyhip_codegen_add_residual_table) */
 } pyhip_multi_request;
 
 /* Numbering of the outputs (CustomMultiAssembleHandler, src/bifurcation.cpp:5393-5480):
- * return_indices[i] = k >= 0 for the k-th vector, -1 - m for the m-th matrix, counted in
- * request order. Errors as the reference's: the same quantity requested twice, a parameter
+ * return_indices[i] = k >= 0 for the k-th vector, -1 - m for the m-th matrix, counted per
+ * contribution in order of first appearance and within it in request order. Errors as the reference's: the same quantity requested twice, a parameter
  * or Hessian vector missing, a Hessian vector index >= nhvec, transposed and plain Hessian
  * products in one call. Host-only: no device is touched. */
 int pyhip_multi_assembly_plan(int nreq, const pyhip_multi_request *req, int nhvec,

@@ -94,7 +94,7 @@ class ArclengthState(ctypes.Structure):
 
 class MultiRequest(ctypes.Structure):
     _fields_ = [("what", ctypes.c_int), ("param", ctypes.c_int), ("hvec", ctypes.c_int),
-                ("transposed", ctypes.c_int)]
+                ("transposed", ctypes.c_int), ("contribution", ctypes.c_char_p)]
 
 
 # request kinds of the multi-assembly (PYHIP_MA_*), by the reference's "what" strings
@@ -131,6 +131,9 @@ SIGNATURES = {
     "pyhip_codegen_set_fields": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
     "pyhip_codegen_set_layout": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "pyhip_codegen_set_face_form": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "pyhip_codegen_add_residual_table": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
+    "pyhip_problem_set_solved_residual": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p,
+                                                         ctypes.POINTER(ctypes.c_int)]),
     "pyhip_newton_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(NewtonOpts),
                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(NewtonStepState)]),
     "pyhip_problem_set_aux": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dp, ctypes.c_int64]),

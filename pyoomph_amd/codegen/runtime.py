@@ -66,6 +66,13 @@ def compile_form(form, kind, load=True):
     # (Taylor-Hood tables list velocity and pressure)
     _lib.check(L.pyhip_codegen_set_fields(fid.value, b"u", b"f" if form.aux else None,
                                           int(form.global_param or 0)))
+    # named residual tables: each a generated form of its own, attached under its name
+    for k, (tname, (tflux, tsource)) in enumerate(getattr(form, "tables", {}).items()):
+        tform = scalar.ScalarWeakForm("%s__table%d" % (form.name, k), tflux, tsource, nparams=form.nparams,
+                                      global_param=form.global_param,
+                                      global_param_name=form.global_param_name or "lambda", aux=form.aux)
+        tid = compile_form(tform, kid)
+        _lib.check(L.pyhip_codegen_add_residual_table(fid.value, tname.encode(), tid))
     _registry[key] = fid.value
     return fid.value
 

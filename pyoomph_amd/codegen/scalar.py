@@ -47,14 +47,18 @@ class ScalarWeakForm:
 
     nparams: parameters per sub-problem (params array of the problem); global_param: index
     of the parameter with a derivative entry (or None) and its name; aux: the form reads the
-    auxiliary nodal field f."""
+    auxiliary nodal field f; tables: named residual tables {name: (flux, source)} of the same
+    unknown (selected with Problem._set_solved_residual / assemble_jacobian(which_one=name))."""
 
     def __init__(self, name, flux, source, *, nparams=0, global_param=None, global_param_name="lambda",
-                 aux=False):
+                 aux=False, tables=None):
         if not name.isidentifier():
             raise ValueError("form name must be a C identifier")
         self.name = name
         self.flux_fn, self.source_fn = flux, source
+        # named residual tables: {name: (flux, source)}, the residuals pyoomph collects with
+        # add_residual(..., destination=name) (pyoomph/generic/codegen.py:777-800)
+        self.tables = dict(tables or {})
         self.nparams = int(nparams)
         self.global_param = global_param
         self.global_param_name = global_param_name if global_param is not None else None

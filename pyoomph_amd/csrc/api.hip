@@ -180,6 +180,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
         G.nn = elem_nnode(G.kind);
         G.dim = elem_dim(G.kind);
     }
+    for (auto &G : P->groups) G.active = G.form;
     P->subs.resize(nsub);
     P->dof_off_h.assign(nsub + 1, 0);
     int64_t node_off = 0;
@@ -475,6 +476,25 @@ int pyhip_problem_set_stream(pyhip_problem *p, void *stream) {
     return PYHIP_OK;
 }
 
+int pyhip_problem_set_solved_residual(pyhip_problem *p, const char *name, int *ngroups) {
+    PYHIP_REQUIRE(p && name, "null argument");
+    const std::string n(name);
+    int found = 0;
+    for (auto &G : p->groups) {
+        if (n.empty()) {
+            G.active = G.form;
+        } else if (!face_group_form(G.form) && G.form >= PYHIP_GENERATED_FORM_BASE) {
+            G.active = gen_form_table(G.form, n);
+        } else {
+            G.active = -1;   // the compiled forms and the faces have the default residual only
+        }
+        if (G.active >= 0 && !face_group_form(G.form)) found++;
+    }
+    p->solved_residual = n;
+    if (ngroups) *ngroups = found;
+    return PYHIP_OK;
+}
+
 int pyhip_problem_set_aux(pyhip_problem *p, int sub, const double *aux, int64_t nnode) {
     PYHIP_REQUIRE(p && sub >= 0 && sub < p->nsub && aux, "bad arguments");
     const SubInfo &S = p->subs[sub];
@@ -541,6 +561,19 @@ static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int der
     int rc;
     for (size_t g = 0; g < p->groups.size(); g++) {
         const Group &G = p->groups[g];
+        if (G.active < 0) {
+            // no table of the solved residual on this group: its elements contribute nothing
+            const size_t es = precision == PYHIP_F64 ? sizeof(double) : sizeof(float);
+            for (const auto &run : element_runs(p, (int)g, sub_on_h)) {
+                const size_t n = (size_t)(run.second - run.first);
+                PYHIP_CHECK_HIP(hipMemsetAsync((char *)Er + (G.er_off + run.first * G.ns) * es, 0, n * G.ns * es, p->st));
+                if (flag)
+                    PYHIP_CHECK_HIP(hipMemsetAsync((char *)Ej + (G.ej_off + run.first * G.ns * G.ns) * es, 0,
+                                                   n * G.ns * G.ns * es, p->st));
+            }
+            continue;
+        }
+        const int form = G.active;
         for (const auto &run : element_runs(p, (int)g, sub_on_h)) {
             if (precision == PYHIP_F64) {
                 ElemArgs<double> a{};
@@ -562,11 +595,11 @@ static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int der
                 a.htr = p->htr;
                 a.fcoef = G.coef;
                 TimeScope ts(&p->timer, "element", p->st);
-                if (G.form == kFormFlux) {
+                if (form == kFormFlux) {
                     if ((rc = launch_faces_f64(G.kind, a, p->st))) return rc;
-                } else if (G.form >= PYHIP_GENERATED_FORM_BASE) {
-                    if ((rc = launch_generated_f64(G.form, a, p->st))) return rc;
-                } else if ((rc = launch_elements_f64(G.kind, G.form, a, p->st)))
+                } else if (form >= PYHIP_GENERATED_FORM_BASE) {
+                    if ((rc = launch_generated_f64(form, a, p->st))) return rc;
+                } else if ((rc = launch_elements_f64(G.kind, form, a, p->st)))
                     return rc;
             } else {
                 ElemArgs<float> a{};
@@ -588,11 +621,11 @@ static int assemble_into(pyhip_problem *p, const double *dofs, int flag, int der
                 a.htr = p->htr;
                 a.fcoef = G.coef;
                 TimeScope ts(&p->timer, "element", p->st);
-                if (G.form == kFormFlux) {
+                if (form == kFormFlux) {
                     if ((rc = launch_faces_f32(G.kind, a, p->st))) return rc;
-                } else if (G.form >= PYHIP_GENERATED_FORM_BASE) {
-                    if ((rc = launch_generated_f32(G.form, a, p->st))) return rc;
-                } else if ((rc = launch_elements_f32(G.kind, G.form, a, p->st)))
+                } else if (form >= PYHIP_GENERATED_FORM_BASE) {
+                    if ((rc = launch_generated_f32(form, a, p->st))) return rc;
+                } else if ((rc = launch_elements_f32(G.kind, form, a, p->st)))
                     return rc;
             }
         }

@@ -21,6 +21,7 @@ struct GenForm {
     int global_param = 0;                 // the parameter dR/dp is taken in
     int layout = 0;                       // 0 one value per node, 1 Taylor-Hood (22 slots), 2 boundary faces
     int ncoef = 0;                        // layout 2: coefficients per face
+    std::vector<std::pair<std::string, int>> tables;   // named residual tables: (name, form id)
     hipModule_t mod = nullptr;
     hipFunction_t f64 = nullptr, f64_tr = nullptr, f32 = nullptr, f32_tr = nullptr;
 };
@@ -100,6 +101,15 @@ int gen_form_layout(int form) {
     std::lock_guard<std::mutex> lock(g_mu);
     const int k = form - PYHIP_GENERATED_FORM_BASE;
     return (k >= 0 && k < (int)g_forms.size()) ? g_forms[k].layout : 0;
+}
+
+int gen_form_table(int form, const std::string &name) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    const int k = form - PYHIP_GENERATED_FORM_BASE;
+    if (k < 0 || k >= (int)g_forms.size()) return -1;
+    for (const auto &t : g_forms[k].tables)
+        if (t.first == name) return t.second;
+    return -1;
 }
 
 int gen_form_ncoef(int form) {
@@ -212,6 +222,22 @@ int pyhip_codegen_set_face_form(int form_id, int ncoef) {
                   "boundary-face forms are generated for the line faces of P1 / P2 triangles and Q2 quadrilaterals");
     g_forms[k].layout = 2;
     g_forms[k].ncoef = ncoef;
+    return PYHIP_OK;
+}
+
+int pyhip_codegen_add_residual_table(int form_id, const char *name, int table_form_id) {
+    PYHIP_REQUIRE(name && name[0], "a residual table needs a name (\"\" is the form itself)");
+    std::lock_guard<std::mutex> lock(g_mu);
+    const int k = form_id - PYHIP_GENERATED_FORM_BASE, t = table_form_id - PYHIP_GENERATED_FORM_BASE;
+    PYHIP_REQUIRE(k >= 0 && k < (int)g_forms.size() && t >= 0 && t < (int)g_forms.size(), "unknown generated form");
+    PYHIP_REQUIRE(g_forms[k].kind == g_forms[t].kind && g_forms[k].layout == g_forms[t].layout && g_forms[k].layout != 2,
+                  "a residual table is a bulk form of the same element family and value layout");
+    for (auto &e : g_forms[k].tables)
+        if (e.first == name) {
+            e.second = table_form_id;
+            return PYHIP_OK;
+        }
+    g_forms[k].tables.push_back({name, table_form_id});
     return PYHIP_OK;
 }
 

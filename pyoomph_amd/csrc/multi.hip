@@ -13,6 +13,7 @@
 #include "common.h"
 #include "problem.h"
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -41,14 +42,26 @@ static const char *ma_name(int w) {
     return "?";
 }
 
-// Numbering of CustomMultiAssembleHandler for the one contribution a device problem has
-// (its residual): vectors and matrices are counted in request order, a vector request
-// returns its index k >= 0, a matrix request -1 - m. The same quantity twice is an error,
-// and so is a mix of transposed and plain Hessian products, as in the reference.
+static std::string ma_contrib(const pyhip_multi_request &r) { return r.contribution ? r.contribution : ""; }
+
+// the distinct contributions of the requests, in order of first appearance
+static std::vector<std::string> ma_contributions(int nreq, const pyhip_multi_request *req) {
+    std::vector<std::string> u;
+    for (int i = 0; i < nreq; i++) {
+        const std::string c = ma_contrib(req[i]);
+        if (std::find(u.begin(), u.end(), c) == u.end()) u.push_back(c);
+    }
+    return u;
+}
+
+// Numbering of CustomMultiAssembleHandler (src/bifurcation.cpp:5393-5480): per contribution
+// (residual table) in order of first appearance, its requests in request order; a vector
+// request returns its index k >= 0, a matrix request -1 - m. The same quantity twice for
+// one contribution is an error, and so is a mix of transposed and plain Hessian products,
+// as in the reference.
 static int ma_plan(int nreq, const pyhip_multi_request *req, int nhvec, int *ret, int *nvec, int *nmat) {
     PYHIP_REQUIRE(nreq >= 0 && (nreq == 0 || (req && ret)) && nvec && nmat, "null argument");
     bool plain = false, transposed = false;
-    int nv = 0, nm = 0;
     for (int i = 0; i < nreq; i++) {
         const pyhip_multi_request &r = req[i];
         PYHIP_REQUIRE(r.what >= PYHIP_MA_RESIDUALS && r.what <= PYHIP_MA_MASS_MATRIX_HESSIAN_VECTOR_PRODUCT,
@@ -63,14 +76,22 @@ static int ma_plan(int nreq, const pyhip_multi_request *req, int nhvec, int *ret
             PYHIP_REQUIRE(!(plain && transposed),
                           "cannot assemble transposed and non-transposed Hessian vector products simultaneously");
         }
-        for (int j = 0; j < i; j++) {
-            const pyhip_multi_request &q = req[j];
-            bool same = q.what == r.what;
-            if (same && ma_param_kind(r.what)) same = q.param == r.param;
-            if (same && ma_hessian_kind(r.what)) same = q.hvec == r.hvec && (q.transposed != 0) == (r.transposed != 0);
-            PYHIP_REQUIRE(!same, std::string("multiple '") + ma_name(r.what) + "' requests for the same contribution");
+    }
+    int nv = 0, nm = 0;
+    for (const std::string &c : ma_contributions(nreq, req)) {
+        for (int i = 0; i < nreq; i++) {
+            const pyhip_multi_request &r = req[i];
+            if (ma_contrib(r) != c) continue;
+            for (int j = 0; j < i; j++) {
+                const pyhip_multi_request &q = req[j];
+                bool same = q.what == r.what && ma_contrib(q) == c;
+                if (same && ma_param_kind(r.what)) same = q.param == r.param;
+                if (same && ma_hessian_kind(r.what)) same = q.hvec == r.hvec && (q.transposed != 0) == (r.transposed != 0);
+                PYHIP_REQUIRE(!same, std::string("multiple '") + ma_name(r.what) + "' requests for the same contribution '" +
+                                         c + "'");
+            }
+            ret[i] = ma_vector_kind(r.what) ? nv++ : -1 - nm++;
         }
-        ret[i] = ma_vector_kind(r.what) ? nv++ : -1 - nm++;
     }
     *nvec = nv;
     *nmat = nm;
@@ -127,10 +148,12 @@ int pyhip_assemble_multi(pyhip_problem *p, const double *dofs, int nreq, const p
     }
     if ((rc = ma_scratch(p))) return rc;
 
+    std::string cur;   // the contribution being assembled
+    auto mine = [&](int i) { return ma_contrib(req[i]) == cur; };
     auto find = [&](int what, int param, int hvec, int tr) -> int {
         for (int i = 0; i < nreq; i++) {
             const pyhip_multi_request &r = req[i];
-            if (r.what != what) continue;
+            if (r.what != what || !mine(i)) continue;
             if (ma_param_kind(what) && r.param != param) continue;
             if (ma_hessian_kind(what) && (r.hvec != hvec || (r.transposed != 0) != (tr != 0))) continue;
             return i;
@@ -151,70 +174,93 @@ int pyhip_assemble_multi(pyhip_problem *p, const double *dofs, int nreq, const p
         return PYHIP_OK;
     };
 
-    // residual and Jacobian: one element pass
-    const int iR = find(PYHIP_MA_RESIDUALS, 0, 0, 0), iJ = find(PYHIP_MA_JACOBIAN, 0, 0, 0);
-    if (iR >= 0 || iJ >= 0) {
-        if ((rc = assemble_subset(p, dofs, iJ >= 0 ? 1 : 0, -1, PYHIP_F64, vec_out(iR), p->ma_vals, nullptr)))
-            return rc;
-        if (iJ >= 0 && (rc = export_to(iJ))) return rc;
-    }
-    // mass matrix dR/d(du/dt): the flag-2 pass with the time derivative on (BDF1 weights
-    // over the current dofs when the problem is steady; the matrix does not depend on them)
-    const int iM = find(PYHIP_MA_MASS_MATRIX, 0, 0, 0);
-    if (iM >= 0) {
-        const bool was = p->transient;
-        const double tw[3] = {p->tw[0], p->tw[1], p->tw[2]};
-        const double *h1 = p->hist1, *h2 = p->hist2;
-        if (!was) {
-            p->transient = true;
-            p->tw[0] = 1.0, p->tw[1] = -1.0, p->tw[2] = 0.0;
-            p->hist1 = p->hist2 = dofs;
+    // one contribution: its residual table is the solved residual meanwhile
+    auto run = [&]() -> int {
+        int rc;
+        // residual and Jacobian: one element pass
+        const int iR = find(PYHIP_MA_RESIDUALS, 0, 0, 0), iJ = find(PYHIP_MA_JACOBIAN, 0, 0, 0);
+        if (iR >= 0 || iJ >= 0) {
+            if ((rc = assemble_subset(p, dofs, iJ >= 0 ? 1 : 0, -1, PYHIP_F64, vec_out(iR), p->ma_vals, nullptr)))
+                return rc;
+            if (iJ >= 0 && (rc = export_to(iJ))) return rc;
         }
-        rc = assemble_subset(p, dofs, 2, -1, PYHIP_F64, p->ma_res, p->ma_vals, nullptr);
-        p->transient = was;
-        p->tw[0] = tw[0], p->tw[1] = tw[1], p->tw[2] = tw[2];
-        p->hist1 = h1, p->hist2 = h2;
-        if (rc || (rc = export_to(iM))) return rc;
-    }
-    // parameter derivatives, one element pass per parameter
-    for (int i = 0; i < nreq; i++) {
-        if (!ma_param_kind(req[i].what)) continue;
-        const int prm = req[i].param;
-        bool first = true;
-        for (int j = 0; j < i; j++) first = first && !(ma_param_kind(req[j].what) && req[j].param == prm);
-        if (!first) continue;
-        const int iDR = find(PYHIP_MA_DRESIDUALS_DPARAMETER, prm, 0, 0);
-        const int iDJ = find(PYHIP_MA_DJACOBIAN_DPARAMETER, prm, 0, 0);
-        const int iDM = find(PYHIP_MA_DMASS_MATRIX_DPARAMETER, prm, 0, 0);
-        // the same argument checks as a plain assembly (the parameter must exist)
-        if ((rc = assemble_subset(p, dofs, iDJ >= 0 ? 1 : 0, prm, PYHIP_F64, vec_out(iDR), p->ma_vals, nullptr)))
-            return rc;
-        if (iDJ >= 0 && (rc = export_to(iDJ))) return rc;
-        // the mass coefficients of the compiled and generated scalar forms do not depend on
-        // a parameter: dM/dp has no non-zero element contribution
-        if (iDM >= 0 && (rc = zero_to(iDM))) return rc;
-    }
-    // Hessian-vector products, one element pass per direction
-    for (int i = 0; i < nreq; i++) {
-        if (!ma_hessian_kind(req[i].what)) continue;
-        const int hv = req[i].hvec, tr = req[i].transposed != 0;
-        bool first = true;
-        for (int j = 0; j < i; j++)
-            first = first && !(ma_hessian_kind(req[j].what) && req[j].hvec == hv && (req[j].transposed != 0) == tr);
-        if (!first) continue;
-        const int iH = find(PYHIP_MA_HESSIAN_VECTOR_PRODUCT, 0, hv, tr);
-        const int iMH = find(PYHIP_MA_MASS_MATRIX_HESSIAN_VECTOR_PRODUCT, 0, hv, tr);
-        if (iH >= 0) {
-            p->hvec = hvecs + (int64_t)hv * p->ndof;
-            p->htr = tr;
-            rc = assemble_subset(p, dofs, 1, -1, PYHIP_F64, p->ma_res, p->ma_vals, nullptr);
-            p->hvec = nullptr;
-            p->htr = 0;
-            if (rc || (rc = export_to(iH))) return rc;
+        // mass matrix dR/d(du/dt): the flag-2 pass with the time derivative on (BDF1 weights
+        // over the current dofs when the problem is steady; the matrix does not depend on them)
+        const int iM = find(PYHIP_MA_MASS_MATRIX, 0, 0, 0);
+        if (iM >= 0) {
+            const bool was = p->transient;
+            const double tw[3] = {p->tw[0], p->tw[1], p->tw[2]};
+            const double *h1 = p->hist1, *h2 = p->hist2;
+            if (!was) {
+                p->transient = true;
+                p->tw[0] = 1.0, p->tw[1] = -1.0, p->tw[2] = 0.0;
+                p->hist1 = p->hist2 = dofs;
+            }
+            rc = assemble_subset(p, dofs, 2, -1, PYHIP_F64, p->ma_res, p->ma_vals, nullptr);
+            p->transient = was;
+            p->tw[0] = tw[0], p->tw[1] = tw[1], p->tw[2] = tw[2];
+            p->hist1 = h1, p->hist2 = h2;
+            if (rc || (rc = export_to(iM))) return rc;
         }
-        // the mass matrices of these forms do not depend on the dofs
-        if (iMH >= 0 && (rc = zero_to(iMH))) return rc;
+        // parameter derivatives, one element pass per parameter
+        for (int i = 0; i < nreq; i++) {
+            if (!ma_param_kind(req[i].what) || !mine(i)) continue;
+            const int prm = req[i].param;
+            bool first = true;
+            for (int j = 0; j < i; j++) first = first && !(mine(j) && ma_param_kind(req[j].what) && req[j].param == prm);
+            if (!first) continue;
+            const int iDR = find(PYHIP_MA_DRESIDUALS_DPARAMETER, prm, 0, 0);
+            const int iDJ = find(PYHIP_MA_DJACOBIAN_DPARAMETER, prm, 0, 0);
+            const int iDM = find(PYHIP_MA_DMASS_MATRIX_DPARAMETER, prm, 0, 0);
+            // the same argument checks as a plain assembly (the parameter must exist)
+            if ((rc = assemble_subset(p, dofs, iDJ >= 0 ? 1 : 0, prm, PYHIP_F64, vec_out(iDR), p->ma_vals, nullptr)))
+                return rc;
+            if (iDJ >= 0 && (rc = export_to(iDJ))) return rc;
+            // the mass coefficients of the compiled and generated scalar forms do not depend on
+            // a parameter: dM/dp has no non-zero element contribution
+            if (iDM >= 0 && (rc = zero_to(iDM))) return rc;
+        }
+        // Hessian-vector products, one element pass per direction
+        for (int i = 0; i < nreq; i++) {
+            if (!ma_hessian_kind(req[i].what) || !mine(i)) continue;
+            const int hv = req[i].hvec, tr = req[i].transposed != 0;
+            bool first = true;
+            for (int j = 0; j < i; j++)
+                first = first && !(mine(j) && ma_hessian_kind(req[j].what) && req[j].hvec == hv &&
+                                   (req[j].transposed != 0) == tr);
+            if (!first) continue;
+            const int iH = find(PYHIP_MA_HESSIAN_VECTOR_PRODUCT, 0, hv, tr);
+            const int iMH = find(PYHIP_MA_MASS_MATRIX_HESSIAN_VECTOR_PRODUCT, 0, hv, tr);
+            if (iH >= 0) {
+                p->hvec = hvecs + (int64_t)hv * p->ndof;
+                p->htr = tr;
+                rc = assemble_subset(p, dofs, 1, -1, PYHIP_F64, p->ma_res, p->ma_vals, nullptr);
+                p->hvec = nullptr;
+                p->htr = 0;
+                if (rc || (rc = export_to(iH))) return rc;
+            }
+            // the mass matrices of these forms do not depend on the dofs
+            if (iMH >= 0 && (rc = zero_to(iMH))) return rc;
+        }
+        return PYHIP_OK;
+    };
+    const std::string saved = p->solved_residual;
+    for (const std::string &c : ma_contributions(nreq, req)) {
+        cur = c;
+        int ng = 0;
+        if ((rc = pyhip_problem_set_solved_residual(p, c.c_str(), &ng))) return rc;
+        if (!c.empty() && ng == 0) {
+            pyhip_problem_set_solved_residual(p, saved.c_str(), nullptr);
+            set_error("no equations define the residual contribution '" + c + "'");
+            return PYHIP_ERR_INVALID;
+        }
+        rc = run();
+        if (rc) {
+            pyhip_problem_set_solved_residual(p, saved.c_str(), nullptr);
+            return rc;
+        }
     }
+    if ((rc = pyhip_problem_set_solved_residual(p, saved.c_str(), nullptr))) return rc;
     PYHIP_CHECK_HIP(hipStreamSynchronize(p->st));
     return PYHIP_OK;
 }

@@ -44,6 +44,11 @@ struct SubInfo {
 
 struct Group {
     int kind, form, ns, nn, dim;
+    // the form whose kernels the group runs for the residual being solved (the reference's
+    // current_res_jac of the element code, src/problem.cpp:186-200): form itself for the
+    // default residual "", a generated form's named residual table, or -1 when the group
+    // has no such table (its elements then contribute nothing)
+    int active = 0;
     int64_t nel = 0;
     int64_t er_off = 0, ej_off = 0;   // offsets into the element vector / matrix buffers
     int32_t *conn = nullptr;          // [nn][nel]
@@ -139,6 +144,8 @@ inline bool th_layout(int form) {
 }
 // per-face coefficients of a generated boundary-face form (layout 2), -1 for other forms
 int gen_form_ncoef(int form);
+// id of the residual table `name` of a generated form (-1: none)
+int gen_form_table(int form, const std::string &name);
 inline bool gen_face_form(int form) { return form >= PYHIP_GENERATED_FORM_BASE && gen_form_layout(form) == 2; }
 // groups of boundary-face elements: the built-in flux / traction / EnforcedBC faces and the
 // generated boundary-face forms
@@ -172,6 +179,7 @@ struct pyhip_problem {
     int nsub = 0;
     std::vector<pyhip::SubInfo> subs;
     std::vector<pyhip::Group> groups;
+    std::string solved_residual;             // name of the residual the groups run ("": default)
     std::vector<int64_t> dof_off_h;          // [nsub+1]
     int64_t ndof = 0, nnode = 0, nnz = 0, nocc = 0, ncontrib = 0, n_er = 0, n_ej = 0;
     int64_t device_bytes = 0;

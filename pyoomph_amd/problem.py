@@ -325,12 +325,41 @@ class Problem:
                                                    col.ctypes.data_as(_lib.c_i32p)))
         return rs, col[:self.nnz]
 
-    def assemble_jacobian(self, dparameter=None):
+    def _set_solved_residual(self, name, raise_error=True):
+        """The residual the following assemblies and solves use (the reference's
+        Problem._set_solved_residual, src/problem.cpp:621-633): "" the default one, or a
+        named residual table of the generated forms; returns whether any equations define it."""
+        n = ctypes.c_int()
+        _lib.check(_lib.load().pyhip_problem_set_solved_residual(self._handle, str(name).encode(), ctypes.byref(n)))
+        if name and n.value == 0:
+            _lib.load().pyhip_problem_set_solved_residual(self._handle, self._solved_residual.encode(), None)
+            if raise_error:
+                raise RuntimeError("Cannot activate the residual-Jacobian pair named '%s', since it is defined "
+                                   "in no equations at all" % name)
+            return False
+        self._solved_residual = str(name)
+        return True
+
+    _solved_residual = ""
+
+    def _get_solved_residual(self):
+        return self._solved_residual
+
+    def assemble_jacobian(self, dparameter=None, which_one=""):
         """(residual, scipy.sparse.csr_matrix) in the reference's CSR form; with
-        ``dparameter`` (dR/dp, dJ/dp), dJ/dp's exact-zero element contributions dropped."""
+        ``dparameter`` (dR/dp, dJ/dp), dJ/dp's exact-zero element contributions dropped;
+        ``which_one``: a named residual table (pyoomph's assemble_jacobian(which_one=...),
+        problem.py:711-717), the solved residual being restored afterwards."""
         import scipy.sparse
-        res, vals = self.assemble_device("f64", dparameter)
-        rs, col, val = self._export(vals)
+        old = self._solved_residual
+        if which_one != old:
+            self._set_solved_residual(which_one)
+        try:
+            res, vals = self.assemble_device("f64", dparameter)
+            rs, col, val = self._export(vals)
+        finally:
+            if which_one != old:
+                self._set_solved_residual(old)
         R = self._host(res[:self.ndof])
         J = scipy.sparse.csr_matrix((val, col, rs), shape=(self.ndof, self.ndof))
         return R, J
@@ -505,18 +534,17 @@ class Problem:
         contributions = list(contributions)
         if len(what) != len(contributions):
             raise RuntimeError("Number of what and contributions must match")
-        for c in contributions:
-            if c != "":
-                raise RuntimeError("unknown residual contribution %r: a device problem has one residual" % c)
         params = list(params)
         hidx = list(hessian_vector_indices)
         reqs = (_lib.MultiRequest * max(len(what), 1))()
+        cnames = [str(c).encode() for c in contributions]
         pi = hi = 0
         for i, w in enumerate(what):
             if w not in _lib.MULTI_WHAT:
                 raise RuntimeError("Unknown what type '%s'" % w)
             kind, tr = _lib.MULTI_WHAT[w]
             reqs[i].what, reqs[i].transposed, reqs[i].param, reqs[i].hvec = kind, tr, -1, -1
+            reqs[i].contribution = cnames[i]
             if kind in (3, 4, 5):
                 if pi >= len(params):
                     raise RuntimeError("You have not provided enough parameters for the what type '%s'" % w)

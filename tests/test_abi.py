@@ -109,3 +109,24 @@ def test_multi_assembly_distinct_requests_pass():
     assert ret == [-1, -2, -3] and counts == (0, 3)
     ret, counts = _plan([(6, -1, 0, 1), (7, -1, 0, 1)], 1)
     assert ret == [-1, -2]
+
+
+def test_multi_assembly_numbering_per_contribution():
+    """Contributions (residual tables) are numbered one after the other in order of first
+    appearance, each in request order (src/bifurcation.cpp:5436-5480)."""
+    L = _lib.load()
+    reqs = [(0, ""), (1, "mass"), (1, ""), (0, "mass")]
+    arr = (_lib.MultiRequest * 4)()
+    keep = []
+    for i, (w, c) in enumerate(reqs):
+        arr[i].what, arr[i].param, arr[i].hvec, arr[i].transposed = w, -1, -1, 0
+        keep.append(c.encode())
+        arr[i].contribution = keep[-1]
+    ret = (ctypes.c_int * 4)()
+    nv, nm = ctypes.c_int(), ctypes.c_int()
+    assert L.pyhip_multi_assembly_plan(4, arr, 0, ret, ctypes.byref(nv), ctypes.byref(nm)) == 0
+    assert list(ret) == [0, -2, -1, 1] and (nv.value, nm.value) == (2, 2)
+    # the same quantity in two contributions is two quantities; twice in one is an error
+    arr[3].what, arr[3].contribution = 1, keep[0]
+    assert L.pyhip_multi_assembly_plan(4, arr, 0, ret, ctypes.byref(nv), ctypes.byref(nm)) == _lib.PYHIP_ERR_INVALID
+    assert b"for the same contribution ''" in L.pyhip_last_error()

@@ -158,3 +158,67 @@ def test_taylor_hood_has_no_parameter_derivative(gpu):
     P, _ = build("ns", dict(N=3))
     with pytest.raises((RuntimeError, ValueError)):
         P.start_multiassembly().dRdp("lambda").assemble()
+
+
+MASS = "_simple_mass_matrix_of_defined_fields"
+
+
+def _bratu_with_mass_table(N, space, lam):
+    """A generated Bratu form with a second residual table int u psi_i (the mass matrix of
+    the field, the table pyoomph's stability setup adds with destination=...,
+    pyoomph/generic/problem.py:1677-1684)."""
+    from pyoomph_amd.codegen import expr as E
+    from pyoomph_amd.codegen.runtime import GeneratedEquation
+    from pyoomph_amd.codegen.scalar import ScalarWeakForm
+    from pyoomph_amd.meshes import RectangularQuadMesh
+    from pyoomph_amd.problem import MeshProblem
+    form = ScalarWeakForm("gen_bratu_tables", lambda s: [s.grad_u[0], s.grad_u[1]],
+                          lambda s: -s.p[0] * E.exp(s.u), nparams=1, global_param=0,
+                          tables={MASS: (lambda s: [0, 0], lambda s: s.u)})
+    eq = GeneratedEquation(form, space=space, params=[lam])
+    return MeshProblem(RectangularQuadMesh(N, split_in_tris="left", space=space), eq,
+                       dirichlet=[(("left", "right", "top", "bottom"), {0: 0.0})])
+
+
+@pytest.mark.parametrize("space", ["C1", "C2"])
+def test_residual_tables(gpu, space):
+    """A named residual table: assemble_jacobian(which_one=...) gives the oracle's mass matrix
+    bit for bit and restores the solved residual; multi-assembly contributions; a batch with
+    a compiled form (no such table) contributes nothing to it; unknown names raise."""
+    from pyoomph_amd.problem import Problem
+    mp = _bratu_with_mass_table(6, space, 1.3)
+    op = oracle_problem("bratu", N=6, space=space, lam=1.3)
+    assert np.array_equal(mp.eqn, op.eqn)
+    P = Problem(mp)
+    x = np.random.default_rng(2).uniform(-0.3, 0.3, P.ndof)
+    P.set_current_dofs(x)
+    R0, J0 = P.assemble_jacobian()
+    Rm, Jm = P.assemble_jacobian(which_one=MASS)
+    op.set_time(ots.bdf1_weights(1.0), x, x)
+    _, rs, col, val = op.get_mass_matrix(x)
+    op.set_time(None, None, None)
+    assert np.array_equal(Jm.indptr, rs) and np.array_equal(Jm.indices, col) and np.array_equal(Jm.data, val)
+    assert np.allclose(Rm, Jm @ x, rtol=1e-12, atol=1e-15)     # int u psi_i = M u
+    assert P._get_solved_residual() == ""
+    R1, J1 = P.assemble_jacobian()
+    assert np.array_equal(R0, R1) and np.array_equal(J0.data, J1.data)
+    # multi-assembly over two contributions
+    Jd, Jt, Rt = P.start_multiassembly().J().J(MASS).R(MASS).assemble()
+    assert (Jd - J0).count_nonzero() == 0 and (Jt - Jm).count_nonzero() == 0 and np.array_equal(Rt, Rm)
+    with pytest.raises(RuntimeError, match="no equations at all"):
+        P._set_solved_residual("nothing")
+    assert P._set_solved_residual("nothing", raise_error=False) is False
+    assert P._get_solved_residual() == ""
+    # batched with a compiled Bratu problem: its rows have no table, so no entries
+    from pyoomph_amd import workloads as wl
+    B = Problem([mp, wl.bratu(5, "C1", 1.0)])
+    xb = np.random.default_rng(3).uniform(-0.3, 0.3, B.ndof)
+    B.set_current_dofs(xb)
+    Rb, Jb = B.assemble_jacobian(which_one=MASS)
+    o = B.dof_offsets
+    assert np.all(Rb[o[1]:] == 0.0) and Jb[o[1]:].nnz == 0
+    P.set_current_dofs(xb[:o[1]])
+    Ra, Ja = P.assemble_jacobian(which_one=MASS)
+    assert np.array_equal(Rb[:o[1]], Ra) and (Jb[:o[1], :o[1]] - Ja).count_nonzero() == 0
+    st = P.solve()
+    assert st[0]["converged"]
