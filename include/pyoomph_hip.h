@@ -208,11 +208,12 @@ int pyhip_set_time_stepping(pyhip_problem *p, int enabled, const double *weights
 int pyhip_assemble_noalloc(pyhip_problem *p, const double *dofs_dev, int flag, int deriv_param,
                            int precision, double *residual_dev, void *values_dev);
 
-/* Hessian-vector product of the scalar forms and of Taylor-Hood: values_dev [nnz]
- * (structural pattern) receives sum_k dJ_ij/du_k y_k at dofs_dev, residual_dev the
- * residual; the reference's HessianVectorProduct with one vector (src/codegen.cpp:
- * 3646-3910, assembled by src/elements.cpp:4964-4995, flag 1; for the scalar forms the
- * transposed product, flag 4, is the same matrix). Taylor-Hood: the convection block with
+/* Hessian-vector product of the scalar forms (compiled or generated) and of Taylor-Hood:
+ * values_dev [nnz] (structural pattern) receives sum_k dJ_ij/du_k y_k at dofs_dev,
+ * residual_dev the residual; the reference's HessianVectorProduct with one vector
+ * (src/codegen.cpp:3646-3910, assembled by src/elements.cpp:4964-4995, flag 1; for the
+ * compiled scalar forms the transposed product, flag 4, is the same matrix, for a generated
+ * form with a non-symmetric Jacobian it is not). Taylor-Hood: the convection block with
  * the velocity replaced by y (the only non-linear term). Several vectors: one call per
  * vector; the transposed product: pyhip_assemble_multi. pyhip_export_reference_csr then
  * gives its reference CSR. */
@@ -271,8 +272,9 @@ int pyhip_multi_assembly_plan(int nreq, const pyhip_multi_request *req, int nhve
  * :2232-2262). The mass matrix is dR/d(du/dt) (flag 2) whether or not time stepping is on;
  * dM/dp and the mass-matrix Hessian products have no non-zero element contribution for
  * these forms (their mass coefficients depend neither on the parameter nor on the dofs):
- * empty matrices. Hessian products of run-time generated forms, and dM/dp of generated
- * Taylor-Hood forms, are refused with PYHIP_ERR_INVALID. */
+ * empty matrices. Hessian products of run-time generated forms without Hessian kernels
+ * (Taylor-Hood and boundary-face forms), and dM/dp of generated Taylor-Hood forms, are
+ * refused with PYHIP_ERR_INVALID. */
 int pyhip_assemble_multi(pyhip_problem *p, const double *dofs_dev, int nreq,
                          const pyhip_multi_request *req, const double *hvecs_dev, int nhvec,
                          double *const *vectors, int32_t *const *row_start_dev,

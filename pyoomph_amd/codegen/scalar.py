@@ -177,14 +177,19 @@ KERNEL = r'''
 namespace pyhip_gen_{name} {{
 using namespace pyhip;
 constexpr int KIND = {kind};
-template <typename T, bool TR>
+// HV: 0 the residual / Jacobian / mass matrix kernels; 1 the Hessian-vector product
+// sum_j y_j d2R_i/du_j du_k (the Jacobian of the form's derivative in the direction y),
+// 2 the transposed product sum_j y_j d2R_j/du_i du_k (the Jacobian of the first variation of
+// int F . grad y + S y); the residual output stays the form's residual
+template <typename T, bool TR, int HV>
 __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
     using Tr = ElemTraits<KIND>;
     constexpr int NN = Tr::NN, DIM = Tr::DIM, NQ = Tr::NQ, NS = NN;
     constexpr int EPB = scalar_epb<KIND>();
     constexpr int QC = ScalarChunk<KIND>::QC;
     constexpr int NQV = {nqv};     // point quantities
-    constexpr int NQS = NQV + (TR ? 1 : 0);   // [+ du/dt]
+    constexpr int NQH = HV ? {nqh} : 0;   // [+ Hessian point quantities]
+    constexpr int NQS = NQV + NQH + (TR ? 1 : 0);   // [+ du/dt]
     __shared__ T sX[EPB][NN][DIM];
     __shared__ T sU[EPB][NN];
     __shared__ T sH[TR ? EPB : 1][NN];
@@ -193,6 +198,7 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
     __shared__ T sW[EPB][QC];
     __shared__ T sDx[EPB][QC][NN][DIM];
     __shared__ T sQ[EPB][QC][NQS];
+    __shared__ T sY[HV ? EPB : 1][NN];
     const int le = threadIdx.x / NS, row = threadIdx.x % NS;
     const int e = a.e0 + (int)blockIdx.x * EPB + le;
     const bool active = (le < EPB) && (e < a.e1);
@@ -208,6 +214,7 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
             const double u1 = eq >= 0 ? a.hist1[eq] : pin, u2 = eq >= 0 ? a.hist2[eq] : pin;
             sH[le][row] = T(a.tw1 * u1 + a.tw2 * u2);
         }}
+        if constexpr (HV > 0) sY[le][row] = T(eq >= 0 ? a.hvec[eq] : 0.0);
         sEq[le][row] = eq;
     }}
     __syncthreads();
@@ -249,11 +256,34 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
             }} else {{
 {phase_a}
             }}
+            if constexpr (HV > 0) {{
+                // the direction y and its gradient at the point
+                T yv = T(0);
+#pragma unroll
+                for (int l = 0; l < NN; l++) yv = yv + sY[le][l] * P[l];
+                T gyv[DIM];
+#pragma unroll
+                for (int i = 0; i < DIM; i++) {{
+                    T sacc = T(0);
+#pragma unroll
+                    for (int l = 0; l < NN; l++) sacc = sacc + sY[le][l] * sDx[le][qq][l][i];
+                    gyv[i] = sacc;
+                }}
+{gy_names}
+                T *QH = Q + NQV;
+                (void)QH;
+                (void)yv;
+                if constexpr (HV == 1) {{
+{phase_a_hv}
+                }} else {{
+{phase_a_hvt}
+                }}
+            }}
             if constexpr (TR) {{
                 T hq = T(0);
 #pragma unroll
                 for (int l = 0; l < NN; l++) hq = hq + sH[le][l] * P[l];
-                sQ[le][qq][NQV] = T(a.tw0) * u + hq;
+                sQ[le][qq][NQV + NQH] = T(a.tw0) * u + hq;
             }}
         }}
         __syncthreads();
@@ -272,6 +302,8 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
                 else
                     Racc = Racc + W * ({res});
                 if (a.flag == 2 && !TR) continue;
+                const T *QH = Q + NQV;
+                (void)QH;
                 if (a.flag) {{
                     const T tw0 = T(a.tw0);
 #pragma unroll
@@ -279,7 +311,11 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
                         if (sEq[le][j] < 0) continue;
                         const T *dxj = sDx[le][qq][j];
                         T v;
-                        if (a.flag == 2)
+                        if constexpr (HV == 1)
+                            v = W * ({jac_hv});
+                        else if constexpr (HV == 2)
+                            v = W * ({jac_hvt});
+                        else if (a.flag == 2)
                             v = W * (P[j] * P[row]);
                         else if (a.deriv)
                             v = W * ({jac_deriv});
@@ -303,16 +339,22 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
 }}
 }}  // namespace pyhip_gen_{name}
 extern "C" __global__ __launch_bounds__(256) void {name}_f64(pyhip::ElemArgs<double> a) {{
-    pyhip_gen_{name}::body<double, false>(a);
+    pyhip_gen_{name}::body<double, false, 0>(a);
 }}
 extern "C" __global__ __launch_bounds__(256) void {name}_f64_tr(pyhip::ElemArgs<double> a) {{
-    pyhip_gen_{name}::body<double, true>(a);
+    pyhip_gen_{name}::body<double, true, 0>(a);
 }}
 extern "C" __global__ __launch_bounds__(256) void {name}_f32(pyhip::ElemArgs<float> a) {{
-    pyhip_gen_{name}::body<float, false>(a);
+    pyhip_gen_{name}::body<float, false, 0>(a);
 }}
 extern "C" __global__ __launch_bounds__(256) void {name}_f32_tr(pyhip::ElemArgs<float> a) {{
-    pyhip_gen_{name}::body<float, true>(a);
+    pyhip_gen_{name}::body<float, true, 0>(a);
+}}
+extern "C" __global__ __launch_bounds__(256) void {name}_f64_hv(pyhip::ElemArgs<double> a) {{
+    pyhip_gen_{name}::body<double, false, 1>(a);
+}}
+extern "C" __global__ __launch_bounds__(256) void {name}_f64_hvt(pyhip::ElemArgs<double> a) {{
+    pyhip_gen_{name}::body<double, false, 2>(a);
 }}
 '''
 
@@ -340,6 +382,19 @@ def emit_kernels(form, kind):
     else:
         q2, res_d, jac_d = [], E.ZERO, E.ZERO
     nqv = max(nq_main, len(q2), 1)
+    # Hessian-vector products: the Jacobians of two derived forms in the direction y
+    yv = E.sym("yv")
+    gy = [E.sym("gy%d" % a_) for a_ in range(dim)]
+    dirv = lambda e: E.diff(e, s.u) * yv + _dot([E.diff(e, s.grad_u[b]) for b in range(dim)], gy)  # noqa: E731
+    Fy, Sy = [dirv(v) for v in F], dirv(S)
+    phi = _dot(F, gy) + S * yv
+    Ft, St = [E.diff(phi, s.grad_u[a_]) for a_ in range(dim)], E.diff(phi, s.u)
+    BH = _PhaseB(dim)
+    qh, sth = [], {}
+    _, jac_hv = _residual_jacobian_exprs(dim, _entries(form, dim, s, Fy, Sy), qh, sth, BH)
+    qht, stht = [], {}
+    _, jac_hvt = _residual_jacobian_exprs(dim, _entries(form, dim, s, Ft, St), qht, stht, BH)
+    nqh = max(len(qh), len(qht), 1)
 
     def phase_b_text(e):
         return E.CPrinter("T", prefix="b").expr_text(e)
@@ -364,13 +419,27 @@ def emit_kernels(form, kind):
                           12)
     grad_names = _indent(["const T g%d = gv[%d];" % (a_, a_) for a_ in range(dim)] +
                          ["(void)g%d;" % a_ for a_ in range(dim)], 12)
+    gy_names = _indent(["const T gy%d = gyv[%d];" % (a_, a_) for a_ in range(dim)] +
+                       ["(void)gy%d;" % a_ for a_ in range(dim)], 16)
+
+    def hv_lines(qs, prefix):
+        p = E.CPrinter("T", prefix=prefix)
+        return p.assign([("QH[%d]" % k, e) for k, e in enumerate(qs)])
+
+    def hv_text(e):
+        # phase B reads the Hessian quantities through QH = Q + NQV
+        return phase_b_text(e).replace("Q[", "QH[")
     src = KERNEL.format(
         name=form.name, kind=kid, nqv=nqv, aux="true" if form.aux else "false",
         grad_names=grad_names, x_block=x_block, f_block=f_block,
         phase_a=_indent(phase_a_lines(main_q, "a") or ["(void)Q;"], 16),
         phase_a_deriv=_indent(phase_a_lines(q2, "d") or ["(void)Q;"], 16),
         res=phase_b_text(res), jac=phase_b_text(jac),
-        res_deriv=phase_b_text(res_d), jac_deriv=phase_b_text(jac_d))
+        res_deriv=phase_b_text(res_d), jac_deriv=phase_b_text(jac_d),
+        nqh=nqh, gy_names=gy_names,
+        phase_a_hv=_indent(hv_lines(qh, "h") or ["(void)QH;"], 20),
+        phase_a_hvt=_indent(hv_lines(qht, "k") or ["(void)QH;"], 20),
+        jac_hv=hv_text(jac_hv), jac_hvt=hv_text(jac_hvt))
     info = {"kind": kid, "dim": dim, "nslot": KIND_NN[kid], "nquantities": nqv,
             "symmetric": form.symmetric(dim), "has_param_derivative": form.global_param is not None}
     return src, form.name, info

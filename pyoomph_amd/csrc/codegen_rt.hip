@@ -24,6 +24,9 @@ struct GenForm {
     std::vector<std::pair<std::string, int>> tables;   // named residual tables: (name, form id)
     hipModule_t mod = nullptr;
     hipFunction_t f64 = nullptr, f64_tr = nullptr, f32 = nullptr, f32_tr = nullptr;
+    // Hessian-vector products (FP64; scalar bulk forms): sum_j y_j d2R_i/du_j du_k and the
+    // transposed product, or null where the source does not define them
+    hipFunction_t f64_hv = nullptr, f64_hvt = nullptr;
 };
 
 std::mutex g_mu;
@@ -103,6 +106,12 @@ int gen_form_layout(int form) {
     return (k >= 0 && k < (int)g_forms.size()) ? g_forms[k].layout : 0;
 }
 
+bool gen_form_has_hv(int form) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    const int k = form - PYHIP_GENERATED_FORM_BASE;
+    return k >= 0 && k < (int)g_forms.size() && g_forms[k].f64_hv && g_forms[k].f64_hvt;
+}
+
 int gen_form_table(int form, const std::string &name) {
     std::lock_guard<std::mutex> lock(g_mu);
     const int k = form - PYHIP_GENERATED_FORM_BASE;
@@ -137,10 +146,11 @@ static int launch_gen(int form, const ElemArgs<T> &a, hipStream_t st) {
         const GenForm &G = g_forms[form - PYHIP_GENERATED_FORM_BASE];
         const bool f64 = std::is_same<T, double>::value;
         f = f64 ? (a.transient ? G.f64_tr : G.f64) : (a.transient ? G.f32_tr : G.f32);
+        if (a.hvec) f = f64 ? (a.htr ? G.f64_hvt : G.f64_hv) : nullptr;
         kind = G.kind;
         layout = G.layout;
     }
-    PYHIP_REQUIRE(a.hvec == nullptr, "Hessian-vector products are not generated for run-time forms");
+    PYHIP_REQUIRE(f != nullptr, "Hessian-vector products are generated for the scalar bulk forms only (FP64)");
     // elements per workgroup: one thread per slot (Taylor-Hood: 22) or per node / point;
     // boundary-face forms: one thread per face
     const int epb = layout == 2 ? 256 : (layout == 1 ? 256 / 22 : gen_epb(kind));
@@ -184,6 +194,13 @@ int pyhip_codegen_compile(const char *name, const char *source, int nheaders, co
     PYHIP_CHECK_HIP(hipModuleGetFunction(&G.f64_tr, G.mod, (n + "_f64_tr").c_str()));
     PYHIP_CHECK_HIP(hipModuleGetFunction(&G.f32, G.mod, (n + "_f32").c_str()));
     PYHIP_CHECK_HIP(hipModuleGetFunction(&G.f32_tr, G.mod, (n + "_f32_tr").c_str()));
+    // the Hessian-vector product kernels are optional: looked up only when the source defines
+    // them (a failed lookup would leave an error for the next HIP call to report)
+    const std::string src(source);
+    if (src.find(" " + n + "_f64_hv(") != std::string::npos)
+        PYHIP_CHECK_HIP(hipModuleGetFunction(&G.f64_hv, G.mod, (n + "_f64_hv").c_str()));
+    if (src.find(" " + n + "_f64_hvt(") != std::string::npos)
+        PYHIP_CHECK_HIP(hipModuleGetFunction(&G.f64_hvt, G.mod, (n + "_f64_hvt").c_str()));
     std::lock_guard<std::mutex> lock(g_mu);
     g_forms.push_back(G);
     *form_id = PYHIP_GENERATED_FORM_BASE + (int)g_forms.size() - 1;

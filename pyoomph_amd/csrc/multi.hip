@@ -132,16 +132,16 @@ int pyhip_assemble_multi(pyhip_problem *p, const double *dofs, int nreq, const p
     PYHIP_REQUIRE(nhvec == 0 || hvecs, "Hessian vectors required");
     for (int k = 0; k < nv; k++) PYHIP_REQUIRE(vectors[k], "null vector output");
     for (int m = 0; m < nm; m++) PYHIP_REQUIRE(row_start[m] && col[m] && val[m], "null matrix output");
-    bool generated = false, generated_th = false;
+    bool no_hv = false, generated_th = false;
     for (const auto &G : p->groups) {
         if (G.form >= PYHIP_GENERATED_FORM_BASE) {
-            generated = true;
+            no_hv = no_hv || !gen_form_has_hv(G.form);
             generated_th = generated_th || th_layout(G.form);
         }
     }
     for (int i = 0; i < nreq; i++) {
         if (ma_hessian_kind(req[i].what))
-            PYHIP_REQUIRE(!generated, "Hessian-vector products are not generated for run-time forms");
+            PYHIP_REQUIRE(!no_hv, "Hessian-vector products are generated for the scalar bulk forms only");
         // the mass coefficient of a generated Taylor-Hood form may depend on the parameter
         if (req[i].what == PYHIP_MA_DMASS_MATRIX_DPARAMETER)
             PYHIP_REQUIRE(!generated_th, "d(mass matrix)/d(parameter) is not generated for run-time Taylor-Hood forms");

@@ -144,6 +144,8 @@ inline bool th_layout(int form) {
 }
 // per-face coefficients of a generated boundary-face form (layout 2), -1 for other forms
 int gen_form_ncoef(int form);
+// the generated form has Hessian-vector product kernels
+bool gen_form_has_hv(int form);
 // id of the residual table `name` of a generated form (-1: none)
 int gen_form_table(int form, const std::string &name);
 inline bool gen_face_form(int form) { return form >= PYHIP_GENERATED_FORM_BASE && gen_form_layout(form) == 2; }

@@ -354,3 +354,51 @@ def test_generated_nonlinear_flux_matches_the_oracle(gpu, space, split):
     st = P.solve()
     assert st[0]["converged"]
     assert rel(P.get_current_dofs(), on.newton_solve(op, np.zeros(op.ndof), tol=1e-12)) <= 1e-8
+
+
+# ---- generated Hessian-vector products ------------------------------------------------------
+@pytest.mark.parametrize("kind,kw", CASES)
+def test_generated_hessian_products_match_the_compiled_ones(gpu, kind, kw):
+    """sum_j y_j d2R_i/du_j du_k from the generated kernels (the Jacobian of the form's
+    derivative in the direction y) against the compiled HV kernels: the same pattern, values
+    within 1e-13 (the products may associate differently), plain and transposed alike."""
+    from pyoomph_amd.problem import Problem
+    a, b = pair(kind, **kw)
+    A, B = Problem(a), Problem(b)
+    rng = np.random.default_rng(6)
+    x, y = rng.uniform(-0.3, 0.3, A.ndof), rng.uniform(-1, 1, A.ndof)
+    A.set_current_dofs(x)
+    B.set_current_dofs(x)
+    for tr in (False, True):
+        HA = A.assemble_hessian_vector_product(y, transposed=tr)
+        HB = B.assemble_hessian_vector_product(y, transposed=tr)
+        assert np.array_equal(HA.indptr, HB.indptr) and np.array_equal(HA.indices, HB.indices)
+        if HA.nnz:
+            assert rel(HB.data, HA.data) <= 1e-13
+
+
+def test_generated_hessian_products_of_a_non_symmetric_form(gpu):
+    """Nonlinear diffusion (1 + k u^2) grad u: H v = d/de J(x + e v) y and the transposed
+    product d/de J(x + e v)^T y, by central differences of the generated Jacobian; the two
+    products differ (the Jacobian is not symmetric)."""
+    from pyoomph_amd.problem import Problem
+    mesh = RectangularQuadMesh(6, split_in_tris="left", space="C2")
+    from pyoomph_amd.problem import MeshProblem
+    mp = MeshProblem(mesh, GeneratedEquation(library.nonlinear_diffusion(), space="C2", params=[2.0, 8.0]),
+                     dirichlet=[(ALL2D, {0: 0.0})])
+    P = Problem(mp)
+    rng = np.random.default_rng(8)
+    x, y, v = rng.uniform(-0.5, 0.5, P.ndof), rng.uniform(-1, 1, P.ndof), rng.uniform(-1, 1, P.ndof)
+    P.set_current_dofs(x)
+    H = P.assemble_hessian_vector_product(y)
+    Ht = P.assemble_hessian_vector_product(y, transposed=True)
+    eps = 1e-5
+    P.set_current_dofs(x + eps * v)
+    Jp = P.assemble_jacobian()[1]
+    P.set_current_dofs(x - eps * v)
+    Jm = P.assemble_jacobian()[1]
+    fd = (Jp @ y - Jm @ y) / (2 * eps)
+    fdt = (Jp.T @ y - Jm.T @ y) / (2 * eps)
+    assert np.max(np.abs(H @ v - fd)) <= 1e-7 * np.max(np.abs(fd))
+    assert np.max(np.abs(Ht @ v - fdt)) <= 1e-7 * np.max(np.abs(fdt))
+    assert np.max(np.abs((H - Ht).toarray())) > 1e-3 * np.max(np.abs(H.toarray()))
