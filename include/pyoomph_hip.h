@@ -389,8 +389,8 @@ int pyhip_problem_set_aux(pyhip_problem *p, int sub, const double *aux, int64_t 
 /* ---- arclength continuation ------------------------------------------------------------------
  * One pseudo-arclength step in global parameter `param` (0: the Bratu lambda) for every
  * sub-problem with active[s] != 0, each with its own step length ds[s] and state:
- * oomph's Problem::arc_length_step_solve (problem.cc:10718-11080, no spatial adaptation,
- * no bifurcation detection) with newton_solve_continuation (problem.cc:9599-9917), the
+ * oomph's Problem::arc_length_step_solve (problem.cc:10718-11080, no spatial adaptation)
+ * with newton_solve_continuation (problem.cc:9599-9917), the
  * block elimination of the bordered system (two solves with J per Newton step,
  * J y = R and J z = dR/dparam), calculate_continuation_derivatives (:9918-10020,
  * Scale_arc_length with Desired_proportion_of_arc_length 0.5) and the step-length rule
@@ -416,6 +416,20 @@ typedef struct {
                                     returns PYHIP_ERR_NOT_CONVERGED after finishing the
                                     other sub-problems (the reference throws,
                                     problem.cc:10905-10915)                              */
+    int turning_point;           /* out: 1 if Parameter_derivative changed sign in this
+                                    step (0 on a sub-problem's first step): a fold lies
+                                    between parameter_current and parameter, where det J
+                                    changes sign. oomph's detection reads the sign of det J
+                                    from SuperLU (problem.cc:10944-10975) and is off
+                                    unless Bifurcation_detection is set (pyoomph never
+                                    sets it, and its SuperLU plugin returns sign 0,
+                                    pyoomph/solvers/scipy.py:101); the batched Krylov
+                                    solves have no determinant, and the bordered system's
+                                    determinant, det J (Parameter_derivative -
+                                    Theta^2 Dof_derivative.z), keeps its sign along a
+                                    branch, so the sign of Parameter_derivative tracks
+                                    that of det J. Branch points (det J changing sign
+                                    without a fold) are not detected.                    */
 } pyhip_arclength_state;
 int pyhip_arclength_step(pyhip_problem *p, double *dofs_dev, int param, const pyhip_newton_opts *opts,
                          const int32_t *active, const double *ds, pyhip_arclength_state *state,

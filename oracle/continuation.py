@@ -9,6 +9,15 @@ calculate_continuation_derivatives (:9918-10020, :10115-10200; Scale_arc_length 
 Desired_proportion_of_arc_length 0.5) and the step-length rule (Desired_newton_iterations_ds
 5; Minimum_ds 1e-10; a failed Newton solve retries with ds * 2/3). Defaults of the state:
 Theta_squared 1, Parameter_derivative 1, Dof_derivative 0 (problem.cc:115-140).
+
+turning_point: the step's Parameter_derivative has the opposite sign of the previous step's
+(a fold of the branch between the two parameter values). The bordered matrix
+[[J, dR/dlambda], [Theta^2 Dof_derivative^T, Parameter_derivative]] has the determinant
+det J * (Parameter_derivative - Theta^2 Dof_derivative . z), J z = dR/dlambda, and keeps
+its sign along a branch without branch points, while the new Parameter_derivative takes the
+sign of the continuation direction Parameter_derivative - Dof_derivative . z (:10138-10183):
+its sign change marks the fold, where det J changes sign and oomph's Sign_of_jacobian
+check (problem.cc:10944-10975, off unless Bifurcation_detection is set) would report it.
 """
 import numpy as np
 
@@ -25,6 +34,7 @@ class ArclengthState:
         self.dof_current = np.zeros(ndof)
         self.dof_derivative = np.zeros(ndof)
         self.newton_iters = 0
+        self.turning_point = False
 
 
 def _get_lam(op):
@@ -97,6 +107,7 @@ def arc_length_step(op, x, st, ds, desired_newton_iterations=5, min_ds=1e-10, **
         except NewtonSolverError:
             st.ds_current *= 2.0 / 3.0
     st.newton_iters = count
+    pd_before, taken_before = st.parameter_derivative, st.step_taken
     _helper(st, z, float(np.dot(st.dof_derivative, z)))
     pd = st.parameter_derivative
     if pd != 1.0:
@@ -104,6 +115,7 @@ def arc_length_step(op, x, st, ds, desired_newton_iterations=5, min_ds=1e-10, **
         _helper(st, z, float(np.dot(st.dof_derivative, z)))
     if not st.step_taken:
         st.ds_current = (_get_lam(op) - st.parameter_current) / st.parameter_derivative
+    st.turning_point = bool(taken_before) and pd_before * st.parameter_derivative < 0.0
     st.step_taken = True
     if count < desired_newton_iterations:
         return x, st.ds_current * 1.5

@@ -89,7 +89,7 @@ class ArclengthState(ctypes.Structure):
                 ("parameter_current", ctypes.c_double), ("ds_current", ctypes.c_double),
                 ("step_taken", ctypes.c_int), ("parameter", ctypes.c_double),
                 ("newton_iters", ctypes.c_int), ("rejected", ctypes.c_int),
-                ("linear_failed", ctypes.c_int)]
+                ("linear_failed", ctypes.c_int), ("turning_point", ctypes.c_int)]
 
 
 class MultiRequest(ctypes.Structure):

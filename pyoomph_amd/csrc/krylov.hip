@@ -2434,8 +2434,10 @@ int pyhip_arclength_step(pyhip_problem *p, double *x, int param, const pyhip_new
         for (int s = 0; s < ns; s++) back[s] = lin_failed[s];
         if ((rc = set_lambda()) || (rc = upload(back, alpha)) || (rc = vec(4, W->ac_cur, nullptr, x))) return rc;
     }
+    for (int s = 0; s < ns; s++) st[s].turning_point = 0;
     for (int s = 0; s < ns; s++) {
         if (!act[s]) continue;
+        const double pd_before = st[s].parameter_derivative;
         helper(s, dz[s]);
         const double pd = st[s].parameter_derivative;
         if (pd != 1.0) {
@@ -2445,6 +2447,9 @@ int pyhip_arclength_step(pyhip_problem *p, double *x, int param, const pyhip_new
         alpha[s] = st[s].parameter_derivative;
         if (!st[s].step_taken)
             st[s].ds_current = (lam[s] - st[s].parameter_current) / st[s].parameter_derivative;
+        // a fold between parameter_current and the new parameter: the sign of
+        // Parameter_derivative follows that of det J along the branch
+        st[s].turning_point = st[s].step_taken && pd_before * st[s].parameter_derivative < 0.0;
         st[s].step_taken = 1;
         const int c = st[s].newton_iters;
         ds_next[s] = c < kDesiredIters ? st[s].ds_current * 1.5

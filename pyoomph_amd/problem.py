@@ -681,8 +681,12 @@ class Problem:
         return np.array([a.parameter for a in self._arc])
 
     def arclength_stats(self):
+        """Per sub-problem, the last step's Newton iterations, rejected tries, continuation
+        state, whether its linear solve failed and whether it crossed a fold (turning_point:
+        Parameter_derivative changed sign, det J with it)."""
         return [{"newton_iters": a.newton_iters, "rejected": a.rejected, "theta_squared": a.theta_squared,
-                 "parameter_derivative": a.parameter_derivative, "linear_failed": bool(a.linear_failed)}
+                 "parameter_derivative": a.parameter_derivative, "linear_failed": bool(a.linear_failed),
+                 "turning_point": bool(a.turning_point)}
                 for a in self._arc]
 
     def solve(self, raise_on_failure=True):

@@ -3,8 +3,8 @@ lambda, step length and continuation state) against the oracle's restatement of 
 arc_length_step_solve. Bar: every continuation point is a solution for the oracle
 (max|R(x, lambda)| < 1e-8 with the oracle's element code); while the Newton iteration
 counts agree (they set the next step length), lambda and the dofs of every step agree with
-the oracle's to 1e-8; a sub-problem gives the same bits alone and in a batch; the branch
-turns at the fold."""
+the oracle's to 1e-8, and so does the turning-point flag; a sub-problem gives the same
+bits alone and in a batch; the branch turns at the fold, flagged on the step across it."""
 import numpy as np
 import pytest
 
@@ -28,7 +28,8 @@ def _gpu_run(specs, steps, ds=0.5, max_ds=0.3, precond="ilu0"):
         dsv = P.arclength_continuation("lambda", dsv, max_ds=max_ds)
         x = P.get_current_dofs()
         out.append((P.global_parameters().copy(), [P.sub_dofs(i, x).copy() for i in range(len(specs))],
-                    [a["newton_iters"] for a in P.arclength_stats()]))
+                    [a["newton_iters"] for a in P.arclength_stats()],
+                    [a["turning_point"] for a in P.arclength_stats()]))
     return out
 
 
@@ -52,6 +53,7 @@ def test_continuation_matches_oracle(gpu):
             if st.newton_iters != its:
                 agree = False          # different step lengths from here on
                 continue
+            assert run[k][3][i] == st.turning_point, (i, k)
             assert abs(lam_g - op.params[0]) <= 1e-8 * max(1.0, abs(op.params[0])), (i, k)
             assert np.max(np.abs(xs - x)) <= 1e-8 * max(1.0, np.max(np.abs(x))), (i, k)
         assert agree or k > 5
@@ -75,3 +77,9 @@ def test_continuation_passes_the_fold(gpu):
     c = np.polyfit(norms[k - 1:k + 2], lams[k - 1:k + 2], 2)
     assert abs(c[2] - c[1] ** 2 / (4 * c[0]) - 6.808) < 0.02
     assert lams[-1] < lams[k] - 0.5
+    # one turning point, on the step over the fold (lambda grows before it, falls after)
+    flags = [r[3][0] for r in run]
+    assert sum(flags) == 1
+    j = flags.index(True)
+    d = np.diff(np.concatenate([[0.0], lams]))
+    assert np.all(d[:j] > 0) and np.all(d[j + 1:] < 0)

@@ -45,3 +45,39 @@ def test_lower_branch_points_are_steady_solutions():
         op = oracle_problem("bratu", N=6, space="C1", lam=lam)
         x = newton_solve(op, np.zeros(op.ndof))
         assert abs(np.max(np.abs(x)) - nm) < 1e-8 * max(1.0, nm)
+
+
+def test_turning_point_flag_marks_the_sign_change_of_det_j():
+    """The turning-point flag against the sign of det J (dense LU, numpy.linalg.slogdet) at
+    every continuation point: set on exactly the steps across which det J changes sign, which
+    for Bratu is the one step over the fold (lambda grows up to the step's start and falls
+    from its end on)."""
+    op = oracle_problem("bratu", N=6, space="C1", lam=0.0)
+    st = oc.ArclengthState(op.ndof)
+    x = np.zeros(op.ndof)
+    ds, flags, signs, lams = 0.5, [], [], [0.0]
+    _, rs, col, val = op.get_jacobian(x)
+    signs.append(np.linalg.slogdet(_dense(op.ndof, rs, col, val))[0])
+    for _ in range(40):
+        ds = min(abs(ds), 0.3) * (1 if ds > 0 else -1)
+        x, ds = oc.arc_length_step(op, x, st, ds)
+        flags.append(st.turning_point)
+        lams.append(float(op.params[0]))
+        _, rs, col, val = op.get_jacobian(x)
+        signs.append(np.linalg.slogdet(_dense(op.ndof, rs, col, val))[0])
+    signs = np.array(signs)
+    assert np.all(signs != 0)
+    changes = list(signs[1:] != signs[:-1])
+    assert changes == flags
+    assert sum(flags) == 1
+    k = flags.index(True)              # the step from lams[k] to lams[k + 1]
+    d = np.diff(lams)
+    assert np.all(d[:k] > 0) and np.all(d[k + 1:] < 0)
+
+
+def _dense(n, rs, col, val):
+    A = np.zeros((n, n))
+    for i in range(n):
+        for q in range(rs[i], rs[i + 1]):
+            A[i, col[q]] += val[q]
+    return A
