@@ -116,6 +116,19 @@ def assembly_bytes(P):
     return b + 8 * P.nnz
 
 
+# sources that decide the spmv family's launches and their bytes on the sweep
+LAUNCH_SOURCES = ("spmv.h", "krylov.hip", "amg.hip", "amg_plan.cpp", "amg_plan.h")
+
+
+def launch_sources_sha256():
+    """SHA-256 over LAUNCH_SOURCES (in pyoomph_amd/csrc, in that order)."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in LAUNCH_SOURCES:
+        h.update(open(os.path.join(ROOT, "pyoomph_amd", "csrc", f), "rb").read())
+    return h.hexdigest()
+
+
 def timing(P, fam):
     import ctypes
     from pyoomph_amd import _lib
@@ -652,20 +665,20 @@ def main():
                          % (dom, "" if args.precond == "amg" else "_" + args.precond))
     traffic_note = None
     if os.path.exists(tfile) and args.precision == "f64":
-        # the profile names the spmv kernel source it was measured with; a figure taken with
-        # other kernels is not reported
-        import hashlib
-        src = hashlib.sha256(open(os.path.join(ROOT, "pyoomph_amd", "csrc", "spmv.h"), "rb").read()).hexdigest()
+        # the profile names the sources it was measured with (the spmv kernels and the
+        # solver and multigrid code that decide the launches); a figure taken with other
+        # sources is not reported
+        src = launch_sources_sha256()
         rec, tsrc = None, None
         for line in open(tfile):
             if line.startswith("traffic_bytes_per_launch="):
                 rec = float(line.split("=", 1)[1])
-            if line.startswith("spmv_source_sha256="):
+            if line.startswith("launch_sources_sha256="):
                 tsrc = line.split("=", 1)[1].strip()
         if rec is not None and tsrc == src:
             traffic = rec
         elif rec is not None:
-            traffic_note = "the traffic profile was measured with other spmv kernels; not reported"
+            traffic_note = "the traffic profile was measured with other spmv or solver sources; not reported"
 
     # single-problem GPU paths on a sample (one problem per Problem, pattern and plan set up
     # outside the clock, the same stopping rules): the plain FP64 path (CG + Jacobi) and the

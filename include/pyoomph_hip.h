@@ -314,6 +314,11 @@ typedef struct {
     int max_refinements;/* FP64 iterative-refinement sweeps around an FP32 solve       */
     double refine_rtol; /* relative residual target of the refined solution            */
     int ilu_order;      /* PYHIP_ILU_ORDER_*: row order of the ILU(0) factor           */
+    int amg_element_level; /* 1 (default): in the multigrid, sub-problems on P2 / Q2
+                           elements of a scalar form coarsen first to the linear (P1 / Q1)
+                           functions on their vertices, then by aggregation; 0: by
+                           aggregation from the first level. Changing it rebuilds the
+                           hierarchy.                                                   */
 } pyhip_linear_opts;
 void pyhip_linear_opts_default(pyhip_linear_opts *o);
 

@@ -60,7 +60,7 @@ class LinearOpts(ctypes.Structure):
                 ("restart", ctypes.c_int), ("max_iters", ctypes.c_int),
                 ("rtol", ctypes.c_double), ("rtol_f32", ctypes.c_double),
                 ("max_refinements", ctypes.c_int),
-                ("refine_rtol", ctypes.c_double), ("ilu_order", ctypes.c_int)]
+                ("refine_rtol", ctypes.c_double), ("ilu_order", ctypes.c_int), ("amg_element_level", ctypes.c_int)]
 
 
 class NewtonOpts(ctypes.Structure):

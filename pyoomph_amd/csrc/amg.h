@@ -26,6 +26,7 @@ struct AmgDevLevel {
     const double *vals = nullptr;      // A_l values (level 0: the problem's)
     double *own_vals = nullptr;        // A_l values computed here (levels > 0)
     double *d = nullptr, *w = nullptr, *pv = nullptr, *qv = nullptr, *ptv = nullptr;
+    double *pfix = nullptr;            // per P entry: its value before the smoothing
     double *b = nullptr, *x = nullptr, *x2 = nullptr, *r = nullptr, *pw = nullptr, *py = nullptr;
     // host: per sub-problem rows, A entries and P entries (for counting the bytes of the
     // rows a filtered V-cycle visits)
@@ -41,6 +42,7 @@ struct AmgDevLevel {
 
 struct AmgDev {
     bool built = false;
+    bool element_level = false;        // built with the element level (an AmgFixedInterp)
     // non-symmetric operators (a convection-diffusion velocity block): the coarsest solve
     // uses a Gauss-Jordan inverse instead of the Cholesky factor (set before amg_build)
     bool nonsym = false;
@@ -54,9 +56,10 @@ struct AmgDev {
     float *chol32 = nullptr;           // FP32 copy of the coarsest inverses
 };
 
-// plan (host) and device buffers, once per pattern; sizes counted in `bytes`
+// plan (host) and device buffers, once per pattern; sizes counted in `bytes`; with fix,
+// its sub-problems coarsen first to the element level
 int amg_build(AmgDev &A, int64_t n, const int32_t *rs_h, const int32_t *col_h,
-              const std::vector<int64_t> &sub_off, int64_t &bytes);
+              const std::vector<int64_t> &sub_off, int64_t &bytes, const AmgFixedInterp *fix = nullptr);
 struct KernelTimer;
 // numeric set-up for the matrix values `vals` (level-0 pattern); with a timer, the
 // power-iteration spmv launches are timed and counted as "spmv" launches

@@ -108,10 +108,9 @@ __global__ __launch_bounds__(256) void amg_weights_kernel(int64_t n, const int32
 // P[q] = [pcol(q) == agg(i)] - w_i * (sum of the A entries listed for q), i = prow(q): one
 // thread per entry of P, which also writes the value to its place in P^T (tinv)
 __global__ __launch_bounds__(256) void amg_pvals_kernel(int64_t nnzp, const int32_t *__restrict__ prow,
-                                                        const int32_t *__restrict__ pcol,
                                                         const int32_t *__restrict__ pcptr,
                                                         const int32_t *__restrict__ pcidx,
-                                                        const int32_t *__restrict__ agg,
+                                                        const double *__restrict__ pfix,
                                                         const double *__restrict__ vals,
                                                         const double *__restrict__ w,
                                                         const int32_t *__restrict__ tinv, double *__restrict__ pv,
@@ -121,7 +120,9 @@ __global__ __launch_bounds__(256) void amg_pvals_kernel(int64_t nnzp, const int3
     const int32_t i = prow[q];
     double s = 0.0;
     for (int32_t c = pcptr[q]; c < pcptr[q + 1]; c++) s += vals[pcidx[c]];
-    const double v = (pcol[q] == agg[i] ? 1.0 : 0.0) - w[i] * s;
+    // P = P_t - W A P_t on aggregates (pfix 1 or 0); the element level's entries have no A
+    // entries (s = 0): their fixed weight
+    const double v = pfix[q] - w[i] * s;
     pv[q] = v;
     ptv[tinv[q]] = v;
 }
@@ -409,7 +410,7 @@ static std::vector<int32_t> row_sub_of(const std::vector<int64_t> &off) {
 
 void amg_free(AmgDev &A) {
     auto fl = [](AmgDevLevel &L) {
-        void *ptrs[] = {L.rs, L.col, L.diag, L.agg, L.prs, L.pcol, L.pcptr, L.pcidx, L.trs, L.trow, L.tsrc,
+        void *ptrs[] = {L.rs, L.col, L.diag, L.agg, L.pfix, L.prs, L.pcol, L.pcptr, L.pcidx, L.trs, L.trow, L.tsrc,
                         L.qrs, L.qcol, L.row_sub, L.sub_off, L.own_vals, L.d, L.w, L.pv, L.qv, L.b, L.x,
                         L.x2, L.r, L.pw, L.py, L.cont, L.ends, L.ptv, L.prow, L.tinv, L.own_vals32, L.w32,
                         L.pv32, L.ptv32, L.b32, L.x32, L.x2_32, L.r32};
@@ -451,11 +452,12 @@ static int upload_level(AmgDevLevel &D, const std::vector<int32_t> &rs, const st
 }
 
 int amg_build(AmgDev &A, int64_t n, const int32_t *rs_h, const int32_t *col_h, const std::vector<int64_t> &sub_off,
-              int64_t &B) {
+              int64_t &B, const AmgFixedInterp *fix) {
     if (A.built) return PYHIP_OK;
     AmgPlan plan;
     std::string err;
-    if (amg_plan_build(n, rs_h, col_h, sub_off, kAggregateCap, kBottomRows, kMaxLevels, plan, err)) {
+    A.element_level = fix != nullptr;
+    if (amg_plan_build(n, rs_h, col_h, sub_off, kAggregateCap, kBottomRows, kMaxLevels, plan, err, fix)) {
         set_error(err);
         return PYHIP_ERR_INVALID;
     }
@@ -483,7 +485,7 @@ int amg_build(AmgDev &A, int64_t n, const int32_t *rs_h, const int32_t *col_h, c
             for (int32_t q = P.prs[i]; q < P.prs[i + 1]; q++) prow[q] = (int32_t)i;
         for (int64_t t = 0; t < D.nnzp; t++) tinv[P.tsrc[t]] = (int32_t)t;
         if ((rc = aup(&D.prow, prow, B)) || (rc = aup(&D.tinv, tinv, B))) return rc;
-        if ((rc = aup(&D.agg, P.agg, B)) || (rc = aup(&D.prs, P.prs, B)) || (rc = aup(&D.pcol, P.pcol, B)) ||
+        if ((rc = aup(&D.pfix, P.pfix, B)) || (rc = aup(&D.prs, P.prs, B)) || (rc = aup(&D.pcol, P.pcol, B)) ||
             (rc = aup(&D.pcptr, P.pcptr, B)) || (rc = aup(&D.pcidx, P.pcidx, B)) || (rc = aup(&D.trs, P.trs, B)) ||
             (rc = aup(&D.trow, P.trow, B)) || (rc = aup(&D.tsrc, P.tsrc, B)) || (rc = aup(&D.qrs, P.qrs, B)) ||
             (rc = aup(&D.qcol, P.qcol, B)) || (rc = dev_alloc(&D.pv, D.nnzp, B)) ||
@@ -641,8 +643,8 @@ int amg_setup(AmgDev &A, hipStream_t st, const double *vals0, KernelTimer *tm, c
             hipLaunchKernelGGL(A.nonsym ? amg_gj_inverse_kernel : amg_chol_factor_kernel, dim3(L.nends),
                                dim3(kCholThreads), 0, st, L.ends, L.sub_off, L.rs, L.col, vals, A.chol, A.chol_fail);
         if (L.nnzp > 0)
-            hipLaunchKernelGGL(amg_pvals_kernel, dim3(ablk(L.nnzp)), dim3(256), 0, st, L.nnzp, L.prow, L.pcol,
-                               L.pcptr, L.pcidx, L.agg, vals, L.w, L.tinv, L.pv, L.ptv);
+            hipLaunchKernelGGL(amg_pvals_kernel, dim3(ablk(L.nnzp)), dim3(256), 0, st, L.nnzp, L.prow,
+                               L.pcptr, L.pcidx, L.pfix, vals, L.w, L.tinv, L.pv, L.ptv);
         // Q = A_l P (terms: A's row entries; source rows: P's rows)
         hipLaunchKernelGGL((amg_rowprod_kernel<kQLanes, kQCap>), dim3(ablk(L.n, 256 / kQLanes)), dim3(256), 0, st,
                            L.n, L.qrs, L.qcol, L.qv, L.rs, L.col, vals, L.prs, L.pcol, L.pv);

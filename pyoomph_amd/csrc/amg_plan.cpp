@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <thread>
 
+#include "../../include/pyoomph_hip.h"
+
 namespace pyhip {
 
 namespace {
@@ -95,7 +97,8 @@ bool diag_positions(int64_t n, const std::vector<int32_t> &rs, const std::vector
 }  // namespace
 
 int amg_plan_build(int64_t n0, const int32_t *rs0, const int32_t *col0, const std::vector<int64_t> &sub_off0,
-                   int cap, int bottom_rows, int max_levels, AmgPlan &plan, std::string &err) {
+                   int cap, int bottom_rows, int max_levels, AmgPlan &plan, std::string &err,
+                   const AmgFixedInterp *fix) {
     plan = AmgPlan();
     const int nsub = (int)sub_off0.size() - 1;
     std::vector<int32_t> rs(rs0, rs0 + n0 + 1), col(col0, col0 + rs0[n0]);
@@ -121,6 +124,31 @@ int amg_plan_build(int64_t n0, const int32_t *rs0, const int32_t *col0, const st
         for (int s = 0; s < nsub; s++)
             if (!cont[s] && sub_off[s + 1] > sub_off[s]) L.ends.push_back(s);
         if (!diag_positions(n, rs, col, L.diag, err)) return -1;
+        // sub-problems whose first coarsening is the element level (level 0 only; a fixed
+        // interpolation that does not coarsen falls back to aggregation)
+        std::vector<char> fixed_sub(nsub, 0);
+        std::vector<int32_t> rsub;
+        if (plan.lv.empty() && fix) {
+            if ((int64_t)fix->rs.size() != n + 1 || (int)fix->on.size() != nsub || (int)fix->nc.size() != nsub) {
+                err = "multigrid: element-level interpolation of another pattern";
+                return -1;
+            }
+            rsub.assign(n, 0);
+            for (int s = 0; s < nsub; s++) {
+                const int64_t rows = sub_off[s + 1] - sub_off[s];
+                fixed_sub[s] = cont[s] && fix->on[s] && fix->nc[s] > 0 && fix->nc[s] < rows;
+                for (int64_t r = sub_off[s]; r < sub_off[s + 1]; r++) {
+                    rsub[r] = s;
+                    if (!fixed_sub[s]) continue;
+                    for (int32_t t = fix->rs[r]; t < fix->rs[r + 1]; t++)
+                        if (fix->col[t] < 0 || fix->col[t] >= fix->nc[s] || (t > fix->rs[r] && fix->col[t] <= fix->col[t - 1])) {
+                            err = "multigrid: element-level interpolation columns out of range or unsorted";
+                            return -1;
+                        }
+                }
+            }
+        }
+        auto fixed_row = [&](int64_t r) { return !rsub.empty() && fixed_sub[rsub[r]]; };
         // aggregation of the continuing sub-problems (independent), numbered in row order
         L.agg.assign(n, -1);
         std::vector<int64_t> nagg(nsub + 1, 0);
@@ -128,6 +156,13 @@ int amg_plan_build(int64_t n0, const int32_t *rs0, const int32_t *col0, const st
         parallel_for(nsub, [&](int64_t b, int64_t e) {
             for (int64_t s = b; s < e; s++) {
                 if (!cont[s]) continue;
+                if (fixed_sub[s]) {
+                    // the row's first coarse column marks it as continuing (P comes from fix)
+                    for (int64_t r = sub_off[s]; r < sub_off[s + 1]; r++)
+                        L.agg[r] = fix->rs[r] < fix->rs[r + 1] ? fix->col[fix->rs[r]] : 0;
+                    nagg[s + 1] = fix->nc[s];
+                    continue;
+                }
                 int32_t na = 0;
                 for (int64_t r = sub_off[s]; r < sub_off[s + 1]; r++) {
                     if (L.agg[r] >= 0) continue;
@@ -161,12 +196,30 @@ int amg_plan_build(int64_t n0, const int32_t *rs0, const int32_t *col0, const st
         });
         const int64_t nc = coff[nsub];
         L.nc = nc;
-        // P pattern: aggregates of the row's neighbours
+        // P pattern: aggregates of the row's neighbours (element level: the fixed columns)
         if (!build_rows(n, nc, [&](int64_t r, std::vector<int32_t> &out) {
                 if (L.agg[r] < 0) return;   // the sub-problem ends on this level
+                if (fixed_row(r)) {
+                    for (int32_t t = fix->rs[r]; t < fix->rs[r + 1]; t++)
+                        out.push_back((int32_t)(coff[rsub[r]] + fix->col[t]));
+                    return;
+                }
                 for (int32_t k = rs[r]; k < rs[r + 1]; k++) out.push_back(L.agg[col[k]]);
             }, L.prs, L.pcol, err))
             return -1;
+        // values before the smoothing: 1 on the row's own aggregate; the element level's
+        // weights (its columns ascending and distinct, so in the same order as the pattern)
+        L.pfix.assign(L.prs[n], 0.0);
+        parallel_for(n, [&](int64_t b, int64_t e) {
+            for (int64_t r = b; r < e; r++) {
+                if (L.agg[r] < 0) continue;
+                if (fixed_row(r)) {
+                    for (int32_t q = L.prs[r], t = fix->rs[r]; q < L.prs[r + 1]; q++, t++) L.pfix[q] = fix->w[t];
+                } else {
+                    for (int32_t q = L.prs[r]; q < L.prs[r + 1]; q++) L.pfix[q] = L.pcol[q] == L.agg[r] ? 1.0 : 0.0;
+                }
+            }
+        });
         // P contributions: for entry (r, J) the A entries k of row r with agg(col k) == J
         const int64_t nnzp = L.prs[n];
         L.pcptr.assign(nnzp + 1, 0);
@@ -176,10 +229,11 @@ int amg_plan_build(int64_t n0, const int32_t *rs0, const int32_t *col0, const st
                 const int32_t *b0 = L.pcol.data() + L.prs[r], *b1 = L.pcol.data() + L.prs[r + 1];
                 return (int32_t)(std::lower_bound(b0, b1, L.agg[col[k]]) - L.pcol.data());
             };
+            // (the element level's entries are not smoothed: no A entries)
             std::vector<int32_t> cnt(nnzp, 0);
             parallel_for(n, [&](int64_t b, int64_t e) {
                 for (int64_t r = b; r < e; r++)
-                    if (L.agg[r] >= 0)
+                    if (L.agg[r] >= 0 && !fixed_row(r))
                         for (int32_t k = rs[r]; k < rs[r + 1]; k++) cnt[pentry(r, k)]++;
             });
             for (int64_t q = 0; q < nnzp; q++) L.pcptr[q + 1] = L.pcptr[q] + cnt[q];
@@ -189,7 +243,7 @@ int amg_plan_build(int64_t n0, const int32_t *rs0, const int32_t *col0, const st
             std::vector<int32_t> fill(L.pcptr.begin(), L.pcptr.end() - 1);
             parallel_for(n, [&](int64_t b, int64_t e) {
                 for (int64_t r = b; r < e; r++)
-                    if (L.agg[r] >= 0)
+                    if (L.agg[r] >= 0 && !fixed_row(r))
                         for (int32_t k = rs[r]; k < rs[r + 1]; k++) L.pcidx[fill[pentry(r, k)]++] = k;
             });
         }
@@ -245,6 +299,85 @@ int amg_plan_build(int64_t n0, const int32_t *rs0, const int32_t *col0, const st
         for (int s : L.ends) mb = std::max(mb, L.sub_off[s + 1] - L.sub_off[s]);
     plan.max_bot_rows = (int)mb;
     return 0;
+}
+
+void amg_element_level_rows(int kind, int64_t nnode, int64_t nelem, const int32_t *conn, const int32_t *eqn,
+                            int64_t ndof, bool applies, AmgFixedInterp &F) {
+    const bool quad = kind == PYHIP_ELEM_TRI_P2 || kind == PYHIP_ELEM_QUAD_Q2 || kind == PYHIP_ELEM_HEX_Q2;
+    const int64_t base = (int64_t)F.col.size();
+    if (!quad || !applies || ndof == 0) {
+        F.on.push_back(0);
+        F.nc.push_back(0);
+        for (int64_t q = 0; q < ndof; q++) F.rs.push_back((int32_t)base);
+        return;
+    }
+    const int nn = kind == PYHIP_ELEM_TRI_P2 ? 6 : (kind == PYHIP_ELEM_QUAD_Q2 ? 9 : 27);
+    // vertex local nodes and the weight of vertex v at local node l
+    int nv = 0, vloc[8];
+    double wt[27][8] = {};
+    if (kind == PYHIP_ELEM_TRI_P2) {
+        nv = 3;
+        for (int v = 0; v < 3; v++) vloc[v] = v, wt[v][v] = 1.0;
+        const int mid[3][2] = {{0, 1}, {1, 2}, {0, 2}};   // nodes 3, 4, 5
+        for (int m = 0; m < 3; m++) wt[3 + m][mid[m][0]] = wt[3 + m][mid[m][1]] = 0.5;
+    } else {
+        // tensor nodes l = i + 3 j (+ 9 k), i, j, k in 0..2; vertices at 0 and 2
+        const int dim = kind == PYHIP_ELEM_HEX_Q2 ? 3 : 2;
+        nv = 1 << dim;
+        for (int v = 0; v < nv; v++) {
+            int l = 0, st = 1;
+            for (int a = 0; a < dim; a++, st *= 3) l += ((v >> a) & 1) * 2 * st;
+            vloc[v] = l;
+        }
+        for (int l = 0; l < nn; l++)
+            for (int v = 0; v < nv; v++) {
+                double w = 1.0;
+                for (int a = 0, t = l; a < dim; a++, t /= 3) {
+                    const int i = t % 3, hi = (v >> a) & 1;
+                    w *= i == 1 ? 0.5 : ((i == 2) == (hi == 1) ? 1.0 : 0.0);
+                }
+                wt[l][v] = w;
+            }
+    }
+    std::vector<char> isv(nnode, 0);
+    for (int64_t e = 0; e < nelem; e++)
+        for (int v = 0; v < nv; v++) isv[conn[e * nn + vloc[v]]] = 1;
+    std::vector<int32_t> cdof(nnode, -1);
+    int64_t nc = 0;
+    for (int64_t n = 0; n < nnode; n++)
+        if (isv[n] && eqn[n] >= 0) cdof[n] = (int32_t)nc++;
+    // each unknown's entries from the first element holding it (at most 8, ascending)
+    std::vector<int8_t> cnt(ndof, -1);
+    std::vector<int32_t> ecol((size_t)ndof * nv);
+    std::vector<double> ew((size_t)ndof * nv);
+    for (int64_t e = 0; e < nelem; e++)
+        for (int l = 0; l < nn; l++) {
+            const int32_t n = conn[e * nn + l], q = eqn[n];
+            if (q < 0 || cnt[q] >= 0) continue;
+            int c = 0;
+            for (int v = 0; v < nv; v++) {
+                const int32_t cv = cdof[conn[e * nn + vloc[v]]];
+                if (wt[l][v] == 0.0 || cv < 0) continue;
+                int k = c++;
+                while (k > 0 && ecol[(size_t)q * nv + k - 1] > cv) {
+                    ecol[(size_t)q * nv + k] = ecol[(size_t)q * nv + k - 1];
+                    ew[(size_t)q * nv + k] = ew[(size_t)q * nv + k - 1];
+                    k--;
+                }
+                ecol[(size_t)q * nv + k] = cv;
+                ew[(size_t)q * nv + k] = wt[l][v];
+            }
+            cnt[q] = (int8_t)c;
+        }
+    F.on.push_back(nc > 0 && nc < ndof ? 1 : 0);
+    F.nc.push_back(nc);
+    for (int64_t q = 0; q < ndof; q++) {
+        F.rs.push_back((int32_t)F.col.size());
+        for (int k = 0; k < std::max<int>(0, cnt[q]); k++) {
+            F.col.push_back(ecol[(size_t)q * nv + k]);
+            F.w.push_back(ew[(size_t)q * nv + k]);
+        }
+    }
 }
 
 }  // namespace pyhip

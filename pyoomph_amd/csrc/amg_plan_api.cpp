@@ -30,6 +30,39 @@ void *pyhip_amg_plan_new(int64_t n, const int32_t *rs, const int32_t *col, int n
     return P;
 }
 
+// the same with the element level: on[nsub], nc[nsub], rows frs[n+1] / fcol / fw over the
+// level-0 rows (AmgFixedInterp)
+void *pyhip_amg_plan_new_fixed(int64_t n, const int32_t *rs, const int32_t *col, int nsub, const int64_t *sub_off,
+                               int cap, int bottom_rows, int max_levels, const int32_t *on, const int64_t *nc,
+                               const int32_t *frs, const int32_t *fcol, const double *fw) {
+    if (n < 0 || !rs || !col || nsub < 1 || !sub_off || !on || !nc || !frs || (frs[n] > 0 && (!fcol || !fw))) {
+        g_plan_err = "invalid arguments";
+        return nullptr;
+    }
+    AmgFixedInterp F;
+    F.on.assign(on, on + nsub);
+    F.nc.assign(nc, nc + nsub);
+    F.rs.assign(frs, frs + n + 1);
+    F.col.assign(fcol, fcol + frs[n]);
+    F.w.assign(fw, fw + frs[n]);
+    AmgPlan *P = new AmgPlan();
+    std::vector<int64_t> off(sub_off, sub_off + nsub + 1);
+    if (amg_plan_build(n, rs, col, off, cap, bottom_rows, max_levels, *P, g_plan_err, &F)) {
+        delete P;
+        return nullptr;
+    }
+    return P;
+}
+
+// values of level l's P entries before the smoothing [nnz(P)]
+int pyhip_amg_plan_level_pfix(const void *h, int l, double *pfix) {
+    const AmgPlan *P = (const AmgPlan *)h;
+    if (!P || l < 0 || l >= (int)P->lv.size() || !pfix) return -1;
+    const AmgLevelPlan &L = P->lv[l];
+    if (!L.pfix.empty()) std::memcpy(pfix, L.pfix.data(), L.pfix.size() * sizeof(double));
+    return 0;
+}
+
 int pyhip_amg_plan_levels(const void *h) { return h ? (int)((const AmgPlan *)h)->lv.size() : -1; }
 
 // sizes[0..5] = n, nc, nnz(A_l), nnz(P), nnz(Q), contributions of P

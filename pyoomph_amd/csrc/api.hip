@@ -102,6 +102,13 @@ static int face_kind(int elem_kind, int form) {
     }
 }
 
+// The multigrid's element level of one sub-problem (amg_element_level_rows): scalar forms
+// without EnforcedBC multipliers
+static void element_level_rows(const pyhip_subproblem_desc &d, AmgFixedInterp &F) {
+    const bool scalar = d.nvalue == 1 && !th_layout(d.form) && d.nenforced == 0;
+    amg_element_level_rows(d.elem_kind, d.nnode, d.nelem, d.conn, d.eqn, d.ndof, scalar, F);
+}
+
 // face element kind of the EnforcedBC faces of a bulk element kind (-1: not built)
 static int enforced_face_kind(int elem_kind) {
     switch (elem_kind) {
@@ -327,6 +334,10 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
             }
         }
     }
+    // the multigrid's element level, rows over all unknowns
+    P->amg_fix = AmgFixedInterp();
+    for (int s = 0; s < nsub; s++) element_level_rows(subs[s], P->amg_fix);
+    P->amg_fix.rs.push_back((int32_t)P->amg_fix.col.size());
     // pattern: element blocks in element order of each sub-problem
     std::vector<ElementBlock> blocks;
     for (int s = 0; s < nsub; s++) {
@@ -824,3 +835,27 @@ int pyhip_timing_bytes(pyhip_problem *p, const char *family, int64_t *bytes) {
 }
 
 }  // extern "C"
+
+// Host only (tests on the CPU): the multigrid element-level rows of one sub-problem
+// descriptor: nc coarse unknowns, rs [ndof+1], and (with col / w non-null) the entries,
+// nnz_out of them. No device work.
+extern "C" int pyhip_element_level_rows(const pyhip_subproblem_desc *d, int64_t *nc, int32_t *rs, int32_t *col,
+                                        double *w, int64_t *nnz_out) {
+    PYHIP_REQUIRE(d && nc && nnz_out, "null argument");
+    PYHIP_REQUIRE(elem_nnode(d->elem_kind) > 0 && d->nnode > 0 && d->nelem > 0 && d->conn && d->eqn && d->ndof >= 0,
+                  "invalid descriptor");
+    const int nn = elem_nnode(d->elem_kind);
+    for (int64_t k = 0; k < d->nelem * nn; k++)
+        PYHIP_REQUIRE(d->conn[k] >= 0 && d->conn[k] < d->nnode, "connectivity out of range");
+    for (int64_t k = 0; k < d->nnode * d->nvalue; k++)
+        PYHIP_REQUIRE(d->eqn[k] >= -1 && d->eqn[k] < d->ndof, "equation number out of range");
+    AmgFixedInterp F;
+    element_level_rows(*d, F);
+    F.rs.push_back((int32_t)F.col.size());
+    *nc = F.on[0] ? F.nc[0] : 0;
+    *nnz_out = (int64_t)F.col.size();
+    if (rs) std::memcpy(rs, F.rs.data(), F.rs.size() * sizeof(int32_t));
+    if (col && !F.col.empty()) std::memcpy(col, F.col.data(), F.col.size() * sizeof(int32_t));
+    if (w && !F.w.empty()) std::memcpy(w, F.w.data(), F.w.size() * sizeof(double));
+    return PYHIP_OK;
+}

@@ -983,7 +983,8 @@ static int need_on_host(pyhip_problem *p, KrylovWork *W, const int32_t *need) {
 }
 
 template <typename TV, typename T>
-static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals, const int32_t *need = nullptr) {
+static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals, const int32_t *need = nullptr,
+                         bool element_level = true) {
     TimeScope ts(&p->timer, "precond", p->st);
     if (pc == PYHIP_PRECOND_JACOBI || pc == PYHIP_PRECOND_AMG)
         for (const auto &S : p->subs)
@@ -996,8 +997,15 @@ static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals
         return PYHIP_ERR_INVALID;
     }
     if (pc == PYHIP_PRECOND_AMG) {
+        // the element level where the problem has one (problems built from elements)
+        const AmgFixedInterp *fix =
+            element_level && p->amg_fix.rs.size() == (size_t)p->ndof + 1 ? &p->amg_fix : nullptr;
+        if (W->amg.built && W->amg.element_level != (fix != nullptr)) {
+            amg_free(W->amg);
+            W->amg_stale = true;
+        }
         if constexpr (std::is_same<T, double>::value && std::is_same<TV, double>::value) {
-            int rc = amg_build(W->amg, p->ndof, p->row_start_h.data(), p->col_h.data(), p->dof_off_h, W->bytes);
+            int rc = amg_build(W->amg, p->ndof, p->row_start_h.data(), p->col_h.data(), p->dof_off_h, W->bytes, fix);
             if (rc) return rc;
             // the hierarchy (weights, prolongators, coarse operators) is computed from the
             // first matrix of a solve and kept for its later Newton steps; the finest level
@@ -1010,7 +1018,7 @@ static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals
         } else if constexpr (std::is_same<T, float>::value && std::is_same<TV, float>::value) {
             // FP32 V-cycle: the hierarchy is set up in FP64 (from the FP64 Jacobian when the
             // caller has one, else from the FP32 values widened), then rounded to FP32
-            int rc = amg_build(W->amg, p->ndof, p->row_start_h.data(), p->col_h.data(), p->dof_off_h, W->bytes);
+            int rc = amg_build(W->amg, p->ndof, p->row_start_h.data(), p->col_h.data(), p->dof_off_h, W->bytes, fix);
             if (rc) return rc;
             if (W->amg_stale) {
                 const double *src = W->amg_src64;
@@ -1868,7 +1876,8 @@ static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts
     if (o.precision == PYHIP_F64) {
         PYHIP_REQUIRE(jac_prec == PYHIP_F64, "an FP64 Krylov solve needs the FP64 Jacobian");
         W->ilu_order = o.ilu_order;
-        if ((rc = precond_setup<double, double>(p, W, o.precond, W->vals64, mask))) return rc;
+        if ((rc = precond_setup<double, double>(p, W, o.precond, W->vals64, mask, o.amg_element_level != 0)))
+            return rc;
         return krylov<double, double>(p, W, W->vals64, r, d, o, o.rtol, 0.0, mask, iters, failed);
     }
     // FP32 inner solves inside FP64 iterative refinement:
@@ -1880,7 +1889,7 @@ static int linear_solve(pyhip_problem *p, KrylovWork *W, const pyhip_linear_opts
     }
     W->ilu_order = o.ilu_order;
     W->amg_src64 = jac_prec == PYHIP_F64 ? W->vals64 : nullptr;
-    if ((rc = precond_setup<float, float>(p, W, o.precond, W->vals32, mask))) return rc;
+    if ((rc = precond_setup<float, float>(p, W, o.precond, W->vals32, mask, o.amg_element_level != 0))) return rc;
     PYHIP_CHECK_HIP(hipMemsetAsync(d, 0, n * sizeof(double), p->st));
     hipLaunchKernelGGL((convert_kernel<double, double>), dim3(nblk(n)), dim3(256), 0, p->st, n, r, W->r64);
     KCHECK();
@@ -1936,7 +1945,7 @@ int plugin_setup(pyhip_problem *p, const pyhip_linear_opts &o) {
 double *plugin_values(pyhip_problem *p) { return p->kw->vals64; }
 int plugin_factor(pyhip_problem *p, const pyhip_linear_opts &o) {
     p->kw->ilu_order = o.ilu_order;
-    return precond_setup<double, double>(p, p->kw, o.precond, p->kw->vals64);
+    return precond_setup<double, double>(p, p->kw, o.precond, p->kw->vals64, nullptr, o.amg_element_level != 0);
 }
 int plugin_solve(pyhip_problem *p, const pyhip_linear_opts &o, const double *b, double *x, int *iters) {
     KrylovWork *W = p->kw;
@@ -1969,6 +1978,7 @@ void pyhip_linear_opts_default(pyhip_linear_opts *o) {
     o->max_refinements = 8;
     o->refine_rtol = 1e-10;
     o->ilu_order = PYHIP_ILU_ORDER_AUTO;
+    o->amg_element_level = 1;
 }
 
 void pyhip_newton_opts_default(pyhip_newton_opts *o) {

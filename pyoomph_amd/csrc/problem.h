@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/pyoomph_codegen.h"
+#include "amg_plan.h"
 #include "common.h"
 #include "element_kernels.h"
 #include "pattern.h"
@@ -202,6 +203,7 @@ struct pyhip_problem {
     std::vector<int32_t> row_start_h, col_h;
     std::vector<int8_t> dof_val_h;           // [ndof] nodal value index of each unknown (Taylor-Hood: 0 ux, 1 uy, 2 p)
     std::vector<double> pmass_h;             // [ndof] lumped Q1 pressure mass of Taylor-Hood pressure unknowns
+    pyhip::AmgFixedInterp amg_fix;           // the multigrid's element level (quadratic elements of scalar forms)
     // element buffers (lazy)
     double *Er64 = nullptr, *Ej64 = nullptr;
     float *Er32 = nullptr, *Ej32 = nullptr;

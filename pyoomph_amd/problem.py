@@ -180,6 +180,9 @@ class Problem:
         # ILU(0) row order: "auto" (multicolour when the natural order has more than 64
         # triangular-solve levels), "natural" or "multicolor"
         self.ilu_order = "auto"
+        # multigrid: sub-problems on P2 / Q2 elements coarsen first to the linear functions
+        # on their vertices (False: aggregation from the first level)
+        self.amg_element_level = True
         self.gmres_max_iters = 5000
         self.gmres_rtol = 1e-10
         self.rtol_f32 = 1e-5
@@ -407,6 +410,7 @@ class Problem:
         o.linear.refine_rtol = self.refine_rtol
         o.linear.ilu_order = {"auto": _lib.ILU_ORDER_AUTO, "natural": _lib.ILU_ORDER_NATURAL,
                               "multicolor": _lib.ILU_ORDER_MULTICOLOR}[self.ilu_order]
+        o.linear.amg_element_level = 1 if self.amg_element_level else 0
         o.correction_tol = self.newton_correction_tolerance
         return o
 
