@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void amg_rowprod_kernel(int64_t nout, const in
                                                           const int32_t *__restrict__ srs,
                                                           const int32_t *__restrict__ scol,
                                                           const double *__restrict__ sval) {
-    static_assert(G == 16 || G == 32 || G == 64, "lanes per row");
+    static_assert(G == 4 || G == 8 || G == 16 || G == 32 || G == 64, "lanes per row");
     constexpr int NG = 256 / G;
     __shared__ int32_t lc[NG][CAPR];
     __shared__ double la[NG][CAPR];
@@ -223,7 +223,29 @@ __global__ __launch_bounds__(256) void amg_rowprod_kernel(int64_t nout, const in
     }
     for (int32_t k = lane; k < R; k += G) oval[o0 + k] = ma[k];
 }
-constexpr int kQLanes = 16, kQCap = 64, kCLanes = 32, kCCap = 128;
+constexpr int kQCap = 64, kCCap = 128;
+
+// One row-product launch with G lanes per output row, G chosen from the mean source-row
+// length (the lanes spread over a source row's entries: the element level's prolongator
+// rows hold 1-8 entries, an aggregation level's 10-40). Every G gives the same sums.
+template <int CAPR>
+static void launch_rowprod(double mean_src, hipStream_t st, int64_t nout, const int32_t *ors, const int32_t *ocol,
+                           double *oval, const int32_t *trs, const int32_t *trow, const double *tval,
+                           const int32_t *srs, const int32_t *scol, const double *sval) {
+#define PYHIP_ROWPROD(G)                                                                                          \
+    hipLaunchKernelGGL((amg_rowprod_kernel<G, CAPR>), dim3(ablk(nout, 256 / G)), dim3(256), 0, st, nout, ors, ocol, \
+                       oval, trs, trow, tval, srs, scol, sval)
+    if constexpr (CAPR <= 64) {   // 4 lanes: 64 rows per workgroup, their LDS rows CAPR long
+        if (mean_src <= 3.0) {
+            PYHIP_ROWPROD(4);
+            return;
+        }
+    }
+    if (mean_src <= 6.0) PYHIP_ROWPROD(8);
+    else if (mean_src <= 12.0) PYHIP_ROWPROD(16);
+    else PYHIP_ROWPROD(32);
+#undef PYHIP_ROWPROD
+}
 
 // ---- V-cycle kernels -----------------------------------------------------------------------
 
@@ -646,13 +668,13 @@ int amg_setup(AmgDev &A, hipStream_t st, const double *vals0, KernelTimer *tm, c
             hipLaunchKernelGGL(amg_pvals_kernel, dim3(ablk(L.nnzp)), dim3(256), 0, st, L.nnzp, L.prow,
                                L.pcptr, L.pcidx, L.pfix, vals, L.w, L.tinv, L.pv, L.ptv);
         // Q = A_l P (terms: A's row entries; source rows: P's rows)
-        hipLaunchKernelGGL((amg_rowprod_kernel<kQLanes, kQCap>), dim3(ablk(L.n, 256 / kQLanes)), dim3(256), 0, st,
-                           L.n, L.qrs, L.qcol, L.qv, L.rs, L.col, vals, L.prs, L.pcol, L.pv);
+        launch_rowprod<kQCap>(L.n ? (double)L.nnzp / L.n : 0.0, st, L.n, L.qrs, L.qcol, L.qv, L.rs, L.col, vals,
+                              L.prs, L.pcol, L.pv);
         AmgDevLevel &C = (l + 1 < A.lv.size()) ? A.lv[l + 1] : A.bot;
         // A_{l+1} = P^T Q (terms: P^T's row entries; source rows: Q's rows)
         if (L.nc)
-            hipLaunchKernelGGL((amg_rowprod_kernel<kCLanes, kCCap>), dim3(ablk(L.nc, 256 / kCLanes)), dim3(256), 0,
-                               st, L.nc, C.rs, C.col, C.own_vals, L.trs, L.trow, L.ptv, L.qrs, L.qcol, L.qv);
+            launch_rowprod<kCCap>(L.n ? (double)L.nnzq / L.n : 0.0, st, L.nc, C.rs, C.col, C.own_vals, L.trs, L.trow,
+                                  L.ptv, L.qrs, L.qcol, L.qv);
         ACHECK();
     }
     A.bot.vals = A.lv.empty() ? vals0 : A.bot.own_vals;
