@@ -360,6 +360,12 @@ def full_main(args):
         el_cg, st_cg, _ = run(P)
         x_cg = P.get_current_dofs()
         diff_cg = float(np.max(np.abs(x_cg - x64)) / max(np.max(np.abs(x64)), 1e-300))
+        # FP64 CG with the multigrid (its element level: Q1 functions on the brick vertices)
+        P.krylov, P.preconditioner = "cg", "amg"
+        P.jacobian_precision, P.linear_precision = "f64", "f64"
+        P.newton_solver_tolerance = 1e-15
+        el_amg, st_amg, _ = run(P)
+        diff_amg = float(np.max(np.abs(P.get_current_dofs() - x64)) / max(np.max(np.abs(x64)), 1e-300))
         out = {"workload": "3D reaction-diffusion -lap u + u^3 = f, 64^3 Q2 bricks, f = smoothed U(-1,1) "
                            "noise, Newton from 0 to max|R| < 1e-15", "ndof": P.ndof, "nnz": P.nnz, "setup_s": setup,
                "solve_s": el, "solver": "FP32 element kernels + FP32 ILU(0)-GMRES(50) inside FP64 iterative "
@@ -367,7 +373,9 @@ def full_main(args):
                "fp64_cg_jacobi_solve_s": el64, "fp64_stats": st64,
                "max_rel_diff_vs_fp64_solve": diff,
                "mixed_cg_jacobi_solve_s": el_cg, "mixed_cg_jacobi_stats": st_cg,
-               "mixed_cg_jacobi_max_rel_diff_vs_fp64_solve": diff_cg}
+               "mixed_cg_jacobi_max_rel_diff_vs_fp64_solve": diff_cg,
+               "fp64_cg_amg_solve_s": el_amg, "fp64_cg_amg_stats": st_amg,
+               "fp64_cg_amg_max_rel_diff_vs_fp64_cg_jacobi": diff_amg}
     print(json.dumps(out), flush=True)
 
 

@@ -302,7 +302,7 @@ int amg_plan_build(int64_t n0, const int32_t *rs0, const int32_t *col0, const st
 }
 
 void amg_element_level_rows(int kind, int64_t nnode, int64_t nelem, const int32_t *conn, const int32_t *eqn,
-                            int64_t ndof, bool applies, AmgFixedInterp &F) {
+                            int nvalue, int ncomp, int64_t ndof, bool applies, AmgFixedInterp &F) {
     const bool quad = kind == PYHIP_ELEM_TRI_P2 || kind == PYHIP_ELEM_QUAD_Q2 || kind == PYHIP_ELEM_HEX_Q2;
     const int64_t base = (int64_t)F.col.size();
     if (!quad || !applies || ndof == 0) {
@@ -342,33 +342,37 @@ void amg_element_level_rows(int kind, int64_t nnode, int64_t nelem, const int32_
     std::vector<char> isv(nnode, 0);
     for (int64_t e = 0; e < nelem; e++)
         for (int v = 0; v < nv; v++) isv[conn[e * nn + vloc[v]]] = 1;
-    std::vector<int32_t> cdof(nnode, -1);
+    // coarse unknowns: (vertex, component) pairs that are unknowns, node-major
+    std::vector<int32_t> cdof((size_t)nnode * ncomp, -1);
     int64_t nc = 0;
     for (int64_t n = 0; n < nnode; n++)
-        if (isv[n] && eqn[n] >= 0) cdof[n] = (int32_t)nc++;
-    // each unknown's entries from the first element holding it (at most 8, ascending)
+        for (int c = 0; c < ncomp; c++)
+            if (isv[n] && eqn[n * nvalue + c] >= 0) cdof[(size_t)n * ncomp + c] = (int32_t)nc++;
+    // each unknown's entries from the first element holding it (at most 8, ascending; the
+    // vertices' unknowns of its own component)
     std::vector<int8_t> cnt(ndof, -1);
     std::vector<int32_t> ecol((size_t)ndof * nv);
     std::vector<double> ew((size_t)ndof * nv);
     for (int64_t e = 0; e < nelem; e++)
-        for (int l = 0; l < nn; l++) {
-            const int32_t n = conn[e * nn + l], q = eqn[n];
-            if (q < 0 || cnt[q] >= 0) continue;
-            int c = 0;
-            for (int v = 0; v < nv; v++) {
-                const int32_t cv = cdof[conn[e * nn + vloc[v]]];
-                if (wt[l][v] == 0.0 || cv < 0) continue;
-                int k = c++;
-                while (k > 0 && ecol[(size_t)q * nv + k - 1] > cv) {
-                    ecol[(size_t)q * nv + k] = ecol[(size_t)q * nv + k - 1];
-                    ew[(size_t)q * nv + k] = ew[(size_t)q * nv + k - 1];
-                    k--;
+        for (int l = 0; l < nn; l++)
+            for (int comp = 0; comp < ncomp; comp++) {
+                const int32_t n = conn[e * nn + l], q = eqn[(int64_t)n * nvalue + comp];
+                if (q < 0 || cnt[q] >= 0) continue;
+                int c = 0;
+                for (int v = 0; v < nv; v++) {
+                    const int32_t cv = cdof[(size_t)conn[e * nn + vloc[v]] * ncomp + comp];
+                    if (wt[l][v] == 0.0 || cv < 0) continue;
+                    int k = c++;
+                    while (k > 0 && ecol[(size_t)q * nv + k - 1] > cv) {
+                        ecol[(size_t)q * nv + k] = ecol[(size_t)q * nv + k - 1];
+                        ew[(size_t)q * nv + k] = ew[(size_t)q * nv + k - 1];
+                        k--;
+                    }
+                    ecol[(size_t)q * nv + k] = cv;
+                    ew[(size_t)q * nv + k] = wt[l][v];
                 }
-                ecol[(size_t)q * nv + k] = cv;
-                ew[(size_t)q * nv + k] = wt[l][v];
+                cnt[q] = (int8_t)c;
             }
-            cnt[q] = (int8_t)c;
-        }
     F.on.push_back(nc > 0 && nc < ndof ? 1 : 0);
     F.nc.push_back(nc);
     for (int64_t q = 0; q < ndof; q++) {

@@ -70,14 +70,16 @@ struct AmgFixedInterp {
 
 // Appends one sub-problem's rows of the element level to F (its unknowns' rows; F.rs gets
 // no closing entry): on P2 triangles, Q2 quadrilaterals and Q2 bricks (PYHIP_ELEM_*) when
-// `applies` (a scalar field, one value per node), the interpolation of the linear (P1 / Q1)
-// functions on the element vertices into the element nodes; else empty rows. conn [nelem]
-// [nodes per element], eqn [nnode] (the node's unknown or -1). Coarse unknowns: the vertices
-// that are unknowns, in node order. A vertex row holds itself with weight 1; a node between
-// vertices (edge midpoints, face and cell centres) holds the vertices' linear shape
-// functions at its position (1/2, 1/4, 1/8), pinned vertices left out.
+// `applies`, the interpolation of the linear (P1 / Q1) functions on the element vertices
+// into the element nodes, for each of the nodal values 0..ncomp-1 (a scalar field, or the
+// velocity components of the Taylor-Hood layout) separately; the rows of other values
+// stay empty. conn [nelem][nodes per element], eqn [nnode][nvalue] (the unknown or -1).
+// Coarse unknowns: the (vertex, value) pairs that are unknowns, node-major. A vertex row
+// holds itself with weight 1; a node between vertices (edge midpoints, face and cell
+// centres) holds the vertices' linear shape functions at its position (1/2, 1/4, 1/8),
+// pinned vertices left out.
 void amg_element_level_rows(int elem_kind, int64_t nnode, int64_t nelem, const int32_t *conn, const int32_t *eqn,
-                            int64_t ndof, bool applies, AmgFixedInterp &F);
+                            int nvalue, int ncomp, int64_t ndof, bool applies, AmgFixedInterp &F);
 
 // Builds the plan from the level-0 pattern. A sub-problem stops coarsening once it has at
 // most `bottom_rows` rows; the hierarchy has at most `max_levels` coarsenings (the

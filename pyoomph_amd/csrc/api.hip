@@ -103,10 +103,15 @@ static int face_kind(int elem_kind, int form) {
 }
 
 // The multigrid's element level of one sub-problem (amg_element_level_rows): scalar forms
-// without EnforcedBC multipliers
-static void element_level_rows(const pyhip_subproblem_desc &d, AmgFixedInterp &F) {
+// without EnforcedBC multipliers; with `velocity`, also the two velocity components of the
+// Taylor-Hood layout (the block preconditioner's velocity V-cycle)
+static void element_level_rows(const pyhip_subproblem_desc &d, AmgFixedInterp &F, bool velocity = false) {
+    if (velocity && th_layout(d.form)) {
+        amg_element_level_rows(d.elem_kind, d.nnode, d.nelem, d.conn, d.eqn, d.nvalue, 2, d.ndof, true, F);
+        return;
+    }
     const bool scalar = d.nvalue == 1 && !th_layout(d.form) && d.nenforced == 0;
-    amg_element_level_rows(d.elem_kind, d.nnode, d.nelem, d.conn, d.eqn, d.ndof, scalar, F);
+    amg_element_level_rows(d.elem_kind, d.nnode, d.nelem, d.conn, d.eqn, 1, 1, d.ndof, scalar, F);
 }
 
 // face element kind of the EnforcedBC faces of a bulk element kind (-1: not built)
@@ -334,10 +339,20 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
             }
         }
     }
-    // the multigrid's element level, rows over all unknowns
+    // the multigrid's element level, rows over all unknowns; with Taylor-Hood sub-problems
+    // also the block preconditioner's (velocity components interpolated too)
     P->amg_fix = AmgFixedInterp();
-    for (int s = 0; s < nsub; s++) element_level_rows(subs[s], P->amg_fix);
+    P->amg_fix_vel = AmgFixedInterp();
+    bool any_th = false;
+    for (int s = 0; s < nsub; s++) {
+        element_level_rows(subs[s], P->amg_fix);
+        any_th = any_th || th_layout(subs[s].form);
+    }
     P->amg_fix.rs.push_back((int32_t)P->amg_fix.col.size());
+    if (any_th) {
+        for (int s = 0; s < nsub; s++) element_level_rows(subs[s], P->amg_fix_vel, true);
+        P->amg_fix_vel.rs.push_back((int32_t)P->amg_fix_vel.col.size());
+    }
     // pattern: element blocks in element order of each sub-problem
     std::vector<ElementBlock> blocks;
     for (int s = 0; s < nsub; s++) {
@@ -837,7 +852,7 @@ int pyhip_timing_bytes(pyhip_problem *p, const char *family, int64_t *bytes) {
 }  // extern "C"
 
 // Host only (tests on the CPU): the multigrid element-level rows of one sub-problem
-// descriptor: nc coarse unknowns, rs [ndof+1], and (with col / w non-null) the entries,
+// descriptor (Taylor-Hood: the block preconditioner's, velocity components): nc coarse unknowns, rs [ndof+1], and (with col / w non-null) the entries,
 // nnz_out of them. No device work.
 extern "C" int pyhip_element_level_rows(const pyhip_subproblem_desc *d, int64_t *nc, int32_t *rs, int32_t *col,
                                         double *w, int64_t *nnz_out) {
@@ -850,7 +865,7 @@ extern "C" int pyhip_element_level_rows(const pyhip_subproblem_desc *d, int64_t 
     for (int64_t k = 0; k < d->nnode * d->nvalue; k++)
         PYHIP_REQUIRE(d->eqn[k] >= -1 && d->eqn[k] < d->ndof, "equation number out of range");
     AmgFixedInterp F;
-    element_level_rows(*d, F);
+    element_level_rows(*d, F, true);
     F.rs.push_back((int32_t)F.col.size());
     *nc = F.on[0] ? F.nc[0] : 0;
     *nnz_out = (int64_t)F.col.size();

@@ -40,6 +40,7 @@ struct IluPlan {
 //   scalar forms: their whole matrix), on its own multigrid hierarchy.
 struct BlockPlan {
     bool built = false;
+    bool element_level = false;    // A_d's multigrid built with the element level
     int64_t nv = 0, np = 0, nnzv = 0;
     int32_t *vrows = nullptr;      // [nv] problem row of each velocity-block row
     int32_t *vsrc = nullptr;       // [nnzv] index of each A_d entry in the problem's CSR values

@@ -836,7 +836,7 @@ __global__ void blk_vscatter_kernel(int64_t nv, const int32_t *__restrict__ vrow
 
 // host plan, once per pattern: the velocity / pressure split, A_d's pattern with the source
 // of each entry, and A_d's multigrid plan
-static int blk_plan_build(pyhip_problem *p, KrylovWork *W) {
+static int blk_plan_build(pyhip_problem *p, KrylovWork *W, bool element_level) {
     BlockPlan &B = W->blk;
     if (B.built) return PYHIP_OK;
     const int64_t n = p->ndof;
@@ -897,7 +897,25 @@ static int blk_plan_build(pyhip_problem *p, KrylovWork *W) {
     if (B.np) PYHIP_CHECK_HIP(hipMemcpy(B.pmass, pmass.data(), B.np * sizeof(double), hipMemcpyHostToDevice));
     PYHIP_CHECK_HIP(hipMemset(B.pz, 0, std::max<int64_t>(1, n) * sizeof(double)));
     B.amg.nonsym = true;
-    if ((rc = amg_build(B.amg, B.nv, B.rs_h.data(), B.col_h.data(), B.off_h, W->bytes))) return rc;
+    // the element level of A_d: the problem's rows restricted to the velocity rows
+    AmgFixedInterp F;
+    const AmgFixedInterp &G = p->amg_fix_vel.rs.empty() ? p->amg_fix : p->amg_fix_vel;   // no Taylor-Hood: scalar rows
+    const bool fix = element_level && G.rs.size() == (size_t)n + 1;
+    if (fix) {
+        F.on = G.on;
+        F.nc = G.nc;
+        F.rs.assign(1, 0);
+        for (const int32_t r : vrows) {
+            for (int32_t t = G.rs[r]; t < G.rs[r + 1]; t++) {
+                F.col.push_back(G.col[t]);
+                F.w.push_back(G.w[t]);
+            }
+            F.rs.push_back((int32_t)F.col.size());
+        }
+    }
+    if ((rc = amg_build(B.amg, B.nv, B.rs_h.data(), B.col_h.data(), B.off_h, W->bytes, fix ? &F : nullptr)))
+        return rc;
+    B.element_level = element_level;
     B.built = true;
     return PYHIP_OK;
 }
@@ -911,8 +929,9 @@ static void blk_free(KrylovWork *W) {
 }
 
 // numeric set-up from the FP64 Jacobian values: A_d's values and hierarchy, mu / m_p
-static int blk_setup(pyhip_problem *p, KrylovWork *W, const double *vals, const int32_t *need) {
-    int rc = blk_plan_build(p, W);
+static int blk_setup(pyhip_problem *p, KrylovWork *W, const double *vals, const int32_t *need, bool element_level) {
+    if (W->blk.built && W->blk.element_level != element_level) blk_free(W);
+    int rc = blk_plan_build(p, W, element_level);
     if (rc) return rc;
     BlockPlan &B = W->blk;
     B.jvals = vals;
@@ -992,7 +1011,7 @@ static int precond_setup(pyhip_problem *p, KrylovWork *W, int pc, const TV *vals
                                        "use the ILU(0) preconditioner");
     if (pc == PYHIP_PRECOND_BLOCK) {
         if constexpr (std::is_same<T, double>::value && std::is_same<TV, double>::value)
-            return blk_setup(p, W, vals, need);
+            return blk_setup(p, W, vals, need, element_level);
         set_error("the block preconditioner runs FP64 solves");
         return PYHIP_ERR_INVALID;
     }

@@ -204,6 +204,7 @@ struct pyhip_problem {
     std::vector<int8_t> dof_val_h;           // [ndof] nodal value index of each unknown (Taylor-Hood: 0 ux, 1 uy, 2 p)
     std::vector<double> pmass_h;             // [ndof] lumped Q1 pressure mass of Taylor-Hood pressure unknowns
     pyhip::AmgFixedInterp amg_fix;           // the multigrid's element level (quadratic elements of scalar forms)
+    pyhip::AmgFixedInterp amg_fix_vel;       // ... and with the Taylor-Hood velocity components (block preconditioner)
     // element buffers (lazy)
     double *Er64 = nullptr, *Ej64 = nullptr;
     float *Er32 = nullptr, *Ej32 = nullptr;

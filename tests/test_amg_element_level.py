@@ -15,7 +15,7 @@ from pyoomph_amd.meshes import CuboidBrickMesh, RectangularQuadMesh
 from tests.test_amg_plan import _bool, _same_pattern
 
 
-def _rows(mesh, eqn, ndof, form=_lib.FORM_POISSON):
+def _rows(mesh, eqn, ndof, form=_lib.FORM_POISSON, nvalue=1):
     L = _lib.load()
     L.pyhip_element_level_rows.argtypes = [ctypes.POINTER(_lib.SubproblemDesc), _lib.c_i64p, _lib.c_i32p,
                                            _lib.c_i32p, _lib.c_dp, _lib.c_i64p]
@@ -23,7 +23,7 @@ def _rows(mesh, eqn, ndof, form=_lib.FORM_POISSON):
     eqn = np.ascontiguousarray(eqn, np.int32)
     d = _lib.SubproblemDesc()
     d.form, d.elem_kind = form, _lib.ELEM_KIND[mesh.kind]
-    d.nnode, d.nelem, d.nvalue, d.ndof = mesh.nnode, mesh.nelement, 1, ndof
+    d.nnode, d.nelem, d.nvalue, d.ndof = mesh.nnode, mesh.nelement, nvalue, ndof
     d.conn = conn.ctypes.data_as(_lib.c_i32p)
     d.eqn = eqn.ctypes.data_as(_lib.c_i32p)
     nc, nnz = ctypes.c_int64(), ctypes.c_int64()
@@ -191,3 +191,33 @@ def test_plan_with_the_element_level():
     P = _bool(prs, pcol, (n, lv["nc"]))
     assert _same_pattern(P.T, lv["trs"], lv["trow"])
     assert _same_pattern(_bool(A.indptr, A.indices, A.shape) @ P, lv["qrs"], lv["qcol"])
+
+
+def test_taylor_hood_velocity_components():
+    """The block preconditioner's element level on Taylor-Hood (ux, uy at every Q2 node, p at
+    the vertices): each velocity component is interpolated from its own vertex unknowns
+    (coarse unknowns node-major: vertex k's ux, uy are 2k, 2k+1), reproducing bilinear
+    functions; pressure rows are empty."""
+    mesh = RectangularQuadMesh(4, space="C2")
+    n = mesh.nnode
+    verts = _vertices(mesh)
+    eqn = -np.ones((n, 3), int)
+    q = 0
+    for node in range(n):
+        for c in range(3):
+            if c < 2 or node in set(verts):
+                eqn[node, c] = q
+                q += 1
+    nc, rs, col, w = _rows(mesh, eqn.ravel(), q, form=_lib.FORM_NAVIER_STOKES_TH, nvalue=3)
+    assert nc == 2 * len(verts)
+    P = sp.csr_matrix((w, col, rs), shape=(q, nc))
+    assert np.all(np.diff(rs)[eqn[verts, 2]] == 0)
+    X = np.asarray(mesh.coords)
+    rng = np.random.default_rng(4)
+    for c in range(2):
+        a = rng.uniform(-1, 1, 4)
+        f = a[0] + a[1] * X[:, 0] + a[2] * X[:, 1] + a[3] * X[:, 0] * X[:, 1]
+        coarse = np.zeros(nc)
+        coarse[2 * np.arange(len(verts)) + c] = f[verts]
+        assert np.max(np.abs((P @ coarse)[eqn[:, c]] - f)) < 1e-14
+        assert np.max(np.abs((P @ coarse)[eqn[:, 1 - c]])) == 0.0
