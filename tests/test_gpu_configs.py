@@ -247,7 +247,7 @@ def converged_slice():
 
 @pytest.mark.parametrize("precision", ["f64", "mixed"])
 def test_benchmark_settings_reach_the_converged_solution(gpu, converged_slice, precision):
-    """The benchmark's solver settings on a slice of the sweep: inexact Newton (CG to 3e-3
+    """The benchmark's solver settings on a slice of the sweep: inexact Newton (CG to 5e-3
     relative residual per step; in mixed precision FP32 element kernels for J and one FP32
     CG solve with the FP32 V-cycle per step, FP64 residual) stopping at max|R| < 1e-8 and
     an FP64 update below 1e-6 of the solution. Every sub-problem is within 1e-8 of the
@@ -258,7 +258,7 @@ def test_benchmark_settings_reach_the_converged_solution(gpu, converged_slice, p
     P = Problem([wl.bratu(*s) for s in sl])
     P.krylov, P.preconditioner = "cg", "amg"
     P.jacobian_precision = P.linear_precision = "f32" if precision == "mixed" else "f64"
-    P.gmres_rtol = P.rtol_f32 = 3e-3
+    P.gmres_rtol = P.rtol_f32 = 5e-3
     P.max_refinements = 1
     P.newton_correction_tolerance = 1e-6
     P.max_newton_iterations = 20
