@@ -54,7 +54,8 @@ int pyhip_codegen_set_layout(int form_id, int layout);
 /* Marks a generated form as a boundary-face form (layout 2): its kernels run one thread
  * per face of the line faces of the element family it was compiled for (P1 / P2 triangles,
  * Q2 quadrilaterals), the faces of pyhip_subproblem_desc whose face_form names it, with
- * ncoef coefficients per face (face_coef [nface][ncoef]). It replaces the built-in flux
+ * ncoef coefficients per face (face_coef [nface][ncoef]; a generated flux that reads the
+ * outward normal takes the face's side, +-1, as its last coefficient). It replaces the built-in flux
  * alpha*u + beta of NeumannBC by the generated flux expression (the reference generates
  * the interface element code of NeumannBC(u=expression) like any residual,
  * src/codegen.cpp:3912-4109, pyoomph/meshes/bcs.py:71-89). Set before the form is used. */

@@ -87,14 +87,16 @@ class OracleProblem:
     face_expr -- (flux, dflux/du, dflux/dparam, d2flux/du dparam; the last two may be None)
                  callables (u, x0, x1, C) replacing
                  the flux alpha*u + beta of the faces by an expression, C = face_coef
-                 [nface, ncoef] (``faces.expression_face_outputs``)
+                 [nface, ncoef] (``faces.expression_face_outputs``); with face_normal the
+                 expressions also take the outward unit normal (n0, n1)
     enforced  -- [nenf, 2|3] boundary-face nodes of EnforcedBC(u=enf_alpha*u + enf_beta)
                  (``faces.enforced_face_outputs``), or None; the scalar forms then have
                  nvalue 2 (u, the multiplier); these faces follow the flux faces
     """
 
     def __init__(self, mesh, form, nvalue, pinned, bc, params, aux=None, faces=None, face_alpha=0.0,
-                 face_beta=0.0, enforced=None, enf_alpha=1.0, enf_beta=0.0, face_expr=None, face_coef=None):
+                 face_beta=0.0, enforced=None, enf_alpha=1.0, enf_beta=0.0, face_expr=None, face_coef=None,
+                 face_normal=False):
         from .meshes import number_equations
         self.mesh, self.form, self.nvalue = mesh, form, nvalue
         self.pinned = np.asarray(pinned, dtype=bool)
@@ -127,6 +129,7 @@ class OracleProblem:
         self.geqn[rows[0], self.leqn[rows]] = g[rows]
         self.faces = None
         self.face_expr = face_expr
+        self.face_normal = bool(face_normal)
         self.face_coef = None if face_coef is None else np.asarray(face_coef, dtype=np.float64)
         if faces is not None and len(faces):
             nf = len(faces)
@@ -279,14 +282,18 @@ class OracleProblem:
             Xf = np.ascontiguousarray(self.mesh.coords[F][:, :, :2])
             quad = self.mesh.kind == "quad_q2"
             Uf = np.ascontiguousarray(U[F, 0])
+            sides = None
+            if self.face_normal:
+                from .faces import outward_sides
+                sides = outward_sides(self.mesh.coords, self.mesh.elements, F)
             if deriv:
                 # dR/dp and dJ/dp: the flux's parameter derivative and its u-derivative
-                zero = lambda u, x0, x1, C: np.zeros_like(u)  # noqa: E731
+                zero = lambda u, *args: np.zeros_like(u)  # noqa: E731
                 f = dflux_dp if dflux_dp is not None else zero
                 d = d2flux_dudp if d2flux_dudp is not None else zero
-                Rs, Js = expression_face_outputs(Xf, Uf, self.face_coef, f, d, fl, quad=quad)
+                Rs, Js = expression_face_outputs(Xf, Uf, self.face_coef, f, d, fl, quad=quad, sides=sides)
             else:
-                Rs, Js = expression_face_outputs(Xf, Uf, self.face_coef, flux, dflux, fl, quad=quad)
+                Rs, Js = expression_face_outputs(Xf, Uf, self.face_coef, flux, dflux, fl, quad=quad, sides=sides)
             if flag == 2 or y is not None:
                 Js[:] = 0.0
             deriv = False   # handled: the derivative entries are the expression's

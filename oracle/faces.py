@@ -188,11 +188,14 @@ def enforced_face_outputs(X, U, Lm, alpha, beta, flag, quad=False):
     return R, J
 
 
-def expression_face_outputs(X, U, C, flux, dflux, flag, quad=False):
+def expression_face_outputs(X, U, C, flux, dflux, flag, quad=False, sides=None):
     """Faces of NeumannBC(u=flux) with a flux expression (the generated boundary-face forms):
     flux(u, x0, x1, C) and its u-derivative dflux(...) evaluated per point for all faces at
     once (C [nf, ncoef] the faces' coefficients); otherwise face_outputs' rules and order:
-    R_i += W (flux psi_i), J_ij += W ((dflux psi_j) psi_i)."""
+    R_i += W (flux psi_i), J_ij += W ((dflux psi_j) psi_i). With sides [nf] (+-1), the
+    expressions also take the outward unit normal (n0, n1): the line element's normal of
+    the reference, the tangent turned counter-clockwise over its length (src/elements.cpp:
+    1724-1750), times the face's side."""
     nf, nn = U.shape
     knots, weights = _RULE_Q3 if quad else _RULES[nn]
     R = np.zeros((nf, nn))
@@ -213,11 +216,38 @@ def expression_face_outputs(X, U, C, flux, dflux, flag, quad=False):
         for l in range(nn):
             x1 = x1 + X[:, l, 1] * psi[l]
         W = weights[g] * np.sqrt(tx * tx + ty * ty)
-        F = flux(u, x0, x1, C)
-        D = dflux(u, x0, x1, C)
+        if sides is None:
+            F = flux(u, x0, x1, C)
+            D = dflux(u, x0, x1, C)
+        else:
+            tl = np.sqrt(tx * tx + ty * ty)
+            n0, n1 = sides * (-ty) / tl, sides * tx / tl
+            F = flux(u, x0, x1, C, n0, n1)
+            D = dflux(u, x0, x1, C, n0, n1)
         for i in range(nn):
             R[:, i] = R[:, i] + W * (F * psi[i])
             if flag == 1:
                 for j in range(nn):
                     J[:, i, j] = J[:, i, j] + W * ((D * psi[j]) * psi[i])
     return R, J
+
+
+def outward_sides(coords, elements, faces):
+    """+-1 per line face so that the reference's line normal (tangent from the first to the
+    last face node turned counter-clockwise) leaves the bulk element holding the face (found
+    by its nodes; its node centroid is inside it)."""
+    coords = np.asarray(coords)[:, :2]
+    elements = np.asarray(elements)
+    holder = {}
+    for e, nodes in enumerate(elements):
+        for n in nodes:
+            holder.setdefault(int(n), set()).add(e)
+    sides = np.zeros(len(faces))
+    for f, fn in enumerate(faces):
+        common = set.intersection(*[holder[int(n)] for n in fn])
+        e = min(common)
+        t = coords[fn[-1]] - coords[fn[0]]
+        n = np.array([-t[1], t[0]])
+        mid = 0.5 * (coords[fn[-1]] + coords[fn[0]])
+        sides[f] = 1.0 if np.dot(n, mid - coords[elements[e]].mean(axis=0)) > 0.0 else -1.0
+    return sides

@@ -10,6 +10,11 @@ an expression in
     c[k]    coefficients of each face (one row of the faces' coefficient array),
     p[k]    the sub-problem's parameters (one of them may be the global parameter, with
             dR/dp as ``dResidual{i}dParameter_{p}``, src/codegen.cpp:4733-4752),
+    n0, n1  the outward unit normal (pyoomph's var("normal") on an interface; the
+            reference's line-element normal is the tangent dx/ds turned by a quarter over
+            its length, src/elements.cpp:1724-1750): here at every point the tangent turned
+            clockwise, times the face's side (an extra coefficient after the form's own,
+            +-1, which MeshProblem sets so that the normal leaves the bulk element),
 
 so that, on the line faces of P1 / P2 triangles and Q2 quadrilaterals,
 
@@ -20,7 +25,7 @@ its nodal coordinates and values in registers, per point the line's tangent, W =
 and x, then the point quantities (flux and its u-derivative, printed from the expression
 tree by ``expr.CPrinter``), and every residual and Jacobian entry accumulated in point order.
 The flux alpha*u + beta written as ``c[0] * u + c[1]`` reproduces the built-in faces bit for
-bit. Faces have no time derivative (the mass matrix is zero) and no generated Hessian.
+bit; a flux that reads the normal gets it per point from the tangent. Faces have no time derivative (the mass matrix is zero) and no generated Hessian.
 """
 from . import expr as E
 
@@ -36,6 +41,7 @@ class FaceSymbols:
         self.x = [E.sym("x0"), E.sym("x1")]
         self.c = [E.sym("T(cf[%d])" % k) for k in range(ncoef)]
         self.p = [E.sym("T(par[%d])" % k) for k in range(nparams)]
+        self.n = [E.sym("n0"), E.sym("n1")]
 
 
 class FaceFluxForm:
@@ -59,6 +65,15 @@ class FaceFluxForm:
         F = E._wrap(self.flux_fn(s))
         return s, F
 
+    def uses_normal(self):
+        """The flux reads the normal: each face then carries its side as coefficient ncoef."""
+        s, F = self.build()
+        return _uses(F, s.n)
+
+    def device_ncoef(self):
+        """Coefficients per face on the device (the side of each face after the form's own)."""
+        return self.ncoef + (1 if self.uses_normal() else 0)
+
 
 KERNEL = r'''
 #include "element_common.h"
@@ -70,7 +85,7 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
     using RULE = {rule};
     const int e = a.e0 + (int)(blockIdx.x * 256 + threadIdx.x);
     if (e >= a.e1) return;
-    const double *cf = a.fcoef + (int64_t)e * {ncoef};
+    const double *cf = a.fcoef + (int64_t)e * {dncoef};
     const double *par = a.params + (int64_t)a.sub[e] * a.pstride;
     (void)cf;
     (void)par;
@@ -107,6 +122,7 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
         (void)u;
 {x_block}
         const T W = wq * sqrt(tx * tx + ty * ty);
+{n_block}
         T Q[{nqv}];
         if (a.deriv) {{
 {phase_a_deriv}
@@ -213,17 +229,24 @@ def emit_face_kernels(form, kind):
             xl.append("#pragma unroll")
             xl.append("for (int l = 0; l < NN; l++) x%d = x%d + X[l][%d] * psi[l];" % (a_, a_, a_))
         x_block = _indent(xl, 8)
+    n_block = ""
+    normal = _uses(F, s.n)
+    if normal:
+        # the tangent turned clockwise, oriented by the face's side, over its length
+        n_block = _indent(["const T tl = sqrt(tx * tx + ty * ty), side = T(cf[%d]);" % form.ncoef,
+                           "const T n0 = side * ty / tl, n1 = -(side * tx) / tl;",
+                           "(void)n0;", "(void)n1;"], 8)
     # the derivative entries of a form without a global parameter are zero: Q stays unset
     # and the integrands are the constant 0
     src = KERNEL.format(
-        name=form.name, nn=nn, rule=rule, ncoef=form.ncoef, nqv=nqv,
-        x_block=x_block,
+        name=form.name, nn=nn, rule=rule, dncoef=form.ncoef + (1 if normal else 0), nqv=nqv,
+        x_block=x_block, n_block=n_block,
         phase_a=_indent(lines(quantities, "a") or ["(void)Q;"], 12),
         phase_a_deriv=_indent(lines(q2, "d") or ["(void)Q;"], 12),
         res="a.deriv ? (%s) : (%s)" % (b_text(res_d), b_text(res)),
         jac="a.deriv ? (%s) : (%s)" % (b_text(jac_d), b_text(jac)))
     info = {"kind": kid, "face_nodes": nn, "ncoef": form.ncoef, "nquantities": nqv, "symmetric": False,
-            "has_param_derivative": form.global_param is not None}
+            "has_param_derivative": form.global_param is not None, "normal": normal}
     return src, form.name, info
 
 

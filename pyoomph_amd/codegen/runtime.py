@@ -99,7 +99,7 @@ def compile_face_form(form, kind, load=True):
                                        ctypes.byref(size)))
     if not load:
         return size.value
-    _lib.check(L.pyhip_codegen_set_face_form(fid.value, form.ncoef))
+    _lib.check(L.pyhip_codegen_set_face_form(fid.value, form.ncoef + (1 if info["normal"] else 0)))
     _lib.check(L.pyhip_codegen_set_fields(fid.value, b"u", None, int(form.global_param or 0)))
     _registry[key] = fid.value
     return fid.value

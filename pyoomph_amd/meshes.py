@@ -75,18 +75,19 @@ _FACE_EDGES = {"tri_p1": ((0, 1), (1, 2), (2, 0)), "tri_p2": ((0, 3, 1), (1, 4, 
                           (0, 3, 6, 9, 12, 15, 18, 21, 24), (2, 5, 8, 11, 14, 17, 20, 23, 26))}
 
 
-def boundary_faces(mesh, names):
+def boundary_faces(mesh, names, return_elements=False):
     """Nodes of the bulk faces on the named boundaries, for the interface elements that
     carry a boundary flux (the reference builds one interface element per bulk face on
     the boundary, src/mesh.cpp:615-660): int32 [nface, 2] (P1 triangles: end, end) or
     [nface, 3] (P2 triangles, Q2 quadrilaterals: end, midside, end) or [nface, 9] (Q2 bricks:
     the face's 3x3 node lattice); one boundary after the other, bulk elements in element
-    order, faces of each in the order of ``_FACE_EDGES``."""
+    order, faces of each in the order of ``_FACE_EDGES``. With return_elements, also the bulk
+    element of each face."""
     if mesh.kind not in _FACE_EDGES:
         raise ValueError("boundary-face fluxes are built for P1 / P2 triangle, Q2 quadrilateral and Q2 brick meshes")
     E = mesh.elements
     edges = _FACE_EDGES[mesh.kind]
-    out = []
+    out, owners = [], []
     for name in names:
         on = mesh.boundary_nodes(name)
         sel = np.stack([np.all(on[E[:, list(ed)]], axis=1) for ed in edges], axis=1)   # [nel, 3]
@@ -94,6 +95,7 @@ def boundary_faces(mesh, names):
         f = np.stack([E[el, ed] for ed in np.array(edges)[k].T], axis=1) if len(el) else \
             np.zeros((0, len(edges[0])), dtype=np.int64)
         out.append(f)
+        owners.append(el)
     faces = np.ascontiguousarray(np.concatenate(out), dtype=np.int32)
     c = mesh.coords
     if faces.shape[1] == 3 and len(faces):
@@ -105,7 +107,19 @@ def boundary_faces(mesh, names):
                         rtol=0, atol=1e-12)
         if not ok:
             raise ValueError("boundary face nodes of a Q2 brick are not in face-lattice order")
+    if return_elements:
+        return faces, np.concatenate(owners).astype(np.int64)
     return faces
+
+
+def face_sides(mesh, faces, elements):
+    """+1 or -1 per line face: the side whose clockwise-turned tangent (last node minus
+    first) leaves its bulk element (points away from the element's node centroid)."""
+    c = np.asarray(mesh.coords)[:, :2]
+    t = c[faces[:, -1]] - c[faces[:, 0]]
+    n = np.stack([t[:, 1], -t[:, 0]], axis=1)
+    away = 0.5 * (c[faces[:, -1]] + c[faces[:, 0]]) - c[np.asarray(mesh.elements)[elements]].mean(axis=1)
+    return np.where(np.sum(n * away, axis=1) > 0.0, 1.0, -1.0)
 
 
 def _number(keys, coords):

@@ -59,7 +59,8 @@ class MeshProblem:
     saddle point: solve with GMRES and ILU(0).
     flux_form: a generated boundary-face form (``codegen.face.FaceFluxForm``) whose flux
     expression replaces alpha*u + beta on the flux faces; flux then lists (boundary names,
-    coefficients) with the form's ``ncoef`` coefficients per condition.
+    coefficients) with the form's ``ncoef`` coefficients per condition (a form reading the
+    normal gets each face's side appended).
     """
 
     def __init__(self, mesh, equations, dirichlet=(), pin_extra=(), flux=None, enforced=None, flux_form=None):
@@ -114,18 +115,23 @@ class MeshProblem:
         self.faces, self.face_coef = None, None
         self.flux_form = flux_form
         if flux_form is not None:
-            from .meshes import boundary_faces
+            from .meshes import boundary_faces, face_sides
             if nv != 1:
                 raise ValueError("generated boundary-face forms are built for the scalar forms")
+            normal = flux_form.uses_normal()
             faces, coef = [], []
             for names, c in flux:
                 names = [names] if isinstance(names, str) else list(names)
                 c = np.asarray(c, dtype=np.float64).reshape(-1)
                 if c.size != flux_form.ncoef:
                     raise ValueError("the face form takes %d coefficients per condition" % flux_form.ncoef)
-                f = boundary_faces(mesh, names)
+                f, owner = boundary_faces(mesh, names, return_elements=True)
                 faces.append(f)
-                coef.append(np.tile(c, (len(f), 1)))
+                cf = np.tile(c, (len(f), 1))
+                if normal:
+                    # the face's side for the outward normal, after the form's coefficients
+                    cf = np.concatenate([cf, face_sides(mesh, f, owner)[:, None]], axis=1)
+                coef.append(cf)
             self.faces = np.ascontiguousarray(np.concatenate(faces), dtype=np.int32)
             self.face_coef = np.ascontiguousarray(np.concatenate(coef).reshape(len(self.faces), -1),
                                                   dtype=np.float64)

@@ -402,3 +402,38 @@ def test_generated_hessian_products_of_a_non_symmetric_form(gpu):
     assert np.max(np.abs(H @ v - fd)) <= 1e-7 * np.max(np.abs(fd))
     assert np.max(np.abs(Ht @ v - fdt)) <= 1e-7 * np.max(np.abs(fdt))
     assert np.max(np.abs((H - Ht).toarray())) > 1e-3 * np.max(np.abs(H.toarray()))
+
+
+@pytest.mark.parametrize("space,split", FACE_CASES)
+def test_generated_flux_with_the_normal_matches_the_oracle(gpu, space, split):
+    """NeumannBC(u = c0 n_x + c1 n_y u) on the right, top and bottom sides (the normal from
+    the tangent and each face's side): assembly within 1e-13 of the oracle's expression faces
+    with its own outward normal, and on the right side alone the same residual as the
+    constant flux c0 (n = (1, 0) there)."""
+    from oracle import assembly as oa
+    from oracle import faces as of
+    from oracle import meshes as om
+    from pyoomph_amd.codegen.face import FaceFluxForm
+    from pyoomph_amd.problem import Problem
+    c = (0.7, -1.3)
+    form = FaceFluxForm("gen_normal_flux", lambda s: s.c[0] * s.n[0] + s.c[1] * s.n[1] * s.u, ncoef=2)
+    sides = ("right", "top", "bottom")
+    P = Problem(_face_pair(space, split, flux_form=form, flux=[(sides, c)]))
+    m = om.rectangular_quad_mesh(7, split, space)
+    faces = of.boundary_faces(m, list(sides))
+    op = oa.OracleProblem(m, oa.FORM_BRATU, 1, m.on_boundary(["left"])[:, None], np.zeros((m.nnode, 1)), [1.0],
+                          faces=faces, face_coef=np.tile(np.asarray(c), (len(faces), 1)), face_normal=True,
+                          face_expr=(lambda u, x0, x1, C, n0, n1: C[:, 0] * n0 + (C[:, 1] * n1) * u,
+                                     lambda u, x0, x1, C, n0, n1: C[:, 1] * n1, None, None))
+    assert P.ndof == op.ndof
+    x = np.random.default_rng(5).uniform(-0.3, 0.3, P.ndof)
+    P.set_current_dofs(x)
+    R, J = P.assemble_jacobian()
+    r, rs, col, val = op.get_jacobian(x)
+    assert np.array_equal(J.indptr, rs) and np.array_equal(J.indices, col)
+    assert rel(J.data, val) <= 1e-13 and rel(R, r) <= 1e-13
+    right = Problem(_face_pair(space, split, flux_form=form, flux=[(("right",), c)]))
+    const = Problem(_face_pair(space, split, flux=[(("right",), 0.0, c[0])]))
+    for Q in (right, const):
+        Q.set_current_dofs(x)
+    assert rel(right.get_residuals(), const.get_residuals()) <= 1e-14

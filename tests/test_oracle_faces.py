@@ -260,3 +260,50 @@ def test_expression_faces_closed_form_and_derivative(nn, quad):
         Rp, _ = of.expression_face_outputs(X, U + e, C, F, D, 0, quad=quad)
         Rm, _ = of.expression_face_outputs(X, U - e, C, F, D, 0, quad=quad)
         assert np.allclose(J[0][:, k], (Rp[0] - Rm[0]) / (2 * h), rtol=1e-8, atol=1e-10)
+
+
+@pytest.mark.parametrize("space,split", [("C1", "left"), ("C2", "left"), ("C2", False)])
+def test_outward_normal_of_the_unit_square(space, split):
+    """The normal the expression faces get: on each side of the unit square, the known
+    outward normal (+-1, 0) / (0, +-1) at every point, whatever the face's node order."""
+    from oracle import meshes as om
+    m = om.rectangular_quad_mesh(4, split, space)
+    want = {"right": (1.0, 0.0), "left": (-1.0, 0.0), "top": (0.0, 1.0), "bottom": (0.0, -1.0)}
+    for name, (w0, w1) in want.items():
+        F = of.boundary_faces(m, [name])
+        sides = of.outward_sides(m.coords, m.elements, F)
+        X = np.ascontiguousarray(m.coords[F][:, :, :2])
+        got = []
+        of.expression_face_outputs(X, np.zeros((len(F), F.shape[1])), np.zeros((len(F), 0)),
+                                   lambda u, x0, x1, C, n0, n1: got.append((n0, n1)) or 0.0 * u,
+                                   lambda u, x0, x1, C, n0, n1: 0.0 * u, 0, quad=m.kind == "quad_q2",
+                                   sides=sides)
+        for n0, n1 in got:
+            assert np.allclose(n0, w0, atol=1e-15) and np.allclose(n1, w1, atol=1e-15), name
+
+
+def test_normal_flux_equals_the_per_side_constant_flux():
+    """NeumannBC(u = c . n) on all four sides: the same residual as constant fluxes c0, -c0,
+    c1, -c1 on the right, left, top and bottom faces."""
+    from oracle import meshes as om
+    m = om.rectangular_quad_mesh(5, "left", "C2")
+    c = np.array([0.7, -1.3])
+    names = ["right", "left", "top", "bottom"]
+    faces = of.boundary_faces(m, names)
+    C = np.tile(c, (len(faces), 1))
+    pinned = np.zeros((m.nnode, 1), bool)
+    pinned[0, 0] = True
+    kw = dict(faces=faces, face_coef=C)
+    normal = oa.OracleProblem(m, oa.FORM_POISSON, 1, pinned, np.zeros((m.nnode, 1)), [0],
+                              face_expr=(lambda u, x0, x1, C, n0, n1: C[:, 0] * n0 + C[:, 1] * n1,
+                                         lambda u, x0, x1, C, n0, n1: 0.0 * u, None, None),
+                              face_normal=True, **kw)
+    per_side = np.concatenate([np.full(len(of.boundary_faces(m, [n])), v)
+                               for n, v in zip(names, (c[0], -c[0], c[1], -c[1]))])
+    const = oa.OracleProblem(m, oa.FORM_POISSON, 1, pinned, np.zeros((m.nnode, 1)), [0],
+                             face_expr=(lambda u, x0, x1, C: C[:, 0], lambda u, x0, x1, C: 0.0 * u, None, None),
+                             faces=faces, face_coef=per_side[:, None])
+    x = np.random.default_rng(1).uniform(-1, 1, normal.ndof)
+    assert np.allclose(normal.get_residuals(x), const.get_residuals(x), rtol=0, atol=1e-14)
+    assert np.max(np.abs(normal.get_residuals(x) - oa.OracleProblem(
+        m, oa.FORM_POISSON, 1, pinned, np.zeros((m.nnode, 1)), [0]).get_residuals(x))) > 1e-3
