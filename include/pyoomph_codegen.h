@@ -61,6 +61,15 @@ int pyhip_codegen_set_layout(int form_id, int layout);
  * src/codegen.cpp:3912-4109, pyoomph/meshes/bcs.py:71-89). Set before the form is used. */
 int pyhip_codegen_set_face_form(int form_id, int ncoef);
 
+/* Marks a generated form as an EnforcedBC constraint form (layout 3): its kernels run one
+ * thread per face of the line faces of its element family (P1 / P2 triangles, Q2
+ * quadrilaterals) with the slots (u, multiplier) of each face node, for the EnforcedBC faces
+ * of pyhip_subproblem_desc whose enforced_form names it (enforced_coef [nenforced][ncoef]).
+ * It replaces the built-in constraint alpha*u + beta of EnforcedBC(u=constraint) by the
+ * generated constraint expression (pyoomph/meshes/bcs.py:129-253, whose interface code the
+ * reference generates like any residual, src/codegen.cpp:3912-4109). Set before use. */
+int pyhip_codegen_set_constraint_form(int form_id, int ncoef);
+
 /* Named residual tables of a generated bulk form: pyoomph's add_residual(...,
  * destination=name) collects a second residual (and Jacobian) of the same equations under
  * a name (pyoomph/generic/codegen.py:777-800; the functable's res_jac_names and

@@ -126,6 +126,11 @@ typedef struct {
      * and Q2 quadrilaterals) whose flux expression replaces it, face_coef then holding its
      * ncoef coefficients per face */
     int face_form;
+    /* 0: the EnforcedBC faces carry the built-in constraint alpha*u + beta; else the id of
+     * a generated constraint form (pyhip_codegen_set_constraint_form, same element
+     * families) whose constraint expression c replaces it, enforced_coef then holding its
+     * ncoef coefficients per face: R[u_i] += int l psi_i ds, R[l_i] += int c psi_i ds */
+    int enforced_form;
 } pyhip_subproblem_desc;
 
 typedef struct pyhip_problem pyhip_problem;

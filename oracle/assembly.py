@@ -89,6 +89,9 @@ class OracleProblem:
                  the flux alpha*u + beta of the faces by an expression, C = face_coef
                  [nface, ncoef] (``faces.expression_face_outputs``); with face_normal the
                  expressions also take the outward unit normal (n0, n1)
+    enf_expr  -- (c, dc/du, dc/dp, d2c/du dp) of a constraint expression replacing
+                 enf_alpha*u + enf_beta (functions of u, x0, x1, C = enf_coef [nenf, ncoef];
+                 the last two may be None), or None
     enforced  -- [nenf, 2|3] boundary-face nodes of EnforcedBC(u=enf_alpha*u + enf_beta)
                  (``faces.enforced_face_outputs``), or None; the scalar forms then have
                  nvalue 2 (u, the multiplier); these faces follow the flux faces
@@ -96,7 +99,7 @@ class OracleProblem:
 
     def __init__(self, mesh, form, nvalue, pinned, bc, params, aux=None, faces=None, face_alpha=0.0,
                  face_beta=0.0, enforced=None, enf_alpha=1.0, enf_beta=0.0, face_expr=None, face_coef=None,
-                 face_normal=False):
+                 face_normal=False, enf_expr=None, enf_coef=None):
         from .meshes import number_equations
         self.mesh, self.form, self.nvalue = mesh, form, nvalue
         self.pinned = np.asarray(pinned, dtype=bool)
@@ -130,6 +133,8 @@ class OracleProblem:
         self.faces = None
         self.face_expr = face_expr
         self.face_normal = bool(face_normal)
+        self.enf_expr = enf_expr
+        self.enf_coef = None if enf_coef is None else np.asarray(enf_coef, dtype=np.float64)
         self.face_coef = None if face_coef is None else np.asarray(face_coef, dtype=np.float64)
         if faces is not None and len(faces):
             nf = len(faces)
@@ -244,12 +249,20 @@ class OracleProblem:
         nf, nn = F.shape
         quad = self.mesh.kind == "quad_q2"
         Xf = np.ascontiguousarray(self.mesh.coords[F][:, :, :2])
-        Rs, Js = enforced_face_outputs(Xf, np.ascontiguousarray(U[F, 0]), np.ascontiguousarray(U[F, 1]),
-                                       self.enf_alpha, self.enf_beta, 1 if flag == 1 else 0, quad=quad)
-        if deriv:
-            Rs[:] = 0.0
-        if deriv or flag == 2 or y is not None:
-            Js[:] = 0.0
+        if self.enf_expr is not None:
+            # a constraint expression: its parameter derivatives stay in the constraint rows
+            Rs, Js = enforced_face_outputs(Xf, np.ascontiguousarray(U[F, 0]), np.ascontiguousarray(U[F, 1]),
+                                           None, None, 1 if flag == 1 else 0, quad=quad, cexpr=self.enf_expr,
+                                           C=self.enf_coef, deriv=bool(deriv))
+            if flag == 2 or y is not None:
+                Js[:] = 0.0
+        else:
+            Rs, Js = enforced_face_outputs(Xf, np.ascontiguousarray(U[F, 0]), np.ascontiguousarray(U[F, 1]),
+                                           self.enf_alpha, self.enf_beta, 1 if flag == 1 else 0, quad=quad)
+            if deriv:
+                Rs[:] = 0.0
+            if deriv or flag == 2 or y is not None:
+                Js[:] = 0.0
         md = self.maxdof
         Rf = np.zeros((nf, md))
         Jf = np.zeros((nf, md * md))

@@ -152,7 +152,7 @@ def face_outputs(X, U, alpha, beta, flag, quad=False):
     return R, J
 
 
-def enforced_face_outputs(X, U, Lm, alpha, beta, flag, quad=False):
+def enforced_face_outputs(X, U, Lm, alpha, beta, flag, quad=False, cexpr=None, C=None, deriv=False):
     """Faces of pyoomph's EnforcedBC(u=alpha*u + beta) @ boundary (pyoomph/meshes/bcs.py:
     129-253): the Lagrange multiplier l (value 1 of the face nodes, u's space) adds the
     residuals weak(alpha*u + beta, l_test) and weak(l, u_test) (bcs.py:185-189). Element
@@ -160,7 +160,10 @@ def enforced_face_outputs(X, U, Lm, alpha, beta, flag, quad=False):
     ...) of the face nodes' values; X [nf, nn, 2], U / Lm [nf, nn] nodal u and l, alpha /
     beta scalars or one value per face; the rules and geometry of face_outputs. Per point:
     R[u_i] += W (l psi_i), R[l_i] += W ((alpha u + beta) psi_i), J[u_i][l_j] += W (psi_j
-    psi_i), J[l_i][u_j] += W ((alpha psi_j) psi_i); the u-u and l-l blocks stay zero."""
+    psi_i), J[l_i][u_j] += W ((alpha psi_j) psi_i); the u-u and l-l blocks stay zero.
+    With cexpr = (c, dc/du, dc/dp, d2c/du dp) (functions of u, x0, x1, C; the last two may
+    be None) the constraint is the expression c with C [nf, ncoef]; with deriv, the
+    parameter derivatives: the u rows vanish, the l rows take dc/dp and d2c/du dp."""
     nf, nn = U.shape
     knots, weights = _RULE_Q3 if quad else _RULES[nn]
     R = np.zeros((nf, 2 * nn))
@@ -177,14 +180,27 @@ def enforced_face_outputs(X, U, Lm, alpha, beta, flag, quad=False):
             u = u + U[:, l] * psi[l]
             lm = lm + Lm[:, l] * psi[l]
         W = weights[g] * np.sqrt(tx * tx + ty * ty)
-        c = alpha * u + beta
+        if cexpr is None:
+            c, dc = alpha * u + beta, alpha
+        else:
+            x0 = np.zeros(nf)
+            x1 = np.zeros(nf)
+            for l in range(nn):
+                x0 = x0 + X[:, l, 0] * psi[l]
+                x1 = x1 + X[:, l, 1] * psi[l]
+            fc, fd = (cexpr[2], cexpr[3]) if deriv else (cexpr[0], cexpr[1])
+            c = fc(u, x0, x1, C) if fc is not None else np.zeros(nf)
+            dc = fd(u, x0, x1, C) if fd is not None else np.zeros(nf)
         for i in range(nn):
             R[:, 2 * i] = R[:, 2 * i] + W * (lm * psi[i])
             R[:, 2 * i + 1] = R[:, 2 * i + 1] + W * (c * psi[i])
             if flag == 1:
                 for j in range(nn):
                     J[:, 2 * i, 2 * j + 1] = J[:, 2 * i, 2 * j + 1] + W * (psi[j] * psi[i])
-                    J[:, 2 * i + 1, 2 * j] = J[:, 2 * i + 1, 2 * j] + W * ((alpha * psi[j]) * psi[i])
+                    J[:, 2 * i + 1, 2 * j] = J[:, 2 * i + 1, 2 * j] + W * ((dc * psi[j]) * psi[i])
+    if deriv:
+        R[:, 0::2] = 0.0
+        J[:, 0::2, :] = 0.0
     return R, J
 
 

@@ -41,7 +41,7 @@ class SubproblemDesc(ctypes.Structure):
                 ("params", c_dp), ("nparams", ctypes.c_int), ("aux", c_dp),
                 ("nface", ctypes.c_int64), ("face_conn", c_i32p), ("face_coef", c_dp),
                 ("nenforced", ctypes.c_int64), ("enforced_conn", c_i32p), ("enforced_coef", c_dp),
-                ("face_form", ctypes.c_int)]
+                ("face_form", ctypes.c_int), ("enforced_form", ctypes.c_int)]
 
 
 class ElementTable(ctypes.Structure):
@@ -131,6 +131,7 @@ SIGNATURES = {
     "pyhip_codegen_set_fields": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
     "pyhip_codegen_set_layout": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "pyhip_codegen_set_face_form": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "pyhip_codegen_set_constraint_form": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "pyhip_codegen_add_residual_table": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
     "pyhip_problem_set_solved_residual": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p,
                                                          ctypes.POINTER(ctypes.c_int)]),

@@ -261,3 +261,173 @@ def _uses(e, syms):
             return True
         stack.extend(x.args)
     return False
+
+
+class ConstraintForm:
+    """constraint(s) -> Expr with s a FaceSymbols: the constraint c of pyoomph's
+    EnforcedBC(u=c) @ boundary (pyoomph/meshes/bcs.py:129-253) on the faces, enforced by a
+    Lagrange multiplier l (value 1 of the face nodes): R[u_i] += int l psi_i ds,
+    R[l_i] += int c psi_i ds, J[u_i][l_j] = int psi_j psi_i ds, J[l_i][u_j] = int (dc/du psi_j)
+    psi_i ds (the residuals weak(l, u_test) and weak(c, l_test), bcs.py:185-189). ncoef,
+    nparams, global_param as for FaceFluxForm; c[0] * u + c[1] reproduces the built-in
+    constraint faces bit for bit."""
+
+    def __init__(self, name, constraint, *, ncoef=0, nparams=0, global_param=None):
+        if not name.isidentifier():
+            raise ValueError("form name must be a C identifier")
+        self.name = name
+        self.constraint_fn = constraint
+        self.ncoef = int(ncoef)
+        self.nparams = int(nparams)
+        self.global_param = global_param
+
+    def build(self):
+        s = FaceSymbols(self.ncoef, self.nparams)
+        return s, E._wrap(self.constraint_fn(s))
+
+    def uses_normal(self):
+        s, F = self.build()
+        return _uses(F, s.n)
+
+
+CONSTRAINT_KERNEL = r'''
+#include "element_common.h"
+namespace pyhip_gen_{name} {{
+using namespace pyhip;
+template <typename T>
+__device__ __forceinline__ void body(const ElemArgs<T> &a) {{
+    constexpr int NN = {nn};
+    constexpr int NS = 2 * NN;   // slots (u_l, l_l) per face node l
+    using RULE = {rule};
+    const int e = a.e0 + (int)(blockIdx.x * 256 + threadIdx.x);
+    if (e >= a.e1) return;
+    const double *cf = a.fcoef + (int64_t)e * {dncoef};
+    const double *par = a.params + (int64_t)a.sub[e] * a.pstride;
+    (void)cf;
+    (void)par;
+    T X[NN][2], U[NN], Lm[NN];
+#pragma unroll
+    for (int l = 0; l < NN; l++) {{
+        const int32_t n = a.conn[(int64_t)l * a.nel + e];
+        X[l][0] = (T)a.coords[(int64_t)n * 3];
+        X[l][1] = (T)a.coords[(int64_t)n * 3 + 1];
+        const int32_t qu = a.seqn[(int64_t)(2 * l) * a.nel + e];
+        const int32_t ql = a.seqn[(int64_t)(2 * l + 1) * a.nel + e];
+        U[l] = (T)(qu >= 0 ? a.dofs[qu] : a.pinned[(int64_t)n * a.nvalue]);
+        Lm[l] = (T)(ql >= 0 ? a.dofs[ql] : a.pinned[(int64_t)n * a.nvalue + 1]);
+    }}
+    T R[NS], J[NS][NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) {{
+        R[i] = T(0);
+#pragma unroll
+        for (int j = 0; j < NS; j++) J[i][j] = T(0);
+    }}
+#pragma unroll
+    for (int g = 0; g < NN; g++) {{
+        double sd, wd;
+        RULE::point(g, sd, wd);
+        const T sq = (T)sd, wq = (T)wd;
+        T psi[NN], dpsi[NN];
+        RULE::shape(sq, psi, dpsi);
+        T tx = T(0), ty = T(0), u = T(0), lm = T(0);
+#pragma unroll
+        for (int l = 0; l < NN; l++) {{
+            tx = tx + X[l][0] * dpsi[l];
+            ty = ty + X[l][1] * dpsi[l];
+            u = u + U[l] * psi[l];
+            lm = lm + Lm[l] * psi[l];
+        }}
+        (void)u;
+{x_block}
+        const T W = wq * sqrt(tx * tx + ty * ty);
+{n_block}
+        T Q[2];
+        if (a.deriv) {{
+{phase_a_deriv}
+        }} else {{
+{phase_a}
+        }}
+#pragma unroll
+        for (int i = 0; i < NN; i++) {{
+            R[2 * i] = R[2 * i] + W * (lm * psi[i]);
+            R[2 * i + 1] = R[2 * i + 1] + W * (Q[0] * psi[i]);
+            if (a.flag == 1) {{
+#pragma unroll
+                for (int j = 0; j < NN; j++) {{
+                    J[2 * i][2 * j + 1] = J[2 * i][2 * j + 1] + W * (psi[j] * psi[i]);
+                    J[2 * i + 1][2 * j] = J[2 * i + 1][2 * j] + W * ((Q[1] * psi[j]) * psi[i]);
+                }}
+            }}
+        }}
+    }}
+    // derivative in the parameter: only the constraint rows (l does not depend on it); no
+    // time derivative (the mass matrix is zero)
+    const bool zero_j = a.flag == 2;
+#pragma unroll
+    for (int i = 0; i < NS; i++) {{
+        const bool zero_row = a.deriv && (i % 2 == 0);
+        a.Er[(int64_t)e * NS + i] = zero_row ? T(0) : R[i];
+        if (a.Ej) {{
+#pragma unroll
+            for (int j = 0; j < NS; j++)
+                a.Ej[((int64_t)e * NS + i) * NS + j] = (zero_j || zero_row) ? T(0) : J[i][j];
+        }}
+    }}
+}}
+}}  // namespace pyhip_gen_{name}
+extern "C" __global__ __launch_bounds__(256) void {name}_f64(pyhip::ElemArgs<double> a) {{
+    pyhip_gen_{name}::body<double>(a);
+}}
+extern "C" __global__ __launch_bounds__(256) void {name}_f64_tr(pyhip::ElemArgs<double> a) {{
+    pyhip_gen_{name}::body<double>(a);
+}}
+extern "C" __global__ __launch_bounds__(256) void {name}_f32(pyhip::ElemArgs<float> a) {{
+    pyhip_gen_{name}::body<float>(a);
+}}
+extern "C" __global__ __launch_bounds__(256) void {name}_f32_tr(pyhip::ElemArgs<float> a) {{
+    pyhip_gen_{name}::body<float>(a);
+}}
+'''
+
+
+def emit_constraint_kernels(form, kind):
+    """HIP source of the constraint form's kernels on the line faces of element family
+    ``kind``. Returns (source, kernel base name, info)."""
+    from .scalar import KIND_IDS
+    kid = KIND_IDS[kind] if isinstance(kind, str) else int(kind)
+    if kid not in FACE_RULES:
+        raise ValueError("constraint forms are generated for P1 / P2 triangles and Q2 quadrilaterals")
+    nn, rule = FACE_RULES[kid]
+    s, F = form.build()
+    c, dc = _pieces(s, F)
+    if form.global_param is not None:
+        cd, dcd = _pieces(s, F, s.p[form.global_param])
+    else:
+        cd, dcd = E.ZERO, E.ZERO
+
+    def lines(exprs, prefix):
+        return E.CPrinter("T", prefix=prefix).assign([("Q[%d]" % k, e) for k, e in enumerate(exprs)])
+
+    normal = _uses(F, s.n)
+    x_block = ""
+    if _uses(F, s.x):
+        xl = []
+        for a_ in range(2):
+            xl.append("T x%d = T(0);" % a_)
+            xl.append("#pragma unroll")
+            xl.append("for (int l = 0; l < NN; l++) x%d = x%d + X[l][%d] * psi[l];" % (a_, a_, a_))
+        x_block = _indent(xl, 8)
+    n_block = ""
+    if normal:
+        n_block = _indent(["const T tl = sqrt(tx * tx + ty * ty), side = T(cf[%d]);" % form.ncoef,
+                           "const T n0 = side * ty / tl, n1 = -(side * tx) / tl;",
+                           "(void)n0;", "(void)n1;"], 8)
+    src = CONSTRAINT_KERNEL.format(
+        name=form.name, nn=nn, rule=rule, dncoef=form.ncoef + (1 if normal else 0),
+        x_block=x_block, n_block=n_block,
+        phase_a=_indent(lines([c, dc], "a"), 12),
+        phase_a_deriv=_indent(lines([cd, dcd], "d"), 12))
+    info = {"kind": kid, "face_nodes": nn, "ncoef": form.ncoef, "symmetric": False,
+            "has_param_derivative": form.global_param is not None, "normal": normal}
+    return src, form.name, info

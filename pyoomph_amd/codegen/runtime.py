@@ -77,6 +77,34 @@ def compile_form(form, kind, load=True):
     return fid.value
 
 
+def compile_constraint_form(form, kind, load=True):
+    """Compiles an EnforcedBC constraint form (face.ConstraintForm) for the line faces of
+    element family ``kind`` and, with load, registers it (pyhip_codegen_set_constraint_form);
+    returns its id (cached per form name and family), or the code-object size without load."""
+    kid = scalar.KIND_IDS[kind] if isinstance(kind, str) else int(kind)
+    key = ("constraint", form.name, kid)
+    if load and key in _registry:
+        return _registry[key]
+    src, name, info = face.emit_constraint_kernels(form, kid)
+    L = _lib.load()
+    _signatures(L)
+    with open(COMMON_HEADER) as f:
+        hdr = f.read()
+    hs = (ctypes.c_char_p * 1)(hdr.encode())
+    hn = (ctypes.c_char_p * 1)(b"element_common.h")
+    fid = ctypes.c_int(-1)
+    size = ctypes.c_size_t(0)
+    _lib.check(L.pyhip_codegen_compile(name.encode(), src.encode(), 1, hs, hn, kid, 0,
+                                       int(info["has_param_derivative"]), int(bool(load)), ctypes.byref(fid),
+                                       ctypes.byref(size)))
+    if not load:
+        return size.value
+    _lib.check(L.pyhip_codegen_set_constraint_form(fid.value, form.ncoef + (1 if info["normal"] else 0)))
+    _lib.check(L.pyhip_codegen_set_fields(fid.value, b"u", None, int(form.global_param or 0)))
+    _registry[key] = fid.value
+    return fid.value
+
+
 def compile_face_form(form, kind, load=True):
     """Compiles a boundary-face form (face.FaceFluxForm) for the line faces of element family
     ``kind`` and, with load, registers it as a face form (pyhip_codegen_set_face_form);

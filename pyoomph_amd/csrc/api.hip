@@ -136,13 +136,19 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
     for (int s = 0; s < nsub; s++) {
         const auto &d = subs[s];
         PYHIP_REQUIRE(elem_nnode(d.elem_kind) > 0, "unknown element family");
-        PYHIP_REQUIRE((d.form >= 1 && d.form <= 4) || (gen_form_known(d.form) && !gen_face_form(d.form)),
+        PYHIP_REQUIRE((d.form >= 1 && d.form <= 4) ||
+                          (gen_form_known(d.form) && !gen_face_form(d.form) && !gen_constraint_form(d.form)),
                       "unknown weak form");
+        PYHIP_REQUIRE(d.enforced_form == 0 || (gen_constraint_form(d.enforced_form) &&
+                                               gen_form_kind(d.enforced_form) == d.elem_kind),
+                      "enforced_form: a generated constraint form compiled for this element family");
         PYHIP_REQUIRE(!gen_form_known(d.form) || gen_form_kind(d.form) == d.elem_kind,
                       "generated form compiled for another element family");
         PYHIP_REQUIRE(d.form != PYHIP_FORM_NAVIER_STOKES_TH || d.elem_kind == PYHIP_ELEM_QUAD_Q2,
                       "Taylor-Hood needs Q2 quadrilaterals");
-        PYHIP_REQUIRE(d.nenforced >= 0 && (d.nenforced == 0 || (d.enforced_conn && d.enforced_coef)),
+        PYHIP_REQUIRE(d.nenforced >= 0 && (d.nenforced == 0 || (d.enforced_conn && (d.enforced_coef ||
+                                                                                   (d.enforced_form != 0 &&
+                                                                                    gen_form_ncoef(d.enforced_form) == 0)))),
                       "EnforcedBC faces without connectivity or coefficients");
         PYHIP_REQUIRE(d.nenforced == 0 || (!th_layout(d.form) && enforced_face_kind(d.elem_kind) > 0),
                       "EnforcedBC multipliers: scalar forms on P1 / P2 triangles and Q2 quadrilaterals");
@@ -174,7 +180,8 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
     }
     for (int s = 0; s < nsub; s++) {
         if (subs[s].nenforced == 0) continue;
-        std::pair<int, int> k(kFormFlux, enforced_face_kind(subs[s].elem_kind));
+        std::pair<int, int> k(subs[s].enforced_form ? subs[s].enforced_form : kFormFlux,
+                              enforced_face_kind(subs[s].elem_kind));
         if (std::find(gkeys.begin(), gkeys.end(), k) == gkeys.end()) gkeys.push_back(k);
     }
     P->groups.resize(gkeys.size());
@@ -219,7 +226,7 @@ int pyhip_problem_create(const pyhip_subproblem_desc *subs, int nsub, void *stre
             P->groups[S.fgroup].nel += d.nface;
         }
         if (d.nenforced > 0) {
-            const std::pair<int, int> ek(kFormFlux, enforced_face_kind(d.elem_kind));
+            const std::pair<int, int> ek(d.enforced_form ? d.enforced_form : kFormFlux, enforced_face_kind(d.elem_kind));
             S.egroup = (int)(std::find(gkeys.begin(), gkeys.end(), ek) - gkeys.begin());
             S.eel_off = P->groups[S.egroup].nel;
             S.nenf = d.nenforced;

@@ -19,8 +19,9 @@ struct GenForm {
     bool symmetric = false, has_pderiv = false;
     std::string field = "u", aux_field;   // nodal field names (JIT table)
     int global_param = 0;                 // the parameter dR/dp is taken in
-    int layout = 0;                       // 0 one value per node, 1 Taylor-Hood (22 slots), 2 boundary faces
-    int ncoef = 0;                        // layout 2: coefficients per face
+    int layout = 0;                       // 0 one value per node, 1 Taylor-Hood (22 slots), 2 boundary
+                                          // faces, 3 EnforcedBC constraint faces
+    int ncoef = 0;                        // layouts 2, 3: coefficients per face
     std::vector<std::pair<std::string, int>> tables;   // named residual tables: (name, form id)
     hipModule_t mod = nullptr;
     hipFunction_t f64 = nullptr, f64_tr = nullptr, f32 = nullptr, f32_tr = nullptr;
@@ -124,7 +125,8 @@ int gen_form_table(int form, const std::string &name) {
 int gen_form_ncoef(int form) {
     std::lock_guard<std::mutex> lock(g_mu);
     const int k = form - PYHIP_GENERATED_FORM_BASE;
-    return (k >= 0 && k < (int)g_forms.size() && g_forms[k].layout == 2) ? g_forms[k].ncoef : -1;
+    return (k >= 0 && k < (int)g_forms.size() && (g_forms[k].layout == 2 || g_forms[k].layout == 3)) ? g_forms[k].ncoef
+                                                                                                      : -1;
 }
 
 bool gen_form_fields(int form, std::string &field, std::string &aux_field, int &global_param) {
@@ -152,8 +154,8 @@ static int launch_gen(int form, const ElemArgs<T> &a, hipStream_t st) {
     }
     PYHIP_REQUIRE(f != nullptr, "Hessian-vector products are generated for the scalar bulk forms only (FP64)");
     // elements per workgroup: one thread per slot (Taylor-Hood: 22) or per node / point;
-    // boundary-face forms: one thread per face
-    const int epb = layout == 2 ? 256 : (layout == 1 ? 256 / 22 : gen_epb(kind));
+    // boundary-face and constraint forms: one thread per face
+    const int epb = (layout == 2 || layout == 3) ? 256 : (layout == 1 ? 256 / 22 : gen_epb(kind));
     const unsigned blocks = (unsigned)((a.e1 - a.e0 + epb - 1) / epb);
     if (a.e1 <= a.e0) return PYHIP_OK;
     ElemArgs<T> arg = a;
@@ -238,6 +240,19 @@ int pyhip_codegen_set_face_form(int form_id, int ncoef) {
     PYHIP_REQUIRE(kind == ELEM_TRI_P1 || kind == ELEM_TRI_P2 || kind == ELEM_QUAD_Q2,
                   "boundary-face forms are generated for the line faces of P1 / P2 triangles and Q2 quadrilaterals");
     g_forms[k].layout = 2;
+    g_forms[k].ncoef = ncoef;
+    return PYHIP_OK;
+}
+
+int pyhip_codegen_set_constraint_form(int form_id, int ncoef) {
+    PYHIP_REQUIRE(ncoef >= 0 && ncoef <= 16, "coefficients per face: 0..16");
+    std::lock_guard<std::mutex> lock(g_mu);
+    const int k = form_id - PYHIP_GENERATED_FORM_BASE;
+    PYHIP_REQUIRE(k >= 0 && k < (int)g_forms.size(), "unknown generated form");
+    const int kind = g_forms[k].kind;
+    PYHIP_REQUIRE(kind == ELEM_TRI_P1 || kind == ELEM_TRI_P2 || kind == ELEM_QUAD_Q2,
+                  "constraint forms are generated for the line faces of P1 / P2 triangles and Q2 quadrilaterals");
+    g_forms[k].layout = 3;
     g_forms[k].ncoef = ncoef;
     return PYHIP_OK;
 }

@@ -143,16 +143,19 @@ int gen_form_layout(int form);
 inline bool th_layout(int form) {
     return form == PYHIP_FORM_NAVIER_STOKES_TH || (form >= PYHIP_GENERATED_FORM_BASE && gen_form_layout(form) == 1);
 }
-// per-face coefficients of a generated boundary-face form (layout 2), -1 for other forms
+// per-face coefficients of a generated boundary-face or constraint form (layouts 2, 3), -1
+// for other forms
 int gen_form_ncoef(int form);
 // the generated form has Hessian-vector product kernels
 bool gen_form_has_hv(int form);
 // id of the residual table `name` of a generated form (-1: none)
 int gen_form_table(int form, const std::string &name);
 inline bool gen_face_form(int form) { return form >= PYHIP_GENERATED_FORM_BASE && gen_form_layout(form) == 2; }
+// a generated EnforcedBC constraint form (layout 3): its faces carry (u, multiplier) slots
+inline bool gen_constraint_form(int form) { return form >= PYHIP_GENERATED_FORM_BASE && gen_form_layout(form) == 3; }
 // groups of boundary-face elements: the built-in flux / traction / EnforcedBC faces and the
 // generated boundary-face forms
-inline bool face_group_form(int form) { return form == kFormFlux || gen_face_form(form); }
+inline bool face_group_form(int form) { return form == kFormFlux || gen_face_form(form) || gen_constraint_form(form); }
 inline int group_ncoef(int form, int kind) { return form == kFormFlux ? face_ncoef(kind) : gen_form_ncoef(form); }
 int launch_generated_f64(int form, const ElemArgs<double> &a, hipStream_t s);
 int launch_generated_f32(int form, const ElemArgs<float> &a, hipStream_t s);

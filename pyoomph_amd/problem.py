@@ -57,20 +57,24 @@ class MeshProblem:
     (EnforcedBC pins redundant multipliers, bcs.py:199-243); the residual gains
     weak(alpha*u + beta, l_test) + weak(l, u_test) on every face. The Jacobian becomes a
     saddle point: solve with GMRES and ILU(0).
+    enforced_form: a generated constraint form (``codegen.face.ConstraintForm``) whose
+    constraint expression replaces alpha*u + beta; enforced then lists (boundary names,
+    coefficients) with the form's ``ncoef`` coefficients per condition.
     flux_form: a generated boundary-face form (``codegen.face.FaceFluxForm``) whose flux
     expression replaces alpha*u + beta on the flux faces; flux then lists (boundary names,
     coefficients) with the form's ``ncoef`` coefficients per condition (a form reading the
     normal gets each face's side appended).
     """
 
-    def __init__(self, mesh, equations, dirichlet=(), pin_extra=(), flux=None, enforced=None, flux_form=None):
+    def __init__(self, mesh, equations, dirichlet=(), pin_extra=(), flux=None, enforced=None, flux_form=None,
+                 enforced_form=None):
         if not isinstance(equations, Equations):
             raise TypeError("equations must be an Equations instance")
         self.mesh = mesh
         self.equations = equations
         equations.check_mesh(mesh)
         nv = equations.nvalue
-        if enforced is not None and len(enforced) == 3 and np.isscalar(enforced[1]):
+        if enforced_form is None and enforced is not None and len(enforced) == 3 and np.isscalar(enforced[1]):
             enforced = [enforced]
         if enforced and nv != 1:
             raise ValueError("EnforcedBC multipliers are built for the scalar forms")
@@ -91,16 +95,31 @@ class MeshProblem:
             self.pinned[node, vi] = True
             self.values[node, vi] = v
         self.enforced, self.enforced_coef = None, None
+        self.enforced_form = enforced_form
         if enforced:
-            from .meshes import boundary_faces
+            from .meshes import boundary_faces, face_sides
             faces, coef = [], []
-            for names, alpha, beta in enforced:
+            normal = enforced_form is not None and enforced_form.uses_normal()
+            for cond in enforced:
+                if enforced_form is None:
+                    names, alpha, beta = cond
+                    cf = [float(alpha), float(beta)]
+                else:
+                    names, cf = cond
+                    cf = np.asarray(cf, dtype=np.float64).reshape(-1)
+                    if cf.size != enforced_form.ncoef:
+                        raise ValueError("the constraint form takes %d coefficients per condition"
+                                         % enforced_form.ncoef)
                 names = [names] if isinstance(names, str) else list(names)
-                f = boundary_faces(mesh, names)
+                f, owner = boundary_faces(mesh, names, return_elements=True)
                 faces.append(f)
-                coef.append(np.tile([float(alpha), float(beta)], (len(f), 1)))
+                c = np.tile(cf, (len(f), 1))
+                if normal:
+                    c = np.concatenate([c, face_sides(mesh, f, owner)[:, None]], axis=1)
+                coef.append(c)
             self.enforced = np.ascontiguousarray(np.concatenate(faces), dtype=np.int32)
-            self.enforced_coef = np.ascontiguousarray(np.concatenate(coef), dtype=np.float64)
+            self.enforced_coef = np.ascontiguousarray(np.concatenate(coef).reshape(len(self.enforced), -1),
+                                                      dtype=np.float64)
             on = np.zeros(mesh.nnode, dtype=bool)
             on[self.enforced.reshape(-1)] = True
             # the multiplier exists on the constrained boundaries only, and is redundant
@@ -244,7 +263,10 @@ class Problem:
                 keep += [mp.enforced, mp.enforced_coef]
                 d.nenforced = mp.enforced.shape[0]
                 d.enforced_conn = mp.enforced.ctypes.data_as(_lib.c_i32p)
-                d.enforced_coef = mp.enforced_coef.ctypes.data_as(_lib.c_dp)
+                d.enforced_coef = mp.enforced_coef.ctypes.data_as(_lib.c_dp) if mp.enforced_coef.size else None
+                if mp.enforced_form is not None:
+                    from .codegen.runtime import compile_constraint_form
+                    d.enforced_form = compile_constraint_form(mp.enforced_form, mp.mesh.kind)
         torch.cuda.set_device(self.device)
         # the problem runs on the stream current at creation; every host wait below is on
         # this stream only, so problems on other streams keep running

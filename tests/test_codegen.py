@@ -198,3 +198,20 @@ def test_face_forms_compile_and_differentiate():
     assert abs(E.evaluate(E.diff(F, s.p[0]), env) + math.exp(0.7)) < 1e-14
     with pytest.raises(ValueError):
         emit_face_kernels(f, "hex_q2")
+
+
+def test_constraint_and_normal_forms_compile_for_the_line_faces():
+    """EnforcedBC constraint forms and flux forms reading the normal compile with hiprtc for
+    the line faces of every family (no GPU); the normal adds the face's side as one more
+    coefficient; bricks are refused."""
+    from pyoomph_amd.codegen.face import ConstraintForm, FaceFluxForm
+    from pyoomph_amd.codegen.runtime import compile_constraint_form, compile_face_form
+    con = ConstraintForm("cg_con_test", lambda s: s.u ** 3 + s.c[0] * s.n[1] - s.p[0] * s.u, ncoef=1, nparams=1,
+                         global_param=0)
+    flux = FaceFluxForm("cg_nflux_test", lambda s: s.c[0] * s.n[0] * s.u, ncoef=1)
+    assert con.uses_normal() and flux.uses_normal() and flux.device_ncoef() == 2
+    for kind in ("tri_p1", "tri_p2", "quad_q2"):
+        assert compile_constraint_form(con, kind, load=False) > 0
+        assert compile_face_form(flux, kind, load=False) > 0
+    with pytest.raises(ValueError):
+        compile_constraint_form(con, "hex_q2", load=False)

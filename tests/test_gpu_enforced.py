@@ -128,3 +128,69 @@ def test_saddle_point_needs_ilu0(gpu):
     P.preconditioner = "amg"
     with pytest.raises(ValueError, match="saddle point"):
         P.solve()
+
+
+def _constraint_pair(space, split, form, coef, cexpr, lam=1.1, N=6):
+    """The product problem with a generated constraint form on the right side and the
+    oracle with the same constraint expression."""
+    from pyoomph_amd.equations import BratuEquation
+    from pyoomph_amd.meshes import RectangularQuadMesh
+    from pyoomph_amd.problem import MeshProblem
+    mp = MeshProblem(RectangularQuadMesh(N, split_in_tris=split, space=space),
+                     BratuEquation("u", space=space, lambda_=lam), dirichlet=[(("left",), {0: 1.0})],
+                     enforced=[(("right",), coef)], enforced_form=form)
+    _, ref = build(N, space, split, lam)
+    faces = of.boundary_faces(ref.mesh, ["right"])
+    op = oa.OracleProblem(ref.mesh, oa.FORM_BRATU, 2, ref.pinned, ref.bc, [lam], enforced=faces, enf_expr=cexpr,
+                          enf_coef=np.tile(np.asarray(coef, float), (len(faces), 1)))
+    assert np.array_equal(mp.eqn, op.eqn)
+    return mp, op
+
+
+@pytest.mark.parametrize("space,split", CASES)
+def test_generated_linear_constraint_equals_the_built_in_faces_bitwise(gpu, space, split):
+    from pyoomph_amd.codegen.face import ConstraintForm
+    from pyoomph_amd.problem import Problem
+    form = ConstraintForm("gen_lin_constraint", lambda s: s.c[0] * s.u + s.c[1], ncoef=2)
+    A = Problem(build(6, space, split, 1.5, enf=(1.7, -0.4))[0])
+    mp, _ = _constraint_pair(space, split, form, (1.7, -0.4), None, lam=1.5)
+    B = Problem(mp)
+    x = np.random.default_rng(1).uniform(-0.3, 0.3, A.ndof)
+    A.set_current_dofs(x)
+    B.set_current_dofs(x)
+    RA, JA = A.assemble_jacobian()
+    RB, JB = B.assemble_jacobian()
+    assert np.array_equal(RA, RB)
+    assert np.array_equal(JA.indptr, JB.indptr) and np.array_equal(JA.indices, JB.indices)
+    assert np.array_equal(JA.data, JB.data)
+    assert np.array_equal(A.get_residuals(dparameter="lambda"), B.get_residuals(dparameter="lambda"))
+
+
+@pytest.mark.parametrize("space,split", CASES)
+def test_generated_nonlinear_constraint_matches_the_oracle(gpu, space, split):
+    """EnforcedBC(u = u^3 + c0 y - lambda c1 u): assembly and dR/dlambda within 1e-12 of
+    the oracle's constraint expression, Newton (GMRES + ILU(0), multipliers last) within
+    1e-8 of the oracle's."""
+    from pyoomph_amd.codegen.face import ConstraintForm
+    from pyoomph_amd.problem import Problem
+    lam, coef = 1.1, (0.5, 2.0)
+    form = ConstraintForm("gen_cubic_constraint", lambda s: s.u ** 3 + s.c[0] * s.x[1] - s.p[0] * s.c[1] * s.u,
+                          ncoef=2, nparams=1, global_param=0)
+    cexpr = (lambda u, x0, x1, C: u ** 3 + C[:, 0] * x1 - lam * C[:, 1] * u,
+             lambda u, x0, x1, C: 3.0 * u ** 2 - lam * C[:, 1],
+             lambda u, x0, x1, C: -(C[:, 1] * u),
+             lambda u, x0, x1, C: -C[:, 1] + 0.0 * u)
+    mp, op = _constraint_pair(space, split, form, coef, cexpr, lam=lam)
+    P = solver(Problem(mp))
+    x = np.random.default_rng(4).uniform(-0.3, 0.3, P.ndof)
+    P.set_current_dofs(x)
+    R, J = P.assemble_jacobian()
+    r, rs, col, val = op.get_jacobian(x)
+    assert np.array_equal(J.indptr, rs) and np.array_equal(J.indices, col)
+    assert rel(J.data, val) <= 1e-12 and rel(R, r) <= 1e-12
+    assert rel(P.get_residuals(dparameter="lambda"), op.get_residuals(x, deriv=1)) <= 1e-12
+    P.dofs.zero_()
+    st = P.solve()
+    assert st[0]["converged"]
+    xo = on.newton_solve(op, np.zeros(op.ndof), tol=1e-11)
+    assert rel(P.get_current_dofs(), xo) <= 1e-8

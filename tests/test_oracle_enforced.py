@@ -129,3 +129,63 @@ def test_mesh_problem_numbering_matches_the_oracle(space, split):
     corner = np.nonzero(m.on_boundary(["right"]) & m.on_boundary(["top"]))[0]
     assert np.all(mp.eqn[corner, 1] == -1)
     assert np.all(np.diff(mp.eqn[mp.eqn >= 0]) == 1)   # node-major order
+
+
+def _expr_problem(N, space, split, lam, cexpr, C):
+    """_problem with the constraint on the right side given by an expression."""
+    op, m = _problem(N, space, split, lam)
+    faces = of.boundary_faces(m, ["right"])
+    op2 = oa.OracleProblem(m, oa.FORM_BRATU, 2, op.pinned, op.bc, [lam], enforced=faces,
+                           enf_expr=cexpr, enf_coef=np.tile(np.asarray(C, float), (len(faces), 1)))
+    return op2, m
+
+
+@pytest.mark.parametrize("space,split", CASES)
+def test_constraint_expressions(space, split):
+    """Constraint expressions (the generated constraint forms): c0 u + c1 gives the built-in
+    faces' residual and Jacobian exactly; u - x1^2 - 1 on the right of -lap u = 0 (u = 1 on
+    the left) gives a solution equal to x1^2 + 1 on that side (the constraint holds pointwise
+    up to the projection of the quadratic on the faces' space: exactly on P2 / Q2 faces); a
+    nonlinear constraint with the parameter has the Jacobian and dR/dlambda of central
+    differences."""
+    lin, m = _expr_problem(4, space, split, 1.3, (lambda u, x0, x1, C: C[:, 0] * u + C[:, 1],
+                                                  lambda u, x0, x1, C: C[:, 0] + 0.0 * u, None, None), (1.7, -0.4))
+    ref, _ = _problem(4, space, split, 1.3, enf=(1.7, -0.4))
+    x = np.random.default_rng(2).uniform(-0.3, 0.3, ref.ndof)
+    r1, rs1, c1, v1 = lin.get_jacobian(x)
+    r2, rs2, c2, v2 = ref.get_jacobian(x)
+    assert np.array_equal(r1, r2) and np.array_equal(rs1, rs2) and np.array_equal(c1, c2) and np.array_equal(v1, v2)
+    quad, m = _expr_problem(5, space, split, 0.0, (lambda u, x0, x1, C: u - x1 * x1 - C[:, 0],
+                                                   lambda u, x0, x1, C: 1.0 + 0.0 * u, None, None), (1.0,))
+    U = quad.nodal_values(on.newton_solve(quad, np.zeros(quad.ndof)))
+    right = m.on_boundary(["right"])
+    want = m.coords[right, 1] ** 2 + 1.0
+    if space == "C2":
+        assert np.max(np.abs(U[right, 0] - want)) < 1e-8
+    else:
+        assert np.max(np.abs(U[right, 0] - want)) < 0.05
+    def cexpr(lam):
+        return (lambda u, x0, x1, C: u ** 3 + C[:, 0] * x1 - lam * u * C[:, 1],
+                lambda u, x0, x1, C: 3.0 * u ** 2 - lam * C[:, 1],
+                lambda u, x0, x1, C: -u * C[:, 1],
+                lambda u, x0, x1, C: -C[:, 1] + 0.0 * u)
+    lam = 1.1
+    nl, _ = _expr_problem(4, space, split, lam, cexpr(lam), (0.5, 2.0))
+    x = np.random.default_rng(3).uniform(-0.4, 0.4, nl.ndof)
+    _, rs, col, val = nl.get_jacobian(x)
+    n = nl.ndof
+    Jd = np.zeros((n, n))
+    for i in range(n):
+        for q in range(rs[i], rs[i + 1]):
+            Jd[i, col[q]] += val[q]
+    h = 1e-6
+    for k in range(0, n, max(1, n // 17)):
+        e = np.zeros(n)
+        e[k] = h
+        fd = (nl.get_residuals(x + e) - nl.get_residuals(x - e)) / (2 * h)
+        assert np.allclose(Jd[:, k], fd, rtol=1e-7, atol=1e-9), k
+    # dR/dlambda (the bulk's and the constraint's) against a central difference in lambda
+    plus, _ = _expr_problem(4, space, split, lam + h, cexpr(lam + h), (0.5, 2.0))
+    minus, _ = _expr_problem(4, space, split, lam - h, cexpr(lam - h), (0.5, 2.0))
+    fd = (plus.get_residuals(x) - minus.get_residuals(x)) / (2 * h)
+    assert np.allclose(nl.get_residuals(x, deriv=1), fd, rtol=1e-7, atol=1e-9)
