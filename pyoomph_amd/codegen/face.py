@@ -25,7 +25,8 @@ its nodal coordinates and values in registers, per point the line's tangent, W =
 and x, then the point quantities (flux and its u-derivative, printed from the expression
 tree by ``expr.CPrinter``), and every residual and Jacobian entry accumulated in point order.
 The flux alpha*u + beta written as ``c[0] * u + c[1]`` reproduces the built-in faces bit for
-bit; a flux that reads the normal gets it per point from the tangent. Faces have no time derivative (the mass matrix is zero) and no generated Hessian.
+bit; a flux that reads the normal gets it per point from the tangent. The _f64_hv /
+_f64_hvt kernels give the Hessian-vector products (the flux's second u-derivative). Faces have no time derivative (the mass matrix is zero).
 """
 from . import expr as E
 
@@ -79,7 +80,7 @@ KERNEL = r'''
 #include "element_common.h"
 namespace pyhip_gen_{name} {{
 using namespace pyhip;
-template <typename T>
+template <typename T, int HV>
 __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
     constexpr int NN = {nn};
     using RULE = {rule};
@@ -89,7 +90,7 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
     const double *par = a.params + (int64_t)a.sub[e] * a.pstride;
     (void)cf;
     (void)par;
-    T X[NN][2], U[NN];
+    T X[NN][2], U[NN], Y[NN];
 #pragma unroll
     for (int l = 0; l < NN; l++) {{
         const int32_t n = a.conn[(int64_t)l * a.nel + e];
@@ -97,6 +98,7 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
         X[l][1] = (T)a.coords[(int64_t)n * 3 + 1];
         const int32_t q = a.seqn[(int64_t)l * a.nel + e];
         U[l] = (T)(q >= 0 ? a.dofs[q] : a.pinned[(int64_t)n * a.nvalue]);
+        Y[l] = (HV && q >= 0) ? (T)a.hvec[q] : T(0);   // the Hessian direction (0 on pinned values)
     }}
     T R[NN], J[NN][NN];
 #pragma unroll
@@ -112,18 +114,24 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
         const T sq = (T)sd, wq = (T)wd;
         T psi[NN], dpsi[NN];
         RULE::shape(sq, psi, dpsi);
-        T tx = T(0), ty = T(0), u = T(0);
+        T tx = T(0), ty = T(0), u = T(0), yh = T(0);
 #pragma unroll
         for (int l = 0; l < NN; l++) {{
             tx = tx + X[l][0] * dpsi[l];
             ty = ty + X[l][1] * dpsi[l];
             u = u + U[l] * psi[l];
+            if (HV) yh = yh + Y[l] * psi[l];
         }}
         (void)u;
+        (void)yh;
 {x_block}
         const T W = wq * sqrt(tx * tx + ty * ty);
 {n_block}
         T Q[{nqv}];
+        T QH = T(0);   // d2 flux / du2 (Hessian-vector products)
+        if (HV) {{
+{phase_h}
+        }}
         if (a.deriv) {{
 {phase_a_deriv}
         }} else {{
@@ -134,7 +142,8 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
             R[i] = R[i] + W * ({res});
             if (a.flag == 1) {{
 #pragma unroll
-                for (int j = 0; j < NN; j++) J[i][j] = J[i][j] + W * ({jac});
+                for (int j = 0; j < NN; j++)
+                    J[i][j] = J[i][j] + W * (HV ? (((QH * yh) * psi[j]) * psi[i]) : ({jac}));
             }}
         }}
     }}
@@ -151,16 +160,24 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
 }}
 }}  // namespace pyhip_gen_{name}
 extern "C" __global__ __launch_bounds__(256) void {name}_f64(pyhip::ElemArgs<double> a) {{
-    pyhip_gen_{name}::body<double>(a);
+    pyhip_gen_{name}::body<double, 0>(a);
 }}
 extern "C" __global__ __launch_bounds__(256) void {name}_f64_tr(pyhip::ElemArgs<double> a) {{
-    pyhip_gen_{name}::body<double>(a);
+    pyhip_gen_{name}::body<double, 0>(a);
 }}
 extern "C" __global__ __launch_bounds__(256) void {name}_f32(pyhip::ElemArgs<float> a) {{
-    pyhip_gen_{name}::body<float>(a);
+    pyhip_gen_{name}::body<float, 0>(a);
 }}
 extern "C" __global__ __launch_bounds__(256) void {name}_f32_tr(pyhip::ElemArgs<float> a) {{
-    pyhip_gen_{name}::body<float>(a);
+    pyhip_gen_{name}::body<float, 0>(a);
+}}
+// Hessian-vector products: sum_k dJ_ij/du_k y_k = int (F_uu y) psi_j psi_i, symmetric in
+// i and j, so the transposed product is the same matrix
+extern "C" __global__ __launch_bounds__(256) void {name}_f64_hv(pyhip::ElemArgs<double> a) {{
+    pyhip_gen_{name}::body<double, 1>(a);
+}}
+extern "C" __global__ __launch_bounds__(256) void {name}_f64_hvt(pyhip::ElemArgs<double> a) {{
+    pyhip_gen_{name}::body<double, 1>(a);
 }}
 '''
 
@@ -238,9 +255,11 @@ def emit_face_kernels(form, kind):
                            "(void)n0;", "(void)n1;"], 8)
     # the derivative entries of a form without a global parameter are zero: Q stays unset
     # and the integrands are the constant 0
+    fuu = E.diff(E.diff(F, s.u), s.u)
     src = KERNEL.format(
         name=form.name, nn=nn, rule=rule, dncoef=form.ncoef + (1 if normal else 0), nqv=nqv,
         x_block=x_block, n_block=n_block,
+        phase_h=_indent(E.CPrinter("T", prefix="h").assign([("QH", fuu)]), 12),
         phase_a=_indent(lines(quantities, "a") or ["(void)Q;"], 12),
         phase_a_deriv=_indent(lines(q2, "d") or ["(void)Q;"], 12),
         res="a.deriv ? (%s) : (%s)" % (b_text(res_d), b_text(res)),
@@ -294,7 +313,7 @@ CONSTRAINT_KERNEL = r'''
 #include "element_common.h"
 namespace pyhip_gen_{name} {{
 using namespace pyhip;
-template <typename T>
+template <typename T, int HV>
 __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
     constexpr int NN = {nn};
     constexpr int NS = 2 * NN;   // slots (u_l, l_l) per face node l
@@ -305,7 +324,7 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
     const double *par = a.params + (int64_t)a.sub[e] * a.pstride;
     (void)cf;
     (void)par;
-    T X[NN][2], U[NN], Lm[NN];
+    T X[NN][2], U[NN], Lm[NN], Yu[NN], Yl[NN];
 #pragma unroll
     for (int l = 0; l < NN; l++) {{
         const int32_t n = a.conn[(int64_t)l * a.nel + e];
@@ -315,6 +334,9 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
         const int32_t ql = a.seqn[(int64_t)(2 * l + 1) * a.nel + e];
         U[l] = (T)(qu >= 0 ? a.dofs[qu] : a.pinned[(int64_t)n * a.nvalue]);
         Lm[l] = (T)(ql >= 0 ? a.dofs[ql] : a.pinned[(int64_t)n * a.nvalue + 1]);
+        // the Hessian direction on the u and multiplier slots (0 on pinned values)
+        Yu[l] = (HV && qu >= 0) ? (T)a.hvec[qu] : T(0);
+        Yl[l] = (HV && ql >= 0) ? (T)a.hvec[ql] : T(0);
     }}
     T R[NS], J[NS][NS];
 #pragma unroll
@@ -330,19 +352,27 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
         const T sq = (T)sd, wq = (T)wd;
         T psi[NN], dpsi[NN];
         RULE::shape(sq, psi, dpsi);
-        T tx = T(0), ty = T(0), u = T(0), lm = T(0);
+        T tx = T(0), ty = T(0), u = T(0), lm = T(0), yu = T(0), yl = T(0);
 #pragma unroll
         for (int l = 0; l < NN; l++) {{
             tx = tx + X[l][0] * dpsi[l];
             ty = ty + X[l][1] * dpsi[l];
             u = u + U[l] * psi[l];
             lm = lm + Lm[l] * psi[l];
+            if (HV) {{
+                yu = yu + Yu[l] * psi[l];
+                yl = yl + Yl[l] * psi[l];
+            }}
         }}
         (void)u;
 {x_block}
         const T W = wq * sqrt(tx * tx + ty * ty);
 {n_block}
         T Q[2];
+        T QH = T(0);   // d2 c / du2 (Hessian-vector products)
+        if (HV) {{
+{phase_h}
+        }}
         if (a.deriv) {{
 {phase_a_deriv}
         }} else {{
@@ -355,8 +385,16 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
             if (a.flag == 1) {{
 #pragma unroll
                 for (int j = 0; j < NN; j++) {{
-                    J[2 * i][2 * j + 1] = J[2 * i][2 * j + 1] + W * (psi[j] * psi[i]);
-                    J[2 * i + 1][2 * j] = J[2 * i + 1][2 * j] + W * ((Q[1] * psi[j]) * psi[i]);
+                    if (HV == 1) {{
+                        // sum_k dJ/dx_k y_k: only J[l_i][u_j] = int c_u psi_j psi_i depends on u
+                        J[2 * i + 1][2 * j] = J[2 * i + 1][2 * j] + W * (((QH * yu) * psi[j]) * psi[i]);
+                    }} else if (HV == 2) {{
+                        // sum_r y_r d2R_r/dx_a dx_b: the constraint rows' int c_uu psi_a psi_b psi_r
+                        J[2 * i][2 * j] = J[2 * i][2 * j] + W * (((QH * yl) * psi[j]) * psi[i]);
+                    }} else {{
+                        J[2 * i][2 * j + 1] = J[2 * i][2 * j + 1] + W * (psi[j] * psi[i]);
+                        J[2 * i + 1][2 * j] = J[2 * i + 1][2 * j] + W * ((Q[1] * psi[j]) * psi[i]);
+                    }}
                 }}
             }}
         }}
@@ -377,16 +415,22 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
 }}
 }}  // namespace pyhip_gen_{name}
 extern "C" __global__ __launch_bounds__(256) void {name}_f64(pyhip::ElemArgs<double> a) {{
-    pyhip_gen_{name}::body<double>(a);
+    pyhip_gen_{name}::body<double, 0>(a);
 }}
 extern "C" __global__ __launch_bounds__(256) void {name}_f64_tr(pyhip::ElemArgs<double> a) {{
-    pyhip_gen_{name}::body<double>(a);
+    pyhip_gen_{name}::body<double, 0>(a);
 }}
 extern "C" __global__ __launch_bounds__(256) void {name}_f32(pyhip::ElemArgs<float> a) {{
-    pyhip_gen_{name}::body<float>(a);
+    pyhip_gen_{name}::body<float, 0>(a);
 }}
 extern "C" __global__ __launch_bounds__(256) void {name}_f32_tr(pyhip::ElemArgs<float> a) {{
-    pyhip_gen_{name}::body<float>(a);
+    pyhip_gen_{name}::body<float, 0>(a);
+}}
+extern "C" __global__ __launch_bounds__(256) void {name}_f64_hv(pyhip::ElemArgs<double> a) {{
+    pyhip_gen_{name}::body<double, 1>(a);
+}}
+extern "C" __global__ __launch_bounds__(256) void {name}_f64_hvt(pyhip::ElemArgs<double> a) {{
+    pyhip_gen_{name}::body<double, 2>(a);
 }}
 '''
 
@@ -426,6 +470,7 @@ def emit_constraint_kernels(form, kind):
     src = CONSTRAINT_KERNEL.format(
         name=form.name, nn=nn, rule=rule, dncoef=form.ncoef + (1 if normal else 0),
         x_block=x_block, n_block=n_block,
+        phase_h=_indent(E.CPrinter("T", prefix="h").assign([("QH", E.diff(dc, s.u))]), 12),
         phase_a=_indent(lines([c, dc], "a"), 12),
         phase_a_deriv=_indent(lines([cd, dcd], "d"), 12))
     info = {"kind": kid, "face_nodes": nn, "ncoef": form.ncoef, "symmetric": False,

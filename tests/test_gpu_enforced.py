@@ -194,3 +194,37 @@ def test_generated_nonlinear_constraint_matches_the_oracle(gpu, space, split):
     assert st[0]["converged"]
     xo = on.newton_solve(op, np.zeros(op.ndof), tol=1e-11)
     assert rel(P.get_current_dofs(), xo) <= 1e-8
+
+
+@pytest.mark.parametrize("space,split", CASES)
+def test_hessian_products_with_generated_faces(gpu, space, split):
+    """Bratu with a generated cubic flux on the top side and a generated cubic constraint on
+    the right: the plain and the transposed Hessian-vector products (bulk and both face
+    kinds) against central differences of the Jacobian, H v = d/de J(x + e v) y and
+    H^T-product v = d/de J(x + e v)^T y."""
+    from pyoomph_amd.codegen.face import ConstraintForm, FaceFluxForm
+    from pyoomph_amd.equations import BratuEquation
+    from pyoomph_amd.meshes import RectangularQuadMesh
+    from pyoomph_amd.problem import MeshProblem, Problem
+    flux = FaceFluxForm("gen_hv_flux", lambda s: s.c[0] * s.u ** 3 + s.c[1] * s.n[1], ncoef=2)
+    con = ConstraintForm("gen_hv_constraint", lambda s: s.u ** 3 + s.c[0] * s.x[1] - s.p[0] * s.u, ncoef=1,
+                         nparams=1, global_param=0)
+    mp = MeshProblem(RectangularQuadMesh(5, split_in_tris=split, space=space),
+                     BratuEquation("u", space=space, lambda_=0.9), dirichlet=[(("left",), {0: 0.5})],
+                     flux=[(("top",), (0.8, -0.2))], flux_form=flux,
+                     enforced=[(("right",), (0.3,))], enforced_form=con)
+    P = Problem(mp)
+    rng = np.random.default_rng(7)
+    x, y, v = (rng.uniform(-0.4, 0.4, P.ndof) for _ in range(3))
+    P.set_current_dofs(x)
+    H = P.assemble_hessian_vector_product(y)
+    Ht = P.assemble_hessian_vector_product(y, transposed=True)
+    eps = 1e-5
+    P.set_current_dofs(x + eps * v)
+    Jp = P.assemble_jacobian()[1]
+    P.set_current_dofs(x - eps * v)
+    Jm = P.assemble_jacobian()[1]
+    fd = (Jp @ y - Jm @ y) / (2 * eps)
+    fdt = (Jp.T @ y - Jm.T @ y) / (2 * eps)
+    assert np.max(np.abs(H @ v - fd)) <= 1e-7 * np.max(np.abs(fd))
+    assert np.max(np.abs(Ht @ v - fdt)) <= 1e-7 * np.max(np.abs(fdt))
