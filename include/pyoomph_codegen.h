@@ -12,8 +12,8 @@
  * its kernels. The source must define four extern "C" kernels <name>_f64, <name>_f64_tr,
  * <name>_f32, <name>_f32_tr taking the library's element-kernel arguments by value (the
  * shape the generator emits), and may define <name>_f64_hv and <name>_f64_hvt, the plain
- * and the transposed Hessian-vector product (the generated scalar bulk, flux-face and
- * constraint-face forms do; without them Hessian products of the form are refused). FP64 kernels are compiled with
+ * and the transposed Hessian-vector product (every form the generator emits does;
+ * without them Hessian products of the form are refused). FP64 kernels are compiled with
  * -ffp-contract=off, like the compiled FP64 element kernels.
  */
 #ifndef PYOOMPH_CODEGEN_H

@@ -277,8 +277,8 @@ int pyhip_multi_assembly_plan(int nreq, const pyhip_multi_request *req, int nhve
  * :2232-2262). The mass matrix is dR/d(du/dt) (flag 2) whether or not time stepping is on;
  * dM/dp and the mass-matrix Hessian products have no non-zero element contribution for
  * these forms (their mass coefficients depend neither on the parameter nor on the dofs):
- * empty matrices. Hessian products of run-time generated forms without Hessian kernels
- * (Taylor-Hood forms), and dM/dp of generated Taylor-Hood forms, are refused with
+ * empty matrices. Hessian products of run-time generated forms whose source defines no
+ * Hessian kernels, and dM/dp of generated Taylor-Hood forms, are refused with
  * PYHIP_ERR_INVALID. */
 int pyhip_assemble_multi(pyhip_problem *p, const double *dofs_dev, int nreq,
                          const pyhip_multi_request *req, const double *hvecs_dev, int nhvec,

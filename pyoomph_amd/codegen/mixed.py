@@ -18,8 +18,9 @@ diagonal velocity blocks.
 Taylor-Hood kernel (element_kernels.h): 22 slots per element in oomph's node-major order
 (u_0, u_1 at every node, p at the corners), nodal data in LDS, per integration point the
 geometry and the point quantities, per slot row the residual entry and the Jacobian row.
-Four extern "C" kernels <name>_f64, <name>_f64_tr, <name>_f32, <name>_f32_tr; the library
-registers them with the Taylor-Hood value layout (``pyhip_codegen_set_layout``).
+Four extern "C" kernels <name>_f64, <name>_f64_tr, <name>_f32, <name>_f32_tr, and the
+Hessian-vector products <name>_f64_hv / _hvt; the library registers them with the
+Taylor-Hood value layout (``pyhip_codegen_set_layout``).
 """
 from . import expr as E
 from .scalar import _store
@@ -36,6 +37,10 @@ class THSymbols:
         self.p = E.sym("pp")
         self.x = [E.sym("x0"), E.sym("x1")]
         self.par = [E.sym("T(par[%d])" % k) for k in range(nparams)]
+        # the Hessian direction at the point (Hessian-vector product kernels only)
+        self.yv = [E.sym("y0"), E.sym("y1")]
+        self.gy = [[E.sym("gy%d%d" % (a, d)) for d in range(2)] for a in range(2)]
+        self.yp = E.sym("yp")
 
 
 class TaylorHoodWeakForm:
@@ -132,12 +137,13 @@ __device__ __constant__ const int8_t kSlotNode[22] = {{0, 0, 0, 1, 1, 2, 2, 2, 3
 __device__ __constant__ const int8_t kSlotKind[22] = {{0, 1, 2, 0, 1, 0, 1, 2, 0, 1, 0, 1, 0, 1, 0, 1, 2, 0, 1, 0, 1, 2}};
 __device__ __constant__ const int8_t kSlotP[22] = {{-1, -1, 0, -1, -1, -1, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, 2, -1, -1, -1, -1, 3}};
 __device__ __constant__ const int8_t kNodeSlot[9] = {{0, 3, 5, 8, 10, 12, 14, 17, 19}};
-template <typename T, bool TR>
+template <typename T, bool TR, int HV>
 __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
     constexpr int NN = 9, DIM = 2, NQ = 9, NS = 22;
     constexpr int EPB = 256 / NS;
     constexpr int NQV = {nqv};                 // point quantities
-    constexpr int NQS = NQV + (TR ? 3 : 0);   // [+ du_0/dt, du_1/dt, m]
+    constexpr int NQH = HV ? {nqh} : 0;        // Hessian-product quantities (HV 1 plain, 2 transposed)
+    constexpr int NQS = NQV + (TR ? 3 : 0) + NQH;   // [+ du_0/dt, du_1/dt, m]
     __shared__ T sX[EPB][NN][DIM];
     __shared__ T sV[EPB][NS];
     __shared__ int sEq[EPB][NS];
@@ -145,6 +151,7 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
     __shared__ T sDx[EPB][NQ][NN][DIM];
     __shared__ T sQ[EPB][NQ][NQS];
     __shared__ T sH[TR ? EPB : 1][NS];
+    __shared__ T sY[HV ? EPB : 1][NS];
     const int le = threadIdx.x / NS, slot = threadIdx.x % NS;
     const int e = a.e0 + (int)blockIdx.x * EPB + le;
     const bool active = (le < EPB) && (e < a.e1);
@@ -164,6 +171,7 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
             sH[le][slot] = T(a.tw1 * h1 + a.tw2 * h2);
         }}
         sEq[le][slot] = eq;
+        if constexpr (HV != 0) sY[le][slot] = T(eq >= 0 ? a.hvec[eq] : 0.0);   // 0 on pinned values
     }}
     __syncthreads();
     const double *par = nullptr;
@@ -198,6 +206,35 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
         (void)g00; (void)g01; (void)g10; (void)g11; (void)u0; (void)u1; (void)pp;
 {x_block}
         T *Q = sQ[le][q];
+        if constexpr (HV != 0) {{
+            // the direction y at the point: values, gradients, pressure
+            T y0 = T(0), y1 = T(0), yp = T(0), gy[2][2];
+#pragma unroll
+            for (int l = 0; l < NN; l++) y0 = y0 + sY[le][kNodeSlot[l]] * P[l];
+#pragma unroll
+            for (int l = 0; l < NN; l++) y1 = y1 + sY[le][1 + kNodeSlot[l]] * P[l];
+#pragma unroll
+            for (int b = 0; b < 2; b++) {{
+                T s0 = T(0), s1 = T(0);
+#pragma unroll
+                for (int l = 0; l < NN; l++) s0 = s0 + sY[le][kNodeSlot[l]] * sDx[le][q][l][b];
+#pragma unroll
+                for (int l = 0; l < NN; l++) s1 = s1 + sY[le][kNodeSlot[l] + 1] * sDx[le][q][l][b];
+                gy[0][b] = s0;
+                gy[1][b] = s1;
+            }}
+            yp = yp + sY[le][2] * C[0];
+            yp = yp + sY[le][7] * C[1];
+            yp = yp + sY[le][16] * C[2];
+            yp = yp + sY[le][21] * C[3];
+            const T gy00 = gy[0][0], gy01 = gy[0][1], gy10 = gy[1][0], gy11 = gy[1][1];
+            (void)gy00; (void)gy01; (void)gy10; (void)gy11; (void)y0; (void)y1; (void)yp;
+            if constexpr (HV == 1) {{
+{phase_hv}
+            }} else {{
+{phase_hvt}
+            }}
+        }}
         if (a.deriv) {{
 {phase_a_deriv}
         }} else {{
@@ -251,6 +288,10 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
             if (a.flag == 2) {{
                 if (kt < 2 && kt == ks) v = W * ((Q[NQV + 2] * P[ls]) * P[lt]);
                 else continue;
+            }} else if (HV == 1) {{
+{jac_hv}
+            }} else if (HV == 2) {{
+{jac_hvt}
             }} else if (a.deriv) {{
 {jac_deriv}
             }} else {{
@@ -269,16 +310,24 @@ __device__ __forceinline__ void body(const ElemArgs<T> &a) {{
 }}
 }}  // namespace pyhip_gen_{name}
 extern "C" __global__ __launch_bounds__(256) void {name}_f64(pyhip::ElemArgs<double> a) {{
-    pyhip_gen_{name}::body<double, false>(a);
+    pyhip_gen_{name}::body<double, false, 0>(a);
 }}
 extern "C" __global__ __launch_bounds__(256) void {name}_f64_tr(pyhip::ElemArgs<double> a) {{
-    pyhip_gen_{name}::body<double, true>(a);
+    pyhip_gen_{name}::body<double, true, 0>(a);
 }}
 extern "C" __global__ __launch_bounds__(256) void {name}_f32(pyhip::ElemArgs<float> a) {{
-    pyhip_gen_{name}::body<float, false>(a);
+    pyhip_gen_{name}::body<float, false, 0>(a);
 }}
 extern "C" __global__ __launch_bounds__(256) void {name}_f32_tr(pyhip::ElemArgs<float> a) {{
-    pyhip_gen_{name}::body<float, true>(a);
+    pyhip_gen_{name}::body<float, true, 0>(a);
+}}
+// Hessian-vector products: the Jacobians of the form's derivative in the direction y
+// (plain) and of the gradient of y . R (transposed), steady
+extern "C" __global__ __launch_bounds__(256) void {name}_f64_hv(pyhip::ElemArgs<double> a) {{
+    pyhip_gen_{name}::body<double, false, 1>(a);
+}}
+extern "C" __global__ __launch_bounds__(256) void {name}_f64_hvt(pyhip::ElemArgs<double> a) {{
+    pyhip_gen_{name}::body<double, false, 2>(a);
 }}
 '''
 
@@ -328,6 +377,44 @@ def emit_th_kernels(form):
     nqv = max(len(main_q), len(q2), 1)
     text = lambda e: E.CPrinter("T", prefix="b").expr_text(e)  # noqa: E731
 
+    # Hessian-vector products: the Jacobian blocks of the directional derivative of (F, S,
+    # S_p) in y (plain), and of the gradient of phi = sum_a F_a . grad y_a + S_a y_a + S_p y_p
+    # (transposed: its Jacobian is sum_r y_r d2R_r / dx dx), point quantities after Q[nqv)
+    def dirv(e):
+        acc = E.ZERO
+        for b in range(2):
+            acc = acc + E.diff(e, s.u[b]) * s.yv[b]
+            for d in range(2):
+                acc = acc + E.diff(e, s.grad_u[b][d]) * s.gy[b][d]
+        return acc + E.diff(e, s.p) * s.yp
+
+    phi = Sp * s.yp
+    for a_ in range(2):
+        phi = phi + S[a_] * s.yv[a_]
+        for d in range(2):
+            phi = phi + F[a_][d] * s.gy[a_][d]
+    hv_sets = []
+    for Fh, Sh, Sph in (([[dirv(v) for v in r] for r in F], [dirv(v) for v in S], dirv(Sp)),
+                        ([[E.diff(phi, s.grad_u[a_][d]) for d in range(2)] for a_ in range(2)],
+                         [E.diff(phi, s.u[a_]) for a_ in range(2)], E.diff(phi, s.p))):
+        qh, sth = [], {}
+
+        def qstore(e, qh=qh, sth=sth):
+            if e.op == "const":
+                return e
+            k = sth.get(e)
+            if k is None:
+                k = len(qh)
+                sth[e] = k
+                qh.append(e)
+            return E.sym("Q[%d]" % (nqv + k))
+        _, jac_h = _blocks(s, Fh, Sh, Sph, qstore)
+        hv_sets.append((qh, jac_h))
+    nqh = max(len(hv_sets[0][0]), len(hv_sets[1][0]), 1)
+
+    def phase_h_lines(qs):
+        return E.CPrinter("T", prefix="h").assign([("Q[%d]" % (nqv + k), e) for k, e in enumerate(qs)]) or ["(void)Q;"]
+
     def phase_a_lines(qs, prefix):
         return E.CPrinter("T", prefix=prefix).assign([("Q[%d]" % k, e) for k, e in enumerate(qs)])
 
@@ -346,7 +433,9 @@ def emit_th_kernels(form):
         mlines = E.CPrinter("T", prefix="m").assign([("Q[NQV + 2]", m)])
         mass_line = _indent(mlines, 12)
     src = KERNEL.format(
-        name=form.name, nqv=nqv, x_block=x_block,
+        name=form.name, nqv=nqv, nqh=nqh, x_block=x_block,
+        phase_hv=_indent(phase_h_lines(hv_sets[0][0]), 16), phase_hvt=_indent(phase_h_lines(hv_sets[1][0]), 16),
+        jac_hv=_indent(_jac_switch(hv_sets[0][1], text), 16), jac_hvt=_indent(_jac_switch(hv_sets[1][1], text), 16),
         phase_a=_indent(phase_a_lines(main_q, "a") or ["(void)Q;"], 12),
         phase_a_deriv=_indent(phase_a_lines(q2, "d") or ["(void)Q;"], 12),
         mass_line=mass_line,

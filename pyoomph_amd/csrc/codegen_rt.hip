@@ -152,7 +152,7 @@ static int launch_gen(int form, const ElemArgs<T> &a, hipStream_t st) {
         kind = G.kind;
         layout = G.layout;
     }
-    PYHIP_REQUIRE(f != nullptr, "Hessian-vector products are generated for the scalar bulk and face forms only (FP64)");
+    PYHIP_REQUIRE(f != nullptr, "Hessian-vector products need the form's _f64_hv / _f64_hvt kernels (FP64)");
     // elements per workgroup: one thread per slot (Taylor-Hood: 22) or per node / point;
     // boundary-face and constraint forms: one thread per face
     const int epb = (layout == 2 || layout == 3) ? 256 : (layout == 1 ? 256 / 22 : gen_epb(kind));

@@ -141,7 +141,7 @@ int pyhip_assemble_multi(pyhip_problem *p, const double *dofs, int nreq, const p
     }
     for (int i = 0; i < nreq; i++) {
         if (ma_hessian_kind(req[i].what))
-            PYHIP_REQUIRE(!no_hv, "Hessian-vector products are generated for the scalar bulk and face forms only");
+            PYHIP_REQUIRE(!no_hv, "Hessian-vector products need every generated form's _f64_hv / _f64_hvt kernels");
         // the mass coefficient of a generated Taylor-Hood form may depend on the parameter
         if (req[i].what == PYHIP_MA_DMASS_MATRIX_DPARAMETER)
             PYHIP_REQUIRE(!generated_th, "d(mass matrix)/d(parameter) is not generated for run-time Taylor-Hood forms");
