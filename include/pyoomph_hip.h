@@ -278,8 +278,9 @@ int pyhip_multi_assembly_plan(int nreq, const pyhip_multi_request *req, int nhve
  * dM/dp and the mass-matrix Hessian products have no non-zero element contribution for
  * these forms (their mass coefficients depend neither on the parameter nor on the dofs):
  * empty matrices. Hessian products of run-time generated forms whose source defines no
- * Hessian kernels, and dM/dp of generated Taylor-Hood forms, are refused with
- * PYHIP_ERR_INVALID. */
+ * Hessian kernels, and dM/dp and the mass-matrix Hessian products of generated Taylor-Hood
+ * forms (whose mass coefficient may depend on the parameter and the dofs), are refused
+ * with PYHIP_ERR_INVALID. */
 int pyhip_assemble_multi(pyhip_problem *p, const double *dofs_dev, int nreq,
                          const pyhip_multi_request *req, const double *hvecs_dev, int nhvec,
                          double *const *vectors, int32_t *const *row_start_dev,

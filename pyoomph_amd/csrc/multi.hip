@@ -143,8 +143,11 @@ int pyhip_assemble_multi(pyhip_problem *p, const double *dofs, int nreq, const p
         if (ma_hessian_kind(req[i].what))
             PYHIP_REQUIRE(!no_hv, "Hessian-vector products need every generated form's _f64_hv / _f64_hvt kernels");
         // the mass coefficient of a generated Taylor-Hood form may depend on the parameter
+        // and on the dofs
         if (req[i].what == PYHIP_MA_DMASS_MATRIX_DPARAMETER)
             PYHIP_REQUIRE(!generated_th, "d(mass matrix)/d(parameter) is not generated for run-time Taylor-Hood forms");
+        if (req[i].what == PYHIP_MA_MASS_MATRIX_HESSIAN_VECTOR_PRODUCT)
+            PYHIP_REQUIRE(!generated_th, "mass-matrix Hessian products are not generated for run-time Taylor-Hood forms");
     }
     if ((rc = ma_scratch(p))) return rc;
 
@@ -239,7 +242,8 @@ int pyhip_assemble_multi(pyhip_problem *p, const double *dofs, int nreq, const p
                 p->htr = 0;
                 if (rc || (rc = export_to(iH))) return rc;
             }
-            // the mass matrices of these forms do not depend on the dofs
+            // the mass matrices of these forms (compiled ones and generated scalar forms, whose
+            // mass coefficient is 1) do not depend on the dofs
             if (iMH >= 0 && (rc = zero_to(iMH))) return rc;
         }
         return PYHIP_OK;

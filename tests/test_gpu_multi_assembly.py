@@ -222,3 +222,19 @@ def test_residual_tables(gpu, space):
     assert np.array_equal(Rb[:o[1]], Ra) and (Jb[:o[1], :o[1]] - Ja).count_nonzero() == 0
     st = P.solve()
     assert st[0]["converged"]
+
+
+def test_generated_taylor_hood_mass_derivatives_are_refused(gpu):
+    """A generated Taylor-Hood form's mass coefficient may depend on the parameter and the
+    dofs: dM/dp and the mass-matrix Hessian product are refused, the Hessian product of
+    the Jacobian (generated kernels) is not."""
+    from pyoomph_amd.problem import Problem
+    from tests.test_gpu_codegen import _th_pair
+    _, mp = _th_pair(4, 50.0)
+    P = Problem(mp)
+    y = np.random.default_rng(2).uniform(-1, 1, P.ndof)
+    P.set_current_dofs(0.1 * y)
+    H, = P.start_multiassembly().dJdU(y).assemble()
+    assert H.nnz > 0
+    with pytest.raises(RuntimeError, match="mass-matrix Hessian"):
+        P.start_multiassembly().dMdU(y).assemble()
