@@ -166,7 +166,8 @@ int pyhip_pattern_copy(const pyhip_problem *p, int32_t *row_start, int32_t *col_
  *   flag          0: residual only, 1: residual and Jacobian, 2: residual and mass
  *                 matrix dR/d(du/dt) (jitbridge.h:285 flag; 2 needs time stepping on,
  *                 else the mass matrix is 0)
- *   deriv_param   -1, or p >= 0: d/d(global parameter p) of both
+ *   deriv_param   -1, or p >= 0: d/d(global parameter p) of both (with flag 2: dM/dp,
+ *                 for problems whose bulk forms are generated Taylor-Hood forms only)
  *                 (get_derivative_wrt_global_parameter, src/problem.cpp:922;
  *                 fill_in_generic_dresidual_contribution_jit, src/elements.cpp:5005-5052)
  *   precision     PYHIP_F64 / PYHIP_F32: arithmetic of the element kernels
@@ -277,10 +278,11 @@ int pyhip_multi_assembly_plan(int nreq, const pyhip_multi_request *req, int nhve
  * :2232-2262). The mass matrix is dR/d(du/dt) (flag 2) whether or not time stepping is on;
  * dM/dp and the mass-matrix Hessian products have no non-zero element contribution for
  * these forms (their mass coefficients depend neither on the parameter nor on the dofs):
- * empty matrices. Hessian products of run-time generated forms whose source defines no
- * Hessian kernels, and dM/dp and the mass-matrix Hessian products of generated Taylor-Hood
- * forms (whose mass coefficient may depend on the parameter and the dofs), are refused
- * with PYHIP_ERR_INVALID. */
+ * empty matrices; a generated Taylor-Hood form's dM/dp comes from its kernels (its mass
+ * coefficient may hold the parameter; refused when other bulk forms share the problem).
+ * Hessian products of run-time generated forms whose source defines no Hessian kernels,
+ * and the mass-matrix Hessian products of generated Taylor-Hood forms (whose mass
+ * coefficient may depend on the dofs), are refused with PYHIP_ERR_INVALID. */
 int pyhip_assemble_multi(pyhip_problem *p, const double *dofs_dev, int nreq,
                          const pyhip_multi_request *req, const double *hvecs_dev, int nhvec,
                          double *const *vectors, int32_t *const *row_start_dev,

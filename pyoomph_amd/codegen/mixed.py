@@ -426,12 +426,15 @@ def emit_th_kernels(form):
             xl.append("#pragma unroll")
             xl.append("for (int l = 0; l < NN; l++) x%d = x%d + sX[le][l][%d] * P[l];" % (a_, a_, a_))
         x_block = _indent(xl, 8)
-    # the mass coefficient at the point (TR only)
-    if E.is_zero(m):
-        mass_line = _indent(["Q[NQV + 2] = T(0);"], 12)
-    else:
-        mlines = E.CPrinter("T", prefix="m").assign([("Q[NQV + 2]", m)])
-        mass_line = _indent(mlines, 12)
+    # the mass coefficient at the point (TR only); in derivative mode its derivative in the
+    # global parameter (the flag-2 pass then gives dM/dp)
+    md = E.diff(m, s.par[form.global_param]) if form.global_param is not None else E.ZERO
+    mlines = (E.CPrinter("T", prefix="m").assign([("Q[NQV + 2]", m)]) if not E.is_zero(m)
+              else ["Q[NQV + 2] = T(0);"])
+    mdlines = (E.CPrinter("T", prefix="md").assign([("Q[NQV + 2]", md)]) if not E.is_zero(md)
+               else ["Q[NQV + 2] = T(0);"])
+    mass_line = _indent(["if (a.deriv) {"] + ["    " + l for l in mdlines] + ["} else {"] +
+                        ["    " + l for l in mlines] + ["}"], 12)
     src = KERNEL.format(
         name=form.name, nqv=nqv, nqh=nqh, x_block=x_block,
         phase_hv=_indent(phase_h_lines(hv_sets[0][0]), 16), phase_hvt=_indent(phase_h_lines(hv_sets[1][0]), 16),

@@ -555,7 +555,12 @@ static int check_assemble_args(pyhip_problem *p, const double *dofs, int flag, i
     PYHIP_REQUIRE(flag == 0 || flag == 1 || flag == 2, "flag must be 0, 1 or 2");
     PYHIP_REQUIRE(precision == PYHIP_F64 || precision == PYHIP_F32, "bad precision");
     PYHIP_REQUIRE(flag == 0 || values, "values buffer required for the Jacobian / mass matrix");
-    PYHIP_REQUIRE(flag != 2 || deriv_param < 0, "no mass matrix of a parameter derivative");
+    // dM/dp: defined by the generated Taylor-Hood forms (their mass coefficient may hold the
+    // parameter); the other forms' mass matrices do not depend on it
+    if (flag == 2 && deriv_param >= 0)
+        for (auto &G : p->groups)
+            PYHIP_REQUIRE(face_group_form(G.form) || (G.form >= PYHIP_GENERATED_FORM_BASE && th_layout(G.form)),
+                          "no mass matrix of a parameter derivative (defined for generated Taylor-Hood forms only)");
     PYHIP_REQUIRE(deriv_param == -1 || deriv_param == 0, "unknown global parameter");
     for (auto &G : p->groups)
         PYHIP_REQUIRE(deriv_param < 0 || G.form == PYHIP_FORM_BRATU || face_group_form(G.form) ||

@@ -132,20 +132,24 @@ int pyhip_assemble_multi(pyhip_problem *p, const double *dofs, int nreq, const p
     PYHIP_REQUIRE(nhvec == 0 || hvecs, "Hessian vectors required");
     for (int k = 0; k < nv; k++) PYHIP_REQUIRE(vectors[k], "null vector output");
     for (int m = 0; m < nm; m++) PYHIP_REQUIRE(row_start[m] && col[m] && val[m], "null matrix output");
-    bool no_hv = false, generated_th = false;
+    bool no_hv = false, generated_th = false, all_generated_th = true;
     for (const auto &G : p->groups) {
         if (G.form >= PYHIP_GENERATED_FORM_BASE) {
             no_hv = no_hv || !gen_form_has_hv(G.form);
             generated_th = generated_th || th_layout(G.form);
         }
+        if (!face_group_form(G.form)) all_generated_th = all_generated_th && G.form >= PYHIP_GENERATED_FORM_BASE &&
+                                                         th_layout(G.form);
     }
     for (int i = 0; i < nreq; i++) {
         if (ma_hessian_kind(req[i].what))
             PYHIP_REQUIRE(!no_hv, "Hessian-vector products need every generated form's _f64_hv / _f64_hvt kernels");
         // the mass coefficient of a generated Taylor-Hood form may depend on the parameter
-        // and on the dofs
+        // and on the dofs: dM/dp comes from its kernels (every bulk group generated
+        // Taylor-Hood), the mass Hessian is not generated
         if (req[i].what == PYHIP_MA_DMASS_MATRIX_DPARAMETER)
-            PYHIP_REQUIRE(!generated_th, "d(mass matrix)/d(parameter) is not generated for run-time Taylor-Hood forms");
+            PYHIP_REQUIRE(!generated_th || all_generated_th,
+                          "d(mass matrix)/d(parameter) of run-time Taylor-Hood forms mixed with other forms");
         if (req[i].what == PYHIP_MA_MASS_MATRIX_HESSIAN_VECTOR_PRODUCT)
             PYHIP_REQUIRE(!generated_th, "mass-matrix Hessian products are not generated for run-time Taylor-Hood forms");
     }
@@ -220,8 +224,26 @@ int pyhip_assemble_multi(pyhip_problem *p, const double *dofs, int nreq, const p
                 return rc;
             if (iDJ >= 0 && (rc = export_to(iDJ))) return rc;
             // the mass coefficients of the compiled and generated scalar forms do not depend on
-            // a parameter: dM/dp has no non-zero element contribution
-            if (iDM >= 0 && (rc = zero_to(iDM))) return rc;
+            // a parameter: dM/dp has no non-zero element contribution; a generated
+            // Taylor-Hood form's is the flag-2 pass in derivative mode (its kernels put dm/dp
+            // in the mass coefficient's place)
+            if (iDM >= 0 && generated_th) {
+                const bool was = p->transient;
+                const double tw[3] = {p->tw[0], p->tw[1], p->tw[2]};
+                const double *h1 = p->hist1, *h2 = p->hist2;
+                if (!was) {
+                    p->transient = true;
+                    p->tw[0] = 1.0, p->tw[1] = -1.0, p->tw[2] = 0.0;
+                    p->hist1 = p->hist2 = dofs;
+                }
+                rc = assemble_subset(p, dofs, 2, prm, PYHIP_F64, p->ma_res, p->ma_vals, nullptr);
+                p->transient = was;
+                p->tw[0] = tw[0], p->tw[1] = tw[1], p->tw[2] = tw[2];
+                p->hist1 = h1, p->hist2 = h2;
+                if (rc || (rc = export_to(iDM))) return rc;
+            } else if (iDM >= 0 && (rc = zero_to(iDM))) {
+                return rc;
+            }
         }
         // Hessian-vector products, one element pass per direction
         for (int i = 0; i < nreq; i++) {

@@ -238,3 +238,36 @@ def test_generated_taylor_hood_mass_derivatives_are_refused(gpu):
     assert H.nnz > 0
     with pytest.raises(RuntimeError, match="mass-matrix Hessian"):
         P.start_multiassembly().dMdU(y).assemble()
+
+
+def test_generated_taylor_hood_mass_parameter_derivative(gpu):
+    """A generated Taylor-Hood form whose mass coefficient is the global parameter squared:
+    dM/dp from the multi-assembly equals the central difference of its mass matrix in the
+    parameter."""
+    from pyoomph_amd.codegen import library
+    from pyoomph_amd.codegen.mixed import TaylorHoodWeakForm
+    from pyoomph_amd.codegen.runtime import GeneratedEquation
+    from pyoomph_amd.meshes import RectangularQuadMesh
+    from pyoomph_amd.problem import MeshProblem, Problem
+    ns = library.navier_stokes()
+    form = TaylorHoodWeakForm("gen_th_pmass", ns.flux_fn, ns.source_fn, ns.cont_fn,
+                              mass=lambda s: s.par[2] * s.par[2], nparams=3, global_param=2)
+    mesh = RectangularQuadMesh(4, split_in_tris=False, space="C2")
+    bcs = [(("top",), {0: 1.0, 1: 0.0}), (("left", "right", "bottom"), {0: 0.0, 1: 0.0})]
+
+    def problem(p):
+        eq = GeneratedEquation(form, space="C2", params=[1.0 / 50.0, 1.0, p])
+        return Problem(MeshProblem(mesh, eq, dirichlet=bcs, pin_extra=[(0, 2, 0.0)]))
+    p0, h = 1.3, 1e-4
+    P = problem(p0)
+    x = np.random.default_rng(5).uniform(-0.3, 0.3, P.ndof)
+    P.set_current_dofs(x)
+    dM, = P.start_multiassembly().dMdp("lambda").assemble()
+    Ms = []
+    for p in (p0 + h, p0 - h):
+        Q = problem(p)
+        Q.set_current_dofs(x)
+        Ms.append(Q.start_multiassembly().M().assemble()[0].toarray())
+    fd = (Ms[0] - Ms[1]) / (2 * h)
+    assert dM.nnz > 0
+    assert np.max(np.abs(dM.toarray() - fd)) <= 1e-8 * np.max(np.abs(fd))
